@@ -1,0 +1,25 @@
+"""gyunity_amd: MI355X-native path-tracing integrator for the Gyunity renderer.
+
+The product is libgyunity.so (HIP kernels for gfx950 behind the C ABI in
+include/gyunity.h) and the C++ Integrator in include/gyunity/; this package is
+the Python binding used by tests and the benchmark.
+"""
+from ._lib import (  # noqa: F401
+    Context,
+    Scene,
+    SceneArrays,
+    GyError,
+    RAY_DTYPE,
+    HIT_DTYPE,
+    camera,
+    camera_matrix,
+    generate_scene,
+    lib,
+    render_params,
+    sobol_enum_index,
+    sobol_sample,
+    GY_SCENE_CORNELL,
+    GY_SCENE_ICOSPHERE_GGX,
+    GY_RENDER_MEGAKERNEL,
+    GY_RENDER_WAVEFRONT,
+)

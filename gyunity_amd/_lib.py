@@ -1,0 +1,388 @@
+"""ctypes binding of libgyunity.so (the C ABI in include/gyunity.h).
+
+Loads the in-tree library next to this file and raises if it is missing:
+there is no fallback path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgyunity.so")
+
+GY_OK = 0
+GY_ERR_INVALID = 1
+GY_ERR_DEVICE = 2
+GY_ERR_ALLOC = 3
+GY_ERR_UNSUPPORTED = 4
+GY_ERR_INJECTED = 5
+MAT_WORDS = 32
+GY_SCENE_CORNELL = 0
+GY_SCENE_ICOSPHERE_GGX = 1
+GY_RENDER_MEGAKERNEL = 0
+GY_RENDER_WAVEFRONT = 1
+FAULT_NONE, FAULT_ALLOC, FAULT_LAUNCH, FAULT_SCENE = 0, 1, 2, 3
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [
+        ("n_tri", C.c_int32),
+        ("pos", C.POINTER(C.c_float)),
+        ("nrm", C.POINTER(C.c_float)),
+        ("face_n", C.POINTER(C.c_float)),
+        ("uv", C.POINTER(C.c_float)),
+        ("mat", C.POINTER(C.c_uint32)),
+        ("light", C.POINTER(C.c_int32)),
+        ("n_mat", C.c_int32),
+        ("mats", C.POINTER(C.c_uint32)),
+        ("n_light", C.c_int32),
+        ("light_tri", C.POINTER(C.c_int32)),
+        ("light_le", C.POINTER(C.c_float)),
+        ("n_tex", C.c_int32),
+        ("tex_info", C.POINTER(C.c_int32)),
+        ("tex_data", C.POINTER(C.c_float)),
+    ]
+
+
+class CameraDesc(C.Structure):
+    _fields_ = [
+        ("pos", C.c_float * 3),
+        ("look_at", C.c_float * 3),
+        ("up", C.c_float * 3),
+        ("fovy", C.c_float),
+        ("film_dist", C.c_float),
+        ("width", C.c_int32),
+        ("height", C.c_int32),
+    ]
+
+
+class RenderParams(C.Structure):
+    _fields_ = [
+        ("spp", C.c_int32),
+        ("max_depth", C.c_int32),
+        ("rr_depth", C.c_int32),
+        ("mode", C.c_int32),
+        ("samples_per_pass", C.c_int32),
+        ("block_size", C.c_int32),
+        ("reserved", C.c_int32 * 2),
+    ]
+
+
+class BVHParams(C.Structure):
+    _fields_ = [("max_leaf_size", C.c_int32), ("builder", C.c_int32), ("reserved", C.c_int32 * 2)]
+
+
+class SceneStats(C.Structure):
+    _fields_ = [
+        ("n_nodes", C.c_int64),
+        ("n_leaves", C.c_int64),
+        ("max_depth", C.c_int32),
+        ("max_leaf_size", C.c_int32),
+        ("build_ms", C.c_double),
+        ("sah_cost", C.c_double),
+    ]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [
+        ("paths", C.c_int64),
+        ("rays", C.c_int64),
+        ("shadow_rays", C.c_int64),
+        ("mis_rays", C.c_int64),
+        ("total_ms", C.c_double),
+        ("kernel_ms", C.c_double * 8),
+        ("kernel_launches", C.c_int64 * 8),
+        ("node_visits", C.c_int64),
+        ("tri_visits", C.c_int64),
+        ("shade_points", C.c_int64),
+    ]
+
+
+RAY_DTYPE = np.dtype([("org", "<f4", 3), ("tmin", "<f4"), ("dir", "<f4", 3), ("tmax", "<f4")])
+HIT_DTYPE = np.dtype([("t", "<f4"), ("prim", "<i4"), ("b1", "<f4"), ("b2", "<f4")])
+
+_lib = None
+
+
+def lib():
+    """Load libgyunity.so; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            "libgyunity.so is not built (expected at %s); run __graft_entry__.build() or make -C gyunity_amd" % LIB_PATH
+        )
+    L = C.CDLL(LIB_PATH)
+    vp, i32, i64, u32, u64, f32 = C.c_void_p, C.c_int32, C.c_int64, C.c_uint32, C.c_uint64, C.c_float
+    P = C.POINTER
+    sig = {
+        "gy_open": (C.c_int, [C.c_int, P(vp)]),
+        "gy_close": (C.c_int, [vp]),
+        "gy_last_error": (C.c_char_p, []),
+        "gy_version": (C.c_char_p, []),
+        "gy_device_name": (C.c_int, [vp, C.c_char_p, C.c_int]),
+        "gy_scene_create": (C.c_int, [vp, P(SceneDesc), P(BVHParams), P(vp)]),
+        "gy_scene_destroy": (C.c_int, [vp]),
+        "gy_scene_stats_get": (C.c_int, [vp, P(SceneStats)]),
+        "gy_scene_bvh_download": (C.c_int, [vp, P(u32), i64, P(i32), i64]),
+        "gy_trace_closest": (C.c_int, [vp, vp, i64, vp]),
+        "gy_trace_any": (C.c_int, [vp, vp, i64, P(i32)]),
+        "gy_trace_closest_device": (C.c_int, [vp, vp, i64, vp]),
+        "gy_trace_any_device": (C.c_int, [vp, vp, i64, vp]),
+        "gy_last_kernel_ms": (C.c_double, [vp]),
+        "gy_count_visits": (C.c_int, [vp, vp, i64, P(i64), P(i64)]),
+        "gy_render": (C.c_int, [vp, P(CameraDesc), P(RenderParams), P(f32), P(f32)]),
+        "gy_film_download": (C.c_int, [vp, P(f32), P(f32)]),
+        "gy_render_stats_get": (C.c_int, [vp, P(RenderStats)]),
+        "gy_kernel_name": (C.c_char_p, [C.c_int]),
+        "gy_li_samples": (C.c_int, [vp, P(CameraDesc), P(RenderParams), i32, P(i32), P(i32), P(i32), P(f32), P(f32), P(i32)]),
+        "gy_count_traffic": (C.c_int, [vp, P(CameraDesc), P(RenderParams), P(RenderStats)]),
+        "gy_sobol_sample": (f32, [u64, u32]),
+        "gy_sobol_enum_index": (u64, [u32, u32, u32, u32]),
+        "gy_camera_matrix": (C.c_int, [P(CameraDesc), P(f32)]),
+        "gy_generate_scene": (C.c_int, [C.c_int, u32, i32, P(vp)]),
+        "gy_generated_desc": (C.c_int, [vp, P(SceneDesc), P(CameraDesc), P(RenderParams)]),
+        "gy_generated_free": (C.c_int, [vp]),
+        "gy_debug_inject_fault": (C.c_int, [vp, C.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+class GyError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("gyunity error %d: %s" % (code, msg))
+        self.code = code
+
+
+def check(rc):
+    if rc != GY_OK:
+        raise GyError(rc, lib().gy_last_error().decode())
+    return rc
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def _up(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+class SceneArrays:
+    """Host copy of a flattened scene (the arrays a gy_scene_desc points at)."""
+
+    def __init__(self, pos, nrm, face_n, mat, light, mats, light_tri, light_le, uv=None, tex_info=None, tex_data=None):
+        self.pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 9)
+        self.nrm = np.ascontiguousarray(nrm, dtype=np.float32).reshape(-1, 9)
+        self.face_n = np.ascontiguousarray(face_n, dtype=np.float32).reshape(-1, 3)
+        self.mat = np.ascontiguousarray(mat, dtype=np.uint32)
+        self.light = np.ascontiguousarray(light, dtype=np.int32)
+        self.mats = np.ascontiguousarray(mats, dtype=np.uint32).reshape(-1, MAT_WORDS)
+        self.light_tri = np.ascontiguousarray(light_tri, dtype=np.int32)
+        self.light_le = np.ascontiguousarray(light_le, dtype=np.float32).reshape(-1, 3)
+        self.uv = None if uv is None else np.ascontiguousarray(uv, dtype=np.float32).reshape(-1, 6)
+        self.tex_info = None if tex_info is None else np.ascontiguousarray(tex_info, dtype=np.int32).reshape(-1, 4)
+        self.tex_data = None if tex_data is None else np.ascontiguousarray(tex_data, dtype=np.float32)
+
+    @property
+    def n_tri(self):
+        return self.pos.shape[0]
+
+    def desc(self):
+        d = SceneDesc()
+        d.n_tri = self.n_tri
+        d.pos, d.nrm, d.face_n = _fp(self.pos), _fp(self.nrm), _fp(self.face_n)
+        d.uv = _fp(self.uv) if self.uv is not None else C.POINTER(C.c_float)()
+        d.mat, d.light = _up(self.mat), _ip(self.light)
+        d.n_mat, d.mats = self.mats.shape[0], _up(self.mats)
+        d.n_light, d.light_tri, d.light_le = self.light_tri.shape[0], _ip(self.light_tri), _fp(self.light_le)
+        if self.tex_info is not None:
+            d.n_tex, d.tex_info, d.tex_data = self.tex_info.shape[0], _ip(self.tex_info), _fp(self.tex_data)
+        else:
+            d.n_tex = 0
+        return d
+
+
+def generate_scene(kind, seed=0, param=0):
+    """Procedural scene -> (SceneArrays, CameraDesc, RenderParams)."""
+    L = lib()
+    h = C.c_void_p()
+    check(L.gy_generate_scene(kind, seed, param, C.byref(h)))
+    try:
+        d, cam, prm = SceneDesc(), CameraDesc(), RenderParams()
+        check(L.gy_generated_desc(h, C.byref(d), C.byref(cam), C.byref(prm)))
+        n = d.n_tri
+        arr = lambda p, cnt, dt: np.ctypeslib.as_array(p, shape=(cnt,)).astype(dt, copy=True)
+        sa = SceneArrays(
+            arr(d.pos, 9 * n, np.float32),
+            arr(d.nrm, 9 * n, np.float32),
+            arr(d.face_n, 3 * n, np.float32),
+            arr(d.mat, n, np.uint32),
+            arr(d.light, n, np.int32),
+            arr(d.mats, MAT_WORDS * d.n_mat, np.uint32),
+            arr(d.light_tri, d.n_light, np.int32),
+            arr(d.light_le, 3 * d.n_light, np.float32),
+        )
+    finally:
+        L.gy_generated_free(h)
+    return sa, cam, prm
+
+
+def camera(pos, look_at, up, fovy, width, height, film_dist=1.0):
+    c = CameraDesc()
+    c.pos[:], c.look_at[:], c.up[:] = pos, look_at, up
+    c.fovy, c.film_dist, c.width, c.height = fovy, film_dist, width, height
+    return c
+
+
+def render_params(spp, max_depth, rr_depth=5, mode=GY_RENDER_MEGAKERNEL, samples_per_pass=0, block_size=0):
+    p = RenderParams()
+    p.spp, p.max_depth, p.rr_depth, p.mode = spp, max_depth, rr_depth, mode
+    p.samples_per_pass, p.block_size = samples_per_pass, block_size
+    return p
+
+
+def sobol_sample(index, dim):
+    return lib().gy_sobol_sample(index, dim)
+
+
+def sobol_enum_index(m, frame, px, py):
+    return lib().gy_sobol_enum_index(m, frame, px, py)
+
+
+def camera_matrix(cam):
+    out = np.zeros(16, dtype=np.float32)
+    check(lib().gy_camera_matrix(C.byref(cam), _fp(out)))
+    return out
+
+
+class Context:
+    """One HIP device and stream (gy_open / gy_close)."""
+
+    def __init__(self, device=0):
+        self.h = C.c_void_p()
+        check(lib().gy_open(device, C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().gy_close(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def device_name(self):
+        buf = C.create_string_buffer(256)
+        check(lib().gy_device_name(self.h, buf, 256))
+        return buf.value.decode()
+
+    def stats(self):
+        s = RenderStats()
+        check(lib().gy_render_stats_get(self.h, C.byref(s)))
+        return s
+
+    def last_kernel_ms(self):
+        return lib().gy_last_kernel_ms(self.h)
+
+    def inject_fault(self, kind):
+        check(lib().gy_debug_inject_fault(self.h, kind))
+
+
+class Scene:
+    """A device scene with its BVH (gy_scene_create)."""
+
+    def __init__(self, ctx, arrays, max_leaf_size=0, builder=0):
+        self.ctx = ctx
+        self.arrays = arrays
+        self._desc = arrays.desc()
+        bp = BVHParams()
+        bp.max_leaf_size, bp.builder = max_leaf_size, builder
+        self.h = C.c_void_p()
+        check(lib().gy_scene_create(ctx.h, C.byref(self._desc), C.byref(bp), C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().gy_scene_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def stats(self):
+        s = SceneStats()
+        check(lib().gy_scene_stats_get(self.h, C.byref(s)))
+        return s
+
+    def bvh(self):
+        st = self.stats()
+        nodes = np.zeros((st.n_nodes, 16), dtype=np.uint32)
+        order = np.zeros(self.arrays.n_tri, dtype=np.int32)
+        check(lib().gy_scene_bvh_download(self.h, _up(nodes), st.n_nodes, _ip(order), order.size))
+        return nodes, order
+
+    def trace_closest(self, rays):
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        hits = np.zeros(rays.shape[0], dtype=HIT_DTYPE)
+        check(lib().gy_trace_closest(self.h, rays.ctypes.data, rays.shape[0], hits.ctypes.data))
+        return hits
+
+    def trace_any(self, rays):
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        occ = np.zeros(rays.shape[0], dtype=np.int32)
+        check(lib().gy_trace_any(self.h, rays.ctypes.data, rays.shape[0], _ip(occ)))
+        return occ
+
+    def count_visits(self, rays):
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        a, b = C.c_int64(), C.c_int64()
+        check(lib().gy_count_visits(self.h, rays.ctypes.data, rays.shape[0], C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def render(self, cam, params, download=True):
+        W, H = cam.width, cam.height
+        if download:
+            color = np.zeros((H, W, 3), dtype=np.float32)
+            weight = np.zeros((H, W), dtype=np.float32)
+            check(lib().gy_render(self.h, C.byref(cam), C.byref(params), _fp(color), _fp(weight)))
+            return color, weight
+        check(lib().gy_render(self.h, C.byref(cam), C.byref(params), C.POINTER(C.c_float)(), C.POINTER(C.c_float)()))
+        return None
+
+    def film(self, W, H):
+        color = np.zeros((H, W, 3), dtype=np.float32)
+        weight = np.zeros((H, W), dtype=np.float32)
+        check(lib().gy_film_download(self.h, _fp(color), _fp(weight)))
+        return color, weight
+
+    def li_samples(self, cam, params, x, y, s):
+        x = np.ascontiguousarray(x, dtype=np.int32)
+        y = np.ascontiguousarray(y, dtype=np.int32)
+        s = np.ascontiguousarray(s, dtype=np.int32)
+        n = x.shape[0]
+        L = np.zeros((n, 3), dtype=np.float32)
+        raster = np.zeros((n, 2), dtype=np.float32)
+        dims = np.zeros(n, dtype=np.int32)
+        check(lib().gy_li_samples(self.h, C.byref(cam), C.byref(params), n, _ip(x), _ip(y), _ip(s), _fp(L), _fp(raster), _ip(dims)))
+        return L, raster, dims
+
+    def count_traffic(self, cam, params):
+        st = RenderStats()
+        check(lib().gy_count_traffic(self.h, C.byref(cam), C.byref(params), C.byref(st)))
+        return st

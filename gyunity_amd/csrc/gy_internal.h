@@ -1,0 +1,62 @@
+// Internal declarations shared by the host C++ sources and the HIP kernels.
+#pragma once
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/gyunity.h"
+
+namespace gy {
+
+// Thread-local error message behind gy_last_error().
+void SetError(const std::string& msg);
+int Fail(int code, const std::string& msg);
+
+namespace sobol {
+constexpr int kDims = 1024;       // dimensions of the Joe-Kuo table
+constexpr int kCols = 52;         // index bits per dimension
+constexpr int kMaxLog2Res = 16;   // enumeration up to 65536^2 pixels
+struct EnumTables {
+    int m;
+    uint64_t inv[64];    // A^-1 by columns (target bit -> low-index bits)
+    uint64_t frame[64];  // pixel bits reached by frame bit j
+};
+const uint32_t* Matrices();        // kDims * kCols
+const EnumTables& Enum(int m);
+float Sample(uint64_t index, uint32_t dim);
+uint64_t EnumIndex(uint32_t m, uint32_t frame, uint32_t px, uint32_t py);
+}  // namespace sobol
+
+// Camera matrix in the reference's FP32 operation order (camera/Camera.cpp:6-38).
+struct CameraRT {
+    float raster_to_world[16];  // column-major, (i, j) at [j*4 + i]
+    float pos[3];
+    int32_t width, height;
+    int32_t res;       // RoundUpPow2(max(width, height))  (sampler/SobolEnum.h:18)
+    int32_t log2res;
+};
+int BuildCamera(const gy_camera_desc& d, CameraRT* out);
+
+// BVH in the device layout: 16 words per node.
+//   words 0-5   child 0 AABB (min xyz, max xyz)
+//   words 6-11  child 1 AABB
+//   word 12/13  child 0/1 reference: node index (interior), first triangle (leaf),
+//               0xffffffff (empty slot)
+//   word 14/15  child 0/1 triangle count (0 = interior)
+struct BVHBuild {
+    std::vector<uint32_t> nodes;     // n_nodes * 16
+    std::vector<uint32_t> tri_order; // leaf order -> primitive id
+    int32_t max_depth = 0;
+    int64_t n_leaves = 0;
+    int32_t max_leaf = 0;
+    double sah_cost = 0;
+};
+constexpr uint32_t kEmptyRef = 0xffffffffu;
+// Host binned-SAH build over triangle bounds.
+void BuildBVHHost(const float* pos, int32_t n_tri, int max_leaf, BVHBuild* out);
+double BVHSahCost(const BVHBuild& b);
+
+struct Generated;
+
+}  // namespace gy

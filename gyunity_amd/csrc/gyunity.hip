@@ -1,0 +1,1210 @@
+// libgyunity: context and scene management, ray-query kernels, the plain
+// path-tracing megakernel, film accumulation with per-pixel ownership, and the
+// C ABI declared in include/gyunity.h.  The wavefront integrator lives in
+// wavefront.hip and shares everything declared in runtime.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+
+#include "device.h"
+#include "gy_internal.h"
+#include "runtime.h"
+
+using namespace gyd;
+
+namespace gy {
+
+static thread_local std::string t_error;
+void SetError(const std::string& msg) { t_error = msg; }
+int Fail(int code, const std::string& msg) {
+    t_error = msg;
+    return code;
+}
+
+int HipFail(hipError_t e, const char* what) {
+    return Fail(GY_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ------------------------------------------------------------------ small kernels
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_fill_f4(float4* p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+int Fill32(Ctx* c, void* p, uint32_t v, int64_t n) {
+    if (n <= 0) return GY_OK;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, (uint32_t*)p, v, n);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? GY_OK : HipFail(e, "fill kernel");
+}
+
+int DevAlloc(Ctx* c, void** p, size_t bytes) {
+    if (c->fault == GY_FAULT_ALLOC) {
+        c->fault = GY_FAULT_NONE;
+        return Fail(GY_ERR_INJECTED, "injected allocation failure");
+    }
+    if (bytes == 0) bytes = 16;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return Fail(GY_ERR_ALLOC, std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    }
+    return GY_OK;
+}
+
+int Upload(Ctx* c, void** p, const void* src, size_t bytes) {
+    int rc = DevAlloc(c, p, bytes);
+    if (rc != GY_OK) return rc;
+    if (bytes == 0) return GY_OK;
+    hipError_t e = hipMemcpyAsync(*p, src, bytes, hipMemcpyHostToDevice, c->stream);
+    if (e != hipSuccess) return HipFail(e, "hipMemcpyAsync H2D");
+    e = hipStreamSynchronize(c->stream);
+    return e == hipSuccess ? GY_OK : HipFail(e, "hipStreamSynchronize");
+}
+
+void DevFree(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+// Growable device scratch buffer.
+int Grow(Ctx* c, DevBuf* b, size_t bytes) {
+    if (b->bytes >= bytes && b->ptr) return GY_OK;
+    DevFree(b->ptr);
+    b->ptr = nullptr;
+    b->bytes = 0;
+    int rc = DevAlloc(c, &b->ptr, bytes);
+    if (rc != GY_OK) return rc;
+    b->bytes = bytes;
+    return GY_OK;
+}
+
+int CheckLaunch(Ctx* c, const char* what) {
+    if (c->fault == GY_FAULT_LAUNCH) {
+        c->fault = GY_FAULT_NONE;
+        return Fail(GY_ERR_INJECTED, std::string("injected launch failure in ") + what);
+    }
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? GY_OK : HipFail(e, what);
+}
+
+// Timed region on the context stream: events recorded around each launch of a
+// kernel category and summed after the stream synchronises.
+void TimerBegin(Ctx* c, int cat) {
+    if (c->ev_used + 2 > (int)c->events.size()) {
+        hipEvent_t a, b;
+        (void)hipEventCreate(&a);
+        (void)hipEventCreate(&b);
+        c->events.push_back(a);
+        c->events.push_back(b);
+    }
+    c->ev_cat.resize(c->events.size() / 2);
+    c->ev_cat[c->ev_used / 2] = cat;
+    (void)hipEventRecord(c->events[c->ev_used], c->stream);
+}
+void TimerEnd(Ctx* c) {
+    (void)hipEventRecord(c->events[c->ev_used + 1], c->stream);
+    c->ev_used += 2;
+}
+void TimersCollect(Ctx* c) {
+    for (int i = 0; i + 1 < c->ev_used; i += 2) {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, c->events[i], c->events[i + 1]);
+        const int cat = c->ev_cat[i / 2];
+        c->stats.kernel_ms[cat] += ms;
+        c->stats.kernel_launches[cat] += 1;
+    }
+    c->ev_used = 0;
+}
+
+// ------------------------------------------------------------------ ray query kernels
+template <int STACK>
+__global__ void __launch_bounds__(256) k_trace_closest(SceneDev S, const gy_ray* __restrict__ rays, int64_t n,
+                                                       gy_hit* __restrict__ hits) {
+    __shared__ uint32_t stack[STACK * 256];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4* rp = reinterpret_cast<const float4*>(rays + i);
+    const float4 a = rp[0], b = rp[1];
+    Hit h;
+    gy_hit out;
+    if (traverse<false, STACK>(S, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, b.w, &h, stack + threadIdx.x, blockDim.x)) {
+        out.t = h.t;
+        out.prim = h.prim;
+        out.b1 = h.b1;
+        out.b2 = h.b2;
+    } else {
+        out.t = 0.f;
+        out.prim = -1;
+        out.b1 = 0.f;
+        out.b2 = 0.f;
+    }
+    *reinterpret_cast<float4*>(hits + i) = make_float4(out.t, __int_as_float(out.prim), out.b1, out.b2);
+}
+
+template <int STACK>
+__global__ void __launch_bounds__(256) k_trace_any(SceneDev S, const gy_ray* __restrict__ rays, int64_t n,
+                                                   int32_t* __restrict__ occ) {
+    __shared__ uint32_t stack[STACK * 256];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4* rp = reinterpret_cast<const float4*>(rays + i);
+    const float4 a = rp[0], b = rp[1];
+    Hit h;
+    occ[i] = traverse<true, STACK>(S, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, b.w, &h, stack + threadIdx.x, blockDim.x) ? 1 : 0;
+}
+
+__device__ inline void wave_add_u64(unsigned long long* dst, uint64_t v) {
+    // reduce across the wavefront, one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, (unsigned long long)v);
+}
+
+template <int STACK>
+__global__ void __launch_bounds__(256) k_count_visits(SceneDev S, const gy_ray* __restrict__ rays, int64_t n,
+                                                      unsigned long long* counters) {
+    __shared__ uint32_t stack[STACK * 256];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t nn = 0, nt = 0;
+    if (i < n) {
+        const float4* rp = reinterpret_cast<const float4*>(rays + i);
+        const float4 a = rp[0], b = rp[1];
+        Hit h;
+        traverse<false, STACK>(S, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, b.w, &h, stack + threadIdx.x, blockDim.x, &nn, &nt);
+    }
+    wave_add_u64(&counters[0], nn);
+    wave_add_u64(&counters[1], nt);
+}
+
+// ------------------------------------------------------------------ path tracing (megakernel)
+// The whole of TiledIntegrator's per-sample work (integrator/TiledIntegrator.cpp:40-47)
+// and PathTracing::Li (integrator/PT.cpp:10-67) for one camera sample per thread.
+struct PathCounters {
+    uint32_t rays, shadow, mis, shade, nodes, tris;
+};
+
+template <int STACK, bool COUNT>
+__device__ V3 direct_illumination(const SceneDev& S, const Isect& it, const BSDF& b, float uL, float u0, float u1,
+                                  float v0, float v1, float v2, uint32_t* stack, int stride, PathCounters& pc) {
+    // Integrator::DirectIllumination (integrator/Integrator.cpp:9-107)
+    float lsp;
+    const int light = sample_one_light(S, uL, &lsp);
+    const int ltri = S.light_tri[light];
+    const V3 Le = v3(S.light_le[3 * light], S.light_le[3 * light + 1], S.light_le[3 * light + 2]);
+    V3 L1 = v3s(0.f), L2 = v3s(0.f);
+    float w1 = 1.f, w2 = 1.f;
+    {
+        float pdf;
+        const LightPoint lp = sample_light_point(S, ltri, it.pos, u0, u1, &pdf);
+        const V3 wi = norm(lp.pos - it.pos);
+        V3 Li = dot(lp.n, neg(wi)) > 0.f ? Le : v3(0.f, 0.f, 0.f);
+        if (pdf > 0.f && !is_zero(Li)) {
+            const float sp = bsdf_pdf(b, it.outDir, wi);
+            const V3 f = bsdf_eval(b, it.outDir, wi);
+            const float c = fabsf(dot(it.n, wi));
+            if (!is_zero(f)) {
+                V3 so, sd;
+                float stmax;
+                spawn_to(it, lp, &so, &sd, &stmax);
+                Hit hs;
+                uint32_t* nn = COUNT ? &pc.nodes : nullptr;
+                uint32_t* nt = COUNT ? &pc.tris : nullptr;
+                pc.shadow++;
+                if (traverse<true, STACK>(S, so, sd, 0.f, stmax, &hs, stack, stride, nn, nt)) Li = v3s(0.f);
+                w1 = power_heuristic(pdf, sp);
+                L1 = Li * v3s(1.f) * f * c / pdf;
+            }
+        }
+    }
+    {
+        V3 wi;
+        float pdf;
+        const V3 f = bsdf_sample(b, it.outDir, &wi, &pdf, v0, v1, v2);
+        const float c = fabsf(dot(it.n, wi));
+        if (pdf != 0.f && !is_zero(f)) {
+            const float lpdf = light_pdf(S, ltri, it, wi);
+            if (lpdf == 0.f) {
+                w2 = 0.f;
+            } else {
+                w2 = power_heuristic(pdf, lpdf);
+                V3 ro;
+                spawn_ray(it, wi, &ro);
+                Hit h2;
+                uint32_t* nn = COUNT ? &pc.nodes : nullptr;
+                uint32_t* nt = COUNT ? &pc.tris : nullptr;
+                pc.mis++;
+                if (traverse<false, STACK>(S, ro, wi, 0.f, kInf, &h2, stack, stride, nn, nt)) {
+                    const int hl = __float_as_int(S.shade[(size_t)h2.prim * 8 + 1].w);
+                    if (hl == light) {
+                        const Isect li = query(S, ro, wi, h2);
+                        const V3 Lem = dot(li.n, neg(wi)) > 0.f ? Le : v3(0.f, 0.f, 0.f);
+                        L2 = Lem * f * v3s(1.f) * c / pdf;
+                    }
+                }
+            }
+        }
+    }
+    return (L1 * w1 + L2 * w2) / lsp;
+}
+
+struct PathParams {
+    CamDev cam;
+    EnumDev en;
+    const uint32_t* sobol;
+    int32_t max_depth, rr_depth;
+};
+
+template <int STACK, bool COUNT>
+__device__ V3 trace_path(const SceneDev& S, const PathParams& P, int px, int py, int s, uint32_t* stack, int stride,
+                         float* fx, float* fy, uint32_t* dims, PathCounters& pc) {
+    const uint32_t* M = P.sobol;
+    Seq q;
+    q.index = enum_index(P.en, (uint32_t)s, (uint32_t)px, (uint32_t)py);
+    q.cursor = 0;
+    const float u = P.cam.res * seq_next(M, q) - (float)px;  // SobolEnum::SampleX (sampler/SobolEnum.h:30-32)
+    const float v = P.cam.res * seq_next(M, q) - (float)py;
+    V3 o, d;
+    generate_ray(P.cam, px, py, u, v, &o, &d);
+    bool delta = false;
+    V3 T = v3(1.f, 1.f, 1.f);
+    V3 L = v3s(0.f);
+    for (int i = 0; i < P.max_depth; ++i) {
+        Hit h;
+        pc.rays++;
+        uint32_t* nn = COUNT ? &pc.nodes : nullptr;
+        uint32_t* nt = COUNT ? &pc.tris : nullptr;
+        if (!traverse<false, STACK>(S, o, d, 0.f, kInf, &h, stack, stride, nn, nt)) break;
+        const Isect it = query(S, o, d, h);
+        const BSDF b = make_bsdf(S, it);
+        pc.shade++;
+        if ((i == 0 || delta) && it.light >= 0) {
+            const V3 Le = v3(S.light_le[3 * it.light], S.light_le[3 * it.light + 1], S.light_le[3 * it.light + 2]);
+            const V3 e = dot(it.n, it.outDir) > 0.f ? Le : v3(0.f, 0.f, 0.f);
+            L = L + T * e;
+        } else {
+            const float uL = seq_next(M, q);
+            const float u0 = seq_next(M, q), u1 = seq_next(M, q);
+            const float v0 = seq_next(M, q), v1 = seq_next(M, q), v2 = seq_next(M, q);
+            L = L + T * direct_illumination<STACK, COUNT>(S, it, b, uL, u0, u1, v0, v1, v2, stack, stride, pc);
+        }
+        const float r0 = seq_next(M, q), r1 = seq_next(M, q), r2 = seq_next(M, q);
+        V3 wi;
+        float pdf;
+        const V3 f = bsdf_sample(b, neg(d), &wi, &pdf, r0, r1, r2);
+        if (is_zero(f) || pdf == 0.f) break;
+        const V3 est = f * fabsf(dot(it.n, wi)) / pdf;
+        delta = bsdf_delta(b);
+        const float p = rmin(1.0f, lumY(est * T) / lumY(T));
+        if (p < 1.f && i > P.rr_depth) {
+            if (seq_next(M, q) < p) T = T / p;
+            else break;
+        }
+        T = T * est;
+        spawn_ray(it, wi, &o);
+        d = wi;
+    }
+    *fx = px + u;
+    *fy = py + v;
+    if (dims) *dims = q.cursor;
+    return L;
+}
+
+// Film::AddSample footprint with the radius-0.5 box filter (visual/Film.h:61-74).
+__device__ inline void film_footprint(float fx, float fy, int W, int H, int* x0, int* x1, int* y0, int* y1) {
+    const float x = fx - 0.5f, y = fy - 0.5f;
+    int minX = (int)ceilf(x - 0.5f), maxX = (int)floorf(x + 0.5f);
+    int minY = (int)ceilf(y - 0.5f), maxY = (int)floorf(y + 0.5f);
+    *x0 = minX < 0 ? 0 : minX;
+    *x1 = maxX > W - 1 ? W - 1 : maxX;
+    *y0 = minY < 0 ? 0 : minY;
+    *y1 = maxY > H - 1 ? H - 1 : maxY;
+}
+
+// Record one finished sample: own-pixel samples go to the pass buffer slot
+// (owner adds them in film_accumulate); any sample whose box-filter footprint is
+// not exactly its own pixel goes to the side list, one entry per covered pixel.
+__device__ inline void record_sample(const FilmDev& F, int s_local, int px, int py, float fx, float fy, V3 L) {
+    const int W = F.width, H = F.height;
+    int x0, x1, y0, y1;
+    film_footprint(fx, fy, W, H, &x0, &x1, &y0, &y1);
+    const int64_t WH = (int64_t)W * H;
+    const int64_t slot = (int64_t)s_local * WH + (int64_t)py * W + px;
+    const bool own = (x0 == px && x1 == px && y0 == py && y1 == py);
+    F.pass[slot] = make_float4(L.x, L.y, L.z, own ? 1.f : 0.f);
+    if (own) return;
+    for (int yy = y0; yy <= y1; ++yy)
+        for (int xx = x0; xx <= x1; ++xx) {
+            const uint32_t k = atomicAdd(F.side_count, 1u);
+            if (k < F.side_cap) {
+                SideEntry e;
+                e.key = ((uint64_t)((uint32_t)(yy * W + xx)) << 32) | (uint64_t)(uint32_t)slot;
+                e.L = make_float4(L.x, L.y, L.z, 0.f);
+                F.side[k] = e;
+            }
+        }
+}
+
+__device__ inline void pass_pixel(const FilmDev& F, int64_t t, int* s_local, int* px, int* py) {
+    // 8x8 pixel tiles, one tile per wavefront, tiles in row-major order.
+    const int tilesX = (F.width + 7) >> 3, tilesY = (F.height + 7) >> 3;
+    const int64_t per_s = (int64_t)tilesX * tilesY * 64;
+    *s_local = (int)(t / per_s);
+    const int64_t r = t - (int64_t)(*s_local) * per_s;
+    const int64_t tile = r >> 6;
+    const int local = (int)(r & 63);
+    *px = (int)(tile % tilesX) * 8 + (local & 7);
+    *py = (int)(tile / tilesX) * 8 + (local >> 3);
+}
+
+template <int STACK, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_pt_megakernel(SceneDev S, PathParams P, FilmDev F, int s0, int n_s,
+                                                         unsigned long long* counters) {
+    __shared__ uint32_t stack[STACK * BLOCK];
+    const int64_t t = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    int s_local, px, py;
+    pass_pixel(F, t, &s_local, &px, &py);
+    PathCounters pc = {0, 0, 0, 0, 0, 0};
+    if (s_local < n_s && px < F.width && py < F.height) {
+        float fx, fy;
+        const V3 L = trace_path<STACK, false>(S, P, px, py, s0 + s_local, stack + threadIdx.x, BLOCK, &fx, &fy, nullptr, pc);
+        record_sample(F, s_local, px, py, fx, fy, L);
+    }
+    wave_add_u64(&counters[0], pc.rays);
+    wave_add_u64(&counters[1], pc.shadow);
+    wave_add_u64(&counters[2], pc.mis);
+}
+
+// Instrumented copy: counts node and triangle visits and shading points.
+template <int STACK, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_pt_count(SceneDev S, PathParams P, FilmDev F, int s0, int n_s,
+                                                    unsigned long long* counters) {
+    __shared__ uint32_t stack[STACK * BLOCK];
+    const int64_t t = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    int s_local, px, py;
+    pass_pixel(F, t, &s_local, &px, &py);
+    PathCounters pc = {0, 0, 0, 0, 0, 0};
+    if (s_local < n_s && px < F.width && py < F.height) {
+        float fx, fy;
+        (void)trace_path<STACK, true>(S, P, px, py, s0 + s_local, stack + threadIdx.x, BLOCK, &fx, &fy, nullptr, pc);
+    }
+    wave_add_u64(&counters[0], pc.rays);
+    wave_add_u64(&counters[1], pc.shadow);
+    wave_add_u64(&counters[2], pc.mis);
+    wave_add_u64(&counters[3], pc.shade);
+    wave_add_u64(&counters[4], pc.nodes);
+    wave_add_u64(&counters[5], pc.tris);
+}
+
+template <int STACK>
+__global__ void __launch_bounds__(128) k_li_samples(SceneDev S, PathParams P, int n, const int32_t* __restrict__ xs,
+                                                    const int32_t* __restrict__ ys, const int32_t* __restrict__ ss,
+                                                    float* __restrict__ L, float* __restrict__ raster, int32_t* __restrict__ dims) {
+    __shared__ uint32_t stack[STACK * 128];
+    const int i = blockIdx.x * 128 + threadIdx.x;
+    if (i >= n) return;
+    PathCounters pc = {0, 0, 0, 0, 0, 0};
+    float fx, fy;
+    uint32_t nd;
+    const V3 v = trace_path<STACK, false>(S, P, xs[i], ys[i], ss[i], stack + threadIdx.x, 128, &fx, &fy, &nd, pc);
+    L[3 * i] = v.x;
+    L[3 * i + 1] = v.y;
+    L[3 * i + 2] = v.z;
+    raster[2 * i] = fx;
+    raster[2 * i + 1] = fy;
+    dims[i] = (int32_t)nd;
+}
+
+// ------------------------------------------------------------------ film
+// Each pixel's owner thread adds the pass's own samples in sample order
+// (Film::AddSample with weight 1: weight += 1, color = color + L, visual/Film.h:80-85).
+__global__ void k_film_accumulate(FilmDev F, int n_s) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t WH = (int64_t)F.width * F.height;
+    if (p >= WH) return;
+    float4 acc = F.film[p];
+    for (int s = 0; s < n_s; ++s) {
+        const float4 e = F.pass[(int64_t)s * WH + p];
+        if (e.w != 0.f) {
+            acc.w += 1.0f;
+            acc.x = acc.x + e.x;
+            acc.y = acc.y + e.y;
+            acc.z = acc.z + e.z;
+        }
+    }
+    F.film[p] = acc;
+}
+
+// Side entries (samples straddling pixel borders, see record_sample) are ranked
+// by key (target pixel, source slot), which orders them deterministically, and
+// each target's run is applied by one thread in key order.
+__global__ void k_side_rank(FilmDev F) {
+    const uint32_t n = min(*F.side_count, F.side_cap);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t k = F.side[i].key;
+    uint32_t r = 0;
+    for (uint32_t j = 0; j < n; ++j) r += (F.side[j].key < k) ? 1u : 0u;
+    F.side_sorted[r] = F.side[i];
+}
+__global__ void k_side_apply(FilmDev F) {
+    const uint32_t n = min(*F.side_count, F.side_cap);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t tgt = (uint32_t)(F.side_sorted[i].key >> 32);
+    if (i > 0 && (uint32_t)(F.side_sorted[i - 1].key >> 32) == tgt) return;
+    float4 acc = F.film[tgt];
+    for (uint32_t j = i; j < n && (uint32_t)(F.side_sorted[j].key >> 32) == tgt; ++j) {
+        const float4 e = F.side_sorted[j].L;
+        acc.w += 1.0f;
+        acc.x = acc.x + e.x;
+        acc.y = acc.y + e.y;
+        acc.z = acc.z + e.z;
+    }
+    F.film[tgt] = acc;
+}
+__global__ void k_side_overflow(FilmDev F) {
+    if (threadIdx.x == 0 && blockIdx.x == 0 && *F.side_count > F.side_cap) atomicMax(F.side_overflow, *F.side_count);
+}
+
+// ------------------------------------------------------------------ launch helpers
+template <int STACK>
+int LaunchTraceClosestT(Ctx* c, const SceneDev& S, const gy_ray* rays, int64_t n, gy_hit* hits) {
+    hipLaunchKernelGGL(k_trace_closest<STACK>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, S, rays, n, hits);
+    return CheckLaunch(c, "k_trace_closest");
+}
+template <int STACK>
+int LaunchTraceAnyT(Ctx* c, const SceneDev& S, const gy_ray* rays, int64_t n, int32_t* occ) {
+    hipLaunchKernelGGL(k_trace_any<STACK>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, S, rays, n, occ);
+    return CheckLaunch(c, "k_trace_any");
+}
+
+int LaunchTraceClosest(Ctx* c, Scene* sc, const gy_ray* rays, int64_t n, gy_hit* hits) {
+    if (n <= 0) return GY_OK;
+    return sc->stack <= 32 ? LaunchTraceClosestT<32>(c, sc->dev, rays, n, hits) : LaunchTraceClosestT<64>(c, sc->dev, rays, n, hits);
+}
+int LaunchTraceAny(Ctx* c, Scene* sc, const gy_ray* rays, int64_t n, int32_t* occ) {
+    if (n <= 0) return GY_OK;
+    return sc->stack <= 32 ? LaunchTraceAnyT<32>(c, sc->dev, rays, n, occ) : LaunchTraceAnyT<64>(c, sc->dev, rays, n, occ);
+}
+
+// ------------------------------------------------------------------ scene upload
+static float HostDot(const float* a, const float* b) { return (a[0] * b[0] + a[1] * b[1]) + (a[2] * b[2] + 0.f); }
+
+int SceneUpload(Ctx* c, const gy_scene_desc& d, const gy_bvh_params* bp, Scene** out) {
+    if (c->fault == GY_FAULT_SCENE) {
+        c->fault = GY_FAULT_NONE;
+        return Fail(GY_ERR_INJECTED, "injected scene failure");
+    }
+    if (d.n_tri <= 0 || !d.pos || !d.nrm || !d.face_n || !d.mat || d.n_mat <= 0 || !d.mats)
+        return Fail(GY_ERR_INVALID, "scene: need n_tri > 0, pos, nrm, face_n, mat, mats");
+    if (d.n_light <= 0 || !d.light_tri || !d.light_le) return Fail(GY_ERR_INVALID, "scene: need at least one light");
+    for (int i = 0; i < d.n_tri; ++i)
+        if (d.mat[i] >= (uint32_t)d.n_mat) return Fail(GY_ERR_INVALID, "scene: material id out of range");
+    for (int i = 0; i < d.n_light; ++i)
+        if (d.light_tri[i] < 0 || d.light_tri[i] >= d.n_tri) return Fail(GY_ERR_INVALID, "scene: light triangle out of range");
+    if (d.light)
+        for (int i = 0; i < d.n_tri; ++i)
+            if (d.light[i] >= d.n_light) return Fail(GY_ERR_INVALID, "scene: light id out of range");
+    for (int i = 0; i < d.n_mat; ++i) {
+        const uint32_t* w = d.mats + (size_t)i * GY_MAT_WORDS;
+        if (w[0] > 3) return Fail(GY_ERR_INVALID, "scene: unknown material type");
+        for (int k = 21; k <= 22; ++k)
+            if ((int32_t)w[k] >= d.n_tex) return Fail(GY_ERR_INVALID, "scene: texture id out of range");
+    }
+    for (int i = 0; i < d.n_tex; ++i)
+        if (d.tex_info[4 * i] <= 0 || d.tex_info[4 * i + 1] <= 0 || d.tex_info[4 * i + 2] <= 0 || d.tex_info[4 * i + 2] > 4)
+            return Fail(GY_ERR_INVALID, "scene: bad texture dimensions");
+    auto t0 = std::chrono::steady_clock::now();
+    Scene* sc = new Scene();
+    sc->ctx = c;
+    sc->n_tri = d.n_tri;
+    // Shading records by primitive id; area as Triangle::Area (shape/Triangle.h:67-69).
+    std::vector<float4> shade((size_t)d.n_tri * 8);
+    std::vector<float> area(d.n_tri);
+    for (int i = 0; i < d.n_tri; ++i) {
+        const float* p = d.pos + 9 * (size_t)i;
+        const float* n = d.nrm + 9 * (size_t)i;
+        const float* f = d.face_n + 3 * (size_t)i;
+        const float e1[3] = {p[3] - p[0], p[4] - p[1], p[5] - p[2]};
+        const float e2[3] = {p[6] - p[0], p[7] - p[1], p[8] - p[2]};
+        const float cr[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+        area[i] = (float)(0.5 * std::sqrt(HostDot(cr, cr)));
+        float uv[6] = {0, 0, 0, 0, 0, 0};
+        if (d.uv) std::memcpy(uv, d.uv + 6 * (size_t)i, sizeof(uv));
+        uint32_t mat = d.mat[i];
+        int32_t light = d.light ? d.light[i] : -1;
+        float matf, lightf;
+        std::memcpy(&matf, &mat, 4);
+        std::memcpy(&lightf, &light, 4);
+        float4* q = &shade[(size_t)i * 8];
+        q[0] = make_float4(p[0], p[1], p[2], matf);
+        q[1] = make_float4(p[3], p[4], p[5], lightf);
+        q[2] = make_float4(p[6], p[7], p[8], area[i]);
+        q[3] = make_float4(n[0], n[1], n[2], uv[0]);
+        q[4] = make_float4(n[3], n[4], n[5], uv[1]);
+        q[5] = make_float4(n[6], n[7], n[8], uv[2]);
+        q[6] = make_float4(f[0], f[1], f[2], uv[3]);
+        q[7] = make_float4(uv[4], uv[5], 0.f, 0.f);
+    }
+    // Light power distribution (visual/Scene.h:121-128, visual/Sampling.h:16-45,
+    // light/AreaLight.h:45-48) in FP32.
+    const int nl = d.n_light;
+    std::vector<float> func(nl), cdf(nl + 1);
+    const float PI = 3.14159265358979f;
+    for (int i = 0; i < nl; ++i) {
+        const float* le = d.light_le + 3 * i;
+        const float a = area[d.light_tri[i]];
+        const float P[3] = {(le[0] * a) * PI, (le[1] * a) * PI, (le[2] * a) * PI};
+        func[i] = 0.212671f * P[0] + 0.715160f * P[1] + 0.072169f * P[2];
+    }
+    cdf[0] = 0.f;
+    for (int i = 1; i < nl + 1; ++i) cdf[i] = cdf[i - 1] + func[i - 1] / nl;
+    const float func_int = cdf[nl];
+    if (func_int == 0.f) {
+        for (int i = 1; i < nl + 1; ++i) cdf[i] = float(i) / float(nl);
+    } else {
+        for (int i = 1; i < nl + 1; ++i) cdf[i] /= func_int;
+    }
+    sc->dev.light_func_int = func_int;
+    sc->dev.n_light = nl;
+    sc->dev.n_tri = d.n_tri;
+    // BVH
+    gy_bvh_params bvh = {};
+    if (bp) bvh = *bp;
+    const int leaf = bvh.max_leaf_size > 0 ? bvh.max_leaf_size : 4;
+    int rc = GY_OK;
+    BVHBuild b;
+    auto tb0 = std::chrono::steady_clock::now();
+    if (bvh.builder == 0) {
+        rc = BuildBVHDevice(c, d.pos, d.n_tri, leaf, &b);
+        if (rc != GY_OK) {
+            delete sc;
+            return rc;
+        }
+    } else {
+        BuildBVHHost(d.pos, d.n_tri, leaf, &b);
+    }
+    auto tb1 = std::chrono::steady_clock::now();
+    sc->stats.build_ms = std::chrono::duration<double, std::milli>(tb1 - tb0).count();
+    sc->stats.n_nodes = (int64_t)(b.nodes.size() / 16);
+    sc->stats.n_leaves = b.n_leaves;
+    sc->stats.max_depth = b.max_depth;
+    sc->stats.max_leaf_size = b.max_leaf;
+    sc->stats.sah_cost = b.sah_cost;
+    if (b.max_depth + 1 > 64) {
+        delete sc;
+        return Fail(GY_ERR_UNSUPPORTED, "BVH deeper than the 64-entry traversal stack");
+    }
+    sc->stack = (b.max_depth + 1 <= 32) ? 32 : 64;
+    // Pad every child box outward so that the traversal never rejects a box
+    // containing a triangle the triangle test accepts (rounding in that test
+    // can place an accepted hit slightly outside the triangle).
+    float ext = 0.f;
+    for (size_t i = 0; i < (size_t)d.n_tri * 9; ++i) ext = std::max(ext, std::fabs(d.pos[i]));
+    const float pad = 4e-6f * (1.f + ext);
+    for (size_t nidx = 0; nidx < b.nodes.size() / 16; ++nidx) {
+        uint32_t* nd = &b.nodes[nidx * 16];
+        for (int s = 0; s < 2; ++s) {
+            if (nd[12 + s] == kEmptyRef) continue;
+            float v[6];
+            std::memcpy(v, &nd[s * 6], 24);
+            for (int k = 0; k < 3; ++k) {
+                v[k] -= pad;
+                v[3 + k] += pad;
+            }
+            std::memcpy(&nd[s * 6], v, 24);
+        }
+    }
+    sc->tri_order = b.tri_order;
+    std::vector<float4> tri((size_t)d.n_tri * 3);
+    for (size_t k = 0; k < b.tri_order.size(); ++k) {
+        const uint32_t id = b.tri_order[k];
+        const float* p = d.pos + 9 * (size_t)id;
+        int32_t idi = (int32_t)id;
+        float idf;
+        std::memcpy(&idf, &idi, 4);
+        tri[k * 3 + 0] = make_float4(p[0], p[1], p[2], idf);
+        tri[k * 3 + 1] = make_float4(p[3], p[4], p[5], 0.f);
+        tri[k * 3 + 2] = make_float4(p[6], p[7], p[8], 0.f);
+    }
+    std::vector<float> texdata;
+    std::vector<int4> texinfo(std::max(1, d.n_tex));
+    size_t texn = 0;
+    for (int i = 0; i < d.n_tex; ++i)
+        texn = std::max(texn, (size_t)d.tex_info[4 * i + 3] + (size_t)d.tex_info[4 * i] * d.tex_info[4 * i + 1] * d.tex_info[4 * i + 2]);
+    if (d.n_tex) texdata.assign(d.tex_data, d.tex_data + texn);
+    for (int i = 0; i < d.n_tex; ++i)
+        texinfo[i] = make_int4(d.tex_info[4 * i], d.tex_info[4 * i + 1], d.tex_info[4 * i + 2], d.tex_info[4 * i + 3]);
+    void* p;
+    if ((rc = Upload(c, &p, tri.data(), tri.size() * sizeof(float4))) != GY_OK) goto fail;
+    sc->d_tri = p;
+    sc->dev.tri = (const float4*)p;
+    if ((rc = Upload(c, &p, shade.data(), shade.size() * sizeof(float4))) != GY_OK) goto fail;
+    sc->d_shade = p;
+    sc->dev.shade = (const float4*)p;
+    if ((rc = Upload(c, &p, b.nodes.data(), b.nodes.size() * 4)) != GY_OK) goto fail;
+    sc->d_nodes = p;
+    sc->dev.nodes = (const uint4*)p;
+    sc->dev.n_nodes = (int32_t)(b.nodes.size() / 16);
+    if ((rc = Upload(c, &p, d.mats, (size_t)d.n_mat * GY_MAT_WORDS * 4)) != GY_OK) goto fail;
+    sc->d_mats = p;
+    sc->dev.mats = (const uint32_t*)p;
+    if ((rc = Upload(c, &p, texdata.data(), texdata.size() * 4)) != GY_OK) goto fail;
+    sc->d_tex = p;
+    sc->dev.tex = (const float*)p;
+    if ((rc = Upload(c, &p, texinfo.data(), texinfo.size() * sizeof(int4))) != GY_OK) goto fail;
+    sc->d_texinfo = p;
+    sc->dev.tex_info = (const int4*)p;
+    if ((rc = Upload(c, &p, d.light_tri, (size_t)nl * 4)) != GY_OK) goto fail;
+    sc->d_ltri = p;
+    sc->dev.light_tri = (const int32_t*)p;
+    if ((rc = Upload(c, &p, d.light_le, (size_t)nl * 12)) != GY_OK) goto fail;
+    sc->d_lle = p;
+    sc->dev.light_le = (const float*)p;
+    if ((rc = Upload(c, &p, cdf.data(), cdf.size() * 4)) != GY_OK) goto fail;
+    sc->d_lcdf = p;
+    sc->dev.light_cdf = (const float*)p;
+    if ((rc = Upload(c, &p, func.data(), func.size() * 4)) != GY_OK) goto fail;
+    sc->d_lfunc = p;
+    sc->dev.light_func = (const float*)p;
+    sc->host_nodes = std::move(b.nodes);
+    {
+        auto t1 = std::chrono::steady_clock::now();
+        sc->upload_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    }
+    *out = sc;
+    return GY_OK;
+fail:
+    SceneFree(sc);
+    return rc;
+}
+
+void SceneFree(Scene* sc) {
+    if (!sc) return;
+    void* bufs[] = {sc->d_tri, sc->d_shade, sc->d_nodes, sc->d_mats, sc->d_tex, sc->d_texinfo, sc->d_ltri, sc->d_lle, sc->d_lcdf, sc->d_lfunc};
+    for (void* b : bufs) DevFree(b);
+    FilmFree(&sc->film);
+    WavefrontFree(sc);
+    delete sc;
+}
+
+void FilmFree(FilmState* f) {
+    DevFree(f->film);
+    DevFree(f->pass);
+    DevFree(f->side);
+    DevFree(f->side_sorted);
+    DevFree(f->side_count);
+    DevFree(f->counters);
+    *f = FilmState();
+}
+
+int FilmPrepare(Ctx* c, Scene* sc, int W, int H, int n_pass) {
+    FilmState& f = sc->film;
+    const int64_t WH = (int64_t)W * H;
+    if (f.width != W || f.height != H || !f.film) {
+        DevFree(f.film);
+        f.film = nullptr;
+        int rc = DevAlloc(c, (void**)&f.film, (size_t)WH * sizeof(float4));
+        if (rc != GY_OK) return rc;
+        f.width = W;
+        f.height = H;
+    }
+    if (f.pass_slots < WH * n_pass) {
+        DevFree(f.pass);
+        f.pass = nullptr;
+        int rc = DevAlloc(c, (void**)&f.pass, (size_t)WH * n_pass * sizeof(float4));
+        if (rc != GY_OK) return rc;
+        f.pass_slots = WH * n_pass;
+    }
+    if (!f.side) {
+        f.side_cap = 1u << 18;
+        int rc = DevAlloc(c, (void**)&f.side, (size_t)f.side_cap * sizeof(SideEntry));
+        if (rc == GY_OK) rc = DevAlloc(c, (void**)&f.side_sorted, (size_t)f.side_cap * sizeof(SideEntry));
+        if (rc == GY_OK) rc = DevAlloc(c, (void**)&f.side_count, 16);
+        if (rc == GY_OK) rc = DevAlloc(c, (void**)&f.counters, 16 * sizeof(unsigned long long));
+        if (rc != GY_OK) return rc;
+    }
+    hipLaunchKernelGGL(k_fill_f4, dim3((unsigned)((WH + 255) / 256)), dim3(256), 0, c->stream, f.film, WH);
+    int rc = CheckLaunch(c, "film clear");
+    if (rc != GY_OK) return rc;
+    rc = Fill32(c, f.side_count, 0u, 4);
+    if (rc != GY_OK) return rc;
+    return Fill32(c, f.counters, 0u, 32);
+}
+
+FilmDev FilmView(Scene* sc) {
+    FilmDev F;
+    F.film = sc->film.film;
+    F.pass = sc->film.pass;
+    F.side = sc->film.side;
+    F.side_sorted = sc->film.side_sorted;
+    F.side_count = sc->film.side_count;
+    F.side_overflow = sc->film.side_count + 1;
+    F.side_cap = sc->film.side_cap;
+    F.width = sc->film.width;
+    F.height = sc->film.height;
+    return F;
+}
+
+// Film accumulation of one pass: owners add own samples, then the side list.
+int FilmResolvePass(Ctx* c, Scene* sc, int n_s) {
+    FilmDev F = FilmView(sc);
+    const int64_t WH = (int64_t)F.width * F.height;
+    TimerBegin(c, kKernFilm);
+    hipLaunchKernelGGL(k_film_accumulate, dim3((unsigned)((WH + 255) / 256)), dim3(256), 0, c->stream, F, n_s);
+    int rc = CheckLaunch(c, "k_film_accumulate");
+    if (rc != GY_OK) return rc;
+    const unsigned sb = (F.side_cap + 255) / 256;
+    hipLaunchKernelGGL(k_side_overflow, dim3(1), dim3(64), 0, c->stream, F);
+    hipLaunchKernelGGL(k_side_rank, dim3(sb), dim3(256), 0, c->stream, F);
+    hipLaunchKernelGGL(k_side_apply, dim3(sb), dim3(256), 0, c->stream, F);
+    TimerEnd(c);
+    rc = CheckLaunch(c, "side list");
+    if (rc != GY_OK) return rc;
+    return Fill32(c, F.side_count, 0u, 1);
+}
+
+int MakePathParams(Ctx* c, Scene* sc, const gy_camera_desc& cam, const gy_render_params& p, PathParams* P) {
+    CameraRT rt;
+    int rc = BuildCamera(cam, &rt);
+    if (rc != GY_OK) return rc;
+    if (rt.log2res > sobol::kMaxLog2Res) return Fail(GY_ERR_UNSUPPORTED, "image too large for the Sobol enumeration");
+    std::memcpy(P->cam.m, rt.raster_to_world, sizeof(P->cam.m));
+    std::memcpy(P->cam.pos, rt.pos, sizeof(P->cam.pos));
+    P->cam.width = rt.width;
+    P->cam.height = rt.height;
+    P->cam.res = rt.res;
+    P->cam.log2res = rt.log2res;
+    const sobol::EnumTables& E = sobol::Enum(rt.log2res);
+    P->en.m = (uint32_t)rt.log2res;
+    P->en.pad = 0;
+    for (int i = 0; i < 32; ++i) {
+        P->en.inv[i] = E.inv[i];
+        P->en.frame[i] = E.frame[i];
+    }
+    P->sobol = c->d_sobol;
+    P->max_depth = p.max_depth;
+    P->rr_depth = p.rr_depth;
+    // dimensions a path may consume: 2 + depth * (6 + 3 + 1)
+    const int64_t dims = 2 + (int64_t)p.max_depth * 10;
+    if (dims > sobol::kDims) return Fail(GY_ERR_UNSUPPORTED, "max_depth needs more Sobol dimensions than the table holds");
+    const int bits = 2 * rt.log2res + (p.spp > 1 ? 32 - __builtin_clz((unsigned)(p.spp - 1)) : 0);
+    if (bits > sobol::kCols) return Fail(GY_ERR_UNSUPPORTED, "sample index beyond 52 bits");
+    (void)sc;
+    return GY_OK;
+}
+
+int ValidateRender(const gy_camera_desc* cam, const gy_render_params* p) {
+    if (!cam || !p) return Fail(GY_ERR_INVALID, "render: null camera or params");
+    if (p->spp <= 0 || p->max_depth < 0 || p->max_depth > 100) return Fail(GY_ERR_INVALID, "render: spp > 0 and 0 <= max_depth <= 100");
+    if ((int64_t)cam->width * cam->height > (1ll << 26)) return Fail(GY_ERR_UNSUPPORTED, "render: image larger than 2^26 pixels");
+    return GY_OK;
+}
+
+template <int STACK, int BLOCK>
+int LaunchMegaT(Ctx* c, Scene* sc, const PathParams& P, int s0, int n_s, bool count) {
+    FilmDev F = FilmView(sc);
+    const int tilesX = (F.width + 7) >> 3, tilesY = (F.height + 7) >> 3;
+    const int64_t threads = (int64_t)tilesX * tilesY * 64 * n_s;
+    const unsigned blocks = (unsigned)((threads + BLOCK - 1) / BLOCK);
+    unsigned long long* ctr = sc->film.counters;
+    if (count) {
+        hipLaunchKernelGGL((k_pt_count<STACK, BLOCK>), dim3(blocks), dim3(BLOCK), 0, c->stream, sc->dev, P, F, s0, n_s, ctr);
+        return CheckLaunch(c, "k_pt_count");
+    }
+    TimerBegin(c, kKernMega);
+    hipLaunchKernelGGL((k_pt_megakernel<STACK, BLOCK>), dim3(blocks), dim3(BLOCK), 0, c->stream, sc->dev, P, F, s0, n_s, ctr);
+    TimerEnd(c);
+    return CheckLaunch(c, "k_pt_megakernel");
+}
+
+int LaunchMega(Ctx* c, Scene* sc, const PathParams& P, int s0, int n_s, int block, bool count) {
+    if (sc->stack <= 32) {
+        if (block == 64) return LaunchMegaT<32, 64>(c, sc, P, s0, n_s, count);
+        if (block == 256) return LaunchMegaT<32, 256>(c, sc, P, s0, n_s, count);
+        return LaunchMegaT<32, 128>(c, sc, P, s0, n_s, count);
+    }
+    if (block == 64) return LaunchMegaT<64, 64>(c, sc, P, s0, n_s, count);
+    return LaunchMegaT<64, 128>(c, sc, P, s0, n_s, count);
+}
+
+int SamplesPerPass(const gy_camera_desc& cam, const gy_render_params& p) {
+    if (p.samples_per_pass > 0) return std::min(p.samples_per_pass, p.spp);
+    const int64_t WH = (int64_t)cam.width * cam.height;
+    int64_t s = (4ll << 20) / std::max<int64_t>(1, WH);
+    if (s < 1) s = 1;
+    if (s > p.spp) s = p.spp;
+    return (int)s;
+}
+
+}  // namespace gy
+
+// ====================================================================== C ABI
+using namespace gy;
+
+extern "C" const char* gy_last_error(void) { return t_error.c_str(); }
+extern "C" const char* gy_version(void) { return "gyunity-amd 0.1 (gfx950)"; }
+
+extern "C" int gy_open(int device, gy_context** out) {
+    if (!out) return Fail(GY_ERR_INVALID, "gy_open: null output");
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) return Fail(GY_ERR_DEVICE, "gy_open: no HIP device available");
+    if (device < 0 || device >= n) return Fail(GY_ERR_INVALID, "gy_open: device index out of range");
+    e = hipSetDevice(device);
+    if (e != hipSuccess) return HipFail(e, "hipSetDevice");
+    Ctx* c = new Ctx();
+    c->device = device;
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return HipFail(e, "hipStreamCreate");
+    }
+    void* p = nullptr;
+    int rc = Upload(c, &p, sobol::Matrices(), (size_t)sobol::kDims * sobol::kCols * 4);
+    if (rc != GY_OK) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return rc;
+    }
+    c->d_sobol = (const uint32_t*)p;
+    *out = reinterpret_cast<gy_context*>(c);
+    return GY_OK;
+}
+
+extern "C" int gy_close(gy_context* ctx) {
+    if (!ctx) return GY_OK;
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    DevFree((void*)c->d_sobol);
+    DevFree(c->rays.ptr);
+    DevFree(c->hits.ptr);
+    DevFree(c->ctr.ptr);
+    for (hipEvent_t ev : c->events) (void)hipEventDestroy(ev);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return GY_OK;
+}
+
+extern "C" int gy_device_name(gy_context* ctx, char* buf, int len) {
+    if (!ctx || !buf || len <= 0) return Fail(GY_ERR_INVALID, "gy_device_name: bad argument");
+    hipDeviceProp_t prop;
+    hipError_t e = hipGetDeviceProperties(&prop, reinterpret_cast<Ctx*>(ctx)->device);
+    if (e != hipSuccess) return HipFail(e, "hipGetDeviceProperties");
+    std::snprintf(buf, len, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+    return GY_OK;
+}
+
+extern "C" int gy_debug_inject_fault(gy_context* ctx, int kind) {
+    if (!ctx) return Fail(GY_ERR_INVALID, "gy_debug_inject_fault: null context");
+    if (kind < GY_FAULT_NONE || kind > GY_FAULT_SCENE) return Fail(GY_ERR_INVALID, "gy_debug_inject_fault: unknown kind");
+    reinterpret_cast<Ctx*>(ctx)->fault = kind;
+    return GY_OK;
+}
+
+extern "C" int gy_scene_create(gy_context* ctx, const gy_scene_desc* desc, const gy_bvh_params* bvh, gy_scene** out) {
+    if (!ctx || !desc || !out) return Fail(GY_ERR_INVALID, "gy_scene_create: null argument");
+    *out = nullptr;
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    (void)hipSetDevice(c->device);
+    Scene* sc = nullptr;
+    int rc = SceneUpload(c, *desc, bvh, &sc);
+    if (rc != GY_OK) return rc;
+    *out = reinterpret_cast<gy_scene*>(sc);
+    return GY_OK;
+}
+
+extern "C" int gy_scene_destroy(gy_scene* s) {
+    if (!s) return GY_OK;
+    Scene* sc = reinterpret_cast<Scene*>(s);
+    (void)hipStreamSynchronize(sc->ctx->stream);
+    SceneFree(sc);
+    return GY_OK;
+}
+
+extern "C" int gy_scene_stats_get(gy_scene* s, gy_scene_stats* out) {
+    if (!s || !out) return Fail(GY_ERR_INVALID, "gy_scene_stats_get: null argument");
+    *out = reinterpret_cast<Scene*>(s)->stats;
+    return GY_OK;
+}
+
+extern "C" int gy_scene_bvh_download(gy_scene* s, uint32_t* nodes, int64_t max_nodes, int32_t* tri_order, int64_t max_tris) {
+    if (!s) return Fail(GY_ERR_INVALID, "gy_scene_bvh_download: null scene");
+    Scene* sc = reinterpret_cast<Scene*>(s);
+    const int64_t nn = (int64_t)sc->host_nodes.size() / 16;
+    if (nodes) {
+        if (max_nodes < nn) return Fail(GY_ERR_INVALID, "gy_scene_bvh_download: node buffer too small");
+        std::memcpy(nodes, sc->host_nodes.data(), sc->host_nodes.size() * 4);
+    }
+    if (tri_order) {
+        if (max_tris < (int64_t)sc->tri_order.size()) return Fail(GY_ERR_INVALID, "gy_scene_bvh_download: order buffer too small");
+        for (size_t i = 0; i < sc->tri_order.size(); ++i) tri_order[i] = (int32_t)sc->tri_order[i];
+    }
+    return GY_OK;
+}
+
+static int TraceHost(gy_scene* s, const gy_ray* rays, int64_t n, void* out, bool any) {
+    if (!s || (n > 0 && (!rays || !out)) || n < 0) return Fail(GY_ERR_INVALID, "trace: bad argument");
+    if (n == 0) return GY_OK;
+    Scene* sc = reinterpret_cast<Scene*>(s);
+    Ctx* c = sc->ctx;
+    (void)hipSetDevice(c->device);
+    int rc = Grow(c, &c->rays, (size_t)n * sizeof(gy_ray));
+    if (rc == GY_OK) rc = Grow(c, &c->hits, (size_t)n * sizeof(gy_hit));
+    if (rc != GY_OK) return rc;
+    hipError_t e = hipMemcpyAsync(c->rays.ptr, rays, (size_t)n * sizeof(gy_ray), hipMemcpyHostToDevice, c->stream);
+    if (e != hipSuccess) return HipFail(e, "trace H2D");
+    hipEvent_t a, b;
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+    (void)hipEventRecord(a, c->stream);
+    rc = any ? LaunchTraceAny(c, sc, (const gy_ray*)c->rays.ptr, n, (int32_t*)c->hits.ptr)
+             : LaunchTraceClosest(c, sc, (const gy_ray*)c->rays.ptr, n, (gy_hit*)c->hits.ptr);
+    (void)hipEventRecord(b, c->stream);
+    if (rc == GY_OK) {
+        e = hipMemcpyAsync(out, c->hits.ptr, (size_t)n * (any ? 4 : sizeof(gy_hit)), hipMemcpyDeviceToHost, c->stream);
+        if (e != hipSuccess) rc = HipFail(e, "trace D2H");
+    }
+    e = hipStreamSynchronize(c->stream);
+    if (rc == GY_OK && e != hipSuccess) rc = HipFail(e, "trace sync");
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, a, b);
+    c->last_kernel_ms = ms;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return rc;
+}
+
+extern "C" int gy_trace_closest(gy_scene* s, const gy_ray* rays, int64_t n, gy_hit* hits) { return TraceHost(s, rays, n, hits, false); }
+extern "C" int gy_trace_any(gy_scene* s, const gy_ray* rays, int64_t n, int32_t* occ) { return TraceHost(s, rays, n, occ, true); }
+
+static int TraceDevice(gy_scene* s, const gy_ray* rays, int64_t n, void* out, bool any) {
+    if (!s || n < 0 || (n > 0 && (!rays || !out))) return Fail(GY_ERR_INVALID, "trace_device: bad argument");
+    if (n == 0) return GY_OK;
+    Scene* sc = reinterpret_cast<Scene*>(s);
+    Ctx* c = sc->ctx;
+    (void)hipSetDevice(c->device);
+    hipEvent_t a, b;
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+    (void)hipEventRecord(a, c->stream);
+    int rc = any ? LaunchTraceAny(c, sc, rays, n, (int32_t*)out) : LaunchTraceClosest(c, sc, rays, n, (gy_hit*)out);
+    (void)hipEventRecord(b, c->stream);
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (rc == GY_OK && e != hipSuccess) rc = HipFail(e, "trace sync");
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, a, b);
+    c->last_kernel_ms = ms;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return rc;
+}
+extern "C" int gy_trace_closest_device(gy_scene* s, const gy_ray* rays, int64_t n, gy_hit* hits) {
+    return TraceDevice(s, rays, n, hits, false);
+}
+extern "C" int gy_trace_any_device(gy_scene* s, const gy_ray* rays, int64_t n, int32_t* occ) { return TraceDevice(s, rays, n, occ, true); }
+
+extern "C" double gy_last_kernel_ms(gy_context* ctx) { return ctx ? reinterpret_cast<Ctx*>(ctx)->last_kernel_ms : -1.0; }
+
+extern "C" int gy_count_visits(gy_scene* s, const gy_ray* rays, int64_t n, int64_t* node_visits, int64_t* tri_visits) {
+    if (!s || n < 0 || (n > 0 && !rays) || !node_visits || !tri_visits) return Fail(GY_ERR_INVALID, "gy_count_visits: bad argument");
+    Scene* sc = reinterpret_cast<Scene*>(s);
+    Ctx* c = sc->ctx;
+    (void)hipSetDevice(c->device);
+    int rc = Grow(c, &c->rays, (size_t)std::max<int64_t>(n, 1) * sizeof(gy_ray));
+    if (rc == GY_OK) rc = Grow(c, &c->ctr, 64);
+    if (rc != GY_OK) return rc;
+    if (n > 0) {
+        hipError_t e = hipMemcpyAsync(c->rays.ptr, rays, (size_t)n * sizeof(gy_ray), hipMemcpyHostToDevice, c->stream);
+        if (e != hipSuccess) return HipFail(e, "count H2D");
+    }
+    rc = Fill32(c, c->ctr.ptr, 0u, 4);
+    if (rc != GY_OK) return rc;
+    if (n > 0) {
+        if (sc->stack <= 32)
+            hipLaunchKernelGGL(k_count_visits<32>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, sc->dev,
+                               (const gy_ray*)c->rays.ptr, n, (unsigned long long*)c->ctr.ptr);
+        else
+            hipLaunchKernelGGL(k_count_visits<64>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, sc->dev,
+                               (const gy_ray*)c->rays.ptr, n, (unsigned long long*)c->ctr.ptr);
+        rc = CheckLaunch(c, "k_count_visits");
+        if (rc != GY_OK) return rc;
+    }
+    unsigned long long h[2] = {0, 0};
+    hipError_t e = hipMemcpyAsync(h, c->ctr.ptr, 16, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return HipFail(e, "count D2H");
+    *node_visits = (int64_t)h[0];
+    *tri_visits = (int64_t)h[1];
+    return GY_OK;
+}
+
+extern "C" int gy_render(gy_scene* s, const gy_camera_desc* cam, const gy_render_params* p, float* color, float* weight) {
+    if (!s) return Fail(GY_ERR_INVALID, "gy_render: null scene");
+    int rc = ValidateRender(cam, p);
+    if (rc != GY_OK) return rc;
+    Scene* sc = reinterpret_cast<Scene*>(s);
+    Ctx* c = sc->ctx;
+    (void)hipSetDevice(c->device);
+    PathParams P;
+    rc = MakePathParams(c, sc, *cam, *p, &P);
+    if (rc != GY_OK) return rc;
+    const int n_pass = SamplesPerPass(*cam, *p);
+    rc = FilmPrepare(c, sc, cam->width, cam->height, n_pass);
+    if (rc != GY_OK) return rc;
+    std::memset(&c->stats, 0, sizeof(c->stats));
+    c->ev_used = 0;
+    hipEvent_t t0, t1;
+    (void)hipEventCreate(&t0);
+    (void)hipEventCreate(&t1);
+    (void)hipEventRecord(t0, c->stream);
+    int block = p->block_size > 0 ? p->block_size : 128;
+    for (int s0 = 0; s0 < p->spp && rc == GY_OK; s0 += n_pass) {
+        const int n_s = std::min(n_pass, p->spp - s0);
+        if (p->mode == GY_RENDER_WAVEFRONT) rc = WavefrontPass(c, sc, P, *p, s0, n_s);
+        else rc = LaunchMega(c, sc, P, s0, n_s, block, false);
+        if (rc == GY_OK) rc = FilmResolvePass(c, sc, n_s);
+    }
+    (void)hipEventRecord(t1, c->stream);
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (rc == GY_OK && e != hipSuccess) rc = HipFail(e, "render sync");
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, t0, t1);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    TimersCollect(c);
+    c->stats.total_ms = ms;
+    c->stats.paths = (int64_t)cam->width * cam->height * p->spp;
+    if (rc == GY_OK) {
+        unsigned long long h[8];
+        uint32_t ovf[2];
+        e = hipMemcpy(h, sc->film.counters, sizeof(h), hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(ovf, sc->film.side_count, 8, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return HipFail(e, "render counters");
+        if (ovf[1] != 0) return Fail(GY_ERR_UNSUPPORTED, "film side list overflow");
+        c->stats.rays = (int64_t)(h[0] + h[1] + h[2]);
+        c->stats.shadow_rays = (int64_t)h[1];
+        c->stats.mis_rays = (int64_t)h[2];
+    }
+    if (rc == GY_OK && (color || weight)) rc = gy_film_download(s, color, weight);
+    return rc;
+}
+
+extern "C" int gy_film_download(gy_scene* s, float* color, float* weight) {
+    if (!s) return Fail(GY_ERR_INVALID, "gy_film_download: null scene");
+    Scene* sc = reinterpret_cast<Scene*>(s);
+    if (!sc->film.film) return Fail(GY_ERR_INVALID, "gy_film_download: nothing rendered");
+    const int64_t WH = (int64_t)sc->film.width * sc->film.height;
+    std::vector<float4> h(WH);
+    hipError_t e = hipMemcpy(h.data(), sc->film.film, WH * sizeof(float4), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return HipFail(e, "film D2H");
+    for (int64_t i = 0; i < WH; ++i) {
+        if (color) {
+            color[3 * i] = h[i].x;
+            color[3 * i + 1] = h[i].y;
+            color[3 * i + 2] = h[i].z;
+        }
+        if (weight) weight[i] = h[i].w;
+    }
+    return GY_OK;
+}
+
+extern "C" int gy_render_stats_get(gy_context* ctx, gy_render_stats* out) {
+    if (!ctx || !out) return Fail(GY_ERR_INVALID, "gy_render_stats_get: null argument");
+    *out = reinterpret_cast<Ctx*>(ctx)->stats;
+    return GY_OK;
+}
+
+extern "C" const char* gy_kernel_name(int i) {
+    static const char* names[8] = {"pt_megakernel", "film_resolve", "wf_raygen", "wf_trace_closest", "wf_shade",
+                                   "wf_trace_shadow", "wf_trace_mis", "wf_other"};
+    return (i >= 0 && i < 8) ? names[i] : "";
+}
+
+extern "C" int gy_li_samples(gy_scene* s, const gy_camera_desc* cam, const gy_render_params* p, int32_t n, const int32_t* x,
+                             const int32_t* y, const int32_t* smp, float* L, float* raster, int32_t* dims) {
+    if (!s || n < 0 || (n > 0 && (!x || !y || !smp || !L || !raster || !dims))) return Fail(GY_ERR_INVALID, "gy_li_samples: bad argument");
+    int rc = ValidateRender(cam, p);
+    if (rc != GY_OK) return rc;
+    if (n == 0) return GY_OK;
+    for (int32_t i = 0; i < n; ++i)
+        if (x[i] < 0 || x[i] >= cam->width || y[i] < 0 || y[i] >= cam->height || smp[i] < 0 || smp[i] >= p->spp)
+            return Fail(GY_ERR_INVALID, "gy_li_samples: sample outside the image or spp");
+    Scene* sc = reinterpret_cast<Scene*>(s);
+    Ctx* c = sc->ctx;
+    (void)hipSetDevice(c->device);
+    PathParams P;
+    rc = MakePathParams(c, sc, *cam, *p, &P);
+    if (rc != GY_OK) return rc;
+    void *dx = nullptr, *dy = nullptr, *ds = nullptr, *dL = nullptr, *dr = nullptr, *dd = nullptr;
+    rc = Upload(c, &dx, x, n * 4);
+    if (rc == GY_OK) rc = Upload(c, &dy, y, n * 4);
+    if (rc == GY_OK) rc = Upload(c, &ds, smp, n * 4);
+    if (rc == GY_OK) rc = DevAlloc(c, &dL, (size_t)n * 12);
+    if (rc == GY_OK) rc = DevAlloc(c, &dr, (size_t)n * 8);
+    if (rc == GY_OK) rc = DevAlloc(c, &dd, (size_t)n * 4);
+    if (rc == GY_OK) {
+        const unsigned blocks = (unsigned)((n + 127) / 128);
+        if (sc->stack <= 32)
+            hipLaunchKernelGGL(k_li_samples<32>, dim3(blocks), dim3(128), 0, c->stream, sc->dev, P, n, (const int32_t*)dx,
+                               (const int32_t*)dy, (const int32_t*)ds, (float*)dL, (float*)dr, (int32_t*)dd);
+        else
+            hipLaunchKernelGGL(k_li_samples<64>, dim3(blocks), dim3(128), 0, c->stream, sc->dev, P, n, (const int32_t*)dx,
+                               (const int32_t*)dy, (const int32_t*)ds, (float*)dL, (float*)dr, (int32_t*)dd);
+        rc = CheckLaunch(c, "k_li_samples");
+    }
+    if (rc == GY_OK) {
+        hipError_t e = hipMemcpyAsync(L, dL, (size_t)n * 12, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(raster, dr, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(dims, dd, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = HipFail(e, "li_samples D2H");
+    }
+    DevFree(dx);
+    DevFree(dy);
+    DevFree(ds);
+    DevFree(dL);
+    DevFree(dr);
+    DevFree(dd);
+    return rc;
+}
+
+extern "C" int gy_count_traffic(gy_scene* s, const gy_camera_desc* cam, const gy_render_params* p, gy_render_stats* out) {
+    if (!s || !out) return Fail(GY_ERR_INVALID, "gy_count_traffic: null argument");
+    int rc = ValidateRender(cam, p);
+    if (rc != GY_OK) return rc;
+    Scene* sc = reinterpret_cast<Scene*>(s);
+    Ctx* c = sc->ctx;
+    (void)hipSetDevice(c->device);
+    PathParams P;
+    rc = MakePathParams(c, sc, *cam, *p, &P);
+    if (rc != GY_OK) return rc;
+    const int n_pass = SamplesPerPass(*cam, *p);
+    rc = FilmPrepare(c, sc, cam->width, cam->height, n_pass);
+    if (rc != GY_OK) return rc;
+    for (int s0 = 0; s0 < p->spp && rc == GY_OK; s0 += n_pass) rc = LaunchMega(c, sc, P, s0, std::min(n_pass, p->spp - s0), 128, true);
+    if (rc != GY_OK) return rc;
+    unsigned long long h[8];
+    hipError_t e = hipMemcpyAsync(h, sc->film.counters, sizeof(h), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return HipFail(e, "count D2H");
+    std::memset(out, 0, sizeof(*out));
+    out->paths = (int64_t)cam->width * cam->height * p->spp;
+    out->rays = (int64_t)(h[0] + h[1] + h[2]);
+    out->shadow_rays = (int64_t)h[1];
+    out->mis_rays = (int64_t)h[2];
+    out->shade_points = (int64_t)h[3];
+    out->node_visits = (int64_t)h[4];
+    out->tri_visits = (int64_t)h[5];
+    return GY_OK;
+}

@@ -1,0 +1,112 @@
+// Host-side runtime structures shared by the HIP translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "device.h"
+#include "gy_internal.h"
+
+namespace gy {
+
+enum KernelCategory {
+    kKernMega = 0,
+    kKernFilm = 1,
+    kKernRaygen = 2,
+    kKernTraceClosest = 3,
+    kKernShade = 4,
+    kKernTraceShadow = 5,
+    kKernTraceMis = 6,
+    kKernOther = 7
+};
+
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    const uint32_t* d_sobol = nullptr;
+    int fault = 0;
+    DevBuf rays, hits, ctr;
+    double last_kernel_ms = 0.0;
+    gy_render_stats stats = {};
+    std::vector<hipEvent_t> events;
+    std::vector<int> ev_cat;
+    int ev_used = 0;
+};
+
+struct SideEntry {
+    uint64_t key;  // (target pixel << 32) | source pass slot
+    uint64_t pad;
+    float4 L;
+};
+
+struct FilmState {
+    float4* film = nullptr;  // rgb sums + weight, one per pixel
+    float4* pass = nullptr;  // one per (pass sample, pixel): rgb + own flag
+    int64_t pass_slots = 0;
+    SideEntry* side = nullptr;
+    SideEntry* side_sorted = nullptr;
+    uint32_t* side_count = nullptr;  // [0] count, [1] overflow high-water mark
+    uint32_t side_cap = 0;
+    unsigned long long* counters = nullptr;
+    int width = 0, height = 0;
+};
+
+struct FilmDev {
+    float4* film;
+    float4* pass;
+    SideEntry* side;
+    SideEntry* side_sorted;
+    uint32_t* side_count;
+    uint32_t* side_overflow;
+    uint32_t side_cap;
+    int width, height;
+};
+
+// Wavefront queues and path state (wavefront.hip).
+struct WavefrontState {
+    int64_t capacity = 0;
+    void* mem = nullptr;
+    size_t bytes = 0;
+};
+
+struct Scene {
+    Ctx* ctx = nullptr;
+    int32_t n_tri = 0;
+    gyd::SceneDev dev = {};
+    void *d_tri = nullptr, *d_shade = nullptr, *d_nodes = nullptr, *d_mats = nullptr, *d_tex = nullptr, *d_texinfo = nullptr;
+    void *d_ltri = nullptr, *d_lle = nullptr, *d_lcdf = nullptr, *d_lfunc = nullptr;
+    int stack = 64;
+    gy_scene_stats stats = {};
+    double upload_ms = 0.0;
+    std::vector<uint32_t> host_nodes;
+    std::vector<uint32_t> tri_order;
+    FilmState film;
+    WavefrontState wf;
+};
+
+int HipFail(hipError_t e, const char* what);
+int DevAlloc(Ctx* c, void** p, size_t bytes);
+int Upload(Ctx* c, void** p, const void* src, size_t bytes);
+void DevFree(void* p);
+int Grow(Ctx* c, DevBuf* b, size_t bytes);
+int Fill32(Ctx* c, void* p, uint32_t v, int64_t n);
+int CheckLaunch(Ctx* c, const char* what);
+void TimerBegin(Ctx* c, int cat);
+void TimerEnd(Ctx* c);
+void SceneFree(Scene* sc);
+void FilmFree(FilmState* f);
+FilmDev FilmView(Scene* sc);
+
+struct PathParams;
+int WavefrontPass(Ctx* c, Scene* sc, const PathParams& P, const gy_render_params& p, int s0, int n_s);
+void WavefrontFree(Scene* sc);
+// GPU binned-SAH BVH build (bvh_gpu.hip).
+int BuildBVHDevice(Ctx* c, const float* pos, int32_t n_tri, int max_leaf, BVHBuild* out);
+
+}  // namespace gy

@@ -1,0 +1,309 @@
+// Procedural benchmark scenes.
+//
+//  GY_SCENE_CORNELL: the Cornell box in [0,1]^3 built the way the reference's
+//    scene library builds its boxes (Gyunity_exe/CornellBox.h:766-926): two
+//    triangles per wall with inward face normals given to the Triangle
+//    constructor, a quad light made of two one-sided triangle AreaLights with a
+//    black Lambert material, Lambert walls.  Two axis-aligned white boxes of 12
+//    triangles each stand on the floor.  Light 0.25 x 0.25 at y = 0.99 facing
+//    down, Le = (17, 12, 4).  Pinhole camera, fovy 40 deg, 256^2, 16 spp,
+//    depth 5, Russian roulette for bounces i > 3.
+//  GY_SCENE_ICOSPHERE_GGX: the same room without the boxes, plus an icosphere
+//    subdivided `param` times (8 -> 1,310,720 triangles), radius 0.3, displaced
+//    radially by 4-octave value noise of amplitude 0.05 seeded by `seed`, with
+//    flat (face) normals.  GGX conductor, alpha 0.2 given directly, Schlick F0
+//    (0.95, 0.64, 0.54).  1024^2, 64 spp, depth 8, roulette for i > 5.
+#include <cmath>
+#include <cstring>
+#include <unordered_map>
+
+#include "gy_internal.h"
+
+namespace gy {
+
+struct Generated {
+    std::vector<float> pos, nrm, fn;
+    std::vector<uint32_t> mat, mats;
+    std::vector<int32_t> light, ltri;
+    std::vector<float> le;
+    gy_camera_desc cam;
+    gy_render_params params;
+};
+
+namespace {
+
+struct V {
+    float x, y, z;
+};
+inline V sub(V a, V b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V cross(V a, V b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+inline V norm(V a) {
+    float l = std::sqrt((a.x * a.x + a.y * a.y) + (a.z * a.z + 0.f));
+    return {a.x / l, a.y / l, a.z / l};
+}
+
+uint32_t AddMaterial(Generated& g, uint32_t type, uint32_t flags, const float R[3]) {
+    uint32_t w[GY_MAT_WORDS];
+    std::memset(w, 0, sizeof(w));
+    w[0] = type;
+    w[1] = flags;
+    std::memcpy(&w[2], R, 12);
+    const float one = 1.f, eta_out = 1.f, eta_in = 1.5f;
+    for (int c = 0; c < 3; ++c) std::memcpy(&w[5 + c], &one, 4);
+    std::memcpy(&w[19], &eta_out, 4);
+    std::memcpy(&w[20], &eta_in, 4);
+    int32_t none = -1;
+    std::memcpy(&w[21], &none, 4);
+    std::memcpy(&w[22], &none, 4);
+    g.mats.insert(g.mats.end(), w, w + GY_MAT_WORDS);
+    return (uint32_t)(g.mats.size() / GY_MAT_WORDS - 1);
+}
+
+void SetWord(Generated& g, uint32_t m, int word, float v) { std::memcpy(&g.mats[(size_t)m * GY_MAT_WORDS + word], &v, 4); }
+
+void AddTri(Generated& g, V p0, V p1, V p2, V n, uint32_t mat, int32_t light) {
+    const V p[3] = {p0, p1, p2};
+    for (int k = 0; k < 3; ++k) {
+        g.pos.push_back(p[k].x);
+        g.pos.push_back(p[k].y);
+        g.pos.push_back(p[k].z);
+        g.nrm.push_back(n.x);
+        g.nrm.push_back(n.y);
+        g.nrm.push_back(n.z);
+    }
+    g.fn.push_back(n.x);
+    g.fn.push_back(n.y);
+    g.fn.push_back(n.z);
+    g.mat.push_back(mat);
+    g.light.push_back(light);
+}
+
+// Quad a-b-c-d as triangles (a, b, c) and (a, c, d), the split the reference's
+// boxes use for their light (Gyunity_exe/CornellBox.h:908-913).
+void AddQuad(Generated& g, V a, V b, V c, V d, V n, uint32_t mat) {
+    AddTri(g, a, b, c, n, mat, -1);
+    AddTri(g, a, c, d, n, mat, -1);
+}
+
+void AddBox(Generated& g, V lo, V hi, uint32_t mat) {
+    const float x0 = lo.x, y0 = lo.y, z0 = lo.z, x1 = hi.x, y1 = hi.y, z1 = hi.z;
+    AddQuad(g, {x0, y1, z0}, {x0, y1, z1}, {x1, y1, z1}, {x1, y1, z0}, {0, 1, 0}, mat);    // top
+    AddQuad(g, {x0, y0, z0}, {x1, y0, z0}, {x1, y0, z1}, {x0, y0, z1}, {0, -1, 0}, mat);   // bottom
+    AddQuad(g, {x0, y0, z0}, {x0, y0, z1}, {x0, y1, z1}, {x0, y1, z0}, {-1, 0, 0}, mat);   // -x
+    AddQuad(g, {x1, y0, z0}, {x1, y1, z0}, {x1, y1, z1}, {x1, y0, z1}, {1, 0, 0}, mat);    // +x
+    AddQuad(g, {x0, y0, z0}, {x0, y1, z0}, {x1, y1, z0}, {x1, y0, z0}, {0, 0, -1}, mat);   // -z
+    AddQuad(g, {x0, y0, z1}, {x1, y0, z1}, {x1, y1, z1}, {x0, y1, z1}, {0, 0, 1}, mat);    // +z
+}
+
+void AddRoomAndLight(Generated& g, bool boxes) {
+    const float white[3] = {0.73f, 0.73f, 0.73f}, red[3] = {0.63f, 0.065f, 0.05f}, green[3] = {0.14f, 0.45f, 0.091f},
+                black[3] = {0.f, 0.f, 0.f};
+    uint32_t mw = AddMaterial(g, GY_MAT_LAMBERT, 0, white);
+    uint32_t mr = AddMaterial(g, GY_MAT_LAMBERT, 0, red);
+    uint32_t mg = AddMaterial(g, GY_MAT_LAMBERT, 0, green);
+    uint32_t mlight = AddMaterial(g, GY_MAT_LAMBERT, 0, black);
+    // walls: inward normals
+    AddQuad(g, {0, 0, 0}, {1, 0, 0}, {1, 0, 1}, {0, 0, 1}, {0, 1, 0}, mw);    // floor
+    AddQuad(g, {0, 1, 0}, {0, 1, 1}, {1, 1, 1}, {1, 1, 0}, {0, -1, 0}, mw);   // ceiling
+    AddQuad(g, {0, 0, 0}, {0, 1, 0}, {1, 1, 0}, {1, 0, 0}, {0, 0, 1}, mw);    // back
+    AddQuad(g, {0, 0, 0}, {0, 0, 1}, {0, 1, 1}, {0, 1, 0}, {1, 0, 0}, mr);    // left (x = 0)
+    AddQuad(g, {1, 0, 0}, {1, 1, 0}, {1, 1, 1}, {1, 0, 1}, {-1, 0, 0}, mg);   // right (x = 1)
+    if (boxes) {
+        AddBox(g, {0.53f, 0.f, 0.45f}, {0.83f, 0.3f, 0.75f}, mw);
+        AddBox(g, {0.17f, 0.f, 0.15f}, {0.47f, 0.6f, 0.45f}, mw);
+    }
+    // light: two one-sided triangles facing down
+    const float y = 0.99f, a = 0.375f, b = 0.625f;
+    const V l0{b, y, b}, l1{b, y, a}, l2{a, y, a}, l3{a, y, b};
+    const V ln{0, -1, 0};
+    int32_t first = (int32_t)g.ltri.size();
+    AddTri(g, l0, l1, l2, ln, mlight, first);
+    g.ltri.push_back((int32_t)(g.fn.size() / 3 - 1));
+    AddTri(g, l0, l2, l3, ln, mlight, first + 1);
+    g.ltri.push_back((int32_t)(g.fn.size() / 3 - 1));
+    for (int i = 0; i < 2; ++i) {
+        g.le.push_back(17.f);
+        g.le.push_back(12.f);
+        g.le.push_back(4.f);
+    }
+}
+
+void SetCamera(Generated& g, int res) {
+    gy_camera_desc& c = g.cam;
+    c.pos[0] = 0.5f; c.pos[1] = 0.5f; c.pos[2] = 2.35f;
+    c.look_at[0] = 0.5f; c.look_at[1] = 0.5f; c.look_at[2] = 0.f;
+    c.up[0] = 0.f; c.up[1] = 1.f; c.up[2] = 0.f;
+    c.fovy = 40.f;
+    c.film_dist = 1.f;
+    c.width = res;
+    c.height = res;
+}
+
+// 3-D value noise: hashed lattice values in [-1, 1], smoothstep-weighted
+// trilinear interpolation; fbm of 4 octaves (frequency x2, amplitude x0.5).
+uint32_t Hash(int32_t x, int32_t y, int32_t z, uint32_t seed) {
+    uint32_t h = seed * 0x9E3779B1u;
+    h ^= (uint32_t)x * 0x85EBCA77u;
+    h = (h << 13) | (h >> 19);
+    h ^= (uint32_t)y * 0xC2B2AE3Du;
+    h = (h << 17) | (h >> 15);
+    h ^= (uint32_t)z * 0x27D4EB2Fu;
+    h ^= h >> 16;
+    h *= 0x7FEB352Du;
+    h ^= h >> 15;
+    h *= 0x846CA68Bu;
+    h ^= h >> 16;
+    return h;
+}
+double Lattice(int32_t x, int32_t y, int32_t z, uint32_t seed) {
+    return (double)Hash(x, y, z, seed) * (2.0 / 4294967295.0) - 1.0;
+}
+double ValueNoise(double x, double y, double z, uint32_t seed) {
+    double fx = std::floor(x), fy = std::floor(y), fz = std::floor(z);
+    int32_t ix = (int32_t)fx, iy = (int32_t)fy, iz = (int32_t)fz;
+    double tx = x - fx, ty = y - fy, tz = z - fz;
+    auto s = [](double t) { return t * t * (3.0 - 2.0 * t); };
+    double sx = s(tx), sy = s(ty), sz = s(tz);
+    double acc = 0.0;
+    for (int dz = 0; dz < 2; ++dz)
+        for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx) {
+                double w = (dx ? sx : 1.0 - sx) * (dy ? sy : 1.0 - sy) * (dz ? sz : 1.0 - sz);
+                acc += w * Lattice(ix + dx, iy + dy, iz + dz, seed);
+            }
+    return acc;
+}
+double Fbm(double x, double y, double z, uint32_t seed) {
+    double sum = 0.0, amp = 1.0, freq = 4.0, norm_ = 0.0;
+    for (int o = 0; o < 4; ++o) {
+        sum += amp * ValueNoise(x * freq, y * freq, z * freq, seed + 7919u * (uint32_t)o);
+        norm_ += amp;
+        amp *= 0.5;
+        freq *= 2.0;
+    }
+    return sum / norm_;
+}
+
+void AddIcosphere(Generated& g, int levels, uint32_t seed, double cx, double cy, double cz, double radius, double amp,
+                  uint32_t mat) {
+    const double t = (1.0 + std::sqrt(5.0)) / 2.0;
+    std::vector<double> vx = {-1, 1, -1, 1, 0, 0, 0, 0, t, t, -t, -t};
+    std::vector<double> vy = {t, t, -t, -t, -1, 1, -1, 1, 0, 0, 0, 0};
+    std::vector<double> vz = {0, 0, 0, 0, t, t, -t, -t, -1, 1, -1, 1};
+    for (size_t i = 0; i < vx.size(); ++i) {
+        double l = std::sqrt(vx[i] * vx[i] + vy[i] * vy[i] + vz[i] * vz[i]);
+        vx[i] /= l; vy[i] /= l; vz[i] /= l;
+    }
+    std::vector<uint32_t> faces = {0, 11, 5, 0, 5, 1, 0, 1, 7, 0, 7, 10, 0, 10, 11, 1, 5, 9, 5, 11, 4, 11, 10, 2, 10, 7, 6, 7, 1, 8,
+                                   3, 9, 4, 3, 4, 2, 3, 2, 6, 3, 6, 8, 3, 8, 9, 4, 9, 5, 2, 4, 11, 6, 2, 10, 8, 6, 7, 9, 8, 1};
+    for (int lv = 0; lv < levels; ++lv) {
+        std::unordered_map<uint64_t, uint32_t> mid;
+        mid.reserve(faces.size());
+        auto midpoint = [&](uint32_t a, uint32_t b) {
+            uint64_t key = a < b ? ((uint64_t)a << 32 | b) : ((uint64_t)b << 32 | a);
+            auto it = mid.find(key);
+            if (it != mid.end()) return it->second;
+            double x = vx[a] + vx[b], y = vy[a] + vy[b], z = vz[a] + vz[b];
+            double l = std::sqrt(x * x + y * y + z * z);
+            vx.push_back(x / l); vy.push_back(y / l); vz.push_back(z / l);
+            uint32_t idx = (uint32_t)vx.size() - 1;
+            mid.emplace(key, idx);
+            return idx;
+        };
+        std::vector<uint32_t> nf;
+        nf.reserve(faces.size() * 4);
+        for (size_t f = 0; f < faces.size(); f += 3) {
+            uint32_t a = faces[f], b = faces[f + 1], c = faces[f + 2];
+            uint32_t ab = midpoint(a, b), bc = midpoint(b, c), ca = midpoint(c, a);
+            uint32_t tri[12] = {a, ab, ca, b, bc, ab, c, ca, bc, ab, bc, ca};
+            nf.insert(nf.end(), tri, tri + 12);
+        }
+        faces.swap(nf);
+    }
+    // displaced FP32 vertex positions
+    std::vector<V> P(vx.size());
+    for (size_t i = 0; i < vx.size(); ++i) {
+        double r = radius + amp * Fbm(vx[i], vy[i], vz[i], seed);
+        P[i] = {(float)(cx + r * vx[i]), (float)(cy + r * vy[i]), (float)(cz + r * vz[i])};
+    }
+    for (size_t f = 0; f < faces.size(); f += 3) {
+        V p0 = P[faces[f]], p1 = P[faces[f + 1]], p2 = P[faces[f + 2]];
+        V n = norm(cross(sub(p1, p0), sub(p2, p0)));
+        // orient outward (away from the centre)
+        double ox = (p0.x + p1.x + p2.x) / 3.0 - cx, oy = (p0.y + p1.y + p2.y) / 3.0 - cy, oz = (p0.z + p1.z + p2.z) / 3.0 - cz;
+        if (n.x * ox + n.y * oy + n.z * oz < 0) {
+            std::swap(p1, p2);
+            n = norm(cross(sub(p1, p0), sub(p2, p0)));
+        }
+        AddTri(g, p0, p1, p2, n, mat, -1);
+    }
+}
+
+}  // namespace
+}  // namespace gy
+
+extern "C" int gy_generate_scene(int kind, uint32_t seed, int32_t param, gy_generated** out) {
+    using namespace gy;
+    if (!out) return Fail(GY_ERR_INVALID, "gy_generate_scene: null output");
+    Generated* g = new Generated();
+    std::memset(&g->params, 0, sizeof(g->params));
+    if (kind == GY_SCENE_CORNELL) {
+        AddRoomAndLight(*g, true);
+        SetCamera(*g, param > 0 ? param : 256);
+        g->params.spp = 16;
+        g->params.max_depth = 5;
+        g->params.rr_depth = 3;
+    } else if (kind == GY_SCENE_ICOSPHERE_GGX) {
+        int levels = param > 0 ? param : 8;
+        if (levels > 10) {
+            delete g;
+            return Fail(GY_ERR_INVALID, "gy_generate_scene: icosphere levels must be <= 10");
+        }
+        AddRoomAndLight(*g, false);
+        const float f0[3] = {0.95f, 0.64f, 0.54f}, one[3] = {1.f, 1.f, 1.f};
+        uint32_t m = AddMaterial(*g, GY_MAT_CONDUCTOR, GY_MATF_ALPHA_DIRECT | GY_MATF_SCHLICK_F0, one);
+        SetWord(*g, m, 8, 0.2f);
+        SetWord(*g, m, 9, 0.2f);
+        for (int c = 0; c < 3; ++c) SetWord(*g, m, 16 + c, f0[c]);
+        AddIcosphere(*g, levels, seed, 0.5, 0.4, 0.5, 0.3, 0.05, m);
+        SetCamera(*g, 1024);
+        g->params.spp = 64;
+        g->params.max_depth = 8;
+        g->params.rr_depth = 5;
+    } else {
+        delete g;
+        return Fail(GY_ERR_INVALID, "gy_generate_scene: unknown scene kind");
+    }
+    g->params.mode = GY_RENDER_WAVEFRONT;
+    *out = reinterpret_cast<gy_generated*>(g);
+    return GY_OK;
+}
+
+extern "C" int gy_generated_desc(gy_generated* gg, gy_scene_desc* d, gy_camera_desc* cam, gy_render_params* params) {
+    using namespace gy;
+    if (!gg) return Fail(GY_ERR_INVALID, "gy_generated_desc: null handle");
+    Generated* g = reinterpret_cast<Generated*>(gg);
+    if (d) {
+        std::memset(d, 0, sizeof(*d));
+        d->n_tri = (int32_t)g->mat.size();
+        d->pos = g->pos.data();
+        d->nrm = g->nrm.data();
+        d->face_n = g->fn.data();
+        d->uv = nullptr;
+        d->mat = g->mat.data();
+        d->light = g->light.data();
+        d->n_mat = (int32_t)(g->mats.size() / GY_MAT_WORDS);
+        d->mats = g->mats.data();
+        d->n_light = (int32_t)g->ltri.size();
+        d->light_tri = g->ltri.data();
+        d->light_le = g->le.data();
+    }
+    if (cam) *cam = g->cam;
+    if (params) *params = g->params;
+    return GY_OK;
+}
+
+extern "C" int gy_generated_free(gy_generated* g) {
+    delete reinterpret_cast<gy::Generated*>(g);
+    return GY_OK;
+}

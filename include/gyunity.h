@@ -1,0 +1,193 @@
+/* gyunity.h - C ABI of the MI355X path-tracing integrator (libgyunity.so).
+ *
+ * This is the only seam between host code and the HIP kernels.  Plain C types,
+ * host pointers unless a name ends in _device, int status codes (GY_OK = 0),
+ * and a thread-local message from gy_last_error().  No call throws or exits.
+ *
+ * The reference has no C ABI; its integrator is the C++ virtual
+ *     class Integrator { virtual void Render(const Scene&, const Camera&) = 0; }
+ * (Gyunity/src/integrator/Integrator.h:17-27), implemented on the CPU by
+ *     PathTracing(spp, maxDepth, sampler, samplerEnum)  (integrator/PT.h:12)
+ *     TiledIntegrator::Render                           (integrator/TiledIntegrator.cpp:7-60)
+ * Each entry point below names the reference interface it stands in for.
+ * The C++ class gy::HipPathTracing (gyunity_amd/include/gyunity/HipPathTracing.h)
+ * implements that Integrator interface on top of these calls.
+ */
+#ifndef GYUNITY_H
+#define GYUNITY_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    GY_OK = 0,
+    GY_ERR_INVALID = 1,   /* bad argument or descriptor */
+    GY_ERR_DEVICE = 2,    /* HIP runtime error (no device, launch failure) */
+    GY_ERR_ALLOC = 3,     /* device or host allocation failed */
+    GY_ERR_UNSUPPORTED = 4,
+    GY_ERR_INJECTED = 5   /* failure forced by gy_debug_inject_fault */
+};
+
+typedef struct gy_context gy_context;
+typedef struct gy_scene gy_scene;
+
+/* Material record: 32 four-byte words.
+ *   word 0  type: GY_MAT_LAMBERT (DiffuseMaterial, material/Diffuse.h:27-31),
+ *                 GY_MAT_CONDUCTOR (RoughnessMaterial, material/Roughness.h:36-46),
+ *                 GY_MAT_GLASS (GlassMaterial, material/Glass.h:30-34),
+ *                 GY_MAT_ROUGH_DIELECTRIC (RoughDielectricMaterial, material/RoughDielectric.h:21-41)
+ *   word 1  flags: GY_MATF_ALPHA_DIRECT (alpha given; else RoughnessToAlpha(roughness)),
+ *                  GY_MATF_SCHLICK_F0 (conductor Fresnel from F0; else FrConductor(eta, k))
+ *   2-4 R, 5-7 T, 8-9 alpha (or roughness) x/y, 10-12 eta, 13-15 k, 16-18 F0,
+ *   19 dielectric eta outside, 20 eta inside, 21 albedo texture id, 22 alpha texture id. */
+#define GY_MAT_WORDS 32
+enum { GY_MAT_LAMBERT = 0, GY_MAT_CONDUCTOR = 1, GY_MAT_GLASS = 2, GY_MAT_ROUGH_DIELECTRIC = 3 };
+enum { GY_MATF_ALPHA_DIRECT = 1, GY_MATF_SCHLICK_F0 = 2 };
+
+/* Flattened scene: what Scene::AddPrimitive / AddLight (visual/Scene.h:39-65)
+ * accumulate.  Primitive ids are triangle indices (Scene.h:43). */
+typedef struct {
+    int32_t n_tri;
+    const float* pos;        /* n_tri*9  p0,p1,p2 */
+    const float* nrm;        /* n_tri*9  vertex normals n0,n1,n2 */
+    const float* face_n;     /* n_tri*3  face normal (Triangle ctor argument) */
+    const float* uv;         /* n_tri*6 or NULL (all zero, as shape/Triangle.h:93-105 gives) */
+    const uint32_t* mat;     /* n_tri material ids */
+    const int32_t* light;    /* n_tri light ids, -1 if not emissive (NULL = none) */
+    int32_t n_mat;
+    const uint32_t* mats;    /* n_mat*GY_MAT_WORDS */
+    int32_t n_light;
+    const int32_t* light_tri;  /* triangle of each AreaLight (light/AreaLight.h:12-85) */
+    const float* light_le;     /* n_light*3 emitted radiance */
+    int32_t n_tex;
+    const int32_t* tex_info;   /* n_tex*4: width, height, channels, float offset into tex_data */
+    const float* tex_data;
+} gy_scene_desc;
+
+/* PinHoleCamera(film, position, lookAt, up, fovy, filmDistance)
+ * (camera/PinholeCamera.h:16-20, camera/Camera.cpp:68-72) on a width x height film. */
+typedef struct {
+    float pos[3], look_at[3], up[3];
+    float fovy;
+    float film_dist;
+    int32_t width, height;
+} gy_camera_desc;
+
+enum { GY_RENDER_MEGAKERNEL = 0, GY_RENDER_WAVEFRONT = 1 };
+
+/* PathTracing(spp, maxDepth, SobolSampler, SobolEnum) (integrator/PT.h:12) plus
+ * the Russian-roulette threshold (reference: bounce i > 5, PT.cpp:53). */
+typedef struct {
+    int32_t spp;
+    int32_t max_depth;
+    int32_t rr_depth;
+    int32_t mode;            /* GY_RENDER_MEGAKERNEL or GY_RENDER_WAVEFRONT */
+    int32_t samples_per_pass;/* 0 = automatic */
+    int32_t block_size;      /* 0 = automatic (autotuned); else 64..1024 */
+    int32_t reserved[2];
+} gy_render_params;
+
+/* BVH build knobs (BVHAccel(prims, maxPrims, "sah"), accelerator/BVHAccel.h:50-51). */
+typedef struct {
+    int32_t max_leaf_size;   /* 0 = 4 */
+    int32_t builder;         /* 0 = GPU binned SAH, 1 = host binned SAH */
+    int32_t reserved[2];
+} gy_bvh_params;
+
+typedef struct {
+    int64_t n_nodes;
+    int64_t n_leaves;
+    int32_t max_depth;
+    int32_t max_leaf_size;
+    double build_ms;
+    double sah_cost;
+} gy_scene_stats;
+
+/* One ray: origin, tmin, direction, tmax (math/Ray.h:9-29 without the
+ * constructor's RayEps shift: callers pass the final origin). 32 bytes. */
+typedef struct { float org[3]; float tmin; float dir[3]; float tmax; } gy_ray;
+/* Closest hit: t, primitive id (-1 = miss), barycentrics b1, b2
+ * (shape/Triangle.cpp:71-103, visual/Primitive.cpp:5-18). 16 bytes. */
+typedef struct { float t; int32_t prim; float b1; float b2; } gy_hit;
+
+typedef struct {
+    int64_t paths;
+    int64_t rays;            /* every traced ray: camera/continuation, shadow, MIS */
+    int64_t shadow_rays;
+    int64_t mis_rays;
+    double total_ms;         /* wall time of the render call on the device stream */
+    double kernel_ms[8];     /* per-kernel time, see gy_kernel_name */
+    int64_t kernel_launches[8];
+    int64_t node_visits;     /* filled by gy_count_traffic */
+    int64_t tri_visits;
+    int64_t shade_points;
+} gy_render_stats;
+
+/* ---- context ------------------------------------------------------------ */
+int gy_open(int device, gy_context** out);
+int gy_close(gy_context* ctx);
+const char* gy_last_error(void);
+const char* gy_version(void);
+int gy_device_name(gy_context* ctx, char* buf, int len);
+
+/* ---- scene (Scene::Initialize + BVHAccel::BVHAccel, visual/Scene.h:26-33,
+ *      accelerator/BVHAccel.cpp:154-200) ------------------------------------ */
+int gy_scene_create(gy_context* ctx, const gy_scene_desc* desc, const gy_bvh_params* bvh, gy_scene** out);
+int gy_scene_destroy(gy_scene* scene);
+int gy_scene_stats_get(gy_scene* scene, gy_scene_stats* out);
+/* Copy the device BVH back: nodes (n_nodes*16 words) and leaf triangle order. */
+int gy_scene_bvh_download(gy_scene* scene, uint32_t* nodes, int64_t max_nodes, int32_t* tri_order, int64_t max_tris);
+
+/* ---- ray queries (Scene::Intersect, visual/Scene.h:83-89 ->
+ *      BVHAccel::Intersect closest / any, accelerator/BVHAccel.cpp:401-487) -- */
+int gy_trace_closest(gy_scene* scene, const gy_ray* rays, int64_t n, gy_hit* hits);
+int gy_trace_any(gy_scene* scene, const gy_ray* rays, int64_t n, int32_t* occluded);
+int gy_trace_closest_device(gy_scene* scene, const gy_ray* rays_device, int64_t n, gy_hit* hits_device);
+int gy_trace_any_device(gy_scene* scene, const gy_ray* rays_device, int64_t n, int32_t* occluded_device);
+/* Device time (ms) of the last trace kernel, measured with HIP events on the
+ * stream it ran on. */
+double gy_last_kernel_ms(gy_context* ctx);
+/* Count BVH nodes and triangles the closest-hit traversal visits. */
+int gy_count_visits(gy_scene* scene, const gy_ray* rays, int64_t n, int64_t* node_visits, int64_t* tri_visits);
+
+/* ---- rendering (Integrator::Render + Film::AddSample,
+ *      integrator/TiledIntegrator.cpp:7-60, visual/Film.h:61-87) ---------- */
+/* color: width*height*3 sums, weight: width*height sample counts (either may
+ * be NULL to leave the film on the device; gy_film_download fetches it). */
+int gy_render(gy_scene* scene, const gy_camera_desc* cam, const gy_render_params* p, float* color, float* weight);
+int gy_film_download(gy_scene* scene, float* color, float* weight);
+int gy_render_stats_get(gy_context* ctx, gy_render_stats* out);
+const char* gy_kernel_name(int index);
+/* Radiance of individual camera samples (x, y, s): PathTracing::Li for the path
+ * a tile worker traces (TiledIntegrator.cpp:40-47).  L: 3n floats, raster: 2n
+ * floats (x+u, y+v), dims: n sample dimensions consumed. */
+int gy_li_samples(gy_scene* scene, const gy_camera_desc* cam, const gy_render_params* p, int32_t n,
+                  const int32_t* x, const int32_t* y, const int32_t* s, float* L, float* raster, int32_t* dims);
+/* Algorithmic HBM traffic of a render: node/triangle visits of every traced ray
+ * and shading points, counted by an instrumented copy of the kernels. */
+int gy_count_traffic(gy_scene* scene, const gy_camera_desc* cam, const gy_render_params* p, gy_render_stats* out);
+
+/* ---- samplers (SobolSampler::Sample, sampler/Sobol.h:66-77;
+ *      SobolEnum::GetIndex, sampler/SobolEnum.h:24-28) ------------------- */
+float gy_sobol_sample(uint64_t index, uint32_t dim);
+uint64_t gy_sobol_enum_index(uint32_t log2_res, uint32_t frame, uint32_t px, uint32_t py);
+int gy_camera_matrix(const gy_camera_desc* cam, float* raster_to_world16);
+
+/* ---- procedural scenes (the benchmark configurations) ---------------------- */
+typedef struct gy_generated gy_generated;
+enum { GY_SCENE_CORNELL = 0, GY_SCENE_ICOSPHERE_GGX = 1 };
+int gy_generate_scene(int kind, uint32_t seed, int32_t param, gy_generated** out);
+int gy_generated_desc(gy_generated* g, gy_scene_desc* desc, gy_camera_desc* cam, gy_render_params* params);
+int gy_generated_free(gy_generated* g);
+
+/* ---- failure injection (host-forced, for tests) --------------------------- */
+enum { GY_FAULT_NONE = 0, GY_FAULT_ALLOC = 1, GY_FAULT_LAUNCH = 2, GY_FAULT_SCENE = 3 };
+int gy_debug_inject_fault(gy_context* ctx, int kind);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

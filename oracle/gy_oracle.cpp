@@ -1,0 +1,1754 @@
+// CPU oracle for the Gyunity path-tracing hot path.  TEST INFRASTRUCTURE ONLY:
+// only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it,
+// as the checker or as the timed CPU baseline.  See gy_oracle.h.
+//
+// Everything here is a restatement of the reference CPU code, templated on the
+// scalar type R: R = float reproduces the reference's FP32 rounding (SSE dot
+// product order, no FMA, IEEE divide and sqrt; the Makefile builds with
+// -ffp-contract=off), R = double runs the same algorithm in double for the
+// converged reference image.  File:line citations are into the reference tree.
+//
+// Parity status of each part (what pins it) is recorded beside its tests.
+#include "gy_oracle.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <limits>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace orc {
+
+// ---------------------------------------------------------------- constants
+// common/Core.h:185-199
+template <class R> struct K {
+    static constexpr R PI = (R)3.14159265358979;
+    static constexpr R INV_PI = (R)0.31830988618379067154;
+    static constexpr R PI_Over2 = (R)1.57079632679489661923;
+    static constexpr R PI_Over4 = (R)0.78539816339744830961;
+    static constexpr R RayEps = (R)1e-4;
+    static constexpr R ShadowRayEps = (R)1e-4;
+    static constexpr R MachineEps = std::numeric_limits<R>::epsilon() * (R)0.5;
+    static constexpr R Inf = std::numeric_limits<R>::infinity();
+};
+
+// ---------------------------------------------------------------- vectors
+// Vec3 follows the SSE Vector<3,float> of math/Linagl.h: lane-wise + - * /,
+// Dot = DPPS mask 0x71 = (x*x' + y*y') + (z*z' + 0) (Linagl.h:496-500),
+// Norm = v / sqrt(Dot) with a true divide (Linagl.h:527-531), unary minus =
+// 0 - v (Linagl.h:294-297), Vec / scalar returns 0 when the scalar is 0
+// (Linagl.h:366-369).
+template <class R> struct V3 {
+    R x, y, z;
+    V3() : x(0), y(0), z(0) {}
+    V3(R a, R b, R c) : x(a), y(b), z(c) {}
+    explicit V3(R a) : x(a), y(a), z(a) {}
+    R operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+    R& operator[](int i) { return i == 0 ? x : (i == 1 ? y : z); }
+};
+template <class R> inline V3<R> operator+(const V3<R>& a, const V3<R>& b) { return V3<R>(a.x + b.x, a.y + b.y, a.z + b.z); }
+template <class R> inline V3<R> operator-(const V3<R>& a, const V3<R>& b) { return V3<R>(a.x - b.x, a.y - b.y, a.z - b.z); }
+template <class R> inline V3<R> operator*(const V3<R>& a, const V3<R>& b) { return V3<R>(a.x * b.x, a.y * b.y, a.z * b.z); }
+template <class R> inline V3<R> operator/(const V3<R>& a, const V3<R>& b) { return V3<R>(a.x / b.x, a.y / b.y, a.z / b.z); }
+template <class R> inline V3<R> operator*(const V3<R>& a, R s) { return V3<R>(a.x * s, a.y * s, a.z * s); }
+template <class R> inline V3<R> operator*(R s, const V3<R>& a) { return a * s; }
+template <class R> inline V3<R> operator/(const V3<R>& a, R s) {
+    if (s == 0) return V3<R>(0, 0, 0);
+    return V3<R>(a.x / s, a.y / s, a.z / s);
+}
+template <class R> inline V3<R> operator-(const V3<R>& a) { return V3<R>(R(0) - a.x, R(0) - a.y, R(0) - a.z); }
+template <class R> inline bool operator==(const V3<R>& a, const V3<R>& b) { return a.x == b.x && a.y == b.y && a.z == b.z; }
+template <class R> inline bool operator!=(const V3<R>& a, const V3<R>& b) { return !(a == b); }
+template <class R> inline bool is_zero(const V3<R>& a) { return a.x == 0 && a.y == 0 && a.z == 0; }
+template <class R> inline R dot(const V3<R>& a, const V3<R>& b) { return (a.x * b.x + a.y * b.y) + (a.z * b.z + R(0)); }
+template <class R> inline V3<R> cross(const V3<R>& a, const V3<R>& b) {
+    return V3<R>(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+template <class R> inline R length(const V3<R>& a) { return std::sqrt(dot(a, a)); }
+template <class R> inline V3<R> norm(const V3<R>& a) {
+    R l = length(a);
+    return V3<R>(a.x / l, a.y / l, a.z / l);
+}
+template <class R> inline V3<R> vabs(const V3<R>& a) { return V3<R>(std::abs(a.x), std::abs(a.y), std::abs(a.z)); }
+template <class R> inline V3<R> vsqrt(const V3<R>& a) { return V3<R>(std::sqrt(a.x), std::sqrt(a.y), std::sqrt(a.z)); }
+// Linagl.h:569-572
+template <class R> inline R lumY(const V3<R>& a) {
+    const R w0 = (R)0.212671, w1 = (R)0.715160, w2 = (R)0.072169f;
+    return w0 * a.x + w1 * a.y + w2 * a.z;
+}
+template <class R> inline R rmin(R a, R b) { return (b < a) ? b : a; }  // std::min
+template <class R> inline R rmax(R a, R b) { return (a < b) ? b : a; }  // std::max
+template <class R> inline R sgn(R v) { return (v > 0) ? R(1) : R(-1); }  // math/MathUtils.h:130-132
+template <class R> inline R clampr(R v, R lo, R hi) {                      // math/MathUtils.h:51-56
+    if (v < lo) return lo;
+    if (v > hi) return hi;
+    return v;
+}
+
+// ---------------------------------------------------------------- 4x4 matrices
+// Matrix<4,float> stores columns; (i,j) = d[j][i] (Linagl.h:834-840).  The SSE
+// mat*vec is ((c3*w + c2*z) + c1*y) + c0*x (Linagl.h:937-948), mat*mat applies
+// it per column (Linagl.h:950-952), the inverse is GLM's (Linagl.h:1178-1293).
+template <class R> struct V4 { R v[4]; };
+template <class R> struct M4 {
+    V4<R> c[4];
+    static M4 rows(R m00, R m01, R m02, R m03, R m10, R m11, R m12, R m13, R m20, R m21, R m22, R m23, R m30,
+                   R m31, R m32, R m33) {
+        M4 m;
+        R a[4][4] = {{m00, m01, m02, m03}, {m10, m11, m12, m13}, {m20, m21, m22, m23}, {m30, m31, m32, m33}};
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) m.c[j].v[i] = a[i][j];
+        return m;
+    }
+};
+template <class R> inline V4<R> mul(const M4<R>& m, const V4<R>& a) {
+    V4<R> r;
+    for (int i = 0; i < 4; ++i) r.v[i] = m.c[3].v[i] * a.v[3];
+    for (int i = 0; i < 4; ++i) r.v[i] = m.c[2].v[i] * a.v[2] + r.v[i];
+    for (int i = 0; i < 4; ++i) r.v[i] = m.c[1].v[i] * a.v[1] + r.v[i];
+    for (int i = 0; i < 4; ++i) r.v[i] = m.c[0].v[i] * a.v[0] + r.v[i];
+    return r;
+}
+template <class R> inline M4<R> mul(const M4<R>& a, const M4<R>& b) {
+    M4<R> r;
+    for (int j = 0; j < 4; ++j) r.c[j] = mul(a, b.c[j]);
+    return r;
+}
+template <class R> M4<R> glm_inverse(const M4<R>& M) {
+    auto m = [&](int col, int row) { return M.c[col].v[row]; };
+    R Coef00 = m(2, 2) * m(3, 3) - m(3, 2) * m(2, 3);
+    R Coef02 = m(1, 2) * m(3, 3) - m(3, 2) * m(1, 3);
+    R Coef03 = m(1, 2) * m(2, 3) - m(2, 2) * m(1, 3);
+    R Coef04 = m(2, 1) * m(3, 3) - m(3, 1) * m(2, 3);
+    R Coef06 = m(1, 1) * m(3, 3) - m(3, 1) * m(1, 3);
+    R Coef07 = m(1, 1) * m(2, 3) - m(2, 1) * m(1, 3);
+    R Coef08 = m(2, 1) * m(3, 2) - m(3, 1) * m(2, 2);
+    R Coef10 = m(1, 1) * m(3, 2) - m(3, 1) * m(1, 2);
+    R Coef11 = m(1, 1) * m(2, 2) - m(2, 1) * m(1, 2);
+    R Coef12 = m(2, 0) * m(3, 3) - m(3, 0) * m(2, 3);
+    R Coef14 = m(1, 0) * m(3, 3) - m(3, 0) * m(1, 3);
+    R Coef15 = m(1, 0) * m(2, 3) - m(2, 0) * m(1, 3);
+    R Coef16 = m(2, 0) * m(3, 2) - m(3, 0) * m(2, 2);
+    R Coef18 = m(1, 0) * m(3, 2) - m(3, 0) * m(1, 2);
+    R Coef19 = m(1, 0) * m(2, 2) - m(2, 0) * m(1, 2);
+    R Coef20 = m(2, 0) * m(3, 1) - m(3, 0) * m(2, 1);
+    R Coef22 = m(1, 0) * m(3, 1) - m(3, 0) * m(1, 1);
+    R Coef23 = m(1, 0) * m(2, 1) - m(2, 0) * m(1, 1);
+    R Fac0[4] = {Coef00, Coef00, Coef02, Coef03};
+    R Fac1[4] = {Coef04, Coef04, Coef06, Coef07};
+    R Fac2[4] = {Coef08, Coef08, Coef10, Coef11};
+    R Fac3[4] = {Coef12, Coef12, Coef14, Coef15};
+    R Fac4[4] = {Coef16, Coef16, Coef18, Coef19};
+    R Fac5[4] = {Coef20, Coef20, Coef22, Coef23};
+    R Vec0[4] = {m(1, 0), m(0, 0), m(0, 0), m(0, 0)};
+    R Vec1[4] = {m(1, 1), m(0, 1), m(0, 1), m(0, 1)};
+    R Vec2[4] = {m(1, 2), m(0, 2), m(0, 2), m(0, 2)};
+    R Vec3[4] = {m(1, 3), m(0, 3), m(0, 3), m(0, 3)};
+    R Inv[4][4];
+    for (int k = 0; k < 4; ++k) {
+        Inv[0][k] = (Vec1[k] * Fac0[k] - Vec2[k] * Fac1[k]) + Vec3[k] * Fac2[k];
+        Inv[1][k] = (Vec0[k] * Fac0[k] - Vec2[k] * Fac3[k]) + Vec3[k] * Fac4[k];
+        Inv[2][k] = (Vec0[k] * Fac1[k] - Vec1[k] * Fac3[k]) + Vec3[k] * Fac5[k];
+        Inv[3][k] = (Vec0[k] * Fac2[k] - Vec1[k] * Fac4[k]) + Vec2[k] * Fac5[k];
+    }
+    const R SignA[4] = {+1, -1, +1, -1}, SignB[4] = {-1, +1, -1, +1};
+    M4<R> I;
+    for (int k = 0; k < 4; ++k) {
+        I.c[0].v[k] = Inv[0][k] * SignA[k];
+        I.c[1].v[k] = Inv[1][k] * SignB[k];
+        I.c[2].v[k] = Inv[2][k] * SignA[k];
+        I.c[3].v[k] = Inv[3][k] * SignB[k];
+    }
+    R Row0[4] = {I.c[0].v[0], I.c[1].v[0], I.c[2].v[0], I.c[3].v[0]};
+    R Dot0[4];
+    for (int k = 0; k < 4; ++k) Dot0[k] = M.c[0].v[k] * Row0[k];
+    R Dot1 = (Dot0[0] + Dot0[1]) + (Dot0[2] + Dot0[3]);
+    R One = static_cast<R>(1) / Dot1;
+    for (int j = 0; j < 4; ++j)
+        for (int k = 0; k < 4; ++k) I.c[j].v[k] = I.c[j].v[k] * One;
+    return I;
+}
+template <class R> struct Xf {  // math/Transform.h: matrix + inverse
+    M4<R> m, inv;
+};
+template <class R> Xf<R> xf_compose(const Xf<R>& a, const Xf<R>& b) {  // Transform.h:253-255
+    return Xf<R>{mul(a.m, b.m), mul(b.inv, a.inv)};
+}
+template <class R> Xf<R> xf_scale(R sx, R sy, R sz) {  // Transform.cpp:18-28
+    return Xf<R>{M4<R>::rows(sx, 0, 0, 0, 0, sy, 0, 0, 0, 0, sz, 0, 0, 0, 0, 1),
+                 M4<R>::rows(1 / sx, 0, 0, 0, 0, 1 / sy, 0, 0, 0, 0, 1 / sz, 0, 0, 0, 0, 1)};
+}
+template <class R> Xf<R> xf_translate(R x, R y, R z) {  // Transform.cpp:6-16
+    return Xf<R>{M4<R>::rows(1, 0, 0, x, 0, 1, 0, y, 0, 0, 1, z, 0, 0, 0, 1),
+                 M4<R>::rows(1, 0, 0, -x, 0, 1, 0, -y, 0, 0, 1, -z, 0, 0, 0, 1)};
+}
+// Transform::operator()(point), math/Transform.h:106-115
+template <class R> V3<R> xf_point(const M4<R>& m, const V3<R>& p) {
+    V4<R> pp{{p.x, p.y, p.z, (R)1.0}};
+    pp = mul(m, pp);
+    if (pp.v[3] == 1 || pp.v[3] == 0) return V3<R>(pp.v[0], pp.v[1], pp.v[2]);
+    return V3<R>(pp.v[0], pp.v[1], pp.v[2]) / pp.v[3];
+}
+
+// ---------------------------------------------------------------- camera
+// ProjectiveCamera::Initialize (camera/Camera.cpp:6-38) with the lookAt
+// constructor (Camera.cpp:68-72), Transform::Perspective (Transform.cpp:132-150).
+template <class R> struct Camera {
+    V3<R> pos;
+    M4<R> raster_to_world;
+};
+template <class R> Camera<R> make_camera(const or_camera_desc& d) {
+    Camera<R> c;
+    V3<R> pos((R)d.pos[0], (R)d.pos[1], (R)d.pos[2]);
+    V3<R> look((R)d.look_at[0], (R)d.look_at[1], (R)d.look_at[2]);
+    V3<R> up((R)d.up[0], (R)d.up[1], (R)d.up[2]);
+    V3<R> cz = -norm(pos - look);
+    V3<R> cx = -norm(cross(up, cz));
+    V3<R> cy = -cross(cz, cx);
+    M4<R> c2w = M4<R>::rows(cx.x, cy.x, cz.x, pos.x, cx.y, cy.y, cz.y, pos.y, cx.z, cy.z, cz.z, pos.z, 0, 0, 0, (R)1.f);
+    R fovy = (R)d.fovy, dis = (R)d.film_dist;
+    R aspect = (R)d.width / (R)d.height;  // visual/Film.h:40
+    R radians = (K<R>::PI / 180) * fovy;  // math/MathUtils.h:63
+    R t = dis * std::tan(radians * 0.5);  // double tan, as written (Transform.cpp:134)
+    R r = t * aspect;
+    R f = (R)100000.f, n = dis;
+    M4<R> persp = M4<R>::rows(dis / r, 0, 0, 0, 0, dis / t, 0, 0, 0, 0, (f + n) / (f - n), -2 * f * n / (f - n), 0, 0, 1, 0);
+    M4<R> ndc_to_camera = glm_inverse(persp);
+    Xf<R> ndc_to_raster = xf_compose(xf_compose(xf_scale((R)d.width, (R)d.height, (R)1), xf_scale((R)0.5f, (R)-0.5f, (R)1)),
+                                     xf_translate((R)1, (R)-1, (R)0));
+    M4<R> raster_to_ndc = ndc_to_raster.inv;
+    M4<R> raster_to_camera = mul(ndc_to_camera, raster_to_ndc);
+    c.raster_to_world = mul(c2w, raster_to_camera);
+    c.pos = pos;
+    return c;
+}
+
+template <class R> struct Ray {
+    V3<R> o, d;
+    R tmin, tmax;
+};
+// math/Ray.h:17-19: the constructor moves the origin by RayEps along dir.
+template <class R> inline Ray<R> make_ray(const V3<R>& o, const V3<R>& d, R tmax, R tmin) {
+    return Ray<R>{o + d * K<R>::RayEps, d, tmin, tmax};
+}
+// PinHoleCamera::GenerateRay (camera/PinholeCamera.h:22-29), offset = 0.
+template <class R> Ray<R> generate_ray(const Camera<R>& c, int px, int py, R u, R v) {
+    V3<R> sampled = xf_point(c.raster_to_world, V3<R>(px + u, py + v, (R)-1));
+    V3<R> dir = norm(sampled - c.pos);
+    return make_ray(c.pos + dir * (R)0.0, dir, K<R>::Inf, (R)0);
+}
+
+// ---------------------------------------------------------------- Sobol
+// sobol::sample (sampler/Sobol.h:46-61) over the fixture copy of the
+// reference's matrices; SobolEnum::look_up (sampler/SobolEnum.h:42-72).
+static std::vector<uint32_t> g_sobol;  // ndim * 52
+static uint32_t g_sobol_ndim = 0;
+static uint64_t g_vdc[25][52], g_vdc_inv[26][52];
+static bool g_sobol_ok = false;
+
+inline float sobol_sample(uint64_t index, uint32_t dim) {
+    if (dim >= g_sobol_ndim) {
+        std::fprintf(stderr, "oracle: Sobol dimension %u beyond the %u loaded\n", dim, g_sobol_ndim);
+        std::abort();
+    }
+    unsigned result = 0;
+    for (unsigned i = dim * 52; index; index >>= 1, ++i)
+        if (index & 1) result ^= g_sobol[i];
+    return result * (1.f / (1ULL << 32));
+}
+inline uint64_t enum_look_up(uint32_t m, uint32_t frame, uint32_t px, uint32_t py) {
+    if (m == 0) return 0;
+    const uint32_t m2 = m << 1;
+    uint64_t index = uint64_t(frame) << m2;
+    uint64_t delta = 0;
+    for (uint32_t c = 0; frame; frame >>= 1, ++c)
+        if (frame & 1) delta ^= g_vdc[m - 1][c];
+    uint64_t b = ((uint64_t(px) << m) | py) ^ delta;
+    for (uint32_t c = 0; b; b >>= 1, ++c)
+        if (b & 1) index ^= g_vdc_inv[m - 1][c];
+    return index;
+}
+// math/MathUtils.h RoundUpPow2 / Log2Int, SobolEnum ctor (SobolEnum.h:17-21)
+inline int round_up_pow2(int v) {
+    v--;
+    v |= v >> 1; v |= v >> 2; v |= v >> 4; v |= v >> 8; v |= v >> 16;
+    return v + 1;
+}
+inline int log2_int(int v) {
+    int r = 0;
+    while ((1 << (r + 1)) <= v) ++r;
+    return r;
+}
+// RandomStateSequence::Sample (sampler/Sampler.h:53-64)
+struct Seq {
+    uint64_t instance;
+    int cursor = 0;
+    float next() {
+        float ret = sobol_sample(instance, (uint32_t)cursor++);
+        if (ret >= 1) ret = 0;
+        return ret;
+    }
+};
+
+// ---------------------------------------------------------------- numeric utils
+// numeric/NumericUtils.h:22-51
+inline float next_float_up(float v) {
+    if (std::isinf(v) && v > 0.) return v;
+    if (v == -0.f) v = 0.f;
+    uint32_t ui;
+    std::memcpy(&ui, &v, 4);
+    if (v >= 0) ++ui; else --ui;
+    std::memcpy(&v, &ui, 4);
+    return v;
+}
+inline float next_float_down(float v) {
+    if (std::isinf(v) && v < 0.) return v;
+    if (v == 0.f) v = -0.f;
+    uint32_t ui;
+    std::memcpy(&ui, &v, 4);
+    if (v > 0) --ui; else ++ui;
+    std::memcpy(&v, &ui, 4);
+    return v;
+}
+inline double next_float_up(double v) {
+    if (std::isinf(v) && v > 0.) return v;
+    if (v == -0.0) v = 0.0;
+    uint64_t ui;
+    std::memcpy(&ui, &v, 8);
+    if (v >= 0) ++ui; else --ui;
+    std::memcpy(&v, &ui, 8);
+    return v;
+}
+inline double next_float_down(double v) {
+    if (std::isinf(v) && v < 0.) return v;
+    if (v == 0.0) v = -0.0;
+    uint64_t ui;
+    std::memcpy(&ui, &v, 8);
+    if (v > 0) --ui; else ++ui;
+    std::memcpy(&v, &ui, 8);
+    return v;
+}
+template <class R> inline R gamma_n(int n) { return (n * K<R>::MachineEps) / (1 - n * K<R>::MachineEps); }
+
+// math/GeometryUtils.h:39-64 (the float build: no USING_DOUBLE widening)
+template <class R> V3<R> offset_ray_origin(const V3<R>& p, const V3<R>& pErr, const V3<R>& n, const V3<R>& w) {
+    R d = dot(vabs(n), pErr);
+    V3<R> offset = d * n;
+    if (dot(w, n) < 0) offset = -offset;
+    V3<R> po = p + offset;
+    for (int i = 0; i < 3; ++i) {
+        if (offset[i] > 0) po[i] = next_float_up(po[i]);
+        else if (offset[i] < 0) po[i] = next_float_down(po[i]);
+    }
+    return po;
+}
+
+// ---------------------------------------------------------------- scene
+template <class R> struct AABB {
+    V3<R> mn{K<R>::Inf, K<R>::Inf, K<R>::Inf}, mx{-K<R>::Inf, -K<R>::Inf, -K<R>::Inf};
+    const V3<R>& operator[](int i) const { return i == 0 ? mn : mx; }
+};
+template <class R> inline AABB<R> unite(const AABB<R>& a, const AABB<R>& b) {  // math/AABB.cpp:30-45
+    AABB<R> r;
+    r.mn = V3<R>(rmin(a.mn.x, b.mn.x), rmin(a.mn.y, b.mn.y), rmin(a.mn.z, b.mn.z));
+    r.mx = V3<R>(rmax(a.mx.x, b.mx.x), rmax(a.mx.y, b.mx.y), rmax(a.mx.z, b.mx.z));
+    return r;
+}
+template <class R> inline AABB<R> unite(const AABB<R>& a, const V3<R>& p) {  // math/AABB.cpp:47-60
+    AABB<R> r;
+    r.mn = V3<R>(rmin(a.mn.x, p.x), rmin(a.mn.y, p.y), rmin(a.mn.z, p.z));
+    r.mx = V3<R>(rmax(a.mx.x, p.x), rmax(a.mx.y, p.y), rmax(a.mx.z, p.z));
+    return r;
+}
+template <class R> inline R surface_area(const AABB<R>& b) {  // math/AABB.h:45-48
+    V3<R> d = b.mx - b.mn;
+    return 2 * (d.x * d.y + d.x * d.z + d.y * d.z);
+}
+template <class R> inline int max_extent(const AABB<R>& b) {  // math/AABB.h:55-62
+    V3<R> d = b.mx - b.mn;
+    if (d.x > d.y && d.x > d.z) return 0;
+    else if (d.y > d.z) return 1;
+    return 2;
+}
+
+struct Material {
+    uint32_t type, flags;
+    float R[3], T[3], ax, ay, eta[3], k[3], f0[3], eta1, eta2;
+    int32_t tex_albedo, tex_rough;
+};
+
+template <class R> struct Texture {
+    int w, h, c;
+    std::vector<R> data;  // row-major rows of w texels, c channels
+};
+
+template <class R> struct Tri {
+    V3<R> p0, p1, p2, n0, n1, n2, fn, e1, e2;
+    R uv[3][2];
+    uint32_t mat;
+    int32_t light;
+    R area;
+};
+
+template <class R> struct BVHNode {  // accelerator/BVHAccel.cpp:114-124
+    AABB<R> bounds;
+    uint32_t offset;  // primitivesOffset (leaf) / secondChildOffset (interior)
+    uint8_t nPrims, axis;
+};
+
+template <class R> struct Scene {
+    std::vector<Tri<R>> tris;
+    std::vector<Material> mats;
+    std::vector<int32_t> light_tri;
+    std::vector<V3<R>> light_le;
+    std::vector<R> light_cdf, light_func;
+    R light_func_int = 0;
+    std::vector<Texture<R>> tex;
+    std::vector<BVHNode<R>> nodes;
+    std::vector<uint32_t> ordered;  // BVH leaf order -> primitive id
+    int max_depth = 0;
+};
+
+// Triangle(..., normal) / Triangle(..., n0, n1, n2) + Initialize
+// (shape/Triangle.h:13-55, 107-118); Area (Triangle.h:67-69).
+template <class R> void build_tris(Scene<R>& s, const or_scene_desc& d) {
+    s.tris.resize(d.n_tri);
+    for (int i = 0; i < d.n_tri; ++i) {
+        Tri<R>& t = s.tris[i];
+        const float* p = d.pos + 9 * i;
+        const float* n = d.nrm + 9 * i;
+        t.p0 = V3<R>(p[0], p[1], p[2]);
+        t.p1 = V3<R>(p[3], p[4], p[5]);
+        t.p2 = V3<R>(p[6], p[7], p[8]);
+        t.n0 = V3<R>(n[0], n[1], n[2]);
+        t.n1 = V3<R>(n[3], n[4], n[5]);
+        t.n2 = V3<R>(n[6], n[7], n[8]);
+        t.fn = V3<R>(d.face_n[3 * i], d.face_n[3 * i + 1], d.face_n[3 * i + 2]);
+        for (int k = 0; k < 3; ++k)
+            for (int c = 0; c < 2; ++c) t.uv[k][c] = d.uv ? (R)d.uv[6 * i + 2 * k + c] : (R)0;
+        t.e1 = t.p1 - t.p0;
+        t.e2 = t.p2 - t.p0;
+        t.mat = d.mat[i];
+        t.light = d.light ? d.light[i] : -1;
+        t.area = (R)(0.5 * length(cross(t.p1 - t.p0, t.p2 - t.p0)));
+    }
+}
+
+// ---------------------------------------------------------------- BVH build
+// BVHAccel::BVHAccel / RecursiveBuild / FlattenBVHTree
+// (accelerator/BVHAccel.cpp:39-48, 154-200, 208-393), SAH split, maxPrims = 1.
+template <class R> struct PrimInfo {
+    uint32_t prim;
+    V3<R> centroid;
+    AABB<R> bounds;
+};
+template <class R> struct BuildNode {
+    AABB<R> bounds;
+    BuildNode* child[2] = {nullptr, nullptr};
+    uint32_t axis = 0, first = 0, n = 0;
+};
+template <class R> struct Builder {
+    Scene<R>& s;
+    std::vector<PrimInfo<R>> info;
+    std::deque<BuildNode<R>>* pool;
+    uint32_t total = 0;
+    BuildNode<R>* alloc() {
+        pool->emplace_back();
+        return &pool->back();
+    }
+    BuildNode<R>* build(uint32_t start, uint32_t end, int depth) {
+        total++;
+        if (depth > s.max_depth) s.max_depth = depth;
+        BuildNode<R>* node = alloc();
+        AABB<R> box;
+        for (uint32_t i = start; i < end; ++i) box = unite(box, info[i].bounds);
+        uint32_t nP = end - start;
+        auto make_leaf = [&]() {
+            uint32_t first = (uint32_t)s.ordered.size();
+            for (uint32_t i = start; i < end; ++i) s.ordered.push_back(info[i].prim);
+            node->first = first;
+            node->n = nP;
+            node->bounds = box;
+            return node;
+        };
+        if (nP == 1) return make_leaf();
+        AABB<R> cb;
+        for (uint32_t i = start; i < end; ++i) cb = unite(cb, info[i].centroid);
+        int dim = max_extent(cb);
+        uint32_t mid = (start + end) / 2;
+        auto interior = [&](uint32_t m) {
+            BuildNode<R>* c0 = build(start, m, depth + 1);
+            BuildNode<R>* c1 = build(m, end, depth + 1);
+            node->child[0] = c0;
+            node->child[1] = c1;
+            node->bounds = unite(c0->bounds, c1->bounds);
+            node->axis = dim;
+            node->n = 0;
+            return node;
+        };
+        if (cb.mx[dim] == cb.mn[dim]) {
+            // maxPrimsInNode = 1 < nP here: split at the midpoint
+            return interior(mid);
+        }
+        if (nP <= 4) {
+            mid = (start + end) / 2;
+            std::nth_element(&info[start], &info[mid], &info[end - 1] + 1,
+                             [dim](const PrimInfo<R>& a, const PrimInfo<R>& b) { return a.centroid[dim] < b.centroid[dim]; });
+        } else {
+            const int nB = 12;
+            int count[nB] = {0};
+            AABB<R> bb[nB];
+            for (uint32_t i = start; i < end; ++i) {
+                int b = nB * ((info[i].centroid[dim] - cb.mn[dim]) / (cb.mx[dim] - cb.mn[dim]));
+                if (b == nB) b = nB - 1;
+                count[b]++;
+                bb[b] = unite(bb[b], info[i].bounds);
+            }
+            R cost[nB - 1];
+            for (int i = 0; i < nB - 1; ++i) {
+                AABB<R> b0, b1;
+                int c0 = 0, c1 = 0;
+                for (int j = 0; j <= i; ++j) { b0 = unite(b0, bb[j]); c0 += count[j]; }
+                for (int j = i + 1; j < nB; ++j) { b1 = unite(b1, bb[j]); c1 += count[j]; }
+                cost[i] = (R).125f + (c0 * surface_area(b0) + c1 * surface_area(b1)) / surface_area(box);
+            }
+            R minCost = cost[0];
+            uint32_t split = 0;
+            for (int i = 1; i < nB - 1; ++i)
+                if (cost[i] < minCost) { minCost = cost[i]; split = i; }
+            // nPrimitives > maxPrimsInNode (= 1) always holds here
+            const AABB<R> cbc = cb;
+            PrimInfo<R>* pmid = std::partition(&info[start], &info[end - 1] + 1, [&](const PrimInfo<R>& p) {
+                int b = nB * ((p.centroid[dim] - cbc.mn[dim]) / (cbc.mx[dim] - cbc.mn[dim]));
+                if (b == nB) b = nB - 1;
+                return b <= (int)split;
+            });
+            mid = (uint32_t)(pmid - &info[0]);
+        }
+        return interior(mid);
+    }
+    uint32_t flatten(BuildNode<R>* node, uint32_t* offset) {
+        BVHNode<R>& ln = s.nodes[*offset];
+        ln.bounds = node->bounds;
+        uint32_t my = (*offset)++;
+        if (node->n > 0) {
+            ln.offset = node->first;
+            ln.nPrims = (uint8_t)node->n;
+        } else {
+            ln.axis = (uint8_t)node->axis;
+            ln.nPrims = 0;
+            flatten(node->child[0], offset);
+            uint32_t second = flatten(node->child[1], offset);
+            s.nodes[my].offset = second;
+        }
+        return my;
+    }
+};
+template <class R> void build_bvh(Scene<R>& s) {
+    Builder<R> b{s};
+    std::deque<BuildNode<R>> pool;
+    b.pool = &pool;
+    b.info.resize(s.tris.size());
+    for (size_t i = 0; i < s.tris.size(); ++i) {
+        const Tri<R>& t = s.tris[i];
+        AABB<R> box;
+        box.mn = V3<R>(rmin(t.p0.x, t.p1.x), rmin(t.p0.y, t.p1.y), rmin(t.p0.z, t.p1.z));
+        box.mx = V3<R>(rmax(t.p0.x, t.p1.x), rmax(t.p0.y, t.p1.y), rmax(t.p0.z, t.p1.z));
+        box = unite(box, t.p2);
+        b.info[i].prim = (uint32_t)i;
+        b.info[i].bounds = box;
+        b.info[i].centroid = (R).5f * box.mn + (R).5f * box.mx;
+    }
+    if (s.tris.empty()) return;
+    s.ordered.reserve(s.tris.size());
+    BuildNode<R>* root = b.build(0, (uint32_t)s.tris.size(), 0);
+    s.nodes.resize(b.total);
+    uint32_t off = 0;
+    b.flatten(root, &off);
+}
+
+// ---------------------------------------------------------------- intersection
+// Triangle::Intersect (shape/Triangle.cpp:71-103, 106-139)
+template <class R> inline bool tri_hit(const Tri<R>& tr, const Ray<R>& ray, R* t, R* b1o, R* b2o) {
+    V3<R> s1 = cross(ray.d, tr.e2);
+    R divisor = dot(s1, tr.e1);
+    if (divisor == 0.) return false;
+    R inv = 1.f / divisor;
+    V3<R> s = ray.o - tr.p0;
+    R b1 = dot(s, s1) * inv;
+    if (b1 < 0. || b1 > 1.) return false;
+    V3<R> s2 = cross(s, tr.e1);
+    R b2 = dot(ray.d, s2) * inv;
+    if (b2 < 0. || b1 + b2 > 1.) return false;
+    R th = dot(tr.e2, s2) * inv;
+    if (th < ray.tmin || th > ray.tmax) return false;
+    *t = th;
+    *b1o = b1;
+    *b2o = b2;
+    return true;
+}
+// IntersectP (accelerator/BVHAccel.cpp:127-149)
+template <class R> inline bool box_hit(const AABB<R>& bounds, const Ray<R>& ray, const V3<R>& invDir, const uint32_t neg[3]) {
+    R tmin = (bounds[neg[0]].x - ray.o.x) * invDir.x;
+    R tmax = (bounds[1 - neg[0]].x - ray.o.x) * invDir.x;
+    R tymin = (bounds[neg[1]].y - ray.o.y) * invDir.y;
+    R tymax = (bounds[1 - neg[1]].y - ray.o.y) * invDir.y;
+    if ((tmin > tymax) || (tymin > tmax)) return false;
+    if (tymin > tmin) tmin = tymin;
+    if (tymax < tmax) tmax = tymax;
+    R tzmin = (bounds[neg[2]].z - ray.o.z) * invDir.z;
+    R tzmax = (bounds[1 - neg[2]].z - ray.o.z) * invDir.z;
+    if ((tmin > tzmax) || (tzmin > tmax)) return false;
+    if (tzmin > tmin) tmin = tzmin;
+    if (tzmax < tmax) tmax = tzmax;
+    return (tmin < ray.tmax) && (tmax > ray.tmin);
+}
+template <class R> struct Hit {
+    R t, b1, b2;
+    int32_t prim = -1;
+};
+// BVHAccel::Intersect closest (BVHAccel.cpp:401-445) with Primitive::Intersect
+// shrinking tMax (visual/Primitive.cpp:5-18); brute force in scene order
+// (accelerator/BruteForceAccel.h:20-29).
+template <class R> bool intersect(const Scene<R>& s, Ray<R>& ray, Hit<R>* h, bool use_bvh, int64_t* cnt_node = nullptr,
+                                  int64_t* cnt_tri = nullptr) {
+    bool hit = false;
+    if (!use_bvh) {
+        for (size_t i = 0; i < s.tris.size(); ++i) {
+            R t, b1, b2;
+            if (tri_hit(s.tris[i], ray, &t, &b1, &b2)) {
+                ray.tmax = t;
+                h->t = t; h->b1 = b1; h->b2 = b2; h->prim = (int32_t)i;
+                hit = true;
+            }
+        }
+        return hit;
+    }
+    if (s.nodes.empty()) return false;
+    V3<R> invDir(1.f / ray.d.x, 1.f / ray.d.y, 1.f / ray.d.z);
+    uint32_t neg[3] = {invDir.x < 0, invDir.y < 0, invDir.z < 0};
+    uint32_t todoOffset = 0, nodeNum = 0;
+    uint32_t todo[256];
+    while (true) {
+        const BVHNode<R>& node = s.nodes[nodeNum];
+        if (cnt_node) ++*cnt_node;
+        if (box_hit(node.bounds, ray, invDir, neg)) {
+            if (node.nPrims > 0) {
+                for (uint32_t i = 0; i < node.nPrims; ++i) {
+                    uint32_t prim = s.ordered[node.offset + i];
+                    if (cnt_tri) ++*cnt_tri;
+                    R t, b1, b2;
+                    if (tri_hit(s.tris[prim], ray, &t, &b1, &b2)) {
+                        ray.tmax = t;
+                        h->t = t; h->b1 = b1; h->b2 = b2; h->prim = (int32_t)prim;
+                        hit = true;
+                    }
+                }
+                if (todoOffset == 0) break;
+                nodeNum = todo[--todoOffset];
+            } else {
+                if (todoOffset >= 255) { std::fprintf(stderr, "oracle: BVH stack overflow\n"); std::abort(); }
+                if (neg[node.axis]) {
+                    todo[todoOffset++] = nodeNum + 1;
+                    nodeNum = node.offset;
+                } else {
+                    todo[todoOffset++] = node.offset;
+                    nodeNum = nodeNum + 1;
+                }
+            }
+        } else {
+            if (todoOffset == 0) break;
+            nodeNum = todo[--todoOffset];
+        }
+    }
+    return hit;
+}
+// BVHAccel::Intersect any-hit (BVHAccel.cpp:448-487)
+template <class R> bool intersect_any(const Scene<R>& s, const Ray<R>& ray, bool use_bvh) {
+    R t, b1, b2;
+    if (!use_bvh) {
+        for (size_t i = 0; i < s.tris.size(); ++i)
+            if (tri_hit(s.tris[i], ray, &t, &b1, &b2)) return true;
+        return false;
+    }
+    if (s.nodes.empty()) return false;
+    V3<R> invDir(1.f / ray.d.x, 1.f / ray.d.y, 1.f / ray.d.z);
+    uint32_t neg[3] = {invDir.x < 0, invDir.y < 0, invDir.z < 0};
+    uint32_t todo[256];
+    uint32_t todoOffset = 0, nodeNum = 0;
+    while (true) {
+        const BVHNode<R>& node = s.nodes[nodeNum];
+        if (box_hit(node.bounds, ray, invDir, neg)) {
+            if (node.nPrims > 0) {
+                for (uint32_t i = 0; i < node.nPrims; ++i)
+                    if (tri_hit(s.tris[s.ordered[node.offset + i]], ray, &t, &b1, &b2)) return true;
+                if (todoOffset == 0) break;
+                nodeNum = todo[--todoOffset];
+            } else {
+                if (todoOffset >= 255) { std::fprintf(stderr, "oracle: BVH stack overflow\n"); std::abort(); }
+                if (neg[node.axis]) {
+                    todo[todoOffset++] = nodeNum + 1;
+                    nodeNum = node.offset;
+                } else {
+                    todo[todoOffset++] = node.offset;
+                    nodeNum = nodeNum + 1;
+                }
+            }
+        } else {
+            if (todoOffset == 0) break;
+            nodeNum = todo[--todoOffset];
+        }
+    }
+    return false;
+}
+
+// ---------------------------------------------------------------- surface interaction
+template <class R> struct Isect {
+    V3<R> pos, n, absN, ng, outDir, sdpdu, sdpdv, pErr;
+    R u = 0, v = 0;
+    int32_t prim = -1;
+};
+// Triangle::QueryIntersectionInfo (shape/Triangle.cpp:6-69); mPos from
+// Triangle::Intersect (Triangle.cpp:97).  dpdu/dpdv do not reach the
+// integrator and are not formed.
+template <class R> Isect<R> query(const Scene<R>& s, const Ray<R>& ray, const Hit<R>& h) {
+    const Tri<R>& t = s.tris[h.prim];
+    Isect<R> it;
+    R b1 = h.b1, b2 = h.b2;
+    it.pos = ((1 - b1 - b2) * t.p0 + b1 * t.p1) + b2 * t.p2;
+    R b0 = 1 - b1 - b2;
+    it.u = (b0 * t.uv[0][0] + b1 * t.uv[1][0]) + b2 * t.uv[2][0];
+    it.v = (b0 * t.uv[0][1] + b1 * t.uv[1][1]) + b2 * t.uv[2][1];
+    R xs = (std::abs(b0 * t.p0.x) + std::abs(b1 * t.p1.x)) + std::abs(b2 * t.p2.x);
+    R ys = (std::abs(b0 * t.p0.y) + std::abs(b1 * t.p1.y)) + std::abs(b2 * t.p2.y);
+    R zs = (std::abs(b0 * t.p0.z) + std::abs(b1 * t.p1.z)) + std::abs(b2 * t.p2.z);
+    V3<R> ns = (b0 * t.n0 + b1 * t.n1) + b2 * t.n2;
+    ns = norm(ns);  // Normalize(): v / Length() lane-wise
+    it.ng = (dot(t.fn, ns) < 0) ? -t.fn : t.fn;
+    it.absN = dot(it.ng, ray.d) < 0 ? it.ng : it.ng * (R)-1;
+    it.outDir = -ray.d;
+    it.pErr = gamma_n<R>(7) * V3<R>(xs, ys, zs);
+    R sg = (dot(ray.o - t.p0, ns) > 0) ? R(1.0f) : R(-1.0f);
+    V3<R> dpdus = norm(t.e1);
+    V3<R> dpdvs = norm(cross(sg * ns, dpdus));
+    dpdus = norm(cross(dpdvs, ns));
+    it.n = ns;
+    it.sdpdu = dpdus;
+    it.sdpdv = dpdvs;
+    it.prim = h.prim;
+    return it;
+}
+// Intersection::SpawnRay / SpawnTo (visual/Intersection.h:64-78)
+template <class R> Ray<R> spawn_ray(const Isect<R>& it, const V3<R>& d) {
+    V3<R> o = offset_ray_origin(it.pos, it.pErr, it.absN, d);
+    return make_ray(o, d, K<R>::Inf, (R)0);
+}
+template <class R> struct LightPoint {
+    V3<R> pos, n, fn, pErr;
+};
+// The Ray constructor moves the origin RayEps along d, which cancels the
+// ShadowRayEps shortening: the segment ends on the offset light point, and the
+// light's own triangle can occlude it depending on rounding.  g_shadow_short is
+// a test-only switch that ends the segment ShadowRayEps before the target.
+static int g_shadow_short = 0;
+template <class R> Ray<R> spawn_to(const Isect<R>& a, const LightPoint<R>& b) {
+    V3<R> origin = offset_ray_origin(a.pos, a.pErr, a.absN, b.pos - a.pos);
+    V3<R> target = offset_ray_origin(b.pos, b.pErr, b.fn, origin - b.pos);
+    V3<R> d = target - origin;
+    R dis = length(d);
+    d = norm(d);
+    Ray<R> r = make_ray(origin, d, dis - K<R>::ShadowRayEps, (R)0);
+    if (g_shadow_short) r.tmax = r.tmax - K<R>::RayEps;
+    return r;
+}
+
+// ---------------------------------------------------------------- sampling warps
+// visual/Sampling.cpp:5-41
+template <class R> inline void concentric_disk(R u0, R u1, R* dx, R* dy) {
+    R ox = (R)2.0 * u0 - 1, oy = (R)2.0 * u1 - 1;
+    if (ox == 0 && oy == 0) { *dx = 0; *dy = 0; return; }
+    R theta, r;
+    if (std::abs(ox) > std::abs(oy)) {
+        r = ox;
+        theta = K<R>::PI_Over4 * (oy / ox);
+    } else {
+        r = oy;
+        theta = K<R>::PI_Over2 - K<R>::PI_Over4 * (ox / oy);
+    }
+    *dx = r * std::cos(theta);
+    *dy = r * std::sin(theta);
+}
+template <class R> inline V3<R> cosine_hemisphere(R u0, R u1) {
+    R dx, dy;
+    concentric_disk(u0, u1, &dx, &dy);
+    R z = std::sqrt(rmax((R)0, 1 - dx * dx - dy * dy));
+    return V3<R>(dx, dy, z);
+}
+
+// ---------------------------------------------------------------- textures
+// ImageTexture::Triangle / Texel (texture/ImageTexture.h:50-69), CLAMP wrap.
+template <class R> inline R texel(const Texture<R>& t, int u, int v, int ch) {
+    u = u < 0 ? 0 : (u > t.w - 1 ? t.w - 1 : u);
+    v = v < 0 ? 0 : (v > t.h - 1 ? t.h - 1 : v);
+    return t.data[((size_t)v * t.w + u) * t.c + ch];
+}
+template <class R> inline R tex_bilinear(const Texture<R>& t, R u, R v, int ch) {
+    R s = t.w * u - (R)0.5f, tt = t.h * v - (R)0.5f;
+    int s0 = (int)std::floor(s), t0 = (int)std::floor(tt);
+    R ds = s - s0, dt = tt - t0;
+    R a = (1 - ds) * texel(t, s0, t0, ch) + ds * texel(t, s0 + 1, t0, ch);
+    R b = (1 - ds) * texel(t, s0, t0 + 1, ch) + ds * texel(t, s0 + 1, t0 + 1, ch);
+    return (1 - dt) * a + dt * b;
+}
+
+// ---------------------------------------------------------------- microfacet (GGX)
+// GGXDistribution (visual/Microfacet.cpp:232-379), sample-visible-area mode.
+template <class R> struct GGX {
+    R ax, ay;
+    R D(const V3<R>& m) const {
+        if (m.z <= 0) return (R)0.0f;
+        R cos2 = m.z * m.z;
+        R be = ((m.x * m.x) / (ax * ax) + (m.y * m.y) / (ay * ay)) / cos2;
+        R root = ((R)1 + be) * cos2;
+        R result = (R)1 / (K<R>::PI * ax * ay * root * root);
+        if (result * m.z < 1e-20f) result = 0;
+        return result;
+    }
+    R project_roughness(const V3<R>& v) const {
+        R invSin2 = 1 / rmax((R)0, (R)1 - v.z * v.z);
+        if (ax == ay || invSin2 <= 0) return ax;
+        R cosPhi2 = v.x * v.x * invSin2;
+        R sinPhi2 = v.y * v.y * invSin2;
+        return std::sqrt(cosPhi2 * ax * ax + sinPhi2 * ay * ay);
+    }
+    R smithG1(const V3<R>& v, const V3<R>& m) const {
+        if (dot(v, m) * v.z <= 0) return (R)0.0f;
+        R sinT = std::sqrt(rmax((R)0, (R)1 - v.z * v.z));
+        R tanT = std::abs(sinT / v.z);
+        if (tanT == 0.0f) return (R)1.0f;
+        R alpha = project_roughness(v);
+        R root = alpha * tanT;
+        return (R)2.0f / ((R)1.0f + std::sqrt(1 + root * root));
+    }
+    R G(const V3<R>& wo, const V3<R>& wi, const V3<R>& wh) const { return smithG1(wo, wh) * smithG1(wi, wh); }
+    R pdf(const V3<R>& wo, const V3<R>& wh) const {
+        if (wo.z == 0) return (R)0.0f;
+        return smithG1(wo, wh) * std::abs(dot(wo, wh)) * D(wh) / std::abs(wo.z);
+    }
+    // SampleGGXVNDF (Heitz 2018), Microfacet.cpp:318-339
+    V3<R> sample_wh(const V3<R>& v_, R u0, R u1) const {
+        V3<R> v = norm(V3<R>(ax * v_.x, ay * v_.y, v_.z));
+        V3<R> T1 = (v.z < 0.9999) ? norm(cross(v, V3<R>(0, 0, 1))) : V3<R>(1, 0, 0);
+        V3<R> T2 = cross(T1, v);
+        R a = 1 / ((R)1.f + v.z);
+        R r = std::sqrt(u0);
+        R phi = (u1 < a) ? u1 / a * K<R>::PI : K<R>::PI + (u1 - a) / ((R)1.f - a) * K<R>::PI;
+        R P1 = r * std::cos(phi);
+        R P2 = r * std::sin(phi) * ((u1 < a) ? (R)1.0f : v.z);
+        V3<R> dir = (P1 * T1 + P2 * T2) + std::sqrt(rmax((R)0, 1 - P1 * P1 - P2 * P2)) * v;
+        return norm(V3<R>(ax * dir.x, ay * dir.y, rmax((R)0, dir.z)));
+    }
+};
+template <class R> inline V3<R> reflect(const V3<R>& in, const V3<R>& n) {  // GeometryUtils.h:20-22
+    return 2.f * dot(in, n) * n - in;
+}
+template <class R> inline bool same_hemi(const V3<R>& a, const V3<R>& b) { return a.z * b.z > 0; }
+// FrDielectric (bsdf/BSDF.cpp:5-26)
+template <class R> R fr_dielectric(R cosI, R etaI, R etaT) {
+    cosI = clampr(cosI, (R)-1.f, (R)1.f);
+    bool entering = cosI > 0.f;
+    if (!entering) {
+        std::swap(etaI, etaT);
+        cosI = std::abs(cosI);
+    }
+    R sinI = std::sqrt(rmax((R)0, 1 - cosI * cosI));
+    R sinT = etaI / etaT * sinI;
+    if (sinT >= 1) return 1;
+    R cosT = std::sqrt(rmax((R)0, 1 - sinT * sinT));
+    R Rparl = ((etaT * cosI) - (etaI * cosT)) / ((etaT * cosI) + (etaI * cosT));
+    R Rperp = ((etaI * cosI) - (etaT * cosT)) / ((etaI * cosI) + (etaT * cosT));
+    return (Rparl * Rparl + Rperp * Rperp) / 2;
+}
+// FrConductor (bsdf/BSDF.cpp:29-51) with etai = (1,1,1)
+template <class R> V3<R> fr_conductor(R cosI, const V3<R>& etai, const V3<R>& etat, const V3<R>& k) {
+    cosI = clampr(cosI, (R)-1, (R)1);
+    V3<R> eta = etat / etai;
+    V3<R> etak = k / etai;
+    R cos2 = cosI * cosI;
+    R sin2 = (R)(1. - cos2);
+    V3<R> eta2 = eta * eta;
+    V3<R> etak2 = etak * etak;
+    V3<R> t0 = eta2 - etak2 - V3<R>(sin2);
+    V3<R> a2pb2 = vsqrt(t0 * t0 + (R)4 * eta2 * etak2);
+    V3<R> t1 = a2pb2 + V3<R>(cos2);
+    V3<R> a = vsqrt((R)0.5f * (a2pb2 + t0));
+    V3<R> t2 = (R)2 * cosI * a;
+    V3<R> Rs = (t1 - t2) / (t1 + t2);
+    V3<R> t3 = cos2 * a2pb2 + V3<R>(sin2 * sin2);
+    V3<R> t4 = t2 * sin2;
+    V3<R> Rp = Rs * (t3 - t4) / (t3 + t4);
+    return (R)0.5 * (Rp + Rs);
+}
+// Schlick Fresnel from F0: an extension the configurations require (the
+// reference conductor takes eta/k only).  F = F0 + (1 - F0) (1 - |cos|)^5.
+template <class R> V3<R> fr_schlick(R cosI, const V3<R>& f0) {
+    R c = (R)1 - std::abs(cosI);
+    R c2 = c * c;
+    R c5 = c2 * c2 * c;
+    return f0 + (V3<R>(1) - f0) * c5;
+}
+// TransmissionBSDF::Refract (bsdf/BSDF.h:292-304)
+template <class R> bool refract(const V3<R>& wi, const V3<R>& n, R eta, V3<R>* wt) {
+    R cosI = dot(n, wi);
+    R sin2I = rmax(R(0), R(1 - cosI * cosI));
+    R sin2T = eta * eta * sin2I;
+    if (sin2T >= 1) return false;
+    R cosT = std::sqrt(1 - sin2T);
+    *wt = eta * (wi * (R)-1) + (eta * cosI - cosT) * n;
+    return true;
+}
+// visual/Microfacet.h:49-54
+template <class R> inline R roughness_to_alpha(R r) {
+    r = rmax(r, (R)1e-3);
+    R x = std::log(r);
+    return (R)1.62142f + (R)0.819955f * x + (R)0.1734f * x * x + (R)0.0171201f * x * x * x + (R)0.000640711f * x * x * x * x;
+}
+
+// ---------------------------------------------------------------- BSDF
+// BSDF with one lobe (material/*.h build exactly one BxDF), bsdf/BSDF.cpp:53-119.
+enum { MAT_LAMBERT = 0, MAT_CONDUCTOR = 1, MAT_GLASS = 2, MAT_ROUGH_DIELECTRIC = 3 };
+template <class R> struct BSDF {
+    int type;
+    V3<R> ng, X, Y, Z;
+    V3<R> Rc, Tc;  // reflectance / transmittance
+    GGX<R> dist;
+    int fresnel_schlick;
+    V3<R> eta, k, f0;
+    R eta1, eta2;
+    bool delta() const { return type == MAT_GLASS; }
+    V3<R> to_local(const V3<R>& v) const { return V3<R>(dot(v, X), dot(v, Y), dot(v, Z)); }
+    V3<R> to_world(const V3<R>& v) const { return (X * v.x + Y * v.y) + Z * v.z; }
+    V3<R> fresnel_cond(R c) const {
+        return fresnel_schlick ? fr_schlick(c, f0) : fr_conductor(std::abs(c), V3<R>(1.0, 1.0, 1.0), eta, k);
+    }
+    // --- lobe functions in the local frame
+    R lobe_pdf(const V3<R>& wo, const V3<R>& wi) const {
+        switch (type) {
+            case MAT_LAMBERT:  // DiffuseBSDF::Pdf (bsdf/BSDF.h:170-173)
+                return same_hemi(wo, wi) ? std::abs(wi.z) * K<R>::INV_PI : (R)0.f;
+            case MAT_CONDUCTOR: {  // MicrofacetReflectionBSDF::Pdf (bsdf/BSDF.h:323-333)
+                V3<R> wh = norm(wo + wi);
+                R dwh = 1.f / (4 * dot(wi, wh));
+                wh = wh * sgn(wh.z);
+                return std::abs(dist.pdf(wo, wh) * dwh);
+            }
+            case MAT_GLASS:
+                return (R)0.0;
+            default: {  // RoughDielectricBSDF::Pdf (bsdf/BSDF.h:513-543)
+                bool refl = wo.z * wi.z > 0;
+                V3<R> wh;
+                R dwh;
+                if (refl) {
+                    wh = norm(wo + wi);
+                    dwh = 1.f / (4 * dot(wi, wh));
+                } else {
+                    R e = wo.z > 0 ? (eta2 / eta1) : (eta1 / eta2);
+                    wh = -(norm(wo + wi * e));
+                    R sd = dot(wo, wh) + e * dot(wi, wh);
+                    dwh = std::abs((e * e * dot(wi, wh)) / (sd * sd));
+                }
+                wh = wh * sgn(wh.z);
+                R pdf = std::abs(dist.pdf(wo, wh) * dwh);
+                R F = fr_dielectric(dot(wo, wh), eta1, eta2);
+                pdf *= refl ? F : (1 - F);
+                return pdf;
+            }
+        }
+    }
+    V3<R> lobe_eval(const V3<R>& wo, const V3<R>& wi) const {
+        switch (type) {
+            case MAT_LAMBERT:
+                return Rc * K<R>::INV_PI;
+            case MAT_CONDUCTOR: {  // MicrofacetReflectionBSDF::Evaluate (bsdf/BSDF.h:356-373)
+                if (!same_hemi(wo, wi)) return V3<R>();
+                R cO = std::abs(wo.z), cI = std::abs(wi.z);
+                V3<R> wh = wi + wo;
+                if (cI == 0 || cO == 0) return V3<R>(0, 0, 0);
+                if (wh.x == 0 && wh.y == 0 && wh.z == 0) return V3<R>(0, 0, 0);
+                wh = norm(wh);
+                wh = sgn(wh.z) * wh;
+                V3<R> F = fresnel_cond(dot(wo, wh));
+                V3<R> D(dist.D(wh));
+                V3<R> G(dist.G(wo, wi, wh));
+                return Rc * F * D * G / (4 * cI * cO);
+            }
+            case MAT_GLASS:
+                return V3<R>();
+            default: {  // RoughDielectricBSDF::Evaluate (bsdf/BSDF.h:469-511)
+                bool refl = wo.z * wi.z > 0;
+                if (refl) {
+                    R cO = std::abs(wo.z), cI = std::abs(wi.z);
+                    V3<R> wh = wi + wo;
+                    if (cI == 0 || cO == 0) return V3<R>(0, 0, 0);
+                    if (wh.x == 0 && wh.y == 0 && wh.z == 0) return V3<R>(0, 0, 0);
+                    wh = norm(wh);
+                    wh = sgn(wh.z) * wh;
+                    R F = fr_dielectric(dot(wo, wh), eta1, eta2);
+                    V3<R> D(dist.D(wh));
+                    V3<R> G(dist.G(wo, wi, wh));
+                    return Rc * V3<R>(F) * D * G / (4 * cI * cO);
+                }
+                R cO = wo.z, cI = wi.z;
+                if (cI == 0 || cO == 0) return V3<R>();
+                R e = wo.z > 0 ? (eta2 / eta1) : (eta1 / eta2);
+                V3<R> wh = (-norm(wo + wi * e));
+                wh = sgn(wh.z) * wh;
+                R F = fr_dielectric(dot(wo, wh), eta1, eta2);
+                R D = dist.D(wh);
+                R G = dist.G(wo, wi, wh);
+                R sd = dot(wo, wh) + e * dot(wi, wh);
+                R factor = 1 / e;  // radiance transport
+                R value = (1 - F) * D * G * e * e * std::abs(dot(wi, wh)) * std::abs(dot(wo, wh)) /
+                          std::abs(cI * cO * sd * sd);
+                return Tc * std::abs(value * factor * factor);
+            }
+        }
+    }
+    V3<R> lobe_sample(const V3<R>& wo, V3<R>* wi, R* pdf, const V3<R>& rnd) const {
+        switch (type) {
+            case MAT_LAMBERT: {  // DiffuseBSDF::Sample (bsdf/BSDF.h:175-181)
+                *wi = cosine_hemisphere(rnd.x, rnd.y);
+                if (wo.z < 0) wi->z *= -1;
+                *pdf = lobe_pdf(wo, *wi);
+                return lobe_eval(wo, *wi);
+            }
+            case MAT_CONDUCTOR: {  // MicrofacetReflectionBSDF::Sample (bsdf/BSDF.h:335-354)
+                if (wo.z == 0) return V3<R>(0.);
+                V3<R> wh = dist.sample_wh(sgn(wo.z) * wo, rnd.x, rnd.y);
+                R mpdf = dist.pdf(wo, wh);
+                if (mpdf == 0) return V3<R>();
+                *wi = reflect(wo, wh);
+                if (!same_hemi(wo, *wi)) return V3<R>(0, 0, 0);
+                wh = sgn(wh.z) * wh;
+                *pdf = std::abs(dist.pdf(wo, wh) / (4 * dot(*wi, wh)));
+                return lobe_eval(wo, *wi);
+            }
+            case MAT_GLASS: {  // TransmissionBSDF::Sample (bsdf/BSDF.h:238-270)
+                R Re = fr_dielectric(wo.z, eta1, eta2);
+                bool entering = wo.z > 0;
+                R etaI = entering ? eta1 : eta2;
+                R etaT = entering ? eta2 : eta1;
+                V3<R> nF = wo.z > 0 ? V3<R>(0, 0, 1) : V3<R>(0, 0, -1);
+                R P = Re;
+                if (rnd.y < P) {
+                    *wi = V3<R>(-wo.x, -wo.y, wo.z);
+                    *pdf = P;
+                    R c = std::abs(wi->z);
+                    return Rc * Re / c;
+                }
+                if (!refract(wo, nF, etaI / etaT, wi)) return V3<R>();
+                *pdf = 1 - P;
+                R c = std::abs(wi->z);
+                R e = etaI / etaT;
+                return e * e * Tc * ((R)1.f - Re) / c;
+            }
+            default: {  // RoughDielectricBSDF::Sample (bsdf/BSDF.h:545-615)
+                V3<R> wh = dist.sample_wh(sgn(wo.z) * wo, rnd.x, rnd.y);
+                R mpdf = dist.pdf(wo, wh);
+                if (mpdf == 0) return V3<R>();
+                R F = fr_dielectric(dot(wo, wh), eta1, eta2);
+                R P = F;
+                *pdf = 0;
+                if (rnd.z < P) {
+                    if (wo.z == 0) return V3<R>();
+                    *wi = reflect(wo, wh);
+                    if (!same_hemi(wo, *wi)) return V3<R>(0, 0, 0);
+                    R dwh = 1.f / (4 * dot(*wi, wh));
+                    *pdf = P * std::abs(dist.pdf(wo, wh) * dwh);
+                    R cO = std::abs(wo.z), cI = std::abs(wi->z);
+                    V3<R> D(dist.D(wh));
+                    V3<R> G(dist.G(wo, *wi, wh));
+                    return Rc * V3<R>(F) * D * G / (4 * cI * cO);
+                }
+                if (wo.z == 0) return V3<R>();
+                R etaR = wo.z > 0 ? (eta1 / eta2) : (eta2 / eta1);
+                V3<R> ff = (dot(wh, wo) < 0.f) ? -wh : wh;
+                if (!refract(wo, ff, etaR, wi)) {
+                    *pdf = 0;
+                    return V3<R>();
+                }
+                if (wo.z * wi->z >= 0) return V3<R>();
+                R e = wo.z > 0 ? (eta2 / eta1) : (eta1 / eta2);
+                R sd = dot(wo, wh) + e * dot(*wi, wh);
+                R dwh = std::abs((e * e * dot(*wi, wh)) / (sd * sd));
+                *pdf = std::abs((1 - P) * dist.pdf(wo, wh) * dwh);
+                R cO = wo.z, cI = wi->z;
+                if (cI == 0 || cO == 0) return V3<R>();
+                R D = dist.D(wh);
+                R G = dist.G(wo, *wi, wh);
+                R factor = 1 / e;
+                R value = (1 - F) * D * G * e * e * std::abs(dot(*wi, wh)) * std::abs(dot(wo, wh)) /
+                          std::abs(cI * cO * sd * sd);
+                return Tc * std::abs(value * factor * factor);
+            }
+        }
+    }
+    // --- BSDF wrappers (bsdf/BSDF.cpp:58-119)
+    R Pdf(const V3<R>& wo, const V3<R>& wi) const {
+        V3<R> woL = to_local(wo), wiL = to_local(wi);
+        R pdf = 0.f;
+        pdf += lobe_pdf(woL, wiL);
+        return pdf / 1;
+    }
+    V3<R> Sample(const V3<R>& wo, V3<R>* wi, R* pdf, const V3<R>& rnd) const {
+        V3<R> woL = to_local(wo), wiL;
+        *pdf = 0;
+        V3<R> f = lobe_sample(woL, &wiL, pdf, rnd);
+        *wi = to_world(wiL);
+        *pdf /= 1;
+        return f;
+    }
+    V3<R> Evaluate(const V3<R>& wo, const V3<R>& wi) const {
+        V3<R> f(0, 0, 0);
+        V3<R> woL = to_local(wo), wiL = to_local(wi);
+        bool refl = dot(wi, ng) * dot(wo, ng) > 0;
+        bool has_refl = true;                         // every lobe here has BSDF_REFLECTION
+        bool has_trans = type == MAT_GLASS || type == MAT_ROUGH_DIELECTRIC;
+        if ((refl && has_refl) || (!refl && has_trans)) f = f + lobe_eval(woL, wiL);
+        return f;
+    }
+};
+// Material::ComputeScatteringFunction (material/Diffuse.h:27-31, Roughness.h:36-46,
+// Glass.h:30-34, RoughDielectric.h:21-41)
+template <class R> BSDF<R> make_bsdf(const Scene<R>& s, const Isect<R>& it) {
+    const Material& m = s.mats[s.tris[it.prim].mat];
+    BSDF<R> b;
+    b.type = (int)m.type;
+    b.ng = it.ng;
+    b.X = it.sdpdu;
+    b.Y = it.sdpdv;
+    b.Z = it.n;
+    b.Rc = V3<R>(m.R[0], m.R[1], m.R[2]);
+    b.Tc = V3<R>(m.T[0], m.T[1], m.T[2]);
+    if (m.tex_albedo >= 0) {
+        const Texture<R>& t = s.tex[m.tex_albedo];
+        b.Rc = V3<R>(tex_bilinear(t, it.u, it.v, 0), tex_bilinear(t, it.u, it.v, t.c > 1 ? 1 : 0),
+                     tex_bilinear(t, it.u, it.v, t.c > 2 ? 2 : 0));
+    }
+    R ax = m.ax, ay = m.ay;
+    if (m.tex_rough >= 0) {
+        ax = ay = tex_bilinear(s.tex[m.tex_rough], it.u, it.v, 0);
+    }
+    if (!(m.flags & 1u)) {
+        ax = roughness_to_alpha(ax);
+        ay = roughness_to_alpha(ay);
+    }
+    b.dist.ax = ax;
+    b.dist.ay = ay;
+    b.fresnel_schlick = (m.flags & 2u) ? 1 : 0;
+    b.eta = V3<R>(m.eta[0], m.eta[1], m.eta[2]);
+    b.k = V3<R>(m.k[0], m.k[1], m.k[2]);
+    b.f0 = V3<R>(m.f0[0], m.f0[1], m.f0[2]);
+    b.eta1 = m.eta1;
+    b.eta2 = m.eta2;
+    return b;
+}
+
+// ---------------------------------------------------------------- lights
+// Scene::ComputeLightPowerDistribution + Distribution1D (visual/Scene.h:121-128,
+// visual/Sampling.h:16-97), AreaLight::Power (light/AreaLight.h:45-48).
+template <class R> void build_light_distribution(Scene<R>& s) {
+    int n = (int)s.light_tri.size();
+    s.light_func.resize(n);
+    for (int i = 0; i < n; ++i) {
+        V3<R> P = s.light_le[i] * s.tris[s.light_tri[i]].area * K<R>::PI;
+        s.light_func[i] = lumY(P);
+    }
+    s.light_cdf.assign(n + 1, 0);
+    s.light_cdf[0] = 0.;
+    for (int i = 1; i < n + 1; ++i) s.light_cdf[i] = s.light_cdf[i - 1] + s.light_func[i - 1] / n;
+    s.light_func_int = s.light_cdf[n];
+    if (s.light_func_int == 0.f) {
+        for (int i = 1; i < n + 1; ++i) s.light_cdf[i] = R(i) / R(n);
+    } else {
+        for (int i = 1; i < n + 1; ++i) s.light_cdf[i] /= s.light_func_int;
+    }
+}
+template <class R> int sample_one_light(const Scene<R>& s, R u, R* pdf) {
+    int n = (int)s.light_func.size();
+    auto it = std::upper_bound(s.light_cdf.begin(), s.light_cdf.begin() + n + 1, u);
+    int off = std::max(0, int(std::distance(s.light_cdf.begin(), it)) - 1);
+    *pdf = s.light_func[off] / (s.light_func_int * n);
+    return off;
+}
+// Triangle::Sample (shape/Triangle.cpp:141-158) + Shape::Sample (shape/Shape.cpp:6-19)
+template <class R> LightPoint<R> sample_light_point(const Tri<R>& t, const Isect<R>& it, R u0, R u1, R* pdf) {
+    R su0 = std::sqrt(u0);
+    R b0 = 1 - su0, b1 = u1 * su0;
+    LightPoint<R> lp;
+    R b2 = 1 - b0 - b1;
+    lp.pos = (b0 * t.p0 + b1 * t.p1) + b2 * t.p2;
+    lp.fn = t.fn;
+    lp.n = (b0 * t.n0 + b1 * t.n1) + b2 * t.n2;
+    V3<R> pAbs = (vabs(b0 * t.p0) + vabs(b1 * t.p1)) + vabs(b2 * t.p2);
+    lp.pErr = gamma_n<R>(6) * pAbs;
+    *pdf = 1 / t.area;
+    V3<R> wi = it.pos - lp.pos;
+    if (length(wi) == 0) {
+        *pdf = 0;
+    } else {
+        wi = norm(wi);
+        V3<R> dd = lp.pos - it.pos;
+        *pdf = *pdf * (dot(dd, dd) / std::abs(dot(lp.n, -wi)));
+        if (std::isinf(*pdf)) *pdf = 0;
+    }
+    return lp;
+}
+// Shape::Pdf (shape/Shape.h:34-47): intersect only the light's triangle.
+template <class R> R light_pdf(const Scene<R>& s, int tri, const Isect<R>& it, const V3<R>& wi) {
+    Ray<R> ray = spawn_ray(it, wi);
+    const Tri<R>& t = s.tris[tri];
+    Hit<R> h;
+    R th, b1, b2;
+    if (!tri_hit(t, ray, &th, &b1, &b2)) return 0;
+    h.t = th; h.b1 = b1; h.b2 = b2; h.prim = tri;
+    Isect<R> lp = query(s, ray, h);
+    V3<R> d = lp.pos - it.pos;
+    R pdf = dot(d, d) / (std::abs(dot(lp.n, wi * (R)-1)) * t.area);
+    if (std::isinf(pdf)) return 0;
+    return pdf;
+}
+template <class R> inline R power_heuristic(R f, R g) {  // integrator/Integrator.h:11-15
+    f = 1 * f;
+    g = 1 * g;
+    return (f * f) / (f * f + g * g);
+}
+
+// Integrator::DirectIllumination (integrator/Integrator.cpp:9-107), surfaces only.
+template <class R> V3<R> direct_illumination(const Scene<R>& s, const Isect<R>& it, const BSDF<R>& bsdf, R uLight,
+                                             R u0, R u1, const V3<R>& v, bool use_bvh) {
+    R lightSamplingPdf;
+    R weight1 = 1, weight2 = 1;
+    V3<R> L1, L2;
+    int light = sample_one_light(s, uLight, &lightSamplingPdf);
+    int ltri = s.light_tri[light];
+    V3<R> Le = s.light_le[light];
+    {
+        R pdf;
+        LightPoint<R> lp = sample_light_point(s.tris[ltri], it, u0, u1, &pdf);
+        V3<R> wi = norm(lp.pos - it.pos);
+        V3<R> Li = dot(lp.n, -wi) > 0 ? Le : V3<R>(0, 0, 0);
+        if (pdf > 0 && Li != V3<R>(0)) {
+            R scatteringPdf = bsdf.Pdf(it.outDir, wi);
+            V3<R> f = bsdf.Evaluate(it.outDir, wi);
+            R cosTheta = std::abs(dot(it.n, wi));
+            if (f != V3<R>()) {
+                V3<R> Tr(1.f);
+                Ray<R> sr = spawn_to(it, lp);
+                if (intersect_any(s, sr, use_bvh)) Li = V3<R>();
+                weight1 = power_heuristic(pdf, scatteringPdf);
+                L1 = Li * Tr * f * cosTheta / pdf;
+            }
+        }
+    }
+    {
+        V3<R> wi;
+        R pdf;
+        V3<R> f = bsdf.Sample(it.outDir, &wi, &pdf, v);
+        R cosTheta = std::abs(dot(it.n, wi));
+        if (pdf != 0 && f != V3<R>()) {
+            R lpdf = light_pdf(s, ltri, it, wi);
+            if (lpdf == 0) {
+                weight2 = 0;
+            } else {
+                weight2 = power_heuristic(pdf, lpdf);
+                Ray<R> ray = spawn_ray(it, wi);
+                Hit<R> h;
+                if (intersect(s, ray, &h, use_bvh)) {
+                    if (s.tris[h.prim].light == light) {
+                        Isect<R> li = query(s, ray, h);
+                        V3<R> Lemit = dot(li.n, -wi) > 0 ? Le : V3<R>(0, 0, 0);
+                        V3<R> Tr(1.f);
+                        L2 = Lemit * f * Tr * cosTheta / pdf;
+                    }
+                }
+            }
+        }
+    }
+    return (L1 * weight1 + L2 * weight2) / lightSamplingPdf;
+}
+
+// PathTracing::Li (integrator/PT.cpp:10-67).  The dimensions of the
+// DirectIllumination(rand(), Vec2(rand(), rand()), Vec3(rand(), rand(), rand()))
+// call (PT.cpp:38) and of Vec3(rand(), rand(), rand()) (PT.cpp:47) are drawn in
+// the order written, left to right (the canonical order of this build; C++
+// leaves argument evaluation order unspecified).
+template <class R> V3<R> Li(const Scene<R>& s, Ray<R> ray, Seq& rnd, int maxDepth, int rrDepth, bool use_bvh) {
+    bool delta = false;
+    V3<R> T(1, 1, 1);
+    V3<R> L;
+    for (int i = 0; i < maxDepth; ++i) {
+        Hit<R> h;
+        if (!intersect(s, ray, &h, use_bvh)) break;
+        Isect<R> it = query(s, ray, h);
+        BSDF<R> bsdf = make_bsdf(s, it);
+        int light = s.tris[it.prim].light;
+        if ((i == 0 || delta) && light >= 0) {
+            V3<R> Le = s.light_le[light];
+            V3<R> e = dot(it.n, it.outDir) > 0 ? Le : V3<R>(0, 0, 0);
+            L = L + T * e;
+        } else {
+            R uL = rnd.next();
+            R u0 = rnd.next(), u1 = rnd.next();
+            R v0 = rnd.next(), v1 = rnd.next(), v2 = rnd.next();
+            L = L + T * direct_illumination(s, it, bsdf, uL, u0, u1, V3<R>(v0, v1, v2), use_bvh);
+        }
+        R r0 = rnd.next(), r1 = rnd.next(), r2 = rnd.next();
+        V3<R> wi;
+        R pdf;
+        V3<R> f = bsdf.Sample(-ray.d, &wi, &pdf, V3<R>(r0, r1, r2));
+        if (f == V3<R>() || pdf == 0) break;
+        V3<R> est = f * std::abs(dot(it.n, wi)) / pdf;
+        delta = bsdf.delta();
+        R p = rmin((R)1.0, lumY(est * T) / lumY(T));
+        if (p < 1 && i > rrDepth) {
+            if ((R)rnd.next() < p) T = T / p;
+            else break;
+        }
+        T = T * est;
+        ray = spawn_ray(it, wi);
+    }
+    return L;
+}
+
+// One camera sample of TiledIntegrator::Render (integrator/TiledIntegrator.cpp:40-47).
+template <class R> V3<R> camera_sample(const Scene<R>& s, const Camera<R>& cam, int res, int m, int x, int y, int smp,
+                                       int maxDepth, int rrDepth, bool use_bvh, R* fx, R* fy, int* dims) {
+    Seq rnd;
+    rnd.instance = enum_look_up((uint32_t)m, (uint32_t)smp, (uint32_t)x, (uint32_t)y);
+    R u = res * (R)rnd.next() - R(x);  // SobolEnum::SampleX (SobolEnum.h:30-32)
+    R v = res * (R)rnd.next() - R(y);
+    Ray<R> ray = generate_ray(cam, x, y, u, v);
+    V3<R> L = Li(s, ray, rnd, maxDepth, rrDepth, use_bvh);
+    *fx = x + u;
+    *fy = y + v;
+    if (dims) *dims = rnd.cursor;
+    return L;
+}
+
+// Film::AddSample with the radius-0.5 box filter (visual/Film.h:61-87, math/Filter.h:26-38)
+template <class R> struct Film {
+    int W, H;
+    std::vector<R> color, weight;
+    std::vector<std::atomic_flag> locks;
+    Film(int w, int h) : W(w), H(h), color((size_t)w * h * 3, 0), weight((size_t)w * h, 0), locks((size_t)w * h) {
+        for (auto& l : locks) l.clear();
+    }
+    void add(R x, R y, const V3<R>& L) {
+        x -= 0.5;
+        y -= 0.5;
+        const R rad = (R)0.5;
+        int minX = (int)std::ceil(x - rad), maxX = (int)std::floor(x + rad);
+        int minY = (int)std::ceil(y - rad), maxY = (int)std::floor(y + rad);
+        minX = std::max(0, minX);
+        maxX = std::min(maxX, W - 1);
+        minY = std::max(0, minY);
+        maxY = std::min(maxY, H - 1);
+        for (int i = minY; i <= maxY; i++)
+            for (int j = minX; j <= maxX; j++) {
+                size_t p = (size_t)i * W + j;
+                while (locks[p].test_and_set(std::memory_order_acquire)) {
+                }
+                R w = 1.0;
+                weight[p] += w;
+                color[3 * p + 0] = color[3 * p + 0] + L.x * w;
+                color[3 * p + 1] = color[3 * p + 1] + L.y * w;
+                color[3 * p + 2] = color[3 * p + 2] + L.z * w;
+                locks[p].clear(std::memory_order_release);
+            }
+    }
+};
+
+struct SceneHandle {
+    or_scene_desc desc;  // points into the owned copies below
+    std::vector<float> pos, nrm, fn, uv, le, texd;
+    std::vector<uint32_t> mat, mats;
+    std::vector<int32_t> light, ltri, texi;
+    Scene<float> sf;
+    std::unique_ptr<Scene<double>> sd;
+    std::mutex mu;
+};
+
+template <class R> void fill_scene(Scene<R>& s, const or_scene_desc& d) {
+    build_tris(s, d);
+    s.mats.resize(d.n_mat);
+    for (int i = 0; i < d.n_mat; ++i) {
+        const uint32_t* w = d.mats + OR_MAT_WORDS * i;
+        Material& m = s.mats[i];
+        auto f = [&](int k) { float v; std::memcpy(&v, w + k, 4); return v; };
+        m.type = w[0];
+        m.flags = w[1];
+        for (int c = 0; c < 3; ++c) {
+            m.R[c] = f(2 + c); m.T[c] = f(5 + c); m.eta[c] = f(10 + c); m.k[c] = f(13 + c); m.f0[c] = f(16 + c);
+        }
+        m.ax = f(8); m.ay = f(9); m.eta1 = f(19); m.eta2 = f(20);
+        m.tex_albedo = (int32_t)w[21];
+        m.tex_rough = (int32_t)w[22];
+    }
+    s.light_tri.assign(d.light_tri, d.light_tri + d.n_light);
+    s.light_le.resize(d.n_light);
+    for (int i = 0; i < d.n_light; ++i) s.light_le[i] = V3<R>(d.light_le[3 * i], d.light_le[3 * i + 1], d.light_le[3 * i + 2]);
+    s.tex.resize(d.n_tex);
+    for (int i = 0; i < d.n_tex; ++i) {
+        Texture<R>& t = s.tex[i];
+        t.w = d.tex_info[4 * i]; t.h = d.tex_info[4 * i + 1]; t.c = d.tex_info[4 * i + 2];
+        const float* src = d.tex_data + d.tex_info[4 * i + 3];
+        t.data.assign(src, src + (size_t)t.w * t.h * t.c);
+    }
+    if (!s.light_tri.empty()) build_light_distribution(s);
+    build_bvh(s);
+}
+
+template <class R> Scene<R>& scene_of(SceneHandle* h);
+template <> Scene<float>& scene_of<float>(SceneHandle* h) { return h->sf; }
+template <> Scene<double>& scene_of<double>(SceneHandle* h) {
+    std::lock_guard<std::mutex> g(h->mu);
+    if (!h->sd) {
+        h->sd.reset(new Scene<double>());
+        fill_scene(*h->sd, h->desc);
+    }
+    return *h->sd;
+}
+
+template <class R> int li_samples_t(SceneHandle* h, const or_camera_desc* cd, const or_render_params* p, int32_t n,
+                                    const int32_t* x, const int32_t* y, const int32_t* s, double* out_L, float* out_raster,
+                                    int32_t* out_dims) {
+    Scene<R>& sc = scene_of<R>(h);
+    Camera<R> cam = make_camera<R>(*cd);
+    int res = round_up_pow2(std::max(cd->width, cd->height));
+    int m = log2_int(res);
+    int nth = p->nthreads > 0 ? p->nthreads : (int)std::thread::hardware_concurrency();
+    std::atomic<int32_t> next(0);
+    auto work = [&]() {
+        for (;;) {
+            int32_t i = next.fetch_add(64);
+            if (i >= n) break;
+            for (int32_t k = i; k < std::min(n, i + 64); ++k) {
+                R fx, fy;
+                int dims;
+                V3<R> L = camera_sample(sc, cam, res, m, x[k], y[k], s[k], p->max_depth, p->rr_depth, p->use_bvh != 0, &fx, &fy,
+                                        &dims);
+                out_L[3 * k] = L.x; out_L[3 * k + 1] = L.y; out_L[3 * k + 2] = L.z;
+                if (out_raster) { out_raster[2 * k] = (float)fx; out_raster[2 * k + 1] = (float)fy; }
+                if (out_dims) out_dims[k] = dims;
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 0; t < nth; ++t) th.emplace_back(work);
+    for (auto& t : th) t.join();
+    return 0;
+}
+
+template <class R> int64_t render_t(SceneHandle* h, const or_camera_desc* cd, const or_render_params* p, int32_t row0,
+                                    int32_t row1, double* color, double* weight, double* seconds) {
+    Scene<R>& sc = scene_of<R>(h);
+    Camera<R> cam = make_camera<R>(*cd);
+    int W = cd->width, H = cd->height;
+    if (row1 <= 0 || row1 > H) row1 = H;
+    if (row0 < 0) row0 = 0;
+    int res = round_up_pow2(std::max(W, H));
+    int m = log2_int(res);
+    Film<R> film(W, H);
+    // 32x32 tiles in row-major order (integrator/TiledIntegrator.cpp:13-25)
+    struct Tile { int x0, y0, x1, y1; };
+    std::vector<Tile> tiles;
+    for (int ty = 0; ty < H; ty += 32)
+        for (int tx = 0; tx < W; tx += 32) {
+            Tile t{tx, std::max(ty, row0), std::min(W, tx + 32), std::min(std::min(H, ty + 32), row1)};
+            if (t.y0 < t.y1) tiles.push_back(t);
+        }
+    int nth = p->nthreads > 0 ? p->nthreads : (int)std::thread::hardware_concurrency();
+    std::atomic<int> next(0);
+    std::atomic<int64_t> paths(0);
+    auto t0 = std::chrono::steady_clock::now();
+    auto work = [&]() {
+        int64_t local = 0;
+        for (;;) {
+            int ti = next.fetch_add(1);
+            if (ti >= (int)tiles.size()) break;
+            const Tile& t = tiles[ti];
+            for (int y = t.y0; y < t.y1; ++y)
+                for (int x = t.x0; x < t.x1; ++x)
+                    for (int smp = 0; smp < p->spp; ++smp) {
+                        R fx, fy;
+                        V3<R> L = camera_sample(sc, cam, res, m, x, y, smp, p->max_depth, p->rr_depth, p->use_bvh != 0, &fx, &fy,
+                                                (int*)nullptr);
+                        film.add(fx, fy, L);
+                        ++local;
+                    }
+        }
+        paths += local;
+    };
+    std::vector<std::thread> th;
+    for (int t = 0; t < nth; ++t) th.emplace_back(work);
+    for (auto& t : th) t.join();
+    auto t1 = std::chrono::steady_clock::now();
+    if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+    if (color)
+        for (size_t i = 0; i < film.color.size(); ++i) color[i] = film.color[i];
+    if (weight)
+        for (size_t i = 0; i < film.weight.size(); ++i) weight[i] = film.weight[i];
+    return paths.load();
+}
+
+}  // namespace orc
+
+using namespace orc;
+
+extern "C" {
+
+int or_load_sobol(const char* dir) {
+    std::string d(dir);
+    FILE* f = std::fopen((d + "/sobol_matrices.u32").c_str(), "rb");
+    if (!f) return -1;
+    std::fseek(f, 0, SEEK_END);
+    long sz = std::ftell(f);
+    std::fseek(f, 0, SEEK_SET);
+    g_sobol.resize(sz / 4);
+    size_t got = std::fread(g_sobol.data(), 4, g_sobol.size(), f);
+    std::fclose(f);
+    if (got != g_sobol.size()) return -2;
+    g_sobol_ndim = (uint32_t)(g_sobol.size() / 52);
+    f = std::fopen((d + "/sobol_enum_vdc.u64").c_str(), "rb");
+    if (!f) return -3;
+    got = std::fread(g_vdc, 8, 25 * 52, f);
+    std::fclose(f);
+    if (got != 25 * 52) return -4;
+    f = std::fopen((d + "/sobol_enum_vdc_inv.u64").c_str(), "rb");
+    if (!f) return -5;
+    got = std::fread(g_vdc_inv, 8, 26 * 52, f);
+    std::fclose(f);
+    if (got != 26 * 52) return -6;
+    g_sobol_ok = true;
+    return (int)g_sobol_ndim;
+}
+
+float or_sobol_sample(uint64_t index, uint32_t dim) { return sobol_sample(index, dim); }
+uint64_t or_sobol_enum_index(uint32_t m, uint32_t frame, uint32_t px, uint32_t py) { return enum_look_up(m, frame, px, py); }
+
+int or_camera_matrix(const or_camera_desc* cam, float* out) {
+    Camera<float> c = make_camera<float>(*cam);
+    for (int j = 0; j < 4; ++j)
+        for (int i = 0; i < 4; ++i) out[j * 4 + i] = c.raster_to_world.c[j].v[i];
+    return 0;
+}
+
+int or_camera_rays(const or_camera_desc* cam, int32_t n, const int32_t* px, const int32_t* py, const float* u,
+                   const float* v, float* org, float* dir) {
+    Camera<float> c = make_camera<float>(*cam);
+    for (int32_t i = 0; i < n; ++i) {
+        Ray<float> r = generate_ray(c, px[i], py[i], u[i], v[i]);
+        org[3 * i] = r.o.x; org[3 * i + 1] = r.o.y; org[3 * i + 2] = r.o.z;
+        dir[3 * i] = r.d.x; dir[3 * i + 1] = r.d.y; dir[3 * i + 2] = r.d.z;
+    }
+    return 0;
+}
+
+void* or_scene_create(const or_scene_desc* desc) {
+    SceneHandle* h = new SceneHandle();
+    const or_scene_desc& d = *desc;
+    int n = d.n_tri;
+    h->pos.assign(d.pos, d.pos + 9 * (size_t)n);
+    h->nrm.assign(d.nrm, d.nrm + 9 * (size_t)n);
+    h->fn.assign(d.face_n, d.face_n + 3 * (size_t)n);
+    if (d.uv) h->uv.assign(d.uv, d.uv + 6 * (size_t)n);
+    h->mat.assign(d.mat, d.mat + n);
+    if (d.light) h->light.assign(d.light, d.light + n);
+    h->mats.assign(d.mats, d.mats + OR_MAT_WORDS * (size_t)d.n_mat);
+    h->ltri.assign(d.light_tri, d.light_tri + d.n_light);
+    h->le.assign(d.light_le, d.light_le + 3 * (size_t)d.n_light);
+    size_t ntexf = 0;
+    for (int i = 0; i < d.n_tex; ++i)
+        ntexf = std::max(ntexf, (size_t)d.tex_info[4 * i + 3] + (size_t)d.tex_info[4 * i] * d.tex_info[4 * i + 1] * d.tex_info[4 * i + 2]);
+    if (d.n_tex) {
+        h->texi.assign(d.tex_info, d.tex_info + 4 * d.n_tex);
+        h->texd.assign(d.tex_data, d.tex_data + ntexf);
+    }
+    h->desc = d;
+    h->desc.pos = h->pos.data();
+    h->desc.nrm = h->nrm.data();
+    h->desc.face_n = h->fn.data();
+    h->desc.uv = d.uv ? h->uv.data() : nullptr;
+    h->desc.mat = h->mat.data();
+    h->desc.light = d.light ? h->light.data() : nullptr;
+    h->desc.mats = h->mats.data();
+    h->desc.light_tri = h->ltri.data();
+    h->desc.light_le = h->le.data();
+    h->desc.tex_info = d.n_tex ? h->texi.data() : nullptr;
+    h->desc.tex_data = d.n_tex ? h->texd.data() : nullptr;
+    fill_scene(h->sf, h->desc);
+    return h;
+}
+
+void or_scene_destroy(void* s) { delete (SceneHandle*)s; }
+
+int or_bvh_stats(void* s, int64_t* n_nodes, int32_t* max_depth) {
+    SceneHandle* h = (SceneHandle*)s;
+    *n_nodes = (int64_t)h->sf.nodes.size();
+    *max_depth = h->sf.max_depth;
+    return 0;
+}
+
+int or_trace_closest(void* s, int32_t n, const float* org, const float* dir, const float* tmin, const float* tmax,
+                     int use_bvh, float* out_t, int32_t* out_prim, float* out_b1, float* out_b2) {
+    SceneHandle* h = (SceneHandle*)s;
+    for (int32_t i = 0; i < n; ++i) {
+        Ray<float> r{V3<float>(org[3 * i], org[3 * i + 1], org[3 * i + 2]), V3<float>(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]),
+                     tmin[i], tmax[i]};
+        Hit<float> hit;
+        bool ok = intersect(h->sf, r, &hit, use_bvh != 0);
+        out_prim[i] = ok ? hit.prim : -1;
+        out_t[i] = ok ? hit.t : 0.f;
+        if (out_b1) out_b1[i] = ok ? hit.b1 : 0.f;
+        if (out_b2) out_b2[i] = ok ? hit.b2 : 0.f;
+    }
+    return 0;
+}
+
+int or_trace_any(void* s, int32_t n, const float* org, const float* dir, const float* tmin, const float* tmax, int use_bvh,
+                 int32_t* out_hit) {
+    SceneHandle* h = (SceneHandle*)s;
+    for (int32_t i = 0; i < n; ++i) {
+        Ray<float> r{V3<float>(org[3 * i], org[3 * i + 1], org[3 * i + 2]), V3<float>(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]),
+                     tmin[i], tmax[i]};
+        out_hit[i] = intersect_any(h->sf, r, use_bvh != 0) ? 1 : 0;
+    }
+    return 0;
+}
+
+int or_trace_counts(void* s, int32_t n, const float* org, const float* dir, const float* tmin, const float* tmax,
+                    int64_t* n_node, int64_t* n_tri) {
+    SceneHandle* h = (SceneHandle*)s;
+    *n_node = 0;
+    *n_tri = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        Ray<float> r{V3<float>(org[3 * i], org[3 * i + 1], org[3 * i + 2]), V3<float>(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]),
+                     tmin[i], tmax[i]};
+        Hit<float> hit;
+        intersect(h->sf, r, &hit, true, n_node, n_tri);
+    }
+    return 0;
+}
+
+int or_li_samples(void* s, const or_camera_desc* cam, const or_render_params* p, int32_t n, const int32_t* x, const int32_t* y,
+                  const int32_t* smp, double* out_L, float* out_raster, int32_t* out_dims) {
+    if (!g_sobol_ok) return -1;
+    SceneHandle* h = (SceneHandle*)s;
+    if (p->precision == 1) return li_samples_t<double>(h, cam, p, n, x, y, smp, out_L, out_raster, out_dims);
+    return li_samples_t<float>(h, cam, p, n, x, y, smp, out_L, out_raster, out_dims);
+}
+
+int or_set_shadow_short(int on) {
+    g_shadow_short = on;
+    return 0;
+}
+
+int64_t or_render(void* s, const or_camera_desc* cam, const or_render_params* p, int32_t row0, int32_t row1, double* color,
+                  double* weight, double* seconds) {
+    if (!g_sobol_ok) return -1;
+    SceneHandle* h = (SceneHandle*)s;
+    if (p->precision == 1) return render_t<double>(h, cam, p, row0, row1, color, weight, seconds);
+    return render_t<float>(h, cam, p, row0, row1, color, weight, seconds);
+}
+
+int or_mat4_inverse(const float* rows16, float* out_rows16) {
+    M4<float> m = M4<float>::rows(rows16[0], rows16[1], rows16[2], rows16[3], rows16[4], rows16[5], rows16[6], rows16[7],
+                                  rows16[8], rows16[9], rows16[10], rows16[11], rows16[12], rows16[13], rows16[14], rows16[15]);
+    M4<float> inv = glm_inverse(m);
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) out_rows16[i * 4 + j] = inv.c[j].v[i];
+    return 0;
+}
+
+int or_mat4_mul_vec(const float* rows16, const float* v4, float* out4) {
+    M4<float> m = M4<float>::rows(rows16[0], rows16[1], rows16[2], rows16[3], rows16[4], rows16[5], rows16[6], rows16[7],
+                                  rows16[8], rows16[9], rows16[10], rows16[11], rows16[12], rows16[13], rows16[14], rows16[15]);
+    V4<float> a{{v4[0], v4[1], v4[2], v4[3]}};
+    V4<float> r = mul(m, a);
+    for (int i = 0; i < 4; ++i) out4[i] = r.v[i];
+    return 0;
+}
+
+// BSDF of one material record in the local shading frame (X, Y, Z axes, ng = Z).
+static BSDF<double> local_bsdf(const uint32_t* w) {
+    Material m;
+    auto f = [&](int k) { float v; std::memcpy(&v, w + k, 4); return v; };
+    BSDF<double> b;
+    b.type = (int)w[0];
+    b.ng = V3<double>(0, 0, 1);
+    b.X = V3<double>(1, 0, 0);
+    b.Y = V3<double>(0, 1, 0);
+    b.Z = V3<double>(0, 0, 1);
+    b.Rc = V3<double>(f(2), f(3), f(4));
+    b.Tc = V3<double>(f(5), f(6), f(7));
+    double ax = f(8), ay = f(9);
+    if (!(w[1] & 1u)) { ax = roughness_to_alpha(ax); ay = roughness_to_alpha(ay); }
+    b.dist.ax = ax;
+    b.dist.ay = ay;
+    b.fresnel_schlick = (w[1] & 2u) ? 1 : 0;
+    b.eta = V3<double>(f(10), f(11), f(12));
+    b.k = V3<double>(f(13), f(14), f(15));
+    b.f0 = V3<double>(f(16), f(17), f(18));
+    b.eta1 = f(19);
+    b.eta2 = f(20);
+    (void)m;
+    return b;
+}
+
+// Evaluate f (3) and pdf for n direction pairs (wo, wi as 3n doubles each), in double.
+int or_bsdf_eval(const uint32_t* mat_words, int32_t n, const double* wo, const double* wi, double* f, double* pdf) {
+    BSDF<double> b = local_bsdf(mat_words);
+    for (int32_t i = 0; i < n; ++i) {
+        V3<double> o(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]), d(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]);
+        V3<double> v = b.Evaluate(o, d);
+        f[3 * i] = v.x; f[3 * i + 1] = v.y; f[3 * i + 2] = v.z;
+        pdf[i] = b.Pdf(o, d);
+    }
+    return 0;
+}
+
+// Sample n directions for wo with random numbers u (3n); outputs wi (3n), f (3n), pdf (n).
+int or_bsdf_sample(const uint32_t* mat_words, int32_t n, const double* wo, const double* u, double* wi, double* f, double* pdf) {
+    BSDF<double> b = local_bsdf(mat_words);
+    for (int32_t i = 0; i < n; ++i) {
+        V3<double> o(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]), w;
+        double p;
+        V3<double> v = b.Sample(o, &w, &p, V3<double>(u[3 * i], u[3 * i + 1], u[3 * i + 2]));
+        wi[3 * i] = w.x; wi[3 * i + 1] = w.y; wi[3 * i + 2] = w.z;
+        f[3 * i] = v.x; f[3 * i + 1] = v.y; f[3 * i + 2] = v.z;
+        pdf[i] = p;
+    }
+    return 0;
+}
+
+double or_fr_dielectric(double cosI, double etaI, double etaT) { return fr_dielectric(cosI, etaI, etaT); }
+int or_fr_conductor(double cosI, const double* eta, const double* k, double* out) {
+    V3<double> r = fr_conductor(cosI, V3<double>(1, 1, 1), V3<double>(eta[0], eta[1], eta[2]), V3<double>(k[0], k[1], k[2]));
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+    return 0;
+}
+double or_ggx_D(double ax, double ay, const double* m) {
+    GGX<double> g{ax, ay};
+    return g.D(V3<double>(m[0], m[1], m[2]));
+}
+
+}  // extern "C"

@@ -1,0 +1,134 @@
+/* CPU oracle for the Gyunity path-tracing hot path: TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain C++ restatement of the reference CPU integrator (PathTracing over
+ * TiledIntegrator, Sobol sampling with per-pixel enumeration, the pinhole
+ * camera, the 12-bucket SAH BVH and brute-force closest/any hit, the
+ * Lambert / GGX conductor / smooth and rough dielectric BSDFs, NEE + MIS to
+ * triangle area lights, and box-filter film accumulation).  Each function in
+ * gy_oracle.cpp cites the reference file:line it follows.
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load this library, and only as the checker or the timed CPU baseline; the
+ * product path never calls it.
+ *
+ * The oracle runs in two precisions:
+ *   OR_FP32  bit-faithful float mode (SSE-order dot products, no FMA,
+ *            IEEE divide/sqrt; compile with -ffp-contract=off)
+ *   OR_FP64  the same algorithm in double, used for the converged reference
+ */
+#ifndef GY_ORACLE_H
+#define GY_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Material record: 32 four-byte words (the same layout the library's
+ * gy_material uses).  Words:
+ *   0 type (0 Lambert, 1 GGX conductor, 2 smooth dielectric, 3 rough dielectric)
+ *   1 flags (bit0: alpha given directly, else RoughnessToAlpha(roughness);
+ *            bit1: Schlick F0 Fresnel for the conductor, else FrConductor(eta,k))
+ *   2-4 R (Lambert kd / conductor reflectance / dielectric Kr)
+ *   5-7 T (dielectric Kt)
+ *   8-9 alpha_x, alpha_y (or roughness_x, roughness_y when flag bit0 is clear)
+ *   10-12 conductor eta, 13-15 conductor k, 16-18 Schlick F0
+ *   19 dielectric eta1 (outside), 20 eta2 (inside)
+ *   21 albedo texture id (int32, -1 none), 22 roughness texture id (-1 none)
+ */
+#define OR_MAT_WORDS 32
+
+typedef struct {
+    int32_t n_tri;
+    const float* pos;      /* n_tri*9: p0,p1,p2 */
+    const float* nrm;      /* n_tri*9: n0,n1,n2 (vertex normals) */
+    const float* face_n;   /* n_tri*3: face normal as given to Triangle() */
+    const float* uv;       /* n_tri*6 or NULL: per-vertex uv (NULL = all zero) */
+    const uint32_t* mat;   /* n_tri material ids */
+    const int32_t* light;  /* n_tri light ids (-1 = not emissive) */
+    int32_t n_mat;
+    const uint32_t* mats;  /* n_mat*32 words */
+    int32_t n_light;
+    const int32_t* light_tri; /* n_light */
+    const float* light_le;    /* n_light*3 */
+    int32_t n_tex;
+    const int32_t* tex_info;  /* n_tex*4: width, height, channels, float offset */
+    const float* tex_data;
+} or_scene_desc;
+
+typedef struct {
+    float pos[3], look_at[3], up[3];
+    float fovy;        /* degrees */
+    float film_dist;   /* 1 in every reference scene */
+    int32_t width, height;
+} or_camera_desc;
+
+typedef struct {
+    int32_t spp;
+    int32_t max_depth;
+    int32_t rr_depth;    /* Russian roulette when bounce i > rr_depth (reference: 5) */
+    int32_t precision;   /* 0 = FP32 bit-faithful, 1 = FP64 */
+    int32_t nthreads;    /* tile worker threads (0 = hardware concurrency) */
+    int32_t use_bvh;     /* 1 = reference BVH traversal, 0 = brute force */
+} or_render_params;
+
+/* Sobol tables (fixtures extracted from the reference's data tables). */
+int or_load_sobol(const char* golden_dir);
+float or_sobol_sample(uint64_t index, uint32_t dim);
+uint64_t or_sobol_enum_index(uint32_t m, uint32_t frame, uint32_t px, uint32_t py);
+
+/* Camera: RasterToWorld (column-major 4x4, 16 floats, element (i,j) at [j*4+i])
+ * and GenerateRay for raster sample positions. */
+int or_camera_matrix(const or_camera_desc* cam, float* raster_to_world16);
+int or_camera_rays(const or_camera_desc* cam, int32_t n, const int32_t* px, const int32_t* py,
+                   const float* u, const float* v, float* org, float* dir);
+
+void* or_scene_create(const or_scene_desc* desc);
+void or_scene_destroy(void* scene);
+/* Reference BVH statistics: node count, max depth. */
+int or_bvh_stats(void* scene, int64_t* n_nodes, int32_t* max_depth);
+
+/* Closest / any hit for n rays given as org (3n), dir (3n), tmin (n), tmax (n).
+ * Closest: out_t (n), out_prim (n, -1 miss), out_b1, out_b2.  use_bvh selects
+ * the reference BVH traversal (1) or brute force in scene order (0). */
+int or_trace_closest(void* scene, int32_t n, const float* org, const float* dir, const float* tmin,
+                     const float* tmax, int use_bvh, float* out_t, int32_t* out_prim, float* out_b1,
+                     float* out_b2);
+int or_trace_any(void* scene, int32_t n, const float* org, const float* dir, const float* tmin,
+                 const float* tmax, int use_bvh, int32_t* out_hit);
+/* Count BVH nodes and triangles visited by the reference closest-hit traversal. */
+int or_trace_counts(void* scene, int32_t n, const float* org, const float* dir, const float* tmin,
+                    const float* tmax, int64_t* n_node, int64_t* n_tri);
+
+/* Per-sample radiance Li for n (x, y, s) triples (the path a tile worker
+ * would trace for sample s of pixel (x, y)).  out_L: 3n doubles.
+ * out_raster: 2n floats, the film position (x+u, y+v) of each sample.
+ * out_dims: n ints, the number of sample dimensions the path consumed. */
+int or_li_samples(void* scene, const or_camera_desc* cam, const or_render_params* p, int32_t n,
+                  const int32_t* x, const int32_t* y, const int32_t* s, double* out_L,
+                  float* out_raster, int32_t* out_dims);
+
+/* TiledIntegrator::Render into a film: color (W*H*3) and weight (W*H) sums.
+ * rows [row0, row1) only (row1 <= 0 means all rows) so a bounded sample of the
+ * image can be timed.  Returns paths traced; *seconds gets wall time. */
+int64_t or_render(void* scene, const or_camera_desc* cam, const or_render_params* p, int32_t row0,
+                  int32_t row1, double* color, double* weight, double* seconds);
+
+/* Test pins: 4x4 GLM inverse and SSE-order mat*vec (row-major in/out),
+ * BSDF evaluation / sampling in the local frame (double), Fresnel terms and
+ * the GGX distribution. */
+int or_mat4_inverse(const float* rows16, float* out_rows16);
+int or_mat4_mul_vec(const float* rows16, const float* v4, float* out4);
+int or_bsdf_eval(const uint32_t* mat_words, int32_t n, const double* wo, const double* wi, double* f, double* pdf);
+int or_bsdf_sample(const uint32_t* mat_words, int32_t n, const double* wo, const double* u, double* wi, double* f, double* pdf);
+double or_fr_dielectric(double cosI, double etaI, double etaT);
+int or_fr_conductor(double cosI, const double* eta, const double* k, double* out);
+double or_ggx_D(double ax, double ay, const double* m);
+/* Test-only switch: end NEE shadow segments ShadowRayEps before the light point
+ * (the reference's segment reaches it, see spawn_to in gy_oracle.cpp). */
+int or_set_shadow_short(int on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,293 @@
+"""ctypes wrapper of the CPU oracle (oracle/libgy_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg load it, as the checker or the timed CPU baseline.  The
+product path (gyunity_amd) never imports this module.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgy_oracle.so")
+GOLDEN = os.path.join(os.path.dirname(HERE), "tests", "golden")
+
+
+class ORSceneDesc(C.Structure):
+    _fields_ = [
+        ("n_tri", C.c_int32),
+        ("pos", C.c_void_p),
+        ("nrm", C.c_void_p),
+        ("face_n", C.c_void_p),
+        ("uv", C.c_void_p),
+        ("mat", C.c_void_p),
+        ("light", C.c_void_p),
+        ("n_mat", C.c_int32),
+        ("mats", C.c_void_p),
+        ("n_light", C.c_int32),
+        ("light_tri", C.c_void_p),
+        ("light_le", C.c_void_p),
+        ("n_tex", C.c_int32),
+        ("tex_info", C.c_void_p),
+        ("tex_data", C.c_void_p),
+    ]
+
+
+class ORCamera(C.Structure):
+    _fields_ = [
+        ("pos", C.c_float * 3),
+        ("look_at", C.c_float * 3),
+        ("up", C.c_float * 3),
+        ("fovy", C.c_float),
+        ("film_dist", C.c_float),
+        ("width", C.c_int32),
+        ("height", C.c_int32),
+    ]
+
+
+class ORParams(C.Structure):
+    _fields_ = [
+        ("spp", C.c_int32),
+        ("max_depth", C.c_int32),
+        ("rr_depth", C.c_int32),
+        ("precision", C.c_int32),
+        ("nthreads", C.c_int32),
+        ("use_bvh", C.c_int32),
+    ]
+
+
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    L = C.CDLL(LIB_PATH)
+    vp, i32, i64, u32, u64, f32 = C.c_void_p, C.c_int32, C.c_int64, C.c_uint32, C.c_uint64, C.c_float
+    sig = {
+        "or_load_sobol": (C.c_int, [C.c_char_p]),
+        "or_sobol_sample": (f32, [u64, u32]),
+        "or_sobol_enum_index": (u64, [u32, u32, u32, u32]),
+        "or_camera_matrix": (C.c_int, [vp, vp]),
+        "or_camera_rays": (C.c_int, [vp, i32, vp, vp, vp, vp, vp, vp]),
+        "or_scene_create": (vp, [vp]),
+        "or_scene_destroy": (None, [vp]),
+        "or_bvh_stats": (C.c_int, [vp, vp, vp]),
+        "or_trace_closest": (C.c_int, [vp, i32, vp, vp, vp, vp, C.c_int, vp, vp, vp, vp]),
+        "or_trace_any": (C.c_int, [vp, i32, vp, vp, vp, vp, C.c_int, vp]),
+        "or_trace_counts": (C.c_int, [vp, i32, vp, vp, vp, vp, vp, vp]),
+        "or_li_samples": (C.c_int, [vp, vp, vp, i32, vp, vp, vp, vp, vp, vp]),
+        "or_render": (i64, [vp, vp, vp, i32, i32, vp, vp, vp]),
+        "or_mat4_inverse": (C.c_int, [vp, vp]),
+        "or_mat4_mul_vec": (C.c_int, [vp, vp, vp]),
+        "or_bsdf_eval": (C.c_int, [vp, i32, vp, vp, vp, vp]),
+        "or_bsdf_sample": (C.c_int, [vp, i32, vp, vp, vp, vp, vp]),
+        "or_fr_dielectric": (C.c_double, [C.c_double, C.c_double, C.c_double]),
+        "or_fr_conductor": (C.c_int, [C.c_double, vp, vp, vp]),
+        "or_ggx_D": (C.c_double, [C.c_double, C.c_double, vp]),
+        "or_set_shadow_short": (C.c_int, [C.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    rc = L.or_load_sobol(GOLDEN.encode())
+    if rc <= 0:
+        raise RuntimeError("oracle: cannot load Sobol fixtures from %s (rc %d)" % (GOLDEN, rc))
+    _lib = L
+    return L
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def camera_from(cam):
+    """Copy a gyunity_amd CameraDesc (same layout) into the oracle's struct."""
+    o = ORCamera()
+    o.pos[:], o.look_at[:], o.up[:] = list(cam.pos), list(cam.look_at), list(cam.up)
+    o.fovy, o.film_dist, o.width, o.height = cam.fovy, cam.film_dist, cam.width, cam.height
+    return o
+
+
+def params(spp, max_depth, rr_depth, precision=0, nthreads=0, use_bvh=1):
+    p = ORParams()
+    p.spp, p.max_depth, p.rr_depth, p.precision, p.nthreads, p.use_bvh = spp, max_depth, rr_depth, precision, nthreads, use_bvh
+    return p
+
+
+def sobol_sample(index, dim):
+    return lib().or_sobol_sample(index, dim)
+
+
+def sobol_enum_index(m, frame, px, py):
+    return lib().or_sobol_enum_index(m, frame, px, py)
+
+
+def camera_matrix(cam):
+    out = np.zeros(16, dtype=np.float32)
+    c = camera_from(cam)
+    lib().or_camera_matrix(C.byref(c), out.ctypes.data)
+    return out
+
+
+def camera_rays(cam, px, py, u, v):
+    c = camera_from(cam)
+    px = np.ascontiguousarray(px, np.int32)
+    py = np.ascontiguousarray(py, np.int32)
+    u = np.ascontiguousarray(u, np.float32)
+    v = np.ascontiguousarray(v, np.float32)
+    n = px.shape[0]
+    org = np.zeros((n, 3), np.float32)
+    d = np.zeros((n, 3), np.float32)
+    lib().or_camera_rays(C.byref(c), n, _p(px), _p(py), _p(u), _p(v), _p(org), _p(d))
+    return org, d
+
+
+class OracleScene:
+    def __init__(self, arrays):
+        self.arrays = arrays
+        d = ORSceneDesc()
+        a = arrays
+        d.n_tri = a.n_tri
+        d.pos, d.nrm, d.face_n = _p(a.pos), _p(a.nrm), _p(a.face_n)
+        d.uv = _p(a.uv)
+        d.mat, d.light = _p(a.mat), _p(a.light)
+        d.n_mat, d.mats = a.mats.shape[0], _p(a.mats)
+        d.n_light, d.light_tri, d.light_le = a.light_tri.shape[0], _p(a.light_tri), _p(a.light_le)
+        if a.tex_info is not None:
+            d.n_tex, d.tex_info, d.tex_data = a.tex_info.shape[0], _p(a.tex_info), _p(a.tex_data)
+        self.h = lib().or_scene_create(C.byref(d))
+
+    def close(self):
+        if self.h:
+            lib().or_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def bvh_stats(self):
+        n, dmax = C.c_int64(), C.c_int32()
+        lib().or_bvh_stats(self.h, C.byref(n), C.byref(dmax))
+        return n.value, dmax.value
+
+    def _rays(self, rays):
+        org = np.ascontiguousarray(rays["org"], np.float32)
+        d = np.ascontiguousarray(rays["dir"], np.float32)
+        tmin = np.ascontiguousarray(rays["tmin"], np.float32)
+        tmax = np.ascontiguousarray(rays["tmax"], np.float32)
+        return org, d, tmin, tmax
+
+    def trace_closest(self, rays, use_bvh=True):
+        org, d, tmin, tmax = self._rays(rays)
+        n = org.shape[0]
+        t = np.zeros(n, np.float32)
+        prim = np.zeros(n, np.int32)
+        b1 = np.zeros(n, np.float32)
+        b2 = np.zeros(n, np.float32)
+        lib().or_trace_closest(self.h, n, _p(org), _p(d), _p(tmin), _p(tmax), int(use_bvh), _p(t), _p(prim), _p(b1), _p(b2))
+        return t, prim, b1, b2
+
+    def trace_any(self, rays, use_bvh=True):
+        org, d, tmin, tmax = self._rays(rays)
+        n = org.shape[0]
+        occ = np.zeros(n, np.int32)
+        lib().or_trace_any(self.h, n, _p(org), _p(d), _p(tmin), _p(tmax), int(use_bvh), _p(occ))
+        return occ
+
+    def trace_counts(self, rays):
+        org, d, tmin, tmax = self._rays(rays)
+        a, b = C.c_int64(), C.c_int64()
+        lib().or_trace_counts(self.h, org.shape[0], _p(org), _p(d), _p(tmin), _p(tmax), C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def li_samples(self, cam, prm, x, y, s):
+        c = camera_from(cam)
+        x = np.ascontiguousarray(x, np.int32)
+        y = np.ascontiguousarray(y, np.int32)
+        s = np.ascontiguousarray(s, np.int32)
+        n = x.shape[0]
+        L = np.zeros((n, 3), np.float64)
+        raster = np.zeros((n, 2), np.float32)
+        dims = np.zeros(n, np.int32)
+        rc = lib().or_li_samples(self.h, C.byref(c), C.byref(prm), n, _p(x), _p(y), _p(s), _p(L), _p(raster), _p(dims))
+        assert rc == 0
+        return L, raster, dims
+
+    def render(self, cam, prm, row0=0, row1=0):
+        c = camera_from(cam)
+        color = np.zeros((cam.height, cam.width, 3), np.float64)
+        weight = np.zeros((cam.height, cam.width), np.float64)
+        secs = C.c_double()
+        paths = lib().or_render(self.h, C.byref(c), C.byref(prm), row0, row1, _p(color), _p(weight), C.byref(secs))
+        return color, weight, paths, secs.value
+
+
+def mat4_inverse(rows):
+    rows = np.ascontiguousarray(rows, np.float32).reshape(16)
+    out = np.zeros(16, np.float32)
+    lib().or_mat4_inverse(rows.ctypes.data, out.ctypes.data)
+    return out.reshape(4, 4)
+
+
+def mat4_mul_vec(rows, v):
+    rows = np.ascontiguousarray(rows, np.float32).reshape(16)
+    v = np.ascontiguousarray(v, np.float32)
+    out = np.zeros(4, np.float32)
+    lib().or_mat4_mul_vec(rows.ctypes.data, v.ctypes.data, out.ctypes.data)
+    return out
+
+
+def bsdf_eval(mat_words, wo, wi):
+    mat_words = np.ascontiguousarray(mat_words, np.uint32)
+    wo = np.ascontiguousarray(wo, np.float64).reshape(-1, 3)
+    wi = np.ascontiguousarray(wi, np.float64).reshape(-1, 3)
+    n = wo.shape[0]
+    f = np.zeros((n, 3))
+    pdf = np.zeros(n)
+    lib().or_bsdf_eval(mat_words.ctypes.data, n, wo.ctypes.data, wi.ctypes.data, f.ctypes.data, pdf.ctypes.data)
+    return f, pdf
+
+
+def bsdf_sample(mat_words, wo, u):
+    mat_words = np.ascontiguousarray(mat_words, np.uint32)
+    wo = np.ascontiguousarray(wo, np.float64).reshape(-1, 3)
+    u = np.ascontiguousarray(u, np.float64).reshape(-1, 3)
+    n = wo.shape[0]
+    wi = np.zeros((n, 3))
+    f = np.zeros((n, 3))
+    pdf = np.zeros(n)
+    lib().or_bsdf_sample(mat_words.ctypes.data, n, wo.ctypes.data, u.ctypes.data, wi.ctypes.data, f.ctypes.data, pdf.ctypes.data)
+    return wi, f, pdf
+
+
+def fr_dielectric(c, eta_i, eta_t):
+    return lib().or_fr_dielectric(c, eta_i, eta_t)
+
+
+def fr_conductor(c, eta, k):
+    eta = np.ascontiguousarray(eta, np.float64)
+    k = np.ascontiguousarray(k, np.float64)
+    out = np.zeros(3)
+    lib().or_fr_conductor(c, eta.ctypes.data, k.ctypes.data, out.ctypes.data)
+    return out
+
+
+def ggx_D(ax, ay, m):
+    m = np.ascontiguousarray(m, np.float64)
+    return lib().or_ggx_D(ax, ay, m.ctypes.data)
+
+
+def set_shadow_short(on):
+    """Test-only: end NEE shadow segments ShadowRayEps before the light point."""
+    lib().or_set_shadow_short(1 if on else 0)

@@ -1,0 +1,95 @@
+"""Small hand-built scenes and ray batches shared by the tests."""
+import struct
+
+import numpy as np
+
+from gyunity_amd import RAY_DTYPE, SceneArrays
+
+MAT_WORDS = 32
+
+
+def f2w(x):
+    return struct.unpack("<I", struct.pack("<f", float(x)))[0]
+
+
+def material(mtype, R=(1, 1, 1), T=(1, 1, 1), alpha=(0.2, 0.2), alpha_direct=True, eta=(1, 1, 1), k=(0, 0, 0),
+             f0=None, eta1=1.0, eta2=1.5, tex_albedo=-1, tex_rough=-1):
+    w = np.zeros(MAT_WORDS, np.uint32)
+    w[0] = mtype
+    w[1] = (1 if alpha_direct else 0) | (2 if f0 is not None else 0)
+    for i in range(3):
+        w[2 + i] = f2w(R[i])
+        w[5 + i] = f2w(T[i])
+        w[10 + i] = f2w(eta[i])
+        w[13 + i] = f2w(k[i])
+        w[16 + i] = f2w(0 if f0 is None else f0[i])
+    w[8], w[9] = f2w(alpha[0]), f2w(alpha[1])
+    w[19], w[20] = f2w(eta1), f2w(eta2)
+    w[21] = np.uint32(tex_albedo & 0xFFFFFFFF)
+    w[22] = np.uint32(tex_rough & 0xFFFFFFFF)
+    return w
+
+
+def box_triangles(lo, hi, inward):
+    """12 triangles of an axis-aligned box with face normals (inward or outward)."""
+    x0, y0, z0 = lo
+    x1, y1, z1 = hi
+    quads = [
+        ((x0, y0, z0), (x1, y0, z0), (x1, y0, z1), (x0, y0, z1), (0, -1, 0)),
+        ((x0, y1, z0), (x0, y1, z1), (x1, y1, z1), (x1, y1, z0), (0, 1, 0)),
+        ((x0, y0, z0), (x0, y0, z1), (x0, y1, z1), (x0, y1, z0), (-1, 0, 0)),
+        ((x1, y0, z0), (x1, y1, z0), (x1, y1, z1), (x1, y0, z1), (1, 0, 0)),
+        ((x0, y0, z0), (x0, y1, z0), (x1, y1, z0), (x1, y0, z0), (0, 0, -1)),
+        ((x0, y0, z1), (x1, y0, z1), (x1, y1, z1), (x0, y1, z1), (0, 0, 1)),
+    ]
+    tris, normals = [], []
+    for a, b, c, d, n in quads:
+        n = tuple(-v for v in n) if inward else n
+        tris += [(a, b, c), (a, c, d)]
+        normals += [n, n]
+    return np.array(tris, np.float32).reshape(-1, 9), np.array(normals, np.float32)
+
+
+def furnace_scene(albedo=0.5, le=1.0):
+    """Closed unit box; every triangle is an inward-facing emitter with a Lambert
+    material of the given albedo."""
+    pos, fn = box_triangles((0, 0, 0), (1, 1, 1), inward=True)
+    n = pos.shape[0]
+    nrm = np.repeat(fn, 3, axis=0).reshape(-1, 9)
+    mats = material(0, R=(albedo,) * 3)[None, :]
+    return SceneArrays(pos, nrm, fn, np.zeros(n, np.uint32), np.arange(n, dtype=np.int32), mats,
+                       np.arange(n, dtype=np.int32), np.full((n, 3), le, np.float32))
+
+
+def furnace_expected(albedo, depth):
+    """E[L] of PathTracing::Li in the furnace box: Le at the first hit (emission
+    branch, no NEE there), then NEE + MIS estimates albedo*Le at every later
+    vertex k >= 1, weighted by the throughput albedo^k."""
+    return 1.0 + sum(albedo ** k for k in range(2, depth + 1))
+
+
+def random_rays(n, seed, lo=(-0.2, -0.2, -0.2), hi=(1.2, 1.2, 1.2), tmax=np.inf):
+    rng = np.random.default_rng(seed)
+    rays = np.zeros(n, RAY_DTYPE)
+    rays["org"] = rng.uniform(lo, hi, size=(n, 3)).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays["dir"] = d.astype(np.float32)
+    rays["tmin"] = 0.0
+    rays["tmax"] = tmax
+    return rays
+
+
+def camera_rays(cam, n, seed):
+    """Camera rays exactly as the oracle's GenerateRay forms them."""
+    from oracle import oracle as O
+
+    rng = np.random.default_rng(seed)
+    px = rng.integers(0, cam.width, n).astype(np.int32)
+    py = rng.integers(0, cam.height, n).astype(np.int32)
+    u = rng.random(n).astype(np.float32)
+    v = rng.random(n).astype(np.float32)
+    org, d = O.camera_rays(cam, px, py, u, v)
+    rays = np.zeros(n, RAY_DTYPE)
+    rays["org"], rays["dir"], rays["tmin"], rays["tmax"] = org, d, 0.0, np.inf
+    return rays

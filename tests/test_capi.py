@@ -1,0 +1,79 @@
+"""C ABI checks that need no GPU: the library loads, exports every entry point
+include/gyunity.h declares, and fails cleanly where a device is required."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import gyunity_amd as g
+from gyunity_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "gyunity.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(gy_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    names = declared_functions()
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_struct_sizes_match_header_layout():
+    assert ctypes.sizeof(_lib.CameraDesc) == 52
+    assert ctypes.sizeof(_lib.RenderParams) == 32
+    assert g.RAY_DTYPE.itemsize == 32 and g.HIT_DTYPE.itemsize == 16
+
+
+def test_version_and_kernel_names():
+    assert b"gfx950" in _lib.lib().gy_version()
+    assert _lib.lib().gy_kernel_name(0) == b"pt_megakernel"
+    assert _lib.lib().gy_kernel_name(99) == b""
+
+
+def test_open_without_device_fails_cleanly():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(g.GyError) as e:
+        g.Context(0)
+    assert e.value.code == 2 and "device" in str(e.value)
+
+
+def test_null_arguments_are_rejected():
+    L = _lib.lib()
+    assert L.gy_scene_create(None, None, None, None) == 1
+    assert b"null" in L.gy_last_error()
+    assert L.gy_render(None, None, None, None, None) == 1
+    assert L.gy_generate_scene(99, 0, 0, ctypes.byref(ctypes.c_void_p())) == 1
+
+
+def test_generated_scenes_are_well_formed():
+    sa, cam, prm = g.generate_scene(g.GY_SCENE_CORNELL)
+    assert sa.n_tri == 36 and list(sa.light_tri) == [34, 35]
+    assert (cam.width, cam.height, prm.spp, prm.max_depth, prm.rr_depth) == (256, 256, 16, 5, 3)
+    assert np.all(sa.light[sa.light_tri] == [0, 1])
+    sa, cam, prm = g.generate_scene(g.GY_SCENE_ICOSPHERE_GGX, 0, 8)
+    assert sa.n_tri == 1310720 + 12
+    assert (cam.width, prm.spp, prm.max_depth) == (1024, 64, 8)
+    # flat normals, unit length, outward from the sphere centre
+    fn = sa.face_n[12:]
+    assert np.allclose(np.linalg.norm(fn, axis=1), 1.0, atol=1e-6)
+    cen = sa.pos[12:].reshape(-1, 3, 3).mean(axis=1) - np.array([0.5, 0.4, 0.5])
+    assert np.all((fn * cen).sum(1) > 0)
+    r = np.linalg.norm(sa.pos[12:].reshape(-1, 3) - np.array([0.5, 0.4, 0.5]), axis=1)
+    assert r.min() > 0.25 - 1e-6 and r.max() < 0.35 + 1e-6
+    # deterministic in the seed
+    sb, _, _ = g.generate_scene(g.GY_SCENE_ICOSPHERE_GGX, 0, 3)
+    sc, _, _ = g.generate_scene(g.GY_SCENE_ICOSPHERE_GGX, 0, 3)
+    sd, _, _ = g.generate_scene(g.GY_SCENE_ICOSPHERE_GGX, 1, 3)
+    assert np.array_equal(sb.pos, sc.pos) and not np.array_equal(sb.pos, sd.pos)
