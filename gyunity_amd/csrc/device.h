@@ -58,6 +58,12 @@ GY_D float sgnf(float v) { return (v > 0.f) ? 1.f : -1.f; }     // math/MathUtil
 GY_D float clampf(float v, float lo, float hi) { return v < lo ? lo : (v > hi ? hi : v); }
 GY_D float gamma_n(int n) { return (n * kMachineEps) / (1 - n * kMachineEps); }  // numeric/NumericUtils.h:49-51
 GY_D V3 ld3(const float4 q) { return V3{q.x, q.y, q.z}; }
+// cos / sin / log evaluated in double and rounded once to float, which the
+// oracle does too (the reference's CRT float versions are implementation-defined
+// in the last ulp).
+GY_D float dcos(float x) { return (float)cos((double)x); }
+GY_D float dsin(float x) { return (float)sin((double)x); }
+GY_D float dlog(float x) { return (float)log((double)x); }
 
 // ------------------------------------------------------------------ Sobol
 // sobol::sample (sampler/Sobol.h:46-61) + RandomStateSequence (sampler/Sampler.h:53-64)
@@ -399,8 +405,8 @@ GY_D V3 cosine_hemisphere(float u0, float u1) {
             r = oy;
             theta = kPIOver2 - kPIOver4 * (ox / oy);
         }
-        dx = r * cosf(theta);
-        dy = r * sinf(theta);
+        dx = r * dcos(theta);
+        dy = r * dsin(theta);
     }
     const float z = sqrtf(rmax(0.f, 1 - dx * dx - dy * dy));
     return v3(dx, dy, z);
@@ -465,8 +471,8 @@ GY_D V3 ggx_sample_wh(const GGX& g, V3 v_, float u0, float u1) {
     const float a = 1.f / (1.f + v.z);
     const float r = sqrtf(u0);
     const float phi = (u1 < a) ? u1 / a * kPI : kPI + (u1 - a) / (1.f - a) * kPI;
-    const float P1 = r * cosf(phi);
-    const float P2 = r * sinf(phi) * ((u1 < a) ? 1.0f : v.z);
+    const float P1 = r * dcos(phi);
+    const float P2 = r * dsin(phi) * ((u1 < a) ? 1.0f : v.z);
     const V3 dir = (P1 * T1 + P2 * T2) + sqrtf(rmax(0.f, 1 - P1 * P1 - P2 * P2)) * v;
     return norm(v3(g.ax * dir.x, g.ay * dir.y, rmax(0.f, dir.z)));
 }
@@ -525,7 +531,7 @@ GY_D bool refract(V3 wi, V3 n, float eta, V3* wt) {
 // visual/Microfacet.h:49-54
 GY_D float roughness_to_alpha(float r) {
     r = rmax(r, 1e-3f);
-    const float x = logf(r);
+    const float x = dlog(r);
     return 1.62142f + 0.819955f * x + 0.1734f * x * x + 0.0171201f * x * x * x + 0.000640711f * x * x * x * x;
 }
 

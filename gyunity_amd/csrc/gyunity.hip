@@ -459,6 +459,7 @@ __global__ void k_side_apply(FilmDev F) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t tgt = (uint32_t)(F.side_sorted[i].key >> 32);
+    if (tgt >= (uint32_t)F.width * (uint32_t)F.height) return;
     if (i > 0 && (uint32_t)(F.side_sorted[i - 1].key >> 32) == tgt) return;
     float4 acc = F.film[tgt];
     for (uint32_t j = i; j < n && (uint32_t)(F.side_sorted[j].key >> 32) == tgt; ++j) {

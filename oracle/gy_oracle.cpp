@@ -40,6 +40,17 @@ template <class R> struct K {
     static constexpr R Inf = std::numeric_limits<R>::infinity();
 };
 
+// Transcendentals: the reference calls the CRT's float cos/sin/log, whose
+// last-ulp behaviour is implementation-defined.  Float mode evaluates them in
+// double and rounds once (correctly rounded but for ~2^-29 double-rounding
+// cases), which the device code does too; double mode uses double libm.
+inline float rcos(float x) { return (float)std::cos((double)x); }
+inline float rsin(float x) { return (float)std::sin((double)x); }
+inline float rlog(float x) { return (float)std::log((double)x); }
+inline double rcos(double x) { return std::cos(x); }
+inline double rsin(double x) { return std::sin(x); }
+inline double rlog(double x) { return std::log(x); }
+
 // ---------------------------------------------------------------- vectors
 // Vec3 follows the SSE Vector<3,float> of math/Linagl.h: lane-wise + - * /,
 // Dot = DPPS mask 0x71 = (x*x' + y*y') + (z*z' + 0) (Linagl.h:496-500),
@@ -783,8 +794,8 @@ template <class R> inline void concentric_disk(R u0, R u1, R* dx, R* dy) {
         r = oy;
         theta = K<R>::PI_Over2 - K<R>::PI_Over4 * (ox / oy);
     }
-    *dx = r * std::cos(theta);
-    *dy = r * std::sin(theta);
+    *dx = r * rcos(theta);
+    *dy = r * rsin(theta);
 }
 template <class R> inline V3<R> cosine_hemisphere(R u0, R u1) {
     R dx, dy;
@@ -851,8 +862,8 @@ template <class R> struct GGX {
         R a = 1 / ((R)1.f + v.z);
         R r = std::sqrt(u0);
         R phi = (u1 < a) ? u1 / a * K<R>::PI : K<R>::PI + (u1 - a) / ((R)1.f - a) * K<R>::PI;
-        R P1 = r * std::cos(phi);
-        R P2 = r * std::sin(phi) * ((u1 < a) ? (R)1.0f : v.z);
+        R P1 = r * rcos(phi);
+        R P2 = r * rsin(phi) * ((u1 < a) ? (R)1.0f : v.z);
         V3<R> dir = (P1 * T1 + P2 * T2) + std::sqrt(rmax((R)0, 1 - P1 * P1 - P2 * P2)) * v;
         return norm(V3<R>(ax * dir.x, ay * dir.y, rmax((R)0, dir.z)));
     }
@@ -918,7 +929,7 @@ template <class R> bool refract(const V3<R>& wi, const V3<R>& n, R eta, V3<R>* w
 // visual/Microfacet.h:49-54
 template <class R> inline R roughness_to_alpha(R r) {
     r = rmax(r, (R)1e-3);
-    R x = std::log(r);
+    R x = rlog(r);
     return (R)1.62142f + (R)0.819955f * x + (R)0.1734f * x * x + (R)0.0171201f * x * x * x + (R)0.000640711f * x * x * x * x;
 }
 

