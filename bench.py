@@ -25,11 +25,27 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(st, paths):
-    """Bytes the dominant kernels must touch: 64 per BVH node record, 48 per
-    triangle record, 16 per shading point (material record) and 16 per path
-    for the radiance record each sample writes."""
-    return 64.0 * st.node_visits + 48.0 * st.tri_visits + 16.0 * st.shade_points + 16.0 * paths
+def kernel_bytes(name, st, paths):
+    """Algorithmic bytes one render moves through a kernel category: 64 per BVH
+    node record and 48 per triangle record the traversal visits (counted by an
+    instrumented copy of the integrator over the same render), plus the
+    per-ray / per-path records the kernel reads and writes."""
+    nv, tv = list(st.node_visits_kind), list(st.tri_visits_kind)
+    primary = st.rays - st.shadow_rays - st.mis_rays
+    if name == "pt_megakernel":
+        # all traversals, material records, the radiance record of each sample
+        return 64.0 * st.node_visits + 48.0 * st.tri_visits + 16.0 * st.shade_points + 16.0 * paths
+    if name == "wf_trace_closest":
+        # + ray (32 B) read, hit (16 B) written, queue entry read and pushed
+        return 64.0 * nv[0] + 48.0 * tv[0] + 56.0 * primary
+    if name == "wf_trace_shadow":
+        return 64.0 * nv[1] + 48.0 * tv[1] + 36.0 * st.shadow_rays
+    if name == "wf_trace_mis":
+        return 64.0 * nv[2] + 48.0 * tv[2] + 36.0 * st.mis_rays
+    if name == "wf_shade":
+        # shading record 128 B, material 128 B, path state ~200 B per shading point
+        return 456.0 * st.shade_points
+    return float("nan")
 
 
 def cpu_baseline(sa, cam, prm, rows):
@@ -63,6 +79,7 @@ def main():
     ap.add_argument("--level", type=int, default=8)
     ap.add_argument("--cpu-rows", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--single", action="store_true", help="time only --mode, not the other integrator")
     args = ap.parse_args()
     if args.gpus != 1:
         log("bench.py measures one GPU: this build runs on a single MI355X; --gpus %d is not supported" % args.gpus)
@@ -80,20 +97,28 @@ def main():
     sst = sc.stats()
     log("scene: %d triangles, BVH %d nodes depth %d, build %.1f ms (scene create %.1f ms)"
         % (sa.n_tri, sst.n_nodes, sst.max_depth, sst.build_ms, (time.perf_counter() - t0) * 1e3))
-    for i in range(args.warmup):
-        sc.render(cam, prm, download=False)
-        log("warmup %d done" % i)
-    kern_ms = np.zeros(8)
-    kern_n = np.zeros(8)
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        sc.render(cam, prm, download=False)
-        st = ctx.stats()
-        kern_ms += np.array(st.kernel_ms[:])
-        kern_n += np.array(st.kernel_launches[:])
-        log("step %d: %.1f ms, %.1f Mrays" % (i, st.total_ms, st.rays / 1e6))
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    def run(mode, steps, warmup):
+        prm.mode = g.GY_RENDER_WAVEFRONT if mode == "wavefront" else g.GY_RENDER_MEGAKERNEL
+        for i in range(warmup):
+            sc.render(cam, prm, download=False)
+            log("%s warmup %d done" % (mode, i))
+        kms, kn = np.zeros(8), np.zeros(8)
+        t0 = time.perf_counter()
+        for i in range(steps):
+            sc.render(cam, prm, download=False)
+            st = ctx.stats()
+            kms += np.array(st.kernel_ms[:])
+            kn += np.array(st.kernel_launches[:])
+            log("%s step %d: %.1f ms, %.1f Mrays" % (mode, i, st.total_ms, st.rays / 1e6))
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        return ms, kms, kn, st
+
+    other = "megakernel" if args.mode == "wavefront" else "wavefront"
+    other_ms = None
+    if not args.single:
+        other_ms = run(other, max(1, args.steps - 1), 1)[0]
+    ms_per_step, kern_ms, kern_n, st = run(args.mode, args.steps, args.warmup)
+    prm.mode = g.GY_RENDER_WAVEFRONT if args.mode == "wavefront" else g.GY_RENDER_MEGAKERNEL
     rays = st.rays
     value = paths / (ms_per_step / 1e3) / 1e6
 
@@ -104,7 +129,7 @@ def main():
     dom_name = g.lib().gy_kernel_name(dom).decode()
     launches_per_step = kern_n[dom] / args.steps
     avg_launch_ms = kern_ms[dom] / kern_n[dom]
-    bytes_total = algorithmic_bytes(cnt, paths)
+    bytes_total = kernel_bytes(dom_name, cnt, paths)
     share = kern_ms[dom] / max(kern_ms.sum(), 1e-9)
     bytes_per_launch = bytes_total / launches_per_step
     achieved = bytes_per_launch / (avg_launch_ms / 1e3) / 1e9
@@ -145,6 +170,11 @@ def main():
             "workload": "icosphere-L%d GGX conductor, %d triangles, %dx%d, %d spp, depth %d, NEE+MIS"
             % (args.level, sa.n_tri, cam.width, cam.height, prm.spp, prm.max_depth),
             "integrator": args.mode,
+            "megakernel_mpaths_per_s": round(paths / ((ms_per_step if args.mode == "megakernel" else other_ms) / 1e3) / 1e6, 3)
+            if (args.mode == "megakernel" or other_ms) else None,
+            "wavefront_mpaths_per_s": round(paths / ((ms_per_step if args.mode == "wavefront" else other_ms) / 1e3) / 1e6, 3)
+            if (args.mode == "wavefront" or other_ms) else None,
+            "same_estimator": "wavefront and megakernel films are bit-identical (tests/test_gpu_parity.py), so RMSE is equal at equal spp",
             "paths_per_step": paths,
             "mrays_per_s": round(rays / (ms_per_step / 1e3) / 1e6, 2),
             "rays_per_step": int(rays),

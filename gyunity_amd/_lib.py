@@ -96,6 +96,8 @@ class RenderStats(C.Structure):
         ("node_visits", C.c_int64),
         ("tri_visits", C.c_int64),
         ("shade_points", C.c_int64),
+        ("node_visits_kind", C.c_int64 * 3),
+        ("tri_visits_kind", C.c_int64 * 3),
     ]
 
 

@@ -188,7 +188,8 @@ __global__ void __launch_bounds__(256) k_count_visits(SceneDev S, const gy_ray* 
 // The whole of TiledIntegrator's per-sample work (integrator/TiledIntegrator.cpp:40-47)
 // and PathTracing::Li (integrator/PT.cpp:10-67) for one camera sample per thread.
 struct PathCounters {
-    uint32_t rays, shadow, mis, shade, nodes, tris;
+    uint32_t rays, shadow, mis, shade;
+    uint32_t nodes[3], tris[3];  // by ray kind: camera/continuation, shadow, MIS
 };
 
 template <int STACK, bool COUNT>
@@ -215,8 +216,8 @@ __device__ V3 direct_illumination(const SceneDev& S, const Isect& it, const BSDF
                 float stmax;
                 spawn_to(it, lp, &so, &sd, &stmax);
                 Hit hs;
-                uint32_t* nn = COUNT ? &pc.nodes : nullptr;
-                uint32_t* nt = COUNT ? &pc.tris : nullptr;
+                uint32_t* nn = COUNT ? &pc.nodes[1] : nullptr;
+                uint32_t* nt = COUNT ? &pc.tris[1] : nullptr;
                 pc.shadow++;
                 if (traverse<true, STACK>(S, so, sd, 0.f, stmax, &hs, stack, stride, nn, nt)) Li = v3s(0.f);
                 w1 = power_heuristic(pdf, sp);
@@ -238,8 +239,8 @@ __device__ V3 direct_illumination(const SceneDev& S, const Isect& it, const BSDF
                 V3 ro;
                 spawn_ray(it, wi, &ro);
                 Hit h2;
-                uint32_t* nn = COUNT ? &pc.nodes : nullptr;
-                uint32_t* nt = COUNT ? &pc.tris : nullptr;
+                uint32_t* nn = COUNT ? &pc.nodes[2] : nullptr;
+                uint32_t* nt = COUNT ? &pc.tris[2] : nullptr;
                 pc.mis++;
                 if (traverse<false, STACK>(S, ro, wi, 0.f, kInf, &h2, stack, stride, nn, nt)) {
                     const int hl = __float_as_int(S.shade[(size_t)h2.prim * 8 + 1].w);
@@ -255,12 +256,7 @@ __device__ V3 direct_illumination(const SceneDev& S, const Isect& it, const BSDF
     return (L1 * w1 + L2 * w2) / lsp;
 }
 
-struct PathParams {
-    CamDev cam;
-    EnumDev en;
-    const uint32_t* sobol;
-    int32_t max_depth, rr_depth;
-};
+
 
 template <int STACK, bool COUNT>
 __device__ V3 trace_path(const SceneDev& S, const PathParams& P, int px, int py, int s, uint32_t* stack, int stride,
@@ -279,8 +275,8 @@ __device__ V3 trace_path(const SceneDev& S, const PathParams& P, int px, int py,
     for (int i = 0; i < P.max_depth; ++i) {
         Hit h;
         pc.rays++;
-        uint32_t* nn = COUNT ? &pc.nodes : nullptr;
-        uint32_t* nt = COUNT ? &pc.tris : nullptr;
+        uint32_t* nn = COUNT ? &pc.nodes[0] : nullptr;
+        uint32_t* nt = COUNT ? &pc.tris[0] : nullptr;
         if (!traverse<false, STACK>(S, o, d, 0.f, kInf, &h, stack, stride, nn, nt)) break;
         const Isect it = query(S, o, d, h);
         const BSDF b = make_bsdf(S, it);
@@ -371,7 +367,7 @@ __global__ void __launch_bounds__(BLOCK) k_pt_megakernel(SceneDev S, PathParams 
     const int64_t t = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     int s_local, px, py;
     pass_pixel(F, t, &s_local, &px, &py);
-    PathCounters pc = {0, 0, 0, 0, 0, 0};
+    PathCounters pc = {};
     if (s_local < n_s && px < F.width && py < F.height) {
         float fx, fy;
         const V3 L = trace_path<STACK, false>(S, P, px, py, s0 + s_local, stack + threadIdx.x, BLOCK, &fx, &fy, nullptr, pc);
@@ -390,7 +386,7 @@ __global__ void __launch_bounds__(BLOCK) k_pt_count(SceneDev S, PathParams P, Fi
     const int64_t t = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     int s_local, px, py;
     pass_pixel(F, t, &s_local, &px, &py);
-    PathCounters pc = {0, 0, 0, 0, 0, 0};
+    PathCounters pc = {};
     if (s_local < n_s && px < F.width && py < F.height) {
         float fx, fy;
         (void)trace_path<STACK, true>(S, P, px, py, s0 + s_local, stack + threadIdx.x, BLOCK, &fx, &fy, nullptr, pc);
@@ -399,8 +395,10 @@ __global__ void __launch_bounds__(BLOCK) k_pt_count(SceneDev S, PathParams P, Fi
     wave_add_u64(&counters[1], pc.shadow);
     wave_add_u64(&counters[2], pc.mis);
     wave_add_u64(&counters[3], pc.shade);
-    wave_add_u64(&counters[4], pc.nodes);
-    wave_add_u64(&counters[5], pc.tris);
+    for (int k = 0; k < 3; ++k) {
+        wave_add_u64(&counters[4 + k], pc.nodes[k]);
+        wave_add_u64(&counters[7 + k], pc.tris[k]);
+    }
 }
 
 template <int STACK>
@@ -410,7 +408,7 @@ __global__ void __launch_bounds__(128) k_li_samples(SceneDev S, PathParams P, in
     __shared__ uint32_t stack[STACK * 128];
     const int i = blockIdx.x * 128 + threadIdx.x;
     if (i >= n) return;
-    PathCounters pc = {0, 0, 0, 0, 0, 0};
+    PathCounters pc = {};
     float fx, fy;
     uint32_t nd;
     const V3 v = trace_path<STACK, false>(S, P, xs[i], ys[i], ss[i], stack + threadIdx.x, 128, &fx, &fy, &nd, pc);
@@ -1195,7 +1193,7 @@ extern "C" int gy_count_traffic(gy_scene* s, const gy_camera_desc* cam, const gy
     if (rc != GY_OK) return rc;
     for (int s0 = 0; s0 < p->spp && rc == GY_OK; s0 += n_pass) rc = LaunchMega(c, sc, P, s0, std::min(n_pass, p->spp - s0), 128, true);
     if (rc != GY_OK) return rc;
-    unsigned long long h[8];
+    unsigned long long h[16];
     hipError_t e = hipMemcpyAsync(h, sc->film.counters, sizeof(h), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return HipFail(e, "count D2H");
@@ -1205,7 +1203,11 @@ extern "C" int gy_count_traffic(gy_scene* s, const gy_camera_desc* cam, const gy
     out->shadow_rays = (int64_t)h[1];
     out->mis_rays = (int64_t)h[2];
     out->shade_points = (int64_t)h[3];
-    out->node_visits = (int64_t)h[4];
-    out->tri_visits = (int64_t)h[5];
+    for (int k = 0; k < 3; ++k) {
+        out->node_visits_kind[k] = (int64_t)h[4 + k];
+        out->tri_visits_kind[k] = (int64_t)h[7 + k];
+        out->node_visits += (int64_t)h[4 + k];
+        out->tri_visits += (int64_t)h[7 + k];
+    }
     return GY_OK;
 }

@@ -73,6 +73,8 @@ struct WavefrontState {
     int64_t capacity = 0;
     void* mem = nullptr;
     size_t bytes = 0;
+    void* spill = nullptr;
+    size_t spill_bytes = 0;
 };
 
 struct Scene {
@@ -103,7 +105,13 @@ void SceneFree(Scene* sc);
 void FilmFree(FilmState* f);
 FilmDev FilmView(Scene* sc);
 
-struct PathParams;
+// Per-render constants of the integrator kernels.
+struct PathParams {
+    gyd::CamDev cam;
+    gyd::EnumDev en;
+    const uint32_t* sobol;
+    int32_t max_depth, rr_depth;
+};
 int WavefrontPass(Ctx* c, Scene* sc, const PathParams& P, const gy_render_params& p, int s0, int n_s);
 void WavefrontFree(Scene* sc);
 // GPU binned-SAH BVH build (bvh_gpu.hip).

@@ -121,9 +121,11 @@ typedef struct {
     double total_ms;         /* wall time of the render call on the device stream */
     double kernel_ms[8];     /* per-kernel time, see gy_kernel_name */
     int64_t kernel_launches[8];
-    int64_t node_visits;     /* filled by gy_count_traffic */
+    int64_t node_visits;     /* filled by gy_count_traffic: all rays */
     int64_t tri_visits;
     int64_t shade_points;
+    int64_t node_visits_kind[3];  /* by ray kind: 0 camera/continuation, 1 shadow, 2 MIS */
+    int64_t tri_visits_kind[3];
 } gy_render_stats;
 
 /* ---- context ------------------------------------------------------------ */
