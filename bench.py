@@ -80,6 +80,8 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--single", action="store_true", help="time only --mode, not the other integrator")
+    ap.add_argument("--refill", type=int, default=0, help="wavefront refill threshold (0 = library default)")
+    ap.add_argument("--spp-pass", type=int, default=0, help="samples per pass (0 = library default)")
     args = ap.parse_args()
     if args.gpus != 1:
         log("bench.py measures one GPU: this build runs on a single MI355X; --gpus %d is not supported" % args.gpus)
@@ -89,6 +91,8 @@ def main():
 
     sa, cam, prm = g.generate_scene(g.GY_SCENE_ICOSPHERE_GGX, 0, args.level)
     prm.mode = g.GY_RENDER_WAVEFRONT if args.mode == "wavefront" else g.GY_RENDER_MEGAKERNEL
+    prm.refill_threshold = args.refill
+    prm.samples_per_pass = args.spp_pass
     paths = cam.width * cam.height * prm.spp
     ctx = g.Context(0)
     log("device:", ctx.device_name())
@@ -111,6 +115,8 @@ def main():
             kn += np.array(st.kernel_launches[:])
             log("%s step %d: %.1f ms, %.1f Mrays" % (mode, i, st.total_ms, st.rays / 1e6))
         ms = (time.perf_counter() - t0) * 1e3 / steps
+        log("%s per-kernel ms/step: %s" % (mode, ", ".join(
+            "%s %.1f" % (g.lib().gy_kernel_name(k).decode(), kms[k] / steps) for k in range(8) if kn[k] > 0)))
         return ms, kms, kn, st
 
     other = "megakernel" if args.mode == "wavefront" else "wavefront"

@@ -65,7 +65,8 @@ class RenderParams(C.Structure):
         ("mode", C.c_int32),
         ("samples_per_pass", C.c_int32),
         ("block_size", C.c_int32),
-        ("reserved", C.c_int32 * 2),
+        ("refill_threshold", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 

@@ -837,8 +837,12 @@ int LaunchMega(Ctx* c, Scene* sc, const PathParams& P, int s0, int n_s, int bloc
 
 int SamplesPerPass(const gy_camera_desc& cam, const gy_render_params& p) {
     if (p.samples_per_pass > 0) return std::min(p.samples_per_pass, p.spp);
+    // Passes of up to 32M paths: the wavefront keeps every path of a pass
+    // resident (~300 B each, ~10 GB at 32M), which amortises each bounce's
+    // kernel drain over more work; the megakernel uses the same split so both
+    // accumulate the film in the same order.
     const int64_t WH = (int64_t)cam.width * cam.height;
-    int64_t s = (4ll << 20) / std::max<int64_t>(1, WH);
+    int64_t s = (32ll << 20) / std::max<int64_t>(1, WH);
     if (s < 1) s = 1;
     if (s > p.spp) s = p.spp;
     return (int)s;
@@ -863,6 +867,11 @@ extern "C" int gy_open(int device, gy_context** out) {
     if (e != hipSuccess) return HipFail(e, "hipSetDevice");
     Ctx* c = new Ctx();
     c->device = device;
+    {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cu = prop.multiProcessorCount;
+        if (c->num_cu <= 0) c->num_cu = 256;
+    }
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete c;

@@ -28,6 +28,7 @@ struct DevBuf {
 
 struct Ctx {
     int device = 0;
+    int num_cu = 0;
     hipStream_t stream = nullptr;
     const uint32_t* d_sobol = nullptr;
     int fault = 0;

@@ -28,6 +28,9 @@ namespace gy {
 
 
 enum QueueId { Q_ACTIVE = 0, Q_NEXT = 1, Q_SHADE0 = 2, Q_SHADOW = 6, Q_MIS = 7, Q_COUNT = 8 };
+// counts[] words 8..10: work-fetch counters of the persistent trace kernels
+constexpr int C_FETCH0 = 8;
+constexpr int C_WORDS = 16;
 
 // Path flags (PathState.seq.w)
 constexpr uint32_t F_DEPTH_MASK = 0xffu;
@@ -181,6 +184,129 @@ __global__ void __launch_bounds__(256) k_wf_trace_mis(SceneDev S, WfDev W) {
     float4 p3 = W.pend3[slot];
     p3.w = __uint_as_float(__float_as_uint(p3.w) | P_L2);
     W.pend3[slot] = p3;
+}
+
+// ---------------------------------------------------------------- persistent trace
+// MODE 0: closest hit of the active queue (hits binned by material),
+// MODE 1: shadow rays (any hit), MODE 2: MIS rays (closest hit + emission).
+// Each lane takes a new queue entry as soon as its ray finishes (one atomic per
+// wave); the node loop returns to refilling once fewer than `refill` lanes of
+// the wave are still traversing.  A wave exits when the queue is drained and
+// none of its lanes holds a ray.
+constexpr uint32_t kBatch = 64;
+
+template <int S, int MODE>
+__global__ void __launch_bounds__(256) k_wf_trace(SceneDev Sc, WfDev W, int refill) {
+    __shared__ uint32_t stack[S * 256];
+    constexpr int qid = MODE == 0 ? Q_ACTIVE : (MODE == 1 ? Q_SHADOW : Q_MIS);
+    const uint32_t n = W.counts[qid];
+    uint32_t* fetch = &W.counts[C_FETCH0 + MODE];
+    const int lane = threadIdx.x & 63;
+    const int64_t gl = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    uint32_t* lds = stack + threadIdx.x;
+    uint32_t* spill = W.spill + gl;
+    int64_t my = -1;  // -1 wants work, -2 queue drained, >= 0 queue index
+    uint32_t batch_base = 0, batch_left = 0;  // wave-uniform
+    bool fresh = false, fin = false;
+    int mt = -1;
+    uint32_t slot = 0;
+    Trav t;
+    while (true) {
+        if (MODE == 0) {
+#pragma unroll
+            for (int m = 0; m < 4; ++m) wave_push(W.q[Q_SHADE0 + m], &W.counts[Q_SHADE0 + m], slot, fin && mt == m);
+        }
+        fin = false;
+        // Refill idle lanes from the wave's batch of queue entries; when the
+        // batch runs dry, claim the next `kBatch` entries with one atomic.
+        const bool need = my == -1;
+        uint64_t nm = __ballot(need);
+        while (nm) {
+            if (batch_left == 0) {
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(fetch, (uint32_t)kBatch);
+                base = __shfl(base, 0, 64);
+                batch_base = base;
+                batch_left = base < n ? min((uint32_t)kBatch, n - base) : 0u;
+                if (batch_left == 0) {
+                    if (need && my == -1) my = -2;
+                    break;
+                }
+            }
+            const uint32_t rank = (uint32_t)__popcll(nm & ((1ull << lane) - 1ull));
+            const uint32_t take = min((uint32_t)__popcll(nm), batch_left);
+            if (need && my == -1 && rank < take) {
+                my = batch_base + rank;
+                fresh = true;
+            }
+            batch_base += take;
+            batch_left -= take;
+            nm = __ballot(need && my == -1);
+        }
+        if (__ballot(my >= 0) == 0) break;
+        if (my >= 0) {
+            if (fresh) {
+                fresh = false;
+                slot = W.q[qid][my];
+                if (MODE == 0) {
+                    trav_init(t, f3(W.o[slot]), f3(W.d[slot]), 0.f, kInf);
+                } else if (MODE == 1) {
+                    const float4 o = W.sh_o[slot];
+                    trav_init(t, f3(o), f3(W.sh_d[slot]), 0.f, o.w);
+                } else {
+                    trav_init(t, f3(W.mis_o[slot]), f3(W.mis_d[slot]), 0.f, kInf);
+                }
+            }
+            while (true) {
+                const int r = trav_step<MODE == 1, S>(Sc, t, lds, 256, spill, W.spill_stride);
+                if (r != 0) {
+                    if (MODE == 0) {
+                        if (t.best_prim >= 0) {
+                            W.hit[slot] = make_float4(t.best, __int_as_float(t.best_prim), t.bb1, t.bb2);
+                            const uint32_t mat = __float_as_uint(Sc.shade[(size_t)t.best_prim * 8].w);
+                            mt = (int)Sc.mats[(size_t)mat * 32];
+                        } else {
+                            mt = -1;
+                            uint4 sq = W.seq[slot];
+                            sq.w |= F_DONE;
+                            W.seq[slot] = sq;
+                        }
+                    } else if (MODE == 1) {
+                        if (r == 2) {
+                            float4 p3 = W.pend3[slot];
+                            p3.w = __uint_as_float(__float_as_uint(p3.w) | P_OCCLUDED);
+                            W.pend3[slot] = p3;
+                        }
+                    } else if (t.best_prim >= 0) {
+                        // MIS branch of DirectIllumination (integrator/Integrator.cpp:88-98)
+                        const float4 p2 = W.pend2[slot];
+                        const int light = __float_as_int(p2.w);
+                        const int hl = __float_as_int(Sc.shade[(size_t)t.best_prim * 8 + 1].w);
+                        if (hl == light) {
+                            Hit h;
+                            h.t = t.best;
+                            h.prim = t.best_prim;
+                            h.b1 = t.bb1;
+                            h.b2 = t.bb2;
+                            const Isect li = query(Sc, t.o, t.d, h);
+                            const V3 Le = v3(Sc.light_le[3 * light], Sc.light_le[3 * light + 1], Sc.light_le[3 * light + 2]);
+                            const V3 Lem = dot(li.n, neg(t.d)) > 0.f ? Le : v3(0.f, 0.f, 0.f);
+                            const float4 p1 = W.pend1[slot];
+                            const V3 L2 = Lem * f3(p1) * v3s(1.f) * p1.w / p2.x;
+                            W.pend1[slot] = make_float4(L2.x, L2.y, L2.z, p1.w);
+                            float4 p3 = W.pend3[slot];
+                            p3.w = __uint_as_float(__float_as_uint(p3.w) | P_L2);
+                            W.pend3[slot] = p3;
+                        }
+                    }
+                    fin = true;
+                    my = -1;
+                    break;
+                }
+                if (__popcll(__ballot(true)) < refill) break;
+            }
+        }
+    }
 }
 
 // ---------------------------------------------------------------- shade
@@ -373,11 +499,11 @@ __global__ void k_wf_advance(WfDev W, unsigned long long* ctr) {
         ctr[2] += W.counts[Q_MIS];
         W.counts[Q_ACTIVE] = W.counts[Q_NEXT];
         W.counts[Q_NEXT] = 0;
-        for (int q = Q_SHADE0; q < Q_COUNT; ++q) W.counts[q] = 0;
+        for (int q = Q_SHADE0; q < C_WORDS; ++q) W.counts[q] = 0;
     }
 }
 __global__ void k_wf_reset(WfDev W) {
-    if (threadIdx.x < Q_COUNT && blockIdx.x == 0) W.counts[threadIdx.x] = 0;
+    if (threadIdx.x < C_WORDS && blockIdx.x == 0) W.counts[threadIdx.x] = 0;
 }
 
 // ---------------------------------------------------------------- host side
@@ -423,8 +549,21 @@ void WavefrontFree(Scene* sc) {
     sc->wf = WavefrontState();
 }
 
-// Traversal lanes per launch are the pass capacity rounded up to whole blocks.
-static int64_t WfLanes(int64_t n) { return (n + 255) / 256 * 256; }
+constexpr int kStackLds = 16;
+
+// Persistent trace grids: as many 256-lane blocks as the device keeps resident.
+static unsigned PersistBlocks(Ctx* c) {
+    static int per_cu = 0;
+    if (per_cu == 0) {
+        int a = 0, b = 0, d = 0;
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_wf_trace<kStackLds, 0>, 256, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_wf_trace<kStackLds, 1>, 256, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&d, k_wf_trace<kStackLds, 2>, 256, 0);
+        per_cu = std::max(1, std::max(a, std::max(b, d)));
+    }
+    return (unsigned)(per_cu * c->num_cu);
+}
+static int64_t WfLanes(Ctx* c) { return (int64_t)PersistBlocks(c) * 256; }
 
 static int WfPrepare(Ctx* c, Scene* sc, int64_t n) {
     if (sc->wf.capacity >= n && sc->wf.mem) return GY_OK;
@@ -432,7 +571,7 @@ static int WfPrepare(Ctx* c, Scene* sc, int64_t n) {
     const size_t bytes = WfBytes(n);
     int rc = DevAlloc(c, &sc->wf.mem, bytes);
     if (rc != GY_OK) return rc;
-    const size_t sbytes = (size_t)WfLanes(n) * 64 * 4;
+    const size_t sbytes = (size_t)WfLanes(c) * 64 * 4;
     rc = DevAlloc(c, &sc->wf.spill, sbytes);
     if (rc != GY_OK) return rc;
     sc->wf.capacity = n;
@@ -445,9 +584,11 @@ template <int STACK>
 static int WfBounceLoop(Ctx* c, Scene* sc, const WfDev& W0, const PathParams& P, const gy_render_params& p, unsigned blocks) {
     const FilmDev F = FilmView(sc);
     WfDev W = W0;
+    const unsigned pblocks = PersistBlocks(c);
+    const int refill = (p.refill_threshold > 0 && p.refill_threshold <= 64) ? p.refill_threshold : 1;
     for (int depth = 0; depth < p.max_depth; ++depth) {
         TimerBegin(c, kKernTraceClosest);
-        hipLaunchKernelGGL(k_wf_trace_closest<STACK>, dim3(blocks), dim3(256), 0, c->stream, sc->dev, W);
+        hipLaunchKernelGGL((k_wf_trace<STACK, 0>), dim3(pblocks), dim3(256), 0, c->stream, sc->dev, W, refill);
         TimerEnd(c);
         TimerBegin(c, kKernShade);
         hipLaunchKernelGGL(k_wf_shade<0>, dim3(blocks), dim3(256), 0, c->stream, sc->dev, W, P);
@@ -456,10 +597,10 @@ static int WfBounceLoop(Ctx* c, Scene* sc, const WfDev& W0, const PathParams& P,
         hipLaunchKernelGGL(k_wf_shade<3>, dim3(blocks), dim3(256), 0, c->stream, sc->dev, W, P);
         TimerEnd(c);
         TimerBegin(c, kKernTraceShadow);
-        hipLaunchKernelGGL(k_wf_trace_shadow<STACK>, dim3(blocks), dim3(256), 0, c->stream, sc->dev, W);
+        hipLaunchKernelGGL((k_wf_trace<STACK, 1>), dim3(pblocks), dim3(256), 0, c->stream, sc->dev, W, refill);
         TimerEnd(c);
         TimerBegin(c, kKernTraceMis);
-        hipLaunchKernelGGL(k_wf_trace_mis<STACK>, dim3(blocks), dim3(256), 0, c->stream, sc->dev, W);
+        hipLaunchKernelGGL((k_wf_trace<STACK, 2>), dim3(pblocks), dim3(256), 0, c->stream, sc->dev, W, refill);
         TimerEnd(c);
         TimerBegin(c, kKernOther);
         hipLaunchKernelGGL(k_wf_resolve, dim3(blocks), dim3(256), 0, c->stream, W, F);
@@ -486,7 +627,7 @@ int WavefrontPass(Ctx* c, Scene* sc, const PathParams& P, const gy_render_params
     size_t used = 0;
     WfDev W = g_view(sc, &used);
     W.spill = (uint32_t*)sc->wf.spill;
-    W.spill_stride = WfLanes(cap);
+    W.spill_stride = WfLanes(c);
     if ((size_t)W.spill_stride * 64 * 4 > sc->wf.spill_bytes) return Fail(GY_ERR_ALLOC, "wavefront spill buffer too small");
     if (used > sc->wf.bytes) return Fail(GY_ERR_ALLOC, "wavefront state layout exceeds its allocation");
     hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(64), 0, c->stream, W);
@@ -502,7 +643,7 @@ int WavefrontPass(Ctx* c, Scene* sc, const PathParams& P, const gy_render_params
         return CheckLaunch(c, "k_wf_resolve");
     }
     if (sc->stats.max_depth + 1 > 64) return Fail(GY_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
-    return WfBounceLoop<16>(c, sc, W, P, p, blocks);
+    return WfBounceLoop<kStackLds>(c, sc, W, P, p, blocks);
 }
 
 }  // namespace gy
