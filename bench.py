@@ -82,6 +82,9 @@ def main():
     ap.add_argument("--single", action="store_true", help="time only --mode, not the other integrator")
     ap.add_argument("--refill", type=int, default=0, help="wavefront refill threshold (0 = library default)")
     ap.add_argument("--spp-pass", type=int, default=0, help="samples per pass (0 = library default)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="after the run, copy MIB MiB once with a plain elementwise kernel (torch.neg), "
+                         "the known-size reference for HBM counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
         log("bench.py measures one GPU: this build runs on a single MI355X; --gpus %d is not supported" % args.gpus)
@@ -192,6 +195,17 @@ def main():
     }
     sc.close()
     ctx.close()
+    if args.calibrate_copy > 0:
+        import torch
+
+        n = args.calibrate_copy * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        log("calibration copy: %d bytes read, %d bytes written by one torch.neg kernel" % (4 * n, 4 * n))
+        out["calibration_copy_bytes"] = 4 * n
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(sa, cam, prm, args.cpu_rows)

@@ -93,3 +93,56 @@ def camera_rays(cam, n, seed):
     rays = np.zeros(n, RAY_DTYPE)
     rays["org"], rays["dir"], rays["tmin"], rays["tmax"] = org, d, 0.0, np.inf
     return rays
+
+
+def check_bvh(nodes, order, pos, max_leaf):
+    """Structural invariants of a downloaded BVH (16 words per node: two child
+    boxes, two references, two counts).  Every triangle sits in exactly one
+    leaf, every leaf box contains its triangles, every child box contains the
+    boxes of the node it references, every node is reached once, leaves hold at
+    most max_leaf triangles.  Returns the deepest leaf's depth (root children
+    are at depth 1)."""
+    n = pos.shape[0]
+    assert np.array_equal(np.sort(order), np.arange(n)), "leaf order is not a permutation"
+    f = nodes.view(np.float32)
+    tri = pos.reshape(n, 3, 3)
+    vmin, vmax = tri.min(axis=1), tri.max(axis=1)
+    covered = np.zeros(n, np.int64)
+    reached = np.zeros(nodes.shape[0], np.int64)
+    reached[0] = 1
+    frontier = np.array([0])
+    depth, leaf_depth = 0, 0
+    while frontier.size:
+        depth += 1
+        assert depth < 128, "cycle in the node graph"
+        nd, fd = nodes[frontier], f[frontier]
+        nxt = []
+        for s in (0, 1):
+            ref, cnt = nd[:, 12 + s], nd[:, 14 + s]
+            bmn, bmx = fd[:, 6 * s:6 * s + 3], fd[:, 6 * s + 3:6 * s + 6]
+            valid = ref != 0xFFFFFFFF
+            leaf = valid & (cnt > 0)
+            inner = valid & (cnt == 0)
+            if leaf.any():
+                leaf_depth = depth
+                st, ct = ref[leaf].astype(np.int64), cnt[leaf].astype(np.int64)
+                assert ct.max() <= max_leaf
+                assert (st + ct).max() <= n
+                rep = np.repeat(np.arange(st.size), ct)
+                posi = st[rep] + (np.arange(ct.sum()) - (np.cumsum(ct) - ct)[rep])
+                np.add.at(covered, posi, 1)
+                t = order[posi]
+                assert (vmin[t] >= bmn[leaf][rep]).all() and (vmax[t] <= bmx[leaf][rep]).all(), "leaf box misses a triangle"
+            if inner.any():
+                ch = ref[inner].astype(np.int64)
+                assert ch.max() < nodes.shape[0]
+                np.add.at(reached, ch, 1)
+                for cs in (0, 1):
+                    cv = nodes[ch, 12 + cs] != 0xFFFFFFFF
+                    cmn, cmx = f[ch, 6 * cs:6 * cs + 3][cv], f[ch, 6 * cs + 3:6 * cs + 6][cv]
+                    assert (cmn >= bmn[inner][cv]).all() and (cmx <= bmx[inner][cv]).all(), "child box outside parent"
+                nxt.append(ch)
+        frontier = np.concatenate(nxt) if nxt else np.array([], np.int64)
+    assert (covered == 1).all(), "triangle leaf counts range %d..%d" % (covered.min(), covered.max())
+    assert (reached == 1).all(), "nodes reached %d..%d times" % (reached.min(), reached.max())
+    return leaf_depth

@@ -229,3 +229,28 @@ def test_wavefront_depth_edge_cases(gpu_ctx):
             assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
             if depth == 0:
                 assert out[1][0].sum() == 0 and out[1][1].sum() >= 32 * 32 * 2
+
+
+@pytest.mark.parametrize("level", [5, 8])
+def test_gpu_bvh_builder(gpu_ctx, level):
+    # The GPU binned-SAH build: valid structure, SAH cost close to the host
+    # build's, and closest hits that do not depend on which builder ran.
+    from tests.scene_util import check_bvh
+    sa, cam, _ = g.generate_scene(g.GY_SCENE_ICOSPHERE_GGX, 0, level)
+    rays = camera_rays(cam, 20000, 21)
+    out = {}
+    for builder in (0, 1):
+        with g.Scene(gpu_ctx, sa, max_leaf_size=4, builder=builder) as sc:
+            st = sc.stats()
+            nodes, order = sc.bvh()
+            depth = check_bvh(nodes, order, sa.pos.reshape(-1, 9), 4)
+            assert depth <= st.max_depth < 64
+            out[builder] = (st.sah_cost, sc.trace_closest(rays), st.build_ms)
+    print("sah gpu %.3f host %.3f  build ms gpu %.1f host %.1f" % (out[0][0], out[1][0], out[0][2], out[1][2]))
+    assert out[0][0] <= 1.05 * out[1][0]
+    h0, h1 = out[0][1], out[1][1]
+    assert np.array_equal(h0["prim"] < 0, h1["prim"] < 0)
+    diff = h0["prim"] != h1["prim"]
+    tie = np.abs(h0["t"] - h1["t"]) <= T_REL * np.abs(h1["t"])
+    assert (diff & ~tie).sum() == 0
+    assert np.array_equal(h0["t"][~diff], h1["t"][~diff])
