@@ -83,7 +83,7 @@ def main():
     ap.add_argument("--refill", type=int, default=0, help="wavefront refill threshold (0 = library default)")
     ap.add_argument("--spp-pass", type=int, default=0, help="samples per pass (0 = library default)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
-                    help="after the run, copy MIB MiB once with a plain elementwise kernel (torch.neg), "
+                    help="after the run, copy MIB MiB with the library's plain copy kernel (k_copy_probe), "
                          "the known-size reference for HBM counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -142,13 +142,16 @@ def main():
     share = kern_ms[dom] / max(kern_ms.sum(), 1e-9)
     bytes_per_launch = bytes_total / launches_per_step
     achieved = bytes_per_launch / (avg_launch_ms / 1e3) / 1e9
+    # HBM bytes per launch from the counter runs (FETCH_SIZE, WRITE_SIZE, scaled by
+    # the same counters on a plain copy of known size), when they cover this workload.
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_%s.json" % dom_name)
+    tpath = os.path.join(ROOT, "profiles", "traffic_icosphere-L%d-1024-64spp.txt" % args.level)
     if os.path.exists(tpath):
         with open(tpath) as f:
-            t = json.load(f)
-        if t.get("workload") == "icosphere-L%d-1024-64spp" % args.level:
-            traffic = t.get("bytes_per_launch")
+            for line in f:
+                w = line.split()
+                if len(w) == 4 and w[0] == "traffic" and w[1] == dom_name and w[2] == "icosphere-L%d-1024-64spp" % args.level:
+                    traffic = float(w[3])
     roof = {
         "bound": "hbm",
         "kernel": dom_name,
@@ -194,18 +197,14 @@ def main():
         "roofline": roof,
     }
     sc.close()
-    ctx.close()
     if args.calibrate_copy > 0:
-        import torch
-
-        n = args.calibrate_copy * (1 << 20) // 4
-        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        torch.neg(src, out=dst)
-        torch.cuda.synchronize()
-        log("calibration copy: %d bytes read, %d bytes written by one torch.neg kernel" % (4 * n, 4 * n))
-        out["calibration_copy_bytes"] = 4 * n
+        nbytes = args.calibrate_copy << 20
+        cms = ctx.copy_probe(nbytes)
+        log("calibration copy: k_copy_probe read %d and wrote %d bytes in %.3f ms (%.1f GB/s)"
+            % (nbytes, nbytes, cms, 2 * nbytes / cms / 1e6))
+        out["calibration_copy"] = {"bytes_read": nbytes, "bytes_written": nbytes, "ms": round(cms, 4),
+                                   "gbs": round(2 * nbytes / cms / 1e6, 1)}
+    ctx.close()
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(sa, cam, prm, args.cpu_rows)

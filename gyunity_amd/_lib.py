@@ -142,6 +142,7 @@ def lib():
         "gy_kernel_name": (C.c_char_p, [C.c_int]),
         "gy_li_samples": (C.c_int, [vp, P(CameraDesc), P(RenderParams), i32, P(i32), P(i32), P(i32), P(f32), P(f32), P(i32)]),
         "gy_count_traffic": (C.c_int, [vp, P(CameraDesc), P(RenderParams), P(RenderStats)]),
+        "gy_copy_probe": (C.c_int, [vp, i64, P(C.c_double)]),
         "gy_sobol_sample": (f32, [u64, u32]),
         "gy_sobol_enum_index": (u64, [u32, u32, u32, u32]),
         "gy_camera_matrix": (C.c_int, [P(CameraDesc), P(f32)]),
@@ -297,6 +298,12 @@ class Context:
         s = RenderStats()
         check(lib().gy_render_stats_get(self.h, C.byref(s)))
         return s
+
+    def copy_probe(self, nbytes):
+        """Device time (ms) of one plain copy of nbytes between device buffers."""
+        ms = C.c_double()
+        check(lib().gy_copy_probe(self.h, int(nbytes), C.byref(ms)))
+        return ms.value
 
     def last_kernel_ms(self):
         return lib().gy_last_kernel_ms(self.h)

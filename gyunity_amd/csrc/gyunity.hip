@@ -1220,3 +1220,43 @@ extern "C" int gy_count_traffic(gy_scene* s, const gy_camera_desc* cam, const gy
     }
     return GY_OK;
 }
+
+// Plain device-to-device copy of a known size: the HBM reference for counter
+// runs (FETCH_SIZE / WRITE_SIZE of this kernel against its byte count) and the
+// copy bandwidth reachable on the device.
+__global__ void k_copy_probe(const float4* __restrict__ src, float4* __restrict__ dst, int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+extern "C" int gy_copy_probe(gy_context* ctx, int64_t bytes, double* ms_out) {
+    if (!ctx || bytes <= 0 || (bytes & 15) || !ms_out) return Fail(GY_ERR_INVALID, "gy_copy_probe: bytes must be a positive multiple of 16");
+    Ctx* c = reinterpret_cast<Ctx*>(ctx);
+    (void)hipSetDevice(c->device);
+    void *src = nullptr, *dst = nullptr;
+    int rc = DevAlloc(c, &src, (size_t)bytes);
+    if (rc == GY_OK) rc = DevAlloc(c, &dst, (size_t)bytes);
+    if (rc == GY_OK) rc = Fill32(c, src, 0x3f800000u, bytes / 4);
+    const int64_t n = bytes / 16;
+    const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, (int64_t)std::max(c->num_cu, 1) * 32);
+    hipEvent_t a = nullptr, b = nullptr;
+    if (rc == GY_OK) {
+        (void)hipEventCreate(&a);
+        (void)hipEventCreate(&b);
+        hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, c->stream, (const float4*)src, (float4*)dst, n);
+        (void)hipEventRecord(a, c->stream);
+        hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, c->stream, (const float4*)src, (float4*)dst, n);
+        (void)hipEventRecord(b, c->stream);
+        rc = CheckLaunch(c, "copy probe");
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (rc == GY_OK && e != hipSuccess) rc = HipFail(e, "copy probe sync");
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, a, b);
+        *ms_out = ms;
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+    }
+    DevFree(src);
+    DevFree(dst);
+    return rc;
+}

@@ -187,6 +187,13 @@ int gy_generate_scene(int kind, uint32_t seed, int32_t param, gy_generated** out
 int gy_generated_desc(gy_generated* g, gy_scene_desc* desc, gy_camera_desc* cam, gy_render_params* params);
 int gy_generated_free(gy_generated* g);
 
+/* ---- measurement ---------------------------------------------------------
+ * No counterpart in the reference.  Copies `bytes` (a multiple of 16) between
+ * two fresh device buffers with one plain copy kernel (k_copy_probe), twice,
+ * and returns the second launch's device time.  Known-size reference for HBM
+ * counter runs and the copy bandwidth the device reaches. */
+int gy_copy_probe(gy_context* ctx, int64_t bytes, double* ms);
+
 /* ---- failure injection (host-forced, for tests) --------------------------- */
 enum { GY_FAULT_NONE = 0, GY_FAULT_ALLOC = 1, GY_FAULT_LAUNCH = 2, GY_FAULT_SCENE = 3 };
 int gy_debug_inject_fault(gy_context* ctx, int kind);
