@@ -1,0 +1,383 @@
+// HipPathTracing.h - the MI355X path-tracing integrator behind Gyunity's
+// Integrator interface (header-only, on top of the C ABI in gyunity.h).
+//
+// Mirrors, name for name, what the reference's render loop touches:
+//   class Integrator { virtual void Render(const Scene&, const Camera&) = 0; }
+//                                             (integrator/Integrator.h:17-27)
+//   PathTracing(spp, maxDepth, pSampler, pSamplerEnum)   (integrator/PT.h:12)
+//   Film{mResX, mResY}, pixel = color / weight           (visual/Film.h:20-136)
+//   PinHoleCamera(film, position, lookAt, up, fovy, filmDistance)
+//                                             (camera/PinholeCamera.h:16-20)
+//   Scene::AddPrimitive / AddLight / Initialize          (visual/Scene.h:26-65)
+//   SobolSampler, SobolEnum(film)          (sampler/Sobol.h, sampler/SobolEnum.h)
+//
+// The Scene here is the flattened form the device reads: triangles with
+// vertex/face normals and uvs, a material table, area lights by triangle.  An
+// adapter from Gyunity's own Scene fills it from GetPrimitives()/GetLights().
+//
+// Errors: every failing C call throws gy::Error with gy_last_error()'s text
+// (the reference logs and exits instead, common/Core.h:160-166).
+#pragma once
+
+#include <gyunity.h>
+
+#include <algorithm>
+#include <array>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace gy {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& what) : std::runtime_error(what), code(c) {}
+};
+
+inline void Check(int rc, const char* where) {
+    if (rc != GY_OK) throw Error(rc, std::string(where) + ": " + gy_last_error());
+}
+
+using Vec3 = std::array<float, 3>;
+
+// ---------------------------------------------------------------- device
+// One HIP device and stream.  Not re-entrant: one render at a time.
+class Device {
+public:
+    explicit Device(int ordinal = 0) { Check(gy_open(ordinal, &ctx_), "gy_open"); }
+    ~Device() { gy_close(ctx_); }
+    Device(const Device&) = delete;
+    Device& operator=(const Device&) = delete;
+    gy_context* Handle() const { return ctx_; }
+    std::string Name() const {
+        char buf[256] = {0};
+        Check(gy_device_name(ctx_, buf, sizeof(buf)), "gy_device_name");
+        return buf;
+    }
+    gy_render_stats Stats() const {
+        gy_render_stats s;
+        Check(gy_render_stats_get(ctx_, &s), "gy_render_stats_get");
+        return s;
+    }
+
+private:
+    gy_context* ctx_ = nullptr;
+};
+
+// ---------------------------------------------------------------- film
+class Film {
+public:
+    Film(int w, int h) : mResX(w), mResY(h), color((size_t)w * h * 3, 0.f), weight((size_t)w * h, 0.f) {}
+    int mResX, mResY;
+    std::vector<float> color;   // rgb sums, row-major
+    std::vector<float> weight;  // box-filter weights
+    // Pixel value as Film::WriteToPixelBuffer forms it (no splats on this path).
+    Vec3 Pixel(int x, int y) const {
+        const size_t i = (size_t)y * mResX + x;
+        const float w = weight[i];
+        return {color[3 * i] / w, color[3 * i + 1] / w, color[3 * i + 2] / w};
+    }
+};
+
+// ---------------------------------------------------------------- camera
+class Camera {
+public:
+    virtual ~Camera() = default;
+    std::shared_ptr<Film> GetFilm() const { return mpFilm; }
+    virtual gy_camera_desc Desc() const = 0;
+
+protected:
+    explicit Camera(std::shared_ptr<Film> f) : mpFilm(std::move(f)) {}
+    std::shared_ptr<Film> mpFilm;
+};
+
+class PinHoleCamera : public Camera {
+public:
+    PinHoleCamera(std::shared_ptr<Film> film, const Vec3& position, const Vec3& lookAt, const Vec3& up, float fovy,
+                  float filmDistance = 1.f)
+        : Camera(std::move(film)), mPos(position), mLookAt(lookAt), mUp(up), mFovy(fovy), mFilmDistance(filmDistance) {}
+    gy_camera_desc Desc() const override {
+        gy_camera_desc d;
+        for (int k = 0; k < 3; ++k) {
+            d.pos[k] = mPos[k];
+            d.look_at[k] = mLookAt[k];
+            d.up[k] = mUp[k];
+        }
+        d.fovy = mFovy;
+        d.film_dist = mFilmDistance;
+        d.width = mpFilm->mResX;
+        d.height = mpFilm->mResY;
+        return d;
+    }
+    Vec3 mPos, mLookAt, mUp;
+    float mFovy, mFilmDistance;
+};
+
+// ---------------------------------------------------------------- samplers
+// The device implements the Sobol sampler with the Sobol enumeration
+// (unscrambled, sampler/SobolEnum.h:24-25).  The objects carry the choice.
+class Sampler {
+public:
+    virtual ~Sampler() = default;
+    virtual const char* Name() const = 0;
+};
+class SobolSampler : public Sampler {
+public:
+    const char* Name() const override { return "sobol"; }
+};
+class SamplerEnum {
+public:
+    virtual ~SamplerEnum() = default;
+};
+class SobolEnum : public SamplerEnum {
+public:
+    explicit SobolEnum(const Film&) {}
+};
+
+// ---------------------------------------------------------------- materials
+struct Material {
+    std::array<uint32_t, GY_MAT_WORDS> words{};
+    static uint32_t Bits(float f) {
+        uint32_t u;
+        std::memcpy(&u, &f, 4);
+        return u;
+    }
+    static void Put3(uint32_t* w, const Vec3& v) {
+        for (int k = 0; k < 3; ++k) w[k] = Bits(v[k]);
+    }
+    // DiffuseMaterial(R) (material/Diffuse.h:27-31)
+    static Material Diffuse(const Vec3& R, int albedoTex = -1) {
+        Material m;
+        m.words[0] = GY_MAT_LAMBERT;
+        Put3(&m.words[2], R);
+        m.words[21] = (uint32_t)albedoTex;
+        m.words[22] = (uint32_t)-1;
+        return m;
+    }
+    // RoughnessMaterial(R, roughness, eta, k) (material/Roughness.h:36-46)
+    static Material Conductor(const Vec3& R, float roughness, const Vec3& eta, const Vec3& k, int roughnessTex = -1) {
+        Material m;
+        m.words[0] = GY_MAT_CONDUCTOR;
+        Put3(&m.words[2], R);
+        m.words[8] = m.words[9] = Bits(roughness);
+        Put3(&m.words[10], eta);
+        Put3(&m.words[13], k);
+        m.words[21] = (uint32_t)-1;
+        m.words[22] = (uint32_t)roughnessTex;
+        return m;
+    }
+    // GGX conductor with alpha given directly and Schlick F0 (benchmark material).
+    static Material ConductorF0(const Vec3& R, float alpha, const Vec3& F0) {
+        Material m;
+        m.words[0] = GY_MAT_CONDUCTOR;
+        m.words[1] = GY_MATF_ALPHA_DIRECT | GY_MATF_SCHLICK_F0;
+        Put3(&m.words[2], R);
+        m.words[8] = m.words[9] = Bits(alpha);
+        Put3(&m.words[16], F0);
+        m.words[21] = m.words[22] = (uint32_t)-1;
+        return m;
+    }
+    // GlassMaterial(R, T, eta) (material/Glass.h:30-34)
+    static Material Glass(const Vec3& R, const Vec3& T, float etaOut = 1.f, float etaIn = 1.5f) {
+        Material m;
+        m.words[0] = GY_MAT_GLASS;
+        Put3(&m.words[2], R);
+        Put3(&m.words[5], T);
+        m.words[19] = Bits(etaOut);
+        m.words[20] = Bits(etaIn);
+        m.words[21] = m.words[22] = (uint32_t)-1;
+        return m;
+    }
+    // RoughDielectricMaterial(R, T, roughness, eta) (material/RoughDielectric.h:21-41)
+    static Material RoughDielectric(const Vec3& R, const Vec3& T, float roughness, float etaOut = 1.f, float etaIn = 1.5f) {
+        Material m = Glass(R, T, etaOut, etaIn);
+        m.words[0] = GY_MAT_ROUGH_DIELECTRIC;
+        m.words[8] = m.words[9] = Bits(roughness);
+        return m;
+    }
+};
+
+// ---------------------------------------------------------------- scene
+class Scene {
+public:
+    // Scene::AddPrimitive for one triangle (primitive id = insertion order,
+    // visual/Scene.h:43).  Returns the primitive id.
+    int AddTriangle(const std::array<Vec3, 3>& p, const std::array<Vec3, 3>& n, const Vec3& faceNormal, int material,
+                    const std::array<float, 6>& uv = {0, 0, 0, 0, 0, 0}) {
+        for (int v = 0; v < 3; ++v)
+            for (int k = 0; k < 3; ++k) {
+                pos.push_back(p[v][k]);
+                nrm.push_back(n[v][k]);
+            }
+        for (int k = 0; k < 3; ++k) face_n.push_back(faceNormal[k]);
+        for (float u : uv) uvs.push_back(u);
+        mat.push_back((uint32_t)material);
+        light.push_back(-1);
+        return (int)mat.size() - 1;
+    }
+    int AddMaterial(const Material& m) {
+        mats.insert(mats.end(), m.words.begin(), m.words.end());
+        return (int)(mats.size() / GY_MAT_WORDS) - 1;
+    }
+    // Scene::AddLight(AreaLight(shape = triangle, Le)) (light/AreaLight.h:12-85)
+    int AddAreaLight(int triangle, const Vec3& Le) {
+        light[(size_t)triangle] = (int32_t)light_tri.size();
+        light_tri.push_back(triangle);
+        for (float v : Le) light_le.push_back(v);
+        return (int)light_tri.size() - 1;
+    }
+    // ImageTexture (texture/ImageTexture.h:50-69): width x height x channels floats.
+    int AddTexture(int w, int h, int channels, const std::vector<float>& data) {
+        tex_info.insert(tex_info.end(), {w, h, channels, (int32_t)tex_data.size()});
+        tex_data.insert(tex_data.end(), data.begin(), data.end());
+        return (int)(tex_info.size() / 4) - 1;
+    }
+    gy_scene_desc Desc() const {
+        gy_scene_desc d = {};
+        d.n_tri = (int32_t)mat.size();
+        d.pos = pos.data();
+        d.nrm = nrm.data();
+        d.face_n = face_n.data();
+        d.uv = uvs.data();
+        d.mat = mat.data();
+        d.light = light.data();
+        d.n_mat = (int32_t)(mats.size() / GY_MAT_WORDS);
+        d.mats = mats.data();
+        d.n_light = (int32_t)light_tri.size();
+        d.light_tri = light_tri.data();
+        d.light_le = light_le.data();
+        d.n_tex = (int32_t)(tex_info.size() / 4);
+        d.tex_info = tex_info.data();
+        d.tex_data = tex_data.data();
+        return d;
+    }
+    // Copy a flattened description (for example a generated benchmark scene).
+    static Scene FromDesc(const gy_scene_desc& d) {
+        Scene s;
+        const size_t n = (size_t)d.n_tri;
+        s.pos.assign(d.pos, d.pos + 9 * n);
+        s.nrm.assign(d.nrm, d.nrm + 9 * n);
+        s.face_n.assign(d.face_n, d.face_n + 3 * n);
+        if (d.uv) s.uvs.assign(d.uv, d.uv + 6 * n);
+        else s.uvs.assign(6 * n, 0.f);
+        s.mat.assign(d.mat, d.mat + n);
+        if (d.light) s.light.assign(d.light, d.light + n);
+        else s.light.assign(n, -1);
+        s.mats.assign(d.mats, d.mats + (size_t)d.n_mat * GY_MAT_WORDS);
+        s.light_tri.assign(d.light_tri, d.light_tri + d.n_light);
+        s.light_le.assign(d.light_le, d.light_le + 3 * (size_t)d.n_light);
+        if (d.n_tex > 0) {
+            s.tex_info.assign(d.tex_info, d.tex_info + 4 * (size_t)d.n_tex);
+            size_t total = 0;
+            for (int i = 0; i < d.n_tex; ++i) {
+                const int32_t* t = d.tex_info + 4 * i;
+                total = std::max(total, (size_t)t[3] + (size_t)t[0] * t[1] * t[2]);
+            }
+            s.tex_data.assign(d.tex_data, d.tex_data + total);
+        }
+        return s;
+    }
+    size_t TriangleCount() const { return mat.size(); }
+
+    std::vector<float> pos, nrm, face_n, uvs;
+    std::vector<uint32_t> mat;
+    std::vector<int32_t> light;
+    std::vector<uint32_t> mats;
+    std::vector<int32_t> light_tri;
+    std::vector<float> light_le;
+    std::vector<int32_t> tex_info;
+    std::vector<float> tex_data;
+};
+
+// Scene resident on a device: triangles, BVH (built on the GPU), shading data.
+class DeviceScene {
+public:
+    DeviceScene(const Device& dev, const Scene& s, const gy_bvh_params* bvh = nullptr) {
+        const gy_scene_desc d = s.Desc();
+        Check(gy_scene_create(dev.Handle(), &d, bvh, &h_), "gy_scene_create");
+    }
+    ~DeviceScene() { gy_scene_destroy(h_); }
+    DeviceScene(const DeviceScene&) = delete;
+    DeviceScene& operator=(const DeviceScene&) = delete;
+    gy_scene* Handle() const { return h_; }
+    gy_scene_stats Stats() const {
+        gy_scene_stats st;
+        Check(gy_scene_stats_get(h_, &st), "gy_scene_stats_get");
+        return st;
+    }
+
+private:
+    gy_scene* h_ = nullptr;
+};
+
+// ---------------------------------------------------------------- integrators
+class Integrator {
+public:
+    virtual void Render(const Scene& scene, const Camera& camera) = 0;
+    virtual ~Integrator() {}
+};
+
+// Settings the reference fixes in code: Russian roulette starts after bounce
+// rrStartDepth (5 in PathTracing::Li, integrator/PT.cpp:53).
+struct RenderParams {
+    int rrStartDepth = 5;
+    int mode = GY_RENDER_WAVEFRONT;  // or GY_RENDER_MEGAKERNEL (same film, bit for bit)
+    int samplesPerPass = 0;          // 0 = automatic
+    int blockSize = 0;               // megakernel block size, 0 = automatic
+    int refillThreshold = 0;         // wavefront traversal refill, 0 = automatic
+};
+
+// PathTracing on the GPU.  Render() uploads the scene on first use (or when a
+// different Scene object is passed), renders, and fills camera.GetFilm().
+class HipPathTracing : public Integrator {
+public:
+    HipPathTracing(int samplePerPixel, int maxDepth, const std::shared_ptr<Sampler>& pSampler,
+                   const std::shared_ptr<SamplerEnum>& pSamplerEnum, RenderParams params = RenderParams(),
+                   std::shared_ptr<Device> device = nullptr)
+        : spp(samplePerPixel), mMaxDepth(maxDepth), mpSampler(pSampler), mpSamplerEnum(pSamplerEnum), mParams(params),
+          mpDevice(device ? std::move(device) : std::make_shared<Device>(0)) {
+        if (!mpSampler || std::string(mpSampler->Name()) != "sobol")
+            throw Error(GY_ERR_UNSUPPORTED, "HipPathTracing: the device path implements the Sobol sampler");
+    }
+
+    void Render(const Scene& scene, const Camera& camera) override {
+        if (mpSceneSrc != &scene || !mpScene) {
+            mpScene.reset();
+            mpScene.reset(new DeviceScene(*mpDevice, scene));
+            mpSceneSrc = &scene;
+        }
+        const gy_camera_desc cam = camera.Desc();
+        gy_render_params p = Params();
+        auto film = camera.GetFilm();
+        Check(gy_render(mpScene->Handle(), &cam, &p, film->color.data(), film->weight.data()), "gy_render");
+    }
+
+    gy_render_params Params() const {
+        gy_render_params p = {};
+        p.spp = spp;
+        p.max_depth = mMaxDepth;
+        p.rr_depth = mParams.rrStartDepth;
+        p.mode = mParams.mode;
+        p.samples_per_pass = mParams.samplesPerPass;
+        p.block_size = mParams.blockSize;
+        p.refill_threshold = mParams.refillThreshold;
+        return p;
+    }
+    const Device& GetDevice() const { return *mpDevice; }
+    const DeviceScene* GetDeviceScene() const { return mpScene.get(); }
+
+private:
+    int spp;
+    int mMaxDepth;
+    std::shared_ptr<Sampler> mpSampler;
+    std::shared_ptr<SamplerEnum> mpSamplerEnum;
+    RenderParams mParams;
+    std::shared_ptr<Device> mpDevice;
+    std::unique_ptr<DeviceScene> mpScene;
+    const Scene* mpSceneSrc = nullptr;
+};
+
+}  // namespace gy

@@ -1151,8 +1151,9 @@ template <class R> BSDF<R> make_bsdf(const Scene<R>& s, const Isect<R>& it) {
     b.Tc = V3<R>(m.T[0], m.T[1], m.T[2]);
     if (m.tex_albedo >= 0) {
         const Texture<R>& t = s.tex[m.tex_albedo];
-        b.Rc = V3<R>(tex_bilinear(t, it.u, it.v, 0), tex_bilinear(t, it.u, it.v, t.c > 1 ? 1 : 0),
-                     tex_bilinear(t, it.u, it.v, t.c > 2 ? 2 : 0));
+        // channels past the texture's last one read the last one
+        b.Rc = V3<R>(tex_bilinear(t, it.u, it.v, 0), tex_bilinear(t, it.u, it.v, std::min(1, t.c - 1)),
+                     tex_bilinear(t, it.u, it.v, std::min(2, t.c - 1)));
     }
     R ax = m.ax, ay = m.ay;
     if (m.tex_rough >= 0) {
@@ -1302,13 +1303,30 @@ template <class R> V3<R> direct_illumination(const Scene<R>& s, const Isect<R>& 
 // call (PT.cpp:38) and of Vec3(rand(), rand(), rand()) (PT.cpp:47) are drawn in
 // the order written, left to right (the canonical order of this build; C++
 // leaves argument evaluation order unspecified).
+// Test-only path record: one row per bounce (see or_li_path).
+constexpr int kRecWords = 16;
+static thread_local double* t_rec = nullptr;
+static thread_local int t_rec_max = 0, t_rec_n = 0;
+
 template <class R> V3<R> Li(const Scene<R>& s, Ray<R> ray, Seq& rnd, int maxDepth, int rrDepth, bool use_bvh) {
     bool delta = false;
     V3<R> T(1, 1, 1);
     V3<R> L;
     for (int i = 0; i < maxDepth; ++i) {
         Hit<R> h;
-        if (!intersect(s, ray, &h, use_bvh)) break;
+        const bool hit = intersect(s, ray, &h, use_bvh);
+        if (t_rec && t_rec_n < t_rec_max) {
+            double* r = t_rec + (size_t)kRecWords * t_rec_n++;
+            r[0] = ray.o.x; r[1] = ray.o.y; r[2] = ray.o.z;
+            r[3] = ray.d.x; r[4] = ray.d.y; r[5] = ray.d.z;
+            r[6] = ray.tmin; r[7] = ray.tmax;
+            r[8] = hit ? h.prim : -1; r[9] = hit ? h.t : 0;
+            r[10] = rnd.cursor;
+            r[11] = T.x; r[12] = T.y; r[13] = T.z;
+            r[14] = hit ? s.tris[h.prim].mat : -1;
+            r[15] = delta ? 1 : 0;
+        }
+        if (!hit) break;
         Isect<R> it = query(s, ray, h);
         BSDF<R> bsdf = make_bsdf(s, it);
         int light = s.tris[it.prim].light;
@@ -1463,6 +1481,10 @@ template <class R> int li_samples_t(SceneHandle* h, const or_camera_desc* cd, co
             }
         }
     };
+    if (nth <= 1) {
+        work();  // on the calling thread (the path recorder is thread-local)
+        return 0;
+    }
     std::vector<std::thread> th;
     for (int t = 0; t < nth; ++t) th.emplace_back(work);
     for (auto& t : th) t.join();
@@ -1668,6 +1690,27 @@ int or_li_samples(void* s, const or_camera_desc* cam, const or_render_params* p,
     return li_samples_t<float>(h, cam, p, n, x, y, smp, out_L, out_raster, out_dims);
 }
 
+// Test-only: the oracle's path for one camera sample, one row of kRecWords
+// doubles per bounce: ray origin, direction, tmin, tmax, hit primitive (-1
+// miss), t, sample dimensions used before the bounce, throughput, material id,
+// delta flag.  Returns the number of rows written.
+int or_li_path(void* hv, const or_camera_desc* cam, const or_render_params* p, int x, int y, int smp, int max_rows,
+               double* rows) {
+    t_rec = rows;
+    t_rec_max = max_rows;
+    t_rec_n = 0;
+    double L[3];
+    float raster[2];
+    int32_t dims;
+    or_render_params q = *p;
+    q.nthreads = 1;
+    int32_t xs = x, ys = y, ss = smp;
+    or_li_samples(hv, cam, &q, 1, &xs, &ys, &ss, L, raster, &dims);
+    const int n = t_rec_n;
+    t_rec = nullptr;
+    t_rec_max = t_rec_n = 0;
+    return n;
+}
 int or_set_shadow_short(int on) {
     g_shadow_short = on;
     return 0;
@@ -1760,6 +1803,16 @@ int or_fr_conductor(double cosI, const double* eta, const double* k, double* out
 double or_ggx_D(double ax, double ay, const double* m) {
     GGX<double> g{ax, ay};
     return g.D(V3<double>(m[0], m[1], m[2]));
+}
+// ImageTexture bilinear lookup in FP32 (texture/ImageTexture.h:50-69).
+int or_tex_lookup(int w, int h, int c, const float* data, int n, const float* u, const float* v, int ch, float* out) {
+    Texture<float> t;
+    t.w = w;
+    t.h = h;
+    t.c = c;
+    t.data.assign(data, data + (size_t)w * h * c);
+    for (int i = 0; i < n; ++i) out[i] = tex_bilinear(t, u[i], v[i], ch);
+    return 0;
 }
 
 }  // extern "C"

@@ -124,9 +124,12 @@ int or_bsdf_sample(const uint32_t* mat_words, int32_t n, const double* wo, const
 double or_fr_dielectric(double cosI, double etaI, double etaT);
 int or_fr_conductor(double cosI, const double* eta, const double* k, double* out);
 double or_ggx_D(double ax, double ay, const double* m);
+int or_tex_lookup(int w, int h, int c, const float* data, int n, const float* u, const float* v, int ch, float* out);
 /* Test-only switch: end NEE shadow segments ShadowRayEps before the light point
  * (the reference's segment reaches it, see spawn_to in gy_oracle.cpp). */
 int or_set_shadow_short(int on);
+int or_li_path(void* scene, const or_camera_desc* cam, const or_render_params* p, int x, int y, int smp, int max_rows,
+               double* rows);
 
 #ifdef __cplusplus
 }

@@ -94,6 +94,8 @@ def lib():
         "or_fr_dielectric": (C.c_double, [C.c_double, C.c_double, C.c_double]),
         "or_fr_conductor": (C.c_int, [C.c_double, vp, vp, vp]),
         "or_ggx_D": (C.c_double, [C.c_double, C.c_double, vp]),
+        "or_li_path": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp]),
+        "or_tex_lookup": (C.c_int, [C.c_int, C.c_int, C.c_int, vp, C.c_int, vp, vp, C.c_int, vp]),
         "or_set_shadow_short": (C.c_int, [C.c_int]),
     }
     for name, (res, args) in sig.items():
@@ -224,6 +226,15 @@ class OracleScene:
         assert rc == 0
         return L, raster, dims
 
+    def li_path(self, cam, prm, x, y, s, max_rows=64):
+        """Test-only: per-bounce record of one sample's path (16 doubles per
+        bounce: origin, direction, tmin, tmax, hit prim or -1, t, dims used
+        before the bounce, throughput, material id, delta flag)."""
+        c = camera_from(cam)
+        rows = np.zeros((max_rows, 16), np.float64)
+        n = lib().or_li_path(self.h, C.byref(c), C.byref(prm), int(x), int(y), int(s), max_rows, _p(rows))
+        return rows[:n]
+
     def render(self, cam, prm, row0=0, row1=0):
         c = camera_from(cam)
         color = np.zeros((cam.height, cam.width, 3), np.float64)
@@ -286,6 +297,17 @@ def fr_conductor(c, eta, k):
 def ggx_D(ax, ay, m):
     m = np.ascontiguousarray(m, np.float64)
     return lib().or_ggx_D(ax, ay, m.ctypes.data)
+
+
+def tex_lookup(data, u, v, ch):
+    """Bilinear lookup (CLAMP) of an (h, w, c) float32 texture at (u, v)."""
+    data = np.ascontiguousarray(data, np.float32)
+    h, w, c = data.shape
+    u = np.ascontiguousarray(u, np.float32)
+    v = np.ascontiguousarray(v, np.float32)
+    out = np.zeros(u.shape[0], np.float32)
+    lib().or_tex_lookup(w, h, c, data.ctypes.data, u.shape[0], u.ctypes.data, v.ctypes.data, ch, out.ctypes.data)
+    return out
 
 
 def set_shadow_short(on):

@@ -146,3 +146,83 @@ def check_bvh(nodes, order, pos, max_leaf):
     assert (covered == 1).all(), "triangle leaf counts range %d..%d" % (covered.min(), covered.max())
     assert (reached == 1).all(), "nodes reached %d..%d times" % (reached.min(), reached.max())
     return leaf_depth
+
+
+def mixed_material_scene(seed=0, res=64):
+    """Cornell-style room exercising every material path of the device shader:
+    textured Lambert walls (RGB and one-channel albedo textures with explicit
+    uvs), a smooth glass box (GlassMaterial), a rough dielectric box
+    (RoughDielectricMaterial, roughness -> alpha), a rough conductor box with
+    FrConductor(eta, k) and a roughness texture, and a quad area light."""
+    import gyunity_amd as g
+
+    rng = np.random.default_rng(seed)
+    tex_rgb = rng.uniform(0.1, 0.9, (16, 16, 3)).astype(np.float32)
+    tex_one = rng.uniform(0.2, 0.8, (8, 8, 1)).astype(np.float32)
+    tex_rough = rng.uniform(0.05, 0.8, (8, 8, 1)).astype(np.float32)
+    texs = [tex_rgb, tex_one, tex_rough]
+    tex_info, off = [], 0
+    for t in texs:
+        tex_info.append((t.shape[1], t.shape[0], t.shape[2], off))
+        off += t.size
+    tex_data = np.concatenate([t.reshape(-1) for t in texs])
+    mats = [
+        material(0, R=(0.73, 0.73, 0.73)),                          # 0 white
+        material(0, R=(1, 1, 1), tex_albedo=0),                     # 1 RGB texture
+        material(0, R=(1, 1, 1), tex_albedo=1),                     # 2 one-channel texture
+        material(0, R=(0.14, 0.45, 0.091)),                         # 3 green
+        material(2, R=(1, 1, 1), T=(1, 1, 1), eta1=1.0, eta2=1.5),  # 4 glass
+        material(3, R=(1, 1, 1), T=(1, 1, 1), alpha=(0.3, 0.3), alpha_direct=False, eta1=1.0, eta2=1.5),  # 5 rough dielectric
+        material(1, R=(1, 1, 1), alpha=(0.3, 0.3), alpha_direct=False, eta=(0.143, 0.374, 1.442),
+                 k=(3.983, 2.385, 1.603), tex_rough=2),             # 6 gold-like conductor, roughness texture
+        material(0, R=(0, 0, 0)),                                   # 7 light (black)
+    ]
+    quads = []  # (a, b, c, d, normal, material, emissive)
+
+    def quad(a, b, c, d, n, m, emit=False):
+        quads.append((a, b, c, d, n, m, emit))
+
+    quad((0, 0, 0), (1, 0, 0), (1, 0, 1), (0, 0, 1), (0, 1, 0), 0)        # floor
+    quad((0, 1, 0), (0, 1, 1), (1, 1, 1), (1, 1, 0), (0, -1, 0), 0)       # ceiling
+    quad((0, 0, 0), (0, 1, 0), (1, 1, 0), (1, 0, 0), (0, 0, 1), 2)        # back, one-channel texture
+    quad((0, 0, 0), (0, 0, 1), (0, 1, 1), (0, 1, 0), (1, 0, 0), 1)        # left, RGB texture
+    quad((1, 0, 0), (1, 1, 0), (1, 1, 1), (1, 0, 1), (-1, 0, 0), 3)       # right, green
+    quad((0.375, 0.99, 0.375), (0.625, 0.99, 0.375), (0.625, 0.99, 0.625), (0.375, 0.99, 0.625), (0, -1, 0), 7, True)
+
+    def box(lo, hi, m):
+        x0, y0, z0 = lo
+        x1, y1, z1 = hi
+        quad((x0, y0, z0), (x1, y0, z0), (x1, y0, z1), (x0, y0, z1), (0, -1, 0), m)
+        quad((x0, y1, z0), (x0, y1, z1), (x1, y1, z1), (x1, y1, z0), (0, 1, 0), m)
+        quad((x0, y0, z0), (x0, y0, z1), (x0, y1, z1), (x0, y1, z0), (-1, 0, 0), m)
+        quad((x1, y0, z0), (x1, y1, z0), (x1, y1, z1), (x1, y0, z1), (1, 0, 0), m)
+        quad((x0, y0, z0), (x0, y1, z0), (x1, y1, z0), (x1, y0, z0), (0, 0, -1), m)
+        quad((x0, y0, z1), (x1, y0, z1), (x1, y1, z1), (x0, y1, z1), (0, 0, 1), m)
+
+    # The boxes float 1 mm above the floor: a face coplanar with the floor
+    # would make every ray leaving a dielectric box downwards an exact-t tie,
+    # which the reference resolves in its BVH's visit order.
+    box((0.12, 0.001, 0.55), (0.36, 0.3, 0.8), 4)   # glass
+    box((0.42, 0.001, 0.2), (0.62, 0.45, 0.42), 5)  # rough dielectric
+    box((0.66, 0.001, 0.55), (0.88, 0.25, 0.78), 6)  # rough conductor
+    pos, fn, uv, mat, light, ltri = [], [], [], [], [], []
+    for a, b, c, d, n, m, emit in quads:
+        for tri, tuv in (((a, b, c), (0, 0, 1, 0, 1, 1)), ((a, c, d), (0, 0, 1, 1, 0, 1))):
+            if emit:
+                light.append(len(ltri))
+                ltri.append(len(pos))
+            else:
+                light.append(-1)
+            pos.append(np.array(tri, np.float32).reshape(9))
+            fn.append(n)
+            uv.append(tuv)
+            mat.append(m)
+    pos = np.array(pos, np.float32)
+    fn = np.array(fn, np.float32)
+    nrm = np.repeat(fn, 3, axis=0).reshape(-1, 9)
+    sa = SceneArrays(pos, nrm, fn, np.array(mat, np.uint32), np.array(light, np.int32), np.stack(mats),
+                     np.array(ltri, np.int32), np.tile(np.array([17, 12, 4], np.float32), (len(ltri), 1)),
+                     uv=np.array(uv, np.float32), tex_info=np.array(tex_info, np.int32), tex_data=tex_data)
+    cam = g.camera((0.5, 0.5, 2.35), (0.5, 0.5, 0.0), (0, 1, 0), 40.0, res, res)
+    prm = g.render_params(spp=16, max_depth=6, rr_depth=3)
+    return sa, cam, prm

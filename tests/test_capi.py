@@ -77,3 +77,16 @@ def test_generated_scenes_are_well_formed():
     sc, _, _ = g.generate_scene(g.GY_SCENE_ICOSPHERE_GGX, 0, 3)
     sd, _, _ = g.generate_scene(g.GY_SCENE_ICOSPHERE_GGX, 1, 3)
     assert np.array_equal(sb.pos, sc.pos) and not np.array_equal(sb.pos, sd.pos)
+
+
+def test_cpp_integrator_adapter_without_device():
+    # gy::HipPathTracing (the Integrator adapter) turns a failing gy_open into a
+    # gy::Error; the example binary checks exactly that when no GPU is present.
+    import subprocess
+    exe = os.path.join(ROOT, "gyunity_amd", "render_cornell")
+    if not os.path.exists(exe):
+        pytest.skip("render_cornell not built (make -C gyunity_amd)")
+    r = subprocess.run([exe, "--expect-no-device"], capture_output=True, text=True, timeout=60,
+                       env=dict(os.environ, HIP_VISIBLE_DEVICES="-1"))
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "no device" in r.stdout

@@ -238,6 +238,112 @@ def test_glass_energy_split():
     assert np.allclose(pdf[~refl], 0.96)
 
 
+def _sphere_grid(n):
+    th = (np.arange(2 * n) + 0.5) / (2 * n) * np.pi
+    ph = (np.arange(2 * n) + 0.5) / (2 * n) * (2 * np.pi)
+    T, P = np.meshgrid(th, ph, indexing="ij")
+    w = np.stack([np.sin(T) * np.cos(P), np.sin(T) * np.sin(P), np.cos(T)], -1).reshape(-1, 3)
+    dA = (np.sin(T) * (np.pi / (2 * n)) * (2 * np.pi / (2 * n))).reshape(-1)
+    return w, dA
+
+
+def _rough_transmission_valid(wo0, grid, eta):
+    # Transmission configurations the sampler can produce: the half vector the
+    # reference's Pdf() forms, -(wo + eta wi) flipped up, faces wo and not wi.
+    wh = -(wo0[None, :] + grid * eta)
+    wh /= np.linalg.norm(wh, axis=1, keepdims=True)
+    wh *= np.sign(wh[:, 2:3])
+    return (wh @ wo0 > 0) & (np.sum(grid * wh, 1) < 0)
+
+
+def test_rough_dielectric_sampling_chi2():
+    # RoughDielectricBSDF (bsdf/BSDF.h:448-616): sampled directions over the
+    # whole sphere (reflection and refraction) against the integrated pdf of
+    # Pdf().  The reference's transmission Pdf() does not reject half vectors
+    # the visible-normal sampler never produces (wo.wh < 0 or wi.wh > 0), and
+    # the VNDF pdf takes |wo.wh|, so Pdf() integrates above the sampled share
+    # by exactly that mass; restricted to realisable configurations it equals
+    # the sampler's density, and the excess is positive (the reference's own
+    # behaviour, kept).
+    eta = 1.5
+    w = material(3, alpha=(0.35, 0.35), eta1=1.0, eta2=eta)
+    rng = np.random.default_rng(11)
+    n = 80000
+    ct = 0.7
+    wo0 = np.array([np.sqrt(1 - ct * ct), 0.0, ct])
+    wi, f, pdf = O.bsdf_sample(w, np.tile(wo0, (n, 1)), rng.random((n, 3)))
+    keep = pdf > 0
+    wi = wi[keep]
+    grid, dA = _sphere_grid(64)  # 128 x 128 cells, nested in the 16 x 16 bins
+    fg, pg = O.bsdf_eval(w, np.tile(wo0, (grid.shape[0], 1)), grid)
+    real = (grid[:, 2] > 0) | _rough_transmission_valid(wo0, grid, eta)
+    excess = (pg * dA)[~real].sum()
+    assert excess > 0.01
+    assert abs((pg * dA)[real].sum() - keep.mean()) < 0.005, ((pg * dA)[real].sum(), keep.mean())
+    nth, nph = 16, 16
+    th = np.arccos(np.clip(wi[:, 2], -1, 1))
+    ph = np.mod(np.arctan2(wi[:, 1], wi[:, 0]), 2 * np.pi)
+    hist, _, _ = np.histogram2d(th, ph, bins=[nth, nph], range=[[0, np.pi], [0, 2 * np.pi]])
+    prob = np.where(real, pg * dA, 0).reshape(128, 128).reshape(nth, 8, nph, 8).sum(axis=(1, 3))
+    expect = prob / prob.sum() * keep.sum()
+    m = expect > 20
+    chi2 = (((hist - expect) ** 2) / expect)[m].sum()
+    dof = m.sum() - 1
+    assert chi2 < dof + 6 * np.sqrt(2 * dof), (chi2, dof)
+    # sample weights f|cos|/pdf integrate what Evaluate() returns over the
+    # realisable directions
+    wt = f[keep, 0] * np.abs(wi[:, 2]) / pdf[keep]
+    integral = (np.where(real, fg[:, 0], 0) * np.abs(grid[:, 2]) * dA).sum()
+    assert abs(wt.sum() / n - integral) < 4 * wt.std() / np.sqrt(n) + 5e-3, (wt.sum() / n, integral)
+
+
+def test_rough_dielectric_tends_to_smooth_glass():
+    # As alpha -> 0 the rough dielectric's reflected share becomes the Fresnel
+    # reflectance (FrDielectric(1, 1, 1.5) = 0.04 at normal incidence) and its
+    # transmitted weight the smooth GlassMaterial's (material/Glass.h:30-34).
+    n = 40000
+    rng = np.random.default_rng(12)
+    wo = np.tile([0.0, 0.0, 1.0], (n, 1))
+    rough = material(3, alpha=(0.002, 0.002), eta1=1.0, eta2=1.5)
+    wi, f, pdf = O.bsdf_sample(rough, wo, rng.random((n, 3)))
+    ok = pdf > 0
+    wt = np.zeros(n)
+    wt[ok] = f[ok, 0] * np.abs(wi[ok, 2]) / pdf[ok]
+    refl = wi[:, 2] > 0
+    e_refl = wt[refl].sum() / n
+    e_tran = wt[~refl].sum() / n
+    gw, gf, gp = O.bsdf_sample(material(2, eta1=1.0, eta2=1.5), wo, rng.random((n, 3)))
+    gt = ~(gw[:, 2] > 0)
+    g_tran = (gf[gt, 0] * np.abs(gw[gt, 2]) / gp[gt]).sum() / n
+    assert abs(e_refl - 0.04) < 0.006, e_refl
+    assert abs(e_tran - g_tran) < 0.02 * g_tran, (e_tran, g_tran)
+
+
+def test_texture_bilinear_affine_and_clamp():
+    # ImageTexture::Triangle (texture/ImageTexture.h:50-69): s = w*u - 0.5,
+    # bilinear between texel centres, CLAMP outside.  An affine texture is
+    # reproduced exactly between the outer texel centres; outside it, the edge
+    # texels hold.
+    h, w_ = 8, 16
+    jj, ii = np.meshgrid(np.arange(w_), np.arange(h))
+    a, b, c = 0.25, 0.125, 0.0625
+    tex = np.stack([a + b * jj + c * ii, 1.0 + 0 * jj, (jj == 3) * 1.0], -1).astype(np.float32)
+    rng = np.random.default_rng(13)
+    u = rng.uniform(0.5 / w_, 1 - 0.5 / w_, 2000).astype(np.float32)
+    v = rng.uniform(0.5 / h, 1 - 0.5 / h, 2000).astype(np.float32)
+    got = O.tex_lookup(tex, u, v, 0)
+    s = w_ * u.astype(np.float64) - 0.5
+    t = h * v.astype(np.float64) - 0.5
+    assert np.allclose(got, a + b * s + c * t, rtol=0, atol=2e-6)
+    # clamp: left of the first texel centre the first column holds
+    uc = np.full(50, 0.2 / w_, np.float32)
+    vc = np.linspace(0.5 / h, 1 - 0.5 / h, 50).astype(np.float32)
+    assert np.allclose(O.tex_lookup(tex, uc, vc, 0), a + c * (h * vc.astype(np.float64) - 0.5), atol=2e-6)
+    # a single texel's hat function: peak 1 at its centre, 0.5 half a texel away
+    uu = np.array([3.5 / w_, 4.0 / w_, 5.5 / w_], np.float32)
+    assert np.allclose(O.tex_lookup(tex, uu, np.full(3, 0.5, np.float32), 2), [1.0, 0.5, 0.0], atol=1e-6)
+
+
 # ---------------------------------------------------------------- integrator closed form
 @pytest.mark.parametrize("albedo,depth", [(0.5, 4), (0.8, 3)])
 def test_furnace_closed_form(albedo, depth):
