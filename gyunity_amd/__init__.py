@@ -18,8 +18,12 @@ from ._lib import (  # noqa: F401
     render_params,
     sobol_enum_index,
     sobol_sample,
+    halton_sample,
+    halton_enum_index,
     GY_SCENE_CORNELL,
     GY_SCENE_ICOSPHERE_GGX,
     GY_RENDER_MEGAKERNEL,
     GY_RENDER_WAVEFRONT,
+    GY_SAMPLER_SOBOL,
+    GY_SAMPLER_HALTON,
 )

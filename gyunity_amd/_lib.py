@@ -22,6 +22,8 @@ GY_SCENE_CORNELL = 0
 GY_SCENE_ICOSPHERE_GGX = 1
 GY_RENDER_MEGAKERNEL = 0
 GY_RENDER_WAVEFRONT = 1
+GY_SAMPLER_SOBOL = 0
+GY_SAMPLER_HALTON = 1
 FAULT_NONE, FAULT_ALLOC, FAULT_LAUNCH, FAULT_SCENE = 0, 1, 2, 3
 
 
@@ -66,7 +68,7 @@ class RenderParams(C.Structure):
         ("samples_per_pass", C.c_int32),
         ("block_size", C.c_int32),
         ("refill_threshold", C.c_int32),
-        ("reserved", C.c_int32),
+        ("sampler", C.c_int32),
     ]
 
 
@@ -144,6 +146,8 @@ def lib():
         "gy_count_traffic": (C.c_int, [vp, P(CameraDesc), P(RenderParams), P(RenderStats)]),
         "gy_copy_probe": (C.c_int, [vp, i64, P(C.c_double)]),
         "gy_sobol_sample": (f32, [u64, u32]),
+        "gy_halton_sample": (f32, [u64, u32]),
+        "gy_halton_enum_index": (u64, [u32, u32, u64, u32, u32]),
         "gy_sobol_enum_index": (u64, [u32, u32, u32, u32]),
         "gy_camera_matrix": (C.c_int, [P(CameraDesc), P(f32)]),
         "gy_generate_scene": (C.c_int, [C.c_int, u32, i32, P(vp)]),
@@ -250,11 +254,20 @@ def camera(pos, look_at, up, fovy, width, height, film_dist=1.0):
     return c
 
 
-def render_params(spp, max_depth, rr_depth=5, mode=GY_RENDER_MEGAKERNEL, samples_per_pass=0, block_size=0):
+def render_params(spp, max_depth, rr_depth=5, mode=GY_RENDER_MEGAKERNEL, samples_per_pass=0, block_size=0,
+                  sampler=GY_SAMPLER_SOBOL):
     p = RenderParams()
     p.spp, p.max_depth, p.rr_depth, p.mode = spp, max_depth, rr_depth, mode
-    p.samples_per_pass, p.block_size = samples_per_pass, block_size
+    p.samples_per_pass, p.block_size, p.sampler = samples_per_pass, block_size, sampler
     return p
+
+
+def halton_sample(index, dim):
+    return lib().gy_halton_sample(index, dim)
+
+
+def halton_enum_index(width, height, i, x, y):
+    return lib().gy_halton_enum_index(width, height, i, x, y)
 
 
 def sobol_sample(index, dim):

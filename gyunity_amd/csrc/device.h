@@ -77,24 +77,73 @@ GY_D float sobol_sample(const uint32_t* __restrict__ M, uint64_t index, uint32_t
     }
     return (float)r * 2.3283064365386963e-10f;
 }
+// HaltonSampler::sample with Faure permutations (sampler/Halton.h:648-910,
+// 1441-3308): base 2 writes the bit-reversed index into a float mantissa;
+// other bases sum digit-table entries over groups of k digits (host tables,
+// halton.cpp) and scale the integer by float((1 - 2^-23) / tsize^groups).
+struct SampleSrc {
+    const uint32_t* __restrict__ sobol;  // Sobol matrices, 52 columns per dimension
+    const uint16_t* __restrict__ htab;   // Halton digit tables
+    const uint4* __restrict__ hdim;      // per dimension: base, table size, groups, table offset
+    const float* __restrict__ hscale;    // per dimension: scale
+    int32_t kind;                        // GY_SAMPLER_SOBOL / GY_SAMPLER_HALTON
+};
+GY_D float halton_sample(const SampleSrc& M, uint64_t index, uint32_t dim) {
+    if (dim == 0) return __uint_as_float(0x3f800000u | (uint32_t)(__brevll(index) >> 41)) - 1.f;
+    if (dim >= 256u) return 0.f;
+    const uint4 di = M.hdim[dim];
+    const uint16_t* t = M.htab + di.w;
+    const uint32_t ts = di.y;
+    uint32_t sum = 0;  // < 2^32 by construction of the groups
+    if ((index >> 32) == 0) {
+        uint32_t x = (uint32_t)index;
+        for (uint32_t g = 0; g < di.z; ++g) {
+            sum = sum * ts + t[x % ts];
+            x /= ts;
+        }
+    } else {
+        uint64_t x = index;
+        for (uint32_t g = 0; g < di.z; ++g) {
+            sum = sum * ts + t[x % ts];
+            x /= ts;
+        }
+    }
+    return (float)sum * M.hscale[dim];
+}
 struct Seq {
     uint64_t index;
     uint32_t cursor;
 };
-GY_D float seq_next(const uint32_t* __restrict__ M, Seq& s) {
-    float v = sobol_sample(M, s.index, s.cursor++);
+// RandomStateSequence::Sample (sampler/Sampler.h:53-64): values >= 1 wrap to 0.
+GY_D float seq_next(const SampleSrc& M, Seq& s) {
+    float v = M.kind == 1 ? halton_sample(M, s.index, s.cursor) : sobol_sample(M.sobol, s.index, s.cursor);
+    ++s.cursor;
     if (v >= 1.f) v = 0.f;
     return v;
 }
 
-// SobolEnum::GetIndex (sampler/SobolEnum.h:24-28), derived tables (sobol.cpp).
+// SobolEnum::GetIndex (sampler/SobolEnum.h:24-28) from derived tables
+// (sobol.cpp), or HaltonEnum::get_index (sampler/HaltonEnum.h:96-103).  sx, sy
+// scale a sample in [0,1) to the enumeration grid (SampleX / SampleY).
 struct EnumDev {
     uint32_t m;
-    uint32_t pad;
+    uint32_t kind;
     uint64_t inv[32];
     uint64_t frame[32];
+    uint32_t p2, p3;
+    uint64_t hmx, hmy, hinc;
+    float sx, sy;
 };
 GY_D uint64_t enum_index(const EnumDev& E, uint32_t frame, uint32_t px, uint32_t py) {
+    if (E.kind == 1) {
+        const uint64_t hx = E.p2 ? (__brevll((uint64_t)px) >> (64 - E.p2)) : 0ull;
+        uint32_t hy = 0, y = py;
+        for (uint32_t d = 0; d < E.p3; ++d) {
+            hy = hy * 3 + y % 3;
+            y /= 3;
+        }
+        return (hx * E.hmx + (uint64_t)hy * E.hmy) % E.hinc + (uint64_t)frame * E.hinc;
+    }
     const uint32_t m = E.m;
     uint64_t delta = 0;
     for (uint32_t j = 0, f = frame; f; f >>= 1, ++j)

@@ -28,6 +28,24 @@ float Sample(uint64_t index, uint32_t dim);
 uint64_t EnumIndex(uint32_t m, uint32_t frame, uint32_t px, uint32_t py);
 }  // namespace sobol
 
+namespace halton {
+constexpr int kDims = 256;  // HaltonSampler::sample covers dimensions 0..255
+struct DimInfo {
+    uint32_t base, tsize, groups, offset;  // prime, digit-table size, digit groups, table offset
+    float scale;                           // float((1 - 2^-23) / tsize^groups)
+};
+const std::vector<DimInfo>& Dims();
+const std::vector<uint16_t>& Tables();
+float Sample(uint64_t index, uint32_t dim);
+struct EnumParams {
+    uint32_t p2, p3;        // 2^p2 >= width, 3^p3 >= height
+    uint64_t mx, my, inc;   // CRT multipliers, 2^p2 * 3^p3
+    float sx, sy;           // 2^p2, 3^p3
+};
+EnumParams Enum(uint32_t width, uint32_t height);
+uint64_t EnumIndex(const EnumParams& e, uint64_t i, uint32_t x, uint32_t y);
+}  // namespace halton
+
 // Camera matrix in the reference's FP32 operation order (camera/Camera.cpp:6-38).
 struct CameraRT {
     float raster_to_world[16];  // column-major, (i, j) at [j*4 + i]

@@ -261,12 +261,13 @@ __device__ V3 direct_illumination(const SceneDev& S, const Isect& it, const BSDF
 template <int STACK, bool COUNT>
 __device__ V3 trace_path(const SceneDev& S, const PathParams& P, int px, int py, int s, uint32_t* stack, int stride,
                          float* fx, float* fy, uint32_t* dims, PathCounters& pc) {
-    const uint32_t* M = P.sobol;
+    const SampleSrc& M = P.smp;
     Seq q;
     q.index = enum_index(P.en, (uint32_t)s, (uint32_t)px, (uint32_t)py);
     q.cursor = 0;
-    const float u = P.cam.res * seq_next(M, q) - (float)px;  // SobolEnum::SampleX (sampler/SobolEnum.h:30-32)
-    const float v = P.cam.res * seq_next(M, q) - (float)py;
+    // SampleX / SampleY (sampler/SobolEnum.h:30-36, sampler/HaltonEnum.h:30-46)
+    const float u = P.en.sx * seq_next(M, q) - (float)px;
+    const float v = P.en.sy * seq_next(M, q) - (float)py;
     V3 o, d;
     generate_ray(P.cam, px, py, u, v, &o, &d);
     bool delta = false;
@@ -782,21 +783,41 @@ int MakePathParams(Ctx* c, Scene* sc, const gy_camera_desc& cam, const gy_render
     P->cam.height = rt.height;
     P->cam.res = rt.res;
     P->cam.log2res = rt.log2res;
-    const sobol::EnumTables& E = sobol::Enum(rt.log2res);
-    P->en.m = (uint32_t)rt.log2res;
-    P->en.pad = 0;
-    for (int i = 0; i < 32; ++i) {
-        P->en.inv[i] = E.inv[i];
-        P->en.frame[i] = E.frame[i];
+    std::memset(&P->en, 0, sizeof(P->en));
+    P->en.kind = (uint32_t)p.sampler;
+    if (p.sampler == GY_SAMPLER_HALTON) {
+        const halton::EnumParams h = halton::Enum((uint32_t)rt.width, (uint32_t)rt.height);
+        P->en.p2 = h.p2;
+        P->en.p3 = h.p3;
+        P->en.hmx = h.mx;
+        P->en.hmy = h.my;
+        P->en.hinc = h.inc;
+        P->en.sx = h.sx;
+        P->en.sy = h.sy;
+    } else {
+        const sobol::EnumTables& E = sobol::Enum(rt.log2res);
+        P->en.m = (uint32_t)rt.log2res;
+        for (int i = 0; i < 32; ++i) {
+            P->en.inv[i] = E.inv[i];
+            P->en.frame[i] = E.frame[i];
+        }
+        P->en.sx = P->en.sy = (float)rt.res;
     }
-    P->sobol = c->d_sobol;
+    P->smp.sobol = c->d_sobol;
+    P->smp.htab = (const uint16_t*)c->d_halton;
+    P->smp.hdim = (const uint4*)((const char*)c->d_halton + c->halton_dim_off);
+    P->smp.hscale = (const float*)((const char*)c->d_halton + c->halton_scale_off);
+    P->smp.kind = p.sampler;
     P->max_depth = p.max_depth;
     P->rr_depth = p.rr_depth;
-    // dimensions a path may consume: 2 + depth * (6 + 3 + 1)
-    const int64_t dims = 2 + (int64_t)p.max_depth * 10;
-    if (dims > sobol::kDims) return Fail(GY_ERR_UNSUPPORTED, "max_depth needs more Sobol dimensions than the table holds");
-    const int bits = 2 * rt.log2res + (p.spp > 1 ? 32 - __builtin_clz((unsigned)(p.spp - 1)) : 0);
-    if (bits > sobol::kCols) return Fail(GY_ERR_UNSUPPORTED, "sample index beyond 52 bits");
+    if (p.sampler == GY_SAMPLER_SOBOL) {
+        // dimensions a path may consume: 2 + depth * (6 + 3 + 1)
+        const int64_t dims = 2 + (int64_t)p.max_depth * 10;
+        if (dims > sobol::kDims) return Fail(GY_ERR_UNSUPPORTED, "max_depth needs more Sobol dimensions than the table holds");
+        const int bits = 2 * rt.log2res + (p.spp > 1 ? 32 - __builtin_clz((unsigned)(p.spp - 1)) : 0);
+        if (bits > sobol::kCols) return Fail(GY_ERR_UNSUPPORTED, "sample index beyond 52 bits");
+    }
+    // Halton: dimensions past 255 read 0 as HaltonSampler::sample returns
     (void)sc;
     return GY_OK;
 }
@@ -805,6 +826,7 @@ int ValidateRender(const gy_camera_desc* cam, const gy_render_params* p) {
     if (!cam || !p) return Fail(GY_ERR_INVALID, "render: null camera or params");
     if (p->spp <= 0 || p->max_depth < 0 || p->max_depth > 100) return Fail(GY_ERR_INVALID, "render: spp > 0 and 0 <= max_depth <= 100");
     if ((int64_t)cam->width * cam->height > (1ll << 26)) return Fail(GY_ERR_UNSUPPORTED, "render: image larger than 2^26 pixels");
+    if (p->sampler != GY_SAMPLER_SOBOL && p->sampler != GY_SAMPLER_HALTON) return Fail(GY_ERR_INVALID, "render: unknown sampler");
     return GY_OK;
 }
 
@@ -885,6 +907,28 @@ extern "C" int gy_open(int device, gy_context** out) {
         return rc;
     }
     c->d_sobol = (const uint32_t*)p;
+    {
+        // Halton tables: digit tables (u16), per-dimension info (uint4), scales
+        const auto& tab = halton::Tables();
+        const auto& dims = halton::Dims();
+        const size_t tab_bytes = (tab.size() * 2 + 255) & ~(size_t)255;
+        c->halton_dim_off = tab_bytes;
+        c->halton_scale_off = tab_bytes + dims.size() * 16;
+        std::vector<char> blob(c->halton_scale_off + dims.size() * 4);
+        std::memcpy(blob.data(), tab.data(), tab.size() * 2);
+        for (size_t d = 0; d < dims.size(); ++d) {
+            const uint32_t info[4] = {dims[d].base, dims[d].tsize, dims[d].groups, dims[d].offset};
+            std::memcpy(blob.data() + c->halton_dim_off + 16 * d, info, 16);
+            std::memcpy(blob.data() + c->halton_scale_off + 4 * d, &dims[d].scale, 4);
+        }
+        rc = Upload(c, &c->d_halton, blob.data(), blob.size());
+        if (rc != GY_OK) {
+            DevFree((void*)c->d_sobol);
+            (void)hipStreamDestroy(c->stream);
+            delete c;
+            return rc;
+        }
+    }
     *out = reinterpret_cast<gy_context*>(c);
     return GY_OK;
 }
@@ -895,6 +939,7 @@ extern "C" int gy_close(gy_context* ctx) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     DevFree((void*)c->d_sobol);
+    DevFree(c->d_halton);
     DevFree(c->rays.ptr);
     DevFree(c->hits.ptr);
     DevFree(c->ctr.ptr);

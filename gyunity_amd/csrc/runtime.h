@@ -31,6 +31,8 @@ struct Ctx {
     int num_cu = 0;
     hipStream_t stream = nullptr;
     const uint32_t* d_sobol = nullptr;
+    void* d_halton = nullptr;  // digit tables, then per-dimension uint4 info and float scales
+    size_t halton_dim_off = 0, halton_scale_off = 0;
     int fault = 0;
     DevBuf rays, hits, ctr;
     double last_kernel_ms = 0.0;
@@ -110,7 +112,7 @@ FilmDev FilmView(Scene* sc);
 struct PathParams {
     gyd::CamDev cam;
     gyd::EnumDev en;
-    const uint32_t* sobol;
+    gyd::SampleSrc smp;
     int32_t max_depth, rr_depth;
 };
 int WavefrontPass(Ctx* c, Scene* sc, const PathParams& P, const gy_render_params& p, int s0, int n_s);

@@ -101,8 +101,8 @@ __global__ void __launch_bounds__(256) k_wf_raygen(WfDev W, PathParams P, int s0
         Seq q;
         q.index = enum_index(P.en, (uint32_t)(s0 + s_local), (uint32_t)px, (uint32_t)py);
         q.cursor = 0;
-        const float u = P.cam.res * seq_next(P.sobol, q) - (float)px;
-        const float v = P.cam.res * seq_next(P.sobol, q) - (float)py;
+        const float u = P.en.sx * seq_next(P.smp, q) - (float)px;
+        const float v = P.en.sy * seq_next(P.smp, q) - (float)py;
         V3 o, d;
         generate_ray(P.cam, px, py, u, v, &o, &d);
         W.o[slot] = make_float4(o.x, o.y, o.z, 0.f);
@@ -320,7 +320,7 @@ __global__ void __launch_bounds__(256) k_wf_shade(SceneDev S, WfDev W, PathParam
     bool push_shadow = false, push_mis = false, push_next = false;
     if (valid) {
         slot = W.q[Q_SHADE0 + MTYPE][i];
-        const uint32_t* M = P.sobol;
+        const SampleSrc& M = P.smp;
         const V3 ro = f3(W.o[slot]), rd = f3(W.d[slot]);
         const float4 hv = W.hit[slot];
         Hit h;

@@ -117,16 +117,26 @@ public:
 };
 
 // ---------------------------------------------------------------- samplers
-// The device implements the Sobol sampler with the Sobol enumeration
-// (unscrambled, sampler/SobolEnum.h:24-25).  The objects carry the choice.
+// The device implements SobolSampler + SobolEnum (unscrambled,
+// sampler/SobolEnum.h:24-25) and HaltonSampler with Faure permutations +
+// HaltonEnum (sampler/Halton.h, sampler/HaltonEnum.h).  The objects carry the
+// choice; the sampler decides it.
 class Sampler {
 public:
     virtual ~Sampler() = default;
-    virtual const char* Name() const = 0;
+    virtual int Kind() const = 0;  // GY_SAMPLER_*
 };
 class SobolSampler : public Sampler {
 public:
-    const char* Name() const override { return "sobol"; }
+    int Kind() const override { return GY_SAMPLER_SOBOL; }
+};
+class HaltonSampler : public Sampler {
+public:
+    HaltonSampler(bool faure = true) {
+        if (!faure)  // init_random shuffles with the STL engine (Halton.h:627-646): not reproducible
+            throw Error(GY_ERR_UNSUPPORTED, "HaltonSampler: only Faure permutations are implemented");
+    }
+    int Kind() const override { return GY_SAMPLER_HALTON; }
 };
 class SamplerEnum {
 public:
@@ -135,6 +145,10 @@ public:
 class SobolEnum : public SamplerEnum {
 public:
     explicit SobolEnum(const Film&) {}
+};
+class HaltonEnum : public SamplerEnum {
+public:
+    HaltonEnum(uint32_t, uint32_t) {}
 };
 
 // ---------------------------------------------------------------- materials
@@ -339,8 +353,7 @@ public:
                    std::shared_ptr<Device> device = nullptr)
         : spp(samplePerPixel), mMaxDepth(maxDepth), mpSampler(pSampler), mpSamplerEnum(pSamplerEnum), mParams(params),
           mpDevice(device ? std::move(device) : std::make_shared<Device>(0)) {
-        if (!mpSampler || std::string(mpSampler->Name()) != "sobol")
-            throw Error(GY_ERR_UNSUPPORTED, "HipPathTracing: the device path implements the Sobol sampler");
+        if (!mpSampler) throw Error(GY_ERR_INVALID, "HipPathTracing: no sampler");
     }
 
     void Render(const Scene& scene, const Camera& camera) override {
@@ -364,6 +377,7 @@ public:
         p.samples_per_pass = mParams.samplesPerPass;
         p.block_size = mParams.blockSize;
         p.refill_threshold = mParams.refillThreshold;
+        p.sampler = mpSampler->Kind();
         return p;
     }
     const Device& GetDevice() const { return *mpDevice; }

@@ -89,8 +89,10 @@ typedef struct {
     int32_t block_size;      /* megakernel block size: 0 = automatic; else 64, 128 or 256 */
     int32_t refill_threshold;/* wavefront traversal: refill rays when fewer lanes of a
                                 wave than this are busy (0 = automatic, 1..64) */
-    int32_t reserved;
+    int32_t sampler;         /* GY_SAMPLER_SOBOL: SobolSampler + SobolEnum;
+                                GY_SAMPLER_HALTON: HaltonSampler (Faure) + HaltonEnum */
 } gy_render_params;
+enum { GY_SAMPLER_SOBOL = 0, GY_SAMPLER_HALTON = 1 };
 
 /* BVH build knobs (BVHAccel(prims, maxPrims, "sah"), accelerator/BVHAccel.h:50-51). */
 typedef struct {
@@ -175,10 +177,16 @@ int gy_li_samples(gy_scene* scene, const gy_camera_desc* cam, const gy_render_pa
 int gy_count_traffic(gy_scene* scene, const gy_camera_desc* cam, const gy_render_params* p, gy_render_stats* out);
 
 /* ---- samplers (SobolSampler::Sample, sampler/Sobol.h:66-77;
- *      SobolEnum::GetIndex, sampler/SobolEnum.h:24-28) ------------------- */
+ *      SobolEnum::GetIndex, sampler/SobolEnum.h:24-28; HaltonSampler,
+ *      HaltonEnum) ------------------------------------------------------------ */
 float gy_sobol_sample(uint64_t index, uint32_t dim);
 uint64_t gy_sobol_enum_index(uint32_t log2_res, uint32_t frame, uint32_t px, uint32_t py);
 int gy_camera_matrix(const gy_camera_desc* cam, float* raster_to_world16);
+/* HaltonSampler::sample with Faure permutations (sampler/Halton.h:595-625,
+ * 648-910; 0 for dim >= 256) and HaltonEnum::get_index for a width x height
+ * film (sampler/HaltonEnum.h:67-103). */
+float gy_halton_sample(uint64_t index, uint32_t dim);
+uint64_t gy_halton_enum_index(uint32_t width, uint32_t height, uint64_t i, uint32_t x, uint32_t y);
 
 /* ---- procedural scenes (the benchmark configurations) ---------------------- */
 typedef struct gy_generated gy_generated;

@@ -299,11 +299,116 @@ inline int log2_int(int v) {
     return r;
 }
 // RandomStateSequence::Sample (sampler/Sampler.h:53-64)
+// ---------------------------------------------------------------- Halton
+// HaltonSampler with Faure permutations (sampler/Halton.h:595-625, 648-923,
+// 1441-3308), evaluated digit by digit: dimension d in base b = the (d+1)-th
+// prime sums perm_b(digit_m) * b^(n-1-m) over the n = k*G lowest digits of the
+// index (k digits per table entry with b^k <= 512, G entries with b^(kG) <=
+// 2^32, as the reference's generated functions use) and scales the integer by
+// float((1 - 2^-23) / b^n).  Base 2 writes the reversed index bits into a
+// float mantissa.  HaltonEnum (sampler/HaltonEnum.h:61-143) by the CRT.
+namespace hal {
+inline uint64_t reverse64(uint64_t v) {
+    uint64_t r = 0;
+    for (int i = 0; i < 64; ++i, v >>= 1) r = (r << 1) | (v & 1);
+    return r;
+}
+struct Tables {
+    std::vector<uint32_t> primes;
+    std::vector<std::vector<uint16_t>> perm;  // Faure permutation by base
+    Tables() {
+        for (uint32_t c = 2; primes.size() < 256; ++c) {
+            bool is_p = true;
+            for (uint32_t q = 2; q * q <= c; ++q)
+                if (c % q == 0) { is_p = false; break; }
+            if (is_p) primes.push_back(c);
+        }
+        const uint32_t maxb = primes.back();
+        perm.resize(maxb + 1);
+        for (uint32_t k = 1; k <= 3; ++k)
+            for (uint32_t i = 0; i < k; ++i) perm[k].push_back((uint16_t)i);
+        for (uint32_t base = 4; base <= maxb; ++base) {
+            perm[base].resize(base);
+            const uint32_t b = base / 2;
+            if (base & 1) {
+                for (uint32_t i = 0; i < base - 1; ++i)
+                    perm[base][i + (i >= b)] = (uint16_t)(perm[base - 1][i] + (perm[base - 1][i] >= b));
+                perm[base][b] = (uint16_t)b;
+            } else {
+                for (uint32_t i = 0; i < b; ++i) {
+                    perm[base][i] = (uint16_t)(2 * perm[b][i]);
+                    perm[base][b + i] = (uint16_t)(2 * perm[b][i] + 1);
+                }
+            }
+        }
+    }
+};
+inline const Tables& tables() {
+    static const Tables t;
+    return t;
+}
+inline float sample(uint32_t dim, uint64_t index) {
+    if (dim == 0) {
+        uint32_t u = 0x3f800000u | (uint32_t)(reverse64(index) >> 41);
+        float f;
+        std::memcpy(&f, &u, 4);
+        return f - 1.f;
+    }
+    if (dim >= 256) return 0.f;
+    const Tables& T = tables();
+    const uint32_t b = T.primes[dim];
+    int k = 1;
+    while (std::pow((double)b, k + 1) <= 512.0) ++k;
+    int G = 1;
+    while (std::pow((double)b, k * (G + 1)) <= 4294967296.0) ++G;
+    const int n = k * G;
+    uint64_t sum = 0, x = index;
+    for (int m = 0; m < n; ++m) {
+        sum = sum * b + T.perm[b][x % b];
+        x /= b;
+    }
+    const double denom = std::pow((double)b, n);
+    return (float)sum * (float)(0x1.fffffcp-1 / denom);
+}
+struct Enum {
+    uint32_t p2 = 0, p3 = 0, mx = 0, my = 0;
+    uint64_t inc = 1;
+    uint32_t w = 1, h = 1;
+    static std::pair<int, int> euclid(int a, int b) {
+        if (!b) return std::make_pair(1, 0);
+        const std::pair<int, int> st = euclid(b, a % b);
+        return std::make_pair(st.second, st.first - (a / b) * st.second);
+    }
+    Enum(uint32_t width, uint32_t height) {
+        while (w < width) { w *= 2; ++p2; }
+        while (h < height) { h *= 3; ++p3; }
+        inc = (uint64_t)w * h;
+        const std::pair<int, int> inv = euclid((int)h, (int)w);
+        const uint32_t inv2 = inv.first < 0 ? (uint32_t)(inv.first + (int)w) : (uint32_t)(inv.first % (int)w);
+        const uint32_t inv3 = inv.second < 0 ? (uint32_t)(inv.second + (int)h) : (uint32_t)(inv.second % (int)h);
+        mx = h * inv2;
+        my = w * inv3;
+    }
+    uint64_t index(uint64_t i, uint32_t x, uint32_t y) const {
+        const uint64_t hx = p2 ? reverse64(x) >> (64 - p2) : 0;
+        uint64_t hy = 0, yy = y;
+        for (uint32_t d = 0; d < p3; ++d) {
+            hy = hy * 3 + yy % 3;
+            yy /= 3;
+        }
+        return (hx * mx + hy * my) % inc + i * inc;
+    }
+};
+}  // namespace hal
+
+// RandomStateSequence (sampler/Sampler.h:42-66) over the chosen sampler.
 struct Seq {
     uint64_t instance;
     int cursor = 0;
+    int kind = 0;  // 0 Sobol, 1 Halton
     float next() {
-        float ret = sobol_sample(instance, (uint32_t)cursor++);
+        float ret = kind == 1 ? hal::sample((uint32_t)cursor, instance) : sobol_sample(instance, (uint32_t)cursor);
+        ++cursor;
         if (ret >= 1) ret = 0;
         return ret;
     }
@@ -1360,11 +1465,21 @@ template <class R> V3<R> Li(const Scene<R>& s, Ray<R> ray, Seq& rnd, int maxDept
 
 // One camera sample of TiledIntegrator::Render (integrator/TiledIntegrator.cpp:40-47).
 template <class R> V3<R> camera_sample(const Scene<R>& s, const Camera<R>& cam, int res, int m, int x, int y, int smp,
-                                       int maxDepth, int rrDepth, bool use_bvh, R* fx, R* fy, int* dims) {
+                                       int maxDepth, int rrDepth, bool use_bvh, R* fx, R* fy, int* dims,
+                                       const hal::Enum* henum = nullptr) {
     Seq rnd;
-    rnd.instance = enum_look_up((uint32_t)m, (uint32_t)smp, (uint32_t)x, (uint32_t)y);
-    R u = res * (R)rnd.next() - R(x);  // SobolEnum::SampleX (SobolEnum.h:30-32)
-    R v = res * (R)rnd.next() - R(y);
+    R u, v;
+    if (henum) {
+        // HaltonEnum::GetIndex / SampleX / SampleY (sampler/HaltonEnum.h:37-47)
+        rnd.kind = 1;
+        rnd.instance = henum->index((uint64_t)smp, (uint32_t)x, (uint32_t)y);
+        u = (R)rnd.next() * (R)henum->w - R(x);
+        v = (R)rnd.next() * (R)henum->h - R(y);
+    } else {
+        rnd.instance = enum_look_up((uint32_t)m, (uint32_t)smp, (uint32_t)x, (uint32_t)y);
+        u = res * (R)rnd.next() - R(x);  // SobolEnum::SampleX (SobolEnum.h:30-32)
+        v = res * (R)rnd.next() - R(y);
+    }
     Ray<R> ray = generate_ray(cam, x, y, u, v);
     V3<R> L = Li(s, ray, rnd, maxDepth, rrDepth, use_bvh);
     *fx = x + u;
@@ -1464,6 +1579,7 @@ template <class R> int li_samples_t(SceneHandle* h, const or_camera_desc* cd, co
     Camera<R> cam = make_camera<R>(*cd);
     int res = round_up_pow2(std::max(cd->width, cd->height));
     int m = log2_int(res);
+    const hal::Enum henum((uint32_t)cd->width, (uint32_t)cd->height);
     int nth = p->nthreads > 0 ? p->nthreads : (int)std::thread::hardware_concurrency();
     std::atomic<int32_t> next(0);
     auto work = [&]() {
@@ -1474,7 +1590,7 @@ template <class R> int li_samples_t(SceneHandle* h, const or_camera_desc* cd, co
                 R fx, fy;
                 int dims;
                 V3<R> L = camera_sample(sc, cam, res, m, x[k], y[k], s[k], p->max_depth, p->rr_depth, p->use_bvh != 0, &fx, &fy,
-                                        &dims);
+                                        &dims, p->sampler == 1 ? &henum : nullptr);
                 out_L[3 * k] = L.x; out_L[3 * k + 1] = L.y; out_L[3 * k + 2] = L.z;
                 if (out_raster) { out_raster[2 * k] = (float)fx; out_raster[2 * k + 1] = (float)fy; }
                 if (out_dims) out_dims[k] = dims;
@@ -1500,6 +1616,7 @@ template <class R> int64_t render_t(SceneHandle* h, const or_camera_desc* cd, co
     if (row0 < 0) row0 = 0;
     int res = round_up_pow2(std::max(W, H));
     int m = log2_int(res);
+    const hal::Enum henum((uint32_t)W, (uint32_t)H);
     Film<R> film(W, H);
     // 32x32 tiles in row-major order (integrator/TiledIntegrator.cpp:13-25)
     struct Tile { int x0, y0, x1, y1; };
@@ -1524,7 +1641,7 @@ template <class R> int64_t render_t(SceneHandle* h, const or_camera_desc* cd, co
                     for (int smp = 0; smp < p->spp; ++smp) {
                         R fx, fy;
                         V3<R> L = camera_sample(sc, cam, res, m, x, y, smp, p->max_depth, p->rr_depth, p->use_bvh != 0, &fx, &fy,
-                                                (int*)nullptr);
+                                                (int*)nullptr, p->sampler == 1 ? &henum : nullptr);
                         film.add(fx, fy, L);
                         ++local;
                     }
@@ -1710,6 +1827,14 @@ int or_li_path(void* hv, const or_camera_desc* cam, const or_render_params* p, i
     t_rec = nullptr;
     t_rec_max = t_rec_n = 0;
     return n;
+}
+float or_halton_sample(uint64_t index, uint32_t dim) { return hal::sample(dim, index); }
+uint64_t or_halton_enum_index(uint32_t w, uint32_t h, uint64_t i, uint32_t x, uint32_t y) { return hal::Enum(w, h).index(i, x, y); }
+int or_faure_perm(uint32_t base, uint16_t* out) {
+    const hal::Tables& T = hal::tables();
+    if (base < 1 || base >= T.perm.size()) return -1;
+    for (uint32_t i = 0; i < base; ++i) out[i] = T.perm[base][i];
+    return 0;
 }
 int or_set_shadow_short(int on) {
     g_shadow_short = on;

@@ -70,6 +70,7 @@ typedef struct {
     int32_t precision;   /* 0 = FP32 bit-faithful, 1 = FP64 */
     int32_t nthreads;    /* tile worker threads (0 = hardware concurrency) */
     int32_t use_bvh;     /* 1 = reference BVH traversal, 0 = brute force */
+    int32_t sampler;     /* 0 = SobolSampler + SobolEnum, 1 = HaltonSampler (Faure) + HaltonEnum */
 } or_render_params;
 
 /* Sobol tables (fixtures extracted from the reference's data tables). */
@@ -128,6 +129,10 @@ int or_tex_lookup(int w, int h, int c, const float* data, int n, const float* u,
 /* Test-only switch: end NEE shadow segments ShadowRayEps before the light point
  * (the reference's segment reaches it, see spawn_to in gy_oracle.cpp). */
 int or_set_shadow_short(int on);
+/* HaltonSampler::sample (Faure) and HaltonEnum::get_index; Faure permutation of a base. */
+float or_halton_sample(uint64_t index, uint32_t dim);
+uint64_t or_halton_enum_index(uint32_t w, uint32_t h, uint64_t i, uint32_t x, uint32_t y);
+int or_faure_perm(uint32_t base, uint16_t* out);
 int or_li_path(void* scene, const or_camera_desc* cam, const or_render_params* p, int x, int y, int smp, int max_rows,
                double* rows);
 

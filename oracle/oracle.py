@@ -55,6 +55,7 @@ class ORParams(C.Structure):
         ("precision", C.c_int32),
         ("nthreads", C.c_int32),
         ("use_bvh", C.c_int32),
+        ("sampler", C.c_int32),
     ]
 
 
@@ -95,6 +96,9 @@ def lib():
         "or_fr_conductor": (C.c_int, [C.c_double, vp, vp, vp]),
         "or_ggx_D": (C.c_double, [C.c_double, C.c_double, vp]),
         "or_li_path": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp]),
+        "or_halton_sample": (C.c_float, [C.c_uint64, C.c_uint32]),
+        "or_halton_enum_index": (C.c_uint64, [C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32]),
+        "or_faure_perm": (C.c_int, [C.c_uint32, vp]),
         "or_tex_lookup": (C.c_int, [C.c_int, C.c_int, C.c_int, vp, C.c_int, vp, vp, C.c_int, vp]),
         "or_set_shadow_short": (C.c_int, [C.c_int]),
     }
@@ -121,9 +125,10 @@ def camera_from(cam):
     return o
 
 
-def params(spp, max_depth, rr_depth, precision=0, nthreads=0, use_bvh=1):
+def params(spp, max_depth, rr_depth, precision=0, nthreads=0, use_bvh=1, sampler=0):
     p = ORParams()
     p.spp, p.max_depth, p.rr_depth, p.precision, p.nthreads, p.use_bvh = spp, max_depth, rr_depth, precision, nthreads, use_bvh
+    p.sampler = sampler
     return p
 
 
@@ -297,6 +302,20 @@ def fr_conductor(c, eta, k):
 def ggx_D(ax, ay, m):
     m = np.ascontiguousarray(m, np.float64)
     return lib().or_ggx_D(ax, ay, m.ctypes.data)
+
+
+def halton_sample(index, dim):
+    return lib().or_halton_sample(index, dim)
+
+
+def halton_enum_index(w, h, i, x, y):
+    return lib().or_halton_enum_index(w, h, i, x, y)
+
+
+def faure_perm(base):
+    out = np.zeros(base, np.uint16)
+    assert lib().or_faure_perm(base, out.ctypes.data) == 0
+    return out
 
 
 def tex_lookup(data, u, v, ch):

@@ -388,3 +388,94 @@ def test_bvh_matches_brute_force_cornell():
         ab = os_.trace_any(rays, True)
         af = os_.trace_any(rays, False)
         assert (ab != af).mean() < 2e-3
+
+
+# ---------------------------------------------------------------- Halton
+def test_faure_permutations_known_values():
+    # Faure's permutations as published (Faure 1992): sigma_4..sigma_7, and
+    # every sigma_b a permutation of 0..b-1 up to the largest base used (1619).
+    assert list(O.faure_perm(4)) == [0, 2, 1, 3]
+    assert list(O.faure_perm(5)) == [0, 3, 2, 1, 4]
+    assert list(O.faure_perm(6)) == [0, 2, 4, 1, 3, 5]
+    assert list(O.faure_perm(7)) == [0, 2, 5, 3, 1, 4, 6]
+    for b in (2, 3, 11, 97, 256, 1013, 1619):
+        assert sorted(O.faure_perm(b)) == list(range(b))
+
+
+def _rev_bits(i, n):
+    return int(format(i, "0%db" % n)[::-1], 2)
+
+
+def test_halton_base2_is_bit_reversal():
+    # dimension 0: the 23 lowest index bits reversed, exactly (Halton.h:1441-1452)
+    rng = np.random.default_rng(21)
+    for i in list(range(64)) + [int(v) for v in rng.integers(0, 2 ** 62, 200)]:
+        assert O.halton_sample(i, 0) == _rev_bits(i & ((1 << 23) - 1), 23) / 2.0 ** 23
+
+
+def _primes(n):
+    out = []
+    c = 2
+    while len(out) < n:
+        if all(c % p for p in out if p * p <= c):
+            out.append(c)
+        c += 1
+    return out
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3, 9, 100, 255])
+def test_halton_permuted_radical_inverse(dim):
+    # value = (1 - 2^-23) * sum_m sigma_b(d_m) b^-(m+1) over the digits the
+    # reference's tables cover, to float rounding (relative 2^-23)
+    b = _primes(256)[dim]
+    k = 1
+    while b ** (k + 1) <= 512:
+        k += 1
+    g = 1
+    while b ** (k * (g + 1)) <= 2 ** 32:
+        g += 1
+    n = k * g
+    sig = [int(v) for v in O.faure_perm(b)]
+    rng = np.random.default_rng(dim)
+    for i in [int(v) for v in rng.integers(0, min(b ** n, 2 ** 62), 300)]:
+        x, acc, scale = i, 0.0, 1.0 / b
+        for _ in range(n):
+            acc += sig[x % b] * scale
+            x //= b
+            scale /= b
+        expect = acc * (1 - 2.0 ** -23)
+        got = O.halton_sample(i, dim)
+        assert abs(got - expect) <= 2.5e-7 * max(expect, 1e-30) + 1e-12, (i, got, expect)
+    assert O.halton_sample(12345, 256) == 0.0 and O.halton_sample(1, 1000) == 0.0
+
+
+@pytest.mark.parametrize("w,h", [(5, 7), (16, 9), (3, 2), (1, 4)])
+def test_halton_enum_elementary_intervals(w, h):
+    # HaltonEnum::get_index: the i-th index of pixel (x, y) has its base-2 and
+    # base-3 radical inverses (identity digit orders) inside that pixel of the
+    # 2^p2 x 3^p3 grid, and consecutive i step by the grid size (brute force)
+    p2 = max(0, (w - 1).bit_length())
+    p3 = 0
+    while 3 ** p3 < h:
+        p3 += 1
+    W2, H3 = 2 ** p2, 3 ** p3
+    for y in range(h):
+        for x in range(w):
+            for i in range(3):
+                idx = O.halton_enum_index(w, h, i, x, y)
+                assert idx == g.halton_enum_index(w, h, i, x, y)
+                assert idx == O.halton_enum_index(w, h, 0, x, y) + i * W2 * H3
+                r2 = _rev_bits(idx % W2, p2) if p2 else 0   # floor(W2 * radinv2(idx))
+                t, r3 = idx, 0
+                for _ in range(p3):
+                    r3 = r3 * 3 + t % 3
+                    t //= 3
+                assert (r2, r3) == (x, y), (x, y, i, idx)
+
+
+def test_library_halton_matches_oracle():
+    rng = np.random.default_rng(22)
+    for _ in range(3000):
+        i = int(rng.integers(0, 2 ** 60))
+        d = int(rng.integers(0, 300))
+        assert g.halton_sample(i, d) == O.halton_sample(i, d)
