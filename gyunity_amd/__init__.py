@@ -5,6 +5,7 @@ include/gyunity.h) and the C++ Integrator in include/gyunity/; this package is
 the Python binding used by tests and the benchmark.
 """
 from ._lib import (  # noqa: F401
+    Batch,
     Context,
     Scene,
     SceneArrays,
@@ -22,6 +23,7 @@ from ._lib import (  # noqa: F401
     halton_enum_index,
     GY_SCENE_CORNELL,
     GY_SCENE_ICOSPHERE_GGX,
+    GY_SCENE_BATCH_MEMBER,
     GY_RENDER_MEGAKERNEL,
     GY_RENDER_WAVEFRONT,
     GY_SAMPLER_SOBOL,

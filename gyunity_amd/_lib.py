@@ -23,6 +23,7 @@ GY_SCENE_ICOSPHERE_GGX = 1
 GY_RENDER_MEGAKERNEL = 0
 GY_RENDER_WAVEFRONT = 1
 GY_SAMPLER_SOBOL = 0
+GY_SCENE_BATCH_MEMBER = 2
 GY_SAMPLER_HALTON = 1
 FAULT_NONE, FAULT_ALLOC, FAULT_LAUNCH, FAULT_SCENE = 0, 1, 2, 3
 
@@ -70,6 +71,10 @@ class RenderParams(C.Structure):
         ("refill_threshold", C.c_int32),
         ("sampler", C.c_int32),
     ]
+
+
+class BatchParams(C.Structure):
+    _fields_ = [("lanes_per_pass", C.c_int64), ("refill_threshold", C.c_int32), ("use_graph", C.c_int32)]
 
 
 class BVHParams(C.Structure):
@@ -140,6 +145,11 @@ def lib():
         "gy_count_visits": (C.c_int, [vp, vp, i64, P(i64), P(i64)]),
         "gy_render": (C.c_int, [vp, P(CameraDesc), P(RenderParams), P(f32), P(f32)]),
         "gy_film_download": (C.c_int, [vp, P(f32), P(f32)]),
+        "gy_batch_create": (C.c_int, [vp, P(BatchParams), P(vp)]),
+        "gy_batch_add": (C.c_int, [vp, vp, P(CameraDesc), P(RenderParams), P(i32)]),
+        "gy_batch_render": (C.c_int, [vp]),
+        "gy_batch_film_download": (C.c_int, [vp, i32, P(f32), P(f32)]),
+        "gy_batch_destroy": (C.c_int, [vp]),
         "gy_render_stats_get": (C.c_int, [vp, P(RenderStats)]),
         "gy_kernel_name": (C.c_char_p, [C.c_int]),
         "gy_li_samples": (C.c_int, [vp, P(CameraDesc), P(RenderParams), i32, P(i32), P(i32), P(i32), P(f32), P(f32), P(i32)]),
@@ -323,6 +333,46 @@ class Context:
 
     def inject_fault(self, kind):
         check(lib().gy_debug_inject_fault(self.h, kind))
+
+
+class Batch:
+    """Many renders traced together (gy_batch_*): add (scene, camera, params)
+    entries, render them all in one lane space, download each entry's film."""
+
+    def __init__(self, ctx, lanes_per_pass=0, refill_threshold=0, use_graph=True):
+        self.ctx = ctx
+        bp = BatchParams()
+        bp.lanes_per_pass, bp.refill_threshold, bp.use_graph = lanes_per_pass, refill_threshold, 1 if use_graph else 0
+        self.h = C.c_void_p()
+        self.cams = []
+        check(lib().gy_batch_create(ctx.h, C.byref(bp), C.byref(self.h)))
+
+    def add(self, scene, cam, params):
+        e = C.c_int32()
+        check(lib().gy_batch_add(self.h, scene.h, C.byref(cam), C.byref(params), C.byref(e)))
+        self.cams.append((cam.width, cam.height))
+        return e.value
+
+    def render(self):
+        check(lib().gy_batch_render(self.h))
+
+    def film(self, entry):
+        W, H = self.cams[entry]
+        color = np.zeros((H, W, 3), dtype=np.float32)
+        weight = np.zeros((H, W), dtype=np.float32)
+        check(lib().gy_batch_film_download(self.h, entry, _fp(color), _fp(weight)))
+        return color, weight
+
+    def close(self):
+        if self.h:
+            lib().gy_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 class Scene:

@@ -1,0 +1,195 @@
+// C ABI of batched rendering: many (scene, view, sample budget) renders traced
+// together by the wavefront kernels in one lane space (wavefront.hip,
+// BatchRender), recorded once into a hipGraph and replayed.
+//
+// No counterpart in the reference: its TiledIntegrator renders one image per
+// Render() call (integrator/TiledIntegrator.cpp:7-60).  Each entry's film is
+// exactly what gy_render gives for that entry alone (same estimator, same
+// per-pixel sample order).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "runtime.h"
+
+namespace gy {
+int MakePathParams(Ctx* c, Scene* sc, const gy_camera_desc& cam, const gy_render_params& p, PathParams* P);
+int ValidateRender(const gy_camera_desc* cam, const gy_render_params* p);
+void TimersCollect(Ctx* c);
+
+namespace {
+
+void BatchDropGraph(Batch* b) {
+    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+    b->graph_exec = nullptr;
+}
+
+// Table of entries, lane space, films: rebuilt after entries change.
+int BatchBuild(Ctx* c, Batch* b, int64_t lanes_req) {
+    if (b->entries.empty()) return Fail(GY_ERR_INVALID, "gy_batch_render: empty batch");
+    BatchDropGraph(b);
+    std::vector<BatchEntryDev> tab(b->entries.size());
+    int64_t first = 0;
+    int depth_max = 0;
+    for (size_t i = 0; i < b->entries.size(); ++i) {
+        BatchEntry& en = b->entries[i];
+        int rc = MakePathParams(c, en.scene, en.cam, en.params, &en.P);
+        if (rc != GY_OK) return rc;
+        const int W = en.cam.width, H = en.cam.height;
+        rc = FilmPrepareState(c, en.film, W, H, (int64_t)W * H * en.params.spp);
+        if (rc != GY_OK) return rc;
+        BatchEntryDev& d = tab[i];
+        std::memset(&d, 0, sizeof(d));
+        d.S = en.scene->dev;
+        d.P = en.P;
+        d.F = FilmViewState(en.film);
+        d.tilesX = (W + 7) >> 3;
+        d.tilesY = (H + 7) >> 3;
+        d.paths = (int64_t)d.tilesX * d.tilesY * 64 * en.params.spp;
+        d.first = first;
+        en.first = first;
+        en.paths = d.paths;
+        first += d.paths;
+        depth_max = std::max(depth_max, en.params.max_depth);
+    }
+    DevFree(b->table);
+    b->table = nullptr;
+    int rc = Upload(c, &b->table, tab.data(), tab.size() * sizeof(BatchEntryDev));
+    if (rc != GY_OK) return rc;
+    if (!b->counters) {
+        rc = DevAlloc(c, (void**)&b->counters, 16 * sizeof(unsigned long long));
+        if (rc != GY_OK) return rc;
+    }
+    b->total_paths = first;
+    b->depth_max = depth_max;
+    const int64_t want = lanes_req > 0 ? lanes_req : (int64_t)32 << 20;
+    b->lanes = std::min(want, (first + 255) / 256 * 256);
+    if (b->lanes >= (1ll << 31)) return Fail(GY_ERR_UNSUPPORTED, "batch pass larger than 2^31 paths");
+    b->dirty = false;
+    return GY_OK;
+}
+
+struct BatchHandle {
+    Batch b;
+    int64_t lanes_req = 0;
+};
+
+}  // namespace
+}  // namespace gy
+
+using namespace gy;
+
+extern "C" int gy_batch_create(gy_context* ctx, const gy_batch_params* p, gy_batch** out) {
+    if (!ctx || !out) return Fail(GY_ERR_INVALID, "gy_batch_create: null argument");
+    if (p && (p->lanes_per_pass < 0 || p->refill_threshold < 0 || p->refill_threshold > 64))
+        return Fail(GY_ERR_INVALID, "gy_batch_create: lanes_per_pass >= 0 and 0 <= refill_threshold <= 64");
+    BatchHandle* h = new BatchHandle;
+    h->b.ctx = reinterpret_cast<Ctx*>(ctx);
+    if (p) {
+        h->lanes_req = p->lanes_per_pass;
+        h->b.refill = p->refill_threshold > 0 ? p->refill_threshold : 1;
+        h->b.use_graph = p->use_graph != 0;
+    }
+    *out = reinterpret_cast<gy_batch*>(h);
+    return GY_OK;
+}
+
+extern "C" int gy_batch_add(gy_batch* bh, gy_scene* scene, const gy_camera_desc* cam, const gy_render_params* p, int32_t* entry) {
+    if (!bh || !scene) return Fail(GY_ERR_INVALID, "gy_batch_add: null batch or scene");
+    int rc = ValidateRender(cam, p);
+    if (rc != GY_OK) return rc;
+    if (p->max_depth < 1) return Fail(GY_ERR_INVALID, "gy_batch_add: max_depth >= 1");
+    if ((int64_t)cam->width * cam->height * p->spp >= (1ll << 32)) return Fail(GY_ERR_UNSUPPORTED, "gy_batch_add: more than 2^32 paths in one entry");
+    Batch& b = reinterpret_cast<BatchHandle*>(bh)->b;
+    Scene* sc = reinterpret_cast<Scene*>(scene);
+    if (sc->ctx != b.ctx) return Fail(GY_ERR_INVALID, "gy_batch_add: scene belongs to another context");
+    if (sc->stats.max_depth + 1 > 64) return Fail(GY_ERR_UNSUPPORTED, "gy_batch_add: BVH deeper than the traversal stack");
+    BatchEntry en;
+    en.scene = sc;
+    en.cam = *cam;
+    en.params = *p;
+    b.entries.push_back(en);
+    b.dirty = true;
+    if (entry) *entry = (int32_t)b.entries.size() - 1;
+    return GY_OK;
+}
+
+extern "C" int gy_batch_render(gy_batch* bh) {
+    if (!bh) return Fail(GY_ERR_INVALID, "gy_batch_render: null batch");
+    BatchHandle* h = reinterpret_cast<BatchHandle*>(bh);
+    Batch& b = h->b;
+    Ctx* c = b.ctx;
+    (void)hipSetDevice(c->device);
+    int rc = GY_OK;
+    if (b.dirty) rc = BatchBuild(c, &b, h->lanes_req);
+    if (rc != GY_OK) return rc;
+    std::memset(&c->stats, 0, sizeof(c->stats));
+    c->ev_used = 0;
+    hipEvent_t t0, t1;
+    (void)hipEventCreate(&t0);
+    (void)hipEventCreate(&t1);
+    (void)hipEventRecord(t0, c->stream);
+    rc = BatchRender(c, &b);
+    (void)hipEventRecord(t1, c->stream);
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (rc == GY_OK && e != hipSuccess) rc = HipFail(e, "batch sync");
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, t0, t1);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    TimersCollect(c);
+    c->stats.total_ms = ms;
+    for (const BatchEntry& en : b.entries) c->stats.paths += (int64_t)en.cam.width * en.cam.height * en.params.spp;
+    if (rc == GY_OK) {
+        unsigned long long cnt[8];
+        e = hipMemcpy(cnt, b.counters, sizeof(cnt), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return HipFail(e, "batch counters");
+        c->stats.rays = (int64_t)(cnt[0] + cnt[1] + cnt[2]);
+        c->stats.shadow_rays = (int64_t)cnt[1];
+        c->stats.mis_rays = (int64_t)cnt[2];
+        for (const BatchEntry& en : b.entries) {
+            uint32_t ovf[2];
+            e = hipMemcpy(ovf, en.film.side_count, 8, hipMemcpyDeviceToHost);
+            if (e != hipSuccess) return HipFail(e, "batch film counters");
+            if (ovf[1] != 0) return Fail(GY_ERR_UNSUPPORTED, "film side list overflow");
+        }
+    }
+    return rc;
+}
+
+extern "C" int gy_batch_film_download(gy_batch* bh, int32_t entry, float* color, float* weight) {
+    if (!bh) return Fail(GY_ERR_INVALID, "gy_batch_film_download: null batch");
+    Batch& b = reinterpret_cast<BatchHandle*>(bh)->b;
+    if (entry < 0 || entry >= (int32_t)b.entries.size()) return Fail(GY_ERR_INVALID, "gy_batch_film_download: no such entry");
+    const FilmState& f = b.entries[entry].film;
+    if (!f.film || b.dirty) return Fail(GY_ERR_INVALID, "gy_batch_film_download: batch not rendered");
+    const int64_t WH = (int64_t)f.width * f.height;
+    std::vector<float4> hbuf(WH);
+    hipError_t e = hipMemcpy(hbuf.data(), f.film, WH * sizeof(float4), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return HipFail(e, "batch film D2H");
+    for (int64_t i = 0; i < WH; ++i) {
+        if (color) {
+            color[3 * i] = hbuf[i].x;
+            color[3 * i + 1] = hbuf[i].y;
+            color[3 * i + 2] = hbuf[i].z;
+        }
+        if (weight) weight[i] = hbuf[i].w;
+    }
+    return GY_OK;
+}
+
+extern "C" int gy_batch_destroy(gy_batch* bh) {
+    if (!bh) return GY_OK;
+    BatchHandle* h = reinterpret_cast<BatchHandle*>(bh);
+    Batch& b = h->b;
+    (void)hipSetDevice(b.ctx->device);
+    (void)hipStreamSynchronize(b.ctx->stream);
+    BatchDropGraph(&b);
+    for (BatchEntry& en : b.entries) FilmFree(&en.film);
+    DevFree(b.table);
+    DevFree(b.counters);
+    WavefrontStateFree(&b.wf);
+    delete h;
+    return GY_OK;
+}

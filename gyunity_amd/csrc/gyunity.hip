@@ -47,6 +47,12 @@ int Fill32(Ctx* c, void* p, uint32_t v, int64_t n) {
     return e == hipSuccess ? GY_OK : HipFail(e, "fill kernel");
 }
 
+int FillF4(Ctx* c, float4* p, int64_t n) {
+    if (n <= 0) return GY_OK;
+    hipLaunchKernelGGL(k_fill_f4, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, p, n);
+    return CheckLaunch(c, "fill");
+}
+
 int DevAlloc(Ctx* c, void** p, size_t bytes) {
     if (c->fault == GY_FAULT_ALLOC) {
         c->fault = GY_FAULT_NONE;
@@ -99,6 +105,7 @@ int CheckLaunch(Ctx* c, const char* what) {
 // Timed region on the context stream: events recorded around each launch of a
 // kernel category and summed after the stream synchronises.
 void TimerBegin(Ctx* c, int cat) {
+    if (c->capturing) return;
     if (c->ev_used + 2 > (int)c->events.size()) {
         hipEvent_t a, b;
         (void)hipEventCreate(&a);
@@ -111,6 +118,7 @@ void TimerBegin(Ctx* c, int cat) {
     (void)hipEventRecord(c->events[c->ev_used], c->stream);
 }
 void TimerEnd(Ctx* c) {
+    if (c->capturing) return;
     (void)hipEventRecord(c->events[c->ev_used + 1], c->stream);
     c->ev_used += 2;
 }
@@ -706,8 +714,7 @@ void FilmFree(FilmState* f) {
     *f = FilmState();
 }
 
-int FilmPrepare(Ctx* c, Scene* sc, int W, int H, int n_pass) {
-    FilmState& f = sc->film;
+int FilmPrepareState(Ctx* c, FilmState& f, int W, int H, int64_t pass_slots) {
     const int64_t WH = (int64_t)W * H;
     if (f.width != W || f.height != H || !f.film) {
         DevFree(f.film);
@@ -717,12 +724,12 @@ int FilmPrepare(Ctx* c, Scene* sc, int W, int H, int n_pass) {
         f.width = W;
         f.height = H;
     }
-    if (f.pass_slots < WH * n_pass) {
+    if (f.pass_slots < pass_slots) {
         DevFree(f.pass);
         f.pass = nullptr;
-        int rc = DevAlloc(c, (void**)&f.pass, (size_t)WH * n_pass * sizeof(float4));
+        int rc = DevAlloc(c, (void**)&f.pass, (size_t)pass_slots * sizeof(float4));
         if (rc != GY_OK) return rc;
-        f.pass_slots = WH * n_pass;
+        f.pass_slots = pass_slots;
     }
     if (!f.side) {
         f.side_cap = 1u << 18;
@@ -740,23 +747,23 @@ int FilmPrepare(Ctx* c, Scene* sc, int W, int H, int n_pass) {
     return Fill32(c, f.counters, 0u, 32);
 }
 
-FilmDev FilmView(Scene* sc) {
+int FilmPrepare(Ctx* c, Scene* sc, int W, int H, int n_pass) { return FilmPrepareState(c, sc->film, W, H, (int64_t)W * H * n_pass); }
+FilmDev FilmViewState(const FilmState& f) {
     FilmDev F;
-    F.film = sc->film.film;
-    F.pass = sc->film.pass;
-    F.side = sc->film.side;
-    F.side_sorted = sc->film.side_sorted;
-    F.side_count = sc->film.side_count;
-    F.side_overflow = sc->film.side_count + 1;
-    F.side_cap = sc->film.side_cap;
-    F.width = sc->film.width;
-    F.height = sc->film.height;
+    F.film = f.film;
+    F.pass = f.pass;
+    F.side = f.side;
+    F.side_sorted = f.side_sorted;
+    F.side_count = f.side_count;
+    F.side_overflow = f.side_count + 1;
+    F.side_cap = f.side_cap;
+    F.width = f.width;
+    F.height = f.height;
     return F;
 }
-
-// Film accumulation of one pass: owners add own samples, then the side list.
-int FilmResolvePass(Ctx* c, Scene* sc, int n_s) {
-    FilmDev F = FilmView(sc);
+FilmDev FilmView(Scene* sc) { return FilmViewState(sc->film); }
+int FilmResolveState(Ctx* c, const FilmState& f, int n_s) {
+    FilmDev F = FilmViewState(f);
     const int64_t WH = (int64_t)F.width * F.height;
     TimerBegin(c, kKernFilm);
     hipLaunchKernelGGL(k_film_accumulate, dim3((unsigned)((WH + 255) / 256)), dim3(256), 0, c->stream, F, n_s);
@@ -771,6 +778,7 @@ int FilmResolvePass(Ctx* c, Scene* sc, int n_s) {
     if (rc != GY_OK) return rc;
     return Fill32(c, F.side_count, 0u, 1);
 }
+int FilmResolvePass(Ctx* c, Scene* sc, int n_s) { return FilmResolveState(c, sc->film, n_s); }
 
 int MakePathParams(Ctx* c, Scene* sc, const gy_camera_desc& cam, const gy_render_params& p, PathParams* P) {
     CameraRT rt;

@@ -40,6 +40,7 @@ struct Ctx {
     std::vector<hipEvent_t> events;
     std::vector<int> ev_cat;
     int ev_used = 0;
+    bool capturing = false;  // a hipGraph capture is recording the stream: no timer events
 };
 
 struct SideEntry {
@@ -78,6 +79,7 @@ struct WavefrontState {
     size_t bytes = 0;
     void* spill = nullptr;
     size_t spill_bytes = 0;
+    bool batch = false;  // carries the per-lane entry and pass-slot arrays
 };
 
 struct Scene {
@@ -101,12 +103,16 @@ int Upload(Ctx* c, void** p, const void* src, size_t bytes);
 void DevFree(void* p);
 int Grow(Ctx* c, DevBuf* b, size_t bytes);
 int Fill32(Ctx* c, void* p, uint32_t v, int64_t n);
+int FillF4(Ctx* c, float4* p, int64_t n);  // zero n float4
 int CheckLaunch(Ctx* c, const char* what);
 void TimerBegin(Ctx* c, int cat);
 void TimerEnd(Ctx* c);
 void SceneFree(Scene* sc);
 void FilmFree(FilmState* f);
 FilmDev FilmView(Scene* sc);
+FilmDev FilmViewState(const FilmState& f);
+int FilmPrepareState(Ctx* c, FilmState& f, int W, int H, int64_t pass_slots);
+int FilmResolveState(Ctx* c, const FilmState& f, int n_s);
 
 // Per-render constants of the integrator kernels.
 struct PathParams {
@@ -117,6 +123,40 @@ struct PathParams {
 };
 int WavefrontPass(Ctx* c, Scene* sc, const PathParams& P, const gy_render_params& p, int s0, int n_s);
 void WavefrontFree(Scene* sc);
+void WavefrontStateFree(WavefrontState* wf);
+
+// A batch of renders (scene, view, sample budget) traced in one lane space.
+struct BatchEntryDev {
+    gyd::SceneDev S;
+    PathParams P;
+    FilmDev F;
+    int64_t first;  // first global path index of the entry
+    int64_t paths;  // tile-padded paths: tilesX * tilesY * 64 * spp
+    int32_t tilesX, tilesY;
+};
+struct BatchEntry {
+    Scene* scene = nullptr;
+    gy_camera_desc cam = {};
+    gy_render_params params = {};
+    PathParams P = {};
+    FilmState film;
+    int64_t first = 0, paths = 0;
+};
+struct Batch {
+    Ctx* ctx = nullptr;
+    std::vector<BatchEntry> entries;
+    void* table = nullptr;  // BatchEntryDev per entry (device)
+    unsigned long long* counters = nullptr;
+    WavefrontState wf;
+    int64_t lanes = 0;        // paths per pass
+    int64_t total_paths = 0;
+    int depth_max = 0;
+    int refill = 1;
+    bool use_graph = true;
+    bool dirty = true;        // entries changed since the table / graph were built
+    hipGraphExec_t graph_exec = nullptr;
+};
+int BatchRender(Ctx* c, Batch* b);
 // GPU binned-SAH BVH build (bvh_gpu.hip).
 int BuildBVHDevice(Ctx* c, const float* pos, int32_t n_tri, int max_leaf, BVHBuild* out);
 

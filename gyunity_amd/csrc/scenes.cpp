@@ -13,6 +13,16 @@
 //    radially by 4-octave value noise of amplitude 0.05 seeded by `seed`, with
 //    flat (face) normals.  GGX conductor, alpha 0.2 given directly, Schlick F0
 //    (0.95, 0.64, 0.54).  1024^2, 64 spp, depth 8, roulette for i > 5.
+//  GY_SCENE_BATCH_MEMBER: member `seed` of the 64-scene batch workload.  The
+//    room and light of the Cornell box; a displaced icosphere whose triangle
+//    count is drawn log-uniform in [1e3, 2e6] (the first N triangles of the
+//    smallest subdivision level that has N); 0-20 random boxes; every object is
+//    a GGX conductor (alpha U[0.05, 0.8], Schlick F0) or, with probability 1/4,
+//    a rough dielectric of IOR 1.5 (alpha U[0.05, 0.8]); resolution from
+//    {128, 256, 512, 1024}^2, spp from {4, 8, 16, 32, 64}, depth U{4..12},
+//    roulette for i > 5.  param > 0 shrinks a member for tests: at most `param`
+//    triangles, resolution / 8 (at least 16), spp <= 4.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <unordered_map>
@@ -185,7 +195,7 @@ double Fbm(double x, double y, double z, uint32_t seed) {
 }
 
 void AddIcosphere(Generated& g, int levels, uint32_t seed, double cx, double cy, double cz, double radius, double amp,
-                  uint32_t mat) {
+                  uint32_t mat, size_t max_tris = (size_t)-1) {
     const double t = (1.0 + std::sqrt(5.0)) / 2.0;
     std::vector<double> vx = {-1, 1, -1, 1, 0, 0, 0, 0, t, t, -t, -t};
     std::vector<double> vy = {t, t, -t, -t, -1, 1, -1, 1, 0, 0, 0, 0};
@@ -226,7 +236,8 @@ void AddIcosphere(Generated& g, int levels, uint32_t seed, double cx, double cy,
         double r = radius + amp * Fbm(vx[i], vy[i], vz[i], seed);
         P[i] = {(float)(cx + r * vx[i]), (float)(cy + r * vy[i]), (float)(cz + r * vz[i])};
     }
-    for (size_t f = 0; f < faces.size(); f += 3) {
+    const size_t nf = std::min(faces.size(), max_tris * 3);
+    for (size_t f = 0; f < nf; f += 3) {
         V p0 = P[faces[f]], p1 = P[faces[f + 1]], p2 = P[faces[f + 2]];
         V n = norm(cross(sub(p1, p0), sub(p2, p0)));
         // orient outward (away from the centre)
@@ -237,6 +248,65 @@ void AddIcosphere(Generated& g, int levels, uint32_t seed, double cx, double cy,
         }
         AddTri(g, p0, p1, p2, n, mat, -1);
     }
+}
+
+// splitmix64: the member's random stream
+struct Rng {
+    uint64_t s;
+    uint64_t next() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double uniform() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
+    double range(double a, double b) { return a + (b - a) * uniform(); }
+    int pick(int n) { return (int)(next() % (uint64_t)n); }
+};
+
+uint32_t AddObjectMaterial(Generated& g, Rng& r) {
+    const float one[3] = {1.f, 1.f, 1.f};
+    const float alpha = (float)r.range(0.05, 0.8);
+    if (r.uniform() < 0.25) {
+        uint32_t m = AddMaterial(g, GY_MAT_ROUGH_DIELECTRIC, GY_MATF_ALPHA_DIRECT, one);
+        SetWord(g, m, 8, alpha);
+        SetWord(g, m, 9, alpha);
+        return m;
+    }
+    uint32_t m = AddMaterial(g, GY_MAT_CONDUCTOR, GY_MATF_ALPHA_DIRECT | GY_MATF_SCHLICK_F0, one);
+    SetWord(g, m, 8, alpha);
+    SetWord(g, m, 9, alpha);
+    for (int c = 0; c < 3; ++c) SetWord(g, m, 16 + c, (float)r.range(0.5, 0.98));
+    return m;
+}
+
+void BatchMember(Generated& g, uint32_t seed, int32_t small) {
+    Rng r{0x6A09E667F3BCC909ull ^ ((uint64_t)seed * 0x2545F4914F6CDD1Dull)};
+    AddRoomAndLight(g, false);
+    double n_tri = std::exp(r.range(std::log(1e3), std::log(2e6)));
+    if (small > 0) n_tri = std::min(n_tri, (double)small);
+    int level = 0;
+    while (20.0 * std::pow(4.0, level) < n_tri) ++level;
+    const double cx = r.range(0.3, 0.7), cy = r.range(0.25, 0.45), cz = r.range(0.3, 0.7);
+    const double rad = r.range(0.12, 0.25);
+    AddIcosphere(g, level, seed, cx, cy, cz, rad, 0.03, AddObjectMaterial(g, r), (size_t)n_tri);
+    const int boxes = r.pick(21);
+    for (int b = 0; b < boxes; ++b) {
+        const double sx = r.range(0.03, 0.15), sy = r.range(0.03, 0.15), sz = r.range(0.03, 0.15);
+        const double x = r.range(0.01, 0.99 - sx), y = r.range(0.005, 0.6), z = r.range(0.01, 0.99 - sz);
+        AddBox(g, {(float)x, (float)y, (float)z}, {(float)(x + sx), (float)(y + sy), (float)(z + sz)}, AddObjectMaterial(g, r));
+    }
+    const int res[4] = {128, 256, 512, 1024}, spp[5] = {4, 8, 16, 32, 64};
+    int rr = res[r.pick(4)], sp = spp[r.pick(5)];
+    const int depth = 4 + r.pick(9);
+    if (small > 0) {
+        rr = std::max(16, rr / 8);
+        sp = std::min(sp, 4);
+    }
+    SetCamera(g, rr);
+    g.params.spp = sp;
+    g.params.max_depth = depth;
+    g.params.rr_depth = 5;
 }
 
 }  // namespace
@@ -270,6 +340,8 @@ extern "C" int gy_generate_scene(int kind, uint32_t seed, int32_t param, gy_gene
         g->params.spp = 64;
         g->params.max_depth = 8;
         g->params.rr_depth = 5;
+    } else if (kind == GY_SCENE_BATCH_MEMBER) {
+        BatchMember(*g, seed, param);
     } else {
         delete g;
         return Fail(GY_ERR_INVALID, "gy_generate_scene: unknown scene kind");

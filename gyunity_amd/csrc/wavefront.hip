@@ -13,6 +13,14 @@
 // ballot and one atomic per wave; launch grids are fixed by the pass capacity
 // and kernels read their queue length from device memory, so a bounce needs no
 // host round trip.
+//
+// The kernels are written once for two sources of per-path constants:
+//   OneRender    one scene / camera / film, passed as kernel arguments;
+//   ManyRenders  a batch: every lane carries the id of its entry (scene, view,
+//                sample budget, film) in a device table, so paths of many
+//                renders of uneven size share one lane space, one set of
+//                queues and the same persistent trace waves, which steal queue
+//                entries 64 at a time whatever render they belong to.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -24,8 +32,6 @@
 using namespace gyd;
 
 namespace gy {
-
-
 
 enum QueueId { Q_ACTIVE = 0, Q_NEXT = 1, Q_SHADE0 = 2, Q_SHADOW = 6, Q_MIS = 7, Q_COUNT = 8 };
 // counts[] words 8..10: work-fetch counters of the persistent trace kernels
@@ -46,7 +52,7 @@ struct WfDev {
     float4* d;      // ray direction (xyz)
     float4* T;      // throughput (xyz), film x in w
     float4* L;      // radiance (xyz), film y in w
-    uint4* seq;     // Sobol index lo/hi, cursor, flags
+    uint4* seq;     // sample index lo/hi, cursor, flags
     float4* hit;    // t, prim, b1, b2
     float4* pend0;  // L1 before the shadow test (xyz), w1
     float4* pend1;  // BSDF value of the MIS sample (xyz) / L2 after trace_mis, cos
@@ -56,11 +62,36 @@ struct WfDev {
     float4* sh_d;   // shadow ray direction
     float4* mis_o;  // MIS ray origin
     float4* mis_d;  // MIS ray direction
+    uint32_t* ent;    // batch: entry of the lane
+    uint32_t* pslot;  // batch: the path's slot in its entry's pass buffer
     uint32_t* q[Q_COUNT];  // Q_ACTIVE, Q_NEXT, 4 shade queues, shadow, mis
     uint32_t* counts;          // Q_COUNT counters
     int64_t capacity;
     uint32_t* spill;       // traversal stack spill, 64 entries per lane
     int64_t spill_stride;  // lanes per launch (grid size)
+};
+
+// ---------------------------------------------------------------- render policies
+struct OneRender {
+    SceneDev S;
+    PathParams P;
+    FilmDev F;
+    static constexpr bool kBatch = false;
+    __device__ uint32_t entry(const WfDev&, uint32_t) const { return 0; }
+    __device__ const SceneDev& scene(uint32_t) const { return S; }
+    __device__ const PathParams& params(uint32_t) const { return P; }
+    __device__ const FilmDev& film(uint32_t) const { return F; }
+    __device__ uint32_t pass_slot(const WfDev&, uint32_t slot) const { return slot; }
+};
+struct ManyRenders {
+    const BatchEntryDev* __restrict__ E;
+    int32_t n;
+    static constexpr bool kBatch = true;
+    __device__ uint32_t entry(const WfDev& W, uint32_t slot) const { return W.ent[slot]; }
+    __device__ const SceneDev& scene(uint32_t e) const { return E[e].S; }
+    __device__ const PathParams& params(uint32_t e) const { return E[e].P; }
+    __device__ const FilmDev& film(uint32_t e) const { return E[e].F; }
+    __device__ uint32_t pass_slot(const WfDev& W, uint32_t slot) const { return W.pslot[slot]; }
 };
 
 // Queue push compacted per wavefront: one atomic per wave, lanes write at
@@ -79,8 +110,26 @@ __device__ inline void wave_push(uint32_t* queue, uint32_t* count, uint32_t slot
 
 __device__ inline V3 f3(const float4 v) { return v3(v.x, v.y, v.z); }
 
+// Camera sample of one path (TiledIntegrator::Render per sample,
+// integrator/TiledIntegrator.cpp:40-45) into lane `slot`.
+__device__ inline void start_path(const WfDev& W, const PathParams& P, uint32_t slot, int s, int px, int py) {
+    Seq q;
+    q.index = enum_index(P.en, (uint32_t)s, (uint32_t)px, (uint32_t)py);
+    q.cursor = 0;
+    const float u = P.en.sx * seq_next(P.smp, q) - (float)px;
+    const float v = P.en.sy * seq_next(P.smp, q) - (float)py;
+    V3 o, d;
+    generate_ray(P.cam, px, py, u, v, &o, &d);
+    W.o[slot] = make_float4(o.x, o.y, o.z, 0.f);
+    W.d[slot] = make_float4(d.x, d.y, d.z, 0.f);
+    W.T[slot] = make_float4(1.f, 1.f, 1.f, px + u);
+    W.L[slot] = make_float4(0.f, 0.f, 0.f, py + v);
+    W.seq[slot] = make_uint4((uint32_t)q.index, (uint32_t)(q.index >> 32), q.cursor, P.max_depth == 0 ? F_DONE : 0u);
+    W.pend3[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
 // ---------------------------------------------------------------- raygen
-// TiledIntegrator::Render per sample (integrator/TiledIntegrator.cpp:40-45).
+// One render: samples s0 .. s0+n_s-1 of every pixel, 8x8 pixel tiles per wave.
 __global__ void __launch_bounds__(256) k_wf_raygen(WfDev W, PathParams P, int s0, int n_s, int width, int height) {
     const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int tilesX = (width + 7) >> 3, tilesY = (height + 7) >> 3;
@@ -97,93 +146,46 @@ __global__ void __launch_bounds__(256) k_wf_raygen(WfDev W, PathParams P, int s0
         valid = px < width && py < height;
     }
     const uint32_t slot = (uint32_t)((int64_t)s_local * width * height + (int64_t)py * width + px);
-    if (valid) {
-        Seq q;
-        q.index = enum_index(P.en, (uint32_t)(s0 + s_local), (uint32_t)px, (uint32_t)py);
-        q.cursor = 0;
-        const float u = P.en.sx * seq_next(P.smp, q) - (float)px;
-        const float v = P.en.sy * seq_next(P.smp, q) - (float)py;
-        V3 o, d;
-        generate_ray(P.cam, px, py, u, v, &o, &d);
-        W.o[slot] = make_float4(o.x, o.y, o.z, 0.f);
-        W.d[slot] = make_float4(d.x, d.y, d.z, 0.f);
-        W.T[slot] = make_float4(1.f, 1.f, 1.f, px + u);
-        W.L[slot] = make_float4(0.f, 0.f, 0.f, py + v);
-        W.seq[slot] = make_uint4((uint32_t)q.index, (uint32_t)(q.index >> 32), q.cursor, P.max_depth == 0 ? F_DONE : 0u);
-        W.pend3[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
-    }
+    if (valid) start_path(W, P, slot, s0 + s_local, px, py);
     wave_push(W.q[Q_ACTIVE], &W.counts[Q_ACTIVE], slot, valid);
 }
 
-// ---------------------------------------------------------------- trace
-template <int STACK>
-__global__ void __launch_bounds__(256) k_wf_trace_closest(SceneDev S, WfDev W) {
-    __shared__ uint32_t stack[STACK * 256];
-    const uint32_t n = W.counts[Q_ACTIVE];
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (blockIdx.x * 256 >= n) return;  // whole block past the queue end
-    const bool valid = i < n;
-    uint32_t slot = 0;
-    int mtype = -1;
+// Batch: lanes [0, count) take global path indices base + lane.  Entry e owns
+// global indices [E[e].first, E[e].first + E[e].paths) in its own tile order
+// (sample-major, 8x8 pixel tiles); the lane records its entry and pass slot.
+__global__ void __launch_bounds__(256) k_wf_raygen_batch(WfDev W, ManyRenders R, int64_t base, int64_t count) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    bool valid = t < count;
+    uint32_t e = 0;
+    int s = 0, px = 0, py = 0;
     if (valid) {
-        slot = W.q[Q_ACTIVE][i];
-        const float4 o = W.o[slot], d = W.d[slot];
-        Hit h;
-        if (traverse_short<false, STACK>(S, f3(o), f3(d), 0.f, kInf, &h, stack + threadIdx.x, 256, W.spill + i, W.spill_stride)) {
-            W.hit[slot] = make_float4(h.t, __int_as_float(h.prim), h.b1, h.b2);
-            const uint32_t mat = __float_as_uint(S.shade[(size_t)h.prim * 8].w);
-            mtype = (int)S.mats[(size_t)mat * 32];
-        } else {
-            uint4 sq = W.seq[slot];
-            sq.w |= F_DONE;
-            W.seq[slot] = sq;
+        const int64_t g = base + t;
+        int lo = 0, hi = R.n - 1;  // last entry with first <= g
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (R.E[mid].first <= g) lo = mid;
+            else hi = mid - 1;
         }
+        e = (uint32_t)lo;
+        const BatchEntryDev& B = R.E[e];
+        const int64_t r = g - B.first;
+        const int64_t per_s = (int64_t)B.tilesX * B.tilesY * 64;
+        s = (int)(r / per_s);
+        const int64_t rr = r - (int64_t)s * per_s;
+        const int64_t tile = rr >> 6;
+        const int local = (int)(rr & 63);
+        px = (int)(tile % B.tilesX) * 8 + (local & 7);
+        py = (int)(tile / B.tilesX) * 8 + (local >> 3);
+        valid = r < B.paths && px < B.P.cam.width && py < B.P.cam.height;
     }
-#pragma unroll
-    for (int m = 0; m < 4; ++m) wave_push(W.q[Q_SHADE0 + m], &W.counts[Q_SHADE0 + m], slot, mtype == m);
-}
-
-template <int STACK>
-__global__ void __launch_bounds__(256) k_wf_trace_shadow(SceneDev S, WfDev W) {
-    __shared__ uint32_t stack[STACK * 256];
-    const uint32_t n = W.counts[Q_SHADOW];
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t slot = W.q[Q_SHADOW][i];
-    const float4 o = W.sh_o[slot], d = W.sh_d[slot];
-    Hit h;
-    if (traverse_short<true, STACK>(S, f3(o), f3(d), 0.f, o.w, &h, stack + threadIdx.x, 256, W.spill + i, W.spill_stride)) {
-        float4 p3 = W.pend3[slot];
-        p3.w = __uint_as_float(__float_as_uint(p3.w) | P_OCCLUDED);
-        W.pend3[slot] = p3;
+    const uint32_t slot = (uint32_t)t;
+    if (valid) {
+        const PathParams& P = R.E[e].P;
+        W.ent[slot] = e;
+        W.pslot[slot] = (uint32_t)((int64_t)s * P.cam.width * P.cam.height + (int64_t)py * P.cam.width + px);
+        start_path(W, P, slot, s, px, py);
     }
-}
-
-// MIS branch of DirectIllumination (integrator/Integrator.cpp:88-98): trace the
-// BSDF-sampled ray; if it hits the sampled light, L2 = Le(front) f Tr cos / pdf.
-template <int STACK>
-__global__ void __launch_bounds__(256) k_wf_trace_mis(SceneDev S, WfDev W) {
-    __shared__ uint32_t stack[STACK * 256];
-    const uint32_t n = W.counts[Q_MIS];
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t slot = W.q[Q_MIS][i];
-    const V3 ro = f3(W.mis_o[slot]), wi = f3(W.mis_d[slot]);
-    Hit h;
-    if (!traverse_short<false, STACK>(S, ro, wi, 0.f, kInf, &h, stack + threadIdx.x, 256, W.spill + i, W.spill_stride)) return;
-    const float4 p2 = W.pend2[slot];
-    const int light = __float_as_int(p2.w);
-    const int hl = __float_as_int(S.shade[(size_t)h.prim * 8 + 1].w);
-    if (hl != light) return;
-    const Isect li = query(S, ro, wi, h);
-    const V3 Le = v3(S.light_le[3 * light], S.light_le[3 * light + 1], S.light_le[3 * light + 2]);
-    const V3 Lem = dot(li.n, neg(wi)) > 0.f ? Le : v3(0.f, 0.f, 0.f);
-    const float4 p1 = W.pend1[slot];
-    const V3 L2 = Lem * f3(p1) * v3s(1.f) * p1.w / p2.x;
-    W.pend1[slot] = make_float4(L2.x, L2.y, L2.z, p1.w);
-    float4 p3 = W.pend3[slot];
-    p3.w = __uint_as_float(__float_as_uint(p3.w) | P_L2);
-    W.pend3[slot] = p3;
+    wave_push(W.q[Q_ACTIVE], &W.counts[Q_ACTIVE], slot, valid);
 }
 
 // ---------------------------------------------------------------- persistent trace
@@ -195,8 +197,8 @@ __global__ void __launch_bounds__(256) k_wf_trace_mis(SceneDev S, WfDev W) {
 // none of its lanes holds a ray.
 constexpr uint32_t kBatch = 64;
 
-template <int S, int MODE>
-__global__ void __launch_bounds__(256) k_wf_trace(SceneDev Sc, WfDev W, int refill) {
+template <int S, int MODE, class R>
+__global__ void __launch_bounds__(256) k_wf_trace(R Rd, WfDev W, int refill) {
     __shared__ uint32_t stack[S * 256];
     constexpr int qid = MODE == 0 ? Q_ACTIVE : (MODE == 1 ? Q_SHADOW : Q_MIS);
     const uint32_t n = W.counts[qid];
@@ -209,7 +211,9 @@ __global__ void __launch_bounds__(256) k_wf_trace(SceneDev Sc, WfDev W, int refi
     uint32_t batch_base = 0, batch_left = 0;  // wave-uniform
     bool fresh = false, fin = false;
     int mt = -1;
-    uint32_t slot = 0;
+    uint32_t slot = 0, e = 0;
+    SceneDev Sc;  // traversal reads nodes and tri only
+    if (!R::kBatch) Sc = Rd.scene(0);
     Trav t;
     while (true) {
         if (MODE == 0) {
@@ -248,6 +252,11 @@ __global__ void __launch_bounds__(256) k_wf_trace(SceneDev Sc, WfDev W, int refi
             if (fresh) {
                 fresh = false;
                 slot = W.q[qid][my];
+                e = Rd.entry(W, slot);
+                if (R::kBatch) {
+                    Sc.nodes = Rd.scene(e).nodes;
+                    Sc.tri = Rd.scene(e).tri;
+                }
                 if (MODE == 0) {
                     trav_init(t, f3(W.o[slot]), f3(W.d[slot]), 0.f, kInf);
                 } else if (MODE == 1) {
@@ -262,9 +271,10 @@ __global__ void __launch_bounds__(256) k_wf_trace(SceneDev Sc, WfDev W, int refi
                 if (r != 0) {
                     if (MODE == 0) {
                         if (t.best_prim >= 0) {
+                            const SceneDev& Sf = Rd.scene(e);
                             W.hit[slot] = make_float4(t.best, __int_as_float(t.best_prim), t.bb1, t.bb2);
-                            const uint32_t mat = __float_as_uint(Sc.shade[(size_t)t.best_prim * 8].w);
-                            mt = (int)Sc.mats[(size_t)mat * 32];
+                            const uint32_t mat = __float_as_uint(Sf.shade[(size_t)t.best_prim * 8].w);
+                            mt = (int)Sf.mats[(size_t)mat * 32];
                         } else {
                             mt = -1;
                             uint4 sq = W.seq[slot];
@@ -279,17 +289,18 @@ __global__ void __launch_bounds__(256) k_wf_trace(SceneDev Sc, WfDev W, int refi
                         }
                     } else if (t.best_prim >= 0) {
                         // MIS branch of DirectIllumination (integrator/Integrator.cpp:88-98)
+                        const SceneDev& Sf = Rd.scene(e);
                         const float4 p2 = W.pend2[slot];
                         const int light = __float_as_int(p2.w);
-                        const int hl = __float_as_int(Sc.shade[(size_t)t.best_prim * 8 + 1].w);
+                        const int hl = __float_as_int(Sf.shade[(size_t)t.best_prim * 8 + 1].w);
                         if (hl == light) {
                             Hit h;
                             h.t = t.best;
                             h.prim = t.best_prim;
                             h.b1 = t.bb1;
                             h.b2 = t.bb2;
-                            const Isect li = query(Sc, t.o, t.d, h);
-                            const V3 Le = v3(Sc.light_le[3 * light], Sc.light_le[3 * light + 1], Sc.light_le[3 * light + 2]);
+                            const Isect li = query(Sf, t.o, t.d, h);
+                            const V3 Le = v3(Sf.light_le[3 * light], Sf.light_le[3 * light + 1], Sf.light_le[3 * light + 2]);
                             const V3 Lem = dot(li.n, neg(t.d)) > 0.f ? Le : v3(0.f, 0.f, 0.f);
                             const float4 p1 = W.pend1[slot];
                             const V3 L2 = Lem * f3(p1) * v3s(1.f) * p1.w / p2.x;
@@ -310,8 +321,8 @@ __global__ void __launch_bounds__(256) k_wf_trace(SceneDev Sc, WfDev W, int refi
 }
 
 // ---------------------------------------------------------------- shade
-template <int MTYPE>
-__global__ void __launch_bounds__(256) k_wf_shade(SceneDev S, WfDev W, PathParams P) {
+template <int MTYPE, class R>
+__global__ void __launch_bounds__(256) k_wf_shade(R Rd, WfDev W) {
     const uint32_t n = W.counts[Q_SHADE0 + MTYPE];
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (blockIdx.x * 256 >= n) return;
@@ -320,6 +331,9 @@ __global__ void __launch_bounds__(256) k_wf_shade(SceneDev S, WfDev W, PathParam
     bool push_shadow = false, push_mis = false, push_next = false;
     if (valid) {
         slot = W.q[Q_SHADE0 + MTYPE][i];
+        const uint32_t e = Rd.entry(W, slot);
+        const SceneDev& S = Rd.scene(e);
+        const PathParams& P = Rd.params(e);
         const SampleSrc& M = P.smp;
         const V3 ro = f3(W.o[slot]), rd = f3(W.d[slot]);
         const float4 hv = W.hit[slot];
@@ -342,8 +356,8 @@ __global__ void __launch_bounds__(256) k_wf_shade(SceneDev S, WfDev W, PathParam
         uint32_t pflags = 0;
         if ((depth == 0 || delta) && it.light >= 0) {
             const V3 Le = v3(S.light_le[3 * it.light], S.light_le[3 * it.light + 1], S.light_le[3 * it.light + 2]);
-            const V3 e = dot(it.n, it.outDir) > 0.f ? Le : v3(0.f, 0.f, 0.f);
-            L = L + T * e;
+            const V3 em = dot(it.n, it.outDir) > 0.f ? Le : v3(0.f, 0.f, 0.f);
+            L = L + T * em;
         } else {
             const float uL = seq_next(M, q);
             const float u0 = seq_next(M, q), u1 = seq_next(M, q);
@@ -441,7 +455,8 @@ __global__ void __launch_bounds__(256) k_wf_shade(SceneDev S, WfDev W, PathParam
 // ---------------------------------------------------------------- resolve
 // L += T_i * (L1 w1 + L2 w2) / P(light) (integrator/Integrator.cpp:106,
 // integrator/PT.cpp:38), then finished paths go to the film.
-__global__ void __launch_bounds__(256) k_wf_resolve(WfDev W, FilmDev F) {
+template <class R>
+__global__ void __launch_bounds__(256) k_wf_resolve(R Rd, WfDev W) {
     const uint32_t n = W.counts[Q_ACTIVE];
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
@@ -462,11 +477,13 @@ __global__ void __launch_bounds__(256) k_wf_resolve(WfDev W, FilmDev F) {
     const uint4 sq = W.seq[slot];
     if (!(sq.w & F_DONE)) return;
     // Film::AddSample (visual/Film.h:61-87) through the pass buffer / side list.
+    const FilmDev& F = Rd.film(Rd.entry(W, slot));
+    const uint32_t ps = Rd.pass_slot(W, slot);
     const int WW = F.width, H = F.height;
     const float fx = W.T[slot].w, fy = Lv.w;
     const int64_t WH = (int64_t)WW * H;
-    const int s_local = (int)(slot / WH);
-    const int64_t pix = slot - (int64_t)s_local * WH;
+    const int s_local = (int)(ps / WH);
+    const int64_t pix = ps - (int64_t)s_local * WH;
     const int px = (int)(pix % WW), py = (int)(pix / WW);
     const float x = fx - 0.5f, y = fy - 0.5f;
     int x0 = (int)ceilf(x - 0.5f), x1 = (int)floorf(x + 0.5f);
@@ -476,16 +493,16 @@ __global__ void __launch_bounds__(256) k_wf_resolve(WfDev W, FilmDev F) {
     y0 = y0 < 0 ? 0 : y0;
     y1 = y1 > H - 1 ? H - 1 : y1;
     const bool own = (x0 == px && x1 == px && y0 == py && y1 == py);
-    F.pass[slot] = make_float4(Lv.x, Lv.y, Lv.z, own ? 1.f : 0.f);
+    F.pass[ps] = make_float4(Lv.x, Lv.y, Lv.z, own ? 1.f : 0.f);
     if (own) return;
     for (int yy = y0; yy <= y1; ++yy)
         for (int xx = x0; xx <= x1; ++xx) {
             const uint32_t k = atomicAdd(F.side_count, 1u);
             if (k < F.side_cap) {
-                SideEntry e;
-                e.key = ((uint64_t)((uint32_t)(yy * WW + xx)) << 32) | (uint64_t)slot;
-                e.L = make_float4(Lv.x, Lv.y, Lv.z, 0.f);
-                F.side[k] = e;
+                SideEntry se;
+                se.key = ((uint64_t)((uint32_t)(yy * WW + xx)) << 32) | (uint64_t)ps;
+                se.L = make_float4(Lv.x, Lv.y, Lv.z, 0.f);
+                F.side[k] = se;
             }
         }
 }
@@ -507,10 +524,10 @@ __global__ void k_wf_reset(WfDev W) {
 }
 
 // ---------------------------------------------------------------- host side
-static WfDev g_view(Scene* sc, size_t* used) {
+static WfDev g_view(const WavefrontState& wf, bool batch, size_t* used) {
     WfDev W;
-    char* base = (char*)sc->wf.mem;
-    const int64_t n = sc->wf.capacity;
+    char* base = (char*)wf.mem;
+    const int64_t n = wf.capacity;
     size_t off = 0;
     auto take = [&](size_t bytes) {
         void* p = base + off;
@@ -531,80 +548,92 @@ static WfDev g_view(Scene* sc, size_t* used) {
     W.sh_d = (float4*)take(n * 16);
     W.mis_o = (float4*)take(n * 16);
     W.mis_d = (float4*)take(n * 16);
+    W.ent = batch ? (uint32_t*)take(n * 4) : nullptr;
+    W.pslot = batch ? (uint32_t*)take(n * 4) : nullptr;
     for (int q = 0; q < Q_COUNT; ++q) W.q[q] = (uint32_t*)take(n * 4);
     W.counts = (uint32_t*)take(64);
     W.capacity = n;
+    W.spill = (uint32_t*)wf.spill;
+    W.spill_stride = 0;
     *used = off;
     return W;
 }
 
-static size_t WfBytes(int64_t n) {
-    const size_t per = 14 * 16 + Q_COUNT * 4;
-    return (size_t)n * per + 256 * (14 + Q_COUNT + 2) + 4096;
+static size_t WfBytes(int64_t n, bool batch) {
+    const size_t per = 14 * 16 + Q_COUNT * 4 + (batch ? 8 : 0);
+    return (size_t)n * per + 256 * (14 + Q_COUNT + 4) + 4096;
 }
 
-void WavefrontFree(Scene* sc) {
-    DevFree(sc->wf.mem);
-    DevFree(sc->wf.spill);
-    sc->wf = WavefrontState();
+void WavefrontStateFree(WavefrontState* wf) {
+    DevFree(wf->mem);
+    DevFree(wf->spill);
+    *wf = WavefrontState();
 }
+void WavefrontFree(Scene* sc) { WavefrontStateFree(&sc->wf); }
 
 constexpr int kStackLds = 16;
 
-// Persistent trace grids: as many 256-lane blocks as the device keeps resident.
+// Persistent trace grids: as many 256-lane blocks as the device keeps resident
+// for the most resident of the trace kernels (the spill buffer is sized for it).
 static unsigned PersistBlocks(Ctx* c) {
     static int per_cu = 0;
     if (per_cu == 0) {
-        int a = 0, b = 0, d = 0;
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_wf_trace<kStackLds, 0>, 256, 0);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_wf_trace<kStackLds, 1>, 256, 0);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&d, k_wf_trace<kStackLds, 2>, 256, 0);
-        per_cu = std::max(1, std::max(a, std::max(b, d)));
+        int k[6] = {0, 0, 0, 0, 0, 0};
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&k[0], k_wf_trace<kStackLds, 0, OneRender>, 256, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&k[1], k_wf_trace<kStackLds, 1, OneRender>, 256, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&k[2], k_wf_trace<kStackLds, 2, OneRender>, 256, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&k[3], k_wf_trace<kStackLds, 0, ManyRenders>, 256, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&k[4], k_wf_trace<kStackLds, 1, ManyRenders>, 256, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&k[5], k_wf_trace<kStackLds, 2, ManyRenders>, 256, 0);
+        per_cu = std::max(1, *std::max_element(k, k + 6));
     }
     return (unsigned)(per_cu * c->num_cu);
 }
 static int64_t WfLanes(Ctx* c) { return (int64_t)PersistBlocks(c) * 256; }
 
-static int WfPrepare(Ctx* c, Scene* sc, int64_t n) {
-    if (sc->wf.capacity >= n && sc->wf.mem) return GY_OK;
-    WavefrontFree(sc);
-    const size_t bytes = WfBytes(n);
-    int rc = DevAlloc(c, &sc->wf.mem, bytes);
+static int WfPrepareState(Ctx* c, WavefrontState& wf, int64_t n, bool batch) {
+    if (wf.capacity >= n && wf.mem && wf.batch == batch) return GY_OK;
+    WavefrontStateFree(&wf);
+    const size_t bytes = WfBytes(n, batch);
+    int rc = DevAlloc(c, &wf.mem, bytes);
     if (rc != GY_OK) return rc;
     const size_t sbytes = (size_t)WfLanes(c) * 64 * 4;
-    rc = DevAlloc(c, &sc->wf.spill, sbytes);
+    rc = DevAlloc(c, &wf.spill, sbytes);
     if (rc != GY_OK) return rc;
-    sc->wf.capacity = n;
-    sc->wf.bytes = bytes;
-    sc->wf.spill_bytes = sbytes;
+    wf.capacity = n;
+    wf.bytes = bytes;
+    wf.spill_bytes = sbytes;
+    wf.batch = batch;
     return GY_OK;
 }
 
-template <int STACK>
-static int WfBounceLoop(Ctx* c, Scene* sc, const WfDev& W0, const PathParams& P, const gy_render_params& p, unsigned blocks) {
-    const FilmDev F = FilmView(sc);
+// The bounce loop of one pass: `depth_max` bounces over the queues.  The
+// active / next queue pointers swap on the host between bounces, fixed per
+// launch, so the whole sequence can be recorded in a graph.
+template <int STACK, class R>
+static int WfBounceLoop(Ctx* c, const R& Rd, const WfDev& W0, int depth_max, int refill, unsigned blocks,
+                        unsigned long long* counters) {
     WfDev W = W0;
     const unsigned pblocks = PersistBlocks(c);
-    const int refill = (p.refill_threshold > 0 && p.refill_threshold <= 64) ? p.refill_threshold : 1;
-    for (int depth = 0; depth < p.max_depth; ++depth) {
+    for (int depth = 0; depth < depth_max; ++depth) {
         TimerBegin(c, kKernTraceClosest);
-        hipLaunchKernelGGL((k_wf_trace<STACK, 0>), dim3(pblocks), dim3(256), 0, c->stream, sc->dev, W, refill);
+        hipLaunchKernelGGL((k_wf_trace<STACK, 0, R>), dim3(pblocks), dim3(256), 0, c->stream, Rd, W, refill);
         TimerEnd(c);
         TimerBegin(c, kKernShade);
-        hipLaunchKernelGGL(k_wf_shade<0>, dim3(blocks), dim3(256), 0, c->stream, sc->dev, W, P);
-        hipLaunchKernelGGL(k_wf_shade<1>, dim3(blocks), dim3(256), 0, c->stream, sc->dev, W, P);
-        hipLaunchKernelGGL(k_wf_shade<2>, dim3(blocks), dim3(256), 0, c->stream, sc->dev, W, P);
-        hipLaunchKernelGGL(k_wf_shade<3>, dim3(blocks), dim3(256), 0, c->stream, sc->dev, W, P);
+        hipLaunchKernelGGL((k_wf_shade<0, R>), dim3(blocks), dim3(256), 0, c->stream, Rd, W);
+        hipLaunchKernelGGL((k_wf_shade<1, R>), dim3(blocks), dim3(256), 0, c->stream, Rd, W);
+        hipLaunchKernelGGL((k_wf_shade<2, R>), dim3(blocks), dim3(256), 0, c->stream, Rd, W);
+        hipLaunchKernelGGL((k_wf_shade<3, R>), dim3(blocks), dim3(256), 0, c->stream, Rd, W);
         TimerEnd(c);
         TimerBegin(c, kKernTraceShadow);
-        hipLaunchKernelGGL((k_wf_trace<STACK, 1>), dim3(pblocks), dim3(256), 0, c->stream, sc->dev, W, refill);
+        hipLaunchKernelGGL((k_wf_trace<STACK, 1, R>), dim3(pblocks), dim3(256), 0, c->stream, Rd, W, refill);
         TimerEnd(c);
         TimerBegin(c, kKernTraceMis);
-        hipLaunchKernelGGL((k_wf_trace<STACK, 2>), dim3(pblocks), dim3(256), 0, c->stream, sc->dev, W, refill);
+        hipLaunchKernelGGL((k_wf_trace<STACK, 2, R>), dim3(pblocks), dim3(256), 0, c->stream, Rd, W, refill);
         TimerEnd(c);
         TimerBegin(c, kKernOther);
-        hipLaunchKernelGGL(k_wf_resolve, dim3(blocks), dim3(256), 0, c->stream, W, F);
-        hipLaunchKernelGGL(k_wf_advance, dim3(1), dim3(64), 0, c->stream, W, sc->film.counters);
+        hipLaunchKernelGGL((k_wf_resolve<R>), dim3(blocks), dim3(256), 0, c->stream, Rd, W);
+        hipLaunchKernelGGL(k_wf_advance, dim3(1), dim3(64), 0, c->stream, W, counters);
         TimerEnd(c);
         int rc = CheckLaunch(c, "wavefront bounce");
         if (rc != GY_OK) return rc;
@@ -616,20 +645,25 @@ static int WfBounceLoop(Ctx* c, Scene* sc, const WfDev& W0, const PathParams& P,
     return GY_OK;
 }
 
+static int RefillOf(int r) { return (r > 0 && r <= 64) ? r : 1; }
+
 int WavefrontPass(Ctx* c, Scene* sc, const PathParams& P, const gy_render_params& p, int s0, int n_s) {
     const int width = P.cam.width, height = P.cam.height;
     const int tilesX = (width + 7) >> 3, tilesY = (height + 7) >> 3;
     const int64_t threads = (int64_t)tilesX * tilesY * 64 * n_s;
     const int64_t cap = (int64_t)width * height * n_s;
     if (cap >= (1ll << 31)) return Fail(GY_ERR_UNSUPPORTED, "wavefront pass larger than 2^31 paths");
-    int rc = WfPrepare(c, sc, std::max<int64_t>(cap, 1));
+    int rc = WfPrepareState(c, sc->wf, std::max<int64_t>(cap, 1), false);
     if (rc != GY_OK) return rc;
     size_t used = 0;
-    WfDev W = g_view(sc, &used);
-    W.spill = (uint32_t*)sc->wf.spill;
+    WfDev W = g_view(sc->wf, false, &used);
     W.spill_stride = WfLanes(c);
     if ((size_t)W.spill_stride * 64 * 4 > sc->wf.spill_bytes) return Fail(GY_ERR_ALLOC, "wavefront spill buffer too small");
     if (used > sc->wf.bytes) return Fail(GY_ERR_ALLOC, "wavefront state layout exceeds its allocation");
+    OneRender Rd;
+    Rd.S = sc->dev;
+    Rd.P = P;
+    Rd.F = FilmView(sc);
     hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(64), 0, c->stream, W);
     TimerBegin(c, kKernRaygen);
     hipLaunchKernelGGL(k_wf_raygen, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, c->stream, W, P, s0, n_s, width, height);
@@ -639,11 +673,78 @@ int WavefrontPass(Ctx* c, Scene* sc, const PathParams& P, const gy_render_params
     const unsigned blocks = (unsigned)((cap + 255) / 256);
     if (p.max_depth == 0) {
         // no bounce: every path finishes with L = 0
-        hipLaunchKernelGGL(k_wf_resolve, dim3(blocks), dim3(256), 0, c->stream, W, FilmView(sc));
+        hipLaunchKernelGGL((k_wf_resolve<OneRender>), dim3(blocks), dim3(256), 0, c->stream, Rd, W);
         return CheckLaunch(c, "k_wf_resolve");
     }
     if (sc->stats.max_depth + 1 > 64) return Fail(GY_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
-    return WfBounceLoop<kStackLds>(c, sc, W, P, p, blocks);
+    return WfBounceLoop<kStackLds>(c, Rd, W, p.max_depth, RefillOf(p.refill_threshold), blocks, sc->film.counters);
+}
+
+// ---------------------------------------------------------------- batch
+// The whole batch: clear every entry's film, run the lane space in passes of
+// `lanes` paths (raygen + bounce loop to the largest depth), resolve every
+// entry's film.  No host round trip, so it is recorded into a graph once.
+static int BatchSequence(Ctx* c, Batch* b) {
+    const int64_t lanes = b->lanes;
+    size_t used = 0;
+    WfDev W = g_view(b->wf, true, &used);
+    W.spill_stride = WfLanes(c);
+    if ((size_t)W.spill_stride * 64 * 4 > b->wf.spill_bytes) return Fail(GY_ERR_ALLOC, "wavefront spill buffer too small");
+    if (used > b->wf.bytes) return Fail(GY_ERR_ALLOC, "wavefront state layout exceeds its allocation");
+    ManyRenders Rd;
+    Rd.E = (const BatchEntryDev*)b->table;
+    Rd.n = (int32_t)b->entries.size();
+    int rc = GY_OK;
+    for (auto& en : b->entries) {
+        if (rc == GY_OK) rc = FillF4(c, en.film.film, (int64_t)en.film.width * en.film.height);
+        if (rc == GY_OK) rc = Fill32(c, en.film.side_count, 0u, 4);
+    }
+    if (rc == GY_OK) rc = Fill32(c, b->counters, 0u, 32);
+    if (rc != GY_OK) return rc;
+    const unsigned blocks = (unsigned)((lanes + 255) / 256);
+    for (int64_t base = 0; base < b->total_paths && rc == GY_OK; base += lanes) {
+        const int64_t count = std::min(lanes, b->total_paths - base);
+        hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(64), 0, c->stream, W);
+        TimerBegin(c, kKernRaygen);
+        hipLaunchKernelGGL(k_wf_raygen_batch, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, c->stream, W, Rd, base, count);
+        TimerEnd(c);
+        rc = CheckLaunch(c, "k_wf_raygen_batch");
+        if (rc == GY_OK) rc = WfBounceLoop<kStackLds>(c, Rd, W, b->depth_max, b->refill, blocks, b->counters);
+    }
+    for (auto& en : b->entries)
+        if (rc == GY_OK) rc = FilmResolveState(c, en.film, en.params.spp);
+    return rc;
+}
+
+int BatchRender(Ctx* c, Batch* b) {
+    int rc = WfPrepareState(c, b->wf, b->lanes, true);
+    if (rc != GY_OK) return rc;
+    if (!b->use_graph) return BatchSequence(c, b);
+    if (!b->graph_exec) {
+        // record the sequence once; replays reuse it (fill kernels only, no memset nodes)
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return HipFail(e, "hipStreamBeginCapture");
+        c->capturing = true;
+        rc = BatchSequence(c, b);
+        c->capturing = false;
+        e = hipStreamEndCapture(c->stream, &g);
+        if (rc != GY_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return HipFail(e, "hipStreamEndCapture");
+        e = hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            b->graph_exec = nullptr;
+            return HipFail(e, "hipGraphInstantiate");
+        }
+    }
+    TimerBegin(c, kKernOther);
+    const hipError_t e = hipGraphLaunch(b->graph_exec, c->stream);
+    TimerEnd(c);
+    return e == hipSuccess ? GY_OK : HipFail(e, "hipGraphLaunch");
 }
 
 }  // namespace gy

@@ -176,6 +176,27 @@ int gy_li_samples(gy_scene* scene, const gy_camera_desc* cam, const gy_render_pa
  * and shading points, counted by an instrumented copy of the kernels. */
 int gy_count_traffic(gy_scene* scene, const gy_camera_desc* cam, const gy_render_params* p, gy_render_stats* out);
 
+/* ---- batched rendering ------------------------------------------------------
+ * No counterpart in the reference (TiledIntegrator::Render draws one image per
+ * call, integrator/TiledIntegrator.cpp:7-60).  A batch holds many renders
+ * (scene, view, sample budget, depth) of uneven size; gy_batch_render traces all
+ * of their paths in one lane space with the wavefront kernels (persistent trace
+ * waves steal queue entries of any render), records the whole sequence into a
+ * hipGraph on the first call and replays it afterwards.  Each entry's film
+ * equals what gy_render gives for that entry alone.  Scenes must outlive the
+ * batch; adding an entry re-records the graph. */
+typedef struct gy_batch gy_batch;
+typedef struct {
+    int64_t lanes_per_pass;   /* paths in flight per pass: 0 = automatic (32M) */
+    int32_t refill_threshold; /* persistent trace refill (0 = automatic, 1..64) */
+    int32_t use_graph;        /* 1: record once into a hipGraph and replay */
+} gy_batch_params;
+int gy_batch_create(gy_context* ctx, const gy_batch_params* p, gy_batch** out);
+int gy_batch_add(gy_batch* b, gy_scene* scene, const gy_camera_desc* cam, const gy_render_params* p, int32_t* entry);
+int gy_batch_render(gy_batch* b);
+int gy_batch_film_download(gy_batch* b, int32_t entry, float* color, float* weight);
+int gy_batch_destroy(gy_batch* b);
+
 /* ---- samplers (SobolSampler::Sample, sampler/Sobol.h:66-77;
  *      SobolEnum::GetIndex, sampler/SobolEnum.h:24-28; HaltonSampler,
  *      HaltonEnum) ------------------------------------------------------------ */
@@ -190,7 +211,7 @@ uint64_t gy_halton_enum_index(uint32_t width, uint32_t height, uint64_t i, uint3
 
 /* ---- procedural scenes (the benchmark configurations) ---------------------- */
 typedef struct gy_generated gy_generated;
-enum { GY_SCENE_CORNELL = 0, GY_SCENE_ICOSPHERE_GGX = 1 };
+enum { GY_SCENE_CORNELL = 0, GY_SCENE_ICOSPHERE_GGX = 1, GY_SCENE_BATCH_MEMBER = 2 };
 int gy_generate_scene(int kind, uint32_t seed, int32_t param, gy_generated** out);
 int gy_generated_desc(gy_generated* g, gy_scene_desc* desc, gy_camera_desc* cam, gy_render_params* params);
 int gy_generated_free(gy_generated* g);

@@ -30,6 +30,7 @@ def test_library_exports_every_declared_symbol():
 def test_struct_sizes_match_header_layout():
     assert ctypes.sizeof(_lib.CameraDesc) == 52
     assert ctypes.sizeof(_lib.RenderParams) == 32
+    assert ctypes.sizeof(_lib.BatchParams) == 16
     assert g.RAY_DTYPE.itemsize == 32 and g.HIT_DTYPE.itemsize == 16
 
 
@@ -90,3 +91,27 @@ def test_cpp_integrator_adapter_without_device():
                        env=dict(os.environ, HIP_VISIBLE_DEVICES="-1"))
     assert r.returncode == 0, r.stdout + r.stderr
     assert "no device" in r.stdout
+
+
+def test_batch_workload_members_follow_the_configuration():
+    # 64 members: triangle counts log-uniform in [1e3, 2e6] (plus the 12-triangle
+    # room and light and 12 per box), resolution, spp and depth from their sets,
+    # about a quarter of the objects rough dielectric
+    tris, res, spp, depth, diel, objs = [], set(), set(), set(), 0, 0
+    for s in range(64):
+        sa, cam, prm = g.generate_scene(g.GY_SCENE_BATCH_MEMBER, s, 0)
+        assert cam.width == cam.height and cam.width in (128, 256, 512, 1024)
+        assert prm.spp in (4, 8, 16, 32, 64) and 4 <= prm.max_depth <= 12
+        res.add(cam.width)
+        spp.add(prm.spp)
+        depth.add(prm.max_depth)
+        boxes = sa.mats.shape[0] - 5  # 4 room materials, then one per object: icosphere, boxes
+        n_ico = sa.n_tri - 12 - 12 * boxes
+        assert 1000 <= n_ico <= 2_000_000, n_ico
+        tris.append(n_ico)
+        objs += boxes + 1
+        diel += int((sa.mats[4:, 0] == 3).sum())
+    assert len(res) == 4 and len(spp) == 5 and len(depth) >= 7
+    lt = np.log10(tris)
+    assert lt.min() < 3.5 and lt.max() > 5.5
+    assert 0.15 < diel / objs < 0.35, diel / objs

@@ -389,3 +389,74 @@ def test_halton_film_wavefront_bit_identical_and_matches_oracle(gpu_ctx):
     oc, ow, _, _ = O.OracleScene(sa).render(cam, O.params(prm.spp, prm.max_depth, prm.rr_depth, 0, 8, 1, sampler=1))
     assert np.array_equal(a[1], ow.astype(np.float32))
     assert np.abs(a[0] - oc).sum() / np.abs(oc).sum() < 2e-3
+
+
+# ---------------------------------------------------------------- batched rendering
+def _batch_entries(gpu_ctx):
+    """Renders of uneven size: scenes, views, samplers, depths, spp."""
+    from tests.scene_util import mixed_material_scene
+
+    out = []  # (scene arrays, camera, params)
+    sa, cam, prm = g.generate_scene(g.GY_SCENE_CORNELL)
+    cam.width, cam.height, prm.spp = 64, 48, 8
+    out.append((sa, cam, prm))
+    sa2, cam2, prm2 = g.generate_scene(g.GY_SCENE_CORNELL)
+    cam2.width, cam2.height, prm2.spp, prm2.max_depth = 40, 40, 3, 7   # the same scene, another view
+    cam2.pos[0] = 0.3
+    out.append((sa, cam2, prm2))
+    sa, cam, prm = g.generate_scene(g.GY_SCENE_ICOSPHERE_GGX, 0, 5)
+    cam.width, cam.height, prm.spp, prm.max_depth = 96, 64, 4, 6
+    out.append((sa, cam, prm))
+    sa, cam, prm = mixed_material_scene(res=56)
+    prm.spp, prm.sampler = 4, g.GY_SAMPLER_HALTON
+    out.append((sa, cam, prm))
+    for seed in range(4):
+        out.append(g.generate_scene(g.GY_SCENE_BATCH_MEMBER, seed, 3000))
+    return out
+
+
+def test_batch_films_equal_single_renders(gpu_ctx):
+    # Every entry's film equals its own single render (same pass split: one
+    # pass) bit for bit; passes of 40000 paths straddle entries; the recorded
+    # graph is replayed three times with eager renders between the replays.
+    entries = _batch_entries(gpu_ctx)
+    scenes, refs = {}, []
+    for sa, cam, prm in entries:
+        key = id(sa)
+        if key not in scenes:
+            scenes[key] = g.Scene(gpu_ctx, sa)
+        prm.mode = g.GY_RENDER_WAVEFRONT
+        prm.samples_per_pass = prm.spp
+        refs.append(scenes[key].render(cam, prm))
+    try:
+        for use_graph in (True, False):
+            with g.Batch(gpu_ctx, lanes_per_pass=40000, use_graph=use_graph) as b:
+                ids = [b.add(scenes[id(sa)], cam, prm) for sa, cam, prm in entries]
+                for rep in range(3 if use_graph else 1):
+                    b.render()
+                    st = gpu_ctx.stats()
+                    assert st.paths == sum(c.width * c.height * p.spp for _, c, p in entries)
+                    for i, e in enumerate(ids):
+                        color, weight = b.film(e)
+                        assert np.array_equal(weight, refs[i][1]), (use_graph, rep, i)
+                        assert np.array_equal(color, refs[i][0]), (use_graph, rep, i)
+                    # eager work on the same stream between replays
+                    sa, cam, prm = entries[0]
+                    scenes[id(sa)].render(cam, prm)
+    finally:
+        for sc in scenes.values():
+            sc.close()
+
+
+def test_batch_errors(gpu_ctx):
+    sa, cam, prm = g.generate_scene(g.GY_SCENE_CORNELL)
+    with g.Scene(gpu_ctx, sa) as sc, g.Batch(gpu_ctx) as b:
+        with pytest.raises(g.GyError):
+            b.render()  # empty
+        prm.max_depth = 0
+        with pytest.raises(g.GyError):
+            b.add(sc, cam, prm)
+        prm.max_depth = 5
+        e = b.add(sc, cam, prm)
+        with pytest.raises(g.GyError):
+            b.film(e)  # not rendered yet
