@@ -5,7 +5,7 @@ One step = one full render (67,108,864 camera paths) of that image with the
 scene and BVH already resident in HBM.  Prints one JSON line on stdout.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode wavefront|megakernel]
-                  [--no-cpu-baseline] [--level 8]
+                  [--no-cpu-baseline] [--level 8] [--workload icosphere|batch64]
 """
 import argparse
 import json
@@ -70,6 +70,79 @@ def cpu_baseline(sa, cam, prm, rows):
     }
 
 
+def batch_main(args):
+    """The 64-scene batch workload: every member traced in one lane space
+    (gy_batch_render, recorded into a hipGraph once and replayed), against the
+    same members rendered one call each with the wavefront and the megakernel."""
+    import gyunity_amd as g
+
+    ctx = g.Context(0)
+    log("device:", ctx.device_name())
+    t0 = time.perf_counter()
+    members = [g.generate_scene(g.GY_SCENE_BATCH_MEMBER, s, 0) for s in range(args.batch_size)]
+    log("generated %d members in %.1f s" % (len(members), time.perf_counter() - t0))
+    scenes = [g.Scene(ctx, sa) for sa, _, _ in members]
+    paths = sum(cam.width * cam.height * prm.spp for _, cam, prm in members)
+    tris = sum(sa.n_tri for sa, _, _ in members)
+    log("scenes resident: %d triangles, %d paths per batch render" % (tris, paths))
+    batch = g.Batch(ctx, lanes_per_pass=args.batch_lanes, refill_threshold=args.refill, use_graph=not args.batch_no_graph)
+    for sc, (_, cam, prm) in zip(scenes, members):
+        batch.add(sc, cam, prm)
+    for i in range(args.warmup):
+        batch.render()
+        log("batch warmup %d done" % i)
+    rays = 0
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        batch.render()
+        st = ctx.stats()
+        rays = st.rays
+        log("batch step %d: %.1f ms, %.1f Mrays" % (i, st.total_ms, st.rays / 1e6))
+    ms_batch = (time.perf_counter() - t0) * 1e3 / args.steps
+    per_scene = {}
+    for mode, name in ((g.GY_RENDER_WAVEFRONT, "wavefront"), (g.GY_RENDER_MEGAKERNEL, "megakernel")):
+        for sc, (_, cam, prm) in zip(scenes, members):  # warm-up: allocations, first launches
+            prm.mode = mode
+            sc.render(cam, prm, download=False)
+        t0 = time.perf_counter()
+        for sc, (_, cam, prm) in zip(scenes, members):
+            prm.mode = mode
+            sc.render(cam, prm, download=False)
+        per_scene[name] = (time.perf_counter() - t0) * 1e3
+        log("one call per member, %s: %.1f ms" % (name, per_scene[name]))
+    batch.close()
+    for sc in scenes:
+        sc.close()
+    ctx.close()
+    out = {
+        "metric": "Mpaths/s at fixed RMSE: batched wavefront vs plain megakernel; adjoint s/grad",
+        "value": round(paths / (ms_batch / 1e3) / 1e6, 3),
+        "unit": "Mpaths/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_batch, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: %d procedural batch members (seeds 0..%d)" % (args.batch_size, args.batch_size - 1),
+        "config": {
+            "workload": "batch of %d scenes: displaced icospheres (1e3..2e6 triangles, log-uniform) + 0-20 boxes, "
+                        "GGX conductor / 25%% rough dielectric, 128^2..1024^2, 4..64 spp, depth 4..12" % args.batch_size,
+            "integrator": "batched wavefront (one lane space, %s)" % ("eager launches" if args.batch_no_graph else "hipGraph replay"),
+            "paths_per_step": paths,
+            "triangles": tris,
+            "mrays_per_s": round(rays / (ms_batch / 1e3) / 1e6, 2),
+            "per_member_wavefront_mpaths_per_s": round(paths / (per_scene["wavefront"] / 1e3) / 1e6, 3),
+            "per_member_megakernel_mpaths_per_s": round(paths / (per_scene["megakernel"] / 1e3) / 1e6, 3),
+            "same_estimator": "each member's batch film equals its own render bit for bit (tests/test_gpu_parity.py)",
+            "parallelism": "single GPU",
+        },
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,6 +155,11 @@ def main():
     ap.add_argument("--single", action="store_true", help="time only --mode, not the other integrator")
     ap.add_argument("--refill", type=int, default=0, help="wavefront refill threshold (0 = library default)")
     ap.add_argument("--spp-pass", type=int, default=0, help="samples per pass (0 = library default)")
+    ap.add_argument("--workload", default="icosphere", choices=["icosphere", "batch64"],
+                    help="icosphere: the 1.31M-triangle GGX render; batch64: the 64-scene batch")
+    ap.add_argument("--batch-size", type=int, default=64)
+    ap.add_argument("--batch-lanes", type=int, default=0, help="batch paths per pass (0 = library default)")
+    ap.add_argument("--batch-no-graph", action="store_true", help="launch the batch sequence eagerly instead of replaying its graph")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="after the run, copy MIB MiB with the library's plain copy kernel (k_copy_probe), "
                          "the known-size reference for HBM counter runs")
@@ -89,6 +167,9 @@ def main():
     if args.gpus != 1:
         log("bench.py measures one GPU: this build runs on a single MI355X; --gpus %d is not supported" % args.gpus)
         sys.exit(2)
+    if args.workload == "batch64":
+        batch_main(args)
+        return
 
     import gyunity_amd as g
 
