@@ -5,6 +5,7 @@ there is no fallback path.
 """
 import ctypes as C
 import os
+import sys
 
 import numpy as np
 
@@ -24,6 +25,7 @@ GY_RENDER_MEGAKERNEL = 0
 GY_RENDER_WAVEFRONT = 1
 GY_SAMPLER_SOBOL = 0
 GY_SCENE_BATCH_MEMBER = 2
+GY_SCENE_ADJOINT = 3
 GY_SAMPLER_HALTON = 1
 FAULT_NONE, FAULT_ALLOC, FAULT_LAUNCH, FAULT_SCENE = 0, 1, 2, 3
 
@@ -113,6 +115,7 @@ RAY_DTYPE = np.dtype([("org", "<f4", 3), ("tmin", "<f4"), ("dir", "<f4", 3), ("t
 HIT_DTYPE = np.dtype([("t", "<f4"), ("prim", "<i4"), ("b1", "<f4"), ("b2", "<f4")])
 
 _lib = None
+_torch_first = False
 
 
 def lib():
@@ -124,6 +127,10 @@ def lib():
         raise RuntimeError(
             "libgyunity.so is not built (expected at %s); run __graft_entry__.build() or make -C gyunity_amd" % LIB_PATH
         )
+    global _torch_first
+    # libgyunity.so and torch both need libamdhip64.so.7; whichever loads first
+    # provides the HIP runtime for the process (see torch_render.py)
+    _torch_first = "torch" in sys.modules
     L = C.CDLL(LIB_PATH)
     vp, i32, i64, u32, u64, f32 = C.c_void_p, C.c_int32, C.c_int64, C.c_uint32, C.c_uint64, C.c_float
     P = C.POINTER
@@ -145,6 +152,13 @@ def lib():
         "gy_count_visits": (C.c_int, [vp, vp, i64, P(i64), P(i64)]),
         "gy_render": (C.c_int, [vp, P(CameraDesc), P(RenderParams), P(f32), P(f32)]),
         "gy_film_download": (C.c_int, [vp, P(f32), P(f32)]),
+        "gy_scene_texture_floats": (i64, [vp]),
+        "gy_ggx_dlogf_dalpha": (f32, [f32, P(f32), P(f32)]),
+        "gy_dalpha_droughness": (f32, [f32]),
+        "gy_scene_set_textures": (C.c_int, [vp, P(f32), i64]),
+        "gy_scene_set_textures_device": (C.c_int, [vp, vp, i64]),
+        "gy_adjoint": (C.c_int, [vp, P(CameraDesc), P(RenderParams), P(f32), P(f32)]),
+        "gy_adjoint_device": (C.c_int, [vp, P(CameraDesc), P(RenderParams), vp, vp]),
         "gy_batch_create": (C.c_int, [vp, P(BatchParams), P(vp)]),
         "gy_batch_add": (C.c_int, [vp, vp, P(CameraDesc), P(RenderParams), P(i32)]),
         "gy_batch_render": (C.c_int, [vp]),
@@ -251,7 +265,13 @@ def generate_scene(kind, seed=0, param=0):
             arr(d.mats, MAT_WORDS * d.n_mat, np.uint32),
             arr(d.light_tri, d.n_light, np.int32),
             arr(d.light_le, 3 * d.n_light, np.float32),
+            uv=arr(d.uv, 6 * n, np.float32) if bool(d.uv) else None,
+            tex_info=arr(d.tex_info, 4 * d.n_tex, np.int32) if d.n_tex > 0 else None,
         )
+        if d.n_tex > 0:
+            ti = sa.tex_info
+            total = int((ti[:, 3] + ti[:, 0] * ti[:, 1] * ti[:, 2]).max())
+            sa.tex_data = arr(d.tex_data, total, np.float32)
     finally:
         L.gy_generated_free(h)
     return sa, cam, prm
@@ -437,6 +457,22 @@ class Scene:
             return color, weight
         check(lib().gy_render(self.h, C.byref(cam), C.byref(params), C.POINTER(C.c_float)(), C.POINTER(C.c_float)()))
         return None
+
+    def texture_floats(self):
+        return lib().gy_scene_texture_floats(self.h)
+
+    def set_textures(self, data):
+        data = np.ascontiguousarray(data, dtype=np.float32)
+        check(lib().gy_scene_set_textures(self.h, _fp(data), data.size))
+
+    def adjoint(self, cam, params, dimage):
+        """Gradient of a loss w.r.t. the texture floats, given dl/dI (H, W, 3)
+        for the film of the last render of this camera and these parameters."""
+        dimage = np.ascontiguousarray(dimage, dtype=np.float32)
+        assert dimage.shape == (cam.height, cam.width, 3)
+        grad = np.zeros(self.texture_floats(), dtype=np.float32)
+        check(lib().gy_adjoint(self.h, C.byref(cam), C.byref(params), _fp(dimage), _fp(grad)))
+        return grad
 
     def film(self, W, H):
         color = np.zeros((H, W, 3), dtype=np.float32)

@@ -95,6 +95,10 @@ struct Scene {
     std::vector<uint32_t> tri_order;
     FilmState film;
     WavefrontState wf;
+    int64_t tex_floats = 0;     // floats of texture data on the device
+    bool film_valid = false;    // the film holds the last gy_render of film_cam / film_params
+    gy_camera_desc film_cam = {};
+    gy_render_params film_params = {};
 };
 
 int HipFail(hipError_t e, const char* what);
@@ -107,6 +111,7 @@ int FillF4(Ctx* c, float4* p, int64_t n);  // zero n float4
 int CheckLaunch(Ctx* c, const char* what);
 void TimerBegin(Ctx* c, int cat);
 void TimerEnd(Ctx* c);
+void TimersCollect(Ctx* c);
 void SceneFree(Scene* sc);
 void FilmFree(FilmState* f);
 FilmDev FilmView(Scene* sc);

@@ -22,6 +22,10 @@
 //    {128, 256, 512, 1024}^2, spp from {4, 8, 16, 32, 64}, depth U{4..12},
 //    roulette for i > 5.  param > 0 shrinks a member for tests: at most `param`
 //    triangles, resolution / 8 (at least 16), spp <= 4.
+//  GY_SCENE_ADJOINT: the room with a seeded RGB albedo texture on every wall
+//    (param texels per side, default 64) and a central 20,480-triangle sphere
+//    of GGX conductor with a seeded roughness texture in [0.05, 0.8]; 256^2,
+//    64 spp, depth 5, roulette for i > 3.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -32,7 +36,9 @@
 namespace gy {
 
 struct Generated {
-    std::vector<float> pos, nrm, fn;
+    std::vector<float> pos, nrm, fn, uv;
+    std::vector<int32_t> texi;
+    std::vector<float> texd;
     std::vector<uint32_t> mat, mats;
     std::vector<int32_t> light, ltri;
     std::vector<float> le;
@@ -86,6 +92,16 @@ void AddTri(Generated& g, V p0, V p1, V p2, V n, uint32_t mat, int32_t light) {
     g.fn.push_back(n.z);
     g.mat.push_back(mat);
     g.light.push_back(light);
+    for (int k = 0; k < 6; ++k) g.uv.push_back(0.f);
+}
+void SetLastUV(Generated& g, float u0, float v0, float u1, float v1, float u2, float v2) {
+    float* t = &g.uv[g.uv.size() - 6];
+    t[0] = u0; t[1] = v0; t[2] = u1; t[3] = v1; t[4] = u2; t[5] = v2;
+}
+int32_t AddTexture(Generated& g, int w, int h, int c, const std::vector<float>& data) {
+    g.texi.insert(g.texi.end(), {w, h, c, (int32_t)g.texd.size()});
+    g.texd.insert(g.texd.end(), data.begin(), data.end());
+    return (int32_t)(g.texi.size() / 4 - 1);
 }
 
 // Quad a-b-c-d as triangles (a, b, c) and (a, c, d), the split the reference's
@@ -195,7 +211,7 @@ double Fbm(double x, double y, double z, uint32_t seed) {
 }
 
 void AddIcosphere(Generated& g, int levels, uint32_t seed, double cx, double cy, double cz, double radius, double amp,
-                  uint32_t mat, size_t max_tris = (size_t)-1) {
+                  uint32_t mat, size_t max_tris = (size_t)-1, bool uvs = false) {
     const double t = (1.0 + std::sqrt(5.0)) / 2.0;
     std::vector<double> vx = {-1, 1, -1, 1, 0, 0, 0, 0, t, t, -t, -t};
     std::vector<double> vy = {t, t, -t, -t, -1, 1, -1, 1, 0, 0, 0, 0};
@@ -242,11 +258,19 @@ void AddIcosphere(Generated& g, int levels, uint32_t seed, double cx, double cy,
         V n = norm(cross(sub(p1, p0), sub(p2, p0)));
         // orient outward (away from the centre)
         double ox = (p0.x + p1.x + p2.x) / 3.0 - cx, oy = (p0.y + p1.y + p2.y) / 3.0 - cy, oz = (p0.z + p1.z + p2.z) / 3.0 - cz;
+        uint32_t i0 = faces[f], i1 = faces[f + 1], i2 = faces[f + 2];
         if (n.x * ox + n.y * oy + n.z * oz < 0) {
             std::swap(p1, p2);
+            std::swap(i1, i2);
             n = norm(cross(sub(p1, p0), sub(p2, p0)));
         }
         AddTri(g, p0, p1, p2, n, mat, -1);
+        if (uvs) {
+            // spherical coordinates of the undisplaced direction
+            auto su = [&](uint32_t i) { return (float)(0.5 + std::atan2(vz[i], vx[i]) / (2.0 * 3.14159265358979)); };
+            auto sv = [&](uint32_t i) { return (float)(std::acos(std::max(-1.0, std::min(1.0, vy[i]))) / 3.14159265358979); };
+            SetLastUV(g, su(i0), sv(i0), su(i1), sv(i1), su(i2), sv(i2));
+        }
     }
 }
 
@@ -309,6 +333,54 @@ void BatchMember(Generated& g, uint32_t seed, int32_t small) {
     g.params.rr_depth = 5;
 }
 
+// The textured Cornell room of the adjoint workload: every wall Lambert with its
+// own RGB albedo texture (uv over the wall), the quad light, and a central
+// sphere (icosphere, 5 subdivisions: 20,480 triangles) of GGX conductor whose
+// roughness (alpha, given directly) comes from a one-channel texture in
+// [0.05, 0.8].  Texels seeded by `seed`; `res` texels per side.
+void AdjointScene(Generated& g, uint32_t seed, int res) {
+    Rng r{0xB5AD4ECEDA1CE2A9ull ^ ((uint64_t)seed * 0x9E3779B97F4A7C15ull)};
+    std::vector<float> tex;
+    auto rgb_texture = [&]() {
+        tex.resize((size_t)res * res * 3);
+        for (float& t : tex) t = (float)r.range(0.2, 0.9);
+        return AddTexture(g, res, res, 3, tex);
+    };
+    auto wall = [&](V a, V b, V c, V d, V n) {
+        const float white[3] = {1.f, 1.f, 1.f};
+        uint32_t m = AddMaterial(g, GY_MAT_LAMBERT, 0, white);
+        const int32_t t = rgb_texture();
+        std::memcpy(&g.mats[(size_t)m * GY_MAT_WORDS + 21], &t, 4);
+        AddTri(g, a, b, c, n, m, -1);
+        SetLastUV(g, 0, 0, 1, 0, 1, 1);
+        AddTri(g, a, c, d, n, m, -1);
+        SetLastUV(g, 0, 0, 1, 1, 0, 1);
+    };
+    wall({0, 0, 0}, {1, 0, 0}, {1, 0, 1}, {0, 0, 1}, {0, 1, 0});    // floor
+    wall({0, 1, 0}, {0, 1, 1}, {1, 1, 1}, {1, 1, 0}, {0, -1, 0});   // ceiling
+    wall({0, 0, 0}, {0, 1, 0}, {1, 1, 0}, {1, 0, 0}, {0, 0, 1});    // back
+    wall({0, 0, 0}, {0, 0, 1}, {0, 1, 1}, {0, 1, 0}, {1, 0, 0});    // left
+    wall({1, 0, 0}, {1, 1, 0}, {1, 1, 1}, {1, 0, 1}, {-1, 0, 0});   // right
+    const float black[3] = {0.f, 0.f, 0.f};
+    uint32_t mlight = AddMaterial(g, GY_MAT_LAMBERT, 0, black);
+    const float y = 0.99f, a = 0.375f, b = 0.625f;
+    const V l0{b, y, b}, l1{b, y, a}, l2{a, y, a}, l3{a, y, b}, ln{0, -1, 0};
+    int32_t first = (int32_t)g.ltri.size();
+    AddTri(g, l0, l1, l2, ln, mlight, first);
+    g.ltri.push_back((int32_t)(g.fn.size() / 3 - 1));
+    AddTri(g, l0, l2, l3, ln, mlight, first + 1);
+    g.ltri.push_back((int32_t)(g.fn.size() / 3 - 1));
+    for (int i = 0; i < 2; ++i) g.le.insert(g.le.end(), {17.f, 12.f, 4.f});
+    const float one[3] = {1.f, 1.f, 1.f}, f0[3] = {0.95f, 0.64f, 0.54f};
+    uint32_t ms = AddMaterial(g, GY_MAT_CONDUCTOR, GY_MATF_ALPHA_DIRECT | GY_MATF_SCHLICK_F0, one);
+    for (int c = 0; c < 3; ++c) SetWord(g, ms, 16 + c, f0[c]);
+    tex.resize((size_t)res * res);
+    for (float& t : tex) t = (float)r.range(0.05, 0.8);
+    const int32_t tr = AddTexture(g, res, res, 1, tex);
+    std::memcpy(&g.mats[(size_t)ms * GY_MAT_WORDS + 22], &tr, 4);
+    AddIcosphere(g, 5, seed, 0.5, 0.3, 0.5, 0.22, 0.0, ms, (size_t)-1, true);
+}
+
 }  // namespace
 }  // namespace gy
 
@@ -342,6 +414,12 @@ extern "C" int gy_generate_scene(int kind, uint32_t seed, int32_t param, gy_gene
         g->params.rr_depth = 5;
     } else if (kind == GY_SCENE_BATCH_MEMBER) {
         BatchMember(*g, seed, param);
+    } else if (kind == GY_SCENE_ADJOINT) {
+        AdjointScene(*g, seed, param > 0 ? param : 64);
+        SetCamera(*g, 256);
+        g->params.spp = 64;
+        g->params.max_depth = 5;
+        g->params.rr_depth = 3;
     } else {
         delete g;
         return Fail(GY_ERR_INVALID, "gy_generate_scene: unknown scene kind");
@@ -361,7 +439,10 @@ extern "C" int gy_generated_desc(gy_generated* gg, gy_scene_desc* d, gy_camera_d
         d->pos = g->pos.data();
         d->nrm = g->nrm.data();
         d->face_n = g->fn.data();
-        d->uv = nullptr;
+        d->uv = g->uv.data();
+        d->n_tex = (int32_t)(g->texi.size() / 4);
+        d->tex_info = g->texi.empty() ? nullptr : g->texi.data();
+        d->tex_data = g->texd.empty() ? nullptr : g->texd.data();
         d->mat = g->mat.data();
         d->light = g->light.data();
         d->n_mat = (int32_t)(g->mats.size() / GY_MAT_WORDS);

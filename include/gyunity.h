@@ -211,7 +211,7 @@ uint64_t gy_halton_enum_index(uint32_t width, uint32_t height, uint64_t i, uint3
 
 /* ---- procedural scenes (the benchmark configurations) ---------------------- */
 typedef struct gy_generated gy_generated;
-enum { GY_SCENE_CORNELL = 0, GY_SCENE_ICOSPHERE_GGX = 1, GY_SCENE_BATCH_MEMBER = 2 };
+enum { GY_SCENE_CORNELL = 0, GY_SCENE_ICOSPHERE_GGX = 1, GY_SCENE_BATCH_MEMBER = 2, GY_SCENE_ADJOINT = 3 };
 int gy_generate_scene(int kind, uint32_t seed, int32_t param, gy_generated** out);
 int gy_generated_desc(gy_generated* g, gy_scene_desc* desc, gy_camera_desc* cam, gy_render_params* params);
 int gy_generated_free(gy_generated* g);
@@ -222,6 +222,31 @@ int gy_generated_free(gy_generated* g);
  * and returns the second launch's device time.  Known-size reference for HBM
  * counter runs and the copy bandwidth the device reaches. */
 int gy_copy_probe(gy_context* ctx, int64_t bytes, double* ms);
+
+/* ---- path-replay adjoint ----------------------------------------------------
+ * No counterpart in the reference (its renderer has no derivatives).  Gradients
+ * of an image loss with respect to texture texels: albedo textures of Lambert
+ * and GGX conductor materials and GGX roughness textures.  dimage holds
+ * dl/dI for every pixel value I = colour / weight (width*height*3 floats); the
+ * gradient has one float per texture float (the layout of
+ * gy_scene_desc.tex_data).  The film on the device must hold gy_render of the
+ * same camera and sample parameters: every camera path is traced again with its
+ * own sample index and replayed (sampling pdfs, MIS weights and roulette held
+ * fixed).  gy_adjoint writes grad; gy_adjoint_device adds to it. */
+int64_t gy_scene_texture_floats(gy_scene* scene);
+/* Pixel values colour / weight of the film on the device into image_device
+ * (width*height*3 floats, device memory). */
+int gy_film_image_device(gy_scene* scene, float* image_device);
+int gy_scene_set_textures(gy_scene* scene, const float* tex_data, int64_t n);
+int gy_scene_set_textures_device(gy_scene* scene, const float* tex_data_device, int64_t n);
+int gy_adjoint(gy_scene* scene, const gy_camera_desc* cam, const gy_render_params* p, const float* dimage, float* grad);
+int gy_adjoint_device(gy_scene* scene, const gy_camera_desc* cam, const gy_render_params* p, const float* dimage_device,
+                      float* grad_device);
+/* The roughness terms of the adjoint, evaluated on the host (tests): d log f / d
+ * alpha of the isotropic GGX reflection lobe for local directions wo, wi, and
+ * d alpha / d roughness of RoughnessToAlpha (visual/Microfacet.h:49-54). */
+float gy_ggx_dlogf_dalpha(float alpha, const float* wo, const float* wi);
+float gy_dalpha_droughness(float roughness);
 
 /* ---- failure injection (host-forced, for tests) --------------------------- */
 enum { GY_FAULT_NONE = 0, GY_FAULT_ALLOC = 1, GY_FAULT_LAUNCH = 2, GY_FAULT_SCENE = 3 };
