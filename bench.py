@@ -143,6 +143,188 @@ def batch_main(args):
     print(json.dumps(out), flush=True)
 
 
+def _image(color, weight):
+    return color.astype(np.float64) / np.maximum(weight, 1e-30)[..., None]
+
+
+def adjoint_main(args):
+    """Adjoint, small: the textured Cornell room (64^2 RGB albedo per wall, GGX
+    roughness texture on a 20,480-triangle sphere), 256^2, 64 spp, L2 loss to a
+    render of the ground-truth textures.  Seconds per gradient (render +
+    path-replay adjoint) from a seeded uniform start, then path-replay against
+    central finite differences on random texels (common random numbers)."""
+    import gyunity_amd as g
+
+    ctx = g.Context(0)
+    log("device:", ctx.device_name())
+    sa, cam, prm = g.generate_scene(g.GY_SCENE_ADJOINT, 0, 64)
+    prm.mode = g.GY_RENDER_WAVEFRONT
+    gt = sa.tex_data.copy()
+    n_alb = int(sa.tex_info[5, 3])
+    rng = np.random.default_rng(1)
+    theta = np.concatenate([rng.uniform(0.2, 0.9, n_alb), rng.uniform(0.05, 0.8, gt.size - n_alb)]).astype(np.float32)
+    sc = g.Scene(ctx, sa)
+    target = _image(*sc.render(cam, prm))
+    sc.set_textures(theta)
+
+    def gradient(p):
+        img = _image(*sc.render(cam, p))
+        dI = (2.0 * (img - target_of[p.spp]) / img.size).astype(np.float32)
+        return float(((img - target_of[p.spp]) ** 2).mean()), sc.adjoint(cam, p, dI)
+
+    target_of = {prm.spp: target}
+    for i in range(args.warmup):
+        gradient(prm)
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        loss, grad = gradient(prm)
+    s_per_grad = (time.perf_counter() - t0) / args.steps
+    log("adjoint: %.4f s per gradient (%d paths rendered + replayed), loss %.6f" % (s_per_grad, cam.width * cam.height * prm.spp, loss))
+    fd_report = None
+    if args.fd_texels > 0:
+        import copy
+
+        pf = copy.copy(prm)
+        pf.spp = args.fd_spp
+        sc.set_textures(gt)
+        target_of[pf.spp] = _image(*sc.render(cam, pf))
+        sc.set_textures(theta)
+        _, gfd = gradient(pf)
+        ks = np.sort(np.random.default_rng(2).choice(gt.size, args.fd_texels, replace=False))
+        fd = np.zeros(len(ks))
+        h = args.fd_h
+        for j, k in enumerate(ks):
+            vals = []
+            for sgn in (1, -1):
+                th = theta.copy()
+                th[k] += sgn * h
+                sc.set_textures(th)
+                vals.append(float(((_image(*sc.render(cam, pf)) - target_of[pf.spp]) ** 2).mean()))
+            fd[j] = (vals[0] - vals[1]) / (2 * h)
+            log("  texel float %6d (%s): replay %+.5e  FD %+.5e" % (k, "albedo" if k < n_alb else "roughness", gfd[k], fd[j]))
+        sc.set_textures(theta)
+        gr = gfd[ks].astype(np.float64)
+
+        def rel(mask):
+            return float(np.linalg.norm(gr[mask] - fd[mask]) / max(np.linalg.norm(fd[mask]), 1e-30)) if mask.any() else None
+
+        alb = ks < n_alb
+        fd_report = {"texels": int(len(ks)), "spp": pf.spp, "h": h, "rel_l2_all": rel(np.ones_like(alb)),
+                     "rel_l2_albedo": rel(alb), "rel_l2_roughness": rel(~alb),
+                     "albedo_texels": int(alb.sum()), "roughness_texels": int((~alb).sum())}
+        log("FD check:", fd_report)
+    sc.close()
+    ctx.close()
+    out = {
+        "metric": "Mpaths/s at fixed RMSE: batched wavefront vs plain megakernel; adjoint s/grad",
+        "value": round(s_per_grad, 4),
+        "unit": "s/grad",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(s_per_grad * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded textures (ground truth seed 0, start seed 1)",
+        "config": {
+            "workload": "adjoint small: textured Cornell room + 20,480-triangle GGX sphere, %d texture floats, 256^2, %d spp, depth %d"
+                        % (gt.size, prm.spp, prm.max_depth),
+            "gradient": "render + path-replay adjoint, albedo and roughness texels",
+            "paths_per_gradient": cam.width * cam.height * prm.spp,
+            "finite_difference_check": fd_report,
+            "parallelism": "single GPU",
+        },
+    }
+    print(json.dumps(out), flush=True)
+
+
+def adjoint_scale_main(args):
+    """Adjoint at scale: the 1.31M-triangle GGX scene with 1024^2 RGB albedo and
+    roughness textures on the sphere (4.2M parameters), 32 seeded views of
+    512^2 at 32 spp, depth 6; Adam (lr 1e-2) from a seeded uniform start
+    towards renders of the ground-truth textures, through the LibTorch
+    autograd.Function (gy_autograd).  Reports loss per step and s/grad."""
+    import torch  # before the library: one HIP runtime for both
+
+    import gyunity_amd as g
+    from gyunity_amd.torch_render import render_textures
+
+    ctx = g.Context(0)
+    log("device:", ctx.device_name())
+    sa, cam0, prm = g.generate_scene(g.GY_SCENE_ICOSPHERE_TEXTURED, 0, args.tex_res)
+    prm.mode = g.GY_RENDER_WAVEFRONT
+    gt = sa.tex_data.copy()
+    n_alb = int(sa.tex_info[1, 3])
+    sc = g.Scene(ctx, sa)
+    rng = np.random.default_rng(3)
+    views = []
+    centre = np.array([0.5, 0.4, 0.5])
+    while len(views) < args.views:
+        dvec = rng.normal(size=3)
+        dvec /= np.linalg.norm(dvec)
+        pos = centre + 0.45 * dvec
+        if (pos < 0.04).any() or (pos > 0.96).any():
+            continue
+        up = (0.0, 1.0, 0.0) if abs(dvec[1]) < 0.9 else (1.0, 0.0, 0.0)
+        views.append(g.camera(tuple(pos), tuple(centre), up, 60.0, args.view_res, args.view_res))
+    tex_gt = torch.from_numpy(gt).cuda()
+    with torch.no_grad():
+        targets = [render_textures(sc, c, prm, tex_gt).clone() for c in views]
+    init = np.concatenate([rng.uniform(0.2, 0.8, n_alb), rng.uniform(0.05, 0.8, gt.size - n_alb)]).astype(np.float32)
+    tex = torch.from_numpy(init).cuda().requires_grad_(True)
+    opt = torch.optim.Adam([tex], lr=1e-2)
+    lo = torch.cat([torch.full((n_alb,), 0.01), torch.full((gt.size - n_alb,), 0.05)]).cuda()
+    hi = torch.cat([torch.full((n_alb,), 0.99), torch.full((gt.size - n_alb,), 0.8)]).cuda()
+    losses, times = [], []
+    for step in range(args.adam_steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        opt.zero_grad()
+        total = 0.0
+        for c, tgt in zip(views, targets):
+            img = render_textures(sc, c, prm, tex)
+            loss = ((img - tgt) ** 2).mean() / len(views)
+            loss.backward()
+            total += float(loss)
+        opt.step()
+        with torch.no_grad():
+            tex.copy_(torch.minimum(torch.maximum(tex, lo), hi))
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        losses.append(total)
+        err = float((tex.detach() - tex_gt).abs().mean())
+        log("adam step %3d: loss %.6e, mean |texel - ground truth| %.4f, %.2f s" % (step, total, err, times[-1]))
+    s_per_grad = float(np.mean(times[1:])) if len(times) > 1 else times[0]
+    paths = len(views) * args.view_res * args.view_res * prm.spp
+    sc.close()
+    ctx.close()
+    out = {
+        "metric": "Mpaths/s at fixed RMSE: batched wavefront vs plain megakernel; adjoint s/grad",
+        "value": round(s_per_grad, 4),
+        "unit": "s/grad",
+        "n_gpus": 1,
+        "steps": args.adam_steps,
+        "warmup": 1,
+        "ms_per_step": round(s_per_grad * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: smooth seeded ground-truth textures, seeded uniform start, seeded views",
+        "config": {
+            "workload": "adjoint at scale: 1.31M-triangle GGX icosphere, %d texture floats (%d^2 RGB albedo + roughness), "
+                        "%d views %d^2, %d spp, depth %d" % (gt.size, args.tex_res, len(views), args.view_res, prm.spp, prm.max_depth),
+            "gradient": "per step: every view rendered and replayed (gy_autograd), Adam lr 1e-2",
+            "paths_per_gradient": paths,
+            "loss_per_step": [float("%.6e" % v) for v in losses],
+            "parallelism": "single GPU",
+        },
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,8 +337,16 @@ def main():
     ap.add_argument("--single", action="store_true", help="time only --mode, not the other integrator")
     ap.add_argument("--refill", type=int, default=0, help="wavefront refill threshold (0 = library default)")
     ap.add_argument("--spp-pass", type=int, default=0, help="samples per pass (0 = library default)")
-    ap.add_argument("--workload", default="icosphere", choices=["icosphere", "batch64"],
-                    help="icosphere: the 1.31M-triangle GGX render; batch64: the 64-scene batch")
+    ap.add_argument("--workload", default="icosphere", choices=["icosphere", "batch64", "adjoint", "adjoint-scale"],
+                    help="icosphere: the 1.31M-triangle GGX render; batch64: the 64-scene batch; adjoint: "
+                         "gradient of the textured Cornell room; adjoint-scale: Adam on 4.2M texels over 32 views")
+    ap.add_argument("--fd-texels", type=int, default=0, help="adjoint: texels checked against finite differences")
+    ap.add_argument("--fd-spp", type=int, default=4096)
+    ap.add_argument("--fd-h", type=float, default=1e-3)
+    ap.add_argument("--adam-steps", type=int, default=20)
+    ap.add_argument("--views", type=int, default=32)
+    ap.add_argument("--view-res", type=int, default=512)
+    ap.add_argument("--tex-res", type=int, default=1024)
     ap.add_argument("--batch-size", type=int, default=64)
     ap.add_argument("--batch-lanes", type=int, default=0, help="batch paths per pass (0 = library default)")
     ap.add_argument("--batch-no-graph", action="store_true", help="launch the batch sequence eagerly instead of replaying its graph")
@@ -169,6 +359,12 @@ def main():
         sys.exit(2)
     if args.workload == "batch64":
         batch_main(args)
+        return
+    if args.workload == "adjoint":
+        adjoint_main(args)
+        return
+    if args.workload == "adjoint-scale":
+        adjoint_scale_main(args)
         return
 
     import gyunity_amd as g

@@ -25,6 +25,7 @@ from ._lib import (  # noqa: F401
     GY_SCENE_ICOSPHERE_GGX,
     GY_SCENE_BATCH_MEMBER,
     GY_SCENE_ADJOINT,
+    GY_SCENE_ICOSPHERE_TEXTURED,
     GY_RENDER_MEGAKERNEL,
     GY_RENDER_WAVEFRONT,
     GY_SAMPLER_SOBOL,

@@ -26,6 +26,7 @@ GY_RENDER_WAVEFRONT = 1
 GY_SAMPLER_SOBOL = 0
 GY_SCENE_BATCH_MEMBER = 2
 GY_SCENE_ADJOINT = 3
+GY_SCENE_ICOSPHERE_TEXTURED = 4
 GY_SAMPLER_HALTON = 1
 FAULT_NONE, FAULT_ALLOC, FAULT_LAUNCH, FAULT_SCENE = 0, 1, 2, 3
 

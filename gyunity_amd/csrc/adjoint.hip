@@ -212,9 +212,11 @@ __device__ void replay_path(const SceneDev& S, const PathParams& P, int px, int 
 }
 
 // One thread per camera sample of the render on the film (same mapping as the
-// megakernel): trace the path for its radiance, form its adjoint radiance from
-// dl/dI over its box-filter footprint, replay it.
-template <int STACK, int BLOCK>
+// megakernel): the path's radiance, its adjoint radiance from dl/dI over its
+// box-filter footprint, then the replay.  STORED: the render left every
+// sample's radiance in its pass slot (one pass covered all samples), so the
+// first trace is skipped.
+template <int STACK, int BLOCK, bool STORED>
 __global__ void __launch_bounds__(BLOCK) k_pt_adjoint(SceneDev S, PathParams P, FilmDev F, const float* __restrict__ dI,
                                                       Grad G, int s0, int n_s) {
     __shared__ uint32_t stack[STACK * BLOCK];
@@ -222,9 +224,22 @@ __global__ void __launch_bounds__(BLOCK) k_pt_adjoint(SceneDev S, PathParams P, 
     int s_local, px, py;
     pass_pixel(F, t, &s_local, &px, &py);
     if (!(s_local < n_s && px < F.width && py < F.height)) return;
-    PathCounters pc = {};
     float fx, fy;
-    const V3 Lt = trace_path<STACK, false>(S, P, px, py, s0 + s_local, stack + threadIdx.x, BLOCK, &fx, &fy, nullptr, pc);
+    V3 Lt;
+    if (STORED) {
+        Seq q;
+        q.index = enum_index(P.en, (uint32_t)(s0 + s_local), (uint32_t)px, (uint32_t)py);
+        q.cursor = 0;
+        const float u = P.en.sx * seq_next(P.smp, q) - (float)px;
+        const float v = P.en.sy * seq_next(P.smp, q) - (float)py;
+        fx = px + u;
+        fy = py + v;
+        const float4 e = F.pass[((int64_t)(s0 + s_local) * F.height + py) * F.width + px];
+        Lt = v3(e.x, e.y, e.z);
+    } else {
+        PathCounters pc = {};
+        Lt = trace_path<STACK, false>(S, P, px, py, s0 + s_local, stack + threadIdx.x, BLOCK, &fx, &fy, nullptr, pc);
+    }
     int x0, x1, y0, y1;
     film_footprint(fx, fy, F.width, F.height, &x0, &x1, &y0, &y1);
     V3 dL = v3s(0.f);
@@ -276,6 +291,9 @@ static int AdjointDevice(Scene* sc, const gy_camera_desc* cam, const gy_render_p
     if (rc != GY_OK) return rc;
     const FilmDev F = FilmView(sc);
     const int n_pass = std::max(1, std::min(p->spp, SamplesPerPass(*cam, *p)));
+    // the render's pass buffer still holds every sample's radiance when one
+    // pass covered all samples
+    const bool stored = sc->film_single_pass && sc->film.pass_slots >= (int64_t)F.width * F.height * p->spp;
     constexpr int kBlock = 128;
     const int tilesX = (F.width + 7) >> 3, tilesY = (F.height + 7) >> 3;
     TimerBegin(c, kKernOther);
@@ -283,10 +301,14 @@ static int AdjointDevice(Scene* sc, const gy_camera_desc* cam, const gy_render_p
         const int n_s = std::min(n_pass, p->spp - s0);
         const int64_t threads = (int64_t)tilesX * tilesY * 64 * n_s;
         const unsigned blocks = (unsigned)((threads + kBlock - 1) / kBlock);
-        if (sc->stack <= 32)
-            hipLaunchKernelGGL((k_pt_adjoint<32, kBlock>), dim3(blocks), dim3(kBlock), 0, c->stream, sc->dev, P, F, dI, Grad{grad}, s0, n_s);
-        else
-            hipLaunchKernelGGL((k_pt_adjoint<64, kBlock>), dim3(blocks), dim3(kBlock), 0, c->stream, sc->dev, P, F, dI, Grad{grad}, s0, n_s);
+        const Grad G{grad};
+        if (sc->stack <= 32) {
+            if (stored) hipLaunchKernelGGL((k_pt_adjoint<32, kBlock, true>), dim3(blocks), dim3(kBlock), 0, c->stream, sc->dev, P, F, dI, G, s0, n_s);
+            else hipLaunchKernelGGL((k_pt_adjoint<32, kBlock, false>), dim3(blocks), dim3(kBlock), 0, c->stream, sc->dev, P, F, dI, G, s0, n_s);
+        } else {
+            if (stored) hipLaunchKernelGGL((k_pt_adjoint<64, kBlock, true>), dim3(blocks), dim3(kBlock), 0, c->stream, sc->dev, P, F, dI, G, s0, n_s);
+            else hipLaunchKernelGGL((k_pt_adjoint<64, kBlock, false>), dim3(blocks), dim3(kBlock), 0, c->stream, sc->dev, P, F, dI, G, s0, n_s);
+        }
         rc = CheckLaunch(c, "k_pt_adjoint");
     }
     TimerEnd(c);

@@ -985,6 +985,7 @@ extern "C" int gy_render(gy_scene* s, const gy_camera_desc* cam, const gy_render
     if (rc == GY_OK) {
         // what the film on the device holds (the adjoint replays exactly this render)
         sc->film_valid = true;
+        sc->film_single_pass = n_pass >= p->spp;
         sc->film_cam = *cam;
         sc->film_params = *p;
     } else {

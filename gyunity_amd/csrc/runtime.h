@@ -97,6 +97,7 @@ struct Scene {
     WavefrontState wf;
     int64_t tex_floats = 0;     // floats of texture data on the device
     bool film_valid = false;    // the film holds the last gy_render of film_cam / film_params
+    bool film_single_pass = false;  // ... rendered in one pass: the pass buffer holds every sample
     gy_camera_desc film_cam = {};
     gy_render_params film_params = {};
 };

@@ -26,6 +26,9 @@
 //    (param texels per side, default 64) and a central 20,480-triangle sphere
 //    of GGX conductor with a seeded roughness texture in [0.05, 0.8]; 256^2,
 //    64 spp, depth 5, roulette for i > 3.
+//  GY_SCENE_ICOSPHERE_TEXTURED: the GGX icosphere scene (8 subdivisions) with
+//    an RGB albedo and a roughness texture on the sphere (param texels per
+//    side, default 1024: 4.2M texture floats); 512^2, 32 spp, depth 6.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -381,6 +384,33 @@ void AdjointScene(Generated& g, uint32_t seed, int res) {
     AddIcosphere(g, 5, seed, 0.5, 0.3, 0.5, 0.22, 0.0, ms, (size_t)-1, true);
 }
 
+// The 1.31M-triangle scene with textures on the icosphere (uv from spherical
+// coordinates): RGB albedo and GGX roughness (alpha given directly), `res`
+// texels per side, smooth seeded patterns: albedo 0.5 + 0.35 sin(.) sin(.),
+// roughness in [0.05, 0.8].
+void TexturedIcosphere(Generated& g, uint32_t seed, int res) {
+    Rng r{0x3C6EF372FE94F82Bull ^ ((uint64_t)seed * 0x9E3779B97F4A7C15ull)};
+    AddRoomAndLight(g, false);
+    double fq[6];
+    for (double& f : fq) f = r.range(2.0, 9.0);
+    std::vector<float> alb((size_t)res * res * 3), rough((size_t)res * res);
+    for (int y = 0; y < res; ++y)
+        for (int x = 0; x < res; ++x) {
+            const double u = (x + 0.5) / res, v = (y + 0.5) / res;
+            for (int c = 0; c < 3; ++c)
+                alb[((size_t)y * res + x) * 3 + c] =
+                    (float)(0.5 + 0.35 * std::sin(2 * 3.14159265358979 * fq[c] * u + c) * std::sin(2 * 3.14159265358979 * fq[3] * v));
+            rough[(size_t)y * res + x] = (float)(0.425 + 0.375 * std::sin(2 * 3.14159265358979 * (fq[4] * u + fq[5] * v)));
+        }
+    const int32_t ta = AddTexture(g, res, res, 3, alb), tr = AddTexture(g, res, res, 1, rough);
+    const float one[3] = {1.f, 1.f, 1.f}, f0[3] = {0.95f, 0.64f, 0.54f};
+    uint32_t m = AddMaterial(g, GY_MAT_CONDUCTOR, GY_MATF_ALPHA_DIRECT | GY_MATF_SCHLICK_F0, one);
+    for (int c = 0; c < 3; ++c) SetWord(g, m, 16 + c, f0[c]);
+    std::memcpy(&g.mats[(size_t)m * GY_MAT_WORDS + 21], &ta, 4);
+    std::memcpy(&g.mats[(size_t)m * GY_MAT_WORDS + 22], &tr, 4);
+    AddIcosphere(g, 8, seed, 0.5, 0.4, 0.5, 0.3, 0.05, m, (size_t)-1, true);
+}
+
 }  // namespace
 }  // namespace gy
 
@@ -414,6 +444,12 @@ extern "C" int gy_generate_scene(int kind, uint32_t seed, int32_t param, gy_gene
         g->params.rr_depth = 5;
     } else if (kind == GY_SCENE_BATCH_MEMBER) {
         BatchMember(*g, seed, param);
+    } else if (kind == GY_SCENE_ICOSPHERE_TEXTURED) {
+        TexturedIcosphere(*g, seed, param > 0 ? param : 1024);
+        SetCamera(*g, 512);
+        g->params.spp = 32;
+        g->params.max_depth = 6;
+        g->params.rr_depth = 5;
     } else if (kind == GY_SCENE_ADJOINT) {
         AdjointScene(*g, seed, param > 0 ? param : 64);
         SetCamera(*g, 256);

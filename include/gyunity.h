@@ -211,7 +211,8 @@ uint64_t gy_halton_enum_index(uint32_t width, uint32_t height, uint64_t i, uint3
 
 /* ---- procedural scenes (the benchmark configurations) ---------------------- */
 typedef struct gy_generated gy_generated;
-enum { GY_SCENE_CORNELL = 0, GY_SCENE_ICOSPHERE_GGX = 1, GY_SCENE_BATCH_MEMBER = 2, GY_SCENE_ADJOINT = 3 };
+enum { GY_SCENE_CORNELL = 0, GY_SCENE_ICOSPHERE_GGX = 1, GY_SCENE_BATCH_MEMBER = 2, GY_SCENE_ADJOINT = 3,
+       GY_SCENE_ICOSPHERE_TEXTURED = 4 };
 int gy_generate_scene(int kind, uint32_t seed, int32_t param, gy_generated** out);
 int gy_generated_desc(gy_generated* g, gy_scene_desc* desc, gy_camera_desc* cam, gy_render_params* params);
 int gy_generated_free(gy_generated* g);

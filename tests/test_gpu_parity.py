@@ -492,6 +492,15 @@ def test_adjoint_matches_finite_differences(gpu_ctx):
         dI = (2.0 * (img - target) / img.size).astype(np.float32)
         grad = sc.adjoint(cam, prm, dI)
         assert np.isfinite(grad).all()
+        # a render in several passes leaves only the last pass's radiance in the
+        # pass buffer: the adjoint then traces every path once more itself
+        import copy
+
+        pm = copy.copy(prm)
+        pm.samples_per_pass = prm.spp // 4
+        sc.render(cam, pm)
+        grad_traced = sc.adjoint(cam, pm, dI)
+        assert np.abs(grad_traced - grad).max() <= 1e-4 * np.abs(grad).max()
         checks = []
         order_a = np.argsort(-np.abs(grad[:n_alb]))[:12]
         order_r = n_alb + np.argsort(-np.abs(grad[n_alb:]))[:6]
