@@ -100,6 +100,21 @@ def batch_main(args):
         log("batch step %d: %.1f ms, %.1f Mrays" % (i, st.total_ms, st.rays / 1e6))
     ms_batch = (time.perf_counter() - t0) * 1e3 / args.steps
     per_scene = {}
+    # per-member launch choice by the autotuner (pilot renders of 4 spp)
+    tuned = []
+    t0 = time.perf_counter()
+    for sc, (_, cam, prm) in zip(scenes, members):
+        tuned.append(sc.autotune(cam, prm, 4)[0])
+    log("autotuned %d members in %.1f s: %d megakernel, %d wavefront" % (
+        len(tuned), time.perf_counter() - t0, sum(t.mode == g.GY_RENDER_MEGAKERNEL for t in tuned),
+        sum(t.mode == g.GY_RENDER_WAVEFRONT for t in tuned)))
+    for sc, (_, cam, _), prm in zip(scenes, members, tuned):
+        sc.render(cam, prm, download=False)
+    t0 = time.perf_counter()
+    for sc, (_, cam, _), prm in zip(scenes, members, tuned):
+        sc.render(cam, prm, download=False)
+    per_scene["autotuned"] = (time.perf_counter() - t0) * 1e3
+    log("one call per member, autotuned launch: %.1f ms" % per_scene["autotuned"])
     for mode, name in ((g.GY_RENDER_WAVEFRONT, "wavefront"), (g.GY_RENDER_MEGAKERNEL, "megakernel")):
         for sc, (_, cam, prm) in zip(scenes, members):  # warm-up: allocations, first launches
             prm.mode = mode
@@ -136,6 +151,7 @@ def batch_main(args):
             "mrays_per_s": round(rays / (ms_batch / 1e3) / 1e6, 2),
             "per_member_wavefront_mpaths_per_s": round(paths / (per_scene["wavefront"] / 1e3) / 1e6, 3),
             "per_member_megakernel_mpaths_per_s": round(paths / (per_scene["megakernel"] / 1e3) / 1e6, 3),
+            "per_member_autotuned_mpaths_per_s": round(paths / (per_scene["autotuned"] / 1e3) / 1e6, 3),
             "same_estimator": "each member's batch film equals its own render bit for bit (tests/test_gpu_parity.py)",
             "parallelism": "single GPU",
         },
@@ -337,6 +353,7 @@ def main():
     ap.add_argument("--single", action="store_true", help="time only --mode, not the other integrator")
     ap.add_argument("--refill", type=int, default=0, help="wavefront refill threshold (0 = library default)")
     ap.add_argument("--spp-pass", type=int, default=0, help="samples per pass (0 = library default)")
+    ap.add_argument("--leaf", type=int, default=0, help="BVH leaf size (0 = library default, 4)")
     ap.add_argument("--workload", default="icosphere", choices=["icosphere", "batch64", "adjoint", "adjoint-scale"],
                     help="icosphere: the 1.31M-triangle GGX render; batch64: the 64-scene batch; adjoint: "
                          "gradient of the textured Cornell room; adjoint-scale: Adam on 4.2M texels over 32 views")
@@ -377,7 +394,7 @@ def main():
     ctx = g.Context(0)
     log("device:", ctx.device_name())
     t0 = time.perf_counter()
-    sc = g.Scene(ctx, sa)
+    sc = g.Scene(ctx, sa, max_leaf_size=args.leaf)
     sst = sc.stats()
     log("scene: %d triangles, BVH %d nodes depth %d, build %.1f ms (scene create %.1f ms)"
         % (sa.n_tri, sst.n_nodes, sst.max_depth, sst.build_ms, (time.perf_counter() - t0) * 1e3))

@@ -154,6 +154,8 @@ def lib():
         "gy_render": (C.c_int, [vp, P(CameraDesc), P(RenderParams), P(f32), P(f32)]),
         "gy_film_download": (C.c_int, [vp, P(f32), P(f32)]),
         "gy_scene_texture_floats": (i64, [vp]),
+        "gy_scene_context": (C.c_int, [vp, P(vp)]),
+        "gy_autotune": (C.c_int, [vp, P(CameraDesc), P(RenderParams), i32, P(RenderParams), P(C.c_double)]),
         "gy_ggx_dlogf_dalpha": (f32, [f32, P(f32), P(f32)]),
         "gy_dalpha_droughness": (f32, [f32]),
         "gy_scene_set_textures": (C.c_int, [vp, P(f32), i64]),
@@ -458,6 +460,13 @@ class Scene:
             return color, weight
         check(lib().gy_render(self.h, C.byref(cam), C.byref(params), C.POINTER(C.c_float)(), C.POINTER(C.c_float)()))
         return None
+
+    def autotune(self, cam, params, pilot_spp=0):
+        """Fastest launch choice for this view (gy_autotune): (params, ms per Mpath)."""
+        best = RenderParams()
+        ms = C.c_double()
+        check(lib().gy_autotune(self.h, C.byref(cam), C.byref(params), pilot_spp, C.byref(best), C.byref(ms)))
+        return best, ms.value
 
     def texture_floats(self):
         return lib().gy_scene_texture_floats(self.h)

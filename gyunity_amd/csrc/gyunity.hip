@@ -818,6 +818,12 @@ extern "C" int gy_scene_destroy(gy_scene* s) {
     return GY_OK;
 }
 
+extern "C" int gy_scene_context(gy_scene* s, gy_context** ctx) {
+    if (!s || !ctx) return Fail(GY_ERR_INVALID, "gy_scene_context: null argument");
+    *ctx = reinterpret_cast<gy_context*>(reinterpret_cast<Scene*>(s)->ctx);
+    return GY_OK;
+}
+
 extern "C" int gy_scene_stats_get(gy_scene* s, gy_scene_stats* out) {
     if (!s || !out) return Fail(GY_ERR_INVALID, "gy_scene_stats_get: null argument");
     *out = reinterpret_cast<Scene*>(s)->stats;

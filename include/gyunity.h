@@ -144,6 +144,8 @@ int gy_device_name(gy_context* ctx, char* buf, int len);
 int gy_scene_create(gy_context* ctx, const gy_scene_desc* desc, const gy_bvh_params* bvh, gy_scene** out);
 int gy_scene_destroy(gy_scene* scene);
 int gy_scene_stats_get(gy_scene* scene, gy_scene_stats* out);
+/* The context a scene was created on. */
+int gy_scene_context(gy_scene* scene, gy_context** ctx);
 /* Copy the device BVH back: nodes (n_nodes*16 words) and leaf triangle order. */
 int gy_scene_bvh_download(gy_scene* scene, uint32_t* nodes, int64_t max_nodes, int32_t* tri_order, int64_t max_tris);
 
@@ -175,6 +177,15 @@ int gy_li_samples(gy_scene* scene, const gy_camera_desc* cam, const gy_render_pa
 /* Algorithmic HBM traffic of a render: node/triangle visits of every traced ray
  * and shading points, counted by an instrumented copy of the kernels. */
 int gy_count_traffic(gy_scene* scene, const gy_camera_desc* cam, const gy_render_params* p, gy_render_stats* out);
+
+/* ---- launch autotuner --------------------------------------------------------
+ * No counterpart in the reference (TiledIntegrator fixes 32x32 tiles,
+ * integrator/TiledIntegrator.cpp:13).  Times pilot renders (pilot_spp samples,
+ * 0 = 4) of this scene and view over the integrator form, the megakernel
+ * workgroup size and the wavefront refill threshold; writes base with the
+ * fastest choice into best.  The image does not depend on the choice. */
+int gy_autotune(gy_scene* scene, const gy_camera_desc* cam, const gy_render_params* base, int32_t pilot_spp,
+                gy_render_params* best, double* best_ms_per_mpath);
 
 /* ---- batched rendering ------------------------------------------------------
  * No counterpart in the reference (TiledIntegrator::Render draws one image per

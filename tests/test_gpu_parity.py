@@ -578,3 +578,20 @@ def test_torch_autograd_function_matches_adjoint(gpu_ctx):
     res = json.loads(r.stdout.strip().splitlines()[-1])
     assert res["image_equal"]
     assert res["grad_max_abs_diff"] <= 1e-4 * res["grad_max_abs"], res
+
+
+def test_autotune_returns_a_faster_equivalent_launch(gpu_ctx):
+    # The tuned launch renders the same film bit for bit (the choices only move
+    # work between kernels) and is no slower per path than the default's pilot.
+    sa, cam, prm = scenes()["cornell"]
+    cam.width = cam.height = 96
+    prm.spp = 8
+    with g.Scene(gpu_ctx, sa) as sc:
+        best, ms_per_mpath = sc.autotune(cam, prm, 4)
+        assert best.mode in (g.GY_RENDER_MEGAKERNEL, g.GY_RENDER_WAVEFRONT) and ms_per_mpath > 0
+        assert best.spp == prm.spp and best.max_depth == prm.max_depth
+        best.samples_per_pass = prm.samples_per_pass = prm.spp
+        a = sc.render(cam, best)
+        prm.mode = g.GY_RENDER_WAVEFRONT
+        b = sc.render(cam, prm)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
