@@ -5,8 +5,8 @@
 //
 // Choices: integrator form (megakernel / wavefront); megakernel workgroup size
 // (64, 128, 256 lanes; its LDS traversal stack follows the BVH depth); the
-// wavefront's refill threshold for its persistent trace waves (1, 8, 32 busy
-// lanes).  No counterpart in the reference, whose TiledIntegrator fixes 32x32
+// wavefront's refill threshold for its persistent trace waves (automatic: 1
+// for closest-hit rays, 16 for shadow and MIS rays; or 1, or 16 for all).  No counterpart in the reference, whose TiledIntegrator fixes 32x32
 // tiles (integrator/TiledIntegrator.cpp:13).
 #include <algorithm>
 #include <chrono>
@@ -31,7 +31,7 @@ extern "C" int gy_autotune(gy_scene* scene, const gy_camera_desc* cam, const gy_
         int mode, block, refill;
     };
     const Cand cands[] = {{GY_RENDER_MEGAKERNEL, 64, 0}, {GY_RENDER_MEGAKERNEL, 128, 0}, {GY_RENDER_MEGAKERNEL, 256, 0},
-                          {GY_RENDER_WAVEFRONT, 0, 1},   {GY_RENDER_WAVEFRONT, 0, 8},   {GY_RENDER_WAVEFRONT, 0, 32}};
+                          {GY_RENDER_WAVEFRONT, 0, 0},   {GY_RENDER_WAVEFRONT, 0, 1},   {GY_RENDER_WAVEFRONT, 0, 16}};
     double best_t = 1e300;
     const double mpaths = (double)cam->width * cam->height * pilot.spp / 1e6;
     for (const Cand& k : cands) {

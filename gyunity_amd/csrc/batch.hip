@@ -88,7 +88,7 @@ extern "C" int gy_batch_create(gy_context* ctx, const gy_batch_params* p, gy_bat
     h->b.ctx = reinterpret_cast<Ctx*>(ctx);
     if (p) {
         h->lanes_req = p->lanes_per_pass;
-        h->b.refill = p->refill_threshold > 0 ? p->refill_threshold : 1;
+        h->b.refill = p->refill_threshold > 0 ? p->refill_threshold : 0;  // 0: per-kind automatic
         h->b.use_graph = p->use_graph != 0;
     }
     *out = reinterpret_cast<gy_batch*>(h);

@@ -67,14 +67,14 @@ GY_D float dlog(float x) { return (float)log((double)x); }
 
 // ------------------------------------------------------------------ Sobol
 // sobol::sample (sampler/Sobol.h:46-61) + RandomStateSequence (sampler/Sampler.h:53-64)
-GY_D float sobol_sample(const uint32_t* __restrict__ M, uint64_t index, uint32_t dim) {
-    const uint32_t* col = M + (size_t)dim * 52;
-    uint32_t r = 0;
-    while (index) {
-        const int c = __builtin_ctzll(index);
-        r ^= col[c];
-        index &= index - 1;
-    }
+// T: byte tables, 7 x 256 words per dimension, entry [k][v] = XOR of the
+// matrix columns 8k + j for the set bits j of v.  The XOR over the set bits of
+// the index splits into one entry per index byte (indices stay below 2^52).
+GY_D float sobol_sample(const uint32_t* __restrict__ T, uint64_t index, uint32_t dim) {
+    const uint32_t* t = T + (size_t)dim * (7 * 256);
+    const uint32_t lo = (uint32_t)index, hi = (uint32_t)(index >> 32);
+    uint32_t r = (t[lo & 255u] ^ t[256 + ((lo >> 8) & 255u)]) ^ (t[512 + ((lo >> 16) & 255u)] ^ t[768 + (lo >> 24)]);
+    if (hi != 0u) r ^= (t[1024 + (hi & 255u)] ^ t[1280 + ((hi >> 8) & 255u)]) ^ t[1536 + ((hi >> 16) & 255u)];
     return (float)r * 2.3283064365386963e-10f;
 }
 // HaltonSampler::sample with Faure permutations (sampler/Halton.h:648-910,
@@ -82,7 +82,7 @@ GY_D float sobol_sample(const uint32_t* __restrict__ M, uint64_t index, uint32_t
 // other bases sum digit-table entries over groups of k digits (host tables,
 // halton.cpp) and scale the integer by float((1 - 2^-23) / tsize^groups).
 struct SampleSrc {
-    const uint32_t* __restrict__ sobol;  // Sobol matrices, 52 columns per dimension
+    const uint32_t* __restrict__ sobol;  // Sobol byte tables, 7 x 256 words per dimension
     const uint16_t* __restrict__ htab;   // Halton digit tables
     const uint4* __restrict__ hdim;      // per dimension: base, table size, groups, table offset
     const float* __restrict__ hscale;    // per dimension: scale
@@ -268,19 +268,27 @@ GY_D bool ref_box(V3 mn, V3 mx, V3 o, V3 inv, float rtmin, float rtmax) {
 
 // One candidate triangle (p0, p1, p2) as the reference tests it: leaf bounds
 // gate, then Triangle::Intersect with e1 = p1 - p0, e2 = p2 - p0
-// (shape/Triangle.h:107-111).
-GY_D bool ref_triangle(V3 p0, V3 p1, V3 p2, V3 o, V3 d, V3 inv, float tmin, float tmax, float* t, float* b1, float* b2) {
+// (shape/Triangle.h:107-111).  The bounds gate takes the ray's own tMax
+// (box_tmax) and the triangle test the current closest distance (tmax).  In
+// the reference both take the shrinking tMax, which makes the winner of an
+// exact-t tie depend on visit order (a flat leaf box entered exactly at the
+// hit fails `tmin < tMax` once tMax has shrunk to that t).  With the gate at
+// the ray's own tMax the closest hit is the minimum of (t, -primId) over every
+// triangle that passes both tests at the ray's own tMax, whatever order the
+// traversal visits them in; for any-hit queries the two bounds are the same.
+GY_D bool ref_triangle(V3 p0, V3 p1, V3 p2, V3 o, V3 d, V3 inv, float tmin, float box_tmax, float tmax, float* t,
+                       float* b1, float* b2) {
     const V3 mn = v3(rmin(rmin(p0.x, p1.x), p2.x), rmin(rmin(p0.y, p1.y), p2.y), rmin(rmin(p0.z, p1.z), p2.z));
     const V3 mx = v3(rmax(rmax(p0.x, p1.x), p2.x), rmax(rmax(p0.y, p1.y), p2.y), rmax(rmax(p0.z, p1.z), p2.z));
-    if (!ref_box(mn, mx, o, inv, tmin, tmax)) return false;
+    if (!ref_box(mn, mx, o, inv, tmin, box_tmax)) return false;
     return tri_test(p0, p1 - p0, p2 - p0, o, d, tmin, tmax, t, b1, b2);
 }
 
 // Closest-hit / any-hit BVH traversal with a per-thread stack column in LDS
 // (stack[i * stride]).  Node boxes are padded and tested conservatively, so
 // every triangle the reference's traversal would test is tested here; each is
-// gated as the reference gates it (ref_triangle).  Closest hit: minimum t, ties
-// to the larger primitive id.  The stack never exceeds the BVH depth, which
+// gated by ref_triangle.  Closest hit: minimum t, ties to the larger primitive
+// id, independent of visit order.  The stack never exceeds the BVH depth, which
 // the host checks against STACK before launch.
 template <bool ANY, int STACK>
 GY_D bool traverse(const SceneDev& S, V3 o, V3 d, float tmin, float tmax, Hit* hit, uint32_t* stack, int stride,
@@ -321,7 +329,7 @@ GY_D bool traverse(const SceneDev& S, V3 o, V3 d, float tmin, float tmax, Hit* h
                 const float4 a = tp[0], pb = tp[1], pc = tp[2];
                 if (n_tri) ++*n_tri;
                 float th, u, v;
-                if (ref_triangle(ld3(a), ld3(pb), ld3(pc), o, d, inv, tmin, best, &th, &u, &v)) {
+                if (ref_triangle(ld3(a), ld3(pb), ld3(pc), o, d, inv, tmin, tmax, best, &th, &u, &v)) {
                     const int32_t prim = __float_as_int(a.w);
                     if (ANY) {
                         hit->prim = prim;
@@ -410,7 +418,7 @@ GY_D bool traverse_short(const SceneDev& Sc, V3 o, V3 d, float tmin, float tmax,
                 const float4* tp = Sc.tri + (size_t)(ref + k) * 3;
                 const float4 a = tp[0], pb = tp[1], pc = tp[2];
                 float th, u, v;
-                if (ref_triangle(ld3(a), ld3(pb), ld3(pc), o, d, inv, tmin, best, &th, &u, &v)) {
+                if (ref_triangle(ld3(a), ld3(pb), ld3(pc), o, d, inv, tmin, tmax, best, &th, &u, &v)) {
                     const int32_t prim = __float_as_int(a.w);
                     if (ANY) {
                         hit->prim = prim;
@@ -477,20 +485,23 @@ GY_D bool traverse_short(const SceneDev& Sc, V3 o, V3 d, float tmin, float tmax,
 // ray finishes.  Same node tests, triangle gating and tie rule as traverse().
 struct Trav {
     V3 o, d, inv;
-    float tmin, best, bb1, bb2;
+    float tmin, tmax0, best, bb1, bb2;  // tmax0: the ray's own tMax
     int32_t best_prim;
     uint32_t node, head, nlds, nspill;
+    uint32_t leaf;  // trav_run: postponed leaf entry, or kTravNone
 };
 GY_D void trav_init(Trav& t, V3 o, V3 d, float tmin, float tmax) {
     t.o = o;
     t.d = d;
     t.inv = v3(1.f / d.x, 1.f / d.y, 1.f / d.z);
     t.tmin = tmin;
+    t.tmax0 = tmax;
     t.best = tmax;
     t.bb1 = t.bb2 = 0.f;
     t.best_prim = -1;
     t.node = 0;
     t.head = t.nlds = t.nspill = 0;
+    t.leaf = 0xffffffffu;
 }
 // Returns 1 when the traversal finished (t.best_prim >= 0 on a hit), 2 when an
 // any-hit query found a hit, 0 to continue.
@@ -523,7 +534,7 @@ GY_D int trav_step(const SceneDev& Sc, Trav& t, uint32_t* lds, int stride, uint3
             const float4* tp = Sc.tri + (size_t)(ref + k) * 3;
             const float4 a = tp[0], pb = tp[1], pc = tp[2];
             float th, u, v;
-            if (ref_triangle(ld3(a), ld3(pb), ld3(pc), t.o, t.d, t.inv, t.tmin, t.best, &th, &u, &v)) {
+            if (ref_triangle(ld3(a), ld3(pb), ld3(pc), t.o, t.d, t.inv, t.tmin, t.tmax0, t.best, &th, &u, &v)) {
                 const int32_t prim = __float_as_int(a.w);
                 if (ANY) {
                     t.best_prim = prim;
@@ -575,6 +586,120 @@ GY_D int trav_step(const SceneDev& Sc, Trav& t, uint32_t* lds, int stride, uint3
         return 0;
     }
     return 1;
+}
+
+// Traversal entries of trav_run: an interior node index (bit 31 clear), a
+// leaf (bit 31 set, triangle count 1..15 in bits 27-30, first triangle slot in
+// bits 0-26), or kTravNone.  Scenes stay below kTravMaxTriangles so that no
+// leaf entry equals kTravNone.
+constexpr uint32_t kTravNone = 0xffffffffu;
+constexpr uint32_t kTravLeaf = 0x80000000u;
+constexpr int64_t kTravMaxTriangles = (1ll << 27) - 16;
+GY_D bool trav_inner(uint32_t e) { return (e & kTravLeaf) == 0u; }
+GY_D bool trav_is_leaf(uint32_t e) { return e != kTravNone && (e & kTravLeaf) != 0u; }
+
+template <int S>
+GY_D void trav_push(Trav& t, uint32_t e, uint32_t* lds, int stride, uint32_t* __restrict__ spill, int64_t spill_stride) {
+    if (t.nlds == (uint32_t)S) {
+        spill[(int64_t)t.nspill * spill_stride] = lds[t.head * stride];
+        ++t.nspill;
+        t.head = t.head + 1 == (uint32_t)S ? 0 : t.head + 1;
+        --t.nlds;
+    }
+    uint32_t pos = t.head + t.nlds;
+    if (pos >= (uint32_t)S) pos -= S;
+    lds[pos * stride] = e;
+    ++t.nlds;
+}
+template <int S>
+GY_D uint32_t trav_pop(Trav& t, uint32_t* lds, int stride, uint32_t* __restrict__ spill, int64_t spill_stride) {
+    if (t.nlds > 0) {
+        --t.nlds;
+        uint32_t pos = t.head + t.nlds;
+        if (pos >= (uint32_t)S) pos -= S;
+        return lds[pos * stride];
+    }
+    if (t.nspill > 0) {
+        --t.nspill;
+        return spill[(int64_t)t.nspill * spill_stride];
+    }
+    return kTravNone;
+}
+
+// Traversal in two phases per round (Aila and Laine, "Understanding the
+// efficiency of ray traversal on GPUs", 2009: while-while with postponed
+// leaves).  Interior nodes are visited until every lane of the wave still
+// traversing holds one leaf (a lane's first leaf is postponed and it keeps
+// walking); then the lanes test their postponed leaves together, and a second
+// leaf met on the way.  The node tests, culling and triangle gate are
+// trav_step's (a leaf child is kept when its padded box is entered, an interior
+// child when its entry distance is also <= best), and triangles are still
+// tested in the order their leaves are reached.  Returns 1 when the ray is
+// finished (t.best_prim >= 0 on a hit), 2 when an any-hit query found a hit,
+// and 0 when fewer than `refill` lanes remain, so that the caller can refill.
+template <bool ANY, int S>
+GY_D int trav_run(const SceneDev& Sc, Trav& t, uint32_t* lds, int stride, uint32_t* __restrict__ spill, int64_t spill_stride,
+                  int refill) {
+    while (true) {
+        while (trav_inner(t.node)) {
+            const uint4* np = Sc.nodes + (size_t)t.node * 4;
+            const uint4 q0 = np[0], q1 = np[1], q2 = np[2], q3 = np[3];
+            const float b0[6] = {__uint_as_float(q0.x), __uint_as_float(q0.y), __uint_as_float(q0.z),
+                                 __uint_as_float(q0.w), __uint_as_float(q1.x), __uint_as_float(q1.y)};
+            const float b1[6] = {__uint_as_float(q1.z), __uint_as_float(q1.w), __uint_as_float(q2.x),
+                                 __uint_as_float(q2.y), __uint_as_float(q2.z), __uint_as_float(q2.w)};
+            const float t0 = (q3.x != 0xffffffffu) ? box_enter(b0, t.o, t.inv, t.tmin, t.best) : kInf;
+            const float t1 = (q3.y != 0xffffffffu) ? box_enter(b1, t.o, t.inv, t.tmin, t.best) : kInf;
+            const bool keep0 = t0 != kInf && (q3.z != 0u || t0 <= t.best);
+            const bool keep1 = t1 != kInf && (q3.w != 0u || t1 <= t.best);
+            const uint32_t e0 = q3.z != 0u ? (kTravLeaf | (q3.z << 27) | q3.x) : q3.x;
+            const uint32_t e1 = q3.w != 0u ? (kTravLeaf | (q3.w << 27) | q3.y) : q3.y;
+            if (keep0 && keep1) {
+                const bool swap = t1 < t0;
+                trav_push<S>(t, swap ? e0 : e1, lds, stride, spill, spill_stride);
+                t.node = swap ? e1 : e0;
+            } else if (keep0) {
+                t.node = e0;
+            } else if (keep1) {
+                t.node = e1;
+            } else {
+                t.node = trav_pop<S>(t, lds, stride, spill, spill_stride);
+            }
+            if (trav_is_leaf(t.node) && t.leaf == kTravNone) {
+                t.leaf = t.node;
+                t.node = trav_pop<S>(t, lds, stride, spill, spill_stride);
+            }
+            if (__ballot(t.leaf == kTravNone) == 0) break;
+        }
+        while (t.leaf != kTravNone) {
+            const uint32_t ref = t.leaf & 0x07ffffffu, cnt = (t.leaf >> 27) & 15u;
+            for (uint32_t k = 0; k < cnt; ++k) {
+                const float4* tp = Sc.tri + (size_t)(ref + k) * 3;
+                const float4 a = tp[0], pb = tp[1], pc = tp[2];
+                float th, u, v;
+                if (ref_triangle(ld3(a), ld3(pb), ld3(pc), t.o, t.d, t.inv, t.tmin, t.tmax0, t.best, &th, &u, &v)) {
+                    const int32_t prim = __float_as_int(a.w);
+                    if (ANY) {
+                        t.best_prim = prim;
+                        return 2;
+                    }
+                    if (th < t.best || prim > t.best_prim) {
+                        t.best = th;
+                        t.best_prim = prim;
+                        t.bb1 = u;
+                        t.bb2 = v;
+                    }
+                }
+            }
+            t.leaf = kTravNone;
+            if (trav_is_leaf(t.node)) {
+                t.leaf = t.node;
+                t.node = trav_pop<S>(t, lds, stride, spill, spill_stride);
+            }
+        }
+        if (t.node == kTravNone) return 1;
+        if (__popcll(__ballot(true)) < refill) return 0;
+    }
 }
 
 // ------------------------------------------------------------------ surface interaction

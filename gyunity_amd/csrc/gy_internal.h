@@ -23,6 +23,11 @@ struct EnumTables {
     uint64_t frame[64];  // pixel bits reached by frame bit j
 };
 const uint32_t* Matrices();        // kDims * kCols
+// Byte tables of the matrices: entry [dim][k][v] is the XOR of the columns
+// 8k + j (j < 8, 8k + j < kCols) for the set bits j of v, so a sample is the
+// XOR of one entry per index byte.  kDims * kBytes * 256 words.
+constexpr int kBytes = 7;
+const uint32_t* ByteTables();
 const EnumTables& Enum(int m);
 float Sample(uint64_t index, uint32_t dim);
 uint64_t EnumIndex(uint32_t m, uint32_t frame, uint32_t px, uint32_t py);

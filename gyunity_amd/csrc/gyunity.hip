@@ -337,6 +337,7 @@ int SceneUpload(Ctx* c, const gy_scene_desc& d, const gy_bvh_params* bp, Scene**
     }
     if (d.n_tri <= 0 || !d.pos || !d.nrm || !d.face_n || !d.mat || d.n_mat <= 0 || !d.mats)
         return Fail(GY_ERR_INVALID, "scene: need n_tri > 0, pos, nrm, face_n, mat, mats");
+    if ((int64_t)d.n_tri > gyd::kTravMaxTriangles) return Fail(GY_ERR_UNSUPPORTED, "scene: more triangles than the traversal addresses (2^27 - 16)");
     if (d.n_light <= 0 || !d.light_tri || !d.light_le) return Fail(GY_ERR_INVALID, "scene: need at least one light");
     for (int i = 0; i < d.n_tri; ++i)
         if (d.mat[i] >= (uint32_t)d.n_mat) return Fail(GY_ERR_INVALID, "scene: material id out of range");
@@ -733,7 +734,7 @@ extern "C" int gy_open(int device, gy_context** out) {
         return HipFail(e, "hipStreamCreate");
     }
     void* p = nullptr;
-    int rc = Upload(c, &p, sobol::Matrices(), (size_t)sobol::kDims * sobol::kCols * 4);
+    int rc = Upload(c, &p, sobol::ByteTables(), (size_t)sobol::kDims * sobol::kBytes * 256 * 4);
     if (rc != GY_OK) {
         (void)hipStreamDestroy(c->stream);
         delete c;

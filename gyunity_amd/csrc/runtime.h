@@ -30,7 +30,7 @@ struct Ctx {
     int device = 0;
     int num_cu = 0;
     hipStream_t stream = nullptr;
-    const uint32_t* d_sobol = nullptr;
+    const uint32_t* d_sobol = nullptr;  // Sobol byte tables (sobol::ByteTables)
     void* d_halton = nullptr;  // digit tables, then per-dimension uint4 info and float scales
     size_t halton_dim_off = 0, halton_scale_off = 0;
     int fault = 0;
@@ -157,7 +157,7 @@ struct Batch {
     int64_t lanes = 0;        // paths per pass
     int64_t total_paths = 0;
     int depth_max = 0;
-    int refill = 1;
+    int refill = 0;  // 0: per-kind automatic threshold
     bool use_graph = true;
     bool dirty = true;        // entries changed since the table / graph were built
     hipGraphExec_t graph_exec = nullptr;

@@ -107,6 +107,24 @@ const uint32_t* Matrices() {
     return g_matrices.data();
 }
 
+const uint32_t* ByteTables() {
+    static std::vector<uint32_t> t;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        const uint32_t* M = Matrices();
+        t.assign((size_t)kDims * kBytes * 256, 0u);
+        for (int d = 0; d < kDims; ++d)
+            for (int k = 0; k < kBytes; ++k)
+                for (int v = 1; v < 256; ++v) {
+                    uint32_t r = 0;
+                    for (int j = 0; j < 8; ++j)
+                        if (((v >> j) & 1) && 8 * k + j < kCols) r ^= M[(size_t)d * kCols + 8 * k + j];
+                    t[((size_t)d * kBytes + k) * 256 + v] = r;
+                }
+    });
+    return t.data();
+}
+
 const EnumTables& Enum(int m) {
     std::call_once(g_once, build);
     return g_enum[m];

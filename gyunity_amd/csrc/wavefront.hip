@@ -34,9 +34,11 @@ using namespace gyd;
 namespace gy {
 
 enum QueueId { Q_ACTIVE = 0, Q_NEXT = 1, Q_SHADE0 = 2, Q_SHADOW = 6, Q_MIS = 7, Q_COUNT = 8 };
-// counts[] words 8..10: work-fetch counters of the persistent trace kernels
-constexpr int C_FETCH0 = 8;
-constexpr int C_WORDS = 16;
+// counts[] words 16..39: work-fetch counters of the persistent trace kernels,
+// eight per ray kind (one per queue chunk, see k_wf_trace)
+constexpr int C_FETCH0 = 16;
+constexpr int kChunks = 8;  // XCDs of the device; workgroup b runs on XCD b % 8
+constexpr int C_WORDS = 48;
 
 // Path flags (PathState.seq.w)
 constexpr uint32_t F_DEPTH_MASK = 0xffu;
@@ -191,18 +193,23 @@ __global__ void __launch_bounds__(256) k_wf_raygen_batch(WfDev W, ManyRenders R,
 // ---------------------------------------------------------------- persistent trace
 // MODE 0: closest hit of the active queue (hits binned by material),
 // MODE 1: shadow rays (any hit), MODE 2: MIS rays (closest hit + emission).
-// Each lane takes a new queue entry as soon as its ray finishes (one atomic per
-// wave); the node loop returns to refilling once fewer than `refill` lanes of
-// the wave are still traversing.  A wave exits when the queue is drained and
-// none of its lanes holds a ray.
+// A wave claims 64 queue entries per atomic.  Traversal (trav_run) returns to
+// refilling once fewer than `refill` lanes of the wave are still traversing;
+// idle lanes then take new entries.  A wave exits when the queue is drained
+// and none of its lanes holds a ray.
 constexpr uint32_t kBatch = 64;
 
 template <int S, int MODE, class R>
-__global__ void __launch_bounds__(256) k_wf_trace(R Rd, WfDev W, int refill) {
+__global__ void __launch_bounds__(256) k_wf_trace(R Rd, WfDev W, int refill, int chunks) {
     __shared__ uint32_t stack[S * 256];
     constexpr int qid = MODE == 0 ? Q_ACTIVE : (MODE == 1 ? Q_SHADOW : Q_MIS);
     const uint32_t n = W.counts[qid];
-    uint32_t* fetch = &W.counts[C_FETCH0 + MODE];
+    // The queue is cut into kChunks contiguous chunks, one per XCD.  Workgroups
+    // are dealt to the XCDs round-robin, so a wave starts on its own XCD's
+    // chunk, where its rays are neighbours of the rays the XCD's L2 already
+    // holds nodes for, and steals from the next chunks once that one is empty.
+    uint32_t* fetch = &W.counts[C_FETCH0 + MODE * kChunks];
+    int ck = (int)(blockIdx.x % kChunks) % chunks, tries = 0;  // wave-uniform
     const int lane = threadIdx.x & 63;
     const int64_t gl = (int64_t)blockIdx.x * 256 + threadIdx.x;
     uint32_t* lds = stack + threadIdx.x;
@@ -227,11 +234,20 @@ __global__ void __launch_bounds__(256) k_wf_trace(R Rd, WfDev W, int refill) {
         uint64_t nm = __ballot(need);
         while (nm) {
             if (batch_left == 0) {
-                uint32_t base = 0;
-                if (lane == 0) base = atomicAdd(fetch, (uint32_t)kBatch);
-                base = __shfl(base, 0, 64);
-                batch_base = base;
-                batch_left = base < n ? min((uint32_t)kBatch, n - base) : 0u;
+                while (tries < chunks) {
+                    const uint32_t lo = (uint32_t)(((uint64_t)n * ck) / chunks);
+                    const uint32_t hi = (uint32_t)(((uint64_t)n * (ck + 1)) / chunks);
+                    uint32_t base = 0;
+                    if (lane == 0) base = atomicAdd(&fetch[ck], (uint32_t)kBatch);
+                    base = __shfl(base, 0, 64);
+                    if (base < hi - lo) {
+                        batch_base = lo + base;
+                        batch_left = min((uint32_t)kBatch, hi - batch_base);
+                        break;
+                    }
+                    ck = ck + 1 == chunks ? 0 : ck + 1;
+                    ++tries;
+                }
                 if (batch_left == 0) {
                     if (need && my == -1) my = -2;
                     break;
@@ -266,8 +282,8 @@ __global__ void __launch_bounds__(256) k_wf_trace(R Rd, WfDev W, int refill) {
                     trav_init(t, f3(W.mis_o[slot]), f3(W.mis_d[slot]), 0.f, kInf);
                 }
             }
-            while (true) {
-                const int r = trav_step<MODE == 1, S>(Sc, t, lds, 256, spill, W.spill_stride);
+            {
+                const int r = trav_run<MODE == 1, S>(Sc, t, lds, 256, spill, W.spill_stride, refill);
                 if (r != 0) {
                     if (MODE == 0) {
                         if (t.best_prim >= 0) {
@@ -312,9 +328,7 @@ __global__ void __launch_bounds__(256) k_wf_trace(R Rd, WfDev W, int refill) {
                     }
                     fin = true;
                     my = -1;
-                    break;
                 }
-                if (__popcll(__ballot(true)) < refill) break;
             }
         }
     }
@@ -551,7 +565,7 @@ static WfDev g_view(const WavefrontState& wf, bool batch, size_t* used) {
     W.ent = batch ? (uint32_t*)take(n * 4) : nullptr;
     W.pslot = batch ? (uint32_t*)take(n * 4) : nullptr;
     for (int q = 0; q < Q_COUNT; ++q) W.q[q] = (uint32_t*)take(n * 4);
-    W.counts = (uint32_t*)take(64);
+    W.counts = (uint32_t*)take(C_WORDS * 4);
     W.capacity = n;
     W.spill = (uint32_t*)wf.spill;
     W.spill_stride = 0;
@@ -610,6 +624,13 @@ static int WfPrepareState(Ctx* c, WavefrontState& wf, int64_t n, bool batch) {
 // The bounce loop of one pass: `depth_max` bounces over the queues.  The
 // active / next queue pointers swap on the host between bounces, fixed per
 // launch, so the whole sequence can be recorded in a graph.
+// Launch shape of the three trace kinds: queue chunks (closest-hit rays spread
+// over four regions of the queue at once; shadow and MIS rays in one) and the
+// automatic refill threshold (closest-hit lanes wait for the wave's slowest ray,
+// which keeps the shade queues in ray order; shadow and MIS results go to their
+// own slots, so their lanes refill once fewer than 16 are busy).
+constexpr int kTraceChunks[3] = {4, 1, 1};
+constexpr int kAutoRefill[3] = {1, 16, 16};
 template <int STACK, class R>
 static int WfBounceLoop(Ctx* c, const R& Rd, const WfDev& W0, int depth_max, int refill, unsigned blocks,
                         unsigned long long* counters) {
@@ -617,7 +638,7 @@ static int WfBounceLoop(Ctx* c, const R& Rd, const WfDev& W0, int depth_max, int
     const unsigned pblocks = PersistBlocks(c);
     for (int depth = 0; depth < depth_max; ++depth) {
         TimerBegin(c, kKernTraceClosest);
-        hipLaunchKernelGGL((k_wf_trace<STACK, 0, R>), dim3(pblocks), dim3(256), 0, c->stream, Rd, W, refill);
+        hipLaunchKernelGGL((k_wf_trace<STACK, 0, R>), dim3(pblocks), dim3(256), 0, c->stream, Rd, W, refill > 0 ? refill : kAutoRefill[0], kTraceChunks[0]);
         TimerEnd(c);
         TimerBegin(c, kKernShade);
         hipLaunchKernelGGL((k_wf_shade<0, R>), dim3(blocks), dim3(256), 0, c->stream, Rd, W);
@@ -626,10 +647,10 @@ static int WfBounceLoop(Ctx* c, const R& Rd, const WfDev& W0, int depth_max, int
         hipLaunchKernelGGL((k_wf_shade<3, R>), dim3(blocks), dim3(256), 0, c->stream, Rd, W);
         TimerEnd(c);
         TimerBegin(c, kKernTraceShadow);
-        hipLaunchKernelGGL((k_wf_trace<STACK, 1, R>), dim3(pblocks), dim3(256), 0, c->stream, Rd, W, refill);
+        hipLaunchKernelGGL((k_wf_trace<STACK, 1, R>), dim3(pblocks), dim3(256), 0, c->stream, Rd, W, refill > 0 ? refill : kAutoRefill[1], kTraceChunks[1]);
         TimerEnd(c);
         TimerBegin(c, kKernTraceMis);
-        hipLaunchKernelGGL((k_wf_trace<STACK, 2, R>), dim3(pblocks), dim3(256), 0, c->stream, Rd, W, refill);
+        hipLaunchKernelGGL((k_wf_trace<STACK, 2, R>), dim3(pblocks), dim3(256), 0, c->stream, Rd, W, refill > 0 ? refill : kAutoRefill[2], kTraceChunks[2]);
         TimerEnd(c);
         TimerBegin(c, kKernOther);
         hipLaunchKernelGGL((k_wf_resolve<R>), dim3(blocks), dim3(256), 0, c->stream, Rd, W);
@@ -645,7 +666,7 @@ static int WfBounceLoop(Ctx* c, const R& Rd, const WfDev& W0, int depth_max, int
     return GY_OK;
 }
 
-static int RefillOf(int r) { return (r > 0 && r <= 64) ? r : 1; }
+static int RefillOf(int r) { return (r > 0 && r <= 64) ? r : 0; }
 
 int WavefrontPass(Ctx* c, Scene* sc, const PathParams& P, const gy_render_params& p, int s0, int n_s) {
     const int width = P.cam.width, height = P.cam.height;

@@ -254,12 +254,39 @@ def test_gpu_bvh_builder(gpu_ctx, level):
             out[builder] = (st.sah_cost, sc.trace_closest(rays), st.build_ms)
     print("sah gpu %.3f host %.3f  build ms gpu %.1f host %.1f" % (out[0][0], out[1][0], out[0][2], out[1][2]))
     assert out[0][0] <= 1.05 * out[1][0]
-    h0, h1 = out[0][1], out[1][1]
-    assert np.array_equal(h0["prim"] < 0, h1["prim"] < 0)
-    diff = h0["prim"] != h1["prim"]
-    tie = np.abs(h0["t"] - h1["t"]) <= T_REL * np.abs(h1["t"])
-    assert (diff & ~tie).sum() == 0
-    assert np.array_equal(h0["t"][~diff], h1["t"][~diff])
+    # closest hits, ties included, are a function of the ray and the triangles
+    assert out[0][1].tobytes() == out[1][1].tobytes()
+
+
+def test_closest_hit_ties_do_not_depend_on_tree(gpu_ctx):
+    # Rays aimed at vertices, midpoints and quarter points of triangle edges
+    # (shared edges of the icosphere and of the room's axis-aligned quads make
+    # exact-t ties between neighbours): the closest hit is the minimum of
+    # (t, -primId), so both builders and every leaf cap give the same bytes.
+    sa, _, _ = g.generate_scene(g.GY_SCENE_ICOSPHERE_GGX, 0, 5)
+    pos = sa.pos.reshape(-1, 3, 3).astype(np.float32)
+    rng = np.random.default_rng(5)
+    n = 30000
+    tri = rng.integers(0, pos.shape[0], n)
+    tri[: n // 4] = rng.integers(0, 12, n // 4)  # the room and light come first
+    e = rng.integers(0, 3, n)
+    a, b = pos[tri, e], pos[tri, (e + 1) % 3]
+    s = rng.choice(np.array([0.0, 0.25, 0.5], np.float32), n)
+    target = (a + s[:, None] * (b - a)).astype(np.float32)
+    org = rng.uniform(0.02, 0.98, (n, 3)).astype(np.float32)
+    d = target - org
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    rays = np.zeros(n, dtype=g.RAY_DTYPE)
+    rays["org"], rays["dir"], rays["tmin"], rays["tmax"] = org, d, 0.0, np.inf
+    ref = None
+    for builder in (0, 1):
+        for leaf in (1, 4, 8):
+            with g.Scene(gpu_ctx, sa, max_leaf_size=leaf, builder=builder) as sc:
+                h = sc.trace_closest(rays)
+            assert (h["prim"] >= 0).mean() > 0.9
+            if ref is None:
+                ref = h
+            assert h.tobytes() == ref.tobytes(), (builder, leaf)
 
 
 # ---------------------------------------------------------------- level (c)
