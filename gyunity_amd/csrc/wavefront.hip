@@ -39,6 +39,9 @@ enum QueueId { Q_ACTIVE = 0, Q_NEXT = 1, Q_SHADE0 = 2, Q_SHADOW = 6, Q_MIS = 7, 
 constexpr int C_FETCH0 = 16;
 constexpr int kChunks = 8;  // XCDs of the device; workgroup b runs on XCD b % 8
 constexpr int C_WORDS = 48;
+// Each counter sits in its own 128-byte line, so that atomics on different
+// queues do not serialise on one L2 channel.
+constexpr int kCtrStride = 32;
 
 // Path flags (PathState.seq.w)
 constexpr uint32_t F_DEPTH_MASK = 0xffu;
@@ -110,6 +113,39 @@ __device__ inline void wave_push(uint32_t* queue, uint32_t* count, uint32_t slot
     if (pred) queue[base + __popcll(mask & ((1ull << lane) - 1ull))] = slot;
 }
 
+// NQ pushes for a 256-lane workgroup with one atomic per queue: each wave
+// counts its pushing lanes, one thread per queue claims the workgroup's block,
+// and each lane writes at its rank within the workgroup, so entries keep lane
+// order.  Same-address atomics serialise at the L2, so one per workgroup
+// instead of one per wave matters for kernels that push from every lane.
+// Every thread of the workgroup must call it.
+template <int NQ>
+__device__ inline void block_push(uint32_t* const (&q)[NQ], uint32_t* const (&cnt)[NQ], uint32_t slot, const bool (&pred)[NQ]) {
+    __shared__ uint32_t nw[NQ][4];
+    __shared__ uint32_t base[NQ];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint64_t mask[NQ];
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+        mask[k] = __ballot(pred[k]);
+        if (lane == 0) nw[k][wave] = (uint32_t)__popcll(mask[k]);
+    }
+    __syncthreads();
+    if (threadIdx.x < NQ) {
+        const int k = threadIdx.x;
+        const uint32_t total = nw[k][0] + nw[k][1] + nw[k][2] + nw[k][3];
+        base[k] = total ? atomicAdd(cnt[k], total) : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+        if (!pred[k]) continue;
+        uint32_t off = base[k];
+        for (int w = 0; w < wave; ++w) off += nw[k][w];
+        q[k][off + __popcll(mask[k] & ((1ull << lane) - 1ull))] = slot;
+    }
+}
+
 __device__ inline V3 f3(const float4 v) { return v3(v.x, v.y, v.z); }
 
 // Camera sample of one path (TiledIntegrator::Render per sample,
@@ -149,7 +185,10 @@ __global__ void __launch_bounds__(256) k_wf_raygen(WfDev W, PathParams P, int s0
     }
     const uint32_t slot = (uint32_t)((int64_t)s_local * width * height + (int64_t)py * width + px);
     if (valid) start_path(W, P, slot, s0 + s_local, px, py);
-    wave_push(W.q[Q_ACTIVE], &W.counts[Q_ACTIVE], slot, valid);
+    uint32_t* const qa[1] = {W.q[Q_ACTIVE]};
+    uint32_t* const ca[1] = {&W.counts[Q_ACTIVE * kCtrStride]};
+    const bool pa[1] = {valid};
+    block_push<1>(qa, ca, slot, pa);
 }
 
 // Batch: lanes [0, count) take global path indices base + lane.  Entry e owns
@@ -187,7 +226,10 @@ __global__ void __launch_bounds__(256) k_wf_raygen_batch(WfDev W, ManyRenders R,
         W.pslot[slot] = (uint32_t)((int64_t)s * P.cam.width * P.cam.height + (int64_t)py * P.cam.width + px);
         start_path(W, P, slot, s, px, py);
     }
-    wave_push(W.q[Q_ACTIVE], &W.counts[Q_ACTIVE], slot, valid);
+    uint32_t* const qa[1] = {W.q[Q_ACTIVE]};
+    uint32_t* const ca[1] = {&W.counts[Q_ACTIVE * kCtrStride]};
+    const bool pa[1] = {valid};
+    block_push<1>(qa, ca, slot, pa);
 }
 
 // ---------------------------------------------------------------- persistent trace
@@ -203,12 +245,12 @@ template <int S, int MODE, class R>
 __global__ void __launch_bounds__(256) k_wf_trace(R Rd, WfDev W, int refill, int chunks) {
     __shared__ uint32_t stack[S * 256];
     constexpr int qid = MODE == 0 ? Q_ACTIVE : (MODE == 1 ? Q_SHADOW : Q_MIS);
-    const uint32_t n = W.counts[qid];
+    const uint32_t n = W.counts[(qid) * kCtrStride];
     // The queue is cut into kChunks contiguous chunks, one per XCD.  Workgroups
     // are dealt to the XCDs round-robin, so a wave starts on its own XCD's
     // chunk, where its rays are neighbours of the rays the XCD's L2 already
     // holds nodes for, and steals from the next chunks once that one is empty.
-    uint32_t* fetch = &W.counts[C_FETCH0 + MODE * kChunks];
+    uint32_t* fetch = &W.counts[(C_FETCH0 + MODE * kChunks) * kCtrStride];
     int ck = (int)(blockIdx.x % kChunks) % chunks, tries = 0;  // wave-uniform
     const int lane = threadIdx.x & 63;
     const int64_t gl = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -225,7 +267,7 @@ __global__ void __launch_bounds__(256) k_wf_trace(R Rd, WfDev W, int refill, int
     while (true) {
         if (MODE == 0) {
 #pragma unroll
-            for (int m = 0; m < 4; ++m) wave_push(W.q[Q_SHADE0 + m], &W.counts[Q_SHADE0 + m], slot, fin && mt == m);
+            for (int m = 0; m < 4; ++m) wave_push(W.q[Q_SHADE0 + m], &W.counts[(Q_SHADE0 + m) * kCtrStride], slot, fin && mt == m);
         }
         fin = false;
         // Refill idle lanes from the wave's batch of queue entries; when the
@@ -238,7 +280,7 @@ __global__ void __launch_bounds__(256) k_wf_trace(R Rd, WfDev W, int refill, int
                     const uint32_t lo = (uint32_t)(((uint64_t)n * ck) / chunks);
                     const uint32_t hi = (uint32_t)(((uint64_t)n * (ck + 1)) / chunks);
                     uint32_t base = 0;
-                    if (lane == 0) base = atomicAdd(&fetch[ck], (uint32_t)kBatch);
+                    if (lane == 0) base = atomicAdd(&fetch[ck * kCtrStride], (uint32_t)kBatch);
                     base = __shfl(base, 0, 64);
                     if (base < hi - lo) {
                         batch_base = lo + base;
@@ -337,7 +379,7 @@ __global__ void __launch_bounds__(256) k_wf_trace(R Rd, WfDev W, int refill, int
 // ---------------------------------------------------------------- shade
 template <int MTYPE, class R>
 __global__ void __launch_bounds__(256) k_wf_shade(R Rd, WfDev W) {
-    const uint32_t n = W.counts[Q_SHADE0 + MTYPE];
+    const uint32_t n = W.counts[(Q_SHADE0 + MTYPE) * kCtrStride];
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (blockIdx.x * 256 >= n) return;
     const bool valid = i < n;
@@ -461,9 +503,10 @@ __global__ void __launch_bounds__(256) k_wf_shade(R Rd, WfDev W) {
         W.L[slot] = make_float4(L.x, L.y, L.z, Lv.w);
         W.seq[slot] = make_uint4(sq.x, sq.y, q.cursor, nflags);
     }
-    wave_push(W.q[Q_SHADOW], &W.counts[Q_SHADOW], slot, push_shadow);
-    wave_push(W.q[Q_MIS], &W.counts[Q_MIS], slot, push_mis);
-    wave_push(W.q[Q_NEXT], &W.counts[Q_NEXT], slot, push_next);
+    uint32_t* const qs[3] = {W.q[Q_SHADOW], W.q[Q_MIS], W.q[Q_NEXT]};
+    uint32_t* const cs[3] = {&W.counts[Q_SHADOW * kCtrStride], &W.counts[Q_MIS * kCtrStride], &W.counts[Q_NEXT * kCtrStride]};
+    const bool ps[3] = {push_shadow, push_mis, push_next};
+    block_push<3>(qs, cs, slot, ps);
 }
 
 // ---------------------------------------------------------------- resolve
@@ -471,7 +514,7 @@ __global__ void __launch_bounds__(256) k_wf_shade(R Rd, WfDev W) {
 // integrator/PT.cpp:38), then finished paths go to the film.
 template <class R>
 __global__ void __launch_bounds__(256) k_wf_resolve(R Rd, WfDev W) {
-    const uint32_t n = W.counts[Q_ACTIVE];
+    const uint32_t n = W.counts[(Q_ACTIVE) * kCtrStride];
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const uint32_t slot = W.q[Q_ACTIVE][i];
@@ -525,16 +568,16 @@ __global__ void __launch_bounds__(256) k_wf_resolve(R Rd, WfDev W) {
 __global__ void k_wf_advance(WfDev W, unsigned long long* ctr) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         // rays traced this bounce: camera/continuation, shadow, MIS
-        ctr[0] += W.counts[Q_ACTIVE];
-        ctr[1] += W.counts[Q_SHADOW];
-        ctr[2] += W.counts[Q_MIS];
-        W.counts[Q_ACTIVE] = W.counts[Q_NEXT];
-        W.counts[Q_NEXT] = 0;
-        for (int q = Q_SHADE0; q < C_WORDS; ++q) W.counts[q] = 0;
+        ctr[0] += W.counts[(Q_ACTIVE) * kCtrStride];
+        ctr[1] += W.counts[(Q_SHADOW) * kCtrStride];
+        ctr[2] += W.counts[(Q_MIS) * kCtrStride];
+        W.counts[(Q_ACTIVE) * kCtrStride] = W.counts[(Q_NEXT) * kCtrStride];
+        W.counts[(Q_NEXT) * kCtrStride] = 0;
+        for (int q = Q_SHADE0; q < C_WORDS; ++q) W.counts[(q) * kCtrStride] = 0;
     }
 }
 __global__ void k_wf_reset(WfDev W) {
-    if (threadIdx.x < C_WORDS && blockIdx.x == 0) W.counts[threadIdx.x] = 0;
+    if (threadIdx.x < C_WORDS && blockIdx.x == 0) W.counts[(threadIdx.x) * kCtrStride] = 0;
 }
 
 // ---------------------------------------------------------------- host side
@@ -565,7 +608,7 @@ static WfDev g_view(const WavefrontState& wf, bool batch, size_t* used) {
     W.ent = batch ? (uint32_t*)take(n * 4) : nullptr;
     W.pslot = batch ? (uint32_t*)take(n * 4) : nullptr;
     for (int q = 0; q < Q_COUNT; ++q) W.q[q] = (uint32_t*)take(n * 4);
-    W.counts = (uint32_t*)take(C_WORDS * 4);
+    W.counts = (uint32_t*)take(C_WORDS * kCtrStride * 4);
     W.capacity = n;
     W.spill = (uint32_t*)wf.spill;
     W.spill_stride = 0;
@@ -575,7 +618,7 @@ static WfDev g_view(const WavefrontState& wf, bool batch, size_t* used) {
 
 static size_t WfBytes(int64_t n, bool batch) {
     const size_t per = 14 * 16 + Q_COUNT * 4 + (batch ? 8 : 0);
-    return (size_t)n * per + 256 * (14 + Q_COUNT + 4) + 4096;
+    return (size_t)n * per + 256 * (14 + Q_COUNT + 4) + C_WORDS * kCtrStride * 4 + 4096;
 }
 
 void WavefrontStateFree(WavefrontState* wf) {
