@@ -150,25 +150,50 @@ __global__ void k_bvh_bin_clear(BuildDev B, uint32_t ntask) {
     if (i < ntask * 2) B.cursor[i] = 0u;
 }
 
-__global__ void k_bvh_bin(BuildDev B, int cur, int32_t n) {
+// Bins of each primitive's task on the task's longest centroid axis.  Tasks
+// are contiguous index ranges, so at the upper levels a whole workgroup
+// belongs to one task (the task of its first index): those primitives are
+// binned in LDS and the workgroup adds its bins to the task's with one global
+// atomic per word.  Primitives of any other task go to global memory directly.
+// Same-address global atomics serialise at the L2, and the upper levels put
+// every primitive of the scene on one task's 16 bins.
+__global__ void __launch_bounds__(256) k_bvh_bin(BuildDev B, int cur, int32_t n) {
+    __shared__ uint32_t lb[kTaskBinWords];
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int32_t t = B.tid[cur][i];
-    if (t < 0) return;
-    const BTask T = B.tasks[cur][t];
-    const uint32_t p = B.idx[cur][i];
-    const float4 a = B.pmin[p], b = B.pmax[p];
-    const float mn[3] = {a.x, a.y, a.z}, mx[3] = {b.x, b.y, b.z};
-    const float c[3] = {0.5f * a.x + 0.5f * b.x, 0.5f * a.y + 0.5f * b.y, 0.5f * a.z + 0.5f * b.z};
-    uint32_t* bins = B.bins + (size_t)t * kTaskBinWords;
-    const int ax = long_axis(T.cmin, T.cmax);
-    uint32_t* w = bins + bin_of(c[ax], T.cmin[ax], T.cmax[ax]) * kBinWords;
-    atomicAdd(&w[0], 1u);
-    for (int k = 0; k < 3; ++k) {
-        atomicMin(&w[1 + k], enc(mn[k]));
-        atomicMax(&w[4 + k], enc(mx[k]));
-        atomicMin(&w[7 + k], enc(c[k]));
-        atomicMax(&w[10 + k], enc(c[k]));
+    const int32_t first = blockIdx.x * blockDim.x;
+    const int32_t t0 = B.tid[cur][first];  // first < n: the grid covers n
+    for (int w = threadIdx.x; w < kTaskBinWords; w += blockDim.x) {
+        const int k = w % kBinWords;
+        lb[w] = k == 0 ? 0u : ((k >= 1 && k <= 3) || (k >= 7 && k <= 9) ? 0xffffffffu : 0u);
+    }
+    __syncthreads();
+    const int32_t t = i < n ? B.tid[cur][i] : -1;
+    if (t >= 0) {
+        const BTask T = B.tasks[cur][t];
+        const uint32_t p = B.idx[cur][i];
+        const float4 a = B.pmin[p], b = B.pmax[p];
+        const float mn[3] = {a.x, a.y, a.z}, mx[3] = {b.x, b.y, b.z};
+        const float c[3] = {0.5f * a.x + 0.5f * b.x, 0.5f * a.y + 0.5f * b.y, 0.5f * a.z + 0.5f * b.z};
+        const int ax = long_axis(T.cmin, T.cmax);
+        const int bin = bin_of(c[ax], T.cmin[ax], T.cmax[ax]);
+        uint32_t* w = (t == t0 ? lb : B.bins + (size_t)t * kTaskBinWords) + bin * kBinWords;
+        atomicAdd(&w[0], 1u);
+        for (int k = 0; k < 3; ++k) {
+            atomicMin(&w[1 + k], enc(mn[k]));
+            atomicMax(&w[4 + k], enc(mx[k]));
+            atomicMin(&w[7 + k], enc(c[k]));
+            atomicMax(&w[10 + k], enc(c[k]));
+        }
+    }
+    __syncthreads();
+    if (t0 < 0) return;
+    for (int w = threadIdx.x; w < kTaskBinWords; w += blockDim.x) {
+        const int bin = w / kBinWords, k = w % kBinWords;
+        if (lb[bin * kBinWords] == 0u) continue;
+        uint32_t* g = B.bins + (size_t)t0 * kTaskBinWords + w;
+        if (k == 0) atomicAdd(g, lb[w]);
+        else if ((k >= 1 && k <= 3) || (k >= 7 && k <= 9)) atomicMin(g, lb[w]);
+        else atomicMax(g, lb[w]);
     }
 }
 
@@ -314,29 +339,60 @@ __global__ void k_bvh_split(BuildDev B, int cur, uint32_t ntask) {
     B.child[2 * t + 1] = emit_child(B, next, T, 1, T.begin + nl, T.end, rbmn, rbmx, rcmn, rcmx);
 }
 
-__global__ void k_bvh_partition(BuildDev B, int cur, int32_t n) {
+// Moves each primitive to its child's range.  As in k_bvh_bin, the primitives
+// of the workgroup's first task take their places with two atomics per
+// workgroup (left and right totals, ranked by wave ballots); the others claim
+// theirs one atomic at a time.
+__global__ void __launch_bounds__(256) k_bvh_partition(BuildDev B, int cur, int32_t n) {
+    __shared__ uint32_t cnt[2][4];
+    __shared__ uint32_t base[2];
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t t0 = B.tid[cur][blockIdx.x * blockDim.x];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int32_t t = i < n ? B.tid[cur][i] : -1;
+    const uint32_t p = i < n ? B.idx[cur][i] : 0u;
+    bool left = false;
+    if (t >= 0) {
+        const BTask& T = B.tasks[cur][t];
+        const uint32_t ax = B.split[3 * t], sb = B.split[3 * t + 1], nl = B.split[3 * t + 2];
+        if (ax == 0xffffffffu) {
+            left = (uint32_t)i - T.begin < nl;
+        } else {
+            const float4 a = B.pmin[p], b = B.pmax[p];
+            const float c = ax == 0 ? 0.5f * a.x + 0.5f * b.x : (ax == 1 ? 0.5f * a.y + 0.5f * b.y : 0.5f * a.z + 0.5f * b.z);
+            left = bin_of(c, T.cmin[ax], T.cmax[ax]) <= (int)sb;
+        }
+    }
+    const bool mine = t >= 0 && t == t0;
+    const uint64_t ml = __ballot(mine && left), mr = __ballot(mine && !left);
+    if (lane == 0) {
+        cnt[0][wave] = (uint32_t)__popcll(ml);
+        cnt[1][wave] = (uint32_t)__popcll(mr);
+    }
+    __syncthreads();
+    if (t0 >= 0 && threadIdx.x < 2) {
+        const int s = threadIdx.x;
+        const uint32_t total = cnt[s][0] + cnt[s][1] + cnt[s][2] + cnt[s][3];
+        base[s] = total ? atomicAdd(&B.cursor[2 * t0 + s], total) : 0u;
+    }
+    __syncthreads();
     if (i >= n) return;
-    const int32_t t = B.tid[cur][i];
-    const uint32_t p = B.idx[cur][i];
     if (t < 0) {
         B.idx[cur ^ 1][i] = p;
         B.tid[cur ^ 1][i] = -1;
         return;
     }
     const BTask& T = B.tasks[cur][t];
-    const uint32_t ax = B.split[3 * t], sb = B.split[3 * t + 1], nl = B.split[3 * t + 2];
-    bool left;
-    if (ax == 0xffffffffu) {
-        left = (uint32_t)i - T.begin < nl;
+    const uint32_t nl = B.split[3 * t + 2];
+    uint32_t rank;
+    if (mine) {
+        const int s = left ? 0 : 1;
+        rank = base[s] + (uint32_t)__popcll((left ? ml : mr) & ((1ull << lane) - 1ull));
+        for (int w = 0; w < wave; ++w) rank += cnt[s][w];
     } else {
-        const float4 a = B.pmin[p], b = B.pmax[p];
-        const float c = ax == 0 ? 0.5f * a.x + 0.5f * b.x : (ax == 1 ? 0.5f * a.y + 0.5f * b.y : 0.5f * a.z + 0.5f * b.z);
-        left = bin_of(c, T.cmin[ax], T.cmax[ax]) <= (int)sb;
+        rank = atomicAdd(&B.cursor[2 * t + (left ? 0 : 1)], 1u);
     }
-    uint32_t dst;
-    if (left) dst = T.begin + atomicAdd(&B.cursor[2 * t], 1u);
-    else dst = T.begin + nl + atomicAdd(&B.cursor[2 * t + 1], 1u);
+    const uint32_t dst = left ? T.begin + rank : T.begin + nl + rank;
     B.idx[cur ^ 1][dst] = p;
     B.tid[cur ^ 1][dst] = B.child[2 * t + (left ? 0 : 1)];
 }
