@@ -100,11 +100,11 @@ def batch_main(args):
         log("batch step %d: %.1f ms, %.1f Mrays" % (i, st.total_ms, st.rays / 1e6))
     ms_batch = (time.perf_counter() - t0) * 1e3 / args.steps
     per_scene = {}
-    # per-member launch choice by the autotuner (pilot renders of 4 spp)
+    # per-member launch choice by the autotuner (its automatic pilot size)
     tuned = []
     t0 = time.perf_counter()
     for sc, (_, cam, prm) in zip(scenes, members):
-        tuned.append(sc.autotune(cam, prm, 4)[0])
+        tuned.append(sc.autotune(cam, prm, 0)[0])
     log("autotuned %d members in %.1f s: %d megakernel, %d wavefront" % (
         len(tuned), time.perf_counter() - t0, sum(t.mode == g.GY_RENDER_MEGAKERNEL for t in tuned),
         sum(t.mode == g.GY_RENDER_WAVEFRONT for t in tuned)))

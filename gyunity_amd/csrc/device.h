@@ -70,11 +70,14 @@ GY_D float dlog(float x) { return (float)log((double)x); }
 // T: byte tables, 7 x 256 words per dimension, entry [k][v] = XOR of the
 // matrix columns 8k + j for the set bits j of v.  The XOR over the set bits of
 // the index splits into one entry per index byte (indices stay below 2^52).
+// The byte loop is kept rolled: unrolled, the compiler hoists the lookups of
+// every sample of a path vertex at once, which costs long kernels (the adjoint
+// replay) registers and occupancy.
 GY_D float sobol_sample(const uint32_t* __restrict__ T, uint64_t index, uint32_t dim) {
     const uint32_t* t = T + (size_t)dim * (7 * 256);
-    const uint32_t lo = (uint32_t)index, hi = (uint32_t)(index >> 32);
-    uint32_t r = (t[lo & 255u] ^ t[256 + ((lo >> 8) & 255u)]) ^ (t[512 + ((lo >> 16) & 255u)] ^ t[768 + (lo >> 24)]);
-    if (hi != 0u) r ^= (t[1024 + (hi & 255u)] ^ t[1280 + ((hi >> 8) & 255u)]) ^ t[1536 + ((hi >> 16) & 255u)];
+    uint32_t r = 0;
+#pragma unroll 1
+    for (; index; index >>= 8, t += 256) r ^= t[index & 255u];
     return (float)r * 2.3283064365386963e-10f;
 }
 // HaltonSampler::sample with Faure permutations (sampler/Halton.h:648-910,

@@ -182,10 +182,12 @@ int gy_count_traffic(gy_scene* scene, const gy_camera_desc* cam, const gy_render
 
 /* ---- launch autotuner --------------------------------------------------------
  * No counterpart in the reference (TiledIntegrator fixes 32x32 tiles,
- * integrator/TiledIntegrator.cpp:13).  Times pilot renders (pilot_spp samples,
- * 0 = 4) of this scene and view over the integrator form, the megakernel
- * workgroup size and the wavefront refill threshold; writes base with the
- * fastest choice into best.  The image does not depend on the choice. */
+ * integrator/TiledIntegrator.cpp:13).  Times pilot renders of this scene and
+ * view (pilot_spp samples per pixel; 0 = the render itself up to 2^23 paths,
+ * else the fewest samples that reach 2^23 paths): the integrator form first,
+ * then the megakernel workgroup size or the wavefront refill threshold of the
+ * faster form; writes base with the fastest choice into best.  The image does
+ * not depend on the choice. */
 int gy_autotune(gy_scene* scene, const gy_camera_desc* cam, const gy_render_params* base, int32_t pilot_spp,
                 gy_render_params* best, double* best_ms_per_mpath);
 
