@@ -483,9 +483,9 @@ GY_D bool traverse_short(const SceneDev& Sc, V3 o, V3 d, float tmin, float tmax,
     return true;
 }
 
-// Resumable traversal state for the persistent trace kernels: one node visit
-// per trav_step call, so a lane can be refilled with a new ray as soon as its
-// ray finishes.  Same node tests, triangle gating and tie rule as traverse().
+// Resumable traversal state for the persistent trace kernels (trav_run), so
+// that a lane can be refilled with a new ray when its ray finishes.  Same node
+// tests, triangle gating and tie rule as traverse().
 struct Trav {
     V3 o, d, inv;
     float tmin, tmax0, best, bb1, bb2;  // tmax0: the ray's own tMax
@@ -506,91 +506,6 @@ GY_D void trav_init(Trav& t, V3 o, V3 d, float tmin, float tmax) {
     t.head = t.nlds = t.nspill = 0;
     t.leaf = 0xffffffffu;
 }
-// Returns 1 when the traversal finished (t.best_prim >= 0 on a hit), 2 when an
-// any-hit query found a hit, 0 to continue.
-template <bool ANY, int S>
-GY_D int trav_step(const SceneDev& Sc, Trav& t, uint32_t* lds, int stride, uint32_t* __restrict__ spill, int64_t spill_stride) {
-    const uint4* np = Sc.nodes + (size_t)t.node * 4;
-    const uint4 q0 = np[0], q1 = np[1], q2 = np[2], q3 = np[3];
-    const float b0[6] = {__uint_as_float(q0.x), __uint_as_float(q0.y), __uint_as_float(q0.z),
-                         __uint_as_float(q0.w), __uint_as_float(q1.x), __uint_as_float(q1.y)};
-    const float b1[6] = {__uint_as_float(q1.z), __uint_as_float(q1.w), __uint_as_float(q2.x),
-                         __uint_as_float(q2.y), __uint_as_float(q2.z), __uint_as_float(q2.w)};
-    const float t0 = (q3.x != 0xffffffffu) ? box_enter(b0, t.o, t.inv, t.tmin, t.best) : kInf;
-    const float t1 = (q3.y != 0xffffffffu) ? box_enter(b1, t.o, t.inv, t.tmin, t.best) : kInf;
-    uint32_t next[2];
-    float tn[2];
-    int nn = 0;
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-        const float tc = c == 0 ? t0 : t1;
-        if (tc == kInf) continue;
-        const uint32_t ref = c == 0 ? q3.x : q3.y;
-        const uint32_t cnt = c == 0 ? q3.z : q3.w;
-        if (cnt == 0) {
-            next[nn] = ref;
-            tn[nn] = tc;
-            ++nn;
-            continue;
-        }
-        for (uint32_t k = 0; k < cnt; ++k) {
-            const float4* tp = Sc.tri + (size_t)(ref + k) * 3;
-            const float4 a = tp[0], pb = tp[1], pc = tp[2];
-            float th, u, v;
-            if (ref_triangle(ld3(a), ld3(pb), ld3(pc), t.o, t.d, t.inv, t.tmin, t.tmax0, t.best, &th, &u, &v)) {
-                const int32_t prim = __float_as_int(a.w);
-                if (ANY) {
-                    t.best_prim = prim;
-                    return 2;
-                }
-                if (th < t.best || prim > t.best_prim) {
-                    t.best = th;
-                    t.best_prim = prim;
-                    t.bb1 = u;
-                    t.bb2 = v;
-                }
-            }
-        }
-    }
-    if (nn == 2 && tn[1] < tn[0]) {
-        const uint32_t tr = next[0];
-        next[0] = next[1];
-        next[1] = tr;
-        const float tt = tn[0];
-        tn[0] = tn[1];
-        tn[1] = tt;
-    }
-    if (nn == 2 && tn[1] <= t.best) {
-        if (t.nlds == (uint32_t)S) {
-            spill[(int64_t)t.nspill * spill_stride] = lds[t.head * stride];
-            ++t.nspill;
-            t.head = t.head + 1 == (uint32_t)S ? 0 : t.head + 1;
-            --t.nlds;
-        }
-        uint32_t pos = t.head + t.nlds;
-        if (pos >= (uint32_t)S) pos -= S;
-        lds[pos * stride] = next[1];
-        ++t.nlds;
-    }
-    if (nn >= 1 && tn[0] <= t.best) {
-        t.node = next[0];
-        return 0;
-    }
-    if (t.nlds > 0) {
-        --t.nlds;
-        uint32_t pos = t.head + t.nlds;
-        if (pos >= (uint32_t)S) pos -= S;
-        t.node = lds[pos * stride];
-        return 0;
-    }
-    if (t.nspill > 0) {
-        --t.nspill;
-        t.node = spill[(int64_t)t.nspill * spill_stride];
-        return 0;
-    }
-    return 1;
-}
-
 // Traversal entries of trav_run: an interior node index (bit 31 clear), a
 // leaf (bit 31 set, triangle count 1..15 in bits 27-30, first triangle slot in
 // bits 0-26), or kTravNone.  Scenes stay below kTravMaxTriangles so that no
@@ -635,8 +550,8 @@ GY_D uint32_t trav_pop(Trav& t, uint32_t* lds, int stride, uint32_t* __restrict_
 // traversing holds one leaf (a lane's first leaf is postponed and it keeps
 // walking); then the lanes test their postponed leaves together, and a second
 // leaf met on the way.  The node tests, culling and triangle gate are
-// trav_step's (a leaf child is kept when its padded box is entered, an interior
-// child when its entry distance is also <= best), and triangles are still
+// traverse()'s (a leaf child is kept when its padded box is entered, an
+// interior child when its entry distance is also <= best), and triangles are
 // tested in the order their leaves are reached.  Returns 1 when the ray is
 // finished (t.best_prim >= 0 on a hit), 2 when an any-hit query found a hit,
 // and 0 when fewer than `refill` lanes remain, so that the caller can refill.
