@@ -6,8 +6,8 @@
 // Choices: integrator form (megakernel / wavefront), compared first; then,
 // within the faster form, the megakernel workgroup size (64, 128, 256 lanes;
 // its LDS traversal stack follows the BVH depth) or the wavefront's refill
-// threshold for its persistent trace waves (automatic: 1 for closest-hit rays,
-// 16 for shadow and MIS rays; or 1, or 16 for all).  No counterpart in the reference, whose TiledIntegrator fixes 32x32
+// threshold for its persistent trace waves (automatic: 48 busy lanes; or 16,
+// or 32).  No counterpart in the reference, whose TiledIntegrator fixes 32x32
 // tiles (integrator/TiledIntegrator.cpp:13).
 #include <algorithm>
 #include <chrono>
@@ -66,7 +66,7 @@ extern "C" int gy_autotune(gy_scene* scene, const gy_camera_desc* cam, const gy_
     const Cand first[] = {{GY_RENDER_WAVEFRONT, 0, 0}, {GY_RENDER_MEGAKERNEL, 128, 0}};
     for (const Cand& k : first)
         if ((rc = trial(k)) != GY_OK) return rc;
-    const Cand more_wf[] = {{GY_RENDER_WAVEFRONT, 0, 1}, {GY_RENDER_WAVEFRONT, 0, 16}};
+    const Cand more_wf[] = {{GY_RENDER_WAVEFRONT, 0, 16}, {GY_RENDER_WAVEFRONT, 0, 32}};
     const Cand more_mk[] = {{GY_RENDER_MEGAKERNEL, 64, 0}, {GY_RENDER_MEGAKERNEL, 256, 0}};
     const Cand(&more)[2] = best->mode == GY_RENDER_WAVEFRONT ? more_wf : more_mk;
     for (const Cand& k : more)

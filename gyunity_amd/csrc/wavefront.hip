@@ -59,6 +59,7 @@ struct WfDev {
     float4* L;      // radiance (xyz), film y in w
     uint4* seq;     // sample index lo/hi, cursor, flags
     float4* hit;    // t, prim, b1, b2
+    uint8_t* mtype;  // per active-queue position: material type of the hit (0..3), 255 for a miss
     float4* pend0;  // L1 before the shadow test (xyz), w1
     float4* pend1;  // BSDF value of the MIS sample (xyz) / L2 after trace_mis, cos
     float4* pend2;  // pdf of the MIS sample, w2, P(light), light id
@@ -258,18 +259,12 @@ __global__ void __launch_bounds__(256) k_wf_trace(R Rd, WfDev W, int refill, int
     uint32_t* spill = W.spill + gl;
     int64_t my = -1;  // -1 wants work, -2 queue drained, >= 0 queue index
     uint32_t batch_base = 0, batch_left = 0;  // wave-uniform
-    bool fresh = false, fin = false;
-    int mt = -1;
+    bool fresh = false;
     uint32_t slot = 0, e = 0;
     SceneDev Sc;  // traversal reads nodes and tri only
     if (!R::kBatch) Sc = Rd.scene(0);
     Trav t;
     while (true) {
-        if (MODE == 0) {
-#pragma unroll
-            for (int m = 0; m < 4; ++m) wave_push(W.q[Q_SHADE0 + m], &W.counts[(Q_SHADE0 + m) * kCtrStride], slot, fin && mt == m);
-        }
-        fin = false;
         // Refill idle lanes from the wave's batch of queue entries; when the
         // batch runs dry, claim the next `kBatch` entries with one atomic.
         const bool need = my == -1;
@@ -332,9 +327,9 @@ __global__ void __launch_bounds__(256) k_wf_trace(R Rd, WfDev W, int refill, int
                             const SceneDev& Sf = Rd.scene(e);
                             W.hit[slot] = make_float4(t.best, __int_as_float(t.best_prim), t.bb1, t.bb2);
                             const uint32_t mat = __float_as_uint(Sf.shade[(size_t)t.best_prim * 8].w);
-                            mt = (int)Sf.mats[(size_t)mat * 32];
+                            W.mtype[my] = (uint8_t)Sf.mats[(size_t)mat * 32];
                         } else {
-                            mt = -1;
+                            W.mtype[my] = 255;
                             uint4 sq = W.seq[slot];
                             sq.w |= F_DONE;
                             W.seq[slot] = sq;
@@ -368,11 +363,62 @@ __global__ void __launch_bounds__(256) k_wf_trace(R Rd, WfDev W, int refill, int
                             W.pend3[slot] = p3;
                         }
                     }
-                    fin = true;
                     my = -1;
                 }
             }
         }
+    }
+}
+
+// ---------------------------------------------------------------- binning
+// The closest-hit trace writes each hit's material type at the ray's position
+// in the active queue; this pass bins the active queue into the four shade
+// queues in queue order.  The order of the shade queues (neighbouring paths
+// together) then does not depend on when each ray finished, which lets the
+// closest-hit waves refill early.  Each thread takes four consecutive entries
+// and the workgroup claims its block of each queue with one atomic.
+constexpr int kBinPerThread = 4;
+__global__ void __launch_bounds__(256) k_wf_bin(WfDev W) {
+    __shared__ uint64_t wsum[4];
+    __shared__ uint32_t base[4];
+    const uint32_t n = W.counts[Q_ACTIVE * kCtrStride];
+    const uint32_t i0 = (blockIdx.x * 256 + threadIdx.x) * kBinPerThread;
+    if (blockIdx.x * 256 * kBinPerThread >= n) return;
+    uint32_t slot[kBinPerThread], m[kBinPerThread];
+    uint64_t cnt = 0;  // four 16-bit counts, one per material type
+#pragma unroll
+    for (int k = 0; k < kBinPerThread; ++k) {
+        const uint32_t i = i0 + k;
+        slot[k] = i < n ? W.q[Q_ACTIVE][i] : 0u;
+        m[k] = i < n ? W.mtype[i] : 255u;
+        if (m[k] < 4u) cnt += 1ull << (16 * m[k]);
+    }
+    // inclusive scan of the packed counts over the wave, then over the waves
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint64_t inc = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint64_t v = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += v;
+    }
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        const uint32_t mt = threadIdx.x;
+        const uint64_t tot = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        const uint32_t c = (uint32_t)((tot >> (16 * mt)) & 0xffffu);
+        base[mt] = c ? atomicAdd(&W.counts[(Q_SHADE0 + mt) * kCtrStride], c) : 0u;
+    }
+    __syncthreads();
+    uint64_t before = inc - cnt;
+    for (int w = 0; w < wave; ++w) before += wsum[w];
+#pragma unroll
+    for (int k = 0; k < kBinPerThread; ++k) {
+        if (m[k] >= 4u) continue;
+        const uint32_t mt = m[k];
+        const uint32_t pos = base[mt] + (uint32_t)((before >> (16 * mt)) & 0xffffu);
+        W.q[Q_SHADE0 + mt][pos] = slot[k];
+        before += 1ull << (16 * mt);
     }
 }
 
@@ -597,6 +643,7 @@ static WfDev g_view(const WavefrontState& wf, bool batch, size_t* used) {
     W.L = (float4*)take(n * 16);
     W.seq = (uint4*)take(n * 16);
     W.hit = (float4*)take(n * 16);
+    W.mtype = (uint8_t*)take(n);
     W.pend0 = (float4*)take(n * 16);
     W.pend1 = (float4*)take(n * 16);
     W.pend2 = (float4*)take(n * 16);
@@ -617,8 +664,8 @@ static WfDev g_view(const WavefrontState& wf, bool batch, size_t* used) {
 }
 
 static size_t WfBytes(int64_t n, bool batch) {
-    const size_t per = 14 * 16 + Q_COUNT * 4 + (batch ? 8 : 0);
-    return (size_t)n * per + 256 * (14 + Q_COUNT + 4) + C_WORDS * kCtrStride * 4 + 4096;
+    const size_t per = 14 * 16 + 1 + Q_COUNT * 4 + (batch ? 8 : 0);
+    return (size_t)n * per + 256 * (15 + Q_COUNT + 4) + C_WORDS * kCtrStride * 4 + 4096;
 }
 
 void WavefrontStateFree(WavefrontState* wf) {
@@ -667,13 +714,13 @@ static int WfPrepareState(Ctx* c, WavefrontState& wf, int64_t n, bool batch) {
 // The bounce loop of one pass: `depth_max` bounces over the queues.  The
 // active / next queue pointers swap on the host between bounces, fixed per
 // launch, so the whole sequence can be recorded in a graph.
-// Launch shape of the three trace kinds: queue chunks (closest-hit rays spread
-// over four regions of the queue at once; shadow and MIS rays in one) and the
-// automatic refill threshold (closest-hit lanes wait for the wave's slowest ray,
-// which keeps the shade queues in ray order; shadow and MIS results go to their
-// own slots, so their lanes refill once fewer than 16 are busy).
-constexpr int kTraceChunks[3] = {4, 1, 1};
-constexpr int kAutoRefill[3] = {1, 16, 16};
+// Launch shape of the three trace kinds.  Queue chunks: closest-hit rays are
+// claimed from eight regions of the queue at once (one per XCD), shadow rays
+// from four, MIS rays from one.  Automatic refill: a lane takes a new ray once
+// fewer than 48 lanes of its wave are still traversing.  Refilling does not
+// reorder the shade queues, which k_wf_bin builds in queue order.
+constexpr int kTraceChunks[3] = {8, 4, 1};
+constexpr int kAutoRefill = 48;
 template <int STACK, class R>
 static int WfBounceLoop(Ctx* c, const R& Rd, const WfDev& W0, int depth_max, int refill, unsigned blocks,
                         unsigned long long* counters) {
@@ -681,19 +728,20 @@ static int WfBounceLoop(Ctx* c, const R& Rd, const WfDev& W0, int depth_max, int
     const unsigned pblocks = PersistBlocks(c);
     for (int depth = 0; depth < depth_max; ++depth) {
         TimerBegin(c, kKernTraceClosest);
-        hipLaunchKernelGGL((k_wf_trace<STACK, 0, R>), dim3(pblocks), dim3(256), 0, c->stream, Rd, W, refill > 0 ? refill : kAutoRefill[0], kTraceChunks[0]);
+        hipLaunchKernelGGL((k_wf_trace<STACK, 0, R>), dim3(pblocks), dim3(256), 0, c->stream, Rd, W, refill > 0 ? refill : kAutoRefill, kTraceChunks[0]);
         TimerEnd(c);
         TimerBegin(c, kKernShade);
+        hipLaunchKernelGGL(k_wf_bin, dim3((blocks + kBinPerThread - 1) / kBinPerThread), dim3(256), 0, c->stream, W);
         hipLaunchKernelGGL((k_wf_shade<0, R>), dim3(blocks), dim3(256), 0, c->stream, Rd, W);
         hipLaunchKernelGGL((k_wf_shade<1, R>), dim3(blocks), dim3(256), 0, c->stream, Rd, W);
         hipLaunchKernelGGL((k_wf_shade<2, R>), dim3(blocks), dim3(256), 0, c->stream, Rd, W);
         hipLaunchKernelGGL((k_wf_shade<3, R>), dim3(blocks), dim3(256), 0, c->stream, Rd, W);
         TimerEnd(c);
         TimerBegin(c, kKernTraceShadow);
-        hipLaunchKernelGGL((k_wf_trace<STACK, 1, R>), dim3(pblocks), dim3(256), 0, c->stream, Rd, W, refill > 0 ? refill : kAutoRefill[1], kTraceChunks[1]);
+        hipLaunchKernelGGL((k_wf_trace<STACK, 1, R>), dim3(pblocks), dim3(256), 0, c->stream, Rd, W, refill > 0 ? refill : kAutoRefill, kTraceChunks[1]);
         TimerEnd(c);
         TimerBegin(c, kKernTraceMis);
-        hipLaunchKernelGGL((k_wf_trace<STACK, 2, R>), dim3(pblocks), dim3(256), 0, c->stream, Rd, W, refill > 0 ? refill : kAutoRefill[2], kTraceChunks[2]);
+        hipLaunchKernelGGL((k_wf_trace<STACK, 2, R>), dim3(pblocks), dim3(256), 0, c->stream, Rd, W, refill > 0 ? refill : kAutoRefill, kTraceChunks[2]);
         TimerEnd(c);
         TimerBegin(c, kKernOther);
         hipLaunchKernelGGL((k_wf_resolve<R>), dim3(blocks), dim3(256), 0, c->stream, Rd, W);

@@ -88,9 +88,7 @@ typedef struct {
     int32_t samples_per_pass;/* 0 = automatic */
     int32_t block_size;      /* megakernel block size: 0 = automatic; else 64, 128 or 256 */
     int32_t refill_threshold;/* wavefront traversal: refill rays when fewer lanes of a
-                                wave than this are busy (0 = automatic: 1 for
-                                closest-hit rays, 16 for shadow and MIS rays; 1..64
-                                for all three) */
+                                wave than this are busy (0 = automatic, 48; 1..64) */
     int32_t sampler;         /* GY_SAMPLER_SOBOL: SobolSampler + SobolEnum;
                                 GY_SAMPLER_HALTON: HaltonSampler (Faure) + HaltonEnum */
 } gy_render_params;
