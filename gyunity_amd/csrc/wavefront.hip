@@ -100,20 +100,6 @@ struct ManyRenders {
     __device__ uint32_t pass_slot(const WfDev& W, uint32_t slot) const { return W.pslot[slot]; }
 };
 
-// Queue push compacted per wavefront: one atomic per wave, lanes write at
-// base + their rank among the wave's pushing lanes.  Every lane of the wave
-// must call it (pred false for lanes that push nothing).
-__device__ inline void wave_push(uint32_t* queue, uint32_t* count, uint32_t slot, bool pred) {
-    const uint64_t mask = __ballot(pred);
-    if (mask == 0) return;
-    const int lane = threadIdx.x & 63;
-    const int leader = __ffsll((unsigned long long)mask) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(mask));
-    base = __shfl(base, leader, 64);
-    if (pred) queue[base + __popcll(mask & ((1ull << lane) - 1ull))] = slot;
-}
-
 // NQ pushes for a 256-lane workgroup with one atomic per queue: each wave
 // counts its pushing lanes, one thread per queue claims the workgroup's block,
 // and each lane writes at its rank within the workgroup, so entries keep lane
