@@ -16,6 +16,7 @@ from ._lib import (  # noqa: F401
     camera_matrix,
     generate_scene,
     lib,
+    load_obj,
     render_params,
     sobol_enum_index,
     sobol_sample,

@@ -181,6 +181,10 @@ def lib():
         "gy_generated_desc": (C.c_int, [vp, P(SceneDesc), P(CameraDesc), P(RenderParams)]),
         "gy_generated_free": (C.c_int, [vp]),
         "gy_debug_inject_fault": (C.c_int, [vp, C.c_int]),
+        "gy_mesh_load_obj": (C.c_int, [C.c_char_p, i32, P(f32), P(f32), P(vp)]),
+        "gy_mesh_triangles": (i64, [vp]),
+        "gy_mesh_arrays": (C.c_int, [vp, P(f32), P(f32), P(f32), P(f32)]),
+        "gy_mesh_free": (C.c_int, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -309,6 +313,26 @@ def sobol_sample(index, dim):
 
 def sobol_enum_index(m, frame, px, py):
     return lib().gy_sobol_enum_index(m, frame, px, py)
+
+
+def load_obj(path, reverse=False, object_to_world=None, world_to_object=None):
+    """OBJ file -> dict of per-triangle float32 arrays in the gy_scene_desc
+    layout: pos (n, 9), nrm (n, 9), face_n (n, 3), uv (n, 6).  object_to_world
+    and world_to_object are row-major 4x4 (None: identity, and the inverse)."""
+    L = lib()
+    m = None if object_to_world is None else np.ascontiguousarray(object_to_world, dtype=np.float32).reshape(16)
+    mi = None if world_to_object is None else np.ascontiguousarray(world_to_object, dtype=np.float32).reshape(16)
+    h = C.c_void_p()
+    check(L.gy_mesh_load_obj(os.fsencode(path), int(reverse), None if m is None else _fp(m), None if mi is None else _fp(mi),
+                             C.byref(h)))
+    try:
+        n = L.gy_mesh_triangles(h)
+        out = {"pos": np.zeros((n, 9), np.float32), "nrm": np.zeros((n, 9), np.float32),
+               "face_n": np.zeros((n, 3), np.float32), "uv": np.zeros((n, 6), np.float32)}
+        check(L.gy_mesh_arrays(h, _fp(out["pos"]), _fp(out["nrm"]), _fp(out["face_n"]), _fp(out["uv"])))
+    finally:
+        L.gy_mesh_free(h)
+    return out
 
 
 def camera_matrix(cam):

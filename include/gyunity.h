@@ -222,6 +222,24 @@ int gy_camera_matrix(const gy_camera_desc* cam, float* raster_to_world16);
 float gy_halton_sample(uint64_t index, uint32_t dim);
 uint64_t gy_halton_enum_index(uint32_t width, uint32_t height, uint64_t i, uint32_t x, uint32_t y);
 
+/* ---- OBJ meshes (Mesh::LoadFromFile, mesh/Mesh.cpp:7-84, through
+ *      tinyobjloader 2.0.0 with triangulation; Scene::AddMesh,
+ *      visual/Scene.h:68-77, and Triangle::SetTransform, shape/Triangle.h:78-88)
+ * Loads the triangles a Gyunity scene holds for the file: one vertex triple
+ * per face (in (0, 2, 1) order when reverse_vertices is set), the file's
+ * normals or a generated face normal, uv (0, 0) where absent; points through
+ * object_to_world (16 floats, row-major; NULL = identity), normals through the
+ * transpose of world_to_object (NULL = the inverse of object_to_world) and
+ * normalised; face normal Cross(p1 - p0, p2 - p1), normalised.  The arrays have
+ * the gy_scene_desc layout: pos and nrm 9 floats, face_n 3, uv 6 per triangle.
+ * gy_mesh_arrays copies into the buffers that are not NULL. */
+typedef struct gy_mesh gy_mesh;
+int gy_mesh_load_obj(const char* path, int32_t reverse_vertices, const float* object_to_world,
+                     const float* world_to_object, gy_mesh** out);
+int64_t gy_mesh_triangles(gy_mesh* mesh);
+int gy_mesh_arrays(gy_mesh* mesh, float* pos, float* nrm, float* face_n, float* uv);
+int gy_mesh_free(gy_mesh* mesh);
+
 /* ---- procedural scenes (the benchmark configurations) ---------------------- */
 typedef struct gy_generated gy_generated;
 enum { GY_SCENE_CORNELL = 0, GY_SCENE_ICOSPHERE_GGX = 1, GY_SCENE_BATCH_MEMBER = 2, GY_SCENE_ADJOINT = 3,

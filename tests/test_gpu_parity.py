@@ -133,6 +133,33 @@ def test_level_b_per_sample_ggx(gpu_ctx):
     _li_parity(gpu_ctx, sa, cam, prm, 20000, 2)
 
 
+def test_level_b_obj_mesh_scene(gpu_ctx, tmp_path):
+    # A mesh read through gy_mesh_load_obj (per-corner normals and uvs from the
+    # file, a translation applied on load) inside the Cornell room: per-sample
+    # radiance on identical sample indices equals the oracle's.
+    sa0, cam, prm = g.generate_scene(g.GY_SCENE_ICOSPHERE_GGX, 0, 3)
+    room = 12  # room and light come first
+    P = sa0.pos[room:].reshape(-1, 3) - np.array([0.0, 0.1, 0.0], np.float32)
+    N = sa0.nrm[room:].reshape(-1, 3)
+    lines = ["v %.9g %.9g %.9g" % tuple(p) for p in P] + ["vn %.9g %.9g %.9g" % tuple(n) for n in N]
+    lines += ["vt %.6f %.6f" % (u, v) for u, v in np.random.default_rng(3).uniform(0, 1, (len(P), 2))]
+    lines += ["f %d/%d/%d %d/%d/%d %d/%d/%d" % (3 * t + 1, 3 * t + 1, 3 * t + 1, 3 * t + 2, 3 * t + 2, 3 * t + 2,
+                                               3 * t + 3, 3 * t + 3, 3 * t + 3) for t in range(len(P) // 3)]
+    path = tmp_path / "sphere.obj"
+    path.write_text("\n".join(lines) + "\n")
+    M = np.eye(4, dtype=np.float32)
+    M[1, 3] = 0.1
+    m = g.load_obj(str(path), object_to_world=M)
+    n = m["pos"].shape[0]
+    assert n == sa0.n_tri - room
+    sa = g.SceneArrays(np.concatenate([sa0.pos[:room], m["pos"]]), np.concatenate([sa0.nrm[:room], m["nrm"]]),
+                       np.concatenate([sa0.face_n[:room], m["face_n"]]), np.concatenate([sa0.mat[:room], sa0.mat[room:]]),
+                       np.concatenate([sa0.light[:room], np.full(n, -1, np.int32)]), sa0.mats, sa0.light_tri, sa0.light_le,
+                       uv=np.concatenate([np.zeros((room, 6), np.float32), m["uv"]]))
+    cam.width = cam.height = 128
+    _li_parity(gpu_ctx, sa, cam, prm, 20000, 5)
+
+
 def test_level_b_reference_roulette_rule(gpu_ctx):
     # rr_depth 0 makes Russian roulette consume a dimension on most bounces
     sa, cam, prm = scenes()["cornell"]
@@ -219,6 +246,22 @@ def test_wavefront_bit_identical_to_megakernel(gpu_ctx, name, res, spp):
     assert np.array_equal(ww2, ww) and np.array_equal(wm2, ww2) and np.array_equal(cm2, cw2)
     assert np.allclose(cw2, cw, rtol=1e-5, atol=1e-6)
     assert st.rays > st.paths
+
+
+def test_wavefront_refill_does_not_change_the_film(gpu_ctx):
+    # The refill threshold decides which rays share a wave and when lanes take
+    # new rays; the shade queues are binned in queue order and closest hits do
+    # not depend on visit order, so the film is the same at every threshold.
+    sa, cam, prm = scenes()["ico5"]
+    cam.width = cam.height = 128
+    prm.spp = 8
+    prm.mode = g.GY_RENDER_WAVEFRONT
+    with g.Scene(gpu_ctx, sa) as sc:
+        ref = sc.render(cam, prm)
+        for r in (1, 7, 64):
+            prm.refill_threshold = r
+            c, w = sc.render(cam, prm)
+            assert np.array_equal(c, ref[0]) and np.array_equal(w, ref[1]), r
 
 
 def test_wavefront_depth_edge_cases(gpu_ctx):
