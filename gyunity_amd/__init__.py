@@ -16,6 +16,8 @@ from ._lib import (  # noqa: F401
     camera_matrix,
     generate_scene,
     lib,
+    image_to_bytes,
+    image_write,
     load_obj,
     render_params,
     sobol_enum_index,

@@ -181,6 +181,9 @@ def lib():
         "gy_generated_desc": (C.c_int, [vp, P(SceneDesc), P(CameraDesc), P(RenderParams)]),
         "gy_generated_free": (C.c_int, [vp]),
         "gy_debug_inject_fault": (C.c_int, [vp, C.c_int]),
+        "gy_image_write": (C.c_int, [C.c_char_p, P(f32), i32, i32]),
+        "gy_image_to_bytes": (C.c_int, [P(f32), i64, P(C.c_uint8)]),
+        "gy_film_save": (C.c_int, [vp, C.c_char_p]),
         "gy_mesh_load_obj": (C.c_int, [C.c_char_p, i32, P(f32), P(f32), P(vp)]),
         "gy_mesh_triangles": (i64, [vp]),
         "gy_mesh_arrays": (C.c_int, [vp, P(f32), P(f32), P(f32), P(f32)]),
@@ -335,6 +338,22 @@ def load_obj(path, reverse=False, object_to_world=None, world_to_object=None):
     return out
 
 
+def image_write(path, rgb):
+    """Write an (H, W, 3) float image as .hdr, .bmp or .png (by suffix), the way
+    the reference's Film::SaveImage does."""
+    rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+    h, w = rgb.shape[:2]
+    check(lib().gy_image_write(os.fsencode(path), _fp(rgb), w, h))
+
+
+def image_to_bytes(rgb):
+    """8-bit values of the reference's ImageIO::toInt (gamma 2.2)."""
+    rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+    out = np.zeros(rgb.shape, np.uint8)
+    check(lib().gy_image_to_bytes(_fp(rgb), rgb.size, out.ctypes.data_as(C.POINTER(C.c_uint8))))
+    return out
+
+
 def camera_matrix(cam):
     out = np.zeros(16, dtype=np.float32)
     check(lib().gy_camera_matrix(C.byref(cam), _fp(out)))
@@ -474,6 +493,10 @@ class Scene:
         a, b = C.c_int64(), C.c_int64()
         check(lib().gy_count_visits(self.h, rays.ctypes.data, rays.shape[0], C.byref(a), C.byref(b)))
         return a.value, b.value
+
+    def save_film(self, path):
+        """Film::SaveImage of the last render: .hdr, .bmp or .png by suffix."""
+        check(lib().gy_film_save(self.h, os.fsencode(path)))
 
     def render(self, cam, params, download=True):
         W, H = cam.width, cam.height

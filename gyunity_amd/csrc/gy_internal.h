@@ -78,6 +78,8 @@ struct BVHBuild {
 constexpr uint32_t kEmptyRef = 0xffffffffu;
 // Host binned-SAH build over triangle bounds.
 void BuildBVHHost(const float* pos, int32_t n_tri, int max_leaf, BVHBuild* out);
+// ImageIO::WriteImage: .hdr, .bmp or .png by suffix (image.cpp).
+int WriteImage(const char* path, const float* rgb, int32_t w, int32_t h);
 double BVHSahCost(const BVHBuild& b);
 
 struct Generated;

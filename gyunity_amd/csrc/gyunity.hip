@@ -1021,6 +1021,26 @@ extern "C" int gy_film_download(gy_scene* s, float* color, float* weight) {
     return GY_OK;
 }
 
+// Film::SaveImage (visual/Film.cpp:6-9): pixel = color / weight + splat
+// (visual/Film.h:128-136), where Vec3 / 0 is the zero vector (math/Linagl.h:
+// 366-369) and the splat stays zero for path tracing.
+extern "C" int gy_film_save(gy_scene* s, const char* path) {
+    if (!s || !path) return Fail(GY_ERR_INVALID, "gy_film_save: null argument");
+    Scene* sc = reinterpret_cast<Scene*>(s);
+    if (!sc->film.film) return Fail(GY_ERR_INVALID, "gy_film_save: nothing rendered");
+    const int64_t WH = (int64_t)sc->film.width * sc->film.height;
+    std::vector<float4> h(WH);
+    hipError_t e = hipMemcpy(h.data(), sc->film.film, WH * sizeof(float4), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return HipFail(e, "film D2H");
+    std::vector<float> rgb((size_t)WH * 3);
+    for (int64_t i = 0; i < WH; ++i) {
+        const float w = h[i].w;
+        const float c[3] = {h[i].x, h[i].y, h[i].z};
+        for (int k = 0; k < 3; ++k) rgb[3 * i + k] = (w == 0.f ? 0.f : c[k] / w) + 0.f;
+    }
+    return WriteImage(path, rgb.data(), sc->film.width, sc->film.height);
+}
+
 extern "C" int gy_render_stats_get(gy_context* ctx, gy_render_stats* out) {
     if (!ctx || !out) return Fail(GY_ERR_INVALID, "gy_render_stats_get: null argument");
     *out = reinterpret_cast<Ctx*>(ctx)->stats;

@@ -222,6 +222,17 @@ int gy_camera_matrix(const gy_camera_desc* cam, float* raster_to_world16);
 float gy_halton_sample(uint64_t index, uint32_t dim);
 uint64_t gy_halton_enum_index(uint32_t width, uint32_t height, uint64_t i, uint32_t x, uint32_t y);
 
+/* ---- film output (Film::SaveImage, visual/Film.cpp:6-9; ImageIO::WriteImage,
+ *      image/ImageIO.cpp:120-149) ------------------------------------------
+ * gy_image_write picks the format by suffix: .hdr (Radiance RGBE, run-length
+ * scanlines), .bmp (24-bit) or .png (8-bit RGB); 8-bit values are
+ * ImageIO::toInt, int(pow(clamp(x, 0, 1), 1 / 2.2) * 255 + 0.5).  rgb holds
+ * width*height*3 floats, rows top to bottom.  gy_film_save writes the scene's
+ * last render as color / weight (0 where the weight is 0). */
+int gy_image_write(const char* path, const float* rgb, int32_t width, int32_t height);
+int gy_image_to_bytes(const float* rgb, int64_t n, uint8_t* out);
+int gy_film_save(gy_scene* scene, const char* path);
+
 /* ---- OBJ meshes (Mesh::LoadFromFile, mesh/Mesh.cpp:7-84, through
  *      tinyobjloader 2.0.0 with triangulation; Scene::AddMesh,
  *      visual/Scene.h:68-77, and Triangle::SetTransform, shape/Triangle.h:78-88)

@@ -264,6 +264,24 @@ def test_wavefront_refill_does_not_change_the_film(gpu_ctx):
             assert np.array_equal(c, ref[0]) and np.array_equal(w, ref[1]), r
 
 
+def test_film_save_matches_downloaded_film(gpu_ctx, tmp_path):
+    # Film::SaveImage: the BMP holds toInt(color / weight) of the film, rows
+    # bottom-up in BGR order; the HDR file is written beside it.
+    sa, cam, prm = scenes()["cornell"]
+    cam.width, cam.height = 50, 30
+    prm.spp = 4
+    with g.Scene(gpu_ctx, sa) as sc:
+        color, weight = sc.render(cam, prm)
+        sc.save_film(str(tmp_path / "f.bmp"))
+        sc.save_film(str(tmp_path / "f.hdr"))
+    px = np.where(weight[..., None] == 0, 0, color / np.where(weight == 0, 1, weight)[..., None]).astype(np.float32)
+    data = open(tmp_path / "f.bmp", "rb").read()
+    pad = (-50 * 3) & 3
+    rows = np.frombuffer(data, np.uint8, offset=54).reshape(30, 50 * 3 + pad)[:, :150].reshape(30, 50, 3)
+    assert np.array_equal(rows[::-1, :, ::-1], g.image_to_bytes(px))
+    assert (tmp_path / "f.hdr").read_bytes().startswith(b"#?RADIANCE\n")
+
+
 def test_wavefront_depth_edge_cases(gpu_ctx):
     sa, cam, prm = scenes()["cornell"]
     cam.width = cam.height = 32
