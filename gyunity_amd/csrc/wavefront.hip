@@ -716,8 +716,10 @@ static int WfBounceLoop(Ctx* c, const R& Rd, const WfDev& W0, int depth_max, int
         TimerBegin(c, kKernTraceClosest);
         hipLaunchKernelGGL((k_wf_trace<STACK, 0, R>), dim3(pblocks), dim3(256), 0, c->stream, Rd, W, refill > 0 ? refill : kAutoRefill, kTraceChunks[0]);
         TimerEnd(c);
-        TimerBegin(c, kKernShade);
+        TimerBegin(c, kKernOther);  // queue bookkeeping
         hipLaunchKernelGGL(k_wf_bin, dim3((blocks + kBinPerThread - 1) / kBinPerThread), dim3(256), 0, c->stream, W);
+        TimerEnd(c);
+        TimerBegin(c, kKernShade);
         hipLaunchKernelGGL((k_wf_shade<0, R>), dim3(blocks), dim3(256), 0, c->stream, Rd, W);
         hipLaunchKernelGGL((k_wf_shade<1, R>), dim3(blocks), dim3(256), 0, c->stream, Rd, W);
         hipLaunchKernelGGL((k_wf_shade<2, R>), dim3(blocks), dim3(256), 0, c->stream, Rd, W);
