@@ -436,8 +436,8 @@ __global__ void __launch_bounds__(256) k_wf_shade(R Rd, WfDev W) {
         q.cursor = sq.z;
         const int depth = (int)(sq.w & F_DEPTH_MASK);
         const bool delta = (sq.w & F_DELTA) != 0;
-        float4 Tv = W.T[slot], Lv = W.L[slot];
-        V3 T = f3(Tv), L = f3(Lv);
+        const float4 Tv = W.T[slot];
+        V3 T = f3(Tv);
         const Isect it = query(S, ro, rd, h);
         BSDF b = make_bsdf(S, it);
         b.type = MTYPE;
@@ -445,7 +445,10 @@ __global__ void __launch_bounds__(256) k_wf_shade(R Rd, WfDev W) {
         if ((depth == 0 || delta) && it.light >= 0) {
             const V3 Le = v3(S.light_le[3 * it.light], S.light_le[3 * it.light + 1], S.light_le[3 * it.light + 2]);
             const V3 em = dot(it.n, it.outDir) > 0.f ? Le : v3(0.f, 0.f, 0.f);
-            L = L + T * em;
+            // radiance changes only here: read and written on this branch alone
+            const float4 Lv = W.L[slot];
+            const V3 L = f3(Lv) + T * em;
+            W.L[slot] = make_float4(L.x, L.y, L.z, Lv.w);
         } else {
             const float uL = seq_next(M, q);
             const float u0 = seq_next(M, q), u1 = seq_next(M, q);
@@ -531,8 +534,7 @@ __global__ void __launch_bounds__(256) k_wf_shade(R Rd, WfDev W) {
         }
         if (!alive) nflags |= F_DONE;
         push_next = alive;
-        W.T[slot] = make_float4(T.x, T.y, T.z, Tv.w);
-        W.L[slot] = make_float4(L.x, L.y, L.z, Lv.w);
+        if (alive) W.T[slot] = make_float4(T.x, T.y, T.z, Tv.w);  // a path that ends keeps its T unchanged
         W.seq[slot] = make_uint4(sq.x, sq.y, q.cursor, nflags);
     }
     uint32_t* const qs[3] = {W.q[Q_SHADOW], W.q[Q_MIS], W.q[Q_NEXT]};
