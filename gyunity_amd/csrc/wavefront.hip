@@ -51,6 +51,8 @@ constexpr uint32_t F_DONE = 1u << 9;
 constexpr uint32_t P_HAS_DI = 1u;
 constexpr uint32_t P_OCCLUDED = 2u;
 constexpr uint32_t P_L2 = 4u;
+constexpr uint32_t P_W1_NF = 8u;   // w1 not finite: 0 * w1 is NaN
+constexpr uint32_t P_W2_NF = 16u;  // w2 not finite: 0 * w2 is NaN
 
 struct WfDev {
     float4* o;      // ray origin (xyz)
@@ -60,9 +62,9 @@ struct WfDev {
     uint4* seq;     // sample index lo/hi, cursor, flags
     float4* hit;    // t, prim, b1, b2
     uint8_t* mtype;  // per active-queue position: material type of the hit (0..3), 255 for a miss
-    float4* pend0;  // L1 before the shadow test (xyz), w1
-    float4* pend1;  // BSDF value of the MIS sample (xyz) / L2 after trace_mis, cos
-    float4* pend2;  // pdf of the MIS sample, w2, P(light), light id
+    float4* pend0;  // L1 w1 before the shadow test (xyz), P(light)
+    float4* pend1;  // MIS ray only: BSDF value of the MIS sample (xyz) / L2 w2 after trace_mis, cos
+    float4* pend2;  // MIS ray only: pdf of the MIS sample, w2, -, light id
     float4* pend3;  // throughput at this bounce (xyz), pending flags
     float4* sh_o;   // shadow ray origin, tmax
     float4* sh_d;   // shadow ray direction
@@ -343,7 +345,8 @@ __global__ void __launch_bounds__(256) k_wf_trace(R Rd, WfDev W, int refill, int
                             const V3 Lem = dot(li.n, neg(t.d)) > 0.f ? Le : v3(0.f, 0.f, 0.f);
                             const float4 p1 = W.pend1[slot];
                             const V3 L2 = Lem * f3(p1) * v3s(1.f) * p1.w / p2.x;
-                            W.pend1[slot] = make_float4(L2.x, L2.y, L2.z, p1.w);
+                            const V3 L2w2 = L2 * p2.y;
+                            W.pend1[slot] = make_float4(L2w2.x, L2w2.y, L2w2.z, p1.w);
                             float4 p3 = W.pend3[slot];
                             p3.w = __uint_as_float(__float_as_uint(p3.w) | P_L2);
                             W.pend3[slot] = p3;
@@ -498,10 +501,19 @@ __global__ void __launch_bounds__(256) k_wf_shade(R Rd, WfDev W) {
                     push_mis = true;
                 }
             }
-            W.pend0[slot] = make_float4(L1pre.x, L1pre.y, L1pre.z, w1);
-            W.pend1[slot] = make_float4(f2.x, f2.y, f2.z, c2);
-            W.pend2[slot] = make_float4(pdf2, w2, lsp, __int_as_float(light));
-            pflags = P_HAS_DI;
+            // L1 w1 here, L2 w2 in the MIS trace: the products (L1 w1 + L2 w2)
+            // of DirectIllumination are formed in the same FP32 operations,
+            // and a zero L1 (occluded) or L2 (no MIS ray) gives +0 for them, as
+            // 0 * w does for a finite weight (NaN for an infinite or NaN one,
+            // flagged below).  The MIS records are written only for a path
+            // that has an MIS ray.
+            const V3 L1w1 = L1pre * w1;
+            W.pend0[slot] = make_float4(L1w1.x, L1w1.y, L1w1.z, lsp);
+            if (push_mis) {
+                W.pend1[slot] = make_float4(f2.x, f2.y, f2.z, c2);
+                W.pend2[slot] = make_float4(pdf2, w2, lsp, __int_as_float(light));
+            }
+            pflags = P_HAS_DI | (isfinite(w1) ? 0u : P_W1_NF) | (isfinite(w2) ? 0u : P_W2_NF);
         }
         W.pend3[slot] = make_float4(T.x, T.y, T.z, __uint_as_float(pflags));
         // BSDF continuation and Russian roulette (integrator/PT.cpp:45-62)
@@ -556,10 +568,11 @@ __global__ void __launch_bounds__(256) k_wf_resolve(R Rd, WfDev W) {
     const float4 p3 = W.pend3[slot];
     const uint32_t pf = __float_as_uint(p3.w);
     if (pf & P_HAS_DI) {
-        const float4 p0 = W.pend0[slot], p1 = W.pend1[slot], p2 = W.pend2[slot];
-        const V3 L1 = (pf & P_OCCLUDED) ? v3s(0.f) : f3(p0);
-        const V3 L2 = (pf & P_L2) ? f3(p1) : v3(0.f, 0.f, 0.f);
-        const V3 DI = (L1 * p0.w + L2 * p2.y) / p2.z;
+        const float4 p0 = W.pend0[slot];
+        const float nan = __builtin_nanf("");
+        const V3 L1w1 = (pf & P_OCCLUDED) ? v3s((pf & P_W1_NF) ? nan : 0.f) : f3(p0);
+        const V3 L2w2 = (pf & P_L2) ? f3(W.pend1[slot]) : v3s((pf & P_W2_NF) ? nan : 0.f);
+        const V3 DI = (L1w1 + L2w2) / p0.w;
         const V3 L = f3(Lv) + f3(p3) * DI;
         Lv = make_float4(L.x, L.y, L.z, Lv.w);
         W.L[slot] = Lv;
