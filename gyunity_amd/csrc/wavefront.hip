@@ -46,13 +46,14 @@ constexpr int kCtrStride = 32;
 // Path flags (PathState.seq.w)
 constexpr uint32_t F_DEPTH_MASK = 0xffu;
 constexpr uint32_t F_DELTA = 1u << 8;
-constexpr uint32_t F_DONE = 1u << 9;
-// Pending direct-light flags (pend3.w as bits)
+// Resolve flags (pend3.w as bits): pending direct light of this bounce, and the
+// path's end.  Zero for a slot with nothing to resolve.
 constexpr uint32_t P_HAS_DI = 1u;
 constexpr uint32_t P_OCCLUDED = 2u;
 constexpr uint32_t P_L2 = 4u;
 constexpr uint32_t P_W1_NF = 8u;   // w1 not finite: 0 * w1 is NaN
 constexpr uint32_t P_W2_NF = 16u;  // w2 not finite: 0 * w2 is NaN
+constexpr uint32_t P_DONE = 32u;   // the path ended this bounce: it goes to the film
 
 struct WfDev {
     float4* o;      // ray origin (xyz)
@@ -65,7 +66,7 @@ struct WfDev {
     float4* pend0;  // L1 w1 before the shadow test (xyz), P(light)
     float4* pend1;  // MIS ray only: BSDF value of the MIS sample (xyz) / L2 w2 after trace_mis, cos
     float4* pend2;  // MIS ray only: pdf of the MIS sample, w2, -, light id
-    float4* pend3;  // throughput at this bounce (xyz), pending flags
+    float4* pend3;  // throughput at this bounce (xyz), resolve flags (P_*)
     float4* sh_o;   // shadow ray origin, tmax
     float4* sh_d;   // shadow ray direction
     float4* mis_o;  // MIS ray origin
@@ -151,8 +152,8 @@ __device__ inline void start_path(const WfDev& W, const PathParams& P, uint32_t 
     W.d[slot] = make_float4(d.x, d.y, d.z, 0.f);
     W.T[slot] = make_float4(1.f, 1.f, 1.f, px + u);
     W.L[slot] = make_float4(0.f, 0.f, 0.f, py + v);
-    W.seq[slot] = make_uint4((uint32_t)q.index, (uint32_t)(q.index >> 32), q.cursor, P.max_depth == 0 ? F_DONE : 0u);
-    W.pend3[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+    W.seq[slot] = make_uint4((uint32_t)q.index, (uint32_t)(q.index >> 32), q.cursor, 0u);
+    W.pend3[slot] = make_float4(0.f, 0.f, 0.f, __uint_as_float(P.max_depth == 0 ? P_DONE : 0u));
 }
 
 // ---------------------------------------------------------------- raygen
@@ -183,7 +184,7 @@ __global__ void __launch_bounds__(256) k_wf_raygen(WfDev W, PathParams P, int s0
 // Batch: lanes [0, count) take global path indices base + lane.  Entry e owns
 // global indices [E[e].first, E[e].first + E[e].paths) in its own tile order
 // (sample-major, 8x8 pixel tiles); the lane records its entry and pass slot.
-__global__ void __launch_bounds__(256) k_wf_raygen_batch(WfDev W, ManyRenders R, int64_t base, int64_t count) {
+__global__ void __launch_bounds__(256) k_wf_raygen_batch(WfDev W, ManyRenders R, int64_t base, int64_t count, int64_t lanes) {
     const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
     bool valid = t < count;
     uint32_t e = 0;
@@ -214,6 +215,8 @@ __global__ void __launch_bounds__(256) k_wf_raygen_batch(WfDev W, ManyRenders R,
         W.ent[slot] = e;
         W.pslot[slot] = (uint32_t)((int64_t)s * P.cam.width * P.cam.height + (int64_t)py * P.cam.width + px);
         start_path(W, P, slot, s, px, py);
+    } else if (t < lanes) {
+        W.pend3[slot].w = __uint_as_float(0u);  // resolve walks every lane: one without a path has no flags
     }
     uint32_t* const qa[1] = {W.q[Q_ACTIVE]};
     uint32_t* const ca[1] = {&W.counts[Q_ACTIVE * kCtrStride]};
@@ -318,15 +321,12 @@ __global__ void __launch_bounds__(256) k_wf_trace(R Rd, WfDev W, int refill, int
                             W.mtype[my] = (uint8_t)Sf.mats[(size_t)mat * 32];
                         } else {
                             W.mtype[my] = 255;
-                            uint4 sq = W.seq[slot];
-                            sq.w |= F_DONE;
-                            W.seq[slot] = sq;
+                            W.pend3[slot].w = __uint_as_float(P_DONE);  // no environment light: the path ends
                         }
                     } else if (MODE == 1) {
                         if (r == 2) {
-                            float4 p3 = W.pend3[slot];
-                            p3.w = __uint_as_float(__float_as_uint(p3.w) | P_OCCLUDED);
-                            W.pend3[slot] = p3;
+                            float& fw = W.pend3[slot].w;
+                            fw = __uint_as_float(__float_as_uint(fw) | P_OCCLUDED);
                         }
                     } else if (t.best_prim >= 0) {
                         // MIS branch of DirectIllumination (integrator/Integrator.cpp:88-98)
@@ -347,9 +347,8 @@ __global__ void __launch_bounds__(256) k_wf_trace(R Rd, WfDev W, int refill, int
                             const V3 L2 = Lem * f3(p1) * v3s(1.f) * p1.w / p2.x;
                             const V3 L2w2 = L2 * p2.y;
                             W.pend1[slot] = make_float4(L2w2.x, L2w2.y, L2w2.z, p1.w);
-                            float4 p3 = W.pend3[slot];
-                            p3.w = __uint_as_float(__float_as_uint(p3.w) | P_L2);
-                            W.pend3[slot] = p3;
+                            float& fw = W.pend3[slot].w;
+                            fw = __uint_as_float(__float_as_uint(fw) | P_L2);
                         }
                     }
                     my = -1;
@@ -514,8 +513,8 @@ __global__ void __launch_bounds__(256) k_wf_shade(R Rd, WfDev W) {
                 W.pend2[slot] = make_float4(pdf2, w2, lsp, __int_as_float(light));
             }
             pflags = P_HAS_DI | (isfinite(w1) ? 0u : P_W1_NF) | (isfinite(w2) ? 0u : P_W2_NF);
+            W.pend3[slot] = make_float4(T.x, T.y, T.z, __uint_as_float(pflags));
         }
-        W.pend3[slot] = make_float4(T.x, T.y, T.z, __uint_as_float(pflags));
         // BSDF continuation and Russian roulette (integrator/PT.cpp:45-62)
         bool alive = depth + 1 < P.max_depth;
         uint32_t nflags = sq.w;
@@ -544,10 +543,15 @@ __global__ void __launch_bounds__(256) k_wf_shade(R Rd, WfDev W) {
                 }
             }
         }
-        if (!alive) nflags |= F_DONE;
         push_next = alive;
-        if (alive) W.T[slot] = make_float4(T.x, T.y, T.z, Tv.w);  // a path that ends keeps its T unchanged
-        W.seq[slot] = make_uint4(sq.x, sq.y, q.cursor, nflags);
+        if (alive) {  // a path that ends keeps T unchanged and needs no sample state
+            W.T[slot] = make_float4(T.x, T.y, T.z, Tv.w);
+            W.seq[slot] = make_uint4(sq.x, sq.y, q.cursor, nflags);
+        }
+        // the flags word: written with the pending record above for a path that
+        // goes on; otherwise here (a path that ends, or one without direct light
+        // whose flags still hold the last bounce's)
+        if (!alive || !(pflags & P_HAS_DI)) W.pend3[slot].w = __uint_as_float(pflags | (alive ? 0u : P_DONE));
     }
     uint32_t* const qs[3] = {W.q[Q_SHADOW], W.q[Q_MIS], W.q[Q_NEXT]};
     uint32_t* const cs[3] = {&W.counts[Q_SHADOW * kCtrStride], &W.counts[Q_MIS * kCtrStride], &W.counts[Q_NEXT * kCtrStride]};
@@ -557,16 +561,19 @@ __global__ void __launch_bounds__(256) k_wf_shade(R Rd, WfDev W) {
 
 // ---------------------------------------------------------------- resolve
 // L += T_i * (L1 w1 + L2 w2) / P(light) (integrator/Integrator.cpp:106,
-// integrator/PT.cpp:38), then finished paths go to the film.
+// integrator/PT.cpp:38), then finished paths go to the film.  The kernel walks
+// the slots [0, n) in order, so its loads are coalesced whatever order the
+// queues left the paths in; a slot with no flags costs one 16-byte load.  A path
+// that goes on keeps its flags, which the next bounce rewrites (shading, or the
+// closest-hit miss) before they are read again.
 template <class R>
-__global__ void __launch_bounds__(256) k_wf_resolve(R Rd, WfDev W) {
-    const uint32_t n = W.counts[(Q_ACTIVE) * kCtrStride];
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t slot = W.q[Q_ACTIVE][i];
-    float4 Lv = W.L[slot];
+__global__ void __launch_bounds__(256) k_wf_resolve(R Rd, WfDev W, uint32_t n) {
+    const uint32_t slot = blockIdx.x * 256 + threadIdx.x;
+    if (slot >= n) return;
     const float4 p3 = W.pend3[slot];
     const uint32_t pf = __float_as_uint(p3.w);
+    if (pf == 0u) return;
+    float4 Lv = W.L[slot];
     if (pf & P_HAS_DI) {
         const float4 p0 = W.pend0[slot];
         const float nan = __builtin_nanf("");
@@ -575,11 +582,14 @@ __global__ void __launch_bounds__(256) k_wf_resolve(R Rd, WfDev W) {
         const V3 DI = (L1w1 + L2w2) / p0.w;
         const V3 L = f3(Lv) + f3(p3) * DI;
         Lv = make_float4(L.x, L.y, L.z, Lv.w);
-        W.L[slot] = Lv;
-        W.pend3[slot] = make_float4(p3.x, p3.y, p3.z, 0.f);
+        if (!(pf & P_DONE)) {
+            W.L[slot] = Lv;
+            return;
+        }
+    } else if (!(pf & P_DONE)) {
+        return;
     }
-    const uint4 sq = W.seq[slot];
-    if (!(sq.w & F_DONE)) return;
+    W.pend3[slot].w = __uint_as_float(0u);  // resolved: later bounces skip the slot
     // Film::AddSample (visual/Film.h:61-87) through the pass buffer / side list.
     const FilmDev& F = Rd.film(Rd.entry(W, slot));
     const uint32_t ps = Rd.pass_slot(W, slot);
@@ -723,7 +733,7 @@ static int WfPrepareState(Ctx* c, WavefrontState& wf, int64_t n, bool batch) {
 constexpr int kTraceChunks[3] = {8, 4, 1};
 constexpr int kAutoRefill = 48;
 template <int STACK, class R>
-static int WfBounceLoop(Ctx* c, const R& Rd, const WfDev& W0, int depth_max, int refill, unsigned blocks,
+static int WfBounceLoop(Ctx* c, const R& Rd, const WfDev& W0, int depth_max, int refill, unsigned blocks, uint32_t n_slots,
                         unsigned long long* counters) {
     WfDev W = W0;
     const unsigned pblocks = PersistBlocks(c);
@@ -747,7 +757,7 @@ static int WfBounceLoop(Ctx* c, const R& Rd, const WfDev& W0, int depth_max, int
         hipLaunchKernelGGL((k_wf_trace<STACK, 2, R>), dim3(pblocks), dim3(256), 0, c->stream, Rd, W, refill > 0 ? refill : kAutoRefill, kTraceChunks[2]);
         TimerEnd(c);
         TimerBegin(c, kKernOther);
-        hipLaunchKernelGGL((k_wf_resolve<R>), dim3(blocks), dim3(256), 0, c->stream, Rd, W);
+        hipLaunchKernelGGL((k_wf_resolve<R>), dim3(blocks), dim3(256), 0, c->stream, Rd, W, n_slots);
         hipLaunchKernelGGL(k_wf_advance, dim3(1), dim3(64), 0, c->stream, W, counters);
         TimerEnd(c);
         int rc = CheckLaunch(c, "wavefront bounce");
@@ -788,11 +798,11 @@ int WavefrontPass(Ctx* c, Scene* sc, const PathParams& P, const gy_render_params
     const unsigned blocks = (unsigned)((cap + 255) / 256);
     if (p.max_depth == 0) {
         // no bounce: every path finishes with L = 0
-        hipLaunchKernelGGL((k_wf_resolve<OneRender>), dim3(blocks), dim3(256), 0, c->stream, Rd, W);
+        hipLaunchKernelGGL((k_wf_resolve<OneRender>), dim3(blocks), dim3(256), 0, c->stream, Rd, W, (uint32_t)cap);
         return CheckLaunch(c, "k_wf_resolve");
     }
     if (sc->stats.max_depth + 1 > 64) return Fail(GY_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
-    return WfBounceLoop<kStackLds>(c, Rd, W, p.max_depth, RefillOf(p.refill_threshold), blocks, sc->film.counters);
+    return WfBounceLoop<kStackLds>(c, Rd, W, p.max_depth, RefillOf(p.refill_threshold), blocks, (uint32_t)cap, sc->film.counters);
 }
 
 // ---------------------------------------------------------------- batch
@@ -821,10 +831,10 @@ static int BatchSequence(Ctx* c, Batch* b) {
         const int64_t count = std::min(lanes, b->total_paths - base);
         hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(64), 0, c->stream, W);
         TimerBegin(c, kKernRaygen);
-        hipLaunchKernelGGL(k_wf_raygen_batch, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, c->stream, W, Rd, base, count);
+        hipLaunchKernelGGL(k_wf_raygen_batch, dim3(blocks), dim3(256), 0, c->stream, W, Rd, base, count, lanes);
         TimerEnd(c);
         rc = CheckLaunch(c, "k_wf_raygen_batch");
-        if (rc == GY_OK) rc = WfBounceLoop<kStackLds>(c, Rd, W, b->depth_max, b->refill, blocks, b->counters);
+        if (rc == GY_OK) rc = WfBounceLoop<kStackLds>(c, Rd, W, b->depth_max, b->refill, blocks, (uint32_t)lanes, b->counters);
     }
     for (auto& en : b->entries)
         if (rc == GY_OK) rc = FilmResolveState(c, en.film, en.params.spp);
