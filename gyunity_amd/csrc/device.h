@@ -80,6 +80,20 @@ GY_D float sobol_sample(const uint32_t* __restrict__ T, uint64_t index, uint32_t
     for (; index; index >>= 8, t += 256) r ^= t[index & 255u];
     return (float)r * 2.3283064365386963e-10f;
 }
+// The same sum with the four low index bytes looked up at once: entry 0 of a
+// byte table is 0, so a zero byte adds nothing, and the four loads are
+// independent.  One memory round trip per sample for indices below 2^32 in
+// place of one per byte, for kernels whose samples sit on the critical path.
+GY_D float sobol_sample_wide(const uint32_t* __restrict__ T, uint64_t index, uint32_t dim) {
+    const uint32_t* t = T + (size_t)dim * (7 * 256);
+    const uint32_t lo = (uint32_t)index;
+    uint32_t r = (t[lo & 255u] ^ t[256 + ((lo >> 8) & 255u)]) ^ (t[512 + ((lo >> 16) & 255u)] ^ t[768 + (lo >> 24)]);
+    uint32_t hi = (uint32_t)(index >> 32);
+    t += 1024;
+#pragma unroll 1
+    for (; hi; hi >>= 8, t += 256) r ^= t[hi & 255u];
+    return (float)r * 2.3283064365386963e-10f;
+}
 // HaltonSampler::sample with Faure permutations (sampler/Halton.h:648-910,
 // 1441-3308): base 2 writes the bit-reversed index into a float mantissa;
 // other bases sum digit-table entries over groups of k digits (host tables,
@@ -120,6 +134,12 @@ struct Seq {
 // RandomStateSequence::Sample (sampler/Sampler.h:53-64): values >= 1 wrap to 0.
 GY_D float seq_next(const SampleSrc& M, Seq& s) {
     float v = M.kind == 1 ? halton_sample(M, s.index, s.cursor) : sobol_sample(M.sobol, s.index, s.cursor);
+    ++s.cursor;
+    if (v >= 1.f) v = 0.f;
+    return v;
+}
+GY_D float seq_next_wide(const SampleSrc& M, Seq& s) {
+    float v = M.kind == 1 ? halton_sample(M, s.index, s.cursor) : sobol_sample_wide(M.sobol, s.index, s.cursor);
     ++s.cursor;
     if (v >= 1.f) v = 0.f;
     return v;

@@ -54,6 +54,8 @@ constexpr uint32_t P_L2 = 4u;
 constexpr uint32_t P_W1_NF = 8u;   // w1 not finite: 0 * w1 is NaN
 constexpr uint32_t P_W2_NF = 16u;  // w2 not finite: 0 * w2 is NaN
 constexpr uint32_t P_DONE = 32u;   // the path ended this bounce: it goes to the film
+constexpr uint32_t P_NEXT = 64u;   // the path goes on without pending direct light
+// A path goes on after this bounce iff its flags are non-zero without P_DONE.
 
 struct WfDev {
     float4* o;      // ray origin (xyz)
@@ -136,6 +138,37 @@ __device__ inline void block_push(uint32_t* const (&q)[NQ], uint32_t* const (&cn
     }
 }
 
+// One queue, K items per thread, one atomic per workgroup.  Item k of thread t
+// ranks at (k, t): entries keep the order of k * 256 + t.
+template <int K>
+__device__ inline void block_push_items(uint32_t* q, uint32_t* cnt, const uint32_t (&item)[K], const bool (&pred)[K]) {
+    __shared__ uint32_t nw[K][4];
+    __shared__ uint32_t base;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint64_t mask[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        mask[k] = __ballot(pred[k]);
+        if (lane == 0) nw[k][wave] = (uint32_t)__popcll(mask[k]);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t total = 0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) total += nw[k][0] + nw[k][1] + nw[k][2] + nw[k][3];
+        base = total ? atomicAdd(cnt, total) : 0u;
+    }
+    __syncthreads();
+    uint32_t off = base;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        uint32_t o = off;
+        for (int w = 0; w < wave; ++w) o += nw[k][w];
+        if (pred[k]) q[o + __popcll(mask[k] & ((1ull << lane) - 1ull))] = item[k];
+        off += nw[k][0] + nw[k][1] + nw[k][2] + nw[k][3];
+    }
+}
+
 __device__ inline V3 f3(const float4 v) { return v3(v.x, v.y, v.z); }
 
 // Camera sample of one path (TiledIntegrator::Render per sample,
@@ -144,8 +177,8 @@ __device__ inline void start_path(const WfDev& W, const PathParams& P, uint32_t 
     Seq q;
     q.index = enum_index(P.en, (uint32_t)s, (uint32_t)px, (uint32_t)py);
     q.cursor = 0;
-    const float u = P.en.sx * seq_next(P.smp, q) - (float)px;
-    const float v = P.en.sy * seq_next(P.smp, q) - (float)py;
+    const float u = P.en.sx * seq_next_wide(P.smp, q) - (float)px;
+    const float v = P.en.sy * seq_next_wide(P.smp, q) - (float)py;
     V3 o, d;
     generate_ray(P.cam, px, py, u, v, &o, &d);
     W.o[slot] = make_float4(o.x, o.y, o.z, 0.f);
@@ -418,7 +451,7 @@ __global__ void __launch_bounds__(256) k_wf_shade(R Rd, WfDev W) {
     if (blockIdx.x * 256 >= n) return;
     const bool valid = i < n;
     uint32_t slot = 0;
-    bool push_shadow = false, push_mis = false, push_next = false;
+    bool push_shadow = false, push_mis = false;
     if (valid) {
         slot = W.q[Q_SHADE0 + MTYPE][i];
         const uint32_t e = Rd.entry(W, slot);
@@ -452,9 +485,9 @@ __global__ void __launch_bounds__(256) k_wf_shade(R Rd, WfDev W) {
             const V3 L = f3(Lv) + T * em;
             W.L[slot] = make_float4(L.x, L.y, L.z, Lv.w);
         } else {
-            const float uL = seq_next(M, q);
-            const float u0 = seq_next(M, q), u1 = seq_next(M, q);
-            const float v0 = seq_next(M, q), v1 = seq_next(M, q), v2 = seq_next(M, q);
+            const float uL = seq_next_wide(M, q);
+            const float u0 = seq_next_wide(M, q), u1 = seq_next_wide(M, q);
+            const float v0 = seq_next_wide(M, q), v1 = seq_next_wide(M, q), v2 = seq_next_wide(M, q);
             // DirectIllumination set-up (integrator/Integrator.cpp:9-88)
             float lsp;
             const int light = sample_one_light(S, uL, &lsp);
@@ -519,7 +552,7 @@ __global__ void __launch_bounds__(256) k_wf_shade(R Rd, WfDev W) {
         bool alive = depth + 1 < P.max_depth;
         uint32_t nflags = sq.w;
         if (alive) {
-            const float r0 = seq_next(M, q), r1 = seq_next(M, q), r2 = seq_next(M, q);
+            const float r0 = seq_next_wide(M, q), r1 = seq_next_wide(M, q), r2 = seq_next_wide(M, q);
             V3 wi;
             float pdf;
             const V3 f = bsdf_sample(b, neg(rd), &wi, &pdf, r0, r1, r2);
@@ -530,7 +563,7 @@ __global__ void __launch_bounds__(256) k_wf_shade(R Rd, WfDev W) {
                 const bool ndelta = bsdf_delta(b);
                 const float p = rmin(1.0f, lumY(est * T) / lumY(T));
                 if (p < 1.f && depth > P.rr_depth) {
-                    if (seq_next(M, q) < p) T = T / p;
+                    if (seq_next_wide(M, q) < p) T = T / p;
                     else alive = false;
                 }
                 if (alive) {
@@ -543,7 +576,6 @@ __global__ void __launch_bounds__(256) k_wf_shade(R Rd, WfDev W) {
                 }
             }
         }
-        push_next = alive;
         if (alive) {  // a path that ends keeps T unchanged and needs no sample state
             W.T[slot] = make_float4(T.x, T.y, T.z, Tv.w);
             W.seq[slot] = make_uint4(sq.x, sq.y, q.cursor, nflags);
@@ -551,28 +583,25 @@ __global__ void __launch_bounds__(256) k_wf_shade(R Rd, WfDev W) {
         // the flags word: written with the pending record above for a path that
         // goes on; otherwise here (a path that ends, or one without direct light
         // whose flags still hold the last bounce's)
-        if (!alive || !(pflags & P_HAS_DI)) W.pend3[slot].w = __uint_as_float(pflags | (alive ? 0u : P_DONE));
+        if (!alive || !(pflags & P_HAS_DI)) W.pend3[slot].w = __uint_as_float(alive ? P_NEXT : (pflags | P_DONE));
     }
-    uint32_t* const qs[3] = {W.q[Q_SHADOW], W.q[Q_MIS], W.q[Q_NEXT]};
-    uint32_t* const cs[3] = {&W.counts[Q_SHADOW * kCtrStride], &W.counts[Q_MIS * kCtrStride], &W.counts[Q_NEXT * kCtrStride]};
-    const bool ps[3] = {push_shadow, push_mis, push_next};
-    block_push<3>(qs, cs, slot, ps);
+    uint32_t* const qs[2] = {W.q[Q_SHADOW], W.q[Q_MIS]};
+    uint32_t* const cs[2] = {&W.counts[Q_SHADOW * kCtrStride], &W.counts[Q_MIS * kCtrStride]};
+    const bool ps[2] = {push_shadow, push_mis};
+    block_push<2>(qs, cs, slot, ps);
 }
 
 // ---------------------------------------------------------------- resolve
 // L += T_i * (L1 w1 + L2 w2) / P(light) (integrator/Integrator.cpp:106,
-// integrator/PT.cpp:38), then finished paths go to the film.  The kernel walks
-// the slots [0, n) in order, so its loads are coalesced whatever order the
-// queues left the paths in; a slot with no flags costs one 16-byte load.  A path
-// that goes on keeps its flags, which the next bounce rewrites (shading, or the
-// closest-hit miss) before they are read again.
+// integrator/PT.cpp:38); finished paths go to the film, and paths that go on
+// form the next bounce's active queue.  Returns whether the path goes on.
 template <class R>
-__global__ void __launch_bounds__(256) k_wf_resolve(R Rd, WfDev W, uint32_t n) {
-    const uint32_t slot = blockIdx.x * 256 + threadIdx.x;
-    if (slot >= n) return;
+__device__ inline bool resolve_slot(const R& Rd, const WfDev& W, uint32_t slot) {
     const float4 p3 = W.pend3[slot];
     const uint32_t pf = __float_as_uint(p3.w);
-    if (pf == 0u) return;
+    if (pf == 0u) return false;  // no path in the slot
+    const bool done = (pf & P_DONE) != 0u;
+    if (!(pf & P_HAS_DI) && !done) return true;
     float4 Lv = W.L[slot];
     if (pf & P_HAS_DI) {
         const float4 p0 = W.pend0[slot];
@@ -582,12 +611,10 @@ __global__ void __launch_bounds__(256) k_wf_resolve(R Rd, WfDev W, uint32_t n) {
         const V3 DI = (L1w1 + L2w2) / p0.w;
         const V3 L = f3(Lv) + f3(p3) * DI;
         Lv = make_float4(L.x, L.y, L.z, Lv.w);
-        if (!(pf & P_DONE)) {
+        if (!done) {
             W.L[slot] = Lv;
-            return;
+            return true;
         }
-    } else if (!(pf & P_DONE)) {
-        return;
     }
     W.pend3[slot].w = __uint_as_float(0u);  // resolved: later bounces skip the slot
     // Film::AddSample (visual/Film.h:61-87) through the pass buffer / side list.
@@ -608,7 +635,7 @@ __global__ void __launch_bounds__(256) k_wf_resolve(R Rd, WfDev W, uint32_t n) {
     y1 = y1 > H - 1 ? H - 1 : y1;
     const bool own = (x0 == px && x1 == px && y0 == py && y1 == py);
     F.pass[ps] = make_float4(Lv.x, Lv.y, Lv.z, own ? 1.f : 0.f);
-    if (own) return;
+    if (own) return false;
     for (int yy = y0; yy <= y1; ++yy)
         for (int xx = x0; xx <= x1; ++xx) {
             const uint32_t k = atomicAdd(F.side_count, 1u);
@@ -619,6 +646,27 @@ __global__ void __launch_bounds__(256) k_wf_resolve(R Rd, WfDev W, uint32_t n) {
                 F.side[k] = se;
             }
         }
+    return false;
+}
+
+// The kernel walks the slots [0, n) in order, kResolvePerThread per thread
+// (slot = (block * K + k) * 256 + thread), so its loads are coalesced whatever
+// order the queues left the paths in, and the next active queue comes out in
+// slot order: shading, which works through queues, then reads and writes the
+// path records of neighbouring slots together.  A slot with no flags costs one
+// 16-byte load.  A path that goes on keeps its flags, which the next bounce
+// rewrites (shading, or the closest-hit miss) before they are read again.
+constexpr int kResolvePerThread = 4;
+template <class R>
+__global__ void __launch_bounds__(256) k_wf_resolve(R Rd, WfDev W, uint32_t n) {
+    uint32_t slot[kResolvePerThread];
+    bool next[kResolvePerThread];
+#pragma unroll
+    for (int k = 0; k < kResolvePerThread; ++k) {
+        slot[k] = (blockIdx.x * kResolvePerThread + k) * 256 + threadIdx.x;
+        next[k] = slot[k] < n && resolve_slot(Rd, W, slot[k]);
+    }
+    block_push_items<kResolvePerThread>(W.q[Q_NEXT], &W.counts[Q_NEXT * kCtrStride], slot, next);
 }
 
 // Queue bookkeeping between bounces: next -> active, counters reset.
@@ -730,6 +778,7 @@ static int WfPrepareState(Ctx* c, WavefrontState& wf, int64_t n, bool batch) {
 // from four, MIS rays from one.  Automatic refill: a lane takes a new ray once
 // fewer than 48 lanes of its wave are still traversing.  Refilling does not
 // reorder the shade queues, which k_wf_bin builds in queue order.
+static unsigned ResolveBlocks(uint32_t n) { return (unsigned)(((uint64_t)n + 256 * kResolvePerThread - 1) / (256 * kResolvePerThread)); }
 constexpr int kTraceChunks[3] = {8, 4, 1};
 constexpr int kAutoRefill = 48;
 template <int STACK, class R>
@@ -757,7 +806,7 @@ static int WfBounceLoop(Ctx* c, const R& Rd, const WfDev& W0, int depth_max, int
         hipLaunchKernelGGL((k_wf_trace<STACK, 2, R>), dim3(pblocks), dim3(256), 0, c->stream, Rd, W, refill > 0 ? refill : kAutoRefill, kTraceChunks[2]);
         TimerEnd(c);
         TimerBegin(c, kKernOther);
-        hipLaunchKernelGGL((k_wf_resolve<R>), dim3(blocks), dim3(256), 0, c->stream, Rd, W, n_slots);
+        hipLaunchKernelGGL((k_wf_resolve<R>), dim3(ResolveBlocks(n_slots)), dim3(256), 0, c->stream, Rd, W, n_slots);
         hipLaunchKernelGGL(k_wf_advance, dim3(1), dim3(64), 0, c->stream, W, counters);
         TimerEnd(c);
         int rc = CheckLaunch(c, "wavefront bounce");
@@ -798,7 +847,7 @@ int WavefrontPass(Ctx* c, Scene* sc, const PathParams& P, const gy_render_params
     const unsigned blocks = (unsigned)((cap + 255) / 256);
     if (p.max_depth == 0) {
         // no bounce: every path finishes with L = 0
-        hipLaunchKernelGGL((k_wf_resolve<OneRender>), dim3(blocks), dim3(256), 0, c->stream, Rd, W, (uint32_t)cap);
+        hipLaunchKernelGGL((k_wf_resolve<OneRender>), dim3(ResolveBlocks((uint32_t)cap)), dim3(256), 0, c->stream, Rd, W, (uint32_t)cap);
         return CheckLaunch(c, "k_wf_resolve");
     }
     if (sc->stats.max_depth + 1 > 64) return Fail(GY_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
