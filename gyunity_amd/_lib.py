@@ -184,6 +184,7 @@ def lib():
         "gy_image_write": (C.c_int, [C.c_char_p, P(f32), i32, i32]),
         "gy_image_to_bytes": (C.c_int, [P(f32), i64, P(C.c_uint8)]),
         "gy_film_save": (C.c_int, [vp, C.c_char_p]),
+        "gy_scene_set_environment": (C.c_int, [vp, vp, C.c_int32, C.c_int32, C.c_float]),
         "gy_mesh_load_obj": (C.c_int, [C.c_char_p, i32, P(f32), P(f32), P(vp)]),
         "gy_mesh_triangles": (i64, [vp]),
         "gy_mesh_arrays": (C.c_int, [vp, P(f32), P(f32), P(f32), P(f32)]),
@@ -493,6 +494,14 @@ class Scene:
         a, b = C.c_int64(), C.c_int64()
         check(lib().gy_count_visits(self.h, rays.ctypes.data, rays.shape[0], C.byref(a), C.byref(b)))
         return a.value, b.value
+
+    def set_environment(self, rgb, scale=1.0):
+        """EnvironmentLight over a latitude-longitude map, added after the area
+        lights (gy_scene_set_environment); rgb is (height, width, 3), top row first."""
+        rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+        if rgb.ndim != 3 or rgb.shape[2] != 3:
+            raise GyError(1, "set_environment: rgb must be (height, width, 3)")
+        check(lib().gy_scene_set_environment(self.h, rgb.ctypes.data, rgb.shape[1], rgb.shape[0], float(scale)))
 
     def save_film(self, path):
         """Film::SaveImage of the last render: .hdr, .bmp or .png by suffix."""

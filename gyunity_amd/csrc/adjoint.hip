@@ -285,6 +285,7 @@ static int AdjointDevice(Scene* sc, const gy_camera_desc* cam, const gy_render_p
         sc->film_params.sampler != p->sampler)
         return Fail(GY_ERR_INVALID, "gy_adjoint: render this camera and these parameters with gy_render first");
     if (sc->tex_floats == 0) return Fail(GY_ERR_INVALID, "gy_adjoint: the scene has no textures");
+    if (sc->dev.env.w > 0) return Fail(GY_ERR_UNSUPPORTED, "gy_adjoint: scenes with an environment light");
     if (sc->stats.max_depth + 1 > 64) return Fail(GY_ERR_UNSUPPORTED, "gy_adjoint: BVH deeper than the traversal stack");
     PathParams P;
     rc = MakePathParams(c, sc, *cam, *p, &P);

@@ -64,6 +64,8 @@ GY_D V3 ld3(const float4 q) { return V3{q.x, q.y, q.z}; }
 GY_D float dcos(float x) { return (float)cos((double)x); }
 GY_D float dsin(float x) { return (float)sin((double)x); }
 GY_D float dlog(float x) { return (float)log((double)x); }
+GY_D float dacos(float x) { return (float)acos((double)x); }
+GY_D float datan2(float y, float x) { return (float)atan2((double)y, (double)x); }
 
 // ------------------------------------------------------------------ Sobol
 // sobol::sample (sampler/Sobol.h:46-61) + RandomStateSequence (sampler/Sampler.h:53-64)
@@ -212,7 +214,21 @@ GY_D void generate_ray(const CamDev& c, int px, int py, float u, float v, V3* o,
 // Shading records by primitive id: 8 x float4 =
 // (p0, mat), (p1, light), (p2, area), (n0, u0), (n1, v0), (n2, u1), (fn, v1), (u2, v2, 0, 0).
 // BVH nodes: 4 x uint4 (both child boxes in the parent, see gy_internal.h).
+// EnvironmentLight over an EnvironmentMap (light/EnvLight.cpp, light/EnvMap.h).
+struct EnvDev {
+    const float* __restrict__ tex;    // RGB texels, ImageTexture rows: the image's last row first
+    const float* __restrict__ cfunc;  // Distribution2D conditionals, one per image row (top first): w values
+    const float* __restrict__ ccdf;   // ... w + 1 CDF values a row
+    const float* __restrict__ cint;   // ... integral a row = the marginal's function
+    const float* __restrict__ mcdf;   // marginal CDF, h + 1 values
+    float mint;                       // marginal integral
+    float scale;                      // EnvironmentMap::mScale
+    float radius;                     // scene bounding-sphere radius
+    int32_t w, h;                     // w == 0: no environment light
+};
+
 struct SceneDev {
+    EnvDev env;
     const float4* __restrict__ tri;
     const float4* __restrict__ shade;
     const uint4* __restrict__ nodes;
@@ -1131,6 +1147,124 @@ GY_D float light_pdf(const SceneDev& S, int tri, const Isect& it, V3 wi) {
     if (isinf(pdf)) pdf = 0.f;
     return pdf;
 }
+// ------------------------------------------------------------------ environment light
+// ImageTexture::Triangle / Texel with CLAMP (texture/ImageTexture.h:50-69).
+GY_D V3 env_texel(const EnvDev& E, int u, int v) {
+    u = u < 0 ? 0 : (u > E.w - 1 ? E.w - 1 : u);
+    v = v < 0 ? 0 : (v > E.h - 1 ? E.h - 1 : v);
+    const float* p = E.tex + ((size_t)v * E.w + u) * 3;
+    return v3(p[0], p[1], p[2]);
+}
+// EnvironmentMap::LookupRadiance = Sample(Vec2(u, 1 - v)) * mScale (light/EnvMap.h:70-72).
+GY_D V3 env_radiance(const EnvDev& E, float u, float v) {
+    const float tv = 1.f - v;
+    const float sx = (float)E.w * u - 0.5f, ty = (float)E.h * tv - 0.5f;
+    const int s0 = (int)floorf(sx), t0 = (int)floorf(ty);
+    const float ds = sx - (float)s0, dt = ty - (float)t0;
+    const V3 a = v3s(1.f - ds) * env_texel(E, s0, t0) + v3s(ds) * env_texel(E, s0 + 1, t0);
+    const V3 b = v3s(1.f - ds) * env_texel(E, s0, t0 + 1) + v3s(ds) * env_texel(E, s0 + 1, t0 + 1);
+    return (v3s(1.f - dt) * a + v3s(dt) * b) * E.scale;
+}
+// DirToLatLong (light/EnvMap.h:106-121).
+GY_D void env_dir_to_uv(V3 d, float* u, float* v) {
+    float phi = (d.x != 0.f || d.z != 0.f) ? datan2(d.z, d.x) : 0.f;
+    const float theta = dacos(d.y);
+    phi = (phi < 0.f) ? (phi + 2 * kPI) : phi;
+    *u = phi * kInvPI * 0.5f;
+    const float vv = theta * kInvPI;
+    *v = vv < 0.f ? 0.f : (vv > 1.f ? 1.f : vv);
+}
+// LatLongToDir (light/EnvMap.h:96-104).
+GY_D V3 env_uv_to_dir(float u, float v) {
+    const float phi = u * 2 * kPI;
+    const float theta = v * kPI;
+    const float st = dsin(theta);
+    return v3(st * dcos(phi), dcos(theta), st * dsin(phi));
+}
+// SinTheta (light/EnvMap.h:124-132).
+GY_D float env_sin_theta(float v, float height) {
+    if (v < 1.f) return dsin(kPI * (float)(int)((v * height) + 0.5f) / height);
+    return dsin(kPI * ((height - 1) + 0.5f) / height);
+}
+// Distribution1D::SampleContinuous (visual/Sampling.h:52-83): upper_bound over
+// the n + 1 CDF values; *off is the segment before the zero-length fix.
+GY_D float dist1d_sample(const float* __restrict__ func, const float* __restrict__ cdf, int n, float func_int, float u,
+                         float* pdf, int* off) {
+    int lo = 0, hi = n + 1;  // first index with cdf > u
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (cdf[mid] > u) hi = mid;
+        else lo = mid + 1;
+    }
+    int offset = lo - 1;
+    offset = offset < 0 ? 0 : (offset > n - 1 ? n - 1 : offset);
+    *off = offset;
+    if (cdf[offset] == cdf[offset + 1]) {  // only u >= 1 or NaN: u in [0, 1) lands on a rising segment
+        do { offset--; } while (offset > 0 && cdf[offset] == cdf[offset + 1]);
+        if (offset < 0) offset = 0;
+    }
+    const float du = (u - cdf[offset]) / (cdf[offset + 1] - cdf[offset]);
+    *pdf = func[offset] / func_int;
+    return ((float)offset + du) / (float)n;
+}
+// Distribution2D::Pdf (visual/Sampling.h:130-140); the marginal's function is cint.
+GY_D float env_dist_pdf(const EnvDev& E, float u, float v) {
+    int iu = (int)(u * (float)E.w), iv = (int)(v * (float)E.h);
+    iu = iu < 0 ? 0 : (iu > E.w - 1 ? E.w - 1 : iu);
+    iv = iv < 0 ? 0 : (iv > E.h - 1 ? E.h - 1 : iv);
+    if (E.cint[iv] * E.mint == 0.f) return 0.f;
+    return (E.cfunc[(size_t)iv * E.w + iu] * E.cint[iv]) / (E.cint[iv] * E.mint);
+}
+// EnvironmentMap::Sample (light/EnvMap.h:18-31), Distribution2D::SampleContinuous
+// (visual/Sampling.h:121-128): marginal row with u1, column with u0.
+GY_D V3 env_sample(const EnvDev& E, float u0, float u1, V3* dir, float* pdfW) {
+    float p0, p1;
+    int v, o;
+    const float uv1 = dist1d_sample(E.cint, E.mcdf, E.h, E.mint, u1, &p1, &v);
+    const float uv0 = dist1d_sample(E.cfunc + (size_t)v * E.w, E.ccdf + (size_t)v * (E.w + 1), E.w, E.cint[v], u0, &p0, &o);
+    const float pdf = p0 * p1;
+    const float st = env_sin_theta(uv1, (float)E.h);
+    *pdfW = (st == 0.f) ? 0.f : pdf / (2 * kPI * kPI * st);
+    *dir = env_uv_to_dir(uv0, uv1);
+    return env_radiance(E, uv0, uv1);
+}
+// EnvironmentMap::Lookup / PdfDir (light/EnvMap.h:33-68).
+GY_D V3 env_lookup(const EnvDev& E, V3 dir) {
+    float u, v;
+    env_dir_to_uv(norm(dir), &u, &v);
+    return env_radiance(E, u, v);
+}
+GY_D float env_pdf_dir(const EnvDev& E, V3 dir) {
+    float u, v;
+    env_dir_to_uv(norm(dir), &u, &v);
+    const float st = env_sin_theta(v, (float)E.h);
+    if (st == 0.f) return 0.f;
+    return env_dist_pdf(E, u, v) / (2 * kPI * kPI * st);
+}
+// EnvironmentLight::Emission(dir) and Emission(ray) = Lookup(dir.Norm()) (light/EnvLight.cpp:11-20).
+GY_D V3 env_emission(const EnvDev& E, V3 dir) { return env_lookup(E, norm(dir)); }
+// EnvironmentLight::Sample_Li (light/EnvLight.cpp:41-52): the light point lies
+// 2 r along wi, its normals are wi and its position error is zero.
+GY_D LightPoint env_light_point(const EnvDev& E, V3 pos, float u0, float u1, V3* wi, V3* Li, float* pdf) {
+    V3 sd;
+    *Li = env_sample(E, u0, u1, &sd, pdf);
+    *wi = norm(sd);
+    LightPoint lp;
+    lp.pos = pos + *wi * (2 * E.radius);
+    lp.n = lp.fn = *wi;
+    lp.pErr = v3s(0.f);
+    return lp;
+}
+
+// Out-of-line forms for the per-path integrator, whose registers are already
+// near the limit: inlined there, the lat-long mapping spills.
+__device__ __noinline__ V3 env_emission_call(const EnvDev& E, V3 dir) { return env_emission(E, dir); }
+__device__ __noinline__ float env_pdf_dir_call(const EnvDev& E, V3 dir) { return env_pdf_dir(E, dir); }
+__device__ __noinline__ LightPoint env_light_point_call(const EnvDev& E, V3 pos, float u0, float u1, V3* wi, V3* Li,
+                                                        float* pdf) {
+    return env_light_point(E, pos, u0, u1, wi, Li, pdf);
+}
+
 // SpawnTo (visual/Intersection.h:64-72): shadow ray from it toward lp.
 GY_D void spawn_to(const Isect& it, const LightPoint& lp, V3* o, V3* d, float* tmax) {
     const V3 origin = offset_origin(it.pos, it.pErr, it.absN, lp.pos - it.pos);

@@ -193,7 +193,7 @@ __global__ void __launch_bounds__(256) k_count_visits(SceneDev S, const gy_ray* 
     wave_add_u64(&counters[1], nt);
 }
 
-template <int STACK, int BLOCK>
+template <int STACK, int BLOCK, bool ENV>
 __global__ void __launch_bounds__(BLOCK) k_pt_megakernel(SceneDev S, PathParams P, FilmDev F, int s0, int n_s,
                                                          unsigned long long* counters) {
     __shared__ uint32_t stack[STACK * BLOCK];
@@ -203,7 +203,7 @@ __global__ void __launch_bounds__(BLOCK) k_pt_megakernel(SceneDev S, PathParams 
     PathCounters pc = {};
     if (s_local < n_s && px < F.width && py < F.height) {
         float fx, fy;
-        const V3 L = trace_path<STACK, false>(S, P, px, py, s0 + s_local, stack + threadIdx.x, BLOCK, &fx, &fy, nullptr, pc);
+        const V3 L = trace_path<STACK, false, ENV>(S, P, px, py, s0 + s_local, stack + threadIdx.x, BLOCK, &fx, &fy, nullptr, pc);
         record_sample(F, s_local, px, py, fx, fy, L);
     }
     wave_add_u64(&counters[0], pc.rays);
@@ -212,7 +212,7 @@ __global__ void __launch_bounds__(BLOCK) k_pt_megakernel(SceneDev S, PathParams 
 }
 
 // Instrumented copy: counts node and triangle visits and shading points.
-template <int STACK, int BLOCK>
+template <int STACK, int BLOCK, bool ENV>
 __global__ void __launch_bounds__(BLOCK) k_pt_count(SceneDev S, PathParams P, FilmDev F, int s0, int n_s,
                                                     unsigned long long* counters) {
     __shared__ uint32_t stack[STACK * BLOCK];
@@ -222,7 +222,7 @@ __global__ void __launch_bounds__(BLOCK) k_pt_count(SceneDev S, PathParams P, Fi
     PathCounters pc = {};
     if (s_local < n_s && px < F.width && py < F.height) {
         float fx, fy;
-        (void)trace_path<STACK, true>(S, P, px, py, s0 + s_local, stack + threadIdx.x, BLOCK, &fx, &fy, nullptr, pc);
+        (void)trace_path<STACK, true, ENV>(S, P, px, py, s0 + s_local, stack + threadIdx.x, BLOCK, &fx, &fy, nullptr, pc);
     }
     wave_add_u64(&counters[0], pc.rays);
     wave_add_u64(&counters[1], pc.shadow);
@@ -234,7 +234,7 @@ __global__ void __launch_bounds__(BLOCK) k_pt_count(SceneDev S, PathParams P, Fi
     }
 }
 
-template <int STACK>
+template <int STACK, bool ENV>
 __global__ void __launch_bounds__(128) k_li_samples(SceneDev S, PathParams P, int n, const int32_t* __restrict__ xs,
                                                     const int32_t* __restrict__ ys, const int32_t* __restrict__ ss,
                                                     float* __restrict__ L, float* __restrict__ raster, int32_t* __restrict__ dims) {
@@ -244,7 +244,7 @@ __global__ void __launch_bounds__(128) k_li_samples(SceneDev S, PathParams P, in
     PathCounters pc = {};
     float fx, fy;
     uint32_t nd;
-    const V3 v = trace_path<STACK, false>(S, P, xs[i], ys[i], ss[i], stack + threadIdx.x, 128, &fx, &fy, &nd, pc);
+    const V3 v = trace_path<STACK, false, ENV>(S, P, xs[i], ys[i], ss[i], stack + threadIdx.x, 128, &fx, &fy, &nd, pc);
     L[3 * i] = v.x;
     L[3 * i + 1] = v.y;
     L[3 * i + 2] = v.z;
@@ -330,6 +330,59 @@ int LaunchTraceAny(Ctx* c, Scene* sc, const gy_ray* rays, int64_t n, int32_t* oc
 // ------------------------------------------------------------------ scene upload
 static float HostDot(const float* a, const float* b) { return (a[0] * b[0] + a[1] * b[1]) + (a[2] * b[2] + 0.f); }
 
+constexpr float kInfHost = __builtin_huge_valf();
+static float StdMin(float a, float b) { return (b < a) ? b : a; }  // std::min
+static float StdMax(float a, float b) { return (a < b) ? b : a; }  // std::max
+
+// Distribution1D over the light powers (visual/Sampling.h:16-45): CDF and integral.
+static float LightCdf(const std::vector<float>& func, std::vector<float>* cdf) {
+    const int n = (int)func.size();
+    cdf->assign(n + 1, 0.f);
+    std::vector<float>& c = *cdf;
+    c[0] = 0.f;
+    for (int i = 1; i < n + 1; ++i) c[i] = c[i - 1] + func[i - 1] / n;
+    const float func_int = c[n];
+    if (func_int == 0.f) {
+        for (int i = 1; i < n + 1; ++i) c[i] = float(i) / float(n);
+    } else {
+        for (int i = 1; i < n + 1; ++i) c[i] /= func_int;
+    }
+    return func_int;
+}
+
+// The light table on the device: triangle (-1: the environment light), Le,
+// power CDF and function.  Replaces the previous table.
+static int UploadLights(Ctx* c, Scene* sc, const std::vector<int32_t>& tri, const std::vector<float>& le,
+                        const std::vector<float>& func, const std::vector<float>& cdf) {
+    void* bufs[4] = {nullptr, nullptr, nullptr, nullptr};
+    const size_t n = tri.size();
+    int rc = GY_OK;
+    if (n > 0) {
+        if (rc == GY_OK) rc = Upload(c, &bufs[0], tri.data(), n * 4);
+        if (rc == GY_OK) rc = Upload(c, &bufs[1], le.data(), n * 12);
+        if (rc == GY_OK) rc = Upload(c, &bufs[2], cdf.data(), cdf.size() * 4);
+        if (rc == GY_OK) rc = Upload(c, &bufs[3], func.data(), func.size() * 4);
+    }
+    if (rc != GY_OK) {
+        for (void* b : bufs) DevFree(b);
+        return rc;
+    }
+    DevFree(sc->d_ltri);
+    DevFree(sc->d_lle);
+    DevFree(sc->d_lcdf);
+    DevFree(sc->d_lfunc);
+    sc->d_ltri = bufs[0];
+    sc->d_lle = bufs[1];
+    sc->d_lcdf = bufs[2];
+    sc->d_lfunc = bufs[3];
+    sc->dev.light_tri = (const int32_t*)bufs[0];
+    sc->dev.light_le = (const float*)bufs[1];
+    sc->dev.light_cdf = (const float*)bufs[2];
+    sc->dev.light_func = (const float*)bufs[3];
+    sc->dev.n_light = (int32_t)n;
+    return GY_OK;
+}
+
 int SceneUpload(Ctx* c, const gy_scene_desc& d, const gy_bvh_params* bp, Scene** out) {
     if (c->fault == GY_FAULT_SCENE) {
         c->fault = GY_FAULT_NONE;
@@ -338,7 +391,8 @@ int SceneUpload(Ctx* c, const gy_scene_desc& d, const gy_bvh_params* bp, Scene**
     if (d.n_tri <= 0 || !d.pos || !d.nrm || !d.face_n || !d.mat || d.n_mat <= 0 || !d.mats)
         return Fail(GY_ERR_INVALID, "scene: need n_tri > 0, pos, nrm, face_n, mat, mats");
     if ((int64_t)d.n_tri > gyd::kTravMaxTriangles) return Fail(GY_ERR_UNSUPPORTED, "scene: more triangles than the traversal addresses (2^27 - 16)");
-    if (d.n_light <= 0 || !d.light_tri || !d.light_le) return Fail(GY_ERR_INVALID, "scene: need at least one light");
+    if (d.n_light < 0 || (d.n_light > 0 && (!d.light_tri || !d.light_le)))
+        return Fail(GY_ERR_INVALID, "scene: n_light >= 0 with light_tri and light_le");
     for (int i = 0; i < d.n_tri; ++i)
         if (d.mat[i] >= (uint32_t)d.n_mat) return Fail(GY_ERR_INVALID, "scene: material id out of range");
     for (int i = 0; i < d.n_light; ++i)
@@ -387,25 +441,40 @@ int SceneUpload(Ctx* c, const gy_scene_desc& d, const gy_bvh_params* bp, Scene**
         q[6] = make_float4(f[0], f[1], f[2], uv[3]);
         q[7] = make_float4(uv[4], uv[5], 0.f, 0.f);
     }
+    // World bound: the union of the triangles' boxes (BVHAccel::WorldBound,
+    // Triangle::ObjectBound, math/AABB.cpp:29-60), for the environment light.
+    {
+        float lo[3] = {kInfHost, kInfHost, kInfHost}, hi[3] = {-kInfHost, -kInfHost, -kInfHost};
+        for (int i = 0; i < d.n_tri; ++i) {
+            const float* p = d.pos + 9 * (size_t)i;
+            for (int k = 0; k < 3; ++k) {
+                float a = StdMin(p[k], p[3 + k]), b = StdMax(p[k], p[3 + k]);
+                a = StdMin(a, p[6 + k]);
+                b = StdMax(b, p[6 + k]);
+                lo[k] = StdMin(lo[k], a);
+                hi[k] = StdMax(hi[k], b);
+            }
+        }
+        for (int k = 0; k < 3; ++k) {
+            sc->world_lo[k] = lo[k];
+            sc->world_hi[k] = hi[k];
+        }
+    }
     // Light power distribution (visual/Scene.h:121-128, visual/Sampling.h:16-45,
     // light/AreaLight.h:45-48) in FP32.
     const int nl = d.n_light;
-    std::vector<float> func(nl), cdf(nl + 1);
     const float PI = 3.14159265358979f;
+    sc->area_func.resize(nl);
     for (int i = 0; i < nl; ++i) {
         const float* le = d.light_le + 3 * i;
         const float a = area[d.light_tri[i]];
         const float P[3] = {(le[0] * a) * PI, (le[1] * a) * PI, (le[2] * a) * PI};
-        func[i] = 0.212671f * P[0] + 0.715160f * P[1] + 0.072169f * P[2];
+        sc->area_func[i] = 0.212671f * P[0] + 0.715160f * P[1] + 0.072169f * P[2];
     }
-    cdf[0] = 0.f;
-    for (int i = 1; i < nl + 1; ++i) cdf[i] = cdf[i - 1] + func[i - 1] / nl;
-    const float func_int = cdf[nl];
-    if (func_int == 0.f) {
-        for (int i = 1; i < nl + 1; ++i) cdf[i] = float(i) / float(nl);
-    } else {
-        for (int i = 1; i < nl + 1; ++i) cdf[i] /= func_int;
-    }
+    sc->area_tri.assign(d.light_tri, d.light_tri + nl);
+    sc->area_le.assign(d.light_le, d.light_le + 3 * (size_t)nl);
+    std::vector<float> func = sc->area_func, cdf;
+    const float func_int = LightCdf(func, &cdf);
     sc->dev.light_func_int = func_int;
     sc->dev.n_light = nl;
     sc->dev.n_tri = d.n_tri;
@@ -497,18 +566,7 @@ int SceneUpload(Ctx* c, const gy_scene_desc& d, const gy_bvh_params* bp, Scene**
     if ((rc = Upload(c, &p, texinfo.data(), texinfo.size() * sizeof(int4))) != GY_OK) goto fail;
     sc->d_texinfo = p;
     sc->dev.tex_info = (const int4*)p;
-    if ((rc = Upload(c, &p, d.light_tri, (size_t)nl * 4)) != GY_OK) goto fail;
-    sc->d_ltri = p;
-    sc->dev.light_tri = (const int32_t*)p;
-    if ((rc = Upload(c, &p, d.light_le, (size_t)nl * 12)) != GY_OK) goto fail;
-    sc->d_lle = p;
-    sc->dev.light_le = (const float*)p;
-    if ((rc = Upload(c, &p, cdf.data(), cdf.size() * 4)) != GY_OK) goto fail;
-    sc->d_lcdf = p;
-    sc->dev.light_cdf = (const float*)p;
-    if ((rc = Upload(c, &p, func.data(), func.size() * 4)) != GY_OK) goto fail;
-    sc->d_lfunc = p;
-    sc->dev.light_func = (const float*)p;
+    if ((rc = UploadLights(c, sc, sc->area_tri, sc->area_le, func, cdf)) != GY_OK) goto fail;
     sc->host_nodes = std::move(b.nodes);
     {
         auto t1 = std::chrono::steady_clock::now();
@@ -523,7 +581,8 @@ fail:
 
 void SceneFree(Scene* sc) {
     if (!sc) return;
-    void* bufs[] = {sc->d_tri, sc->d_shade, sc->d_nodes, sc->d_mats, sc->d_tex, sc->d_texinfo, sc->d_ltri, sc->d_lle, sc->d_lcdf, sc->d_lfunc};
+    void* bufs[] = {sc->d_tri, sc->d_shade, sc->d_nodes, sc->d_mats, sc->d_tex, sc->d_texinfo, sc->d_ltri, sc->d_lle, sc->d_lcdf, sc->d_lfunc,
+                    sc->d_env};
     for (void* b : bufs) DevFree(b);
     FilmFree(&sc->film);
     WavefrontFree(sc);
@@ -652,7 +711,7 @@ int MakePathParams(Ctx* c, Scene* sc, const gy_camera_desc& cam, const gy_render
         if (bits > sobol::kCols) return Fail(GY_ERR_UNSUPPORTED, "sample index beyond 52 bits");
     }
     // Halton: dimensions past 255 read 0 as HaltonSampler::sample returns
-    (void)sc;
+    if (sc->dev.n_light == 0) return Fail(GY_ERR_INVALID, "render: the scene has no light (area or environment)");
     return GY_OK;
 }
 
@@ -672,11 +731,17 @@ int LaunchMegaT(Ctx* c, Scene* sc, const PathParams& P, int s0, int n_s, bool co
     const unsigned blocks = (unsigned)((threads + BLOCK - 1) / BLOCK);
     unsigned long long* ctr = sc->film.counters;
     if (count) {
-        hipLaunchKernelGGL((k_pt_count<STACK, BLOCK>), dim3(blocks), dim3(BLOCK), 0, c->stream, sc->dev, P, F, s0, n_s, ctr);
+        if (sc->dev.env.w > 0)
+            hipLaunchKernelGGL((k_pt_count<STACK, BLOCK, true>), dim3(blocks), dim3(BLOCK), 0, c->stream, sc->dev, P, F, s0, n_s, ctr);
+        else
+            hipLaunchKernelGGL((k_pt_count<STACK, BLOCK, false>), dim3(blocks), dim3(BLOCK), 0, c->stream, sc->dev, P, F, s0, n_s, ctr);
         return CheckLaunch(c, "k_pt_count");
     }
     TimerBegin(c, kKernMega);
-    hipLaunchKernelGGL((k_pt_megakernel<STACK, BLOCK>), dim3(blocks), dim3(BLOCK), 0, c->stream, sc->dev, P, F, s0, n_s, ctr);
+    if (sc->dev.env.w > 0)
+        hipLaunchKernelGGL((k_pt_megakernel<STACK, BLOCK, true>), dim3(blocks), dim3(BLOCK), 0, c->stream, sc->dev, P, F, s0, n_s, ctr);
+    else
+        hipLaunchKernelGGL((k_pt_megakernel<STACK, BLOCK, false>), dim3(blocks), dim3(BLOCK), 0, c->stream, sc->dev, P, F, s0, n_s, ctr);
     TimerEnd(c);
     return CheckLaunch(c, "k_pt_megakernel");
 }
@@ -808,6 +873,98 @@ extern "C" int gy_scene_create(gy_context* ctx, const gy_scene_desc* desc, const
     int rc = SceneUpload(c, *desc, bvh, &sc);
     if (rc != GY_OK) return rc;
     *out = reinterpret_cast<gy_scene*>(sc);
+    return GY_OK;
+}
+
+// EnvironmentLight(image, rotate, scale) added after the area lights, then
+// Scene::Initialize: EnvironmentMap texels (ImageIO::LoadTexture row order),
+// ConvertImageToPdf + Distribution2D (light/EnvMap.h:82-94, visual/Sampling.h:102-119),
+// the bounding-sphere radius (visual/Scene.cpp:9-11) and the light power
+// distribution with EnvironmentLight::Power (light/EnvLight.cpp:60-63), in FP32.
+extern "C" int gy_scene_set_environment(gy_scene* s, const float* rgb, int32_t width, int32_t height, float scale) {
+    if (!s || !rgb || width <= 0 || height <= 0) return Fail(GY_ERR_INVALID, "gy_scene_set_environment: need an image and a positive size");
+    if ((int64_t)width * height > (1ll << 26)) return Fail(GY_ERR_UNSUPPORTED, "gy_scene_set_environment: image larger than 2^26 texels");
+    Scene* sc = reinterpret_cast<Scene*>(s);
+    Ctx* c = sc->ctx;
+    (void)hipSetDevice(c->device);
+    const int w = width, h = height;
+    const float PI = 3.14159265358979f;
+    std::vector<float> tex((size_t)w * h * 3);
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x)
+            for (int k = 0; k < 3; ++k) tex[((size_t)y * w + x) * 3 + k] = rgb[((size_t)(h - 1 - y) * w + x) * 3 + k];
+    std::vector<float> cfunc((size_t)w * h), ccdf((size_t)(w + 1) * h), cint(h), mcdf;
+    std::vector<float> row(w), rcdf;
+    for (int r = 0; r < h; ++r) {
+        const float v = (float)(r + 0.5f) / (float)(h);
+        const float st = (float)std::sin((double)(PI * v));
+        for (int x = 0; x < w; ++x) {
+            const float* t = &tex[((size_t)(h - r - 1) * w + x) * 3];  // ElementAt(x, r): image row r
+            row[x] = st * (0.212671f * t[0] + 0.715160f * t[1] + 0.072169f * t[2]);
+        }
+        cint[r] = LightCdf(row, &rcdf);
+        std::copy(row.begin(), row.end(), cfunc.begin() + (size_t)r * w);
+        std::copy(rcdf.begin(), rcdf.end(), ccdf.begin() + (size_t)r * (w + 1));
+    }
+    const float mint = LightCdf(cint, &mcdf);
+    // bounding-sphere radius: (max - min).Length() * 0.5, DPPS dot
+    const float e[3] = {sc->world_hi[0] - sc->world_lo[0], sc->world_hi[1] - sc->world_lo[1], sc->world_hi[2] - sc->world_lo[2]};
+    const float radius = std::sqrt(HostDot(e, e)) * 0.5f;
+    // LookupRadiance(0.5, 0.5) * PI * r * r
+    auto texel = [&](int u, int v, int k) {
+        u = u < 0 ? 0 : (u > w - 1 ? w - 1 : u);
+        v = v < 0 ? 0 : (v > h - 1 ? h - 1 : v);
+        return tex[((size_t)v * w + u) * 3 + k];
+    };
+    const float sx = (float)w * 0.5f - 0.5f, ty = (float)h * (1.f - 0.5f) - 0.5f;
+    const int s0 = (int)std::floor(sx), t0 = (int)std::floor(ty);
+    const float ds = sx - (float)s0, dt = ty - (float)t0;
+    float Pw[3];
+    for (int k = 0; k < 3; ++k) {
+        const float a = (1.f - ds) * texel(s0, t0, k) + ds * texel(s0 + 1, t0, k);
+        const float b = (1.f - ds) * texel(s0, t0 + 1, k) + ds * texel(s0 + 1, t0 + 1, k);
+        Pw[k] = ((((1.f - dt) * a + dt * b) * scale) * PI * radius) * radius;
+    }
+    std::vector<int32_t> ltri = sc->area_tri;
+    std::vector<float> lle = sc->area_le, func = sc->area_func, cdf;
+    ltri.push_back(-1);
+    lle.insert(lle.end(), {0.f, 0.f, 0.f});
+    func.push_back(0.212671f * Pw[0] + 0.715160f * Pw[1] + 0.072169f * Pw[2]);
+    const float func_int = LightCdf(func, &cdf);
+    // one device block: texels | conditional functions | CDFs | integrals | marginal CDF
+    auto al = [](size_t bytes) { return (bytes + 255) & ~(size_t)255; };
+    const size_t o_tex = 0, o_cf = o_tex + al(tex.size() * 4), o_cc = o_cf + al(cfunc.size() * 4),
+                 o_ci = o_cc + al(ccdf.size() * 4), o_mc = o_ci + al(cint.size() * 4), total = o_mc + al(mcdf.size() * 4);
+    std::vector<char> blk(total, 0);
+    std::memcpy(blk.data() + o_tex, tex.data(), tex.size() * 4);
+    std::memcpy(blk.data() + o_cf, cfunc.data(), cfunc.size() * 4);
+    std::memcpy(blk.data() + o_cc, ccdf.data(), ccdf.size() * 4);
+    std::memcpy(blk.data() + o_ci, cint.data(), cint.size() * 4);
+    std::memcpy(blk.data() + o_mc, mcdf.data(), mcdf.size() * 4);
+    void* d = nullptr;
+    int rc = Upload(c, &d, blk.data(), total);
+    if (rc != GY_OK) return rc;
+    rc = UploadLights(c, sc, ltri, lle, func, cdf);
+    if (rc != GY_OK) {
+        DevFree(d);
+        return rc;
+    }
+    DevFree(sc->d_env);
+    sc->d_env = d;
+    const char* base = (const char*)d;
+    gyd::EnvDev& E = sc->dev.env;
+    E.tex = (const float*)(base + o_tex);
+    E.cfunc = (const float*)(base + o_cf);
+    E.ccdf = (const float*)(base + o_cc);
+    E.cint = (const float*)(base + o_ci);
+    E.mcdf = (const float*)(base + o_mc);
+    E.mint = mint;
+    E.scale = scale;
+    E.radius = radius;
+    E.w = w;
+    E.h = h;
+    sc->dev.light_func_int = func_int;
+    sc->film_valid = false;
     return GY_OK;
 }
 
@@ -1077,12 +1234,11 @@ extern "C" int gy_li_samples(gy_scene* s, const gy_camera_desc* cam, const gy_re
     if (rc == GY_OK) rc = DevAlloc(c, &dd, (size_t)n * 4);
     if (rc == GY_OK) {
         const unsigned blocks = (unsigned)((n + 127) / 128);
-        if (sc->stack <= 32)
-            hipLaunchKernelGGL(k_li_samples<32>, dim3(blocks), dim3(128), 0, c->stream, sc->dev, P, n, (const int32_t*)dx,
-                               (const int32_t*)dy, (const int32_t*)ds, (float*)dL, (float*)dr, (int32_t*)dd);
-        else
-            hipLaunchKernelGGL(k_li_samples<64>, dim3(blocks), dim3(128), 0, c->stream, sc->dev, P, n, (const int32_t*)dx,
-                               (const int32_t*)dy, (const int32_t*)ds, (float*)dL, (float*)dr, (int32_t*)dd);
+        const bool env = sc->dev.env.w > 0;
+        auto kern = sc->stack <= 32 ? (env ? k_li_samples<32, true> : k_li_samples<32, false>)
+                                    : (env ? k_li_samples<64, true> : k_li_samples<64, false>);
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(128), 0, c->stream, sc->dev, P, n, (const int32_t*)dx, (const int32_t*)dy,
+                           (const int32_t*)ds, (float*)dL, (float*)dr, (int32_t*)dd);
         rc = CheckLaunch(c, "k_li_samples");
     }
     if (rc == GY_OK) {

@@ -19,16 +19,18 @@ struct PathCounters {
 // light-sample term L1 with weight w1 towards wl, BSDF-sample (MIS) term L2
 // with weight w2 towards wb, light-selection probability lsp.  The estimate is
 // (L1 w1 + L2 w2) / lsp.
+// ENV: the scene may hold an environment light (light triangle -1); without it
+// the env branches compile away and the kernels keep their registers.
 struct DIParts {
     V3 L1, L2, wl, wb;
     float w1, w2, lsp;
 };
-template <int STACK, bool COUNT>
+template <int STACK, bool COUNT, bool ENV = false>
 __device__ DIParts di_parts(const SceneDev& S, const Isect& it, const BSDF& b, float uL, float u0, float u1, float v0,
                             float v1, float v2, uint32_t* stack, int stride, PathCounters& pc) {
     DIParts r;
     const int light = sample_one_light(S, uL, &r.lsp);
-    const int ltri = S.light_tri[light];
+    const int ltri = S.light_tri[light];  // < 0: the environment light
     const V3 Le = v3(S.light_le[3 * light], S.light_le[3 * light + 1], S.light_le[3 * light + 2]);
     r.L1 = v3s(0.f);
     r.L2 = v3s(0.f);
@@ -37,10 +39,16 @@ __device__ DIParts di_parts(const SceneDev& S, const Isect& it, const BSDF& b, f
     r.w2 = 1.f;
     {
         float pdf;
-        const LightPoint lp = sample_light_point(S, ltri, it.pos, u0, u1, &pdf);
-        const V3 wi = norm(lp.pos - it.pos);
+        LightPoint lp;
+        V3 wi, Li;
+        if (ENV && ltri < 0) {
+            lp = env_light_point_call(S.env, it.pos, u0, u1, &wi, &Li, &pdf);
+        } else {
+            lp = sample_light_point(S, ltri, it.pos, u0, u1, &pdf);
+            wi = norm(lp.pos - it.pos);
+            Li = dot(lp.n, neg(wi)) > 0.f ? Le : v3(0.f, 0.f, 0.f);
+        }
         r.wl = wi;
-        V3 Li = dot(lp.n, neg(wi)) > 0.f ? Le : v3(0.f, 0.f, 0.f);
         if (pdf > 0.f && !is_zero(Li)) {
             const float sp = bsdf_pdf(b, it.outDir, wi);
             const V3 f = bsdf_eval(b, it.outDir, wi);
@@ -66,7 +74,7 @@ __device__ DIParts di_parts(const SceneDev& S, const Isect& it, const BSDF& b, f
         r.wb = wi;
         const float c = fabsf(dot(it.n, wi));
         if (pdf != 0.f && !is_zero(f)) {
-            const float lpdf = light_pdf(S, ltri, it, wi);
+            const float lpdf = (ENV && ltri < 0) ? env_pdf_dir_call(S.env, wi) : light_pdf(S, ltri, it, wi);
             if (lpdf == 0.f) {
                 r.w2 = 0.f;
             } else {
@@ -84,6 +92,8 @@ __device__ DIParts di_parts(const SceneDev& S, const Isect& it, const BSDF& b, f
                         const V3 Lem = dot(li.n, neg(wi)) > 0.f ? Le : v3(0.f, 0.f, 0.f);
                         r.L2 = Lem * f * v3s(1.f) * c / pdf;
                     }
+                } else if (ENV && ltri < 0) {  // integrator/Integrator.cpp:97-99
+                    r.L2 = env_emission_call(S.env, wi) * f * v3s(1.f) * c / pdf;
                 }
             }
         }
@@ -91,14 +101,14 @@ __device__ DIParts di_parts(const SceneDev& S, const Isect& it, const BSDF& b, f
     return r;
 }
 
-template <int STACK, bool COUNT>
+template <int STACK, bool COUNT, bool ENV = false>
 __device__ V3 direct_illumination(const SceneDev& S, const Isect& it, const BSDF& b, float uL, float u0, float u1,
                                   float v0, float v1, float v2, uint32_t* stack, int stride, PathCounters& pc) {
-    const DIParts d = di_parts<STACK, COUNT>(S, it, b, uL, u0, u1, v0, v1, v2, stack, stride, pc);
+    const DIParts d = di_parts<STACK, COUNT, ENV>(S, it, b, uL, u0, u1, v0, v1, v2, stack, stride, pc);
     return (d.L1 * d.w1 + d.L2 * d.w2) / d.lsp;
 }
 
-template <int STACK, bool COUNT>
+template <int STACK, bool COUNT, bool ENV = false>
 __device__ V3 trace_path(const SceneDev& S, const PathParams& P, int px, int py, int s, uint32_t* stack, int stride,
                          float* fx, float* fy, uint32_t* dims, PathCounters& pc) {
     const SampleSrc& M = P.smp;
@@ -118,7 +128,11 @@ __device__ V3 trace_path(const SceneDev& S, const PathParams& P, int px, int py,
         pc.rays++;
         uint32_t* nn = COUNT ? &pc.nodes[0] : nullptr;
         uint32_t* nt = COUNT ? &pc.tris[0] : nullptr;
-        if (!traverse<false, STACK>(S, o, d, 0.f, kInf, &h, stack, stride, nn, nt)) break;
+        if (!traverse<false, STACK>(S, o, d, 0.f, kInf, &h, stack, stride, nn, nt)) {
+            // integrator/PT.cpp:18-21: T * Le(ray) in place of L
+            if (ENV && (i == 0 || delta) && S.env.w > 0) L = T * env_emission_call(S.env, d);
+            break;
+        }
         const Isect it = query(S, o, d, h);
         const BSDF b = make_bsdf(S, it);
         pc.shade++;
@@ -130,7 +144,7 @@ __device__ V3 trace_path(const SceneDev& S, const PathParams& P, int px, int py,
             const float uL = seq_next(M, q);
             const float u0 = seq_next(M, q), u1 = seq_next(M, q);
             const float v0 = seq_next(M, q), v1 = seq_next(M, q), v2 = seq_next(M, q);
-            L = L + T * direct_illumination<STACK, COUNT>(S, it, b, uL, u0, u1, v0, v1, v2, stack, stride, pc);
+            L = L + T * direct_illumination<STACK, COUNT, ENV>(S, it, b, uL, u0, u1, v0, v1, v2, stack, stride, pc);
         }
         const float r0 = seq_next(M, q), r1 = seq_next(M, q), r2 = seq_next(M, q);
         V3 wi;

@@ -88,6 +88,10 @@ struct Scene {
     gyd::SceneDev dev = {};
     void *d_tri = nullptr, *d_shade = nullptr, *d_nodes = nullptr, *d_mats = nullptr, *d_tex = nullptr, *d_texinfo = nullptr;
     void *d_ltri = nullptr, *d_lle = nullptr, *d_lcdf = nullptr, *d_lfunc = nullptr;
+    void* d_env = nullptr;  // environment light: texels and sampling tables in one block
+    float world_lo[3] = {0, 0, 0}, world_hi[3] = {0, 0, 0};
+    std::vector<int32_t> area_tri;  // area lights as created (the environment light follows them)
+    std::vector<float> area_le, area_func;
     int stack = 64;
     gy_scene_stats stats = {};
     double upload_ms = 0.0;

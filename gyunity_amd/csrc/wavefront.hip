@@ -55,6 +55,7 @@ constexpr uint32_t P_W1_NF = 8u;   // w1 not finite: 0 * w1 is NaN
 constexpr uint32_t P_W2_NF = 16u;  // w2 not finite: 0 * w2 is NaN
 constexpr uint32_t P_DONE = 32u;   // the path ended this bounce: it goes to the film
 constexpr uint32_t P_NEXT = 64u;   // the path goes on without pending direct light
+constexpr uint32_t P_ESC = 128u;   // the MIS ray left the scene (k_wf_env adds an environment light's L2)
 // A path goes on after this bounce iff its flags are non-zero without P_DONE.
 
 struct WfDev {
@@ -354,14 +355,20 @@ __global__ void __launch_bounds__(256) k_wf_trace(R Rd, WfDev W, int refill, int
                             W.mtype[my] = (uint8_t)Sf.mats[(size_t)mat * 32];
                         } else {
                             W.mtype[my] = 255;
-                            W.pend3[slot].w = __uint_as_float(P_DONE);  // no environment light: the path ends
+                            W.pend3[slot].w = __uint_as_float(P_DONE);  // the path ends (k_wf_env: environment)
                         }
                     } else if (MODE == 1) {
                         if (r == 2) {
                             float& fw = W.pend3[slot].w;
                             fw = __uint_as_float(__float_as_uint(fw) | P_OCCLUDED);
                         }
-                    } else if (t.best_prim >= 0) {
+                    } else if (t.best_prim < 0) {
+                        // a miss counts for an environment light (k_wf_env)
+                        if (Rd.scene(e).env.w > 0) {
+                            float& fw = W.pend3[slot].w;
+                            fw = __uint_as_float(__float_as_uint(fw) | P_ESC);
+                        }
+                    } else {
                         // MIS branch of DirectIllumination (integrator/Integrator.cpp:88-98)
                         const SceneDev& Sf = Rd.scene(e);
                         const float4 p2 = W.pend2[slot];
@@ -443,8 +450,48 @@ __global__ void __launch_bounds__(256) k_wf_bin(WfDev W) {
     }
 }
 
+// ---------------------------------------------------------------- environment light
+// Lookups of an environment light for rays that left the scene, in a kernel of
+// their own so that the trace kernels keep their registers (the lat-long
+// mapping evaluates atan2 and acos in double).
+// MODE 0, after the closest-hit trace: a camera ray, or one after a delta
+// bounce, returns T * Le(ray) in place of L (integrator/PT.cpp:18-21).
+// MODE 1, after the MIS trace: an escaped MIS ray of a path that chose the
+// environment light gives L2 = Le(wi) f cos / pdf (integrator/Integrator.cpp:97-99),
+// stored as L2 w2 like a hit of an area light.
+template <int MODE, class R>
+__global__ void __launch_bounds__(256) k_wf_env(R Rd, WfDev W) {
+    const uint32_t n = W.counts[(MODE == 0 ? Q_ACTIVE : Q_MIS) * kCtrStride];
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t slot = W.q[MODE == 0 ? Q_ACTIVE : Q_MIS][i];
+    const SceneDev& S = Rd.scene(Rd.entry(W, slot));
+    if (S.env.w == 0) return;
+    if (MODE == 0) {
+        if (W.mtype[i] != 255) return;
+        const uint32_t sw = W.seq[slot].w;
+        if ((sw & F_DEPTH_MASK) != 0u && !(sw & F_DELTA)) return;
+        const float4 Lv = W.L[slot];
+        const V3 L = f3(W.T[slot]) * env_emission(S.env, f3(W.d[slot]));
+        W.L[slot] = make_float4(L.x, L.y, L.z, Lv.w);
+    } else {
+        float& fw = W.pend3[slot].w;
+        const uint32_t pf = __float_as_uint(fw);
+        if (!(pf & P_ESC)) return;
+        const float4 p2 = W.pend2[slot];
+        if (S.light_tri[__float_as_int(p2.w)] >= 0) return;
+        const float4 p1 = W.pend1[slot];
+        const V3 L2 = env_emission(S.env, f3(W.mis_d[slot])) * f3(p1) * v3s(1.f) * p1.w / p2.x;
+        const V3 L2w2 = L2 * p2.y;
+        W.pend1[slot] = make_float4(L2w2.x, L2w2.y, L2w2.z, p1.w);
+        fw = __uint_as_float(pf | P_L2);
+    }
+}
+
 // ---------------------------------------------------------------- shade
-template <int MTYPE, class R>
+// ENV: some scene holds an environment light (light triangle -1); without it
+// the env branches compile away.
+template <int MTYPE, class R, bool ENV>
 __global__ void __launch_bounds__(256) k_wf_shade(R Rd, WfDev W) {
     const uint32_t n = W.counts[(Q_SHADE0 + MTYPE) * kCtrStride];
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
@@ -491,15 +538,21 @@ __global__ void __launch_bounds__(256) k_wf_shade(R Rd, WfDev W) {
             // DirectIllumination set-up (integrator/Integrator.cpp:9-88)
             float lsp;
             const int light = sample_one_light(S, uL, &lsp);
-            const int ltri = S.light_tri[light];
+            const int ltri = S.light_tri[light];  // < 0: the environment light
             const V3 Le = v3(S.light_le[3 * light], S.light_le[3 * light + 1], S.light_le[3 * light + 2]);
             V3 L1pre = v3s(0.f);
             float w1 = 1.f, w2 = 1.f;
             {
                 float pdf;
-                const LightPoint lp = sample_light_point(S, ltri, it.pos, u0, u1, &pdf);
-                const V3 wi = norm(lp.pos - it.pos);
-                const V3 Li = dot(lp.n, neg(wi)) > 0.f ? Le : v3(0.f, 0.f, 0.f);
+                LightPoint lp;
+                V3 wi, Li;
+                if (ENV && ltri < 0) {
+                    lp = env_light_point(S.env, it.pos, u0, u1, &wi, &Li, &pdf);
+                } else {
+                    lp = sample_light_point(S, ltri, it.pos, u0, u1, &pdf);
+                    wi = norm(lp.pos - it.pos);
+                    Li = dot(lp.n, neg(wi)) > 0.f ? Le : v3(0.f, 0.f, 0.f);
+                }
                 if (pdf > 0.f && !is_zero(Li)) {
                     const float sp = bsdf_pdf(b, it.outDir, wi);
                     const V3 f = bsdf_eval(b, it.outDir, wi);
@@ -521,7 +574,7 @@ __global__ void __launch_bounds__(256) k_wf_shade(R Rd, WfDev W) {
             const V3 f2 = bsdf_sample(b, it.outDir, &wi2, &pdf2, v0, v1, v2);
             const float c2 = fabsf(dot(it.n, wi2));
             if (pdf2 != 0.f && !is_zero(f2)) {
-                const float lpdf = light_pdf(S, ltri, it, wi2);
+                const float lpdf = (ENV && ltri < 0) ? env_pdf_dir(S.env, wi2) : light_pdf(S, ltri, it, wi2);
                 if (lpdf == 0.f) {
                     w2 = 0.f;
                 } else {
@@ -783,7 +836,7 @@ constexpr int kTraceChunks[3] = {8, 4, 1};
 constexpr int kAutoRefill = 48;
 template <int STACK, class R>
 static int WfBounceLoop(Ctx* c, const R& Rd, const WfDev& W0, int depth_max, int refill, unsigned blocks, uint32_t n_slots,
-                        unsigned long long* counters) {
+                        bool env, unsigned long long* counters) {
     WfDev W = W0;
     const unsigned pblocks = PersistBlocks(c);
     for (int depth = 0; depth < depth_max; ++depth) {
@@ -792,12 +845,20 @@ static int WfBounceLoop(Ctx* c, const R& Rd, const WfDev& W0, int depth_max, int
         TimerEnd(c);
         TimerBegin(c, kKernOther);  // queue bookkeeping
         hipLaunchKernelGGL(k_wf_bin, dim3((blocks + kBinPerThread - 1) / kBinPerThread), dim3(256), 0, c->stream, W);
+        if (env) hipLaunchKernelGGL((k_wf_env<0, R>), dim3(blocks), dim3(256), 0, c->stream, Rd, W);
         TimerEnd(c);
         TimerBegin(c, kKernShade);
-        hipLaunchKernelGGL((k_wf_shade<0, R>), dim3(blocks), dim3(256), 0, c->stream, Rd, W);
-        hipLaunchKernelGGL((k_wf_shade<1, R>), dim3(blocks), dim3(256), 0, c->stream, Rd, W);
-        hipLaunchKernelGGL((k_wf_shade<2, R>), dim3(blocks), dim3(256), 0, c->stream, Rd, W);
-        hipLaunchKernelGGL((k_wf_shade<3, R>), dim3(blocks), dim3(256), 0, c->stream, Rd, W);
+        if (env) {
+            hipLaunchKernelGGL((k_wf_shade<0, R, true>), dim3(blocks), dim3(256), 0, c->stream, Rd, W);
+            hipLaunchKernelGGL((k_wf_shade<1, R, true>), dim3(blocks), dim3(256), 0, c->stream, Rd, W);
+            hipLaunchKernelGGL((k_wf_shade<2, R, true>), dim3(blocks), dim3(256), 0, c->stream, Rd, W);
+            hipLaunchKernelGGL((k_wf_shade<3, R, true>), dim3(blocks), dim3(256), 0, c->stream, Rd, W);
+        } else {
+            hipLaunchKernelGGL((k_wf_shade<0, R, false>), dim3(blocks), dim3(256), 0, c->stream, Rd, W);
+            hipLaunchKernelGGL((k_wf_shade<1, R, false>), dim3(blocks), dim3(256), 0, c->stream, Rd, W);
+            hipLaunchKernelGGL((k_wf_shade<2, R, false>), dim3(blocks), dim3(256), 0, c->stream, Rd, W);
+            hipLaunchKernelGGL((k_wf_shade<3, R, false>), dim3(blocks), dim3(256), 0, c->stream, Rd, W);
+        }
         TimerEnd(c);
         TimerBegin(c, kKernTraceShadow);
         hipLaunchKernelGGL((k_wf_trace<STACK, 1, R>), dim3(pblocks), dim3(256), 0, c->stream, Rd, W, refill > 0 ? refill : kAutoRefill, kTraceChunks[1]);
@@ -806,6 +867,7 @@ static int WfBounceLoop(Ctx* c, const R& Rd, const WfDev& W0, int depth_max, int
         hipLaunchKernelGGL((k_wf_trace<STACK, 2, R>), dim3(pblocks), dim3(256), 0, c->stream, Rd, W, refill > 0 ? refill : kAutoRefill, kTraceChunks[2]);
         TimerEnd(c);
         TimerBegin(c, kKernOther);
+        if (env) hipLaunchKernelGGL((k_wf_env<1, R>), dim3(blocks), dim3(256), 0, c->stream, Rd, W);
         hipLaunchKernelGGL((k_wf_resolve<R>), dim3(ResolveBlocks(n_slots)), dim3(256), 0, c->stream, Rd, W, n_slots);
         hipLaunchKernelGGL(k_wf_advance, dim3(1), dim3(64), 0, c->stream, W, counters);
         TimerEnd(c);
@@ -851,7 +913,8 @@ int WavefrontPass(Ctx* c, Scene* sc, const PathParams& P, const gy_render_params
         return CheckLaunch(c, "k_wf_resolve");
     }
     if (sc->stats.max_depth + 1 > 64) return Fail(GY_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
-    return WfBounceLoop<kStackLds>(c, Rd, W, p.max_depth, RefillOf(p.refill_threshold), blocks, (uint32_t)cap, sc->film.counters);
+    return WfBounceLoop<kStackLds>(c, Rd, W, p.max_depth, RefillOf(p.refill_threshold), blocks, (uint32_t)cap, sc->dev.env.w > 0,
+                                   sc->film.counters);
 }
 
 // ---------------------------------------------------------------- batch
@@ -876,6 +939,8 @@ static int BatchSequence(Ctx* c, Batch* b) {
     if (rc == GY_OK) rc = Fill32(c, b->counters, 0u, 32);
     if (rc != GY_OK) return rc;
     const unsigned blocks = (unsigned)((lanes + 255) / 256);
+    bool env = false;
+    for (auto& en : b->entries) env = env || en.scene->dev.env.w > 0;
     for (int64_t base = 0; base < b->total_paths && rc == GY_OK; base += lanes) {
         const int64_t count = std::min(lanes, b->total_paths - base);
         hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(64), 0, c->stream, W);
@@ -883,7 +948,7 @@ static int BatchSequence(Ctx* c, Batch* b) {
         hipLaunchKernelGGL(k_wf_raygen_batch, dim3(blocks), dim3(256), 0, c->stream, W, Rd, base, count, lanes);
         TimerEnd(c);
         rc = CheckLaunch(c, "k_wf_raygen_batch");
-        if (rc == GY_OK) rc = WfBounceLoop<kStackLds>(c, Rd, W, b->depth_max, b->refill, blocks, (uint32_t)lanes, b->counters);
+        if (rc == GY_OK) rc = WfBounceLoop<kStackLds>(c, Rd, W, b->depth_max, b->refill, blocks, (uint32_t)lanes, env, b->counters);
     }
     for (auto& en : b->entries)
         if (rc == GY_OK) rc = FilmResolveState(c, en.film, en.params.spp);

@@ -243,6 +243,15 @@ public:
         for (float v : Le) light_le.push_back(v);
         return (int)light_tri.size() - 1;
     }
+    // Scene::AddLight(EnvironmentLight(file, rotate, scale)) after the area
+    // lights (light/EnvLight.h:14-17): the latitude-longitude image, top row
+    // first, w x h x 3 floats.
+    void SetEnvironmentLight(int w, int h, const std::vector<float>& rgb, float scale = 1.f) {
+        env_w = w;
+        env_h = h;
+        env_rgb = rgb;
+        env_scale = scale;
+    }
     // ImageTexture (texture/ImageTexture.h:50-69): width x height x channels floats.
     int AddTexture(int w, int h, int channels, const std::vector<float>& data) {
         tex_info.insert(tex_info.end(), {w, h, channels, (int32_t)tex_data.size()});
@@ -304,6 +313,9 @@ public:
     std::vector<float> light_le;
     std::vector<int32_t> tex_info;
     std::vector<float> tex_data;
+    int env_w = 0, env_h = 0;  // env_w == 0: no environment light
+    std::vector<float> env_rgb;
+    float env_scale = 1.f;
 };
 
 // Scene resident on a device: triangles, BVH (built on the GPU), shading data.
@@ -312,6 +324,13 @@ public:
     DeviceScene(const Device& dev, const Scene& s, const gy_bvh_params* bvh = nullptr) {
         const gy_scene_desc d = s.Desc();
         Check(gy_scene_create(dev.Handle(), &d, bvh, &h_), "gy_scene_create");
+        if (s.env_w > 0) {
+            const int rc = gy_scene_set_environment(h_, s.env_rgb.data(), s.env_w, s.env_h, s.env_scale);
+            if (rc != GY_OK) {
+                gy_scene_destroy(h_);
+                Check(rc, "gy_scene_set_environment");
+            }
+        }
     }
     ~DeviceScene() { gy_scene_destroy(h_); }
     DeviceScene(const DeviceScene&) = delete;

@@ -143,6 +143,16 @@ int gy_device_name(gy_context* ctx, char* buf, int len);
  *      accelerator/BVHAccel.cpp:154-200) ------------------------------------ */
 int gy_scene_create(gy_context* ctx, const gy_scene_desc* desc, const gy_bvh_params* bvh, gy_scene** out);
 int gy_scene_destroy(gy_scene* scene);
+/* Environment light: EnvironmentLight(file, rotate, scale) + Scene::AddLight
+ * (light/EnvLight.h:14-17, visual/Scene.h:59-64) after the area lights, with the
+ * image already loaded: rgb is the latitude-longitude image, width*height*3
+ * floats, top row first (as ImageIO::LoadTexture reads it through stbi_loadf,
+ * image/ImageIO.cpp:100-114).  The map is importance-sampled as EnvironmentMap
+ * does (light/EnvMap.h:7-134) and seen by camera rays and rays after a delta
+ * bounce that leave the scene (integrator/PT.cpp:18-21).  A scene may be
+ * created with no area light and take its light from here.  Calling it again
+ * replaces the map.  Not supported by gy_adjoint. */
+int gy_scene_set_environment(gy_scene* scene, const float* rgb, int32_t width, int32_t height, float scale);
 int gy_scene_stats_get(gy_scene* scene, gy_scene_stats* out);
 /* The context a scene was created on. */
 int gy_scene_context(gy_scene* scene, gy_context** ctx);

@@ -50,6 +50,10 @@ inline float rlog(float x) { return (float)std::log((double)x); }
 inline double rcos(double x) { return std::cos(x); }
 inline double rsin(double x) { return std::sin(x); }
 inline double rlog(double x) { return std::log(x); }
+inline float racos(float x) { return (float)std::acos((double)x); }
+inline float ratan2(float y, float x) { return (float)std::atan2((double)y, (double)x); }
+inline double racos(double x) { return std::acos(x); }
+inline double ratan2(double y, double x) { return std::atan2(y, x); }
 
 // ---------------------------------------------------------------- vectors
 // Vec3 follows the SSE Vector<3,float> of math/Linagl.h: lane-wise + - * /,
@@ -520,7 +524,55 @@ template <class R> struct BVHNode {  // accelerator/BVHAccel.cpp:114-124
     uint8_t nPrims, axis;
 };
 
+// Distribution1D (visual/Sampling.h:8-98): piecewise-constant density on [0, 1).
+template <class R> struct Dist1D {
+    std::vector<R> func, cdf;
+    R funcInt = 0;
+    int count = 0;
+    void build(const R* f, int n) {
+        count = n;
+        func.assign(f, f + n);
+        cdf.assign(n + 1, 0);
+        cdf[0] = 0.;
+        for (int i = 1; i < n + 1; ++i) cdf[i] = cdf[i - 1] + func[i - 1] / n;
+        funcInt = cdf[n];
+        if (funcInt == 0.f) {
+            for (int i = 1; i < n + 1; ++i) cdf[i] = R(i) / R(n);
+        } else {
+            for (int i = 1; i < n + 1; ++i) cdf[i] /= funcInt;
+        }
+    }
+    // SampleContinuous (visual/Sampling.h:52-83); `off` is the segment before
+    // the zero-length fix, as there.
+    R sample_continuous(R u, R* pdf, int* off) const {
+        auto it = std::upper_bound(cdf.begin(), cdf.begin() + count + 1, u);
+        int offset = clampr(int(std::distance(cdf.begin(), it) - 1), 0, count - 1);
+        if (off) *off = offset;
+        if (cdf[offset] == cdf[offset + 1]) {  // only u >= 1 or NaN reach here
+            do { offset--; } while (offset > 0 && cdf[offset] == cdf[offset + 1]);
+            if (offset < 0) offset = 0;
+        }
+        R du = (u - cdf[offset]) / (cdf[offset + 1] - cdf[offset]);
+        if (pdf) *pdf = func[offset] / funcInt;
+        return (offset + du) / count;
+    }
+};
+
+// EnvironmentMap + EnvironmentLight (light/EnvMap.h:7-134, light/EnvLight.cpp).
+// tex holds ImageTexture rows: ImageIO::LoadTexture puts the image's last row
+// first (image/ImageIO.cpp:100-114).
+template <class R> struct Env {
+    int w = 0, h = 0;  // w == 0: no environment light
+    R scale = 1;
+    std::vector<V3<R>> tex;
+    std::vector<Dist1D<R>> cond;  // Distribution2D: one per image row, top row first
+    Dist1D<R> marg;
+    V3<R> center;
+    R radius = 0;
+};
+
 template <class R> struct Scene {
+    Env<R> env;
     std::vector<Tri<R>> tris;
     std::vector<Material> mats;
     std::vector<int32_t> light_tri;
@@ -1282,11 +1334,16 @@ template <class R> BSDF<R> make_bsdf(const Scene<R>& s, const Isect<R>& it) {
 // ---------------------------------------------------------------- lights
 // Scene::ComputeLightPowerDistribution + Distribution1D (visual/Scene.h:121-128,
 // visual/Sampling.h:16-97), AreaLight::Power (light/AreaLight.h:45-48).
+template <class R> V3<R> env_radiance(const Env<R>& e, R u, R v);
 template <class R> void build_light_distribution(Scene<R>& s) {
     int n = (int)s.light_tri.size();
     s.light_func.resize(n);
     for (int i = 0; i < n; ++i) {
-        V3<R> P = s.light_le[i] * s.tris[s.light_tri[i]].area * K<R>::PI;
+        V3<R> P;
+        if (s.light_tri[i] < 0)  // EnvironmentLight::Power (light/EnvLight.cpp:60-63)
+            P = env_radiance(s.env, (R)0.5f, (R)0.5f) * K<R>::PI * s.env.radius * s.env.radius;
+        else
+            P = s.light_le[i] * s.tris[s.light_tri[i]].area * K<R>::PI;
         s.light_func[i] = lumY(P);
     }
     s.light_cdf.assign(n + 1, 0);
@@ -1343,6 +1400,120 @@ template <class R> R light_pdf(const Scene<R>& s, int tri, const Isect<R>& it, c
     if (std::isinf(pdf)) return 0;
     return pdf;
 }
+// ---------------------------------------------------------------- environment light
+// ImageTexture::Triangle / Texel, CLAMP (texture/ImageTexture.h:50-69).
+template <class R> inline V3<R> env_texel(const Env<R>& e, int u, int v) {
+    u = u < 0 ? 0 : (u > e.w - 1 ? e.w - 1 : u);
+    v = v < 0 ? 0 : (v > e.h - 1 ? e.h - 1 : v);
+    return e.tex[(size_t)v * e.w + u];
+}
+// EnvironmentMap::LookupRadiance = mpImage->Sample(Vec2(u, 1 - v)) * mScale (EnvMap.h:70-72).
+template <class R> V3<R> env_radiance(const Env<R>& e, R u, R v) {
+    R tv = 1 - v;
+    R sx = e.w * u - (R)0.5f, ty = e.h * tv - (R)0.5f;
+    int s0 = (int)std::floor(sx), t0 = (int)std::floor(ty);
+    R ds = sx - s0, dt = ty - t0;
+    V3<R> a = (1 - ds) * env_texel(e, s0, t0) + ds * env_texel(e, s0 + 1, t0);
+    V3<R> b = (1 - ds) * env_texel(e, s0, t0 + 1) + ds * env_texel(e, s0 + 1, t0 + 1);
+    return ((1 - dt) * a + dt * b) * e.scale;
+}
+// DirToLatLong (EnvMap.h:106-121): u = phi / 2pi about +y from +x towards +z,
+// v = theta / pi from +y.
+template <class R> void env_dir_to_uv(const V3<R>& d, R* u, R* v) {
+    R phi = (d.x != 0 || d.z != 0) ? ratan2(d.z, d.x) : (R)0;
+    R theta = racos(d.y);
+    phi = (phi < 0) ? (phi + 2 * K<R>::PI) : phi;
+    *u = phi * K<R>::INV_PI * (R)0.5;
+    *v = clampr(theta * K<R>::INV_PI, (R)0, (R)1);
+}
+// LatLongToDir (EnvMap.h:96-104).
+template <class R> V3<R> env_uv_to_dir(R u, R v) {
+    R phi = u * 2 * K<R>::PI;
+    R theta = v * K<R>::PI;
+    R st = rsin(theta);
+    return V3<R>(st * rcos(phi), rcos(theta), st * rsin(phi));
+}
+// SinTheta (EnvMap.h:124-132): sin of the centre of the pixel row holding v.
+template <class R> R env_sin_theta(R v, R height) {
+    if (v < 1) return rsin(K<R>::PI * (int)((v * height) + (R)0.5f) / (R)(height));
+    return rsin(K<R>::PI * (R)((height - 1) + (R)0.5f) / (R)(height));
+}
+// Distribution2D::Pdf (visual/Sampling.h:130-140).
+template <class R> R env_dist_pdf(const Env<R>& e, R u, R v) {
+    int iu = clampr((int)(u * e.cond[0].count), 0, e.cond[0].count - 1);
+    int iv = clampr((int)(v * e.marg.count), 0, e.marg.count - 1);
+    if (e.cond[iv].funcInt * e.marg.funcInt == 0.f) return 0.f;
+    return (e.cond[iv].func[iu] * e.marg.func[iv]) / (e.cond[iv].funcInt * e.marg.funcInt);
+}
+// EnvironmentMap::Sample (EnvMap.h:18-31) with Distribution2D::SampleContinuous
+// (visual/Sampling.h:121-128).
+template <class R> V3<R> env_sample(const Env<R>& e, R u0, R u1, V3<R>* dir, R* pdfW) {
+    R uv[2], pdfs[2];
+    int v;
+    uv[1] = e.marg.sample_continuous(u1, &pdfs[1], &v);
+    uv[0] = e.cond[v].sample_continuous(u0, &pdfs[0], nullptr);
+    R pdf = pdfs[0] * pdfs[1];
+    R st = env_sin_theta(uv[1], (R)e.h);
+    *pdfW = (st == 0) ? (R)0 : pdf / (2 * K<R>::PI * K<R>::PI * st);
+    *dir = env_uv_to_dir(uv[0], uv[1]);
+    return env_radiance(e, uv[0], uv[1]);
+}
+// EnvironmentMap::Lookup / PdfDir (EnvMap.h:33-68).
+template <class R> V3<R> env_lookup(const Env<R>& e, const V3<R>& dir) {
+    R u, v;
+    env_dir_to_uv(norm(dir), &u, &v);
+    return env_radiance(e, u, v);
+}
+template <class R> R env_pdf_dir(const Env<R>& e, const V3<R>& dir) {
+    R u, v;
+    env_dir_to_uv(norm(dir), &u, &v);
+    R st = env_sin_theta(v, (R)e.h);
+    if (st == 0) return 0;
+    return env_dist_pdf(e, u, v) / (2 * K<R>::PI * K<R>::PI * st);
+}
+// EnvironmentLight::Emission(dir) = Lookup(dir.Norm()) (EnvLight.cpp:12-20).
+template <class R> V3<R> env_emission(const Env<R>& e, const V3<R>& dir) { return env_lookup(e, norm(dir)); }
+// EnvironmentMap(filename) after ImageIO::LoadTexture + ConvertImageToPdf
+// (EnvMap.h:10-16, 82-94); rgb is the image top row first, 3 floats a texel.
+template <class R> void build_env(Scene<R>& s, const float* rgb, int w, int h, float scale) {
+    Env<R>& e = s.env;
+    e.w = w;
+    e.h = h;
+    e.scale = scale;
+    e.tex.resize((size_t)w * h);
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            const float* p = rgb + ((size_t)(h - 1 - y) * w + x) * 3;
+            e.tex[(size_t)y * w + x] = V3<R>(p[0], p[1], p[2]);
+        }
+    std::vector<R> data((size_t)w * h);
+    for (int r = 0; r < h; ++r) {
+        R v = (R)(r + (R)0.5f) / (R)(h);
+        R sinTheta = rsin(K<R>::PI * v);
+        for (int c = 0; c < w; ++c) data[c + (size_t)r * w] = sinTheta * lumY(e.tex[(size_t)(h - r - 1) * w + c]);
+    }
+    e.cond.assign(h, Dist1D<R>());
+    std::vector<R> marginal(h);
+    for (int v = 0; v < h; ++v) {
+        e.cond[v].build(&data[(size_t)v * w], w);
+        marginal[v] = e.cond[v].funcInt;
+    }
+    e.marg.build(marginal.data(), h);
+    // Scene::GetBoundingSphere from the BVH's world bound (visual/Scene.cpp:9-11,
+    // math/AABB.h:69-72); the bound is the union of the triangles' boxes.
+    V3<R> mn(K<R>::Inf, K<R>::Inf, K<R>::Inf), mx(-K<R>::Inf, -K<R>::Inf, -K<R>::Inf);
+    for (const Tri<R>& t : s.tris) {
+        V3<R> a(rmin(t.p0.x, t.p1.x), rmin(t.p0.y, t.p1.y), rmin(t.p0.z, t.p1.z));
+        V3<R> b(rmax(t.p0.x, t.p1.x), rmax(t.p0.y, t.p1.y), rmax(t.p0.z, t.p1.z));
+        a = V3<R>(rmin(a.x, t.p2.x), rmin(a.y, t.p2.y), rmin(a.z, t.p2.z));
+        b = V3<R>(rmax(b.x, t.p2.x), rmax(b.y, t.p2.y), rmax(b.z, t.p2.z));
+        mn = V3<R>(rmin(mn.x, a.x), rmin(mn.y, a.y), rmin(mn.z, a.z));
+        mx = V3<R>(rmax(mx.x, b.x), rmax(mx.y, b.y), rmax(mx.z, b.z));
+    }
+    e.center = (mn + mx) * (R)0.5;
+    e.radius = length(mx - mn) * (R)0.5;
+}
+
 template <class R> inline R power_heuristic(R f, R g) {  // integrator/Integrator.h:11-15
     f = 1 * f;
     g = 1 * g;
@@ -1356,13 +1527,26 @@ template <class R> V3<R> direct_illumination(const Scene<R>& s, const Isect<R>& 
     R weight1 = 1, weight2 = 1;
     V3<R> L1, L2;
     int light = sample_one_light(s, uLight, &lightSamplingPdf);
-    int ltri = s.light_tri[light];
+    int ltri = s.light_tri[light];  // < 0: the environment light
     V3<R> Le = s.light_le[light];
     {
         R pdf;
-        LightPoint<R> lp = sample_light_point(s.tris[ltri], it, u0, u1, &pdf);
-        V3<R> wi = norm(lp.pos - it.pos);
-        V3<R> Li = dot(lp.n, -wi) > 0 ? Le : V3<R>(0, 0, 0);
+        LightPoint<R> lp;
+        V3<R> wi, Li;
+        if (ltri < 0) {
+            // EnvironmentLight::Sample_Li (light/EnvLight.cpp:41-52): the light
+            // point sits 2 r along wi, with zero position error
+            V3<R> sd;
+            Li = env_sample(s.env, u0, u1, &sd, &pdf);
+            wi = norm(sd);
+            lp.pos = it.pos + wi * (2 * s.env.radius);
+            lp.n = lp.fn = wi;
+            lp.pErr = V3<R>(0, 0, 0);
+        } else {
+            lp = sample_light_point(s.tris[ltri], it, u0, u1, &pdf);
+            wi = norm(lp.pos - it.pos);
+            Li = dot(lp.n, -wi) > 0 ? Le : V3<R>(0, 0, 0);
+        }
         if (pdf > 0 && Li != V3<R>(0)) {
             R scatteringPdf = bsdf.Pdf(it.outDir, wi);
             V3<R> f = bsdf.Evaluate(it.outDir, wi);
@@ -1382,7 +1566,7 @@ template <class R> V3<R> direct_illumination(const Scene<R>& s, const Isect<R>& 
         V3<R> f = bsdf.Sample(it.outDir, &wi, &pdf, v);
         R cosTheta = std::abs(dot(it.n, wi));
         if (pdf != 0 && f != V3<R>()) {
-            R lpdf = light_pdf(s, ltri, it, wi);
+            R lpdf = ltri < 0 ? env_pdf_dir(s.env, wi) : light_pdf(s, ltri, it, wi);
             if (lpdf == 0) {
                 weight2 = 0;
             } else {
@@ -1396,6 +1580,9 @@ template <class R> V3<R> direct_illumination(const Scene<R>& s, const Isect<R>& 
                         V3<R> Tr(1.f);
                         L2 = Lemit * f * Tr * cosTheta / pdf;
                     }
+                } else if (ltri < 0) {  // Integrator.cpp:97-99
+                    V3<R> Tr(1.f);
+                    L2 = env_emission(s.env, wi) * f * Tr * cosTheta / pdf;
                 }
             }
         }
@@ -1431,7 +1618,12 @@ template <class R> V3<R> Li(const Scene<R>& s, Ray<R> ray, Seq& rnd, int maxDept
             r[14] = hit ? s.tris[h.prim].mat : -1;
             r[15] = delta ? 1 : 0;
         }
-        if (!hit) break;
+        if (!hit) {
+            // PT.cpp:18-21: a camera ray, or one after a delta bounce, that
+            // leaves the scene returns T * Le(ray) in place of L
+            if ((i == 0 || delta) && s.env.w > 0) return T * env_emission(s.env, ray.d);
+            break;
+        }
         Isect<R> it = query(s, ray, h);
         BSDF<R> bsdf = make_bsdf(s, it);
         int light = s.tris[it.prim].light;
@@ -1529,7 +1721,22 @@ struct SceneHandle {
     Scene<float> sf;
     std::unique_ptr<Scene<double>> sd;
     std::mutex mu;
+    std::vector<float> env;  // environment image, top row first (empty: none)
+    int env_w = 0, env_h = 0;
+    float env_scale = 1.f;
 };
+
+// Scene::AddLight of an EnvironmentLight after the area lights, then
+// Scene::Initialize: light init (bounding sphere), power distribution.
+template <class R> void attach_env(Scene<R>& s, const SceneHandle* h) {
+    if (h->env.empty()) return;
+    build_env(s, h->env.data(), h->env_w, h->env_h, h->env_scale);
+    if (s.light_tri.empty() || s.light_tri.back() >= 0) {
+        s.light_tri.push_back(-1);
+        s.light_le.push_back(V3<R>(0, 0, 0));
+    }
+    build_light_distribution(s);
+}
 
 template <class R> void fill_scene(Scene<R>& s, const or_scene_desc& d) {
     build_tris(s, d);
@@ -1568,6 +1775,7 @@ template <> Scene<double>& scene_of<double>(SceneHandle* h) {
     if (!h->sd) {
         h->sd.reset(new Scene<double>());
         fill_scene(*h->sd, h->desc);
+        attach_env(*h->sd, h);
     }
     return *h->sd;
 }
@@ -1750,6 +1958,52 @@ void* or_scene_create(const or_scene_desc* desc) {
 }
 
 void or_scene_destroy(void* s) { delete (SceneHandle*)s; }
+
+int or_scene_set_environment(void* s, const float* rgb, int32_t w, int32_t h, float scale) {
+    SceneHandle* sh = (SceneHandle*)s;
+    if (!rgb || w <= 0 || h <= 0) return -1;
+    std::lock_guard<std::mutex> g(sh->mu);
+    sh->env.assign(rgb, rgb + (size_t)w * h * 3);
+    sh->env_w = w;
+    sh->env_h = h;
+    sh->env_scale = scale;
+    attach_env(sh->sf, sh);
+    if (sh->sd) attach_env(*sh->sd, sh);
+    return 0;
+}
+
+// EnvironmentMap queries for tests (float mode): radiance at (u, v), sample at
+// (u0, u1) -> dir + pdf, pdf of a direction, Le of a direction.
+int or_env_query(void* s, int32_t n, const float* a, const float* b, const float* c, float* out_rgb, float* out_dir,
+                 float* out_pdf, int kind) {
+    SceneHandle* sh = (SceneHandle*)s;
+    const Env<float>& e = sh->sf.env;
+    if (e.w == 0) return -1;
+    for (int32_t i = 0; i < n; ++i) {
+        V3<float> L, d;
+        float pdf = 0;
+        if (kind == 0) {
+            L = env_radiance(e, a[i], b[i]);
+        } else if (kind == 1) {
+            L = env_sample(e, a[i], b[i], &d, &pdf);
+        } else if (kind == 2) {
+            pdf = env_pdf_dir(e, V3<float>(a[i], b[i], c[i]));
+        } else {
+            L = env_emission(e, V3<float>(a[i], b[i], c[i]));
+        }
+        if (out_rgb) { out_rgb[3 * i] = L.x; out_rgb[3 * i + 1] = L.y; out_rgb[3 * i + 2] = L.z; }
+        if (out_dir) { out_dir[3 * i] = d.x; out_dir[3 * i + 1] = d.y; out_dir[3 * i + 2] = d.z; }
+        if (out_pdf) out_pdf[i] = pdf;
+    }
+    return 0;
+}
+int or_env_info(void* s, float* radius, float* light_func, int32_t max_lights) {
+    SceneHandle* sh = (SceneHandle*)s;
+    *radius = sh->sf.env.radius;
+    const int n = (int)sh->sf.light_func.size();
+    for (int i = 0; i < n && i < max_lights; ++i) light_func[i] = sh->sf.light_func[i];
+    return n;
+}
 
 int or_bvh_stats(void* s, int64_t* n_nodes, int32_t* max_depth) {
     SceneHandle* h = (SceneHandle*)s;

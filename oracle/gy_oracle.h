@@ -86,6 +86,14 @@ int or_camera_rays(const or_camera_desc* cam, int32_t n, const int32_t* px, cons
 
 void* or_scene_create(const or_scene_desc* desc);
 void or_scene_destroy(void* scene);
+/* EnvironmentLight appended after the area lights: rgb is the lat-long image,
+ * top row first, 3 floats a texel (as stbi_loadf returns it). */
+int or_scene_set_environment(void* scene, const float* rgb, int32_t w, int32_t h, float scale);
+/* kind 0: LookupRadiance(a, b); 1: Sample(a, b) -> rgb, dir, pdf; 2: PdfDir((a, b, c));
+ * 3: Emission((a, b, c)). */
+int or_env_query(void* scene, int32_t n, const float* a, const float* b, const float* c, float* out_rgb,
+                 float* out_dir, float* out_pdf, int kind);
+int or_env_info(void* scene, float* radius, float* light_func, int32_t max_lights);
 /* Reference BVH statistics: node count, max depth. */
 int or_bvh_stats(void* scene, int64_t* n_nodes, int32_t* max_depth);
 

@@ -101,6 +101,9 @@ def lib():
         "or_faure_perm": (C.c_int, [C.c_uint32, vp]),
         "or_tex_lookup": (C.c_int, [C.c_int, C.c_int, C.c_int, vp, C.c_int, vp, vp, C.c_int, vp]),
         "or_set_shadow_short": (C.c_int, [C.c_int]),
+        "or_scene_set_environment": (C.c_int, [vp, vp, i32, i32, f32]),
+        "or_env_query": (C.c_int, [vp, i32, vp, vp, vp, vp, vp, vp, C.c_int]),
+        "or_env_info": (C.c_int, [vp, vp, vp, i32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -239,6 +242,35 @@ class OracleScene:
         rows = np.zeros((max_rows, 16), np.float64)
         n = lib().or_li_path(self.h, C.byref(c), C.byref(prm), int(x), int(y), int(s), max_rows, _p(rows))
         return rows[:n]
+
+    def set_environment(self, rgb, scale=1.0):
+        """EnvironmentLight(image, rotate, scale) added after the area lights;
+        rgb is (h, w, 3) float32, top row first."""
+        self._env = np.ascontiguousarray(rgb, np.float32)
+        h, w = self._env.shape[:2]
+        rc = lib().or_scene_set_environment(self.h, _p(self._env), w, h, float(scale))
+        assert rc == 0
+
+    def env_query(self, kind, a, b, c=None):
+        """kind 0: LookupRadiance(a, b); 1: Sample(a, b) -> (rgb, dir, pdf);
+        2: PdfDir((a, b, c)); 3: Emission((a, b, c))."""
+        a = np.ascontiguousarray(a, np.float32)
+        b = np.ascontiguousarray(b, np.float32)
+        c = np.ascontiguousarray(c if c is not None else np.zeros_like(a), np.float32)
+        n = a.shape[0]
+        rgb = np.zeros((n, 3), np.float32)
+        d = np.zeros((n, 3), np.float32)
+        pdf = np.zeros(n, np.float32)
+        rc = lib().or_env_query(self.h, n, _p(a), _p(b), _p(c), _p(rgb), _p(d), _p(pdf), kind)
+        assert rc == 0
+        return rgb, d, pdf
+
+    def env_info(self):
+        """(bounding-sphere radius, light power function values)."""
+        r = C.c_float()
+        f = np.zeros(64, np.float32)
+        n = lib().or_env_info(self.h, C.byref(r), _p(f), 64)
+        return r.value, f[:n].copy()
 
     def render(self, cam, prm, row0=0, row1=0):
         c = camera_from(cam)

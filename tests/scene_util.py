@@ -148,12 +148,14 @@ def check_bvh(nodes, order, pos, max_leaf):
     return leaf_depth
 
 
-def mixed_material_scene(seed=0, res=64):
+def mixed_material_scene(seed=0, res=64, closed=True, area_light=True):
     """Cornell-style room exercising every material path of the device shader:
     textured Lambert walls (RGB and one-channel albedo textures with explicit
     uvs), a smooth glass box (GlassMaterial), a rough dielectric box
     (RoughDielectricMaterial, roughness -> alpha), a rough conductor box with
-    FrConductor(eta, k) and a roughness texture, and a quad area light."""
+    FrConductor(eta, k) and a roughness texture, and a quad area light.
+    closed=False leaves out the ceiling and the side walls (a scene open to an
+    environment light); area_light=False leaves out the quad light."""
     import gyunity_amd as g
 
     rng = np.random.default_rng(seed)
@@ -183,11 +185,14 @@ def mixed_material_scene(seed=0, res=64):
         quads.append((a, b, c, d, n, m, emit))
 
     quad((0, 0, 0), (1, 0, 0), (1, 0, 1), (0, 0, 1), (0, 1, 0), 0)        # floor
-    quad((0, 1, 0), (0, 1, 1), (1, 1, 1), (1, 1, 0), (0, -1, 0), 0)       # ceiling
+    if closed:
+        quad((0, 1, 0), (0, 1, 1), (1, 1, 1), (1, 1, 0), (0, -1, 0), 0)   # ceiling
     quad((0, 0, 0), (0, 1, 0), (1, 1, 0), (1, 0, 0), (0, 0, 1), 2)        # back, one-channel texture
-    quad((0, 0, 0), (0, 0, 1), (0, 1, 1), (0, 1, 0), (1, 0, 0), 1)        # left, RGB texture
-    quad((1, 0, 0), (1, 1, 0), (1, 1, 1), (1, 0, 1), (-1, 0, 0), 3)       # right, green
-    quad((0.375, 0.99, 0.375), (0.625, 0.99, 0.375), (0.625, 0.99, 0.625), (0.375, 0.99, 0.625), (0, -1, 0), 7, True)
+    if closed:
+        quad((0, 0, 0), (0, 0, 1), (0, 1, 1), (0, 1, 0), (1, 0, 0), 1)    # left, RGB texture
+        quad((1, 0, 0), (1, 1, 0), (1, 1, 1), (1, 0, 1), (-1, 0, 0), 3)   # right, green
+    if area_light:
+        quad((0.375, 0.99, 0.375), (0.625, 0.99, 0.375), (0.625, 0.99, 0.625), (0.375, 0.99, 0.625), (0, -1, 0), 7, True)
 
     def box(lo, hi, m):
         x0, y0, z0 = lo
@@ -221,8 +226,21 @@ def mixed_material_scene(seed=0, res=64):
     fn = np.array(fn, np.float32)
     nrm = np.repeat(fn, 3, axis=0).reshape(-1, 9)
     sa = SceneArrays(pos, nrm, fn, np.array(mat, np.uint32), np.array(light, np.int32), np.stack(mats),
-                     np.array(ltri, np.int32), np.tile(np.array([17, 12, 4], np.float32), (len(ltri), 1)),
+                     np.array(ltri, np.int32).reshape(-1), np.tile(np.array([17, 12, 4], np.float32), (len(ltri), 1)),
                      uv=np.array(uv, np.float32), tex_info=np.array(tex_info, np.int32), tex_data=tex_data)
     cam = g.camera((0.5, 0.5, 2.35), (0.5, 0.5, 0.0), (0, 1, 0), 40.0, res, res)
     prm = g.render_params(spp=16, max_depth=6, rr_depth=3)
     return sa, cam, prm
+
+
+def env_image(w=64, h=32, seed=0):
+    """A latitude-longitude sky, top row first: a horizon gradient, a bright
+    small sun and seeded texel noise, all positive."""
+    rng = np.random.default_rng(seed)
+    v = (np.arange(h) + 0.5) / h
+    u = (np.arange(w) + 0.5) / w
+    sky = np.stack([0.4 + 0.6 * (1 - v), 0.5 + 0.4 * (1 - v), 0.8 + 0.2 * (1 - v)], -1)[:, None, :] * np.ones((1, w, 1))
+    du = np.minimum(np.abs(u[None, :] - 0.3), 1 - np.abs(u[None, :] - 0.3))
+    sun = np.exp(-((du / 0.04) ** 2 + ((v[:, None] - 0.25) / 0.06) ** 2)) * 40.0
+    img = sky * rng.uniform(0.8, 1.2, (h, w, 3)) + sun[..., None] * np.array([1.0, 0.9, 0.7])
+    return img.astype(np.float32)
