@@ -5,7 +5,7 @@ One step = one full render (67,108,864 camera paths) of that image with the
 scene and BVH already resident in HBM.  Prints one JSON line on stdout.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode wavefront|megakernel]
-                  [--no-cpu-baseline] [--level 8] [--workload icosphere|batch64]
+                  [--no-cpu-baseline] [--level 8] [--workload icosphere|env|batch64]
 """
 import argparse
 import json
@@ -48,7 +48,7 @@ def kernel_bytes(name, st, paths):
     return float("nan")
 
 
-def cpu_baseline(sa, cam, prm, rows):
+def cpu_baseline(sa, cam, prm, rows, env=None):
     """The oracle's FP32 restatement of the reference CPU integrator
     (TiledIntegrator + PathTracing, reference BVH) on the host cores, over a
     band of `rows` image rows of the same workload."""
@@ -56,6 +56,8 @@ def cpu_baseline(sa, cam, prm, rows):
 
     threads = min(16, os.cpu_count() or 1)
     os_ = O.OracleScene(sa)
+    if env is not None:
+        os_.set_environment(env)
     row0 = cam.height // 2 - rows // 2
     p = O.params(prm.spp, prm.max_depth, prm.rr_depth, precision=0, nthreads=threads, use_bvh=1)
     _, _, paths, secs = os_.render(cam, p, row0, row0 + rows)
@@ -68,6 +70,23 @@ def cpu_baseline(sa, cam, prm, rows):
         "sample": "rows %d-%d of the 1024^2 image (%d paths, %.1f s), same scene, spp and depth"
         % (row0, row0 + rows - 1, paths, secs),
     }
+
+
+def open_room_sky(g, sa, w=1024, h=512):
+    """The icosphere workload lit by an environment light: the room keeps its
+    floor and back wall (triangles 0-1 and 4-5; the ceiling, side walls and
+    ceiling light go), and a procedural latitude-longitude sky with a sun
+    replaces the light."""
+    keep = np.r_[0:2, 4:6, 12:sa.n_tri]
+    out = g.SceneArrays(sa.pos[keep], sa.nrm[keep], sa.face_n[keep], sa.mat[keep], np.full(len(keep), -1, np.int32),
+                        sa.mats, np.zeros(0, np.int32), np.zeros((0, 3), np.float32), uv=sa.uv[keep] if sa.uv is not None else None)
+    v = (np.arange(h) + 0.5) / h
+    u = (np.arange(w) + 0.5) / w
+    sky = np.stack([0.3 + 0.5 * (1 - v), 0.4 + 0.4 * (1 - v), 0.7 + 0.3 * (1 - v)], -1)[:, None, :] * np.ones((1, w, 1))
+    du = np.minimum(np.abs(u[None, :] - 0.3), 1 - np.abs(u[None, :] - 0.3))
+    sun = np.exp(-((du / 0.02) ** 2 + ((v[:, None] - 0.2) / 0.03) ** 2)) * 200.0
+    img = (sky + sun[..., None] * np.array([1.0, 0.9, 0.75])).astype(np.float32)
+    return out, img
 
 
 def batch_main(args):
@@ -354,7 +373,7 @@ def main():
     ap.add_argument("--refill", type=int, default=0, help="wavefront refill threshold (0 = library default)")
     ap.add_argument("--spp-pass", type=int, default=0, help="samples per pass (0 = library default)")
     ap.add_argument("--leaf", type=int, default=0, help="BVH leaf size (0 = library default, 4)")
-    ap.add_argument("--workload", default="icosphere", choices=["icosphere", "batch64", "adjoint", "adjoint-scale"],
+    ap.add_argument("--workload", default="icosphere", choices=["icosphere", "env", "batch64", "adjoint", "adjoint-scale"],
                     help="icosphere: the 1.31M-triangle GGX render; batch64: the 64-scene batch; adjoint: "
                          "gradient of the textured Cornell room; adjoint-scale: Adam on 4.2M texels over 32 views")
     ap.add_argument("--fd-texels", type=int, default=0, help="adjoint: texels checked against finite differences")
@@ -387,6 +406,9 @@ def main():
     import gyunity_amd as g
 
     sa, cam, prm = g.generate_scene(g.GY_SCENE_ICOSPHERE_GGX, 0, args.level)
+    env = None
+    if args.workload == "env":
+        sa, env = open_room_sky(g, sa)
     prm.mode = g.GY_RENDER_WAVEFRONT if args.mode == "wavefront" else g.GY_RENDER_MEGAKERNEL
     prm.refill_threshold = args.refill
     prm.samples_per_pass = args.spp_pass
@@ -395,6 +417,8 @@ def main():
     log("device:", ctx.device_name())
     t0 = time.perf_counter()
     sc = g.Scene(ctx, sa, max_leaf_size=args.leaf)
+    if env is not None:
+        sc.set_environment(env)
     sst = sc.stats()
     log("scene: %d triangles, BVH %d nodes depth %d, build %.1f ms (scene create %.1f ms)"
         % (sa.n_tri, sst.n_nodes, sst.max_depth, sst.build_ms, (time.perf_counter() - t0) * 1e3))
@@ -440,7 +464,7 @@ def main():
     # the same counters on a plain copy of known size), when they cover this workload.
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_icosphere-L%d-1024-64spp.txt" % args.level)
-    if os.path.exists(tpath):
+    if os.path.exists(tpath) and env is None:
         with open(tpath) as f:
             for line in f:
                 w = line.split()
@@ -471,10 +495,12 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic: procedural displaced icosphere (seed 0) in the Cornell room",
+        "data": "synthetic: procedural displaced icosphere (seed 0) in the Cornell room"
+        + ("; open room, procedural 1024x512 sky" if env is not None else ""),
         "config": {
-            "workload": "icosphere-L%d GGX conductor, %d triangles, %dx%d, %d spp, depth %d, NEE+MIS"
-            % (args.level, sa.n_tri, cam.width, cam.height, prm.spp, prm.max_depth),
+            "workload": ("icosphere-L%d GGX conductor, %d triangles, %dx%d, %d spp, depth %d, NEE+MIS"
+                         % (args.level, sa.n_tri, cam.width, cam.height, prm.spp, prm.max_depth))
+            + (", environment light (open room, 1024x512 sky)" if env is not None else ""),
             "integrator": args.mode,
             "megakernel_mpaths_per_s": round(paths / ((ms_per_step if args.mode == "megakernel" else other_ms) / 1e3) / 1e6, 3)
             if (args.mode == "megakernel" or other_ms) else None,
@@ -501,7 +527,7 @@ def main():
     ctx.close()
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(sa, cam, prm, args.cpu_rows)
+        out["cpu_baseline"] = cpu_baseline(sa, cam, prm, args.cpu_rows, env)
     print(json.dumps(out), flush=True)
 
 

@@ -16,6 +16,7 @@ from ._lib import (  # noqa: F401
     camera_matrix,
     generate_scene,
     lib,
+    image_read,
     image_to_bytes,
     image_write,
     load_obj,

@@ -183,6 +183,7 @@ def lib():
         "gy_debug_inject_fault": (C.c_int, [vp, C.c_int]),
         "gy_image_write": (C.c_int, [C.c_char_p, P(f32), i32, i32]),
         "gy_image_to_bytes": (C.c_int, [P(f32), i64, P(C.c_uint8)]),
+        "gy_image_read": (C.c_int, [C.c_char_p, P(i32), P(i32), vp, i64]),
         "gy_film_save": (C.c_int, [vp, C.c_char_p]),
         "gy_scene_set_environment": (C.c_int, [vp, vp, C.c_int32, C.c_int32, C.c_float]),
         "gy_mesh_load_obj": (C.c_int, [C.c_char_p, i32, P(f32), P(f32), P(vp)]),
@@ -345,6 +346,16 @@ def image_write(path, rgb):
     rgb = np.ascontiguousarray(rgb, dtype=np.float32)
     h, w = rgb.shape[:2]
     check(lib().gy_image_write(os.fsencode(path), _fp(rgb), w, h))
+
+
+def image_read(path):
+    """A Radiance .hdr file as (height, width, 3) float32, top row first, as
+    stbi_loadf reads it for ImageIO::LoadTexture (gy_image_read)."""
+    w, h = C.c_int32(), C.c_int32()
+    check(lib().gy_image_read(os.fsencode(path), C.byref(w), C.byref(h), None, 0))
+    out = np.zeros((h.value, w.value, 3), np.float32)
+    check(lib().gy_image_read(os.fsencode(path), C.byref(w), C.byref(h), out.ctypes.data, out.size))
+    return out
 
 
 def image_to_bytes(rgb):

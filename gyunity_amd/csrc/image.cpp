@@ -174,6 +174,133 @@ bool WritePng(Out& o, const std::vector<uint8_t>& px, int w, int h) {
     return true;
 }
 
+// ---- reading: Radiance .hdr as stbi_loadf returns it (ImageIO::LoadTexture,
+// image/ImageIO.cpp:100-114, reads environment maps through stbi_loadf with
+// the file's 3 channels).  The decoding follows stb_image's HDR reader
+// (external/stb/stb_image.h): header lines up to an empty one, which must
+// include FORMAT=32-bit_rle_rgbe; a "-Y h +X w" line; flat RGBE pixels when
+// w < 8 or w >= 32768, else per-scanline run-length data per component, with
+// a scanline that does not open with 2 2 switching the rest of the file to
+// flat pixels from the first one on; texel = c * 2^(e - 136), 0 when e = 0.
+struct In {
+    std::vector<uint8_t> d;
+    size_t p = 0;
+    bool eof() const { return p >= d.size(); }
+    int get8() { return eof() ? 0 : d[p++]; }
+};
+
+std::string HdrToken(In& in) {
+    std::string t;
+    int c = in.get8();
+    while (!in.eof() && c != '\n') {
+        t.push_back((char)c);
+        if (t.size() == 1023) {
+            while (!in.eof() && in.get8() != '\n') {
+            }
+            break;
+        }
+        c = in.get8();
+    }
+    return t;
+}
+
+void HdrConvert(float* out, const uint8_t* e) {
+    if (e[3] != 0) {
+        const float f1 = (float)std::ldexp(1.0f, e[3] - (int)(128 + 8));
+        out[0] = e[0] * f1;
+        out[1] = e[1] * f1;
+        out[2] = e[2] * f1;
+    } else {
+        out[0] = out[1] = out[2] = 0.f;
+    }
+}
+
+int ReadHdr(const char* path, int32_t* width, int32_t* height, float* rgb, int64_t max_floats) {
+    In in;
+    {
+        FILE* f = std::fopen(path, "rb");
+        if (!f) return Fail(GY_ERR_INVALID, std::string("gy_image_read: cannot open ") + path);
+        uint8_t buf[1 << 16];
+        size_t n;
+        while ((n = std::fread(buf, 1, sizeof buf, f)) > 0) in.d.insert(in.d.end(), buf, buf + n);
+        std::fclose(f);
+    }
+    const std::string id = HdrToken(in);
+    if (id != "#?RADIANCE" && id != "#?RGBE") return Fail(GY_ERR_INVALID, "gy_image_read: not a Radiance HDR file");
+    bool valid = false;
+    for (;;) {
+        const std::string t = HdrToken(in);
+        if (t.empty()) break;
+        if (t == "FORMAT=32-bit_rle_rgbe") valid = true;
+    }
+    if (!valid) return Fail(GY_ERR_UNSUPPORTED, "gy_image_read: unsupported HDR format");
+    const std::string res = HdrToken(in);
+    const char* tok = res.c_str();
+    if (std::strncmp(tok, "-Y ", 3) != 0) return Fail(GY_ERR_UNSUPPORTED, "gy_image_read: unsupported HDR data layout");
+    char* end = nullptr;
+    const long h = std::strtol(tok + 3, &end, 10);
+    tok = end;
+    while (*tok == ' ') ++tok;
+    if (std::strncmp(tok, "+X ", 3) != 0) return Fail(GY_ERR_UNSUPPORTED, "gy_image_read: unsupported HDR data layout");
+    const long w = std::strtol(tok + 3, nullptr, 10);
+    if (w <= 0 || h <= 0 || (int64_t)w * h > (1ll << 28)) return Fail(GY_ERR_UNSUPPORTED, "gy_image_read: HDR size out of range");
+    *width = (int32_t)w;
+    *height = (int32_t)h;
+    const int64_t nf = (int64_t)w * h * 3;
+    if (!rgb) return GY_OK;  // size query
+    if (max_floats < nf) return Fail(GY_ERR_INVALID, "gy_image_read: output buffer too small");
+    auto need = [&](size_t k) { return in.p + k <= in.d.size(); };
+    auto flat_from = [&](int64_t first) {  // flat RGBE pixels from pixel `first` to the end
+        for (int64_t i = first; i < (int64_t)w * h; ++i) {
+            if (!need(4)) return false;
+            HdrConvert(rgb + 3 * i, &in.d[in.p]);
+            in.p += 4;
+        }
+        return true;
+    };
+    if (w < 8 || w >= 32768) {
+        if (!flat_from(0)) return Fail(GY_ERR_INVALID, "gy_image_read: truncated HDR data");
+        return GY_OK;
+    }
+    std::vector<uint8_t> line((size_t)w * 4);
+    for (long j = 0; j < h; ++j) {
+        if (!need(4)) return Fail(GY_ERR_INVALID, "gy_image_read: truncated HDR data");
+        const uint8_t c1 = in.d[in.p], c2 = in.d[in.p + 1], l0 = in.d[in.p + 2];
+        if (c1 != 2 || c2 != 2 || (l0 & 0x80)) {
+            // not run-length coded: these four bytes are the first pixel of
+            // the image, and the rest of the file is flat pixels
+            HdrConvert(rgb, &in.d[in.p]);
+            in.p += 4;
+            if (!flat_from(1)) return Fail(GY_ERR_INVALID, "gy_image_read: truncated HDR data");
+            return GY_OK;
+        }
+        const int len = (l0 << 8) | in.d[in.p + 3];
+        in.p += 4;
+        if (len != w) return Fail(GY_ERR_INVALID, "gy_image_read: invalid decoded scanline length");
+        for (int k = 0; k < 4; ++k) {
+            long i = 0;
+            while (i < w) {
+                if (!need(1)) return Fail(GY_ERR_INVALID, "gy_image_read: truncated HDR data");
+                int count = in.d[in.p++];
+                const long left = w - i;
+                if (count > 128) {
+                    if (!need(1)) return Fail(GY_ERR_INVALID, "gy_image_read: truncated HDR data");
+                    const uint8_t v = in.d[in.p++];
+                    count -= 128;
+                    if (count > left) return Fail(GY_ERR_INVALID, "gy_image_read: bad RLE data in HDR");
+                    for (int z = 0; z < count; ++z) line[(size_t)(i++) * 4 + k] = v;
+                } else {
+                    if (count > left) return Fail(GY_ERR_INVALID, "gy_image_read: bad RLE data in HDR");
+                    if (!need((size_t)count)) return Fail(GY_ERR_INVALID, "gy_image_read: truncated HDR data");
+                    for (int z = 0; z < count; ++z) line[(size_t)(i++) * 4 + k] = in.d[in.p++];
+                }
+            }
+        }
+        for (long i = 0; i < w; ++i) HdrConvert(rgb + ((int64_t)j * w + i) * 3, &line[(size_t)i * 4]);
+    }
+    return GY_OK;
+}
+
 bool EndsWith(const std::string& s, const char* suf) {
     const size_t n = std::strlen(suf);
     return s.size() >= n && s.compare(s.size() - n, n, suf) == 0;
@@ -207,6 +334,13 @@ int WriteImage(const char* path, const float* rgb, int32_t w, int32_t h) {
 
 extern "C" int gy_image_write(const char* path, const float* rgb, int32_t width, int32_t height) {
     return gy::WriteImage(path, rgb, width, height);
+}
+
+extern "C" int gy_image_read(const char* path, int32_t* width, int32_t* height, float* rgb, int64_t max_floats) {
+    if (!path || !width || !height) return gy::Fail(GY_ERR_INVALID, "gy_image_read: null argument");
+    const std::string p(path);
+    if (!gy::EndsWith(p, ".hdr")) return gy::Fail(GY_ERR_UNSUPPORTED, "gy_image_read: only Radiance .hdr files are read");
+    return gy::ReadHdr(path, width, height, rgb, max_floats);
 }
 
 extern "C" int gy_image_to_bytes(const float* rgb, int64_t n, uint8_t* out) {

@@ -242,6 +242,11 @@ uint64_t gy_halton_enum_index(uint32_t width, uint32_t height, uint64_t i, uint3
 int gy_image_write(const char* path, const float* rgb, int32_t width, int32_t height);
 int gy_image_to_bytes(const float* rgb, int64_t n, uint8_t* out);
 int gy_film_save(gy_scene* scene, const char* path);
+/* Reads a Radiance .hdr file as stbi_loadf does for ImageIO::LoadTexture
+ * (image/ImageIO.cpp:100-114), the loader of EnvironmentMap(filename): rgb gets
+ * width*height*3 floats, top row first.  With rgb NULL only the size is
+ * returned.  Other formats: GY_ERR_UNSUPPORTED. */
+int gy_image_read(const char* path, int32_t* width, int32_t* height, float* rgb, int64_t max_floats);
 
 /* ---- OBJ meshes (Mesh::LoadFromFile, mesh/Mesh.cpp:7-84, through
  *      tinyobjloader 2.0.0 with triangulation; Scene::AddMesh,

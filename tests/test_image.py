@@ -7,6 +7,11 @@
   after the header, whose comment line differs.
 - PNG pixels must equal it once decoded: its deflate encoder and filters are
   not reproduced.
+- gy_image_read (Radiance .hdr, the loader EnvironmentMap(filename) uses
+  through stbi_loadf) must return the floats of the reference's own stb_image
+  (external/stb/stb_image.h, built by oracle/Makefile into oracle/_ref/hdr_ref)
+  bit for bit: files from both writers, flat small-width files, long runs,
+  the non-run-length fallback and header variants.
 - These comparisons skip where the reference tree is absent.
 - The 8-bit mapping ImageIO::toInt is checked against closed forms.
 """
@@ -22,6 +27,7 @@ import gyunity_amd as g
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 IMG_REF = os.path.join(ROOT, "oracle", "_ref", "img_ref")
+HDR_REF = os.path.join(ROOT, "oracle", "_ref", "hdr_ref")
 
 
 def _image(w, h, seed):
@@ -142,3 +148,109 @@ def test_png_pixels_equal_reference_writer(tmp_path):
 def test_image_write_rejects_unknown_suffix(tmp_path):
     with pytest.raises(g.GyError):
         g.image_write(str(tmp_path / "x.jpg"), np.zeros((2, 2, 3), np.float32))
+
+
+def _ref_read(path, tmp):
+    out = os.path.join(tmp, "read.raw")
+    subprocess.check_call([HDR_REF, path, out])
+    raw = open(out, "rb").read()
+    w, h, c = struct.unpack("<iii", raw[:12])
+    return np.frombuffer(raw, np.float32, w * h * c, 12).reshape(h, w, c)
+
+
+def _need_hdr_ref():
+    if not os.path.exists(HDR_REF):
+        pytest.skip("oracle/_ref/hdr_ref is built only where the reference tree is present")
+
+
+def _rgbe(rgb):
+    """RGBE bytes of float pixels (stb_image_write's conversion)."""
+    out = []
+    for r, gch, b in rgb.reshape(-1, 3):
+        m = max(r, gch, b)
+        if m < 1e-32:
+            out.append((0, 0, 0, 0))
+            continue
+        mant, ex = np.frexp(np.float32(m))
+        sc = np.float32(mant * np.float32(256.0) / np.float32(m))
+        out.append((int(np.float32(r) * sc), int(np.float32(gch) * sc), int(np.float32(b) * sc), ex + 128))
+    return np.array(out, np.uint8)
+
+
+@pytest.mark.parametrize("w,h", [(37, 11), (5, 4), (300, 3)])
+def test_hdr_read_equals_reference_loader(tmp_path, w, h):
+    _need_hdr_ref()
+    img = _image(w, h, 5)
+    img[1 % h, : w // 2] = img[1 % h, 0]  # runs
+    ours = str(tmp_path / "ours.hdr")
+    g.image_write(ours, img)
+    got = g.image_read(ours)
+    assert got.shape == (h, w, 3)
+    assert np.array_equal(got, _ref_read(ours, str(tmp_path)))
+    # 8-bit mantissas: within 2^-8 of the largest channel
+    m = img.max(axis=2, keepdims=True)
+    assert np.all(np.abs(got - img) <= m * 2.0 ** -7 + 1e-30)
+    if os.path.exists(IMG_REF):  # the reference writer's file reads the same
+        ref_file = str(tmp_path / "ref.hdr")
+        _ref_write(ref_file, w, h, img)
+        assert np.array_equal(g.image_read(ref_file), got)
+
+
+def test_hdr_read_crafted_files(tmp_path):
+    _need_hdr_ref()
+    rng = np.random.default_rng(9)
+    w, h = 200, 3
+    pix = rng.integers(0, 256, (h, w, 4)).astype(np.uint8)
+    pix[..., 3] = rng.integers(100, 150, (h, w))
+    pix[0, 10:190] = pix[0, 10]  # a run longer than 127
+    pix[2, :, 3] = 0             # exponent 0: zero pixels
+    pix[0, 0, 0] = 7             # the first pixel does not open with 2 2
+    body, first_line = b"", 0
+    for j in range(h):
+        if j == 1:
+            first_line = len(body)
+        body += bytes([2, 2, w >> 8, w & 255])
+        for k in range(4):
+            row = pix[j, :, k]
+            i = 0
+            while i < w:
+                r = i
+                while r < w and r - i < 127 and row[r] == row[i]:
+                    r += 1
+                if r - i >= 3:
+                    body += bytes([128 + r - i, row[i]])
+                    i = r
+                else:
+                    n = min(128, w - i)
+                    body += bytes([n]) + row[i:i + n].tobytes()
+                    i += n
+    files = {
+        "rle": b"#?RADIANCE\n# comment\nFORMAT=32-bit_rle_rgbe\nEXPOSURE=1.0\n\n-Y 3  +X 200\n" + body,
+        "rgbe_id": b"#?RGBE\nFORMAT=32-bit_rle_rgbe\n\n-Y 3 +X 200\n" + body,
+        # the second scanline does not open with 2 2: stb reads the rest as
+        # flat pixels starting over at the first pixel
+        "fallback": b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y 3 +X 200\n" + body[:first_line] +
+                    pix.reshape(-1, 4).tobytes(),
+        "flat": b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y 3 +X 7\n" + pix[:, :7].tobytes(),
+    }
+    for name, data in files.items():
+        path = str(tmp_path / (name + ".hdr"))
+        open(path, "wb").write(data)
+        ref = _ref_read(path, str(tmp_path))
+        assert np.array_equal(g.image_read(path), ref), name
+    # closed form: texel = c * 2^(e - 136), 0 for e = 0
+    got = g.image_read(str(tmp_path / "rle.hdr"))
+    e = pix[..., 3].astype(np.int64)
+    expect = np.where((e > 0)[..., None], pix[..., :3] * np.ldexp(np.float32(1), e - 136)[..., None], 0).astype(np.float32)
+    assert np.array_equal(got, expect)
+
+
+@pytest.mark.parametrize("text", [b"#?RADIANCE\n\n-Y 2 +X 2\n", b"P6\n2 2\n255\n",
+                                  b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n+X 2 -Y 2\n"])
+def test_hdr_read_rejects_bad_files(tmp_path, text):
+    path = str(tmp_path / "bad.hdr")
+    open(path, "wb").write(text + bytes(64))
+    with pytest.raises(g.GyError):
+        g.image_read(path)
+    with pytest.raises(g.GyError):
+        g.image_read(str(tmp_path / "x.png"))
