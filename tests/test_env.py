@@ -256,3 +256,36 @@ def test_env_rejected_by_adjoint(gpu_ctx):
         sc.render(cam, prm)
         with pytest.raises(g.GyError):
             sc.adjoint(cam, prm, np.zeros((cam.height, cam.width, 3), np.float32))
+
+
+@pytest.mark.gpu
+def test_env_batch_entries_equal_single_renders(gpu_ctx):
+    # A batch mixing a scene lit by a map (with and without an area light) and
+    # a Cornell room without one: each entry's film equals its own render, so
+    # the map kernels run only for the entries that hold one.
+    sa_env, cam_env, prm_env, img = _env_scene(True)
+    sa_sky, cam_sky, prm_sky, _ = _env_scene(False)
+    sa_box, cam_box, prm_box = g.generate_scene(g.GY_SCENE_CORNELL)
+    cam_box.width = cam_box.height = 48
+    entries = [(sa_env, cam_env, prm_env, img), (sa_box, cam_box, prm_box, None), (sa_sky, cam_sky, prm_sky, img * 0.5)]
+    scenes, refs = [], []
+    try:
+        for sa, cam, prm, env in entries:
+            sc = g.Scene(gpu_ctx, sa)
+            if env is not None:
+                sc.set_environment(env)
+            scenes.append(sc)
+            prm.spp = 4
+            prm.mode = g.GY_RENDER_WAVEFRONT
+            prm.samples_per_pass = prm.spp
+            refs.append(sc.render(cam, prm))
+        with g.Batch(gpu_ctx, lanes_per_pass=30000) as b:
+            ids = [b.add(sc, cam, prm) for sc, (_, cam, prm, _) in zip(scenes, entries)]
+            b.render()
+            for i, e in enumerate(ids):
+                color, weight = b.film(e)
+                assert np.array_equal(weight, refs[i][1]), i
+                assert np.array_equal(color, refs[i][0]), i
+    finally:
+        for sc in scenes:
+            sc.close()
