@@ -4,12 +4,35 @@
  * TiledIntegrator, Sobol sampling with per-pixel enumeration, the pinhole
  * camera, the 12-bucket SAH BVH and brute-force closest/any hit, the
  * Lambert / GGX conductor / smooth and rough dielectric BSDFs, NEE + MIS to
- * triangle area lights, and box-filter film accumulation).  Each function in
+ * triangle area lights and an environment map, and box-filter film
+ * accumulation).  Each function in
  * gy_oracle.cpp cites the reference file:line it follows.
  *
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
  * load this library, and only as the checker or the timed CPU baseline; the
  * product path never calls it.
+ *
+ * What pins each part (the checks in tests/, none of which retypes the
+ * formula under test):
+ *   Sobol + enumeration  dim-0 = bit reversal, (0,2)-net, brute-force
+ *                        enumeration, the reference's own tables (test_oracle_pins.py)
+ *   Halton               base-2 bit reversal, Faure permutations' known values,
+ *                        radical inverse, CRT strata (test_oracle_pins.py)
+ *   camera, matrices     closed-form rays; the reference's MatrixTest known answers
+ *   BVH + traversal      brute force over the same triangle test (level a)
+ *   BSDFs, Fresnel, GGX  furnace, normalisation, chi-square of Sample vs Pdf,
+ *                        Fresnel closed forms (test_oracle_pins.py)
+ *   textures             affine and clamp closed forms
+ *   integrator           closed-form furnace box; FP64 16k-spp image (level c)
+ *   environment light    lat-long orientation closed forms, Sample pdf = PdfDir,
+ *                        chi-square of the 2-D inversion, uniform-map sphere
+ *                        sampling, constant-sky furnace a Le cos^2(pi / 2h)
+ *                        (test_env.py)
+ *   OBJ, image files     the reference's own tinyobjloader and stb built into
+ *                        oracle/_ref (test_obj.py, test_image.py)
+ * Parity unpinned: the MSVC CRT's last-ulp behaviour of sin/cos/acos/atan2/
+ * log/pow (here: double evaluation rounded once) and the TBB-ordered sums at
+ * film border pixels (here: a fixed order).
  *
  * The oracle runs in two precisions:
  *   OR_FP32  bit-faithful float mode (SSE-order dot products, no FMA,
