@@ -415,110 +415,6 @@ GY_D bool traverse(const SceneDev& S, V3 o, V3 d, float tmin, float tmax, Hit* h
     return true;
 }
 
-// Traversal for the wavefront trace kernels: the same node tests, triangle
-// gating and tie rule as traverse(), with a short per-lane stack of S entries in
-// LDS (column layout, lds[i * stride]).  Deeper entries spill to a per-lane
-// global area (spill[i * spill_stride], entries below the LDS ring); the ring
-// keeps the newest S entries, so pops come from LDS until it empties.
-template <bool ANY, int S>
-GY_D bool traverse_short(const SceneDev& Sc, V3 o, V3 d, float tmin, float tmax, Hit* hit, uint32_t* lds, int stride,
-                         uint32_t* __restrict__ spill, int64_t spill_stride) {
-    const V3 inv = v3(1.f / d.x, 1.f / d.y, 1.f / d.z);
-    float best = tmax;
-    int32_t best_prim = -1;
-    float bb1 = 0.f, bb2 = 0.f;
-    uint32_t head = 0, nlds = 0, nspill = 0;
-    uint32_t node = 0;
-    while (true) {
-        const uint4* np = Sc.nodes + (size_t)node * 4;
-        const uint4 q0 = np[0], q1 = np[1], q2 = np[2], q3 = np[3];
-        const float b0[6] = {__uint_as_float(q0.x), __uint_as_float(q0.y), __uint_as_float(q0.z),
-                             __uint_as_float(q0.w), __uint_as_float(q1.x), __uint_as_float(q1.y)};
-        const float b1[6] = {__uint_as_float(q1.z), __uint_as_float(q1.w), __uint_as_float(q2.x),
-                             __uint_as_float(q2.y), __uint_as_float(q2.z), __uint_as_float(q2.w)};
-        const float t0 = (q3.x != 0xffffffffu) ? box_enter(b0, o, inv, tmin, best) : kInf;
-        const float t1 = (q3.y != 0xffffffffu) ? box_enter(b1, o, inv, tmin, best) : kInf;
-        uint32_t next[2];
-        float tn[2];
-        int nn = 0;
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-            const float tc = c == 0 ? t0 : t1;
-            if (tc == kInf) continue;
-            const uint32_t ref = c == 0 ? q3.x : q3.y;
-            const uint32_t cnt = c == 0 ? q3.z : q3.w;
-            if (cnt == 0) {
-                next[nn] = ref;
-                tn[nn] = tc;
-                ++nn;
-                continue;
-            }
-            for (uint32_t k = 0; k < cnt; ++k) {
-                const float4* tp = Sc.tri + (size_t)(ref + k) * 3;
-                const float4 a = tp[0], pb = tp[1], pc = tp[2];
-                float th, u, v;
-                if (ref_triangle(ld3(a), ld3(pb), ld3(pc), o, d, inv, tmin, tmax, best, &th, &u, &v)) {
-                    const int32_t prim = __float_as_int(a.w);
-                    if (ANY) {
-                        hit->prim = prim;
-                        return true;
-                    }
-                    if (th < best || prim > best_prim) {
-                        best = th;
-                        best_prim = prim;
-                        bb1 = u;
-                        bb2 = v;
-                    }
-                }
-            }
-        }
-        if (nn == 2 && tn[1] < tn[0]) {
-            const uint32_t tr = next[0];
-            next[0] = next[1];
-            next[1] = tr;
-            const float tt = tn[0];
-            tn[0] = tn[1];
-            tn[1] = tt;
-        }
-        if (nn == 2 && tn[1] <= best) {
-            // push next[1]
-            if (nlds == (uint32_t)S) {
-                spill[(int64_t)nspill * spill_stride] = lds[head * stride];
-                ++nspill;
-                head = head + 1 == (uint32_t)S ? 0 : head + 1;
-                --nlds;
-            }
-            uint32_t pos = head + nlds;
-            if (pos >= (uint32_t)S) pos -= S;
-            lds[pos * stride] = next[1];
-            ++nlds;
-        }
-        if (nn >= 1 && tn[0] <= best) {
-            node = next[0];
-            continue;
-        }
-        // pop
-        if (nlds > 0) {
-            --nlds;
-            uint32_t pos = head + nlds;
-            if (pos >= (uint32_t)S) pos -= S;
-            node = lds[pos * stride];
-        } else if (nspill > 0) {
-            --nspill;
-            node = spill[(int64_t)nspill * spill_stride];
-        } else {
-            break;
-        }
-    }
-    if (ANY) return false;
-    if (best_prim < 0) return false;
-    hit->t = best;
-    hit->prim = best_prim;
-    hit->b1 = bb1;
-    hit->b2 = bb2;
-    return true;
-}
-
 // Resumable traversal state for the persistent trace kernels (trav_run), so
 // that a lane can be refilled with a new ray when its ray finishes.  Same node
 // tests, triangle gating and tie rule as traverse().
