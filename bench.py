@@ -89,6 +89,64 @@ def open_room_sky(g, sa, w=1024, h=512):
     return out, img
 
 
+def lt_main(args):
+    """Light tracing (gy_render_light_tracing) on the icosphere workload: the
+    same scene, camera, spp and depth; one light path per (pixel, sample)
+    index, every vertex connected to the camera and splatted."""
+    import gyunity_amd as g
+
+    sa, cam, prm = g.generate_scene(g.GY_SCENE_ICOSPHERE_GGX, 0, args.level)
+    paths = cam.width * cam.height * prm.spp
+    ctx = g.Context(0)
+    log("device:", ctx.device_name())
+    sc = g.Scene(ctx, sa)
+    for i in range(args.warmup):
+        sc.render_light_tracing(cam, prm)
+        log("lt warmup %d done" % i)
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        img = sc.render_light_tracing(cam, prm)
+        log("lt step %d done" % i)
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    value = paths / (ms / 1e3) / 1e6
+    out = {
+        "metric": "Mpaths/s at fixed RMSE: batched wavefront vs plain megakernel; adjoint s/grad",
+        "value": round(value, 3),
+        "unit": "Mpaths/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: procedural displaced icosphere (seed 0) in the Cornell room",
+        "config": {
+            "workload": "light tracing, icosphere-L%d GGX conductor, %d triangles, %dx%d, %d light paths per pixel, depth %d"
+            % (args.level, sa.n_tri, cam.width, cam.height, prm.spp, prm.max_depth),
+            "integrator": "light tracing (one path per thread, float-atomic splats)",
+            "paths_per_step": paths,
+            "finite_fraction": float(np.isfinite(img).mean()),
+            "parallelism": "single GPU",
+        },
+    }
+    sc.close()
+    ctx.close()
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+
+        log("cpu baseline ...")
+        small = g.camera(tuple(cam.pos), tuple(cam.look_at), tuple(cam.up), cam.fovy, 128, 128, cam.film_dist)
+        os_ = O.OracleScene(sa)
+        t0 = time.perf_counter()
+        os_.render_lt(small, O.params(4, prm.max_depth, prm.rr_depth, 0, 1, 1))
+        secs = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(128 * 128 * 4 / secs / 1e6, 4), "unit": "Mpaths/s", "cores": 1, "kind": "port",
+                               "sample": "128x128 image, 4 light paths per pixel (65536 paths, %.1f s), same scene and depth" % secs}
+    print(json.dumps(out), flush=True)
+
+
 def batch_main(args):
     """The 64-scene batch workload: every member traced in one lane space
     (gy_batch_render, recorded into a hipGraph once and replayed), against the
@@ -373,7 +431,7 @@ def main():
     ap.add_argument("--refill", type=int, default=0, help="wavefront refill threshold (0 = library default)")
     ap.add_argument("--spp-pass", type=int, default=0, help="samples per pass (0 = library default)")
     ap.add_argument("--leaf", type=int, default=0, help="BVH leaf size (0 = library default, 4)")
-    ap.add_argument("--workload", default="icosphere", choices=["icosphere", "env", "batch64", "adjoint", "adjoint-scale"],
+    ap.add_argument("--workload", default="icosphere", choices=["icosphere", "env", "lt", "batch64", "adjoint", "adjoint-scale"],
                     help="icosphere: the 1.31M-triangle GGX render; batch64: the 64-scene batch; adjoint: "
                          "gradient of the textured Cornell room; adjoint-scale: Adam on 4.2M texels over 32 views")
     ap.add_argument("--fd-texels", type=int, default=0, help="adjoint: texels checked against finite differences")
@@ -401,6 +459,9 @@ def main():
         return
     if args.workload == "adjoint-scale":
         adjoint_scale_main(args)
+        return
+    if args.workload == "lt":
+        lt_main(args)
         return
 
     import gyunity_amd as g

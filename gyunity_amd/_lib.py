@@ -186,6 +186,7 @@ def lib():
         "gy_image_read": (C.c_int, [C.c_char_p, P(i32), P(i32), vp, i64]),
         "gy_film_save": (C.c_int, [vp, C.c_char_p]),
         "gy_scene_set_environment": (C.c_int, [vp, vp, C.c_int32, C.c_int32, C.c_float]),
+        "gy_render_light_tracing": (C.c_int, [vp, P(CameraDesc), P(RenderParams), vp]),
         "gy_mesh_load_obj": (C.c_int, [C.c_char_p, i32, P(f32), P(f32), P(vp)]),
         "gy_mesh_triangles": (i64, [vp]),
         "gy_mesh_arrays": (C.c_int, [vp, P(f32), P(f32), P(f32), P(f32)]),
@@ -513,6 +514,12 @@ class Scene:
         if rgb.ndim != 3 or rgb.shape[2] != 3:
             raise GyError(1, "set_environment: rgb must be (height, width, 3)")
         check(lib().gy_scene_set_environment(self.h, rgb.ctypes.data, rgb.shape[1], rgb.shape[0], float(scale)))
+
+    def render_light_tracing(self, cam, params):
+        """LightTracing::Render (gy_render_light_tracing): the (H, W, 3) splat sums."""
+        out = np.zeros((cam.height, cam.width, 3), dtype=np.float32)
+        check(lib().gy_render_light_tracing(self.h, C.byref(cam), C.byref(params), out.ctypes.data))
+        return out
 
     def save_film(self, path):
         """Film::SaveImage of the last render: .hdr, .bmp or .png by suffix."""

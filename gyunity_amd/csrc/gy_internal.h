@@ -60,6 +60,15 @@ struct CameraRT {
     int32_t log2res;
 };
 int BuildCamera(const gy_camera_desc& d, CameraRT* out);
+// The pinhole camera as LightTracing sees it: WorldToRaster = CameraToRaster *
+// WorldToCamera (integrator/LT.h:118), the film distance and film area
+// (camera/Camera.cpp:42-44) for PinHoleCamera::We.
+struct LightCameraRT {
+    float world_to_raster[16];  // column-major, (i, j) at [j*4 + i]
+    float pos[3], cz[3];
+    float dis, area;
+};
+int BuildLightCamera(const gy_camera_desc& d, LightCameraRT* out);
 
 // BVH in the device layout: 16 words per node.
 //   words 0-5   child 0 AABB (min xyz, max xyz)

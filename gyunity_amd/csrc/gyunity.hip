@@ -253,6 +253,16 @@ __global__ void __launch_bounds__(128) k_li_samples(SceneDev S, PathParams P, in
     dims[i] = (int32_t)nd;
 }
 
+// LightTracing: one light path per thread, sample index = thread index.
+template <int STACK>
+__global__ void __launch_bounds__(128) k_lt(SceneDev S, PathParams P, LightCamDev C, int64_t n, float total,
+                                            float* __restrict__ splat) {
+    __shared__ uint32_t stack[STACK * 128];
+    const int64_t t = (int64_t)blockIdx.x * 128 + threadIdx.x;
+    if (t >= n) return;
+    lt_path<STACK>(S, P, C, (uint64_t)t, total, splat, stack + threadIdx.x, 128);
+}
+
 // ------------------------------------------------------------------ film
 // Each pixel's owner thread adds the pass's own samples in sample order
 // (Film::AddSample with weight 1: weight += 1, color = color + L, visual/Film.h:80-85).
@@ -475,6 +485,10 @@ int SceneUpload(Ctx* c, const gy_scene_desc& d, const gy_bvh_params* bp, Scene**
     sc->area_le.assign(d.light_le, d.light_le + 3 * (size_t)nl);
     std::vector<float> func = sc->area_func, cdf;
     const float func_int = LightCdf(func, &cdf);
+    for (int i = 0; i < d.n_mat; ++i) {
+        const uint32_t t = d.mats[(size_t)i * GY_MAT_WORDS];
+        if (t == 2 || t == 3) sc->has_transmission = true;
+    }
     sc->dev.light_func_int = func_int;
     sc->dev.n_light = nl;
     sc->dev.n_tri = d.n_tri;
@@ -966,6 +980,60 @@ extern "C" int gy_scene_set_environment(gy_scene* s, const float* rgb, int32_t w
     sc->dev.light_func_int = func_int;
     sc->film_valid = false;
     return GY_OK;
+}
+
+// LightTracing::Render (integrator/LT.h:17-53): spp light paths per pixel
+// index, rgb gets the film's splat sums (W x H x 3; the film has no samples,
+// so they are its pixels).  The splats are added with float atomics, so the
+// summation order, not the per-path values, varies between runs.
+extern "C" int gy_render_light_tracing(gy_scene* s, const gy_camera_desc* cam, const gy_render_params* p, float* rgb) {
+    if (!s || !rgb) return Fail(GY_ERR_INVALID, "gy_render_light_tracing: null argument");
+    int rc = ValidateRender(cam, p);
+    if (rc != GY_OK) return rc;
+    if (p->max_depth < 1) return Fail(GY_ERR_INVALID, "gy_render_light_tracing: max_depth >= 1");
+    Scene* sc = reinterpret_cast<Scene*>(s);
+    Ctx* c = sc->ctx;
+    (void)hipSetDevice(c->device);
+    if (sc->area_tri.empty() || sc->dev.env.w > 0)
+        return Fail(GY_ERR_UNSUPPORTED, "gy_render_light_tracing: area lights only (no environment light)");
+    if (sc->has_transmission)
+        return Fail(GY_ERR_UNSUPPORTED, "gy_render_light_tracing: dielectric materials need the importance transport mode");
+    const int64_t n = (int64_t)cam->width * cam->height * p->spp;
+    if (n >= (1ll << 36)) return Fail(GY_ERR_UNSUPPORTED, "gy_render_light_tracing: more than 2^36 light paths");
+    if (sc->stats.max_depth + 1 > 64) return Fail(GY_ERR_UNSUPPORTED, "gy_render_light_tracing: BVH deeper than the traversal stack");
+    PathParams P;
+    rc = MakePathParams(c, sc, *cam, *p, &P);
+    if (rc != GY_OK) return rc;
+    LightCameraRT lc;
+    rc = BuildLightCamera(*cam, &lc);
+    if (rc != GY_OK) return rc;
+    LightCamDev C;
+    for (int i = 0; i < 16; ++i) C.m[i] = lc.world_to_raster[i];
+    for (int i = 0; i < 3; ++i) {
+        C.pos[i] = lc.pos[i];
+        C.cz[i] = lc.cz[i];
+    }
+    C.dis = lc.dis;
+    C.area = lc.area;
+    C.width = cam->width;
+    C.height = cam->height;
+    const int64_t px = (int64_t)cam->width * cam->height * 3;
+    void* d = nullptr;
+    rc = DevAlloc(c, &d, (size_t)px * 4);
+    if (rc == GY_OK) rc = Fill32(c, d, 0u, px);
+    if (rc == GY_OK) {
+        const unsigned blocks = (unsigned)((n + 127) / 128);
+        auto kern = sc->stack <= 32 ? k_lt<32> : k_lt<64>;
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(128), 0, c->stream, sc->dev, P, C, n, (float)n, (float*)d);
+        rc = CheckLaunch(c, "k_lt");
+    }
+    if (rc == GY_OK) {
+        hipError_t e = hipMemcpyAsync(rgb, d, (size_t)px * 4, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = HipFail(e, "gy_render_light_tracing: download");
+    }
+    DevFree(d);
+    return rc;
 }
 
 extern "C" int gy_scene_destroy(gy_scene* s) {

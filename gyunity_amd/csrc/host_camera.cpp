@@ -57,6 +57,48 @@ int BuildCamera(const gy_camera_desc& d, CameraRT* out) {
     return GY_OK;
 }
 
+int BuildLightCamera(const gy_camera_desc& d, LightCameraRT* out) {
+    CameraRT rt;
+    int rc = BuildCamera(d, &rt);  // validates the description
+    if (rc != GY_OK) return rc;
+    const V3 pos{d.pos[0], d.pos[1], d.pos[2]}, look{d.look_at[0], d.look_at[1], d.look_at[2]},
+        up{d.up[0], d.up[1], d.up[2]};
+    const V3 cz = Neg(Norm(Sub(pos, look)));
+    const V3 cx = Neg(Norm(Cross(up, cz)));
+    const V3 cy = Neg(Cross(cz, cx));
+    const float c2w_r[16] = {cx.x, cy.x, cz.x, pos.x, cx.y, cy.y, cz.y, pos.y, cx.z, cy.z, cz.z, pos.z, 0, 0, 0, 1.f};
+    const float aspect = (float)d.width / (float)d.height;
+    const float PI = 3.14159265358979f;
+    const float radians = (PI / 180) * d.fovy;
+    const float dis = d.film_dist;
+    const float t = dis * std::tan(radians * 0.5);
+    const float r = t * aspect;
+    const float f = 100000.f, n = dis;
+    const float persp_r[16] = {dis / r, 0, 0, 0, 0, dis / t, 0, 0, 0, 0, (f + n) / (f - n), -2 * f * n / (f - n), 0, 0, 1, 0};
+    // NDCToRaster = Scale(W, H, 1) * Scale(0.5, -0.5, 1) * Translate(1, -1, 0)
+    const float W = (float)d.width, H = (float)d.height;
+    const float s1_r[16] = {W, 0, 0, 0, 0, H, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    const float s2_r[16] = {0.5f, 0, 0, 0, 0, -0.5f, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    const float tr_r[16] = {1, 0, 0, 1, 0, 1, 0, -1, 0, 0, 1, 0, 0, 0, 0, 1};
+    const M4 ndc_to_raster = Mul(Mul(Rows(s1_r), Rows(s2_r)), Rows(tr_r));
+    const M4 cam_to_raster = Mul(ndc_to_raster, Rows(persp_r));
+    const M4 w2r = Mul(cam_to_raster, Inverse(Rows(c2w_r)));
+    for (int j = 0; j < 4; ++j)
+        for (int i = 0; i < 4; ++i) out->world_to_raster[j * 4 + i] = w2r.c[j][i];
+    out->pos[0] = pos.x;
+    out->pos[1] = pos.y;
+    out->pos[2] = pos.z;
+    out->cz[0] = cz.x;
+    out->cz[1] = cz.y;
+    out->cz[2] = cz.z;
+    out->dis = dis;
+    // mHeight = d * tan(fovy * 0.5 * PI / 180) * 2 (tan evaluated in double), mWidth = mHeight * aspect
+    const float height = dis * (float)std::tan((double)(d.fovy * 0.5f * PI / 180)) * 2.f;
+    const float width = height * aspect;
+    out->area = width * height;
+    return GY_OK;
+}
+
 }  // namespace gy
 
 extern "C" int gy_camera_matrix(const gy_camera_desc* cam, float* out16) {

@@ -168,6 +168,133 @@ __device__ V3 trace_path(const SceneDev& S, const PathParams& P, int px, int py,
     return L;
 }
 
+// ------------------------------------------------------------------ light tracing
+// LightTracing (integrator/LT.h:17-241) for one light path: the sample index
+// W H p + y W + x drives the sequence from dimension 0.  Each vertex is
+// connected to the camera and splatted as soon as it exists: the reference
+// connects after the whole path is built, but the only random numbers it draws
+// there (PinHoleCamera::Sample_Wi's) are unused and drawn last, so the order
+// changes nothing.  Area lights and transmission-free materials only.
+struct LightCamDev {
+    float m[16];  // WorldToRaster, column-major
+    float pos[3], cz[3];
+    float dis, area;
+    int32_t width, height;
+};
+// Transform::operator()(point) (math/Transform.h:106-115) with the column
+// order of Matrix4 * Vector4 (math/Linagl.h:937-948).
+__device__ inline V3 xf_point(const float* m, V3 p) {
+    const float in[4] = {p.x, p.y, p.z, 1.0f};
+    float r[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) r[i] = m[12 + i] * in[3];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) r[i] = m[8 + i] * in[2] + r[i];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) r[i] = m[4 + i] * in[1] + r[i];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) r[i] = m[0 + i] * in[0] + r[i];
+    V3 q = v3(r[0], r[1], r[2]);
+    if (!(r[3] == 1.f || r[3] == 0.f)) q = q / r[3];
+    return q;
+}
+// CoordinateSystem (math/GeometryUtils.h:11-17)
+__device__ inline void coordinate_system(V3 v1, V3* v2, V3* v3o) {
+    if (fabsf(v1.x) > fabsf(v1.y)) *v2 = v3(-v1.z, 0.f, v1.x) / sqrtf(v1.x * v1.x + v1.z * v1.z);
+    else *v2 = v3(0.f, v1.z, -v1.y) / sqrtf(v1.y * v1.y + v1.z * v1.z);
+    *v3o = cross(v1, *v2);
+}
+// ConnectToCamera (integrator/LT.h:135-183) + WorldToScreen (:116-129) +
+// Film::AddSplat (visual/Film.h:89-103) of one vertex.  s0: the light vertex,
+// whose splat is its throughput (Le) as the reference returns it.
+template <int STACK>
+__device__ void lt_connect(const SceneDev& S, const LightCamDev& C, float* splat, V3 pos, V3 n, V3 outDir, const BSDF* b,
+                           V3 thr, bool s0, float total, uint32_t* stack, int stride) {
+    const V3 cpos = v3(C.pos[0], C.pos[1], C.pos[2]), cz = v3(C.cz[0], C.cz[1], C.cz[2]);
+    V3 toCam = cpos - pos;
+    const float len = length(toCam);
+    toCam = norm(toCam);
+    if (dot(cz, toCam * -1.f) < 0.f) return;
+    const V3 rp = xf_point(C.m, pos);
+    const int ix = (int)rp.x, iy = (int)rp.y;
+    if (!(ix >= 0 && ix <= C.width && iy >= 0 && iy <= C.height)) return;
+    Hit hv;
+    if (traverse<true, STACK>(S, pos + toCam * kRayEps, toCam, 0.f, len, &hv, stack, stride, nullptr, nullptr)) return;
+    V3 L;
+    if (s0) {
+        L = thr;
+    } else {
+        // PinHoleCamera::Sample_Wi / We (camera/PinholeCamera.h:31-49)
+        V3 wi = cpos - pos;
+        const float distance = length(wi);
+        wi = norm(wi);
+        const float cosT = dot(cz, wi * -1.f);
+        const float pdfW = 1.f * (distance * distance) / cosT;
+        const V3 wdir = wi * -1.f;
+        const float cw = dot(cz, wdir);
+        const float c2 = cw * cw;
+        const float We = C.dis * C.dis * 1.0f / (C.area * c2 * c2);
+        const V3 f = bsdf_eval(*b, outDir, wi);
+        L = v3s(We) * f * thr * fabsf(dot(wi, n)) / pdfW;
+    }
+    L = ((L * (float)C.width) * (float)C.height) / total;  // L * resX * resY / totalPhotons (integrator/LT.h:40)
+    int X = (int)floorf((float)ix), Y = (int)floorf((float)iy);
+    X = X < 0 ? 0 : (X > C.width - 1 ? C.width - 1 : X);
+    Y = Y < 0 ? 0 : (Y > C.height - 1 ? C.height - 1 : Y);
+    float* px = splat + ((size_t)Y * C.width + X) * 3;
+    atomicAdd(px, L.x);
+    atomicAdd(px + 1, L.y);
+    atomicAdd(px + 2, L.z);
+}
+// GenerateLightPath + Trace (integrator/LT.h:60-113) with the connections.
+template <int STACK>
+__device__ void lt_path(const SceneDev& S, const PathParams& P, const LightCamDev& C, uint64_t index, float total,
+                        float* splat, uint32_t* stack, int stride) {
+    const SampleSrc& M = P.smp;
+    Seq q;
+    q.index = index;
+    q.cursor = 0;
+    float pdfLight;
+    const int light = sample_one_light(S, seq_next(M, q), &pdfLight);
+    const int ltri = S.light_tri[light];
+    const V3 Le = v3(S.light_le[3 * light], S.light_le[3 * light + 1], S.light_le[3 * light + 2]);
+    const float u0 = seq_next(M, q), u1 = seq_next(M, q);
+    const float w0 = seq_next(M, q), w1 = seq_next(M, q);
+    // AreaLight::SampleOnLight (light/AreaLight.h:60-71), Triangle::Sample (shape/Triangle.cpp:141-158)
+    const float4* tq = S.shade + (size_t)ltri * 8;
+    const V3 p0 = ld3(tq[0]), p1 = ld3(tq[1]), p2 = ld3(tq[2]);
+    const V3 n0 = ld3(tq[3]), n1 = ld3(tq[4]), n2 = ld3(tq[5]);
+    const float su0 = sqrtf(u0);
+    const float b0 = 1 - su0, b1 = u1 * su0;
+    const V3 lpos = (b0 * p0 + b1 * p1) + (1 - b0 - b1) * p2;
+    const V3 ln = (b0 * n0 + b1 * n1) + (1 - b0 - b1) * n2;
+    const float pdfA = 1 / tq[2].w;
+    V3 ss, ts;
+    coordinate_system(ln, &ss, &ts);
+    const V3 dl = cosine_hemisphere(w0, w1);
+    const V3 dir = norm((ss * dl.x + ts * dl.y) + ln * dl.z);
+    float pdfDir = dl.z * kInvPI;
+    lt_connect<STACK>(S, C, splat, lpos, ln, v3s(0.f), nullptr, Le, true, total, stack, stride);
+    V3 T = Le * fabsf(dot(ln, dir)) / pdfLight / pdfA / pdfDir;
+    V3 o = (lpos + dir * kRayEps) + dir * kRayEps;
+    V3 d = dir;
+    for (int bounce = 1;; ) {
+        Hit h;
+        if (!traverse<false, STACK>(S, o, d, 0.f, kInf, &h, stack, stride, nullptr, nullptr)) break;
+        const Isect it = query(S, o, d, h);
+        const BSDF b = make_bsdf(S, it);
+        lt_connect<STACK>(S, C, splat, it.pos, it.n, it.outDir, &b, T, false, total, stack, stride);
+        ++bounce;
+        if (bounce >= P.max_depth) break;
+        const float r0 = seq_next(M, q), r1 = seq_next(M, q), r2 = seq_next(M, q);
+        V3 wi;
+        const V3 f = bsdf_sample(b, neg(d), &wi, &pdfDir, r0, r1, r2);
+        o = (it.pos + wi * kRayEps) + wi * kRayEps;
+        d = wi;
+        T = T * f * fabsf(dot(wi, it.n)) / pdfDir;
+    }
+}
+
 // Film::AddSample footprint with the radius-0.5 box filter (visual/Film.h:61-74).
 __device__ inline void film_footprint(float fx, float fy, int W, int H, int* x0, int* x1, int* y0, int* y1) {
     const float x = fx - 0.5f, y = fy - 0.5f;

@@ -242,6 +242,14 @@ uint64_t gy_halton_enum_index(uint32_t width, uint32_t height, uint64_t i, uint3
 int gy_image_write(const char* path, const float* rgb, int32_t width, int32_t height);
 int gy_image_to_bytes(const float* rgb, int64_t n, uint8_t* out);
 int gy_film_save(gy_scene* scene, const char* path);
+/* LightTracing (integrator/LT.h:11-241): LightTracing(sampler, maxDepth, spp)
+ * ::Render with p->spp, p->max_depth and p->sampler.  Light paths from the
+ * area lights, every vertex connected to the pinhole camera and splatted
+ * (Film::AddSplat, visual/Film.h:89-103); rgb gets the W x H x 3 splat sums.
+ * Area lights and transmission-free materials only (GY_ERR_UNSUPPORTED
+ * otherwise).  Float atomics add the splats: the per-path values are fixed,
+ * their summation order is not. */
+int gy_render_light_tracing(gy_scene* scene, const gy_camera_desc* cam, const gy_render_params* p, float* rgb);
 /* Reads a Radiance .hdr file as stbi_loadf does for ImageIO::LoadTexture
  * (image/ImageIO.cpp:100-114), the loader of EnvironmentMap(filename): rgb gets
  * width*height*3 floats, top row first.  With rgb NULL only the size is

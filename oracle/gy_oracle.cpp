@@ -1680,6 +1680,147 @@ template <class R> V3<R> camera_sample(const Scene<R>& s, const Camera<R>& cam, 
     return L;
 }
 
+// ---------------------------------------------------------------- light tracing
+// LightTracing (integrator/LT.h:17-241): one light path per (x, y, p) sample
+// index W H p + y W + x, every vertex connected to the pinhole camera and
+// splatted with Film::AddSplat (visual/Film.h:89-103).  Area lights only;
+// materials without transmission (their BSDFs do not depend on the transport
+// mode).  The dimensions of the Vec2(rand(), rand()) / Vec3(rand(), rand(),
+// rand()) arguments are drawn left to right, as for PathTracing.
+template <class R> struct LTCamera {
+    V3<R> pos, cz;
+    M4<R> world_to_raster;  // CameraToRaster * WorldToCamera (integrator/LT.h:118-119)
+    R dis, area;            // film distance, film area (camera/Camera.cpp:42-44)
+    int w, h;
+};
+template <class R> LTCamera<R> make_lt_camera(const or_camera_desc& d) {
+    LTCamera<R> c;
+    V3<R> pos((R)d.pos[0], (R)d.pos[1], (R)d.pos[2]);
+    V3<R> look((R)d.look_at[0], (R)d.look_at[1], (R)d.look_at[2]);
+    V3<R> up((R)d.up[0], (R)d.up[1], (R)d.up[2]);
+    V3<R> cz = -norm(pos - look);
+    V3<R> cx = -norm(cross(up, cz));
+    V3<R> cy = -cross(cz, cx);
+    M4<R> c2w = M4<R>::rows(cx.x, cy.x, cz.x, pos.x, cx.y, cy.y, cz.y, pos.y, cx.z, cy.z, cz.z, pos.z, 0, 0, 0, (R)1.f);
+    R fovy = (R)d.fovy, dis = (R)d.film_dist;
+    R aspect = (R)d.width / (R)d.height;
+    R radians = (K<R>::PI / 180) * fovy;
+    R t = dis * std::tan(radians * 0.5);
+    R r = t * aspect;
+    R f = (R)100000.f, n = dis;
+    M4<R> persp = M4<R>::rows(dis / r, 0, 0, 0, 0, dis / t, 0, 0, 0, 0, (f + n) / (f - n), -2 * f * n / (f - n), 0, 0, 1, 0);
+    Xf<R> ndc_to_raster = xf_compose(xf_compose(xf_scale((R)d.width, (R)d.height, (R)1), xf_scale((R)0.5f, (R)-0.5f, (R)1)),
+                                     xf_translate((R)1, (R)-1, (R)0));
+    M4<R> cam_to_raster = mul(ndc_to_raster.m, persp);
+    c.world_to_raster = mul(cam_to_raster, glm_inverse(c2w));
+    c.pos = pos;
+    c.cz = cz;
+    c.dis = dis;
+    // mHeight = d tan(fovy 0.5 PI / 180) 2, mWidth = mHeight aspect (camera/Camera.cpp:42-44)
+    R height = dis * (R)std::tan((double)(fovy * (R)0.5f * K<R>::PI / 180)) * (R)2.f;
+    R width = height * aspect;
+    c.area = width * height;
+    c.w = d.width;
+    c.h = d.height;
+    return c;
+}
+// PinHoleCamera::We (camera/PinholeCamera.h:31-39) for a ray leaving the camera along dir.
+template <class R> inline R lt_we(const LTCamera<R>& c, const V3<R>& dir) {
+    R pdfA = 1.0;
+    R cosTheta = dot(c.cz, dir);
+    R cos2Theta = cosTheta * cosTheta;
+    return c.dis * c.dis * pdfA / (c.area * cos2Theta * cos2Theta);
+}
+// CoordinateSystem (math/GeometryUtils.h:11-17)
+template <class R> inline void coordinate_system(const V3<R>& v1, V3<R>* v2, V3<R>* v3) {
+    if (std::abs(v1.x) > std::abs(v1.y)) *v2 = V3<R>(-v1.z, 0, v1.x) / std::sqrt(v1.x * v1.x + v1.z * v1.z);
+    else *v2 = V3<R>(0, v1.z, -v1.y) / std::sqrt(v1.y * v1.y + v1.z * v1.z);
+    *v3 = cross(v1, *v2);
+}
+template <class R> struct LTVertex {
+    V3<R> pos, n, outDir, throughput;
+    BSDF<R> bsdf;
+};
+// GenerateLightPath + Trace + ConnectToCamera (integrator/LT.h:60-183).  Calls
+// splat(x, y, L) for every vertex that reaches the screen.
+template <class R, class F> void lt_path(const Scene<R>& s, const LTCamera<R>& c, Seq& rnd, int maxDepth, bool use_bvh,
+                                         int64_t total, F splat) {
+    std::vector<LTVertex<R>> path(maxDepth + 1);
+    R pdfLight;
+    const int light = sample_one_light(s, (R)rnd.next(), &pdfLight);
+    const Tri<R>& lt = s.tris[s.light_tri[light]];
+    const V3<R> Le = s.light_le[light];
+    // AreaLight::SampleOnLight (light/AreaLight.h:60-71), Triangle::Sample (shape/Triangle.cpp:141-158)
+    R u0 = (R)rnd.next(), u1 = (R)rnd.next();
+    R v0 = (R)rnd.next(), v1 = (R)rnd.next();
+    R su0 = std::sqrt(u0);
+    R b0 = 1 - su0, b1 = u1 * su0;
+    V3<R> lpos = (b0 * lt.p0 + b1 * lt.p1) + (1 - b0 - b1) * lt.p2;
+    V3<R> ln = (b0 * lt.n0 + b1 * lt.n1) + (1 - b0 - b1) * lt.n2;
+    R pdfA = 1 / lt.area;
+    V3<R> ss, ts;
+    coordinate_system(ln, &ss, &ts);
+    V3<R> dl = cosine_hemisphere(v0, v1);
+    V3<R> dir = norm((ss * dl.x + ts * dl.y) + ln * dl.z);
+    R pdfDir = dl.z * K<R>::INV_PI;
+    path[0].pos = lpos;
+    path[0].n = ln;
+    path[0].throughput = Le;
+    R cosTheta = std::abs(dot(ln, dir));
+    V3<R> T = Le * cosTheta / pdfLight / pdfA / pdfDir;
+    Ray<R> ray = make_ray(lpos + dir * K<R>::RayEps, dir, K<R>::Inf, (R)0);
+    int bounce = 1;
+    while (true) {
+        Hit<R> h;
+        if (!intersect(s, ray, &h, use_bvh)) break;
+        Isect<R> it = query(s, ray, h);
+        LTVertex<R>& vx = path[bounce];
+        vx.pos = it.pos;
+        vx.n = it.n;
+        vx.outDir = it.outDir;
+        vx.bsdf = make_bsdf(s, it);
+        vx.throughput = T;
+        bounce++;
+        if (bounce >= maxDepth) break;
+        R r0 = rnd.next(), r1 = rnd.next(), r2 = rnd.next();
+        V3<R> wi;
+        V3<R> f = vx.bsdf.Sample(-ray.d, &wi, &pdfDir, V3<R>(r0, r1, r2));
+        ray = make_ray(it.pos + wi * K<R>::RayEps, wi, K<R>::Inf, (R)0);
+        T = T * f * std::abs(dot(wi, it.n)) / pdfDir;
+    }
+    for (int sv = 0; sv < bounce; ++sv) {
+        const LTVertex<R>& vx = path[sv];
+        V3<R> toCam = c.pos - vx.pos;
+        R len = length(toCam);
+        toCam = norm(toCam);
+        if (dot(c.cz, -toCam) < 0) continue;
+        // WorldToScreen (integrator/LT.h:116-129): truncated raster position,
+        // in screen for 0 <= x <= W, 0 <= y <= H
+        V3<R> rp = xf_point(c.world_to_raster, vx.pos);
+        int ix = int(rp.x), iy = int(rp.y);
+        bool in = ix >= 0 && ix <= c.w && iy >= 0 && iy <= c.h;
+        Ray<R> vis = make_ray(vx.pos, toCam, len, (R)0);
+        if (intersect_any(s, vis, use_bvh)) in = false;
+        V3<R> L;
+        if (sv == 0) {
+            L = vx.throughput;
+        } else {
+            // PinHoleCamera::Sample_Wi (camera/PinholeCamera.h:41-49); its
+            // three random numbers are drawn but unused
+            (void)rnd.next(); (void)rnd.next(); (void)rnd.next();
+            V3<R> wi = c.pos - vx.pos;
+            R distance = length(wi);
+            wi = norm(wi);
+            R cosT = dot(c.cz, -wi);
+            R pdfW = (R)1.f * (distance * distance) / cosT;
+            R We = lt_we(c, -wi);
+            V3<R> f = vx.bsdf.Evaluate(vx.outDir, wi);
+            L = V3<R>(We, We, We) * f * vx.throughput * std::abs(dot(wi, vx.n)) / pdfW;
+        }
+        if (in) splat((R)ix, (R)iy, L * (R)c.w * (R)c.h / (R)total);
+    }
+}
+
 // Film::AddSample with the radius-0.5 box filter (visual/Film.h:61-87, math/Filter.h:26-38)
 template <class R> struct Film {
     int W, H;
@@ -2092,6 +2233,38 @@ int or_faure_perm(uint32_t base, uint16_t* out) {
 }
 int or_set_shadow_short(int on) {
     g_shadow_short = on;
+    return 0;
+}
+
+// LightTracing::Render (integrator/LT.h:17-53) in float mode, one thread,
+// rows in order: splat (W x H x 3) receives Film::AddSplat's per-pixel sums.
+int or_render_lt(void* hv, const or_camera_desc* cd, const or_render_params* p, float* splat) {
+    SceneHandle* h = (SceneHandle*)hv;
+    const Scene<float>& sc = h->sf;
+    if (sc.light_tri.empty() || sc.env.w > 0) return -1;
+    for (const Material& m : sc.mats)
+        if (m.type == MAT_GLASS || m.type == MAT_ROUGH_DIELECTRIC) return -2;
+    LTCamera<float> cam = make_lt_camera<float>(*cd);
+    const int W = cd->width, H = cd->height;
+    const int64_t total = (int64_t)p->spp * W * H;
+    for (size_t i = 0; i < (size_t)W * H * 3; ++i) splat[i] = 0.f;
+    auto add = [&](float x, float y, const V3<float>& L) {
+        int X = (int)std::floor(x), Y = (int)std::floor(y);
+        X = X < 0 ? 0 : (X > W - 1 ? W - 1 : X);
+        Y = Y < 0 ? 0 : (Y > H - 1 ? H - 1 : Y);
+        float* px = splat + ((size_t)Y * W + X) * 3;
+        px[0] = px[0] + L.x;
+        px[1] = px[1] + L.y;
+        px[2] = px[2] + L.z;
+    };
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x)
+            for (int smp = 0; smp < p->spp; ++smp) {
+                Seq rnd;
+                rnd.kind = p->sampler == 1 ? 1 : 0;
+                rnd.instance = (uint64_t)W * H * smp + (uint64_t)y * W + x;
+                lt_path(sc, cam, rnd, p->max_depth, p->use_bvh != 0, total, add);
+            }
     return 0;
 }
 

@@ -117,6 +117,9 @@ int or_scene_set_environment(void* scene, const float* rgb, int32_t w, int32_t h
 int or_env_query(void* scene, int32_t n, const float* a, const float* b, const float* c, float* out_rgb,
                  float* out_dir, float* out_pdf, int kind);
 int or_env_info(void* scene, float* radius, float* light_func, int32_t max_lights);
+/* LightTracing (integrator/LT.h) in float mode: splat = W x H x 3 sums of
+ * Film::AddSplat.  -1: no area light or an environment map; -2: a dielectric. */
+int or_render_lt(void* scene, const or_camera_desc* cam, const or_render_params* p, float* splat);
 /* Reference BVH statistics: node count, max depth. */
 int or_bvh_stats(void* scene, int64_t* n_nodes, int32_t* max_depth);
 

@@ -104,6 +104,7 @@ def lib():
         "or_scene_set_environment": (C.c_int, [vp, vp, i32, i32, f32]),
         "or_env_query": (C.c_int, [vp, i32, vp, vp, vp, vp, vp, vp, C.c_int]),
         "or_env_info": (C.c_int, [vp, vp, vp, i32]),
+        "or_render_lt": (C.c_int, [vp, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -271,6 +272,15 @@ class OracleScene:
         f = np.zeros(64, np.float32)
         n = lib().or_env_info(self.h, C.byref(r), _p(f), 64)
         return r.value, f[:n].copy()
+
+    def render_lt(self, cam, prm):
+        """LightTracing (integrator/LT.h) in float mode: the (H, W, 3) splat sums."""
+        c = camera_from(cam)
+        out = np.zeros((cam.height, cam.width, 3), np.float32)
+        rc = lib().or_render_lt(self.h, C.byref(c), C.byref(prm), _p(out))
+        if rc != 0:
+            raise ValueError("or_render_lt: unsupported scene (%d)" % rc)
+        return out
 
     def render(self, cam, prm, row0=0, row1=0):
         c = camera_from(cam)

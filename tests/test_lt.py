@@ -1,0 +1,124 @@
+"""Light tracing: LightTracing (integrator/LT.h) through gy_render_light_tracing.
+
+CPU tests pin the oracle's restatement (the reference build needs TBB and
+MSVC):
+- a scene holding only a light that faces the camera: every light path's own
+  vertex reaches the film unoccluded and nothing else does, so the splat sum
+  is Le * W * H exactly (up to float summation), which pins the light-vertex
+  splat and the L * resX * resY / totalPhotons scale;
+- light tracing and path tracing estimate the same image on a diffuse scene
+  (PinHoleCamera::We / Sample_Wi against the camera ray's measurement): their
+  means over interior blocks agree within the noise of 256 spp;
+- scenes the integrator does not take are refused.
+GPU tests compare the device's splat image with the oracle's: each light
+path's splats are the same values, their float-atomic summation order is not
+(tolerance below).
+"""
+import numpy as np
+import pytest
+
+import gyunity_amd as g
+from oracle import oracle as O
+from tests.scene_util import material, mixed_material_scene
+
+S_REL = 1e-4  # per pixel, relative to max(|L|, 1e-3 of the image's mean)
+
+
+def _light_only(le=(3.0, 2.0, 1.0)):
+    """One quad light at z = 0 facing +z; the camera looks at it from z = 2."""
+    pos = np.array([[-0.3, -0.3, 0, 0.3, -0.3, 0, 0.3, 0.3, 0], [-0.3, -0.3, 0, 0.3, 0.3, 0, -0.3, 0.3, 0]], np.float32)
+    fn = np.array([[0, 0, 1], [0, 0, 1]], np.float32)
+    nrm = np.repeat(fn, 3, axis=0).reshape(-1, 9)
+    sa = g.SceneArrays(pos, nrm, fn, np.zeros(2, np.uint32), np.array([0, 1], np.int32), material(0, R=(0, 0, 0))[None, :],
+                       np.array([0, 1], np.int32), np.array([le, le], np.float32))
+    cam = g.camera((0.0, 0.0, 2.0), (0.0, 0.0, 0.0), (0, 1, 0), 40.0, 40, 32)
+    return sa, cam
+
+
+def _cornell(res=32):
+    sa, cam, prm = g.generate_scene(g.GY_SCENE_CORNELL)
+    cam.width = cam.height = res
+    return sa, cam, prm
+
+
+def test_lt_light_only_closed_form():
+    le = (3.0, 2.0, 1.0)
+    sa, cam = _light_only(le)
+    os_ = O.OracleScene(sa)
+    img = os_.render_lt(cam, O.params(16, 4, 5, 0, 1, 1))
+    total = img.astype(np.float64).sum(axis=(0, 1))
+    assert np.allclose(total, np.array(le) * cam.width * cam.height, rtol=1e-4), total
+    # the splats land on the light's footprint only
+    assert img[0, 0].sum() == 0 and img[cam.height // 2, cam.width // 2].sum() > 0
+
+
+def test_lt_converges_to_path_tracing():
+    sa, cam, _ = _cornell(32)
+    os_ = O.OracleScene(sa)
+    lt = os_.render_lt(cam, O.params(256, 5, 100, 0, 1, 1)).astype(np.float64)
+    c, w, _, _ = os_.render(cam, O.params(256, 5, 100, 0, 8, 1))
+    pt = c / w[..., None]
+    # interior blocks away from the ceiling light and the border pixels (which
+    # collect splats truncated from just outside the image, integrator/LT.h:122-127)
+    for (y0, y1, x0, x1) in [(10, 22, 8, 24), (22, 30, 4, 28), (12, 28, 2, 10)]:
+        r = lt[y0:y1, x0:x1].mean() / pt[y0:y1, x0:x1].mean()
+        assert abs(r - 1) < 0.06, ((y0, y1, x0, x1), r)
+
+
+def test_lt_refuses_unsupported_scenes():
+    sa, cam, prm = mixed_material_scene(res=16)  # glass and rough dielectric
+    with pytest.raises(ValueError):
+        O.OracleScene(sa).render_lt(cam, O.params(1, 3, 5, 0, 1, 1))
+
+
+# ---------------------------------------------------------------- GPU
+
+
+def _compare(gpu, orc):
+    assert np.array_equal(np.isnan(gpu), np.isnan(orc))
+    ok = ~np.isnan(orc)
+    scale = max(np.abs(orc[ok]).mean(), 1e-30)
+    err = np.abs(gpu[ok] - orc[ok]) / np.maximum(np.abs(orc[ok]), 1e-3 * scale)
+    assert np.quantile(err, 0.999) <= S_REL, np.quantile(err, [0.5, 0.99, 0.999, 1.0])
+    return err
+
+
+@pytest.mark.gpu
+def test_lt_gpu_matches_oracle_cornell(gpu_ctx):
+    sa, cam, prm = _cornell(48)
+    prm.spp, prm.max_depth = 16, 5
+    orc = O.OracleScene(sa).render_lt(cam, O.params(16, 5, 5, 0, 1, 1))
+    with g.Scene(gpu_ctx, sa) as sc:
+        gpu = sc.render_light_tracing(cam, prm)
+        gpu2 = sc.render_light_tracing(cam, prm)
+    _compare(gpu, orc)
+    assert np.allclose(gpu2, gpu, rtol=1e-5, atol=1e-7 * np.abs(gpu).max())
+
+
+@pytest.mark.gpu
+def test_lt_gpu_matches_oracle_ggx_and_halton(gpu_ctx):
+    sa, cam, prm = g.generate_scene(g.GY_SCENE_ICOSPHERE_GGX, 0, 3)
+    cam.width = cam.height = 40
+    prm.spp, prm.max_depth, prm.sampler = 8, 6, g.GY_SAMPLER_HALTON
+    orc = O.OracleScene(sa).render_lt(cam, O.params(8, 6, 5, 0, 1, 1, sampler=1))
+    with g.Scene(gpu_ctx, sa) as sc:
+        gpu = sc.render_light_tracing(cam, prm)
+    _compare(gpu, orc)
+
+
+@pytest.mark.gpu
+def test_lt_gpu_light_only_closed_form_and_errors(gpu_ctx):
+    le = (3.0, 2.0, 1.0)
+    sa, cam = _light_only(le)
+    prm = g.render_params(spp=16, max_depth=4, rr_depth=5)
+    with g.Scene(gpu_ctx, sa) as sc:
+        img = sc.render_light_tracing(cam, prm)
+        prm.max_depth = 0
+        with pytest.raises(g.GyError):
+            sc.render_light_tracing(cam, prm)
+    total = img.astype(np.float64).sum(axis=(0, 1))
+    assert np.allclose(total, np.array(le) * cam.width * cam.height, rtol=1e-4), total
+    sa2, cam2, prm2 = mixed_material_scene(res=16)
+    with g.Scene(gpu_ctx, sa2) as sc:
+        with pytest.raises(g.GyError):
+            sc.render_light_tracing(cam2, prm2)
