@@ -778,6 +778,7 @@ struct BSDF {
     GGX dist;
     V3 eta, k;  // conductor eta/k, or F0 in eta when schlick
     float eta1, eta2;
+    bool importance;  // TransportMode::Importance (light paths): no eta^2 scaling of transmission
 };
 GY_D bool bsdf_delta(const BSDF& b) { return b.type == MAT_GLASS; }
 GY_D V3 to_local(const BSDF& b, V3 v) { return v3(dot(v, b.X), dot(v, b.Y), dot(v, b.Z)); }
@@ -790,6 +791,7 @@ GY_D BSDF make_bsdf(const SceneDev& S, const Isect& it) {
     const uint32_t* w = S.mats + (size_t)it.mat * 32;
     BSDF b;
     b.type = (int)w[0];
+    b.importance = false;
     const uint32_t flags = w[1];
     b.schlick = (flags & 2u) != 0;
     b.ng = it.ng;
@@ -874,7 +876,7 @@ GY_D V3 lobe_eval(const BSDF& b, V3 wo, V3 wi) {
     const float D = ggx_D(b.dist, wh);
     const float G = ggx_G(b.dist, wo, wi, wh);
     const float sd = dot(wo, wh) + e * dot(wi, wh);
-    const float factor = 1 / e;
+    const float factor = b.importance ? 1.f : 1 / e;  // (bsdf/BSDF.h:505, 608)
     const float value = (1 - F) * D * G * e * e * fabsf(dot(wi, wh)) * fabsf(dot(wo, wh)) / fabsf(cI * cO * sd * sd);
     return b.Tc * fabsf(value * factor * factor);
 }
@@ -915,6 +917,7 @@ GY_D V3 lobe_sample(const BSDF& b, V3 wo, V3* wi, float* pdf, float r0, float r1
         *wi = w;
         *pdf = 1 - P;
         const float e = etaI / etaT;
+        if (b.importance) return b.Tc * (1.f - Re) / fabsf(w.z);
         return e * e * b.Tc * (1.f - Re) / fabsf(w.z);
     }
     // RoughDielectricBSDF::Sample (bsdf/BSDF.h:545-615)
@@ -954,7 +957,7 @@ GY_D V3 lobe_sample(const BSDF& b, V3 wo, V3* wi, float* pdf, float r0, float r1
     if (cI == 0.f || cO == 0.f) return v3s(0.f);
     const float D = ggx_D(b.dist, wh);
     const float G = ggx_G(b.dist, wo, w, wh);
-    const float factor = 1 / e;
+    const float factor = b.importance ? 1.f : 1 / e;  // (bsdf/BSDF.h:505, 608)
     const float value = (1 - F) * D * G * e * e * fabsf(dot(w, wh)) * fabsf(dot(wo, wh)) / fabsf(cI * cO * sd * sd);
     return b.Tc * fabsf(value * factor * factor);
 }

@@ -485,10 +485,6 @@ int SceneUpload(Ctx* c, const gy_scene_desc& d, const gy_bvh_params* bp, Scene**
     sc->area_le.assign(d.light_le, d.light_le + 3 * (size_t)nl);
     std::vector<float> func = sc->area_func, cdf;
     const float func_int = LightCdf(func, &cdf);
-    for (int i = 0; i < d.n_mat; ++i) {
-        const uint32_t t = d.mats[(size_t)i * GY_MAT_WORDS];
-        if (t == 2 || t == 3) sc->has_transmission = true;
-    }
     sc->dev.light_func_int = func_int;
     sc->dev.n_light = nl;
     sc->dev.n_tri = d.n_tri;
@@ -996,8 +992,6 @@ extern "C" int gy_render_light_tracing(gy_scene* s, const gy_camera_desc* cam, c
     (void)hipSetDevice(c->device);
     if (sc->area_tri.empty() || sc->dev.env.w > 0)
         return Fail(GY_ERR_UNSUPPORTED, "gy_render_light_tracing: area lights only (no environment light)");
-    if (sc->has_transmission)
-        return Fail(GY_ERR_UNSUPPORTED, "gy_render_light_tracing: dielectric materials need the importance transport mode");
     const int64_t n = (int64_t)cam->width * cam->height * p->spp;
     if (n >= (1ll << 36)) return Fail(GY_ERR_UNSUPPORTED, "gy_render_light_tracing: more than 2^36 light paths");
     if (sc->stats.max_depth + 1 > 64) return Fail(GY_ERR_UNSUPPORTED, "gy_render_light_tracing: BVH deeper than the traversal stack");

@@ -174,7 +174,8 @@ __device__ V3 trace_path(const SceneDev& S, const PathParams& P, int px, int py,
 // connected to the camera and splatted as soon as it exists: the reference
 // connects after the whole path is built, but the only random numbers it draws
 // there (PinHoleCamera::Sample_Wi's) are unused and drawn last, so the order
-// changes nothing.  Area lights and transmission-free materials only.
+// changes nothing.  Area lights only; light-path BSDFs use the importance
+// transport mode.
 struct LightCamDev {
     float m[16];  // WorldToRaster, column-major
     float pos[3], cz[3];
@@ -282,7 +283,8 @@ __device__ void lt_path(const SceneDev& S, const PathParams& P, const LightCamDe
         Hit h;
         if (!traverse<false, STACK>(S, o, d, 0.f, kInf, &h, stack, stride, nullptr, nullptr)) break;
         const Isect it = query(S, o, d, h);
-        const BSDF b = make_bsdf(S, it);
+        BSDF b = make_bsdf(S, it);
+        b.importance = true;  // ComputeScatteringFunction(arena, TransportMode::Importance) (integrator/LT.h:69)
         lt_connect<STACK>(S, C, splat, it.pos, it.n, it.outDir, &b, T, false, total, stack, stride);
         ++bounce;
         if (bounce >= P.max_depth) break;

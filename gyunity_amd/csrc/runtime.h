@@ -92,7 +92,6 @@ struct Scene {
     float world_lo[3] = {0, 0, 0}, world_hi[3] = {0, 0, 0};
     std::vector<int32_t> area_tri;  // area lights as created (the environment light follows them)
     std::vector<float> area_le, area_func;
-    bool has_transmission = false;  // a glass or rough dielectric material
     int stack = 64;
     gy_scene_stats stats = {};
     double upload_ms = 0.0;

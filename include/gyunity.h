@@ -246,8 +246,8 @@ int gy_film_save(gy_scene* scene, const char* path);
  * ::Render with p->spp, p->max_depth and p->sampler.  Light paths from the
  * area lights, every vertex connected to the pinhole camera and splatted
  * (Film::AddSplat, visual/Film.h:89-103); rgb gets the W x H x 3 splat sums.
- * Area lights and transmission-free materials only (GY_ERR_UNSUPPORTED
- * otherwise).  Float atomics add the splats: the per-path values are fixed,
+ * Area lights only (GY_ERR_UNSUPPORTED otherwise); dielectrics scatter light
+ * paths in the importance transport mode.  Float atomics add the splats: the per-path values are fixed,
  * their summation order is not. */
 int gy_render_light_tracing(gy_scene* scene, const gy_camera_desc* cam, const gy_render_params* p, float* rgb);
 /* Reads a Radiance .hdr file as stbi_loadf does for ImageIO::LoadTexture

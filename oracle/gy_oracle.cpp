@@ -1101,6 +1101,7 @@ template <class R> struct BSDF {
     int fresnel_schlick;
     V3<R> eta, k, f0;
     R eta1, eta2;
+    bool importance = false;  // TransportMode::Importance (light paths)
     bool delta() const { return type == MAT_GLASS; }
     V3<R> to_local(const V3<R>& v) const { return V3<R>(dot(v, X), dot(v, Y), dot(v, Z)); }
     V3<R> to_world(const V3<R>& v) const { return (X * v.x + Y * v.y) + Z * v.z; }
@@ -1183,7 +1184,7 @@ template <class R> struct BSDF {
                 R D = dist.D(wh);
                 R G = dist.G(wo, wi, wh);
                 R sd = dot(wo, wh) + e * dot(wi, wh);
-                R factor = 1 / e;  // radiance transport
+                R factor = importance ? (R)1 : 1 / e;  // (bsdf/BSDF.h:505)
                 R value = (1 - F) * D * G * e * e * std::abs(dot(wi, wh)) * std::abs(dot(wo, wh)) /
                           std::abs(cI * cO * sd * sd);
                 return Tc * std::abs(value * factor * factor);
@@ -1226,6 +1227,7 @@ template <class R> struct BSDF {
                 *pdf = 1 - P;
                 R c = std::abs(wi->z);
                 R e = etaI / etaT;
+                if (importance) return Tc * ((R)1.f - Re) / c;  // TransportMode::Importance (bsdf/BSDF.h:266-271)
                 return e * e * Tc * ((R)1.f - Re) / c;
             }
             default: {  // RoughDielectricBSDF::Sample (bsdf/BSDF.h:545-615)
@@ -1262,7 +1264,7 @@ template <class R> struct BSDF {
                 if (cI == 0 || cO == 0) return V3<R>();
                 R D = dist.D(wh);
                 R G = dist.G(wo, *wi, wh);
-                R factor = 1 / e;
+                R factor = importance ? (R)1 : 1 / e;  // (bsdf/BSDF.h:608)
                 R value = (1 - F) * D * G * e * e * std::abs(dot(*wi, wh)) * std::abs(dot(wo, wh)) /
                           std::abs(cI * cO * sd * sd);
                 return Tc * std::abs(value * factor * factor);
@@ -1778,7 +1780,8 @@ template <class R, class F> void lt_path(const Scene<R>& s, const LTCamera<R>& c
         vx.pos = it.pos;
         vx.n = it.n;
         vx.outDir = it.outDir;
-        vx.bsdf = make_bsdf(s, it);
+        vx.bsdf = make_bsdf(s, it);  // ComputeScatteringFunction(arena, Importance) (integrator/LT.h:69)
+        vx.bsdf.importance = true;
         vx.throughput = T;
         bounce++;
         if (bounce >= maxDepth) break;
@@ -2242,8 +2245,6 @@ int or_render_lt(void* hv, const or_camera_desc* cd, const or_render_params* p, 
     SceneHandle* h = (SceneHandle*)hv;
     const Scene<float>& sc = h->sf;
     if (sc.light_tri.empty() || sc.env.w > 0) return -1;
-    for (const Material& m : sc.mats)
-        if (m.type == MAT_GLASS || m.type == MAT_ROUGH_DIELECTRIC) return -2;
     LTCamera<float> cam = make_lt_camera<float>(*cd);
     const int W = cd->width, H = cd->height;
     const int64_t total = (int64_t)p->spp * W * H;
@@ -2333,8 +2334,15 @@ int or_bsdf_eval(const uint32_t* mat_words, int32_t n, const double* wo, const d
 }
 
 // Sample n directions for wo with random numbers u (3n); outputs wi (3n), f (3n), pdf (n).
+int or_bsdf_sample_mode(const uint32_t* mat_words, int32_t n, const double* wo, const double* u, double* wi, double* f,
+                        double* pdf, int importance);
 int or_bsdf_sample(const uint32_t* mat_words, int32_t n, const double* wo, const double* u, double* wi, double* f, double* pdf) {
+    return or_bsdf_sample_mode(mat_words, n, wo, u, wi, f, pdf, 0);
+}
+int or_bsdf_sample_mode(const uint32_t* mat_words, int32_t n, const double* wo, const double* u, double* wi, double* f,
+                        double* pdf, int importance) {
     BSDF<double> b = local_bsdf(mat_words);
+    b.importance = importance != 0;
     for (int32_t i = 0; i < n; ++i) {
         V3<double> o(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]), w;
         double p;

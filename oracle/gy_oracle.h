@@ -156,6 +156,9 @@ int or_mat4_inverse(const float* rows16, float* out_rows16);
 int or_mat4_mul_vec(const float* rows16, const float* v4, float* out4);
 int or_bsdf_eval(const uint32_t* mat_words, int32_t n, const double* wo, const double* wi, double* f, double* pdf);
 int or_bsdf_sample(const uint32_t* mat_words, int32_t n, const double* wo, const double* u, double* wi, double* f, double* pdf);
+/* The same in TransportMode::Importance when importance != 0 (light paths). */
+int or_bsdf_sample_mode(const uint32_t* mat_words, int32_t n, const double* wo, const double* u, double* wi, double* f,
+                        double* pdf, int importance);
 double or_fr_dielectric(double cosI, double etaI, double etaT);
 int or_fr_conductor(double cosI, const double* eta, const double* k, double* out);
 double or_ggx_D(double ax, double ay, const double* m);

@@ -105,6 +105,7 @@ def lib():
         "or_env_query": (C.c_int, [vp, i32, vp, vp, vp, vp, vp, vp, C.c_int]),
         "or_env_info": (C.c_int, [vp, vp, vp, i32]),
         "or_render_lt": (C.c_int, [vp, vp, vp, vp]),
+        "or_bsdf_sample_mode": (C.c_int, [vp, i32, vp, vp, vp, vp, vp, C.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -326,6 +327,19 @@ def bsdf_sample(mat_words, wo, u):
     f = np.zeros((n, 3))
     pdf = np.zeros(n)
     lib().or_bsdf_sample(mat_words.ctypes.data, n, wo.ctypes.data, u.ctypes.data, wi.ctypes.data, f.ctypes.data, pdf.ctypes.data)
+    return wi, f, pdf
+
+
+def bsdf_sample_importance(mat_words, wo, u):
+    """bsdf_sample in TransportMode::Importance."""
+    wo = np.ascontiguousarray(wo, np.float64)
+    u = np.ascontiguousarray(u, np.float64)
+    n = wo.shape[0]
+    wi = np.zeros((n, 3))
+    f = np.zeros((n, 3))
+    pdf = np.zeros(n)
+    w = np.ascontiguousarray(mat_words, np.uint32)
+    lib().or_bsdf_sample_mode(_p(w), n, _p(wo), _p(u), _p(wi), _p(f), _p(pdf), 1)
     return wi, f, pdf
 
 

@@ -9,7 +9,11 @@ MSVC):
 - light tracing and path tracing estimate the same image on a diffuse scene
   (PinHoleCamera::We / Sample_Wi against the camera ray's measurement): their
   means over interior blocks agree within the noise of 256 spp;
-- scenes the integrator does not take are refused.
+- light paths scatter in TransportMode::Importance: smooth-glass
+  transmission weights f |cos| / pdf are Tc exactly (radiance mode: e^2 Tc),
+  rough-dielectric transmission values are e^2 times the radiance mode's
+  (bsdf/BSDF.h:266-271, 505, 608);
+- scenes the integrator does not take (an environment light) are refused.
 GPU tests compare the device's splat image with the oracle's: each light
 path's splats are the same values, their float-atomic summation order is not
 (tolerance below).
@@ -65,10 +69,38 @@ def test_lt_converges_to_path_tracing():
         assert abs(r - 1) < 0.06, ((y0, y1, x0, x1), r)
 
 
+def test_lt_importance_mode_closed_forms():
+    rng = np.random.default_rng(3)
+    n = 4000
+    wo = rng.normal(size=(n, 3))
+    wo[:, 2] = np.abs(wo[:, 2]) + 0.05  # from outside (eta 1 -> 1.5)
+    wo /= np.linalg.norm(wo, axis=1, keepdims=True)
+    u = rng.random((n, 3))
+    glass = material(2, R=(1, 1, 1), T=(1, 1, 1), eta1=1.0, eta2=1.5)
+    wi_r, f_r, p_r = O.bsdf_sample(glass, wo, u)
+    wi_i, f_i, p_i = O.bsdf_sample_importance(glass, wo, u)
+    assert np.array_equal(wi_r, wi_i) and np.array_equal(p_r, p_i)
+    trans = (wi_i[:, 2] < 0) & (p_i > 0)
+    assert trans.sum() > n // 2
+    w_i = f_i[trans, 0] * np.abs(wi_i[trans, 2]) / p_i[trans]
+    w_r = f_r[trans, 0] * np.abs(wi_r[trans, 2]) / p_r[trans]
+    assert np.allclose(w_i, 1.0, rtol=1e-12) and np.allclose(w_r, (1 / 1.5) ** 2, rtol=1e-12)
+    rough = material(3, R=(1, 1, 1), T=(1, 1, 1), alpha=(0.3, 0.3), eta1=1.0, eta2=1.5)
+    wi_r, f_r, p_r = O.bsdf_sample(rough, wo, u)
+    wi_i, f_i, p_i = O.bsdf_sample_importance(rough, wo, u)
+    trans = (wi_i[:, 2] < 0) & (f_i[:, 0] > 0)
+    assert trans.sum() > n // 4
+    assert np.allclose(f_r[trans, 0] / f_i[trans, 0], 1 / 1.5 ** 2, rtol=1e-12)
+
+
 def test_lt_refuses_unsupported_scenes():
-    sa, cam, prm = mixed_material_scene(res=16)  # glass and rough dielectric
+    from tests.scene_util import env_image
+
+    sa, cam, prm = mixed_material_scene(res=16)
+    os_ = O.OracleScene(sa)
+    os_.set_environment(env_image(8, 4))
     with pytest.raises(ValueError):
-        O.OracleScene(sa).render_lt(cam, O.params(1, 3, 5, 0, 1, 1))
+        os_.render_lt(cam, O.params(1, 3, 5, 0, 1, 1))
 
 
 # ---------------------------------------------------------------- GPU
@@ -107,6 +139,18 @@ def test_lt_gpu_matches_oracle_ggx_and_halton(gpu_ctx):
 
 
 @pytest.mark.gpu
+def test_lt_gpu_matches_oracle_dielectrics(gpu_ctx):
+    # glass, rough dielectric, textured Lambert, rough conductor: light paths
+    # scatter through the dielectrics in the importance transport mode
+    sa, cam, prm = mixed_material_scene(res=40)
+    prm.spp, prm.max_depth = 8, 6
+    orc = O.OracleScene(sa).render_lt(cam, O.params(8, 6, 5, 0, 1, 1))
+    with g.Scene(gpu_ctx, sa) as sc:
+        gpu = sc.render_light_tracing(cam, prm)
+    _compare(gpu, orc)
+
+
+@pytest.mark.gpu
 def test_lt_gpu_light_only_closed_form_and_errors(gpu_ctx):
     le = (3.0, 2.0, 1.0)
     sa, cam = _light_only(le)
@@ -118,7 +162,10 @@ def test_lt_gpu_light_only_closed_form_and_errors(gpu_ctx):
             sc.render_light_tracing(cam, prm)
     total = img.astype(np.float64).sum(axis=(0, 1))
     assert np.allclose(total, np.array(le) * cam.width * cam.height, rtol=1e-4), total
+    from tests.scene_util import env_image
+
     sa2, cam2, prm2 = mixed_material_scene(res=16)
     with g.Scene(gpu_ctx, sa2) as sc:
+        sc.set_environment(env_image(8, 4))
         with pytest.raises(g.GyError):
             sc.render_light_tracing(cam2, prm2)
