@@ -86,7 +86,7 @@ typedef struct {
     int32_t rr_depth;
     int32_t mode;            /* GY_RENDER_MEGAKERNEL or GY_RENDER_WAVEFRONT */
     int32_t samples_per_pass;/* 0 = automatic */
-    int32_t block_size;      /* megakernel block size: 0 = automatic; else 64, 128 or 256 */
+    int32_t block_size;      /* megakernel and light-tracing block size: 0 = automatic; else 64, 128 or 256 */
     int32_t refill_threshold;/* wavefront traversal: refill rays when fewer lanes of a
                                 wave than this are busy (0 = automatic, 48; 1..64) */
     int32_t sampler;         /* GY_SAMPLER_SOBOL: SobolSampler + SobolEnum;
