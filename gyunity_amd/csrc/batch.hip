@@ -34,6 +34,7 @@ int BatchBuild(Ctx* c, Batch* b, int64_t lanes_req) {
     int depth_max = 0;
     for (size_t i = 0; i < b->entries.size(); ++i) {
         BatchEntry& en = b->entries[i];
+        en.scene_revision = en.scene->revision;
         int rc = MakePathParams(c, en.scene, en.cam, en.params, &en.P);
         if (rc != GY_OK) return rc;
         const int W = en.cam.width, H = en.cam.height;
@@ -122,6 +123,10 @@ extern "C" int gy_batch_render(gy_batch* bh) {
     Ctx* c = b.ctx;
     (void)hipSetDevice(c->device);
     int rc = GY_OK;
+    // a scene whose buffers were replaced (gy_scene_set_environment) invalidates
+    // the entry table and the recorded graph, which hold its device pointers
+    for (const BatchEntry& en : b.entries)
+        if (en.scene->revision != en.scene_revision) b.dirty = true;
     if (b.dirty) rc = BatchBuild(c, &b, h->lanes_req);
     if (rc != GY_OK) return rc;
     std::memset(&c->stats, 0, sizeof(c->stats));

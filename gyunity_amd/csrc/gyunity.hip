@@ -975,6 +975,7 @@ extern "C" int gy_scene_set_environment(gy_scene* s, const float* rgb, int32_t w
     E.h = h;
     sc->dev.light_func_int = func_int;
     sc->film_valid = false;
+    ++sc->revision;
     return GY_OK;
 }
 

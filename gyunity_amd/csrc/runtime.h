@@ -89,6 +89,7 @@ struct Scene {
     void *d_tri = nullptr, *d_shade = nullptr, *d_nodes = nullptr, *d_mats = nullptr, *d_tex = nullptr, *d_texinfo = nullptr;
     void *d_ltri = nullptr, *d_lle = nullptr, *d_lcdf = nullptr, *d_lfunc = nullptr;
     void* d_env = nullptr;  // environment light: texels and sampling tables in one block
+    uint64_t revision = 0;  // bumped when device buffers the kernels point at are replaced
     float world_lo[3] = {0, 0, 0}, world_hi[3] = {0, 0, 0};
     std::vector<int32_t> area_tri;  // area lights as created (the environment light follows them)
     std::vector<float> area_le, area_func;
@@ -151,6 +152,7 @@ struct BatchEntry {
     PathParams P = {};
     FilmState film;
     int64_t first = 0, paths = 0;
+    uint64_t scene_revision = 0;  // the scene's revision when the table was built
 };
 struct Batch {
     Ctx* ctx = nullptr;

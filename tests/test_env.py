@@ -286,6 +286,15 @@ def test_env_batch_entries_equal_single_renders(gpu_ctx):
                 color, weight = b.film(e)
                 assert np.array_equal(weight, refs[i][1]), i
                 assert np.array_equal(color, refs[i][0]), i
+            # a map set on a scene the batch holds (its light table is replaced):
+            # the batch rebuilds its table and graph before the next render
+            _, cam1, prm1, _ = entries[1]
+            scenes[1].set_environment(img)
+            ref1 = scenes[1].render(cam1, prm1)
+            assert not np.array_equal(ref1[0], refs[1][0])
+            b.render()
+            color, weight = b.film(ids[1])
+            assert np.array_equal(weight, ref1[1]) and np.array_equal(color, ref1[0])
     finally:
         for sc in scenes:
             sc.close()
