@@ -96,6 +96,7 @@ def lt_main(args):
     import gyunity_amd as g
 
     sa, cam, prm = g.generate_scene(g.GY_SCENE_ICOSPHERE_GGX, 0, args.level)
+    prm.mode = g.GY_RENDER_WAVEFRONT if args.mode == "wavefront" else g.GY_RENDER_MEGAKERNEL
     paths = cam.width * cam.height * prm.spp
     ctx = g.Context(0)
     log("device:", ctx.device_name())
@@ -125,7 +126,8 @@ def lt_main(args):
         "config": {
             "workload": "light tracing, icosphere-L%d GGX conductor, %d triangles, %dx%d, %d light paths per pixel, depth %d"
             % (args.level, sa.n_tri, cam.width, cam.height, prm.spp, prm.max_depth),
-            "integrator": "light tracing (one path per thread, float-atomic splats)",
+            "integrator": "light tracing, %s form, float-atomic splats"
+            % ("wavefront (one bounce of every path per launch)" if args.mode == "wavefront" else "per-path (one path per thread)"),
             "paths_per_step": paths,
             "finite_fraction": float(np.isfinite(img).mean()),
             "parallelism": "single GPU",

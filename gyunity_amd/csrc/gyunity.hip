@@ -979,6 +979,10 @@ extern "C" int gy_scene_set_environment(gy_scene* s, const float* rgb, int32_t w
     return GY_OK;
 }
 
+namespace gy {
+int LightTracingWavefront(Ctx* c, Scene* sc, const PathParams& P, const LightCamDev& C, int64_t n, float total, float* splat);
+}
+
 // LightTracing::Render (integrator/LT.h:17-53): spp light paths per pixel
 // index, rgb gets the film's splat sums (W x H x 3; the film has no samples,
 // so they are its pixels).  The splats are added with float atomics, so the
@@ -1016,7 +1020,9 @@ extern "C" int gy_render_light_tracing(gy_scene* s, const gy_camera_desc* cam, c
     void* d = nullptr;
     rc = DevAlloc(c, &d, (size_t)px * 4);
     if (rc == GY_OK) rc = Fill32(c, d, 0u, px);
-    if (rc == GY_OK) {
+    if (rc == GY_OK && p->mode == GY_RENDER_WAVEFRONT) {
+        rc = gy::LightTracingWavefront(c, sc, P, C, n, (float)n, (float*)d);
+    } else if (rc == GY_OK) {
         // block size: p->block_size (64, 128 or 256); 0 = 64, the fastest on the
         // 1.31M-triangle workload (86.6 Mpaths/s against 84.1 at 128, 81.4 at 256)
         const int bs = (p->block_size == 128 || p->block_size == 256) ? p->block_size : 64;

@@ -28,6 +28,7 @@
 
 #include "device.h"
 #include "runtime.h"
+#include "integrator.h"
 
 using namespace gyd;
 
@@ -915,6 +916,224 @@ int WavefrontPass(Ctx* c, Scene* sc, const PathParams& P, const gy_render_params
     if (sc->stats.max_depth + 1 > 64) return Fail(GY_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
     return WfBounceLoop<kStackLds>(c, Rd, W, p.max_depth, RefillOf(p.refill_threshold), blocks, (uint32_t)cap, sc->dev.env.w > 0,
                                    sc->film.counters);
+}
+
+// ---------------------------------------------------------------- light tracing
+// The wavefront form of LightTracing (integrator/LT.h:17-183), same values as
+// lt_path: lanes hold light paths; each round traces the extension rays
+// (k_wf_trace MODE 0), makes each hit a vertex (k_lt_shade: its camera
+// connection and its next ray), traces the connections (MODE 1, the shadow
+// queue) and splats the unoccluded ones (k_lt_splat).  Per lane: o, d the
+// extension ray, T the throughput, seq (index, cursor, bounce), sh_o / sh_d
+// the connection ray (tmax in sh_o.w), pend0 its splat (L, pixel), pend3.w
+// its occlusion flag.
+struct LtArgs {
+    LightCamDev C;
+    float total;       // light paths of the render (totalPhotons)
+    float* splat;      // W x H x 3
+    int32_t max_depth;
+};
+// ConnectToCamera + WorldToScreen (integrator/LT.h:116-183) up to the
+// visibility test: the connection ray and its splat, or false.
+__device__ inline bool lt_prepare(const WfDev& W, const LtArgs& A, uint32_t slot, V3 pos, V3 n, V3 outDir, const BSDF* b,
+                                  V3 thr, bool s0) {
+    const LightCamDev& C = A.C;
+    const V3 cpos = v3(C.pos[0], C.pos[1], C.pos[2]), cz = v3(C.cz[0], C.cz[1], C.cz[2]);
+    V3 toCam = cpos - pos;
+    const float len = length(toCam);
+    toCam = norm(toCam);
+    if (dot(cz, toCam * -1.f) < 0.f) return false;
+    const V3 rp = xf_point(C.m, pos);
+    const int ix = (int)rp.x, iy = (int)rp.y;
+    if (!(ix >= 0 && ix <= C.width && iy >= 0 && iy <= C.height)) return false;
+    V3 L;
+    if (s0) {
+        L = thr;
+    } else {
+        V3 wi = cpos - pos;
+        const float distance = length(wi);
+        wi = norm(wi);
+        const float cosT = dot(cz, wi * -1.f);
+        const float pdfW = 1.f * (distance * distance) / cosT;
+        const V3 wdir = wi * -1.f;
+        const float cw = dot(cz, wdir);
+        const float c2 = cw * cw;
+        const float We = C.dis * C.dis * 1.0f / (C.area * c2 * c2);
+        const V3 f = bsdf_eval(*b, outDir, wi);
+        L = v3s(We) * f * thr * fabsf(dot(wi, n)) / pdfW;
+    }
+    L = ((L * (float)C.width) * (float)C.height) / A.total;
+    int X = ix < 0 ? 0 : (ix > C.width - 1 ? C.width - 1 : ix);
+    int Y = iy < 0 ? 0 : (iy > C.height - 1 ? C.height - 1 : iy);
+    const V3 so = pos + toCam * kRayEps;
+    W.sh_o[slot] = make_float4(so.x, so.y, so.z, len);
+    W.sh_d[slot] = make_float4(toCam.x, toCam.y, toCam.z, 0.f);
+    W.pend0[slot] = make_float4(L.x, L.y, L.z, __uint_as_float((uint32_t)(Y * C.width + X)));
+    W.pend3[slot].w = __uint_as_float(0u);
+    return true;
+}
+
+// GenerateLightPath up to the first ray (integrator/LT.h:86-113), and the light
+// vertex's connection.  Lane = light path base + lane.
+__global__ void __launch_bounds__(256) k_lt_start(SceneDev S, PathParams P, WfDev W, LtArgs A, int64_t base, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool valid = t < n;
+    const uint32_t slot = (uint32_t)t;
+    bool conn = false;
+    if (valid) {
+        const SampleSrc& M = P.smp;
+        Seq q;
+        q.index = (uint64_t)(base + t);
+        q.cursor = 0;
+        float pdfLight;
+        const int light = sample_one_light(S, seq_next_wide(M, q), &pdfLight);
+        const int ltri = S.light_tri[light];
+        const V3 Le = v3(S.light_le[3 * light], S.light_le[3 * light + 1], S.light_le[3 * light + 2]);
+        const float u0 = seq_next_wide(M, q), u1 = seq_next_wide(M, q);
+        const float w0 = seq_next_wide(M, q), w1 = seq_next_wide(M, q);
+        const float4* tq = S.shade + (size_t)ltri * 8;
+        const V3 p0 = ld3(tq[0]), p1 = ld3(tq[1]), p2 = ld3(tq[2]);
+        const V3 n0 = ld3(tq[3]), n1 = ld3(tq[4]), n2 = ld3(tq[5]);
+        const float su0 = sqrtf(u0);
+        const float b0 = 1 - su0, b1 = u1 * su0;
+        const V3 lpos = (b0 * p0 + b1 * p1) + (1 - b0 - b1) * p2;
+        const V3 ln = (b0 * n0 + b1 * n1) + (1 - b0 - b1) * n2;
+        const float pdfA = 1 / tq[2].w;
+        V3 ss, ts;
+        coordinate_system(ln, &ss, &ts);
+        const V3 dl = cosine_hemisphere(w0, w1);
+        const V3 dir = norm((ss * dl.x + ts * dl.y) + ln * dl.z);
+        const float pdfDir = dl.z * kInvPI;
+        conn = lt_prepare(W, A, slot, lpos, ln, v3s(0.f), nullptr, Le, true);
+        const V3 T = Le * fabsf(dot(ln, dir)) / pdfLight / pdfA / pdfDir;
+        const V3 o = (lpos + dir * kRayEps) + dir * kRayEps;
+        W.o[slot] = make_float4(o.x, o.y, o.z, 0.f);
+        W.d[slot] = make_float4(dir.x, dir.y, dir.z, 0.f);
+        W.T[slot] = make_float4(T.x, T.y, T.z, 0.f);
+        W.seq[slot] = make_uint4((uint32_t)q.index, (uint32_t)(q.index >> 32), q.cursor, 1u);
+    }
+    uint32_t* const qs[2] = {W.q[Q_SHADOW], W.q[Q_NEXT]};
+    uint32_t* const cs[2] = {&W.counts[Q_SHADOW * kCtrStride], &W.counts[Q_NEXT * kCtrStride]};
+    const bool ps[2] = {conn, valid};
+    block_push<2>(qs, cs, slot, ps);
+}
+
+// Trace's loop body after an intersection (integrator/LT.h:62-83) with the
+// vertex's connection.
+__global__ void __launch_bounds__(256) k_lt_shade(SceneDev S, PathParams P, WfDev W, LtArgs A) {
+    const uint32_t n = W.counts[Q_ACTIVE * kCtrStride];
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (blockIdx.x * 256 >= n) return;
+    uint32_t slot = 0;
+    bool conn = false, next = false;
+    if (i < n && W.mtype[i] != 255) {
+        slot = W.q[Q_ACTIVE][i];
+        const V3 o = f3(W.o[slot]), d = f3(W.d[slot]);
+        const float4 hv = W.hit[slot];
+        Hit h;
+        h.t = hv.x;
+        h.prim = __float_as_int(hv.y);
+        h.b1 = hv.z;
+        h.b2 = hv.w;
+        const Isect it = query(S, o, d, h);
+        BSDF b = make_bsdf(S, it);
+        b.importance = true;  // ComputeScatteringFunction(arena, TransportMode::Importance) (integrator/LT.h:69)
+        V3 T = f3(W.T[slot]);
+        conn = lt_prepare(W, A, slot, it.pos, it.n, it.outDir, &b, T, false);
+        uint4 sq = W.seq[slot];
+        const uint32_t bounce = sq.w + 1u;
+        if ((int)bounce < A.max_depth) {
+            Seq q;
+            q.index = (uint64_t)sq.x | ((uint64_t)sq.y << 32);
+            q.cursor = sq.z;
+            const SampleSrc& M = P.smp;
+            const float r0 = seq_next_wide(M, q), r1 = seq_next_wide(M, q), r2 = seq_next_wide(M, q);
+            V3 wi;
+            float pdfDir;
+            const V3 f = bsdf_sample(b, neg(d), &wi, &pdfDir, r0, r1, r2);
+            const V3 no = (it.pos + wi * kRayEps) + wi * kRayEps;
+            T = T * f * fabsf(dot(wi, it.n)) / pdfDir;
+            W.o[slot] = make_float4(no.x, no.y, no.z, 0.f);
+            W.d[slot] = make_float4(wi.x, wi.y, wi.z, 0.f);
+            W.T[slot] = make_float4(T.x, T.y, T.z, 0.f);
+            W.seq[slot] = make_uint4(sq.x, sq.y, q.cursor, bounce);
+            next = true;
+        }
+    }
+    uint32_t* const qs[2] = {W.q[Q_SHADOW], W.q[Q_NEXT]};
+    uint32_t* const cs[2] = {&W.counts[Q_SHADOW * kCtrStride], &W.counts[Q_NEXT * kCtrStride]};
+    const bool ps[2] = {conn, next};
+    block_push<2>(qs, cs, slot, ps);
+}
+
+// Film::AddSplat (visual/Film.h:89-103) of the unoccluded connections.
+__global__ void __launch_bounds__(256) k_lt_splat(WfDev W, LtArgs A) {
+    const uint32_t n = W.counts[Q_SHADOW * kCtrStride];
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t slot = W.q[Q_SHADOW][i];
+    if (__float_as_uint(W.pend3[slot].w) & P_OCCLUDED) return;
+    const float4 p = W.pend0[slot];
+    float* px = A.splat + (size_t)__float_as_uint(p.w) * 3;
+    atomicAdd(px, p.x);
+    atomicAdd(px + 1, p.y);
+    atomicAdd(px + 2, p.z);
+}
+
+int LightTracingWavefront(Ctx* c, Scene* sc, const PathParams& P, const LightCamDev& C, int64_t n, float total, float* splat) {
+    const int64_t cap = std::min<int64_t>(n, (int64_t)1 << 25);
+    int rc = WfPrepareState(c, sc->wf, std::max<int64_t>(cap, 1), false);
+    if (rc != GY_OK) return rc;
+    size_t used = 0;
+    WfDev W = g_view(sc->wf, false, &used);
+    W.spill_stride = WfLanes(c);
+    if ((size_t)W.spill_stride * 64 * 4 > sc->wf.spill_bytes) return Fail(GY_ERR_ALLOC, "wavefront spill buffer too small");
+    if (used > sc->wf.bytes) return Fail(GY_ERR_ALLOC, "wavefront state layout exceeds its allocation");
+    OneRender Rd;
+    Rd.S = sc->dev;
+    Rd.P = P;
+    Rd.F = FilmView(sc);
+    LtArgs A;
+    A.C = C;
+    A.total = total;
+    A.splat = splat;
+    A.max_depth = P.max_depth;
+    const unsigned pblocks = PersistBlocks(c);
+    const unsigned blocks = (unsigned)((cap + 255) / 256);
+    const int rounds = std::max(1, P.max_depth - 1);  // intersections of a path (integrator/LT.h:66-76)
+    void* ctr = nullptr;  // ray counts of k_wf_advance (the scene's film may not exist yet)
+    rc = DevAlloc(c, &ctr, 4 * sizeof(unsigned long long));
+    if (rc == GY_OK) rc = Fill32(c, ctr, 0u, 8);
+    if (rc != GY_OK) {
+        DevFree(ctr);
+        return rc;
+    }
+    for (int64_t base = 0; base < n && rc == GY_OK; base += cap) {
+        const int64_t cnt = std::min(cap, n - base);
+        hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(64), 0, c->stream, W);
+        hipLaunchKernelGGL(k_lt_start, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, c->stream, sc->dev, P, W, A, base, cnt);
+        for (int r = 0; r <= rounds; ++r) {
+            if (r > 0) {
+                hipLaunchKernelGGL((k_wf_trace<kStackLds, 0, OneRender>), dim3(pblocks), dim3(256), 0, c->stream, Rd, W, kAutoRefill,
+                                   kTraceChunks[0]);
+                hipLaunchKernelGGL(k_lt_shade, dim3(blocks), dim3(256), 0, c->stream, sc->dev, P, W, A);
+            }
+            hipLaunchKernelGGL((k_wf_trace<kStackLds, 1, OneRender>), dim3(pblocks), dim3(256), 0, c->stream, Rd, W, kAutoRefill,
+                               kTraceChunks[1]);
+            hipLaunchKernelGGL(k_lt_splat, dim3(blocks), dim3(256), 0, c->stream, W, A);
+            hipLaunchKernelGGL(k_wf_advance, dim3(1), dim3(64), 0, c->stream, W, (unsigned long long*)ctr);
+            rc = CheckLaunch(c, "light tracing round");
+            if (rc != GY_OK) break;
+            uint32_t* tq = W.q[Q_ACTIVE];
+            W.q[Q_ACTIVE] = W.q[Q_NEXT];
+            W.q[Q_NEXT] = tq;
+        }
+    }
+    // the kernels are queued on the stream: release the counters after them
+    const hipError_t e = hipStreamSynchronize(c->stream);
+    DevFree(ctr);
+    if (rc == GY_OK && e != hipSuccess) rc = HipFail(e, "light tracing");
+    return rc;
 }
 
 // ---------------------------------------------------------------- batch

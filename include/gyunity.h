@@ -247,8 +247,11 @@ int gy_film_save(gy_scene* scene, const char* path);
  * area lights, every vertex connected to the pinhole camera and splatted
  * (Film::AddSplat, visual/Film.h:89-103); rgb gets the W x H x 3 splat sums.
  * Area lights only (GY_ERR_UNSUPPORTED otherwise); dielectrics scatter light
- * paths in the importance transport mode.  Float atomics add the splats: the per-path values are fixed,
- * their summation order is not. */
+ * paths in the importance transport mode.  p->mode picks the form:
+ * GY_RENDER_WAVEFRONT traces every path's rays of one bounce together (queues
+ * and persistent trace kernels), GY_RENDER_MEGAKERNEL one path per thread
+ * (p->block_size 64, 128 or 256; 0 = 64).  Float atomics add the splats: the
+ * per-path values are fixed, their summation order is not. */
 int gy_render_light_tracing(gy_scene* scene, const gy_camera_desc* cam, const gy_render_params* p, float* rgb);
 /* Reads a Radiance .hdr file as stbi_loadf does for ImageIO::LoadTexture
  * (image/ImageIO.cpp:100-114), the loader of EnvironmentMap(filename): rgb gets

@@ -14,9 +14,11 @@ MSVC):
   rough-dielectric transmission values are e^2 times the radiance mode's
   (bsdf/BSDF.h:266-271, 505, 608);
 - scenes the integrator does not take (an environment light) are refused.
-GPU tests compare the device's splat image with the oracle's: each light
-path's splats are the same values, their float-atomic summation order is not
-(tolerance below).
+GPU tests compare the device's splat image with the oracle's, for both
+forms of the integrator (one light path per thread, and the wavefront that
+traces every path's rays of one bounce together): each light path's splats
+are the same values, their float-atomic summation order is not (tolerance
+below).
 """
 import numpy as np
 import pytest
@@ -26,6 +28,7 @@ from oracle import oracle as O
 from tests.scene_util import material, mixed_material_scene
 
 S_REL = 1e-4  # per pixel, relative to max(|L|, 1e-3 of the image's mean)
+MODES = pytest.mark.parametrize("mode", [g.GY_RENDER_MEGAKERNEL, g.GY_RENDER_WAVEFRONT], ids=["megakernel", "wavefront"])
 
 
 def _light_only(le=(3.0, 2.0, 1.0)):
@@ -116,9 +119,10 @@ def _compare(gpu, orc):
 
 
 @pytest.mark.gpu
-def test_lt_gpu_matches_oracle_cornell(gpu_ctx):
+@MODES
+def test_lt_gpu_matches_oracle_cornell(gpu_ctx, mode):
     sa, cam, prm = _cornell(48)
-    prm.spp, prm.max_depth = 16, 5
+    prm.spp, prm.max_depth, prm.mode = 16, 5, mode
     orc = O.OracleScene(sa).render_lt(cam, O.params(16, 5, 5, 0, 1, 1))
     with g.Scene(gpu_ctx, sa) as sc:
         gpu = sc.render_light_tracing(cam, prm)
@@ -128,10 +132,11 @@ def test_lt_gpu_matches_oracle_cornell(gpu_ctx):
 
 
 @pytest.mark.gpu
-def test_lt_gpu_matches_oracle_ggx_and_halton(gpu_ctx):
+@MODES
+def test_lt_gpu_matches_oracle_ggx_and_halton(gpu_ctx, mode):
     sa, cam, prm = g.generate_scene(g.GY_SCENE_ICOSPHERE_GGX, 0, 3)
     cam.width = cam.height = 40
-    prm.spp, prm.max_depth, prm.sampler = 8, 6, g.GY_SAMPLER_HALTON
+    prm.spp, prm.max_depth, prm.sampler, prm.mode = 8, 6, g.GY_SAMPLER_HALTON, mode
     orc = O.OracleScene(sa).render_lt(cam, O.params(8, 6, 5, 0, 1, 1, sampler=1))
     with g.Scene(gpu_ctx, sa) as sc:
         gpu = sc.render_light_tracing(cam, prm)
@@ -139,11 +144,12 @@ def test_lt_gpu_matches_oracle_ggx_and_halton(gpu_ctx):
 
 
 @pytest.mark.gpu
-def test_lt_gpu_matches_oracle_dielectrics(gpu_ctx):
+@MODES
+def test_lt_gpu_matches_oracle_dielectrics(gpu_ctx, mode):
     # glass, rough dielectric, textured Lambert, rough conductor: light paths
     # scatter through the dielectrics in the importance transport mode
     sa, cam, prm = mixed_material_scene(res=40)
-    prm.spp, prm.max_depth = 8, 6
+    prm.spp, prm.max_depth, prm.mode = 8, 6, mode
     orc = O.OracleScene(sa).render_lt(cam, O.params(8, 6, 5, 0, 1, 1))
     with g.Scene(gpu_ctx, sa) as sc:
         gpu = sc.render_light_tracing(cam, prm)
@@ -151,10 +157,11 @@ def test_lt_gpu_matches_oracle_dielectrics(gpu_ctx):
 
 
 @pytest.mark.gpu
-def test_lt_gpu_light_only_closed_form_and_errors(gpu_ctx):
+@MODES
+def test_lt_gpu_light_only_closed_form_and_errors(gpu_ctx, mode):
     le = (3.0, 2.0, 1.0)
     sa, cam = _light_only(le)
-    prm = g.render_params(spp=16, max_depth=4, rr_depth=5)
+    prm = g.render_params(spp=16, max_depth=4, rr_depth=5, mode=mode)
     with g.Scene(gpu_ctx, sa) as sc:
         img = sc.render_light_tracing(cam, prm)
         prm.max_depth = 0
@@ -167,5 +174,30 @@ def test_lt_gpu_light_only_closed_form_and_errors(gpu_ctx):
     sa2, cam2, prm2 = mixed_material_scene(res=16)
     with g.Scene(gpu_ctx, sa2) as sc:
         sc.set_environment(env_image(8, 4))
+        prm2.mode = mode
         with pytest.raises(g.GyError):
             sc.render_light_tracing(cam2, prm2)
+
+
+@pytest.mark.gpu
+def test_lt_gpu_wavefront_matches_per_path_form(gpu_ctx):
+    # the two forms on one scene, depth 1 to 8, then a path tracing render and
+    # light tracing again on the same scene (the forms share its wavefront state)
+    sa, cam, prm = mixed_material_scene(res=32)
+    with g.Scene(gpu_ctx, sa) as sc:
+        for depth in (1, 2, 3, 8):
+            prm.spp, prm.max_depth = 4, depth
+            prm.mode = g.GY_RENDER_MEGAKERNEL
+            mk = sc.render_light_tracing(cam, prm)
+            prm.mode = g.GY_RENDER_WAVEFRONT
+            wf = sc.render_light_tracing(cam, prm)
+            assert np.array_equal(np.isnan(mk), np.isnan(wf)), depth
+            ok = ~np.isnan(mk)
+            assert np.allclose(wf[ok], mk[ok], rtol=1e-5, atol=1e-6 * np.abs(mk[ok]).max()), depth
+        prm.max_depth = 5
+        sc.render(cam, prm)
+        wf2 = sc.render_light_tracing(cam, prm)
+        prm.mode = g.GY_RENDER_MEGAKERNEL
+        mk2 = sc.render_light_tracing(cam, prm)
+    ok = ~np.isnan(mk2)
+    assert np.allclose(wf2[ok], mk2[ok], rtol=1e-5, atol=1e-6 * np.abs(mk2[ok]).max())
