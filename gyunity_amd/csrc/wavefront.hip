@@ -35,8 +35,8 @@ using namespace gyd;
 namespace gy {
 
 enum QueueId { Q_ACTIVE = 0, Q_NEXT = 1, Q_SHADE0 = 2, Q_SHADOW = 6, Q_MIS = 7, Q_COUNT = 8 };
-// counts[] words 16..39: work-fetch counters of the persistent trace kernels,
-// eight per ray kind (one per queue chunk, see k_wf_trace)
+// counts[] words 16..47: work-fetch counters of the persistent trace kernels,
+// eight per trace MODE 0..3 (one per queue chunk, see k_wf_trace)
 constexpr int C_FETCH0 = 16;
 constexpr int kChunks = 8;  // XCDs of the device; workgroup b runs on XCD b % 8
 constexpr int C_WORDS = 48;
@@ -95,6 +95,10 @@ struct OneRender {
     __device__ const PathParams& params(uint32_t) const { return P; }
     __device__ const FilmDev& film(uint32_t) const { return F; }
     __device__ uint32_t pass_slot(const WfDev&, uint32_t slot) const { return slot; }
+};
+// Light tracing: one scene, and the splat image its connections add to.
+struct LightRender : OneRender {
+    float* splat;  // W x H x 3
 };
 struct ManyRenders {
     const BatchEntryDev* __restrict__ E;
@@ -261,7 +265,9 @@ __global__ void __launch_bounds__(256) k_wf_raygen_batch(WfDev W, ManyRenders R,
 
 // ---------------------------------------------------------------- persistent trace
 // MODE 0: closest hit of the active queue (hits binned by material),
-// MODE 1: shadow rays (any hit), MODE 2: MIS rays (closest hit + emission).
+// MODE 1: shadow rays (any hit), MODE 2: MIS rays (closest hit + emission),
+// MODE 3: light tracing's camera connections (any hit on the shadow queue;
+// an unoccluded one adds its splat, R = LightRender).
 // A wave claims 64 queue entries per atomic.  Traversal (trav_run) returns to
 // refilling once fewer than `refill` lanes of the wave are still traversing;
 // idle lanes then take new entries.  A wave exits when the queue is drained
@@ -271,7 +277,8 @@ constexpr uint32_t kBatch = 64;
 template <int S, int MODE, class R>
 __global__ void __launch_bounds__(256) k_wf_trace(R Rd, WfDev W, int refill, int chunks) {
     __shared__ uint32_t stack[S * 256];
-    constexpr int qid = MODE == 0 ? Q_ACTIVE : (MODE == 1 ? Q_SHADOW : Q_MIS);
+    constexpr bool kAny = MODE == 1 || MODE == 3;
+    constexpr int qid = MODE == 0 ? Q_ACTIVE : (kAny ? Q_SHADOW : Q_MIS);
     const uint32_t n = W.counts[(qid) * kCtrStride];
     // The queue is cut into kChunks contiguous chunks, one per XCD.  Workgroups
     // are dealt to the XCDs round-robin, so a wave starts on its own XCD's
@@ -338,7 +345,7 @@ __global__ void __launch_bounds__(256) k_wf_trace(R Rd, WfDev W, int refill, int
                 }
                 if (MODE == 0) {
                     trav_init(t, f3(W.o[slot]), f3(W.d[slot]), 0.f, kInf);
-                } else if (MODE == 1) {
+                } else if (kAny) {
                     const float4 o = W.sh_o[slot];
                     trav_init(t, f3(o), f3(W.sh_d[slot]), 0.f, o.w);
                 } else {
@@ -346,7 +353,7 @@ __global__ void __launch_bounds__(256) k_wf_trace(R Rd, WfDev W, int refill, int
                 }
             }
             {
-                const int r = trav_run<MODE == 1, S>(Sc, t, lds, 256, spill, W.spill_stride, refill);
+                const int r = trav_run<kAny, S>(Sc, t, lds, 256, spill, W.spill_stride, refill);
                 if (r != 0) {
                     if (MODE == 0) {
                         if (t.best_prim >= 0) {
@@ -362,6 +369,17 @@ __global__ void __launch_bounds__(256) k_wf_trace(R Rd, WfDev W, int refill, int
                         if (r == 2) {
                             float& fw = W.pend3[slot].w;
                             fw = __uint_as_float(__float_as_uint(fw) | P_OCCLUDED);
+                        }
+                    } else if (MODE == 3) {
+                        if constexpr (MODE == 3) {
+                            // Film::AddSplat (visual/Film.h:89-103) of an unoccluded connection
+                            if (r == 1) {
+                                const float4 p = W.pend0[slot];
+                                float* px = Rd.splat + (size_t)__float_as_uint(p.w) * 3;
+                                atomicAdd(px, p.x);
+                                atomicAdd(px + 1, p.y);
+                                atomicAdd(px + 2, p.z);
+                            }
                         }
                     } else if (t.best_prim < 0) {
                         // a miss counts for an environment light (k_wf_env)
@@ -922,15 +940,13 @@ int WavefrontPass(Ctx* c, Scene* sc, const PathParams& P, const gy_render_params
 // The wavefront form of LightTracing (integrator/LT.h:17-183), same values as
 // lt_path: lanes hold light paths; each round traces the extension rays
 // (k_wf_trace MODE 0), makes each hit a vertex (k_lt_shade: its camera
-// connection and its next ray), traces the connections (MODE 1, the shadow
-// queue) and splats the unoccluded ones (k_lt_splat).  Per lane: o, d the
-// extension ray, T the throughput, seq (index, cursor, bounce), sh_o / sh_d
-// the connection ray (tmax in sh_o.w), pend0 its splat (L, pixel), pend3.w
-// its occlusion flag.
+// connection and its next ray), then traces the connections (MODE 3, the
+// shadow queue), which splats the unoccluded ones as they finish.  Per lane:
+// o, d the extension ray, T the throughput, seq (index, cursor, bounce),
+// sh_o / sh_d the connection ray (tmax in sh_o.w), pend0 its splat (L, pixel).
 struct LtArgs {
     LightCamDev C;
-    float total;       // light paths of the render (totalPhotons)
-    float* splat;      // W x H x 3
+    float total;  // light paths of the render (totalPhotons)
     int32_t max_depth;
 };
 // ConnectToCamera + WorldToScreen (integrator/LT.h:116-183) up to the
@@ -969,7 +985,6 @@ __device__ inline bool lt_prepare(const WfDev& W, const LtArgs& A, uint32_t slot
     W.sh_o[slot] = make_float4(so.x, so.y, so.z, len);
     W.sh_d[slot] = make_float4(toCam.x, toCam.y, toCam.z, 0.f);
     W.pend0[slot] = make_float4(L.x, L.y, L.z, __uint_as_float((uint32_t)(Y * C.width + X)));
-    W.pend3[slot].w = __uint_as_float(0u);
     return true;
 }
 
@@ -1066,20 +1081,6 @@ __global__ void __launch_bounds__(256) k_lt_shade(SceneDev S, PathParams P, WfDe
     block_push<2>(qs, cs, slot, ps);
 }
 
-// Film::AddSplat (visual/Film.h:89-103) of the unoccluded connections.
-__global__ void __launch_bounds__(256) k_lt_splat(WfDev W, LtArgs A) {
-    const uint32_t n = W.counts[Q_SHADOW * kCtrStride];
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t slot = W.q[Q_SHADOW][i];
-    if (__float_as_uint(W.pend3[slot].w) & P_OCCLUDED) return;
-    const float4 p = W.pend0[slot];
-    float* px = A.splat + (size_t)__float_as_uint(p.w) * 3;
-    atomicAdd(px, p.x);
-    atomicAdd(px + 1, p.y);
-    atomicAdd(px + 2, p.z);
-}
-
 int LightTracingWavefront(Ctx* c, Scene* sc, const PathParams& P, const LightCamDev& C, int64_t n, float total, float* splat) {
     const int64_t cap = std::min<int64_t>(n, (int64_t)1 << 25);
     int rc = WfPrepareState(c, sc->wf, std::max<int64_t>(cap, 1), false);
@@ -1089,14 +1090,14 @@ int LightTracingWavefront(Ctx* c, Scene* sc, const PathParams& P, const LightCam
     W.spill_stride = WfLanes(c);
     if ((size_t)W.spill_stride * 64 * 4 > sc->wf.spill_bytes) return Fail(GY_ERR_ALLOC, "wavefront spill buffer too small");
     if (used > sc->wf.bytes) return Fail(GY_ERR_ALLOC, "wavefront state layout exceeds its allocation");
-    OneRender Rd;
+    LightRender Rd;
     Rd.S = sc->dev;
     Rd.P = P;
     Rd.F = FilmView(sc);
+    Rd.splat = splat;
     LtArgs A;
     A.C = C;
     A.total = total;
-    A.splat = splat;
     A.max_depth = P.max_depth;
     const unsigned pblocks = PersistBlocks(c);
     const unsigned blocks = (unsigned)((cap + 255) / 256);
@@ -1114,13 +1115,13 @@ int LightTracingWavefront(Ctx* c, Scene* sc, const PathParams& P, const LightCam
         hipLaunchKernelGGL(k_lt_start, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, c->stream, sc->dev, P, W, A, base, cnt);
         for (int r = 0; r <= rounds; ++r) {
             if (r > 0) {
-                hipLaunchKernelGGL((k_wf_trace<kStackLds, 0, OneRender>), dim3(pblocks), dim3(256), 0, c->stream, Rd, W, kAutoRefill,
+                hipLaunchKernelGGL((k_wf_trace<kStackLds, 0, LightRender>), dim3(pblocks), dim3(256), 0, c->stream, Rd, W, kAutoRefill,
                                    kTraceChunks[0]);
                 hipLaunchKernelGGL(k_lt_shade, dim3(blocks), dim3(256), 0, c->stream, sc->dev, P, W, A);
             }
-            hipLaunchKernelGGL((k_wf_trace<kStackLds, 1, OneRender>), dim3(pblocks), dim3(256), 0, c->stream, Rd, W, kAutoRefill,
+            // connections: visibility and splat in one pass
+            hipLaunchKernelGGL((k_wf_trace<kStackLds, 3, LightRender>), dim3(pblocks), dim3(256), 0, c->stream, Rd, W, kAutoRefill,
                                kTraceChunks[1]);
-            hipLaunchKernelGGL(k_lt_splat, dim3(blocks), dim3(256), 0, c->stream, W, A);
             hipLaunchKernelGGL(k_wf_advance, dim3(1), dim3(64), 0, c->stream, W, (unsigned long long*)ctr);
             rc = CheckLaunch(c, "light tracing round");
             if (rc != GY_OK) break;
