@@ -254,13 +254,13 @@ __global__ void __launch_bounds__(128) k_li_samples(SceneDev S, PathParams P, in
 }
 
 // LightTracing: one light path per thread, sample index = thread index.
-template <int STACK, int BLOCK>
+template <int STACK, int BLOCK, bool ENV>
 __global__ void __launch_bounds__(BLOCK) k_lt(SceneDev S, PathParams P, LightCamDev C, int64_t n, float total,
                                               float* __restrict__ splat) {
     __shared__ uint32_t stack[STACK * BLOCK];
     const int64_t t = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (t >= n) return;
-    lt_path<STACK>(S, P, C, (uint64_t)t, total, splat, stack + threadIdx.x, BLOCK);
+    lt_path<STACK, ENV>(S, P, C, (uint64_t)t, total, splat, stack + threadIdx.x, BLOCK);
 }
 
 // ------------------------------------------------------------------ film
@@ -995,8 +995,7 @@ extern "C" int gy_render_light_tracing(gy_scene* s, const gy_camera_desc* cam, c
     Scene* sc = reinterpret_cast<Scene*>(s);
     Ctx* c = sc->ctx;
     (void)hipSetDevice(c->device);
-    if (sc->area_tri.empty() || sc->dev.env.w > 0)
-        return Fail(GY_ERR_UNSUPPORTED, "gy_render_light_tracing: area lights only (no environment light)");
+    if (sc->area_tri.empty() && sc->dev.env.w <= 0) return Fail(GY_ERR_INVALID, "gy_render_light_tracing: the scene has no light");
     const int64_t n = (int64_t)cam->width * cam->height * p->spp;
     if (n >= (1ll << 36)) return Fail(GY_ERR_UNSUPPORTED, "gy_render_light_tracing: more than 2^36 light paths");
     if (sc->stats.max_depth + 1 > 64) return Fail(GY_ERR_UNSUPPORTED, "gy_render_light_tracing: BVH deeper than the traversal stack");
@@ -1016,6 +1015,8 @@ extern "C" int gy_render_light_tracing(gy_scene* s, const gy_camera_desc* cam, c
     C.area = lc.area;
     C.width = cam->width;
     C.height = cam->height;
+    // AABB::GetBoundingSphere (math/AABB.h:69-72): (min + max) * 0.5
+    for (int i = 0; i < 3; ++i) C.env_c[i] = (sc->world_lo[i] + sc->world_hi[i]) * 0.5f;
     const int64_t px = (int64_t)cam->width * cam->height * 3;
     void* d = nullptr;
     rc = DevAlloc(c, &d, (size_t)px * 4);
@@ -1027,9 +1028,13 @@ extern "C" int gy_render_light_tracing(gy_scene* s, const gy_camera_desc* cam, c
         // 1.31M-triangle workload (86.6 Mpaths/s against 84.1 at 128, 81.4 at 256)
         const int bs = (p->block_size == 128 || p->block_size == 256) ? p->block_size : 64;
         const unsigned blocks = (unsigned)((n + bs - 1) / bs);
-        void (*kern)(SceneDev, PathParams, LightCamDev, int64_t, float, float*) =
-            sc->stack <= 32 ? (bs == 64 ? k_lt<32, 64> : bs == 256 ? k_lt<32, 256> : k_lt<32, 128>)
-                            : (bs == 64 ? k_lt<64, 64> : bs == 256 ? k_lt<64, 256> : k_lt<64, 128>);
+        void (*kern)(SceneDev, PathParams, LightCamDev, int64_t, float, float*);
+        if (sc->dev.env.w > 0)
+            kern = sc->stack <= 32 ? (bs == 64 ? k_lt<32, 64, true> : bs == 256 ? k_lt<32, 256, true> : k_lt<32, 128, true>)
+                                   : (bs == 64 ? k_lt<64, 64, true> : bs == 256 ? k_lt<64, 256, true> : k_lt<64, 128, true>);
+        else
+            kern = sc->stack <= 32 ? (bs == 64 ? k_lt<32, 64, false> : bs == 256 ? k_lt<32, 256, false> : k_lt<32, 128, false>)
+                                   : (bs == 64 ? k_lt<64, 64, false> : bs == 256 ? k_lt<64, 256, false> : k_lt<64, 128, false>);
         hipLaunchKernelGGL(kern, dim3(blocks), dim3(bs), 0, c->stream, sc->dev, P, C, n, (float)n, (float*)d);
         rc = CheckLaunch(c, "k_lt");
     }

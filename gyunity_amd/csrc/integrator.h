@@ -181,6 +181,7 @@ struct LightCamDev {
     float pos[3], cz[3];
     float dis, area;
     int32_t width, height;
+    float env_c[3];  // the scene's bounding-sphere center (environment light paths)
 };
 // Transform::operator()(point) (math/Transform.h:106-115) with the column
 // order of Matrix4 * Vector4 (math/Linagl.h:937-948).
@@ -247,8 +248,50 @@ __device__ void lt_connect(const SceneDev& S, const LightCamDev& C, float* splat
     atomicAdd(px + 1, L.y);
     atomicAdd(px + 2, L.z);
 }
+// EnvironmentLight::SampleLight (light/EnvLight.cpp:22-39) for a light path: a
+// direction from the map and a point on the bounding sphere's disk facing it.
+// Its radiance is dropped: GenerateLightPath takes Le = Emission(), which the
+// environment light leaves zero (light/EnvLight.h:20-22, integrator/LT.h:108-111),
+// so these paths carry zero throughput (NaN where a pdf is zero).  ConcentricSampleDisk
+// is the x, y of cosine_hemisphere.
+__device__ __noinline__ void lt_env_origin(const EnvDev& E, const LightCamDev& C, float u0, float u1, float v0, float v1,
+                                           V3* pos, V3* dir, float* pdfA, float* pdfDir) {
+    V3 sd;
+    (void)env_sample(E, u0, u1, &sd, pdfDir);
+    const V3 d = neg(sd);
+    V3 a1, a2;
+    coordinate_system(neg(d), &a1, &a2);
+    const V3 cd = cosine_hemisphere(v0, v1);
+    const V3 ctr = v3(C.env_c[0], C.env_c[1], C.env_c[2]);
+    const V3 pDisk = ctr + (a1 * cd.x + a2 * cd.y) * E.radius;
+    *pos = pDisk + neg(d) * E.radius;
+    *dir = d;
+    *pdfA = 1.f / ((kPI * E.radius) * E.radius);
+}
+// AreaLight::SampleOnLight (light/AreaLight.h:60-71), Triangle::Sample
+// (shape/Triangle.cpp:141-158); returns Le.
+__device__ inline V3 lt_area_origin(const SceneDev& S, int light, float u0, float u1, float w0, float w1, V3* pos, V3* n,
+                                    V3* dir, float* pdfA, float* pdfDir) {
+    const int ltri = S.light_tri[light];
+    const float4* tq = S.shade + (size_t)ltri * 8;
+    const V3 p0 = ld3(tq[0]), p1 = ld3(tq[1]), p2 = ld3(tq[2]);
+    const V3 n0 = ld3(tq[3]), n1 = ld3(tq[4]), n2 = ld3(tq[5]);
+    const float su0 = sqrtf(u0);
+    const float b0 = 1 - su0, b1 = u1 * su0;
+    *pos = (b0 * p0 + b1 * p1) + (1 - b0 - b1) * p2;
+    const V3 ln = (b0 * n0 + b1 * n1) + (1 - b0 - b1) * n2;
+    *n = ln;
+    *pdfA = 1 / tq[2].w;
+    V3 ss, ts;
+    coordinate_system(ln, &ss, &ts);
+    const V3 dl = cosine_hemisphere(w0, w1);
+    *dir = norm((ss * dl.x + ts * dl.y) + ln * dl.z);
+    *pdfDir = dl.z * kInvPI;
+    return v3(S.light_le[3 * light], S.light_le[3 * light + 1], S.light_le[3 * light + 2]);
+}
 // GenerateLightPath + Trace (integrator/LT.h:60-113) with the connections.
-template <int STACK>
+// ENV: the scene holds an environment light (its branch is compiled only then).
+template <int STACK, bool ENV>
 __device__ void lt_path(const SceneDev& S, const PathParams& P, const LightCamDev& C, uint64_t index, float total,
                         float* splat, uint32_t* stack, int stride) {
     const SampleSrc& M = P.smp;
@@ -257,25 +300,19 @@ __device__ void lt_path(const SceneDev& S, const PathParams& P, const LightCamDe
     q.cursor = 0;
     float pdfLight;
     const int light = sample_one_light(S, seq_next(M, q), &pdfLight);
-    const int ltri = S.light_tri[light];
-    const V3 Le = v3(S.light_le[3 * light], S.light_le[3 * light + 1], S.light_le[3 * light + 2]);
     const float u0 = seq_next(M, q), u1 = seq_next(M, q);
     const float w0 = seq_next(M, q), w1 = seq_next(M, q);
-    // AreaLight::SampleOnLight (light/AreaLight.h:60-71), Triangle::Sample (shape/Triangle.cpp:141-158)
-    const float4* tq = S.shade + (size_t)ltri * 8;
-    const V3 p0 = ld3(tq[0]), p1 = ld3(tq[1]), p2 = ld3(tq[2]);
-    const V3 n0 = ld3(tq[3]), n1 = ld3(tq[4]), n2 = ld3(tq[5]);
-    const float su0 = sqrtf(u0);
-    const float b0 = 1 - su0, b1 = u1 * su0;
-    const V3 lpos = (b0 * p0 + b1 * p1) + (1 - b0 - b1) * p2;
-    const V3 ln = (b0 * n0 + b1 * n1) + (1 - b0 - b1) * n2;
-    const float pdfA = 1 / tq[2].w;
-    V3 ss, ts;
-    coordinate_system(ln, &ss, &ts);
-    const V3 dl = cosine_hemisphere(w0, w1);
-    const V3 dir = norm((ss * dl.x + ts * dl.y) + ln * dl.z);
-    float pdfDir = dl.z * kInvPI;
-    lt_connect<STACK>(S, C, splat, lpos, ln, v3s(0.f), nullptr, Le, true, total, stack, stride);
+    V3 Le, lpos, ln, dir;
+    float pdfA, pdfDir;
+    if (!ENV || S.light_tri[light] >= 0) {
+        Le = lt_area_origin(S, light, u0, u1, w0, w1, &lpos, &ln, &dir, &pdfA, &pdfDir);
+        lt_connect<STACK>(S, C, splat, lpos, ln, v3s(0.f), nullptr, Le, true, total, stack, stride);
+    } else {
+        // the light vertex's splat is Le = 0: adding it changes no pixel
+        Le = v3s(0.f);
+        lt_env_origin(S.env, C, u0, u1, w0, w1, &lpos, &dir, &pdfA, &pdfDir);
+        ln = dir;
+    }
     V3 T = Le * fabsf(dot(ln, dir)) / pdfLight / pdfA / pdfDir;
     V3 o = (lpos + dir * kRayEps) + dir * kRayEps;
     V3 d = dir;

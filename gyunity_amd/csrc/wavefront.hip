@@ -990,6 +990,7 @@ __device__ inline bool lt_prepare(const WfDev& W, const LtArgs& A, uint32_t slot
 
 // GenerateLightPath up to the first ray (integrator/LT.h:86-113), and the light
 // vertex's connection.  Lane = light path base + lane.
+template <bool ENV>
 __global__ void __launch_bounds__(256) k_lt_start(SceneDev S, PathParams P, WfDev W, LtArgs A, int64_t base, int64_t n) {
     const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const bool valid = t < n;
@@ -1002,24 +1003,19 @@ __global__ void __launch_bounds__(256) k_lt_start(SceneDev S, PathParams P, WfDe
         q.cursor = 0;
         float pdfLight;
         const int light = sample_one_light(S, seq_next_wide(M, q), &pdfLight);
-        const int ltri = S.light_tri[light];
-        const V3 Le = v3(S.light_le[3 * light], S.light_le[3 * light + 1], S.light_le[3 * light + 2]);
         const float u0 = seq_next_wide(M, q), u1 = seq_next_wide(M, q);
         const float w0 = seq_next_wide(M, q), w1 = seq_next_wide(M, q);
-        const float4* tq = S.shade + (size_t)ltri * 8;
-        const V3 p0 = ld3(tq[0]), p1 = ld3(tq[1]), p2 = ld3(tq[2]);
-        const V3 n0 = ld3(tq[3]), n1 = ld3(tq[4]), n2 = ld3(tq[5]);
-        const float su0 = sqrtf(u0);
-        const float b0 = 1 - su0, b1 = u1 * su0;
-        const V3 lpos = (b0 * p0 + b1 * p1) + (1 - b0 - b1) * p2;
-        const V3 ln = (b0 * n0 + b1 * n1) + (1 - b0 - b1) * n2;
-        const float pdfA = 1 / tq[2].w;
-        V3 ss, ts;
-        coordinate_system(ln, &ss, &ts);
-        const V3 dl = cosine_hemisphere(w0, w1);
-        const V3 dir = norm((ss * dl.x + ts * dl.y) + ln * dl.z);
-        const float pdfDir = dl.z * kInvPI;
-        conn = lt_prepare(W, A, slot, lpos, ln, v3s(0.f), nullptr, Le, true);
+        V3 Le, lpos, ln, dir;
+        float pdfA, pdfDir;
+        if (!ENV || S.light_tri[light] >= 0) {
+            Le = lt_area_origin(S, light, u0, u1, w0, w1, &lpos, &ln, &dir, &pdfA, &pdfDir);
+            conn = lt_prepare(W, A, slot, lpos, ln, v3s(0.f), nullptr, Le, true);
+        } else {
+            // environment light: the light vertex's splat is 0 and is not made
+            Le = v3s(0.f);
+            lt_env_origin(S.env, A.C, u0, u1, w0, w1, &lpos, &dir, &pdfA, &pdfDir);
+            ln = dir;
+        }
         const V3 T = Le * fabsf(dot(ln, dir)) / pdfLight / pdfA / pdfDir;
         const V3 o = (lpos + dir * kRayEps) + dir * kRayEps;
         W.o[slot] = make_float4(o.x, o.y, o.z, 0.f);
@@ -1112,7 +1108,8 @@ int LightTracingWavefront(Ctx* c, Scene* sc, const PathParams& P, const LightCam
     for (int64_t base = 0; base < n && rc == GY_OK; base += cap) {
         const int64_t cnt = std::min(cap, n - base);
         hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(64), 0, c->stream, W);
-        hipLaunchKernelGGL(k_lt_start, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, c->stream, sc->dev, P, W, A, base, cnt);
+        hipLaunchKernelGGL(sc->dev.env.w > 0 ? k_lt_start<true> : k_lt_start<false>, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0,
+                           c->stream, sc->dev, P, W, A, base, cnt);
         for (int r = 0; r <= rounds; ++r) {
             if (r > 0) {
                 hipLaunchKernelGGL((k_wf_trace<kStackLds, 0, LightRender>), dim3(pblocks), dim3(256), 0, c->stream, Rd, W, kAutoRefill,

@@ -246,8 +246,10 @@ int gy_film_save(gy_scene* scene, const char* path);
  * ::Render with p->spp, p->max_depth and p->sampler.  Light paths from the
  * area lights, every vertex connected to the pinhole camera and splatted
  * (Film::AddSplat, visual/Film.h:89-103); rgb gets the W x H x 3 splat sums.
- * Area lights only (GY_ERR_UNSUPPORTED otherwise); dielectrics scatter light
- * paths in the importance transport mode.  p->mode picks the form:
+ * An environment light's paths carry zero throughput (GenerateLightPath takes
+ * Le = Emission(), zero for it: light/EnvLight.h:20-22) but are traced, so
+ * their NaNs show; a scene with no light is GY_ERR_INVALID.  Dielectrics
+ * scatter light paths in the importance transport mode.  p->mode picks the form:
  * GY_RENDER_WAVEFRONT traces every path's rays of one bounce together (queues
  * and persistent trace kernels), GY_RENDER_MEGAKERNEL one path per thread
  * (p->block_size 64, 128 or 256; 0 = 64).  Float atomics add the splats: the

@@ -1750,21 +1750,44 @@ template <class R, class F> void lt_path(const Scene<R>& s, const LTCamera<R>& c
     std::vector<LTVertex<R>> path(maxDepth + 1);
     R pdfLight;
     const int light = sample_one_light(s, (R)rnd.next(), &pdfLight);
-    const Tri<R>& lt = s.tris[s.light_tri[light]];
-    const V3<R> Le = s.light_le[light];
-    // AreaLight::SampleOnLight (light/AreaLight.h:60-71), Triangle::Sample (shape/Triangle.cpp:141-158)
     R u0 = (R)rnd.next(), u1 = (R)rnd.next();
     R v0 = (R)rnd.next(), v1 = (R)rnd.next();
-    R su0 = std::sqrt(u0);
-    R b0 = 1 - su0, b1 = u1 * su0;
-    V3<R> lpos = (b0 * lt.p0 + b1 * lt.p1) + (1 - b0 - b1) * lt.p2;
-    V3<R> ln = (b0 * lt.n0 + b1 * lt.n1) + (1 - b0 - b1) * lt.n2;
-    R pdfA = 1 / lt.area;
-    V3<R> ss, ts;
-    coordinate_system(ln, &ss, &ts);
-    V3<R> dl = cosine_hemisphere(v0, v1);
-    V3<R> dir = norm((ss * dl.x + ts * dl.y) + ln * dl.z);
-    R pdfDir = dl.z * K<R>::INV_PI;
+    V3<R> Le, lpos, ln, dir;
+    R pdfA, pdfDir;
+    if (s.light_tri[light] >= 0) {
+        const Tri<R>& lt = s.tris[s.light_tri[light]];
+        Le = s.light_le[light];
+        // AreaLight::SampleOnLight (light/AreaLight.h:60-71), Triangle::Sample (shape/Triangle.cpp:141-158)
+        R su0 = std::sqrt(u0);
+        R b0 = 1 - su0, b1 = u1 * su0;
+        lpos = (b0 * lt.p0 + b1 * lt.p1) + (1 - b0 - b1) * lt.p2;
+        ln = (b0 * lt.n0 + b1 * lt.n1) + (1 - b0 - b1) * lt.n2;
+        pdfA = 1 / lt.area;
+        V3<R> ss, ts;
+        coordinate_system(ln, &ss, &ts);
+        V3<R> dl = cosine_hemisphere(v0, v1);
+        dir = norm((ss * dl.x + ts * dl.y) + ln * dl.z);
+        pdfDir = dl.z * K<R>::INV_PI;
+    } else {
+        // EnvironmentLight::SampleLight (light/EnvLight.cpp:22-39): a direction from
+        // the map, a point on the bounding sphere's disk facing it.  Its radiance
+        // is dropped: GenerateLightPath takes Le = Emission(), and
+        // EnvironmentLight::Emission() is zero (light/EnvLight.h:20-22,
+        // integrator/LT.h:108-111), so these paths carry zero throughput (NaN
+        // where a pdf is zero).
+        Le = V3<R>(0, 0, 0);
+        V3<R> sd;
+        (void)env_sample(s.env, u0, u1, &sd, &pdfDir);
+        dir = -sd;
+        ln = dir;
+        V3<R> a1, a2;
+        coordinate_system(-dir, &a1, &a2);
+        R cx, cy;
+        concentric_disk(v0, v1, &cx, &cy);
+        const V3<R> pDisk = s.env.center + s.env.radius * (cx * a1 + cy * a2);
+        lpos = pDisk + s.env.radius * (-dir);
+        pdfA = 1 / (K<R>::PI * s.env.radius * s.env.radius);
+    }
     path[0].pos = lpos;
     path[0].n = ln;
     path[0].throughput = Le;
@@ -2244,7 +2267,7 @@ int or_set_shadow_short(int on) {
 int or_render_lt(void* hv, const or_camera_desc* cd, const or_render_params* p, float* splat) {
     SceneHandle* h = (SceneHandle*)hv;
     const Scene<float>& sc = h->sf;
-    if (sc.light_tri.empty() || sc.env.w > 0) return -1;
+    if (sc.light_tri.empty()) return -1;
     LTCamera<float> cam = make_lt_camera<float>(*cd);
     const int W = cd->width, H = cd->height;
     const int64_t total = (int64_t)p->spp * W * H;

@@ -28,6 +28,10 @@
  *                        chi-square of the 2-D inversion, uniform-map sphere
  *                        sampling, constant-sky furnace a Le cos^2(pi / 2h)
  *                        (test_env.py)
+ *   light tracing        light-only scene's splat sum Le W H; with a map, that
+ *                        times the area lights' share of pdfLight; LT and PT
+ *                        estimates agree; importance-mode transmission closed
+ *                        forms (test_lt.py)
  *   OBJ, image files     the reference's own tinyobjloader and stb built into
  *                        oracle/_ref (test_obj.py, test_image.py)
  * Parity unpinned: the MSVC CRT's last-ulp behaviour of sin/cos/acos/atan2/
@@ -118,7 +122,7 @@ int or_env_query(void* scene, int32_t n, const float* a, const float* b, const f
                  float* out_dir, float* out_pdf, int kind);
 int or_env_info(void* scene, float* radius, float* light_func, int32_t max_lights);
 /* LightTracing (integrator/LT.h) in float mode: splat = W x H x 3 sums of
- * Film::AddSplat.  -1: no area light or an environment map; -2: a dielectric. */
+ * Film::AddSplat.  -1: the scene has no light. */
 int or_render_lt(void* scene, const or_camera_desc* cam, const or_render_params* p, float* splat);
 /* Reference BVH statistics: node count, max depth. */
 int or_bvh_stats(void* scene, int64_t* n_nodes, int32_t* max_depth);

@@ -13,7 +13,10 @@ MSVC):
   transmission weights f |cos| / pdf are Tc exactly (radiance mode: e^2 Tc),
   rough-dielectric transmission values are e^2 times the radiance mode's
   (bsdf/BSDF.h:266-271, 505, 608);
-- scenes the integrator does not take (an environment light) are refused.
+- an environment light's paths carry zero throughput: GenerateLightPath takes
+  Le = Emission(), which EnvironmentLight leaves zero (light/EnvLight.h:20-22,
+  integrator/LT.h:108-111), so with a map added the light vertices' splats
+  sum to the area lights' share of the paths (their pdfLight) times Le W H.
 GPU tests compare the device's splat image with the oracle's, for both
 forms of the integrator (one light path per thread, and the wavefront that
 traces every path's rays of one bounce together): each light path's splats
@@ -25,7 +28,7 @@ import pytest
 
 import gyunity_amd as g
 from oracle import oracle as O
-from tests.scene_util import material, mixed_material_scene
+from tests.scene_util import env_image, material, mixed_material_scene
 
 S_REL = 1e-4  # per pixel, relative to max(|L|, 1e-3 of the image's mean)
 MODES = pytest.mark.parametrize("mode", [g.GY_RENDER_MEGAKERNEL, g.GY_RENDER_WAVEFRONT], ids=["megakernel", "wavefront"])
@@ -96,14 +99,26 @@ def test_lt_importance_mode_closed_forms():
     assert np.allclose(f_r[trans, 0] / f_i[trans, 0], 1 / 1.5 ** 2, rtol=1e-12)
 
 
-def test_lt_refuses_unsupported_scenes():
-    from tests.scene_util import env_image
+ENV_SCALE = 4.0  # the light-only scene's map power, near its quad's
 
-    sa, cam, prm = mixed_material_scene(res=16)
+
+def _p_area(os_):
+    _, f = os_.env_info()  # light powers, the environment light last
+    return float(f[:-1].astype(np.float64).sum() / f.astype(np.float64).sum())
+
+
+def test_lt_environment_paths_carry_zero():
+    le = (3.0, 2.0, 1.0)
+    sa, cam = _light_only(le)
     os_ = O.OracleScene(sa)
-    os_.set_environment(env_image(8, 4))
-    with pytest.raises(ValueError):
-        os_.render_lt(cam, O.params(1, 3, 5, 0, 1, 1))
+    os_.set_environment(env_image(16, 8), ENV_SCALE)
+    p_area = _p_area(os_)
+    assert 0.2 < p_area < 0.8, p_area
+    img = os_.render_lt(cam, O.params(16, 4, 5, 0, 1, 1))
+    total = img.astype(np.float64).sum(axis=(0, 1))
+    # Sobol's first dimension is stratified: the area lights' share of the
+    # 20480 paths is p_area to within a few paths
+    assert np.allclose(total, p_area * np.array(le) * cam.width * cam.height, rtol=2e-3), (total, p_area)
 
 
 # ---------------------------------------------------------------- GPU
@@ -169,14 +184,33 @@ def test_lt_gpu_light_only_closed_form_and_errors(gpu_ctx, mode):
             sc.render_light_tracing(cam, prm)
     total = img.astype(np.float64).sum(axis=(0, 1))
     assert np.allclose(total, np.array(le) * cam.width * cam.height, rtol=1e-4), total
-    from tests.scene_util import env_image
 
-    sa2, cam2, prm2 = mixed_material_scene(res=16)
-    with g.Scene(gpu_ctx, sa2) as sc:
-        sc.set_environment(env_image(8, 4))
-        prm2.mode = mode
-        with pytest.raises(g.GyError):
-            sc.render_light_tracing(cam2, prm2)
+
+@pytest.mark.gpu
+@MODES
+def test_lt_gpu_matches_oracle_environment(gpu_ctx, mode):
+    # environment-light paths: the disk point and direction are sampled and
+    # traced with zero throughput (NaN where a pdf is zero), as in the oracle
+    le = (3.0, 2.0, 1.0)
+    sa, cam = _light_only(le)
+    os_ = O.OracleScene(sa)
+    os_.set_environment(env_image(16, 8), ENV_SCALE)
+    prm = g.render_params(spp=16, max_depth=4, rr_depth=5, mode=mode)
+    with g.Scene(gpu_ctx, sa) as sc:
+        sc.set_environment(env_image(16, 8), ENV_SCALE)
+        img = sc.render_light_tracing(cam, prm)
+    _compare(img, os_.render_lt(cam, O.params(16, 4, 5, 0, 1, 1)))
+    total = img.astype(np.float64).sum(axis=(0, 1))
+    assert np.allclose(total, _p_area(os_) * np.array(le) * cam.width * cam.height, rtol=2e-3), total
+    sa, cam, prm = mixed_material_scene(res=40, closed=False)
+    prm.spp, prm.max_depth, prm.mode = 8, 6, mode
+    os_ = O.OracleScene(sa)
+    os_.set_environment(env_image(32, 16))
+    orc = os_.render_lt(cam, O.params(8, 6, 5, 0, 1, 1))
+    with g.Scene(gpu_ctx, sa) as sc:
+        sc.set_environment(env_image(32, 16))
+        gpu = sc.render_light_tracing(cam, prm)
+    _compare(gpu, orc)
 
 
 @pytest.mark.gpu
