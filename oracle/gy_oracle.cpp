@@ -2271,24 +2271,40 @@ int or_render_lt(void* hv, const or_camera_desc* cd, const or_render_params* p, 
     LTCamera<float> cam = make_lt_camera<float>(*cd);
     const int W = cd->width, H = cd->height;
     const int64_t total = (int64_t)p->spp * W * H;
-    for (size_t i = 0; i < (size_t)W * H * 3; ++i) splat[i] = 0.f;
-    auto add = [&](float x, float y, const V3<float>& L) {
-        int X = (int)std::floor(x), Y = (int)std::floor(y);
-        X = X < 0 ? 0 : (X > W - 1 ? W - 1 : X);
-        Y = Y < 0 ? 0 : (Y > H - 1 ? H - 1 : Y);
-        float* px = splat + ((size_t)Y * W + X) * 3;
-        px[0] = px[0] + L.x;
-        px[1] = px[1] + L.y;
-        px[2] = px[2] + L.z;
+    const size_t npx = (size_t)W * H * 3;
+    for (size_t i = 0; i < npx; ++i) splat[i] = 0.f;
+    // ParallelFor over image rows (integrator/LT.h:25-47): thread t takes rows
+    // t, t + n, ...  and adds its splats into a buffer of its own; the buffers
+    // are then summed in thread order (one thread: straight into splat).
+    int nth = p->nthreads > 0 ? p->nthreads : (int)std::thread::hardware_concurrency();
+    nth = std::max(1, std::min(nth, H));
+    std::vector<std::vector<float>> bufs(nth - 1, std::vector<float>(npx, 0.f));
+    auto worker = [&](int t) {
+        float* buf = t == 0 ? splat : bufs[t - 1].data();
+        auto add = [&](float x, float y, const V3<float>& L) {
+            int X = (int)std::floor(x), Y = (int)std::floor(y);
+            X = X < 0 ? 0 : (X > W - 1 ? W - 1 : X);
+            Y = Y < 0 ? 0 : (Y > H - 1 ? H - 1 : Y);
+            float* px = buf + ((size_t)Y * W + X) * 3;
+            px[0] = px[0] + L.x;
+            px[1] = px[1] + L.y;
+            px[2] = px[2] + L.z;
+        };
+        for (int y = t; y < H; y += nth)
+            for (int x = 0; x < W; ++x)
+                for (int smp = 0; smp < p->spp; ++smp) {
+                    Seq rnd;
+                    rnd.kind = p->sampler == 1 ? 1 : 0;
+                    rnd.instance = (uint64_t)W * H * smp + (uint64_t)y * W + x;
+                    lt_path(sc, cam, rnd, p->max_depth, p->use_bvh != 0, total, add);
+                }
     };
-    for (int y = 0; y < H; ++y)
-        for (int x = 0; x < W; ++x)
-            for (int smp = 0; smp < p->spp; ++smp) {
-                Seq rnd;
-                rnd.kind = p->sampler == 1 ? 1 : 0;
-                rnd.instance = (uint64_t)W * H * smp + (uint64_t)y * W + x;
-                lt_path(sc, cam, rnd, p->max_depth, p->use_bvh != 0, total, add);
-            }
+    std::vector<std::thread> th;
+    for (int t = 1; t < nth; ++t) th.emplace_back(worker, t);
+    worker(0);
+    for (auto& x : th) x.join();
+    for (int t = 1; t < nth; ++t)
+        for (size_t i = 0; i < npx; ++i) splat[i] = splat[i] + bufs[t - 1][i];
     return 0;
 }
 

@@ -99,6 +99,17 @@ def test_lt_importance_mode_closed_forms():
     assert np.allclose(f_r[trans, 0] / f_i[trans, 0], 1 / 1.5 ** 2, rtol=1e-12)
 
 
+def test_lt_oracle_threads_sum_the_same_splats():
+    # rows dealt to threads, per-thread buffers summed afterwards: the same
+    # splats in another summation order
+    sa, cam, _ = _cornell(24)
+    os_ = O.OracleScene(sa)
+    one = os_.render_lt(cam, O.params(8, 5, 5, 0, 1, 1))
+    four = os_.render_lt(cam, O.params(8, 5, 5, 0, 4, 1))
+    assert np.allclose(four, one, rtol=1e-5, atol=1e-6 * np.abs(one).max())
+    assert not np.array_equal(one, 0 * one)
+
+
 ENV_SCALE = 4.0  # the light-only scene's map power, near its quad's
 
 
