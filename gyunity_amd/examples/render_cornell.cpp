@@ -8,6 +8,7 @@
 // a gy::Error (hosts without a GPU) and exits 0.
 #include <gyunity/HipPathTracing.h>
 
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 
@@ -73,6 +74,23 @@ int main(int argc, char** argv) {
                     dev->Name().c_str(), W, H, rp.spp, wsum, W * H * rp.spp, sum / wsum, c[0], c[1], c[2],
                     bad ? "MISMATCH" : "films identical");
         if (wsum < (double)W * H * rp.spp) ++bad;
+
+        // LightTracing through gy::HipLightTracing: the film's splats equal a
+        // direct C-ABI render's up to the float-atomic summation order
+        gy::HipLightTracing lt(sampler, 5, 4, params, dev);
+        lt.Render(scene, camera);
+        gy_render_params p3 = lt.Params();
+        std::vector<float> spl(film->splat.size());
+        gy::Check(gy_render_light_tracing(ds.Handle(), &c2, &p3, spl.data()), "gy_render_light_tracing");
+        double lsum = 0, ldiff = 0;
+        for (size_t i = 0; i < spl.size(); ++i) {
+            lsum += std::fabs(spl[i]);
+            ldiff += std::fabs((double)spl[i] - film->splat[i]);
+        }
+        const bool lt_ok = lsum > 0 && ldiff <= 1e-5 * lsum;
+        std::printf("light tracing: splat sum %.4f, relative difference %.2e, %s\n", lsum, ldiff / (lsum > 0 ? lsum : 1),
+                    lt_ok ? "splats agree" : "SPLAT MISMATCH");
+        if (!lt_ok) ++bad;
         return bad ? 1 : 0;
     } catch (const gy::Error& e) {
         std::printf("error: %s\n", e.what());

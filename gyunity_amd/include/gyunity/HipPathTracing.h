@@ -70,15 +70,18 @@ private:
 // ---------------------------------------------------------------- film
 class Film {
 public:
-    Film(int w, int h) : mResX(w), mResY(h), color((size_t)w * h * 3, 0.f), weight((size_t)w * h, 0.f) {}
+    Film(int w, int h)
+        : mResX(w), mResY(h), color((size_t)w * h * 3, 0.f), weight((size_t)w * h, 0.f), splat((size_t)w * h * 3, 0.f) {}
     int mResX, mResY;
     std::vector<float> color;   // rgb sums, row-major
     std::vector<float> weight;  // box-filter weights
-    // Pixel value as Film::WriteToPixelBuffer forms it (no splats on this path).
+    std::vector<float> splat;   // Film::AddSplat sums (light tracing), rgb
+    // Pixel value as Film::WriteToPixelBuffer forms it: color / weight + splat
+    // (visual/Film.h:128-135).
     Vec3 Pixel(int x, int y) const {
         const size_t i = (size_t)y * mResX + x;
         const float w = weight[i];
-        return {color[3 * i] / w, color[3 * i + 1] / w, color[3 * i + 2] / w};
+        return {color[3 * i] / w + splat[3 * i], color[3 * i + 1] / w + splat[3 * i + 1], color[3 * i + 2] / w + splat[3 * i + 2]};
     }
 };
 
@@ -407,6 +410,56 @@ private:
     int mMaxDepth;
     std::shared_ptr<Sampler> mpSampler;
     std::shared_ptr<SamplerEnum> mpSamplerEnum;
+    RenderParams mParams;
+    std::shared_ptr<Device> mpDevice;
+    std::unique_ptr<DeviceScene> mpScene;
+    const Scene* mpSceneSrc = nullptr;
+};
+
+// LightTracing on the GPU: LightTracing(pSampler, maxDepth, spp)
+// (integrator/LT.h:14-16).  Render() uploads the scene as HipPathTracing does
+// and adds the light paths' splats into camera.GetFilm()->splat, as
+// Film::AddSplat does (visual/Film.h:89-103).  params.mode picks the form
+// (wavefront or one path per thread: the same splats, summed in another
+// order); params.blockSize is the per-path form's block size.
+class HipLightTracing : public Integrator {
+public:
+    HipLightTracing(const std::shared_ptr<Sampler>& pSampler, int maxDepth, int spp, RenderParams params = RenderParams(),
+                    std::shared_ptr<Device> device = nullptr)
+        : mpSampler(pSampler), mMaxDepth(maxDepth), mSpp(spp), mParams(params),
+          mpDevice(device ? std::move(device) : std::make_shared<Device>(0)) {
+        if (!mpSampler) throw Error(GY_ERR_INVALID, "HipLightTracing: no sampler");
+    }
+
+    void Render(const Scene& scene, const Camera& camera) override {
+        if (mpSceneSrc != &scene || !mpScene) {
+            mpScene.reset();
+            mpScene.reset(new DeviceScene(*mpDevice, scene));
+            mpSceneSrc = &scene;
+        }
+        const gy_camera_desc cam = camera.Desc();
+        const gy_render_params p = Params();
+        auto film = camera.GetFilm();
+        std::vector<float> rgb(film->splat.size());
+        Check(gy_render_light_tracing(mpScene->Handle(), &cam, &p, rgb.data()), "gy_render_light_tracing");
+        for (size_t i = 0; i < rgb.size(); ++i) film->splat[i] = film->splat[i] + rgb[i];
+    }
+
+    gy_render_params Params() const {
+        gy_render_params p = {};
+        p.spp = mSpp;
+        p.max_depth = mMaxDepth;
+        p.rr_depth = mParams.rrStartDepth;
+        p.mode = mParams.mode;
+        p.block_size = mParams.blockSize;
+        p.sampler = mpSampler->Kind();
+        return p;
+    }
+
+private:
+    std::shared_ptr<Sampler> mpSampler;
+    int mMaxDepth;
+    int mSpp;
     RenderParams mParams;
     std::shared_ptr<Device> mpDevice;
     std::unique_ptr<DeviceScene> mpScene;
