@@ -1,0 +1,75 @@
+"""Per-sample parity against the oracle (level b), shared by the GPU tests."""
+import numpy as np
+
+import gyunity_amd as g
+from gyunity_amd import RAY_DTYPE
+from oracle import oracle as O
+
+T_REL = 1e-6      # hit distance, relative
+L_REL = 1e-4      # per-sample radiance, relative to max(|L|, 1e-3)
+
+
+def assert_tie_explains(sc, os_, cam, prm_o, x, y, smp):
+    """A sample whose GPU and oracle paths differ must differ first at a
+    closest hit that the reference's own rules leave open.  The oracle's
+    per-bounce record gives every continuation ray of its path (camera and
+    continuation rays have tMax = infinity; the record holds the tMax the hit
+    shrank it to); the GPU traces the same rays.  Up to the first bounce where
+    the two hit different triangles the paths are the same, and there either
+    - both hits lie at the same distance (T_REL): an exact-t tie, which the
+      reference breaks in its BVH's visit order; or
+    - the GPU's hit is the oracle's brute-force hit over the same triangle
+      test (BruteForce::Intersect, accelerator/BruteForceAccel.h:20-38): the
+      reference BVH's slab test, which is not conservative
+      (accelerator/BVHAccel.cpp:127-149), culled it.
+    Returns "tie" or "bvh"."""
+    rows = os_.li_path(cam, prm_o, x, y, smp)
+    assert rows.shape[0] > 0
+    rays = np.zeros(rows.shape[0], RAY_DTYPE)
+    rays["org"], rays["dir"] = rows[:, 0:3], rows[:, 3:6]
+    rays["tmin"], rays["tmax"] = rows[:, 6], np.inf
+    hits = sc.trace_closest(rays)
+    prim_o = rows[:, 8].astype(np.int64)
+    diff = np.nonzero(hits["prim"] != prim_o)[0]
+    assert diff.size > 0, "sample (%d, %d, %d) differs with every closest hit equal" % (x, y, smp)
+    k = diff[0]
+    tg = float(hits["t"][k])
+    if hits["prim"][k] >= 0 and prim_o[k] >= 0 and abs(tg - rows[k, 9]) <= T_REL * abs(rows[k, 9]):
+        return "tie"
+    tb, pb, _, _ = os_.trace_closest(rays[k:k + 1], False)
+    same = pb[0] == hits["prim"][k] or (pb[0] >= 0 and hits["prim"][k] >= 0 and abs(tg - tb[0]) <= T_REL * abs(tb[0]))
+    assert same, ("not a tie, not the brute-force hit", x, y, smp, k, hits["prim"][k], tg, prim_o[k], rows[k, 9], pb[0], tb[0])
+    return "bvh"
+
+
+def li_parity(gpu_ctx, sa, cam, prm, n, seed, xys=None, env=None):
+    """Per-sample radiance on identical sample indices (level b).  Every sample
+    agrees within L_REL (maximum over samples and channels), or its path
+    differs from the oracle's and the difference starts at an exact-t tie."""
+    if xys is None:
+        rng = np.random.default_rng(seed)
+        x = rng.integers(0, cam.width, n).astype(np.int32)
+        y = rng.integers(0, cam.height, n).astype(np.int32)
+        s = rng.integers(0, prm.spp, n).astype(np.int32)
+    else:
+        x, y, s = (np.ascontiguousarray(a, np.int32) for a in xys)
+    os_ = O.OracleScene(sa)
+    if env is not None:
+        os_.set_environment(env)
+    prm_o = O.params(prm.spp, prm.max_depth, prm.rr_depth, 0, 8, 1, sampler=prm.sampler)
+    Lo, ro, do = os_.li_samples(cam, prm_o, x, y, s)
+    with g.Scene(gpu_ctx, sa) as sc:
+        if env is not None:
+            sc.set_environment(env)
+        Lg, rg, dg = sc.li_samples(cam, prm, x, y, s)
+        assert np.array_equal(rg, ro), "film positions differ"
+        with np.errstate(invalid="ignore"):
+            err = np.abs(Lg - Lo).max(axis=1) / np.maximum(np.abs(Lo).max(axis=1), 1e-3)
+            ok = (dg == do) & (err <= L_REL)
+        # Exact-t ties on shared triangle edges are broken by (t, primitive id)
+        # here and by the reference BVH's visit order in the oracle; the rest of
+        # such a path then differs.  Each one is traced back to its tie.
+        assert (~ok).mean() < 2e-3, (~ok).mean()
+        for i in np.nonzero(~ok)[0]:
+            assert_tie_explains(sc, os_, cam, prm_o, int(x[i]), int(y[i]), int(s[i]))
+    return err[ok]

@@ -1,4 +1,5 @@
 """Small hand-built scenes and ray batches shared by the tests."""
+import os
 import struct
 
 import numpy as np
@@ -52,11 +53,12 @@ def box_triangles(lo, hi, inward):
 
 def furnace_scene(albedo=0.5, le=1.0):
     """Closed unit box; every triangle is an inward-facing emitter with a Lambert
-    material of the given albedo."""
+    material of the given albedo (a scalar, or one value per channel)."""
     pos, fn = box_triangles((0, 0, 0), (1, 1, 1), inward=True)
     n = pos.shape[0]
     nrm = np.repeat(fn, 3, axis=0).reshape(-1, 9)
-    mats = material(0, R=(albedo,) * 3)[None, :]
+    R = tuple(albedo) if np.ndim(albedo) else (albedo,) * 3
+    mats = material(0, R=R)[None, :]
     return SceneArrays(pos, nrm, fn, np.zeros(n, np.uint32), np.arange(n, dtype=np.int32), mats,
                        np.arange(n, dtype=np.int32), np.full((n, 3), le, np.float32))
 
@@ -244,3 +246,105 @@ def env_image(w=64, h=32, seed=0):
     sun = np.exp(-((du / 0.04) ** 2 + ((v[:, None] - 0.25) / 0.06) ** 2)) * 40.0
     img = sky * rng.uniform(0.8, 1.2, (h, w, 3)) + sun[..., None] * np.array([1.0, 0.9, 0.7])
     return img.astype(np.float32)
+
+
+# ---------------------------------------------------------------- film footprint by brute force
+_GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def sobol_fixture_matrices():
+    """The reference's Sobol generator matrices (sampler/Sobol.cpp:38-10689),
+    first 256 dimensions x 52 columns, from the committed fixture."""
+    return np.fromfile(os.path.join(_GOLDEN, "sobol_matrices.u32"), dtype="<u4").reshape(256, 52)
+
+
+def sobol_values_u32(index, cols):
+    """XOR of the matrix columns selected by the set bits of index: the 32-bit
+    fixed-point value of sobol::sample (sampler/Sobol.h:46-61) before the
+    conversion to float."""
+    r = np.zeros(index.shape, np.uint32)
+    nbytes = (max(1, int(index.max()).bit_length()) + 7) // 8 if index.size else 1
+    for b in range(nbytes):
+        # XOR of the columns 8b..8b+7 for every value of index byte b
+        tab = np.zeros(256, np.uint32)
+        for v in range(1, 256):
+            low = v & -v
+            tab[v] = tab[v ^ low] ^ cols[8 * b + low.bit_length() - 1]
+        r ^= tab[((index >> np.uint64(8 * b)) & np.uint64(255)).astype(np.intp)]
+    return r
+
+
+def _sample_positions(W, H, spp, chunk=1 << 22):
+    """Every camera sample of TiledIntegrator::Render with the Sobol sampler
+    (scramble 0), in chunks: (index, px, py, u, v, u wrapped, v wrapped).
+
+    - Frame f of every pixel of the 2^m grid (res = 2^m >= max(W, H)) uses
+      the indices [f 4^m, (f + 1) 4^m), so frames 0..spp-1 are exactly the
+      indices [0, spp 4^m), and the frame of index i is i >> 2m.
+    - An index belongs to the pixel given by the top m bits of the exact 32-bit
+      values of dimensions 0 and 1; pixels outside W x H are not rendered.
+    - u = res * s - x in FP32 with s the FP32 sample, which RandomStateSequence
+      maps from >= 1 to 0 (sampler/Sampler.h:60-62; SobolEnum::SampleX,
+      sampler/SobolEnum.h:30-32)."""
+    M = sobol_fixture_matrices()
+    res = 1
+    while res < max(W, H):
+        res *= 2
+    m = res.bit_length() - 1
+    n = spp * res * res
+    sh = np.uint32(32 - m)
+    for a in range(0, n, chunk):
+        idx = np.arange(a, min(n, a + chunk), dtype=np.uint64)
+        r0 = sobol_values_u32(idx, M[0])
+        r1 = sobol_values_u32(idx, M[1])
+        px = (r0 >> sh).astype(np.int64) if m > 0 else np.zeros(idx.shape, np.int64)
+        py = (r1 >> sh).astype(np.int64) if m > 0 else np.zeros(idx.shape, np.int64)
+        keep = (px < W) & (py < H)
+        idx, px, py, r0, r1 = idx[keep], px[keep], py[keep], r0[keep], r1[keep]
+        s0 = r0.astype(np.float32) * np.float32(2.0 ** -32)
+        s1 = r1.astype(np.float32) * np.float32(2.0 ** -32)
+        w0, w1 = s0 >= 1, s1 >= 1
+        s0[w0] = 0
+        s1[w1] = 0
+        u = np.float32(res) * s0 - px.astype(np.float32)
+        v = np.float32(res) * s1 - py.astype(np.float32)
+        yield idx >> np.uint64(2 * m), px, py, u, v, w0, w1
+
+
+def film_weight_scan(W, H, spp):
+    """Per-pixel film weights of TiledIntegrator::Render with the Sobol sampler
+    and the radius-0.5 box filter, predicted by brute force over every sample
+    index (_sample_positions) rather than by Film::AddSample's arithmetic: a
+    box filter of radius 0.5 about the pixel centres counts a sample once in
+    every pixel whose closed unit square [j, j+1] x [i, i+1] holds its raster
+    position x + u, y + v (FP32), inside the image.
+
+    Returns (weights (H, W) int64, counts) where counts tallies the samples
+    with u == 0, u == 1 and u wrapped from 1 to 0."""
+    wgt = np.zeros(H * W, np.int64)
+    counts = {"u0": 0, "u1": 0, "wrap": 0}
+    for _, px, py, u, v, w0, _ in _sample_positions(W, H, spp):
+        fx = (px.astype(np.float32) + u).astype(np.float64)
+        fy = (py.astype(np.float32) + v).astype(np.float64)
+        counts["u0"] += int(np.count_nonzero(u == 0))
+        counts["u1"] += int(np.count_nonzero(u == 1))
+        counts["wrap"] += int(np.count_nonzero(w0))
+        # closed squares holding the point: floor(p), and p - 1 when p is an integer
+        xs = [np.floor(fx), np.where(fx == np.floor(fx), fx - 1, np.nan)]
+        ys = [np.floor(fy), np.where(fy == np.floor(fy), fy - 1, np.nan)]
+        for X in xs:
+            for Y in ys:
+                ok = (X >= 0) & (X < W) & (Y >= 0) & (Y < H)
+                wgt += np.bincount((Y[ok] * W + X[ok]).astype(np.int64), minlength=H * W)
+    return wgt.reshape(H, W), counts
+
+
+def film_edge_samples(W, H, spp):
+    """(x, y, sample) of every camera sample that lies on a pixel edge (u or v
+    equal to 0 or 1) or whose u or v wrapped from 1 to 0."""
+    out = []
+    for frame, px, py, u, v, w0, w1 in _sample_positions(W, H, spp):
+        e = (u == 0) | (u == 1) | (v == 0) | (v == 1) | w0 | w1
+        out.append(np.stack([px[e], py[e], frame[e].astype(np.int64)], 1))
+    xys = np.concatenate(out).astype(np.int32)
+    return xys[:, 0], xys[:, 1], xys[:, 2]
