@@ -122,7 +122,7 @@ __device__ void replay_path(const SceneDev& S, const PathParams& P, int px, int 
     V3 L = v3s(0.f);
     for (int i = 0; i < P.max_depth; ++i) {
         Hit h;
-        if (!traverse<false, STACK>(S, o, d, 0.f, kInf, &h, stack, stride, nullptr, nullptr)) break;
+        if (!find_hit<false, STACK>(S, o, d, 0.f, kInf, &h, stack, stride)) break;
         const Isect it = query(S, o, d, h);
         const BSDF b = make_bsdf(S, it);
         const uint32_t* mw = S.mats + (size_t)it.mat * 32;

@@ -154,10 +154,8 @@ extern "C" int gy_batch_render(gy_batch* bh) {
         c->stats.shadow_rays = (int64_t)cnt[1];
         c->stats.mis_rays = (int64_t)cnt[2];
         for (const BatchEntry& en : b.entries) {
-            uint32_t ovf[2];
-            e = hipMemcpy(ovf, en.film.side_count, 8, hipMemcpyDeviceToHost);
-            if (e != hipSuccess) return HipFail(e, "batch film counters");
-            if (ovf[1] != 0) return Fail(GY_ERR_UNSUPPORTED, "film side list overflow");
+            rc = FilmCheck(en.film);
+            if (rc != GY_OK) return rc;
         }
     }
     return rc;

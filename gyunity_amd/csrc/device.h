@@ -306,8 +306,10 @@ GY_D bool ref_box(V3 mn, V3 mx, V3 o, V3 inv, float rtmin, float rtmax) {
 }
 
 // One candidate triangle (p0, p1, p2) as the reference tests it: leaf bounds
-// gate, then Triangle::Intersect with e1 = p1 - p0, e2 = p2 - p0
-// (shape/Triangle.h:107-111).  The bounds gate takes the ray's own tMax
+// gate and Triangle::Intersect with e1 = p1 - p0, e2 = p2 - p0
+// (shape/Triangle.h:107-111).  Both tests are pure and the hit needs both, so
+// the triangle test runs first and the bounds gate only for its hits, which
+// are few.  The bounds gate takes the ray's own tMax
 // (box_tmax) and the triangle test the current closest distance (tmax).  In
 // the reference both take the shrinking tMax, which makes the winner of an
 // exact-t tie depend on visit order (a flat leaf box entered exactly at the
@@ -317,10 +319,15 @@ GY_D bool ref_box(V3 mn, V3 mx, V3 o, V3 inv, float rtmin, float rtmax) {
 // traversal visits them in; for any-hit queries the two bounds are the same.
 GY_D bool ref_triangle(V3 p0, V3 p1, V3 p2, V3 o, V3 d, V3 inv, float tmin, float box_tmax, float tmax, float* t,
                        float* b1, float* b2) {
+    float th, u, v;
+    if (!tri_test(p0, p1 - p0, p2 - p0, o, d, tmin, tmax, &th, &u, &v)) return false;
     const V3 mn = v3(rmin(rmin(p0.x, p1.x), p2.x), rmin(rmin(p0.y, p1.y), p2.y), rmin(rmin(p0.z, p1.z), p2.z));
     const V3 mx = v3(rmax(rmax(p0.x, p1.x), p2.x), rmax(rmax(p0.y, p1.y), p2.y), rmax(rmax(p0.z, p1.z), p2.z));
     if (!ref_box(mn, mx, o, inv, tmin, box_tmax)) return false;
-    return tri_test(p0, p1 - p0, p2 - p0, o, d, tmin, tmax, t, b1, b2);
+    *t = th;
+    *b1 = u;
+    *b2 = v;
+    return true;
 }
 
 // Closest-hit / any-hit BVH traversal with a per-thread stack column in LDS
@@ -550,6 +557,30 @@ GY_D int trav_run(const SceneDev& Sc, Trav& t, uint32_t* lds, int stride, uint32
         if (t.node == kTravNone) return 1;
         if (__popcll(__ballot(true)) < refill) return 0;
     }
+}
+
+// One ray through trav_run to its end: the per-path kernels (megakernel,
+// per-sample, light tracing, adjoint replay) use the wavefront's traversal.
+// The lanes of a wave that call it together walk the tree with postponed
+// leaves; with refill 0 it returns only when the ray is finished.  The stack
+// never holds more entries than the BVH is deep, which the host keeps below
+// STACK, so the ring never spills and no spill buffer is passed.  COUNT: the
+// counting pass keeps the depth-first traverse(), whose visits are the
+// algorithmic bytes' lower bound.
+template <bool ANY, int STACK, bool COUNT = false>
+GY_D bool find_hit(const SceneDev& S, V3 o, V3 d, float tmin, float tmax, Hit* hit, uint32_t* stack, int stride,
+                   uint32_t* n_node = nullptr, uint32_t* n_tri = nullptr) {
+    if (COUNT) return traverse<ANY, STACK>(S, o, d, tmin, tmax, hit, stack, stride, n_node, n_tri);
+    Trav t;
+    trav_init(t, o, d, tmin, tmax);
+    const int r = trav_run<ANY, STACK>(S, t, stack, stride, nullptr, 0, 0);
+    if (ANY) return r == 2;
+    if (t.best_prim < 0) return false;
+    hit->t = t.best;
+    hit->prim = t.best_prim;
+    hit->b1 = t.bb1;
+    hit->b2 = t.bb2;
+    return true;
 }
 
 // ------------------------------------------------------------------ surface interaction

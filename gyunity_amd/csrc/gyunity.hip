@@ -264,56 +264,54 @@ __global__ void __launch_bounds__(BLOCK) k_lt(SceneDev S, PathParams P, LightCam
 }
 
 // ------------------------------------------------------------------ film
-// Each pixel's owner thread adds the pass's own samples in sample order
-// (Film::AddSample with weight 1: weight += 1, color = color + L, visual/Film.h:80-85).
+// Each pixel's owner thread adds the pass's samples that cover it, in sample
+// order (Film::AddSample with weight 1: weight += 1, color = color + L,
+// visual/Film.h:80-85).  For each pass sample it takes its own slot when the
+// slot's footprint mask holds its pixel, then the slots of the neighbouring
+// pixels (offsets in bit order, around the image's edges) whose masks hold
+// it.  Neighbours are looked at only when their border flag is set, so the
+// common case reads one slot per sample.  The result does not depend on
+// scheduling.
 __global__ void k_film_accumulate(FilmDev F, int n_s) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t WH = (int64_t)F.width * F.height;
+    const int W = F.width, H = F.height;
+    const int64_t WH = (int64_t)W * H;
     if (p >= WH) return;
+    const int x = (int)(p % W), y = (int)(p / W);
+    // bit b = (dy + 1) * 3 + dx + 1: the source pixel (x - dx, y - dy) has border samples
+    uint32_t nb = 0;
+    int64_t src[9];
+    for (int b = 0; b < 9; ++b) {
+        const int dx = b % 3 - 1, dy = b / 3 - 1;
+        int sx = x - dx, sy = y - dy;
+        sx = sx < 0 ? sx + W : (sx >= W ? sx - W : sx);
+        sy = sy < 0 ? sy + H : (sy >= H ? sy - H : sy);
+        src[b] = (int64_t)sy * W + sx;
+        if (b != 4 && F.border[src[b]]) nb |= 1u << b;
+    }
+    const bool own_border = F.border[p] != 0u;
     float4 acc = F.film[p];
     for (int s = 0; s < n_s; ++s) {
-        const float4 e = F.pass[(int64_t)s * WH + p];
-        if (e.w != 0.f) {
+        const int64_t base = (int64_t)s * WH;
+        const float4 e = F.pass[base + p];
+        if (own_border ? (((uint32_t)e.w & kFilmOwn) != 0u) : (e.w != 0.f)) {
             acc.w += 1.0f;
             acc.x = acc.x + e.x;
             acc.y = acc.y + e.y;
             acc.z = acc.z + e.z;
         }
+        for (uint32_t m = nb; m != 0u; m &= m - 1u) {
+            const int b = __builtin_ctz(m);
+            const float4 q = F.pass[base + src[b]];
+            if (((uint32_t)q.w >> b) & 1u) {
+                acc.w += 1.0f;
+                acc.x = acc.x + q.x;
+                acc.y = acc.y + q.y;
+                acc.z = acc.z + q.z;
+            }
+        }
     }
     F.film[p] = acc;
-}
-
-// Side entries (samples straddling pixel borders, see record_sample) are ranked
-// by key (target pixel, source slot), which orders them deterministically, and
-// each target's run is applied by one thread in key order.
-__global__ void k_side_rank(FilmDev F) {
-    const uint32_t n = min(*F.side_count, F.side_cap);
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint64_t k = F.side[i].key;
-    uint32_t r = 0;
-    for (uint32_t j = 0; j < n; ++j) r += (F.side[j].key < k) ? 1u : 0u;
-    F.side_sorted[r] = F.side[i];
-}
-__global__ void k_side_apply(FilmDev F) {
-    const uint32_t n = min(*F.side_count, F.side_cap);
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t tgt = (uint32_t)(F.side_sorted[i].key >> 32);
-    if (tgt >= (uint32_t)F.width * (uint32_t)F.height) return;
-    if (i > 0 && (uint32_t)(F.side_sorted[i - 1].key >> 32) == tgt) return;
-    float4 acc = F.film[tgt];
-    for (uint32_t j = i; j < n && (uint32_t)(F.side_sorted[j].key >> 32) == tgt; ++j) {
-        const float4 e = F.side_sorted[j].L;
-        acc.w += 1.0f;
-        acc.x = acc.x + e.x;
-        acc.y = acc.y + e.y;
-        acc.z = acc.z + e.z;
-    }
-    F.film[tgt] = acc;
-}
-__global__ void k_side_overflow(FilmDev F) {
-    if (threadIdx.x == 0 && blockIdx.x == 0 && *F.side_count > F.side_cap) atomicMax(F.side_overflow, *F.side_count);
 }
 
 // ------------------------------------------------------------------ launch helpers
@@ -602,9 +600,7 @@ void SceneFree(Scene* sc) {
 void FilmFree(FilmState* f) {
     DevFree(f->film);
     DevFree(f->pass);
-    DevFree(f->side);
-    DevFree(f->side_sorted);
-    DevFree(f->side_count);
+    DevFree(f->border);
     DevFree(f->counters);
     *f = FilmState();
 }
@@ -613,8 +609,11 @@ int FilmPrepareState(Ctx* c, FilmState& f, int W, int H, int64_t pass_slots) {
     const int64_t WH = (int64_t)W * H;
     if (f.width != W || f.height != H || !f.film) {
         DevFree(f.film);
+        DevFree(f.border);
         f.film = nullptr;
+        f.border = nullptr;
         int rc = DevAlloc(c, (void**)&f.film, (size_t)WH * sizeof(float4));
+        if (rc == GY_OK) rc = DevAlloc(c, (void**)&f.border, (size_t)(WH + 1) * sizeof(uint32_t));
         if (rc != GY_OK) return rc;
         f.width = W;
         f.height = H;
@@ -626,20 +625,32 @@ int FilmPrepareState(Ctx* c, FilmState& f, int W, int H, int64_t pass_slots) {
         if (rc != GY_OK) return rc;
         f.pass_slots = pass_slots;
     }
-    if (!f.side) {
-        f.side_cap = 1u << 18;
-        int rc = DevAlloc(c, (void**)&f.side, (size_t)f.side_cap * sizeof(SideEntry));
-        if (rc == GY_OK) rc = DevAlloc(c, (void**)&f.side_sorted, (size_t)f.side_cap * sizeof(SideEntry));
-        if (rc == GY_OK) rc = DevAlloc(c, (void**)&f.side_count, 16);
-        if (rc == GY_OK) rc = DevAlloc(c, (void**)&f.counters, 16 * sizeof(unsigned long long));
+    if (!f.counters) {
+        int rc = DevAlloc(c, (void**)&f.counters, 16 * sizeof(unsigned long long));
         if (rc != GY_OK) return rc;
     }
+    return FilmClear(c, f);
+}
+
+// Zero the film, its border flags and error count, and its ray counters
+// (fill kernels only, so a recorded graph holds no memset node).
+int FilmClear(Ctx* c, const FilmState& f) {
+    const int64_t WH = (int64_t)f.width * f.height;
     hipLaunchKernelGGL(k_fill_f4, dim3((unsigned)((WH + 255) / 256)), dim3(256), 0, c->stream, f.film, WH);
     int rc = CheckLaunch(c, "film clear");
-    if (rc != GY_OK) return rc;
-    rc = Fill32(c, f.side_count, 0u, 4);
-    if (rc != GY_OK) return rc;
-    return Fill32(c, f.counters, 0u, 32);
+    if (rc == GY_OK) rc = Fill32(c, f.border, 0u, WH + 1);
+    if (rc == GY_OK) rc = Fill32(c, (uint32_t*)f.counters, 0u, 32);
+    return rc;
+}
+
+// GY_OK, or the error for samples whose footprint reached beyond the
+// neighbouring pixels (counted by film_record).
+int FilmCheck(const FilmState& f) {
+    uint32_t far = 0;
+    hipError_t e = hipMemcpy(&far, f.border + (int64_t)f.width * f.height, 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return HipFail(e, "film error count");
+    if (far != 0) return Fail(GY_ERR_UNSUPPORTED, "film: a sample's footprint reaches beyond its neighbouring pixels");
+    return GY_OK;
 }
 
 int FilmPrepare(Ctx* c, Scene* sc, int W, int H, int n_pass) { return FilmPrepareState(c, sc->film, W, H, (int64_t)W * H * n_pass); }
@@ -647,11 +658,7 @@ FilmDev FilmViewState(const FilmState& f) {
     FilmDev F;
     F.film = f.film;
     F.pass = f.pass;
-    F.side = f.side;
-    F.side_sorted = f.side_sorted;
-    F.side_count = f.side_count;
-    F.side_overflow = f.side_count + 1;
-    F.side_cap = f.side_cap;
+    F.border = f.border;
     F.width = f.width;
     F.height = f.height;
     return F;
@@ -662,16 +669,10 @@ int FilmResolveState(Ctx* c, const FilmState& f, int n_s) {
     const int64_t WH = (int64_t)F.width * F.height;
     TimerBegin(c, kKernFilm);
     hipLaunchKernelGGL(k_film_accumulate, dim3((unsigned)((WH + 255) / 256)), dim3(256), 0, c->stream, F, n_s);
+    TimerEnd(c);
     int rc = CheckLaunch(c, "k_film_accumulate");
     if (rc != GY_OK) return rc;
-    const unsigned sb = (F.side_cap + 255) / 256;
-    hipLaunchKernelGGL(k_side_overflow, dim3(1), dim3(64), 0, c->stream, F);
-    hipLaunchKernelGGL(k_side_rank, dim3(sb), dim3(256), 0, c->stream, F);
-    hipLaunchKernelGGL(k_side_apply, dim3(sb), dim3(256), 0, c->stream, F);
-    TimerEnd(c);
-    rc = CheckLaunch(c, "side list");
-    if (rc != GY_OK) return rc;
-    return Fill32(c, F.side_count, 0u, 1);
+    return Fill32(c, F.border, 0u, WH);  // border flags are per pass
 }
 int FilmResolvePass(Ctx* c, Scene* sc, int n_s) { return FilmResolveState(c, sc->film, n_s); }
 
@@ -1216,11 +1217,10 @@ extern "C" int gy_render(gy_scene* s, const gy_camera_desc* cam, const gy_render
     c->stats.paths = (int64_t)cam->width * cam->height * p->spp;
     if (rc == GY_OK) {
         unsigned long long h[8];
-        uint32_t ovf[2];
         e = hipMemcpy(h, sc->film.counters, sizeof(h), hipMemcpyDeviceToHost);
-        if (e == hipSuccess) e = hipMemcpy(ovf, sc->film.side_count, 8, hipMemcpyDeviceToHost);
         if (e != hipSuccess) return HipFail(e, "render counters");
-        if (ovf[1] != 0) return Fail(GY_ERR_UNSUPPORTED, "film side list overflow");
+        rc = FilmCheck(sc->film);
+        if (rc != GY_OK) return rc;
         c->stats.rays = (int64_t)(h[0] + h[1] + h[2]);
         c->stats.shadow_rays = (int64_t)h[1];
         c->stats.mis_rays = (int64_t)h[2];

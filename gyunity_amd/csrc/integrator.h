@@ -61,7 +61,7 @@ __device__ DIParts di_parts(const SceneDev& S, const Isect& it, const BSDF& b, f
                 uint32_t* nn = COUNT ? &pc.nodes[1] : nullptr;
                 uint32_t* nt = COUNT ? &pc.tris[1] : nullptr;
                 pc.shadow++;
-                if (traverse<true, STACK>(S, so, sd, 0.f, stmax, &hs, stack, stride, nn, nt)) Li = v3s(0.f);
+                if (find_hit<true, STACK, COUNT>(S, so, sd, 0.f, stmax, &hs, stack, stride, nn, nt)) Li = v3s(0.f);
                 r.w1 = power_heuristic(pdf, sp);
                 r.L1 = Li * v3s(1.f) * f * c / pdf;
             }
@@ -85,7 +85,7 @@ __device__ DIParts di_parts(const SceneDev& S, const Isect& it, const BSDF& b, f
                 uint32_t* nn = COUNT ? &pc.nodes[2] : nullptr;
                 uint32_t* nt = COUNT ? &pc.tris[2] : nullptr;
                 pc.mis++;
-                if (traverse<false, STACK>(S, ro, wi, 0.f, kInf, &h2, stack, stride, nn, nt)) {
+                if (find_hit<false, STACK, COUNT>(S, ro, wi, 0.f, kInf, &h2, stack, stride, nn, nt)) {
                     const int hl = __float_as_int(S.shade[(size_t)h2.prim * 8 + 1].w);
                     if (hl == light) {
                         const Isect li = query(S, ro, wi, h2);
@@ -128,7 +128,7 @@ __device__ V3 trace_path(const SceneDev& S, const PathParams& P, int px, int py,
         pc.rays++;
         uint32_t* nn = COUNT ? &pc.nodes[0] : nullptr;
         uint32_t* nt = COUNT ? &pc.tris[0] : nullptr;
-        if (!traverse<false, STACK>(S, o, d, 0.f, kInf, &h, stack, stride, nn, nt)) {
+        if (!find_hit<false, STACK, COUNT>(S, o, d, 0.f, kInf, &h, stack, stride, nn, nt)) {
             // integrator/PT.cpp:18-21: T * Le(ray) in place of L
             if (ENV && (i == 0 || delta) && S.env.w > 0) L = T * env_emission_call(S.env, d);
             break;
@@ -221,7 +221,7 @@ __device__ void lt_connect(const SceneDev& S, const LightCamDev& C, float* splat
     const int ix = (int)rp.x, iy = (int)rp.y;
     if (!(ix >= 0 && ix <= C.width && iy >= 0 && iy <= C.height)) return;
     Hit hv;
-    if (traverse<true, STACK>(S, pos + toCam * kRayEps, toCam, 0.f, len, &hv, stack, stride, nullptr, nullptr)) return;
+    if (find_hit<true, STACK>(S, pos + toCam * kRayEps, toCam, 0.f, len, &hv, stack, stride)) return;
     V3 L;
     if (s0) {
         L = thr;
@@ -318,7 +318,7 @@ __device__ void lt_path(const SceneDev& S, const PathParams& P, const LightCamDe
     V3 d = dir;
     for (int bounce = 1;; ) {
         Hit h;
-        if (!traverse<false, STACK>(S, o, d, 0.f, kInf, &h, stack, stride, nullptr, nullptr)) break;
+        if (!find_hit<false, STACK>(S, o, d, 0.f, kInf, &h, stack, stride)) break;
         const Isect it = query(S, o, d, h);
         BSDF b = make_bsdf(S, it);
         b.importance = true;  // ComputeScatteringFunction(arena, TransportMode::Importance) (integrator/LT.h:69)
@@ -345,28 +345,41 @@ __device__ inline void film_footprint(float fx, float fy, int W, int H, int* x0,
     *y1 = maxY > H - 1 ? H - 1 : maxY;
 }
 
-// Record one finished sample: own-pixel samples go to the pass buffer slot
-// (owner adds them in film_accumulate); any sample whose box-filter footprint is
-// not exactly its own pixel goes to the side list, one entry per covered pixel.
-__device__ inline void record_sample(const FilmDev& F, int s_local, int px, int py, float fx, float fy, V3 L) {
+// Film::AddSample (visual/Film.h:61-87) with ownership instead of locks: the
+// sample is stored in its pass slot (pixel px, py of pass sample s_local) with
+// a 9-bit mask of the pixels its box footprint covers, as offsets dx, dy in
+// {-1, 0, 1} from its own pixel (bit (dy + 1) * 3 + dx + 1).  Offsets are taken
+// around the image's edges: a sample whose u wrapped from 1 to 0 in the last
+// column (sampler/Sampler.h:60-62) lands in column 0, offset +1.  A sample
+// that covers any pixel but its own also flags its pixel in F.border, which
+// tells the neighbours' owners to look at its slot (k_film_accumulate).  A
+// footprint beyond the neighbouring pixels is counted in F.border[W * H] and
+// fails the render.
+constexpr uint32_t kFilmOwn = 1u << 4;
+__device__ inline void film_record(const FilmDev& F, int64_t slot, int px, int py, float fx, float fy, V3 L) {
     const int W = F.width, H = F.height;
     int x0, x1, y0, y1;
     film_footprint(fx, fy, W, H, &x0, &x1, &y0, &y1);
-    const int64_t WH = (int64_t)W * H;
-    const int64_t slot = (int64_t)s_local * WH + (int64_t)py * W + px;
-    const bool own = (x0 == px && x1 == px && y0 == py && y1 == py);
-    F.pass[slot] = make_float4(L.x, L.y, L.z, own ? 1.f : 0.f);
-    if (own) return;
+    uint32_t mask = 0;
+    bool far = false;
     for (int yy = y0; yy <= y1; ++yy)
         for (int xx = x0; xx <= x1; ++xx) {
-            const uint32_t k = atomicAdd(F.side_count, 1u);
-            if (k < F.side_cap) {
-                SideEntry e;
-                e.key = ((uint64_t)((uint32_t)(yy * W + xx)) << 32) | (uint64_t)(uint32_t)slot;
-                e.L = make_float4(L.x, L.y, L.z, 0.f);
-                F.side[k] = e;
-            }
+            int dx = xx - px, dy = yy - py;
+            if (dx > 1) dx -= W;
+            if (dx < -1) dx += W;
+            if (dy > 1) dy -= H;
+            if (dy < -1) dy += H;
+            if (dx < -1 || dx > 1 || dy < -1 || dy > 1) far = true;
+            else mask |= 1u << ((dy + 1) * 3 + dx + 1);
         }
+    F.pass[slot] = make_float4(L.x, L.y, L.z, (float)mask);
+    if (mask != kFilmOwn) F.border[(int64_t)py * W + px] = 1u;
+    if (far) atomicAdd(F.border + (int64_t)W * H, 1u);
+}
+
+__device__ inline void record_sample(const FilmDev& F, int s_local, int px, int py, float fx, float fy, V3 L) {
+    const int64_t WH = (int64_t)F.width * F.height;
+    film_record(F, (int64_t)s_local * WH + (int64_t)py * F.width + px, px, py, fx, fy, L);
 }
 
 __device__ inline void pass_pixel(const FilmDev& F, int64_t t, int* s_local, int* px, int* py) {

@@ -43,20 +43,11 @@ struct Ctx {
     bool capturing = false;  // a hipGraph capture is recording the stream: no timer events
 };
 
-struct SideEntry {
-    uint64_t key;  // (target pixel << 32) | source pass slot
-    uint64_t pad;
-    float4 L;
-};
-
 struct FilmState {
     float4* film = nullptr;  // rgb sums + weight, one per pixel
-    float4* pass = nullptr;  // one per (pass sample, pixel): rgb + own flag
+    float4* pass = nullptr;  // one per (pass sample, pixel): rgb + footprint mask
     int64_t pass_slots = 0;
-    SideEntry* side = nullptr;
-    SideEntry* side_sorted = nullptr;
-    uint32_t* side_count = nullptr;  // [0] count, [1] overflow high-water mark
-    uint32_t side_cap = 0;
+    uint32_t* border = nullptr;  // per pixel: a sample of this pass reaches other pixels; [W*H] error count
     unsigned long long* counters = nullptr;
     int width = 0, height = 0;
 };
@@ -64,11 +55,7 @@ struct FilmState {
 struct FilmDev {
     float4* film;
     float4* pass;
-    SideEntry* side;
-    SideEntry* side_sorted;
-    uint32_t* side_count;
-    uint32_t* side_overflow;
-    uint32_t side_cap;
+    uint32_t* border;
     int width, height;
 };
 
@@ -124,6 +111,8 @@ FilmDev FilmView(Scene* sc);
 FilmDev FilmViewState(const FilmState& f);
 int FilmPrepareState(Ctx* c, FilmState& f, int W, int H, int64_t pass_slots);
 int FilmResolveState(Ctx* c, const FilmState& f, int n_s);
+int FilmClear(Ctx* c, const FilmState& f);  // zero film, border flags, counters
+int FilmCheck(const FilmState& f);           // footprint error of the last render
 
 // Per-render constants of the integrator kernels.
 struct PathParams {

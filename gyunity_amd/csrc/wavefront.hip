@@ -689,35 +689,12 @@ __device__ inline bool resolve_slot(const R& Rd, const WfDev& W, uint32_t slot) 
         }
     }
     W.pend3[slot].w = __uint_as_float(0u);  // resolved: later bounces skip the slot
-    // Film::AddSample (visual/Film.h:61-87) through the pass buffer / side list.
+    // Film::AddSample (visual/Film.h:61-87) through the pass buffer
     const FilmDev& F = Rd.film(Rd.entry(W, slot));
     const uint32_t ps = Rd.pass_slot(W, slot);
-    const int WW = F.width, H = F.height;
-    const float fx = W.T[slot].w, fy = Lv.w;
-    const int64_t WH = (int64_t)WW * H;
-    const int s_local = (int)(ps / WH);
-    const int64_t pix = ps - (int64_t)s_local * WH;
-    const int px = (int)(pix % WW), py = (int)(pix / WW);
-    const float x = fx - 0.5f, y = fy - 0.5f;
-    int x0 = (int)ceilf(x - 0.5f), x1 = (int)floorf(x + 0.5f);
-    int y0 = (int)ceilf(y - 0.5f), y1 = (int)floorf(y + 0.5f);
-    x0 = x0 < 0 ? 0 : x0;
-    x1 = x1 > WW - 1 ? WW - 1 : x1;
-    y0 = y0 < 0 ? 0 : y0;
-    y1 = y1 > H - 1 ? H - 1 : y1;
-    const bool own = (x0 == px && x1 == px && y0 == py && y1 == py);
-    F.pass[ps] = make_float4(Lv.x, Lv.y, Lv.z, own ? 1.f : 0.f);
-    if (own) return false;
-    for (int yy = y0; yy <= y1; ++yy)
-        for (int xx = x0; xx <= x1; ++xx) {
-            const uint32_t k = atomicAdd(F.side_count, 1u);
-            if (k < F.side_cap) {
-                SideEntry se;
-                se.key = ((uint64_t)((uint32_t)(yy * WW + xx)) << 32) | (uint64_t)ps;
-                se.L = make_float4(Lv.x, Lv.y, Lv.z, 0.f);
-                F.side[k] = se;
-            }
-        }
+    const int64_t WH = (int64_t)F.width * F.height;
+    const int64_t pix = ps % WH;
+    film_record(F, ps, (int)(pix % F.width), (int)(pix / F.width), W.T[slot].w, Lv.w, v3(Lv.x, Lv.y, Lv.z));
     return false;
 }
 
@@ -1151,7 +1128,7 @@ static int BatchSequence(Ctx* c, Batch* b) {
     int rc = GY_OK;
     for (auto& en : b->entries) {
         if (rc == GY_OK) rc = FillF4(c, en.film.film, (int64_t)en.film.width * en.film.height);
-        if (rc == GY_OK) rc = Fill32(c, en.film.side_count, 0u, 4);
+        if (rc == GY_OK) rc = Fill32(c, en.film.border, 0u, (int64_t)en.film.width * en.film.height + 1);
     }
     if (rc == GY_OK) rc = Fill32(c, b->counters, 0u, 32);
     if (rc != GY_OK) return rc;
