@@ -8,9 +8,44 @@
 // a gy::Error (hosts without a GPU) and exits 0.
 #include <gyunity/HipPathTracing.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+
+// The reference-side flattening of GpuPathTracing::Flatten,
+// run over the generated scene's arrays, which stand in for
+// Scene::GetPrimitives() and Scene::GetLights(): primitives in insertion
+// order with one gy::Material per distinct material (numbered in first-use
+// order), then the area lights in light order.
+static gy::Scene FlattenPrimitives(const gy_scene_desc& d) {
+    gy::Scene out;
+    std::map<uint32_t, int> mat_ids;
+    for (int i = 0; i < d.n_tri; ++i) {
+        const uint32_t m = d.mat[i];
+        auto it = mat_ids.find(m);
+        int mid;
+        if (it != mat_ids.end()) {
+            mid = it->second;
+        } else {
+            gy::Material gm;
+            std::copy(d.mats + (size_t)m * GY_MAT_WORDS, d.mats + (size_t)(m + 1) * GY_MAT_WORDS, gm.words.begin());
+            mid = mat_ids[m] = out.AddMaterial(gm);
+        }
+        const float* p = d.pos + 9 * (size_t)i;
+        const float* n = d.nrm + 9 * (size_t)i;
+        const float* f = d.face_n + 3 * (size_t)i;
+        std::array<float, 6> uv = {0, 0, 0, 0, 0, 0};
+        if (d.uv) std::copy(d.uv + 6 * (size_t)i, d.uv + 6 * (size_t)i + 6, uv.begin());
+        out.AddTriangle({gy::Vec3{p[0], p[1], p[2]}, gy::Vec3{p[3], p[4], p[5]}, gy::Vec3{p[6], p[7], p[8]}},
+                        {gy::Vec3{n[0], n[1], n[2]}, gy::Vec3{n[3], n[4], n[5]}, gy::Vec3{n[6], n[7], n[8]}},
+                        gy::Vec3{f[0], f[1], f[2]}, mid, uv);
+    }
+    for (int l = 0; l < d.n_light; ++l)
+        out.AddAreaLight(d.light_tri[l], gy::Vec3{d.light_le[3 * l], d.light_le[3 * l + 1], d.light_le[3 * l + 2]});
+    return out;
+}
 
 int main(int argc, char** argv) {
     const bool expect_no_device = argc > 1 && std::strcmp(argv[1], "--expect-no-device") == 0;
@@ -33,6 +68,7 @@ int main(int argc, char** argv) {
         gy_render_params rp;
         gy::Check(gy_generated_desc(g, &desc, &cd, &rp), "gy_generated_desc");
         const gy::Scene scene = gy::Scene::FromDesc(desc);
+        const gy::Scene flat = FlattenPrimitives(desc);
         gy_generated_free(g);
 
         const int W = 128, H = 128;
@@ -61,6 +97,17 @@ int main(int argc, char** argv) {
                 std::printf("film differs from the C-ABI render (mode %d)\n", mode);
                 ++bad;
             }
+        }
+        // the scene flattened primitive by primitive renders the same film
+        {
+            auto film2 = std::make_shared<gy::Film>(W, H);
+            gy::PinHoleCamera camera2(film2, {cd.pos[0], cd.pos[1], cd.pos[2]}, {cd.look_at[0], cd.look_at[1], cd.look_at[2]},
+                                      {cd.up[0], cd.up[1], cd.up[2]}, cd.fovy, cd.film_dist);
+            gy::HipPathTracing pt2(rp.spp, rp.max_depth, sampler, senum, params, dev);
+            pt2.Render(flat, camera2);
+            const bool same = film2->color == film->color && film2->weight == film->weight;
+            std::printf("primitive-by-primitive flattening: %s\n", same ? "same film" : "FILM DIFFERS");
+            if (!same) ++bad;
         }
         double sum = 0, wsum = 0;
         for (size_t i = 0; i < wgt.size(); ++i) {
