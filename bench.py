@@ -48,27 +48,62 @@ def kernel_bytes(name, st, paths):
     return float("nan")
 
 
-def cpu_baseline(sa, cam, prm, rows, env=None):
+def host_cpu():
+    """The host's CPU model, its logical CPU count and the CPUs this process may
+    run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = os.cpu_count() or 1
+    return model, os.cpu_count() or 1, allowed
+
+
+# Worker threads for the CPU baseline: the host CPU share a one-GPU job is
+# given (16 cores per GPU), or fewer where the process may use fewer.
+CPU_SHARE = 16
+
+
+def cpu_baseline(sa, cam, prm, rows, env=None, rows_1core=8):
     """The oracle's FP32 restatement of the reference CPU integrator
     (TiledIntegrator + PathTracing, reference BVH) on the host cores, over a
-    band of `rows` image rows of the same workload."""
+    band of `rows` image rows of the same workload; and the same on one core
+    over `rows_1core` rows."""
     from oracle import oracle as O
 
-    threads = min(16, os.cpu_count() or 1)
+    model, logical, allowed = host_cpu()
+    threads = max(1, min(CPU_SHARE, allowed))
     os_ = O.OracleScene(sa)
     if env is not None:
         os_.set_environment(env)
     row0 = cam.height // 2 - rows // 2
     p = O.params(prm.spp, prm.max_depth, prm.rr_depth, precision=0, nthreads=threads, use_bvh=1)
     _, _, paths, secs = os_.render(cam, p, row0, row0 + rows)
+    r1 = cam.height // 2 - rows_1core // 2
+    p1 = O.params(prm.spp, prm.max_depth, prm.rr_depth, precision=0, nthreads=1, use_bvh=1)
+    _, _, paths1, secs1 = os_.render(cam, p1, r1, r1 + rows_1core)
     os_.close()
     return {
         "value": round(paths / secs / 1e6, 4),
         "unit": "Mpaths/s",
         "cores": threads,
         "kind": "port",
-        "sample": "rows %d-%d of the 1024^2 image (%d paths, %.1f s), same scene, spp and depth"
-        % (row0, row0 + rows - 1, paths, secs),
+        "sample": "rows %d-%d of the %dx%d image (%d paths, %.1f s), same scene, spp and depth"
+        % (row0, row0 + rows - 1, cam.width, cam.height, paths, secs),
+        "one_core": {"value": round(paths1 / secs1 / 1e6, 4), "unit": "Mpaths/s",
+                     "sample": "rows %d-%d (%d paths, %.1f s)" % (r1, r1 + rows_1core - 1, paths1, secs1)},
+        "cpu_model": model,
+        "host_logical_cpus": logical,
+        "cpus_allowed": allowed,
+        "threads_note": "%d threads: the host CPU share of a one-GPU job (%d cores per GPU)" % (threads, CPU_SHARE),
     }
 
 
@@ -141,7 +176,8 @@ def lt_main(args):
         log("cpu baseline ...")
         # the oracle's light tracer over rows on the host's cores, as the
         # reference's ParallelFor runs it; a smaller image, same scene and depth
-        cores = min(16, os.cpu_count() or 1)
+        model, logical, allowed = host_cpu()
+        cores = max(1, min(CPU_SHARE, allowed))
         res, spp = 512, 16
         small = g.camera(tuple(cam.pos), tuple(cam.look_at), tuple(cam.up), cam.fovy, res, res, cam.film_dist)
         os_ = O.OracleScene(sa)
@@ -150,7 +186,8 @@ def lt_main(args):
         secs = time.perf_counter() - t0
         out["cpu_baseline"] = {"value": round(res * res * spp / secs / 1e6, 4), "unit": "Mpaths/s", "cores": cores, "kind": "port",
                                "sample": "%dx%d image, %d light paths per pixel (%d paths, %.1f s), same scene and depth"
-                               % (res, res, spp, res * res * spp, secs)}
+                               % (res, res, spp, res * res * spp, secs),
+                               "cpu_model": model, "host_logical_cpus": logical, "cpus_allowed": allowed}
     print(json.dumps(out), flush=True)
 
 
