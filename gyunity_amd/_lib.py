@@ -152,6 +152,7 @@ def lib():
         "gy_last_kernel_ms": (C.c_double, [vp]),
         "gy_count_visits": (C.c_int, [vp, vp, i64, P(i64), P(i64)]),
         "gy_render": (C.c_int, [vp, P(CameraDesc), P(RenderParams), P(f32), P(f32)]),
+        "gy_render_noalloc": (C.c_int, [vp, P(CameraDesc), P(RenderParams), vp]),
         "gy_film_download": (C.c_int, [vp, P(f32), P(f32)]),
         "gy_scene_texture_floats": (i64, [vp]),
         "gy_scene_context": (C.c_int, [vp, P(vp)]),

@@ -664,12 +664,12 @@ FilmDev FilmViewState(const FilmState& f) {
     return F;
 }
 FilmDev FilmView(Scene* sc) { return FilmViewState(sc->film); }
-int FilmResolveState(Ctx* c, const FilmState& f, int n_s) {
+int FilmResolveState(Ctx* c, const FilmState& f, int n_s, bool timed) {
     FilmDev F = FilmViewState(f);
     const int64_t WH = (int64_t)F.width * F.height;
-    TimerBegin(c, kKernFilm);
+    if (timed) TimerBegin(c, kKernFilm);
     hipLaunchKernelGGL(k_film_accumulate, dim3((unsigned)((WH + 255) / 256)), dim3(256), 0, c->stream, F, n_s);
-    TimerEnd(c);
+    if (timed) TimerEnd(c);
     int rc = CheckLaunch(c, "k_film_accumulate");
     if (rc != GY_OK) return rc;
     return Fill32(c, F.border, 0u, WH);  // border flags are per pass
@@ -735,12 +735,12 @@ int ValidateRender(const gy_camera_desc* cam, const gy_render_params* p) {
 }
 
 template <int STACK, int BLOCK>
-int LaunchMegaT(Ctx* c, Scene* sc, const PathParams& P, int s0, int n_s, bool count) {
-    FilmDev F = FilmView(sc);
+int LaunchMegaT(Ctx* c, Scene* sc, const FilmState& film, const PathParams& P, int s0, int n_s, bool count, bool timed) {
+    FilmDev F = FilmViewState(film);
     const int tilesX = (F.width + 7) >> 3, tilesY = (F.height + 7) >> 3;
     const int64_t threads = (int64_t)tilesX * tilesY * 64 * n_s;
     const unsigned blocks = (unsigned)((threads + BLOCK - 1) / BLOCK);
-    unsigned long long* ctr = sc->film.counters;
+    unsigned long long* ctr = film.counters;
     if (count) {
         if (sc->dev.env.w > 0)
             hipLaunchKernelGGL((k_pt_count<STACK, BLOCK, true>), dim3(blocks), dim3(BLOCK), 0, c->stream, sc->dev, P, F, s0, n_s, ctr);
@@ -748,23 +748,30 @@ int LaunchMegaT(Ctx* c, Scene* sc, const PathParams& P, int s0, int n_s, bool co
             hipLaunchKernelGGL((k_pt_count<STACK, BLOCK, false>), dim3(blocks), dim3(BLOCK), 0, c->stream, sc->dev, P, F, s0, n_s, ctr);
         return CheckLaunch(c, "k_pt_count");
     }
-    TimerBegin(c, kKernMega);
+    if (timed) TimerBegin(c, kKernMega);
     if (sc->dev.env.w > 0)
         hipLaunchKernelGGL((k_pt_megakernel<STACK, BLOCK, true>), dim3(blocks), dim3(BLOCK), 0, c->stream, sc->dev, P, F, s0, n_s, ctr);
     else
         hipLaunchKernelGGL((k_pt_megakernel<STACK, BLOCK, false>), dim3(blocks), dim3(BLOCK), 0, c->stream, sc->dev, P, F, s0, n_s, ctr);
-    TimerEnd(c);
+    if (timed) TimerEnd(c);
     return CheckLaunch(c, "k_pt_megakernel");
 }
 
-int LaunchMega(Ctx* c, Scene* sc, const PathParams& P, int s0, int n_s, int block, bool count) {
+// The megakernel over one pass into `film` (the scene's own film, or the
+// no-alloc entry point's).
+int LaunchMegaFilm(Ctx* c, Scene* sc, const FilmState& film, const PathParams& P, int s0, int n_s, int block, bool count,
+                   bool timed) {
     if (sc->stack <= 32) {
-        if (block == 64) return LaunchMegaT<32, 64>(c, sc, P, s0, n_s, count);
-        if (block == 256) return LaunchMegaT<32, 256>(c, sc, P, s0, n_s, count);
-        return LaunchMegaT<32, 128>(c, sc, P, s0, n_s, count);
+        if (block == 64) return LaunchMegaT<32, 64>(c, sc, film, P, s0, n_s, count, timed);
+        if (block == 256) return LaunchMegaT<32, 256>(c, sc, film, P, s0, n_s, count, timed);
+        return LaunchMegaT<32, 128>(c, sc, film, P, s0, n_s, count, timed);
     }
-    if (block == 64) return LaunchMegaT<64, 64>(c, sc, P, s0, n_s, count);
-    return LaunchMegaT<64, 128>(c, sc, P, s0, n_s, count);
+    if (block == 64) return LaunchMegaT<64, 64>(c, sc, film, P, s0, n_s, count, timed);
+    return LaunchMegaT<64, 128>(c, sc, film, P, s0, n_s, count, timed);
+}
+
+int LaunchMega(Ctx* c, Scene* sc, const PathParams& P, int s0, int n_s, int block, bool count) {
+    return LaunchMegaFilm(c, sc, sc->film, P, s0, n_s, block, count, true);
 }
 
 int SamplesPerPass(const gy_camera_desc& cam, const gy_render_params& p) {
@@ -778,6 +785,35 @@ int SamplesPerPass(const gy_camera_desc& cam, const gy_render_params& p) {
     if (s < 1) s = 1;
     if (s > p.spp) s = p.spp;
     return (int)s;
+}
+
+// ------------------------------------------------------------------ no-alloc scratch
+// Device memory for gy_render_noalloc: the ray counters, the per-pixel border
+// flags and the pass buffer of the megakernel render, in one file-scope array
+// that every GPU the library runs on holds until the process ends.  Host code
+// never passes the array's name to a kernel or a copy: it is a host-side
+// stand-in.  gy_open looks up this device's own address once per device
+// (hipGetSymbolAddress, with the device current) and keeps it per device;
+// kernels get that address.
+constexpr size_t kNoAllocScratchBytes = 64ull << 20;
+constexpr int kNoAllocMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+// This device's scratch address, looked up on the first call for the device
+// (the current device must be `device`).  nullptr if the lookup fails.
+static void* ScratchAddress(int device, hipError_t* err) {
+    *err = hipSuccess;
+    if (device < 0 || device >= kNoAllocMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        *err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (*err != hipSuccess) return nullptr;
+        g_scratch_ptr[device] = p;
+    }
+    return g_scratch_ptr[device];
 }
 
 }  // namespace gy
@@ -808,6 +844,16 @@ extern "C" int gy_open(int device, gy_context** out) {
     if (e != hipSuccess) {
         delete c;
         return HipFail(e, "hipStreamCreate");
+    }
+    {
+        // the no-alloc scratch array's address on this device (gy_render_noalloc)
+        hipError_t se = hipSuccess;
+        c->scratch = ScratchAddress(device, &se);
+        if (se != hipSuccess) {
+            (void)hipStreamDestroy(c->stream);
+            delete c;
+            return HipFail(se, "hipGetSymbolAddress(scratch)");
+        }
     }
     void* p = nullptr;
     int rc = Upload(c, &p, sobol::ByteTables(), (size_t)sobol::kDims * sobol::kBytes * 256 * 4);
@@ -1235,6 +1281,71 @@ extern "C" int gy_render(gy_scene* s, const gy_camera_desc* cam, const gy_render
         sc->film_valid = false;
     }
     if (rc == GY_OK && (color || weight)) rc = gy_film_download(s, color, weight);
+    return rc;
+}
+
+// gy_render without allocating: megakernel form, film into the caller's
+// device buffer, counters, border flags and pass buffer in this device's
+// scratch array.  The film equals gy_render's bit for bit: a pixel sums its
+// samples in sample order whatever the pass split.
+extern "C" int gy_render_noalloc(gy_scene* s, const gy_camera_desc* cam, const gy_render_params* p, float* film_device) {
+    if (!s || !film_device) return Fail(GY_ERR_INVALID, "gy_render_noalloc: null scene or film");
+    int rc = ValidateRender(cam, p);
+    if (rc != GY_OK) return rc;
+    if (p->mode != GY_RENDER_MEGAKERNEL)
+        return Fail(GY_ERR_UNSUPPORTED, "gy_render_noalloc: megakernel form only (the wavefront's path state is allocated)");
+    Scene* sc = reinterpret_cast<Scene*>(s);
+    Ctx* c = sc->ctx;
+    if (!c->scratch) return Fail(GY_ERR_DEVICE, "gy_render_noalloc: no scratch address for this device");
+    (void)hipSetDevice(c->device);
+    PathParams P;
+    rc = MakePathParams(c, sc, *cam, *p, &P);
+    if (rc != GY_OK) return rc;
+    // scratch layout: [0, 256) ray counters, then W*H + 1 border words (256-byte
+    // aligned), then pass slots of W*H float4 per pass sample
+    const int64_t WH = (int64_t)cam->width * cam->height;
+    // 20 bytes per pixel (one pass sample and its border word) and at most
+    // 515 of alignment and counters: max_pixels always fits
+    const size_t max_pixels = (kNoAllocScratchBytes - 1024) / 20;
+    const size_t pass_off = 256 + ((((size_t)WH + 1) * 4 + 255) & ~(size_t)255);
+    if ((size_t)WH > max_pixels || pass_off + (size_t)WH * 16 > kNoAllocScratchBytes) {
+        char msg[160];
+        std::snprintf(msg, sizeof(msg), "gy_render_noalloc: %lld pixels exceed the %zu MiB scratch (at most %zu pixels)",
+                      (long long)WH, kNoAllocScratchBytes >> 20, max_pixels);
+        return Fail(GY_ERR_UNSUPPORTED, msg);
+    }
+    const int64_t cap = (int64_t)((kNoAllocScratchBytes - pass_off) / ((size_t)WH * 16));
+    int n_pass;
+    if (p->samples_per_pass > 0) {
+        n_pass = std::min(p->samples_per_pass, p->spp);
+        if (n_pass > cap) {
+            char msg[160];
+            std::snprintf(msg, sizeof(msg), "gy_render_noalloc: samples_per_pass %d exceeds the scratch (at most %lld at %lld pixels)",
+                          n_pass, (long long)cap, (long long)WH);
+            return Fail(GY_ERR_UNSUPPORTED, msg);
+        }
+    } else {
+        n_pass = (int)std::min<int64_t>(SamplesPerPass(*cam, *p), cap);
+    }
+    char* base = (char*)c->scratch;
+    FilmState f;
+    f.film = (float4*)film_device;
+    f.counters = (unsigned long long*)base;
+    f.border = (uint32_t*)(base + 256);
+    f.pass = (float4*)(base + pass_off);
+    f.pass_slots = cap * WH;
+    f.width = cam->width;
+    f.height = cam->height;
+    rc = FilmClear(c, f);
+    const int block = p->block_size > 0 ? p->block_size : 128;
+    for (int s0 = 0; s0 < p->spp && rc == GY_OK; s0 += n_pass) {
+        const int n_s = std::min(n_pass, p->spp - s0);
+        rc = LaunchMegaFilm(c, sc, f, P, s0, n_s, block, false, false);
+        if (rc == GY_OK) rc = FilmResolveState(c, f, n_s, false);
+    }
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (rc == GY_OK && e != hipSuccess) rc = HipFail(e, "gy_render_noalloc sync");
+    if (rc == GY_OK) rc = FilmCheck(f);
     return rc;
 }
 

@@ -41,6 +41,7 @@ struct Ctx {
     std::vector<int> ev_cat;
     int ev_used = 0;
     bool capturing = false;  // a hipGraph capture is recording the stream: no timer events
+    void* scratch = nullptr;  // this device's copy of the no-alloc scratch array (gy_render_noalloc)
 };
 
 struct FilmState {
@@ -110,7 +111,7 @@ void FilmFree(FilmState* f);
 FilmDev FilmView(Scene* sc);
 FilmDev FilmViewState(const FilmState& f);
 int FilmPrepareState(Ctx* c, FilmState& f, int W, int H, int64_t pass_slots);
-int FilmResolveState(Ctx* c, const FilmState& f, int n_s);
+int FilmResolveState(Ctx* c, const FilmState& f, int n_s, bool timed = true);
 int FilmClear(Ctx* c, const FilmState& f);  // zero film, border flags, counters
 int FilmCheck(const FilmState& f);           // footprint error of the last render
 

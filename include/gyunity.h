@@ -177,6 +177,18 @@ int gy_count_visits(gy_scene* scene, const gy_ray* rays, int64_t n, int64_t* nod
  * be NULL to leave the film on the device; gy_film_download fetches it). */
 int gy_render(gy_scene* scene, const gy_camera_desc* cam, const gy_render_params* p, float* color, float* weight);
 int gy_film_download(gy_scene* scene, float* color, float* weight);
+/* gy_render that allocates no device memory.  No counterpart in the
+ * reference: it is for callers that cannot allocate (tight serving loops,
+ * harnesses that forbid hipMalloc).  Megakernel form only (p->mode
+ * GY_RENDER_MEGAKERNEL; the wavefront's path state is allocated).  film_device:
+ * width*height float4 on the scene's device (rgb sums, weight), written whole;
+ * its film equals gy_render's bit for bit.  Counters, border flags and pass
+ * buffer live in a 64 MiB array each device holds for the life of the process:
+ * at most 3,355,392 pixels (GY_ERR_UNSUPPORTED beyond, or when an explicit
+ * samples_per_pass does not fit; automatic passes take what fits).  Calls that
+ * overlap on one device share that array and race.  Synchronous; the scene's
+ * own film and gy_render_stats_get are left as they were. */
+int gy_render_noalloc(gy_scene* scene, const gy_camera_desc* cam, const gy_render_params* p, float* film_device);
 int gy_render_stats_get(gy_context* ctx, gy_render_stats* out);
 const char* gy_kernel_name(int index);
 /* Radiance of individual camera samples (x, y, s): PathTracing::Li for the path
