@@ -386,7 +386,7 @@ def adjoint_scale_main(args):
     import torch  # before the library: one HIP runtime for both
 
     import gyunity_amd as g
-    from gyunity_amd.torch_render import render_textures
+    from gyunity_amd.torch_render import render_views
 
     ctx = g.Context(0)
     log("device:", ctx.device_name())
@@ -396,19 +396,15 @@ def adjoint_scale_main(args):
     n_alb = int(sa.tex_info[1, 3])
     sc = g.Scene(ctx, sa)
     rng = np.random.default_rng(3)
-    views = []
-    centre = np.array([0.5, 0.4, 0.5])
-    while len(views) < args.views:
-        dvec = rng.normal(size=3)
-        dvec /= np.linalg.norm(dvec)
-        pos = centre + 0.45 * dvec
-        if (pos < 0.04).any() or (pos > 0.96).any():
-            continue
-        up = (0.0, 1.0, 0.0) if abs(dvec[1]) < 0.9 else (1.0, 0.0, 0.0)
-        views.append(g.camera(tuple(pos), tuple(centre), up, 60.0, args.view_res, args.view_res))
+    views = g.orbit_views(args.views, args.view_res)
+    # every view is an entry of one batch: one lane space forward (recorded
+    # graph), one replay launch backward
+    batch = g.Batch(ctx)
+    for c in views:
+        batch.add(sc, c, prm)
     tex_gt = torch.from_numpy(gt).cuda()
     with torch.no_grad():
-        targets = [render_textures(sc, c, prm, tex_gt).clone() for c in views]
+        targets = render_views(sc, batch, tex_gt).clone()
     init = np.concatenate([rng.uniform(0.2, 0.8, n_alb), rng.uniform(0.05, 0.8, gt.size - n_alb)]).astype(np.float32)
     tex = torch.from_numpy(init).cuda().requires_grad_(True)
     opt = torch.optim.Adam([tex], lr=1e-2)
@@ -419,12 +415,10 @@ def adjoint_scale_main(args):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         opt.zero_grad()
-        total = 0.0
-        for c, tgt in zip(views, targets):
-            img = render_textures(sc, c, prm, tex)
-            loss = ((img - tgt) ** 2).mean() / len(views)
-            loss.backward()
-            total += float(loss)
+        imgs = render_views(sc, batch, tex)
+        loss = ((imgs - targets) ** 2).mean()
+        loss.backward()
+        total = float(loss)
         opt.step()
         with torch.no_grad():
             tex.copy_(torch.minimum(torch.maximum(tex, lo), hi))
@@ -435,6 +429,7 @@ def adjoint_scale_main(args):
         log("adam step %3d: loss %.6e, mean |texel - ground truth| %.4f, %.2f s" % (step, total, err, times[-1]))
     s_per_grad = float(np.mean(times[1:])) if len(times) > 1 else times[0]
     paths = len(views) * args.view_res * args.view_res * prm.spp
+    batch.close()
     sc.close()
     ctx.close()
     out = {
@@ -453,7 +448,8 @@ def adjoint_scale_main(args):
         "config": {
             "workload": "adjoint at scale: 1.31M-triangle GGX icosphere, %d texture floats (%d^2 RGB albedo + roughness), "
                         "%d views %d^2, %d spp, depth %d" % (gt.size, args.tex_res, len(views), args.view_res, prm.spp, prm.max_depth),
-            "gradient": "per step: every view rendered and replayed (gy_autograd), Adam lr 1e-2",
+            "gradient": "per step: the views rendered as one batch and replayed in one launch (gy_autograd render_views), "
+                        "Adam lr 1e-2",
             "paths_per_gradient": paths,
             "loss_per_step": [float("%.6e" % v) for v in losses],
             "parallelism": "single GPU",

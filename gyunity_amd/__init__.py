@@ -20,6 +20,7 @@ from ._lib import (  # noqa: F401
     image_to_bytes,
     image_write,
     load_obj,
+    orbit_views,
     render_params,
     sobol_enum_index,
     sobol_sample,

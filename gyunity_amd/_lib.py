@@ -163,6 +163,12 @@ def lib():
         "gy_scene_set_textures_device": (C.c_int, [vp, vp, i64]),
         "gy_adjoint": (C.c_int, [vp, P(CameraDesc), P(RenderParams), P(f32), P(f32)]),
         "gy_adjoint_device": (C.c_int, [vp, P(CameraDesc), P(RenderParams), vp, vp]),
+        "gy_adjoint_emission": (C.c_int, [vp, P(CameraDesc), P(RenderParams), P(f32), P(f32), P(f32)]),
+        "gy_adjoint_emission_device": (C.c_int, [vp, P(CameraDesc), P(RenderParams), vp, vp, vp]),
+        "gy_batch_image_floats": (i64, [vp]),
+        "gy_batch_images_device": (C.c_int, [vp, vp]),
+        "gy_batch_adjoint": (C.c_int, [vp, P(f32), P(f32), P(f32)]),
+        "gy_batch_adjoint_device": (C.c_int, [vp, vp, vp, vp]),
         "gy_batch_create": (C.c_int, [vp, P(BatchParams), P(vp)]),
         "gy_batch_add": (C.c_int, [vp, vp, P(CameraDesc), P(RenderParams), P(i32)]),
         "gy_batch_render": (C.c_int, [vp]),
@@ -296,6 +302,23 @@ def camera(pos, look_at, up, fovy, width, height, film_dist=1.0):
     c.pos[:], c.look_at[:], c.up[:] = pos, look_at, up
     c.fovy, c.film_dist, c.width, c.height = fovy, film_dist, width, height
     return c
+
+
+def orbit_views(n, res, seed=3, centre=(0.5, 0.4, 0.5), radius=0.45, fovy=60.0):
+    """n seeded pinhole views of res x res looking at `centre` from points on a
+    sphere of `radius` about it that stay inside the unit room."""
+    rng = np.random.default_rng(seed)
+    c = np.asarray(centre, np.float64)
+    views = []
+    while len(views) < n:
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        pos = c + radius * d
+        if (pos < 0.04).any() or (pos > 0.96).any():
+            continue
+        up = (0.0, 1.0, 0.0) if abs(d[1]) < 0.9 else (1.0, 0.0, 0.0)
+        views.append(camera(tuple(pos), tuple(c), up, fovy, res, res))
+    return views
 
 
 def render_params(spp, max_depth, rr_depth=5, mode=GY_RENDER_MEGAKERNEL, samples_per_pass=0, block_size=0,
@@ -443,6 +466,18 @@ class Batch:
         check(lib().gy_batch_film_download(self.h, entry, _fp(color), _fp(weight)))
         return color, weight
 
+    def adjoint(self, dimages, n_tex, n_light=0):
+        """Path-replay adjoint of every entry (all views of one scene) in one
+        launch: dimages is the list of the entries' dl/dI (H, W, 3); returns the
+        texture gradient (n_tex floats) and, for n_light > 0, the emission
+        gradient (n_light, 3)."""
+        flat = np.ascontiguousarray(np.concatenate([np.asarray(d, np.float32).ravel() for d in dimages]))
+        assert flat.size == lib().gy_batch_image_floats(self.h)
+        grad = np.zeros(n_tex, dtype=np.float32)
+        gle = np.zeros((max(n_light, 0), 3), dtype=np.float32)
+        check(lib().gy_batch_adjoint(self.h, _fp(flat), _fp(grad), _fp(gle) if n_light > 0 else C.POINTER(C.c_float)()))
+        return (grad, gle) if n_light > 0 else grad
+
     def close(self):
         if self.h:
             lib().gy_batch_destroy(self.h)
@@ -465,6 +500,7 @@ class Scene:
         bp = BVHParams()
         bp.max_leaf_size, bp.builder = max_leaf_size, builder
         self.h = C.c_void_p()
+        self.n_light = int(arrays.light_tri.shape[0])  # area lights
         check(lib().gy_scene_create(ctx.h, C.byref(self._desc), C.byref(bp), C.byref(self.h)))
 
     def close(self):
@@ -550,14 +586,19 @@ class Scene:
         data = np.ascontiguousarray(data, dtype=np.float32)
         check(lib().gy_scene_set_textures(self.h, _fp(data), data.size))
 
-    def adjoint(self, cam, params, dimage):
+    def adjoint(self, cam, params, dimage, emission=False):
         """Gradient of a loss w.r.t. the texture floats, given dl/dI (H, W, 3)
-        for the film of the last render of this camera and these parameters."""
+        for the film of the last render of this camera and these parameters.
+        emission=True: also w.r.t. each area light's Le, (n_light, 3)."""
         dimage = np.ascontiguousarray(dimage, dtype=np.float32)
         assert dimage.shape == (cam.height, cam.width, 3)
         grad = np.zeros(self.texture_floats(), dtype=np.float32)
-        check(lib().gy_adjoint(self.h, C.byref(cam), C.byref(params), _fp(dimage), _fp(grad)))
-        return grad
+        if not emission:
+            check(lib().gy_adjoint(self.h, C.byref(cam), C.byref(params), _fp(dimage), _fp(grad)))
+            return grad
+        gle = np.zeros((self.n_light, 3), dtype=np.float32)
+        check(lib().gy_adjoint_emission(self.h, C.byref(cam), C.byref(params), _fp(dimage), _fp(grad), _fp(gle)))
+        return grad, gle
 
     def film(self, W, H):
         color = np.zeros((H, W, 3), dtype=np.float32)

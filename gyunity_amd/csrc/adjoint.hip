@@ -100,7 +100,8 @@ __host__ __device__ inline float dalpha_droughness(float r) {
 }
 
 struct Grad {
-    float* __restrict__ g;  // one float per texture float
+    float* __restrict__ g;    // one float per texture float
+    float* __restrict__ gle;  // n_light * 3 floats (emitted radiance per light), or null
 };
 
 // One camera sample replayed: the same random numbers, traversals and
@@ -134,8 +135,16 @@ __device__ void replay_path(const SceneDev& S, const PathParams& P, int px, int 
         bool has_di = false;
         if ((i == 0 || delta) && it.light >= 0) {
             const V3 Le = v3(S.light_le[3 * it.light], S.light_le[3 * it.light + 1], S.light_le[3 * it.light + 2]);
-            const V3 e = dot(it.n, it.outDir) > 0.f ? Le : v3(0.f, 0.f, 0.f);
+            const bool facing = dot(it.n, it.outDir) > 0.f;
+            const V3 e = facing ? Le : v3(0.f, 0.f, 0.f);
             L = L + T * e;
+            if (G.gle && facing) {
+                // d(T Le) / d Le = T
+                const V3 ge = dL * T;
+                atomicAdd(&G.gle[3 * it.light], ge.x);
+                atomicAdd(&G.gle[3 * it.light + 1], ge.y);
+                atomicAdd(&G.gle[3 * it.light + 2], ge.z);
+            }
         } else {
             const float uL = seq_next(M, q);
             const float u0 = seq_next(M, q), u1 = seq_next(M, q);
@@ -145,6 +154,14 @@ __device__ void replay_path(const SceneDev& S, const PathParams& P, int px, int 
             Tdi = T * di;
             L = L + Tdi;
             has_di = true;
+            if (G.gle && !(is_zero(dp.F1) && is_zero(dp.F2))) {
+                // the light sample and the MIS sample are linear in the chosen
+                // light's Le; P(light) is a sampling probability, held fixed
+                const V3 ge = dL * T * (dp.F1 * dp.w1 + dp.F2 * dp.w2) / dp.lsp;
+                atomicAdd(&G.gle[3 * dp.light], ge.x);
+                atomicAdd(&G.gle[3 * dp.light + 1], ge.y);
+                atomicAdd(&G.gle[3 * dp.light + 2], ge.z);
+            }
         }
         const float r0 = seq_next(M, q), r1 = seq_next(M, q), r2 = seq_next(M, q);
         V3 wi;
@@ -211,34 +228,29 @@ __device__ void replay_path(const SceneDev& S, const PathParams& P, int px, int 
     }
 }
 
-// One thread per camera sample of the render on the film (same mapping as the
-// megakernel): the path's radiance, its adjoint radiance from dl/dI over its
-// box-filter footprint, then the replay.  STORED: the render left every
-// sample's radiance in its pass slot (one pass covered all samples), so the
-// first trace is skipped.
+// One camera sample (pass sample s of pixel px, py) of a render on film F:
+// its radiance, its adjoint radiance from dl/dI over its box-filter
+// footprint, then the replay.  STORED: the render left every sample's
+// radiance in its pass slot (one pass covered all samples), so the first
+// trace is skipped.
 template <int STACK, int BLOCK, bool STORED>
-__global__ void __launch_bounds__(BLOCK) k_pt_adjoint(SceneDev S, PathParams P, FilmDev F, const float* __restrict__ dI,
-                                                      Grad G, int s0, int n_s) {
-    __shared__ uint32_t stack[STACK * BLOCK];
-    const int64_t t = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    int s_local, px, py;
-    pass_pixel(F, t, &s_local, &px, &py);
-    if (!(s_local < n_s && px < F.width && py < F.height)) return;
+__device__ inline void adjoint_sample(const SceneDev& S, const PathParams& P, const FilmDev& F, const float* __restrict__ dI,
+                                      const Grad& G, int s, int px, int py, uint32_t* stack) {
     float fx, fy;
     V3 Lt;
     if (STORED) {
         Seq q;
-        q.index = enum_index(P.en, (uint32_t)(s0 + s_local), (uint32_t)px, (uint32_t)py);
+        q.index = enum_index(P.en, (uint32_t)s, (uint32_t)px, (uint32_t)py);
         q.cursor = 0;
         const float u = P.en.sx * seq_next(P.smp, q) - (float)px;
         const float v = P.en.sy * seq_next(P.smp, q) - (float)py;
         fx = px + u;
         fy = py + v;
-        const float4 e = F.pass[((int64_t)(s0 + s_local) * F.height + py) * F.width + px];
+        const float4 e = F.pass[((int64_t)s * F.height + py) * F.width + px];
         Lt = v3(e.x, e.y, e.z);
     } else {
         PathCounters pc = {};
-        Lt = trace_path<STACK, false>(S, P, px, py, s0 + s_local, stack + threadIdx.x, BLOCK, &fx, &fy, nullptr, pc);
+        Lt = trace_path<STACK, false>(S, P, px, py, s, stack, BLOCK, &fx, &fy, nullptr, pc);
     }
     int x0, x1, y0, y1;
     film_footprint(fx, fy, F.width, F.height, &x0, &x1, &y0, &y1);
@@ -250,7 +262,52 @@ __global__ void __launch_bounds__(BLOCK) k_pt_adjoint(SceneDev S, PathParams P, 
             if (w > 0.f) dL = dL + v3(dI[3 * q], dI[3 * q + 1], dI[3 * q + 2]) / w;
         }
     if (is_zero(dL)) return;
-    replay_path<STACK>(S, P, px, py, s0 + s_local, stack + threadIdx.x, BLOCK, Lt, dL, G);
+    replay_path<STACK>(S, P, px, py, s, stack, BLOCK, Lt, dL, G);
+}
+
+// One thread per camera sample of the render on the film (same mapping as the
+// megakernel).
+template <int STACK, int BLOCK, bool STORED>
+__global__ void __launch_bounds__(BLOCK) k_pt_adjoint(SceneDev S, PathParams P, FilmDev F, const float* __restrict__ dI,
+                                                      Grad G, int s0, int n_s) {
+    __shared__ uint32_t stack[STACK * BLOCK];
+    const int64_t t = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    int s_local, px, py;
+    pass_pixel(F, t, &s_local, &px, &py);
+    if (!(s_local < n_s && px < F.width && py < F.height)) return;
+    adjoint_sample<STACK, BLOCK, STORED>(S, P, F, dI, G, s0 + s_local, px, py, stack + threadIdx.x);
+}
+
+// Every sample of every entry of a batch render in one launch, in the batch's
+// lane order (entry, then the entry's sample-major 8x8-tile order, as
+// k_wf_raygen_batch deals them).  A batch entry's pass buffer holds all its
+// samples, so the radiance is always stored.  Entry e's dl/dI starts at
+// dI + E[e].image_off.
+template <int STACK, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_batch_adjoint(const BatchEntryDev* __restrict__ E, int32_t n, const float* __restrict__ dI,
+                                                         Grad G, int64_t base, int64_t count) {
+    __shared__ uint32_t stack[STACK * BLOCK];
+    const int64_t t = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (t >= count) return;
+    const int64_t g = base + t;
+    int lo = 0, hi = n - 1;  // last entry with first <= g
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (E[mid].first <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    const BatchEntryDev& B = E[lo];
+    const int64_t r = g - B.first;
+    if (r >= B.paths) return;
+    const int64_t per_s = (int64_t)B.tilesX * B.tilesY * 64;
+    const int s = (int)(r / per_s);
+    const int64_t rr = r - (int64_t)s * per_s;
+    const int64_t tile = rr >> 6;
+    const int local = (int)(rr & 63);
+    const int px = (int)(tile % B.tilesX) * 8 + (local & 7);
+    const int py = (int)(tile / B.tilesX) * 8 + (local >> 3);
+    if (px >= B.P.cam.width || py >= B.P.cam.height) return;
+    adjoint_sample<STACK, BLOCK, true>(B.S, B.P, B.F, dI + B.image_off, G, s, px, py, stack + threadIdx.x);
 }
 
 // Pixel values colour / weight of the film (Film::WriteToPixelBuffer without
@@ -264,6 +321,18 @@ __global__ void k_film_image(FilmDev F, float* __restrict__ out) {
     out[3 * p + 2] = a.z / a.w;
 }
 
+// Every entry's image colour / weight, concatenated (entry e at image_off).
+__global__ void k_batch_images(const BatchEntryDev* __restrict__ E, int32_t e, float* __restrict__ out) {
+    const BatchEntryDev& B = E[e];
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (int64_t)B.F.width * B.F.height) return;
+    const float4 a = B.F.film[p];
+    float* o = out + B.image_off;
+    o[3 * p] = a.x / a.w;
+    o[3 * p + 1] = a.y / a.w;
+    o[3 * p + 2] = a.z / a.w;
+}
+
 __global__ void k_zero_f32(float* p, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0.f;
@@ -275,8 +344,9 @@ int MakePathParams(Ctx* c, Scene* sc, const gy_camera_desc& cam, const gy_render
 int ValidateRender(const gy_camera_desc* cam, const gy_render_params* p);
 int SamplesPerPass(const gy_camera_desc& cam, const gy_render_params& p);
 
-// Adds the gradient of the loss to grad (device, one float per texture float).
-static int AdjointDevice(Scene* sc, const gy_camera_desc* cam, const gy_render_params* p, const float* dI, float* grad) {
+// Adds the gradient of the loss to grad (device, one float per texture float)
+// and, when gle is not null, to gle (device, three floats per area light).
+static int AdjointDevice(Scene* sc, const gy_camera_desc* cam, const gy_render_params* p, const float* dI, float* grad, float* gle) {
     Ctx* c = sc->ctx;
     int rc = ValidateRender(cam, p);
     if (rc != GY_OK) return rc;
@@ -302,7 +372,7 @@ static int AdjointDevice(Scene* sc, const gy_camera_desc* cam, const gy_render_p
         const int n_s = std::min(n_pass, p->spp - s0);
         const int64_t threads = (int64_t)tilesX * tilesY * 64 * n_s;
         const unsigned blocks = (unsigned)((threads + kBlock - 1) / kBlock);
-        const Grad G{grad};
+        const Grad G{grad, gle};
         if (sc->stack <= 32) {
             if (stored) hipLaunchKernelGGL((k_pt_adjoint<32, kBlock, true>), dim3(blocks), dim3(kBlock), 0, c->stream, sc->dev, P, F, dI, G, s0, n_s);
             else hipLaunchKernelGGL((k_pt_adjoint<32, kBlock, false>), dim3(blocks), dim3(kBlock), 0, c->stream, sc->dev, P, F, dI, G, s0, n_s);
@@ -316,33 +386,86 @@ static int AdjointDevice(Scene* sc, const gy_camera_desc* cam, const gy_render_p
     return rc;
 }
 
+int BatchAdjoint(Ctx* c, Batch* b, const float* dI, float* grad, float* gle) {
+    if (b->entries.empty() || b->dirty || !b->rendered) return Fail(GY_ERR_INVALID, "gy_batch_adjoint: render the batch first");
+    Scene* sc = b->entries[0].scene;
+    for (const BatchEntry& en : b->entries) {
+        if (en.scene != sc) return Fail(GY_ERR_UNSUPPORTED, "gy_batch_adjoint: every entry must render the same scene");
+        if (en.tex_revision != sc->tex_revision)
+            return Fail(GY_ERR_INVALID, "gy_batch_adjoint: the textures changed since the batch render");
+    }
+    if (sc->tex_floats == 0) return Fail(GY_ERR_INVALID, "gy_batch_adjoint: the scene has no textures");
+    if (sc->dev.env.w > 0) return Fail(GY_ERR_UNSUPPORTED, "gy_batch_adjoint: scenes with an environment light");
+    if (sc->stats.max_depth + 1 > 64) return Fail(GY_ERR_UNSUPPORTED, "gy_batch_adjoint: BVH deeper than the traversal stack");
+    constexpr int kBlock = 128;
+    const Grad G{grad, gle};
+    // launches of at most 2^30 samples
+    constexpr int64_t kChunk = (int64_t)1 << 30;
+    int rc = GY_OK;
+    TimerBegin(c, kKernOther);
+    for (int64_t base = 0; base < b->total_paths && rc == GY_OK; base += kChunk) {
+        const int64_t count = std::min(kChunk, b->total_paths - base);
+        const unsigned blocks = (unsigned)((count + kBlock - 1) / kBlock);
+        if (sc->stack <= 32)
+            hipLaunchKernelGGL((k_batch_adjoint<32, kBlock>), dim3(blocks), dim3(kBlock), 0, c->stream, (const BatchEntryDev*)b->table,
+                               (int32_t)b->entries.size(), dI, G, base, count);
+        else
+            hipLaunchKernelGGL((k_batch_adjoint<64, kBlock>), dim3(blocks), dim3(kBlock), 0, c->stream, (const BatchEntryDev*)b->table,
+                               (int32_t)b->entries.size(), dI, G, base, count);
+        rc = CheckLaunch(c, "k_batch_adjoint");
+    }
+    TimerEnd(c);
+    return rc;
+}
+
+int BatchImages(Ctx* c, Batch* b, float* images) {
+    if (b->entries.empty() || b->dirty || !b->rendered) return Fail(GY_ERR_INVALID, "gy_batch_images: render the batch first");
+    for (size_t e = 0; e < b->entries.size(); ++e) {
+        const int64_t n = (int64_t)b->entries[e].cam.width * b->entries[e].cam.height;
+        hipLaunchKernelGGL(k_batch_images, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, (const BatchEntryDev*)b->table,
+                           (int32_t)e, images);
+    }
+    int rc = CheckLaunch(c, "k_batch_images");
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (rc == GY_OK && e != hipSuccess) rc = HipFail(e, "batch images sync");
+    return rc;
+}
+
 }  // namespace gy
 
 using namespace gy;
 
-extern "C" int gy_adjoint_device(gy_scene* s, const gy_camera_desc* cam, const gy_render_params* p, const float* dimage_device,
-                                 float* grad_device) {
+extern "C" int gy_adjoint_emission_device(gy_scene* s, const gy_camera_desc* cam, const gy_render_params* p,
+                                          const float* dimage_device, float* grad_device, float* grad_le_device) {
     if (!s || !dimage_device || !grad_device) return Fail(GY_ERR_INVALID, "gy_adjoint_device: null argument");
     Scene* sc = reinterpret_cast<Scene*>(s);
     Ctx* c = sc->ctx;
     (void)hipSetDevice(c->device);
     c->ev_used = 0;
-    int rc = AdjointDevice(sc, cam, p, dimage_device, grad_device);
+    int rc = AdjointDevice(sc, cam, p, dimage_device, grad_device, grad_le_device);
     hipError_t e = hipStreamSynchronize(c->stream);
     if (rc == GY_OK && e != hipSuccess) rc = HipFail(e, "adjoint sync");
     TimersCollect(c);
     return rc;
 }
 
-extern "C" int gy_adjoint(gy_scene* s, const gy_camera_desc* cam, const gy_render_params* p, const float* dimage, float* grad) {
+extern "C" int gy_adjoint_device(gy_scene* s, const gy_camera_desc* cam, const gy_render_params* p, const float* dimage_device,
+                                 float* grad_device) {
+    return gy_adjoint_emission_device(s, cam, p, dimage_device, grad_device, nullptr);
+}
+
+extern "C" int gy_adjoint_emission(gy_scene* s, const gy_camera_desc* cam, const gy_render_params* p, const float* dimage,
+                                   float* grad, float* grad_le) {
     if (!s || !cam || !dimage || !grad) return Fail(GY_ERR_INVALID, "gy_adjoint: null argument");
     Scene* sc = reinterpret_cast<Scene*>(s);
     Ctx* c = sc->ctx;
     (void)hipSetDevice(c->device);
     const int64_t npix = (int64_t)cam->width * cam->height;
-    void *dI = nullptr, *dg = nullptr;
+    const int64_t n_le = (int64_t)sc->area_tri.size() * 3;
+    void *dI = nullptr, *dg = nullptr, *dl = nullptr;
     int rc = DevAlloc(c, &dI, (size_t)npix * 3 * 4);
     if (rc == GY_OK) rc = DevAlloc(c, &dg, (size_t)std::max<int64_t>(sc->tex_floats, 1) * 4);
+    if (rc == GY_OK && grad_le) rc = DevAlloc(c, &dl, (size_t)std::max<int64_t>(n_le, 1) * 4);
     if (rc == GY_OK) {
         hipError_t e = hipMemcpyAsync(dI, dimage, (size_t)npix * 3 * 4, hipMemcpyHostToDevice, c->stream);
         if (e != hipSuccess) rc = HipFail(e, "adjoint H2D");
@@ -352,10 +475,12 @@ extern "C" int gy_adjoint(gy_scene* s, const gy_camera_desc* cam, const gy_rende
                            sc->tex_floats);
         rc = CheckLaunch(c, "zero gradient");
     }
+    if (rc == GY_OK && dl) rc = Fill32(c, dl, 0u, std::max<int64_t>(n_le, 1));
     c->ev_used = 0;
-    if (rc == GY_OK) rc = AdjointDevice(sc, cam, p, (const float*)dI, (float*)dg);
+    if (rc == GY_OK) rc = AdjointDevice(sc, cam, p, (const float*)dI, (float*)dg, (float*)dl);
     if (rc == GY_OK) {
         hipError_t e = hipMemcpyAsync(grad, dg, (size_t)sc->tex_floats * 4, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess && dl) e = hipMemcpyAsync(grad_le, dl, (size_t)n_le * 4, hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) rc = HipFail(e, "adjoint D2H");
     } else {
@@ -364,7 +489,12 @@ extern "C" int gy_adjoint(gy_scene* s, const gy_camera_desc* cam, const gy_rende
     TimersCollect(c);
     DevFree(dI);
     DevFree(dg);
+    DevFree(dl);
     return rc;
+}
+
+extern "C" int gy_adjoint(gy_scene* s, const gy_camera_desc* cam, const gy_render_params* p, const float* dimage, float* grad) {
+    return gy_adjoint_emission(s, cam, p, dimage, grad, nullptr);
 }
 
 // The roughness terms of the adjoint on the host, for tests: d log f / d alpha
@@ -403,6 +533,7 @@ static int SetTextures(Scene* sc, const float* data, int64_t n, hipMemcpyKind ki
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return HipFail(e, "gy_scene_set_textures");
     sc->film_valid = false;
+    ++sc->tex_revision;
     return GY_OK;
 }
 extern "C" int gy_scene_set_textures(gy_scene* s, const float* data, int64_t n) {

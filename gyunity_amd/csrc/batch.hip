@@ -30,8 +30,9 @@ int BatchBuild(Ctx* c, Batch* b, int64_t lanes_req) {
     if (b->entries.empty()) return Fail(GY_ERR_INVALID, "gy_batch_render: empty batch");
     BatchDropGraph(b);
     std::vector<BatchEntryDev> tab(b->entries.size());
-    int64_t first = 0;
+    int64_t first = 0, image_off = 0;
     int depth_max = 0;
+    b->rendered = false;
     for (size_t i = 0; i < b->entries.size(); ++i) {
         BatchEntry& en = b->entries[i];
         en.scene_revision = en.scene->revision;
@@ -49,9 +50,12 @@ int BatchBuild(Ctx* c, Batch* b, int64_t lanes_req) {
         d.tilesY = (H + 7) >> 3;
         d.paths = (int64_t)d.tilesX * d.tilesY * 64 * en.params.spp;
         d.first = first;
+        d.image_off = image_off;
         en.first = first;
         en.paths = d.paths;
+        en.image_off = image_off;
         first += d.paths;
+        image_off += (int64_t)W * H * 3;
         depth_max = std::max(depth_max, en.params.max_depth);
     }
     DevFree(b->table);
@@ -63,6 +67,7 @@ int BatchBuild(Ctx* c, Batch* b, int64_t lanes_req) {
         if (rc != GY_OK) return rc;
     }
     b->total_paths = first;
+    b->image_floats = image_off;
     b->depth_max = depth_max;
     const int64_t want = lanes_req > 0 ? lanes_req : (int64_t)32 << 20;
     b->lanes = std::min(want, (first + 255) / 256 * 256);
@@ -157,7 +162,70 @@ extern "C" int gy_batch_render(gy_batch* bh) {
             rc = FilmCheck(en.film);
             if (rc != GY_OK) return rc;
         }
+        for (BatchEntry& en : b.entries) en.tex_revision = en.scene->tex_revision;
+        b.rendered = true;
     }
+    return rc;
+}
+
+extern "C" int64_t gy_batch_image_floats(gy_batch* bh) {
+    if (!bh) return -1;
+    const Batch& b = reinterpret_cast<BatchHandle*>(bh)->b;
+    return b.dirty ? -1 : b.image_floats;
+}
+
+extern "C" int gy_batch_images_device(gy_batch* bh, float* images_device) {
+    if (!bh || !images_device) return Fail(GY_ERR_INVALID, "gy_batch_images_device: null argument");
+    Batch& b = reinterpret_cast<BatchHandle*>(bh)->b;
+    (void)hipSetDevice(b.ctx->device);
+    return BatchImages(b.ctx, &b, images_device);
+}
+
+extern "C" int gy_batch_adjoint_device(gy_batch* bh, const float* dimages_device, float* grad_tex_device, float* grad_le_device) {
+    if (!bh || !dimages_device || !grad_tex_device) return Fail(GY_ERR_INVALID, "gy_batch_adjoint_device: null argument");
+    Batch& b = reinterpret_cast<BatchHandle*>(bh)->b;
+    Ctx* c = b.ctx;
+    (void)hipSetDevice(c->device);
+    c->ev_used = 0;
+    int rc = BatchAdjoint(c, &b, dimages_device, grad_tex_device, grad_le_device);
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (rc == GY_OK && e != hipSuccess) rc = HipFail(e, "batch adjoint sync");
+    TimersCollect(c);
+    return rc;
+}
+
+extern "C" int gy_batch_adjoint(gy_batch* bh, const float* dimages, float* grad_tex, float* grad_le) {
+    if (!bh || !dimages || !grad_tex) return Fail(GY_ERR_INVALID, "gy_batch_adjoint: null argument");
+    Batch& b = reinterpret_cast<BatchHandle*>(bh)->b;
+    Ctx* c = b.ctx;
+    if (b.entries.empty() || b.dirty || !b.rendered) return Fail(GY_ERR_INVALID, "gy_batch_adjoint: render the batch first");
+    (void)hipSetDevice(c->device);
+    Scene* sc = b.entries[0].scene;
+    const int64_t n_le = (int64_t)sc->area_tri.size() * 3;
+    void *dI = nullptr, *dg = nullptr, *dl = nullptr;
+    int rc = DevAlloc(c, &dI, (size_t)b.image_floats * 4);
+    if (rc == GY_OK) rc = DevAlloc(c, &dg, (size_t)std::max<int64_t>(sc->tex_floats, 1) * 4);
+    if (rc == GY_OK && grad_le) rc = DevAlloc(c, &dl, (size_t)std::max<int64_t>(n_le, 1) * 4);
+    if (rc == GY_OK) {
+        hipError_t e = hipMemcpyAsync(dI, dimages, (size_t)b.image_floats * 4, hipMemcpyHostToDevice, c->stream);
+        if (e != hipSuccess) rc = HipFail(e, "batch adjoint H2D");
+    }
+    if (rc == GY_OK) rc = Fill32(c, dg, 0u, std::max<int64_t>(sc->tex_floats, 1));
+    if (rc == GY_OK && dl) rc = Fill32(c, dl, 0u, std::max<int64_t>(n_le, 1));
+    c->ev_used = 0;
+    if (rc == GY_OK) rc = BatchAdjoint(c, &b, (const float*)dI, (float*)dg, (float*)dl);
+    if (rc == GY_OK) {
+        hipError_t e = hipMemcpyAsync(grad_tex, dg, (size_t)sc->tex_floats * 4, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess && dl) e = hipMemcpyAsync(grad_le, dl, (size_t)n_le * 4, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = HipFail(e, "batch adjoint D2H");
+    } else {
+        (void)hipStreamSynchronize(c->stream);
+    }
+    TimersCollect(c);
+    DevFree(dI);
+    DevFree(dg);
+    DevFree(dl);
     return rc;
 }
 

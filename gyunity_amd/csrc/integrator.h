@@ -24,12 +24,16 @@ struct PathCounters {
 struct DIParts {
     V3 L1, L2, wl, wb;
     float w1, w2, lsp;
+    int light;   // the light chosen (SampleOneLight)
+    V3 F1, F2;   // L1 and L2 per unit emitted radiance (the adjoint's emission terms)
 };
 template <int STACK, bool COUNT, bool ENV = false>
 __device__ DIParts di_parts(const SceneDev& S, const Isect& it, const BSDF& b, float uL, float u0, float u1, float v0,
                             float v1, float v2, uint32_t* stack, int stride, PathCounters& pc) {
     DIParts r;
     const int light = sample_one_light(S, uL, &r.lsp);
+    r.light = light;
+    r.F1 = r.F2 = v3s(0.f);
     const int ltri = S.light_tri[light];  // < 0: the environment light
     const V3 Le = v3(S.light_le[3 * light], S.light_le[3 * light + 1], S.light_le[3 * light + 2]);
     r.L1 = v3s(0.f);
@@ -61,9 +65,11 @@ __device__ DIParts di_parts(const SceneDev& S, const Isect& it, const BSDF& b, f
                 uint32_t* nn = COUNT ? &pc.nodes[1] : nullptr;
                 uint32_t* nt = COUNT ? &pc.tris[1] : nullptr;
                 pc.shadow++;
-                if (find_hit<true, STACK, COUNT>(S, so, sd, 0.f, stmax, &hs, stack, stride, nn, nt)) Li = v3s(0.f);
+                const bool occluded = find_hit<true, STACK, COUNT>(S, so, sd, 0.f, stmax, &hs, stack, stride, nn, nt);
+                if (occluded) Li = v3s(0.f);
                 r.w1 = power_heuristic(pdf, sp);
                 r.L1 = Li * v3s(1.f) * f * c / pdf;
+                if (!occluded && !(ENV && ltri < 0)) r.F1 = v3s(1.f) * f * c / pdf;
             }
         }
     }
@@ -89,8 +95,10 @@ __device__ DIParts di_parts(const SceneDev& S, const Isect& it, const BSDF& b, f
                     const int hl = __float_as_int(S.shade[(size_t)h2.prim * 8 + 1].w);
                     if (hl == light) {
                         const Isect li = query(S, ro, wi, h2);
-                        const V3 Lem = dot(li.n, neg(wi)) > 0.f ? Le : v3(0.f, 0.f, 0.f);
+                        const bool facing = dot(li.n, neg(wi)) > 0.f;
+                        const V3 Lem = facing ? Le : v3(0.f, 0.f, 0.f);
                         r.L2 = Lem * f * v3s(1.f) * c / pdf;
+                        if (facing) r.F2 = f * v3s(1.f) * c / pdf;
                     }
                 } else if (ENV && ltri < 0) {  // integrator/Integrator.cpp:97-99
                     r.L2 = env_emission_call(S.env, wi) * f * v3s(1.f) * c / pdf;

@@ -89,6 +89,7 @@ struct Scene {
     FilmState film;
     WavefrontState wf;
     int64_t tex_floats = 0;     // floats of texture data on the device
+    uint64_t tex_revision = 0;  // bumped whenever the texels are replaced
     bool film_valid = false;    // the film holds the last gy_render of film_cam / film_params
     bool film_single_pass = false;  // ... rendered in one pass: the pass buffer holds every sample
     gy_camera_desc film_cam = {};
@@ -133,6 +134,7 @@ struct BatchEntryDev {
     FilmDev F;
     int64_t first;  // first global path index of the entry
     int64_t paths;  // tile-padded paths: tilesX * tilesY * 64 * spp
+    int64_t image_off;  // the entry's first float in the batch's concatenated images (W * H * 3 per entry)
     int32_t tilesX, tilesY;
 };
 struct BatchEntry {
@@ -142,6 +144,8 @@ struct BatchEntry {
     PathParams P = {};
     FilmState film;
     int64_t first = 0, paths = 0;
+    int64_t image_off = 0;
+    uint64_t tex_revision = 0;  // the scene's texture revision the last batch render saw
     uint64_t scene_revision = 0;  // the scene's revision when the table was built
 };
 struct Batch {
@@ -156,9 +160,15 @@ struct Batch {
     int refill = 0;  // 0: per-kind automatic threshold
     bool use_graph = true;
     bool dirty = true;        // entries changed since the table / graph were built
+    bool rendered = false;    // the films hold a render of the current table
+    int64_t image_floats = 0; // floats of all entries' images, concatenated
     hipGraphExec_t graph_exec = nullptr;
 };
 int BatchRender(Ctx* c, Batch* b);
+// Path-replay adjoint of every entry of the last batch render, in one launch
+// (adjoint.hip); all entries share one scene.
+int BatchAdjoint(Ctx* c, Batch* b, const float* dimages, float* grad_tex, float* grad_le);
+int BatchImages(Ctx* c, Batch* b, float* images);
 // GPU binned-SAH BVH build (bvh_gpu.hip).
 int BuildBVHDevice(Ctx* c, const float* pos, int32_t n_tri, int max_leaf, BVHBuild* out);
 

@@ -1,6 +1,7 @@
-// LibTorch binding of the differentiable render: a torch::autograd::Function
+// LibTorch binding of the differentiable render: torch::autograd::Functions
 // whose forward renders the scene with the texture texels given as a tensor and
-// whose backward runs the path-replay adjoint (gy_adjoint_device).
+// whose backward runs the path-replay adjoint (gy_adjoint_device for one
+// view, gy_batch_adjoint_device for the views of a batch in one launch).
 //
 //   image = RenderTextures::apply(textures, scene, camera, params)
 //     textures: float32 CUDA tensor, one value per texture float of the scene
@@ -91,8 +92,47 @@ struct RenderTextures : public torch::autograd::Function<RenderTextures> {
     }
 };
 
+// Many views of one textured scene as one batch: forward renders every entry
+// of the batch in one lane space (gy_batch_render) and returns the entries'
+// images concatenated (gy_batch_images_device); backward replays every sample
+// of every entry in one launch (gy_batch_adjoint_device).
+struct RenderViews : public torch::autograd::Function<RenderViews> {
+    static torch::Tensor forward(torch::autograd::AutogradContext* ctx, torch::Tensor textures, int64_t batch, int64_t scene) {
+        gy_scene* sc = reinterpret_cast<gy_scene*>(scene);
+        gy_batch* b = reinterpret_cast<gy_batch*>(batch);
+        TORCH_CHECK(textures.is_cuda() && textures.scalar_type() == torch::kFloat32, "textures: float32 CUDA tensor");
+        TORCH_CHECK(textures.numel() == gy_scene_texture_floats(sc), "textures: one value per texture float of the scene");
+        torch::Tensor tex = textures.contiguous();
+        torch::cuda::synchronize();
+        Check(gy_scene_set_textures_device(sc, tex.data_ptr<float>(), tex.numel()), "gy_scene_set_textures_device");
+        Check(gy_batch_render(b), "gy_batch_render");
+        const int64_t n = gy_batch_image_floats(b);
+        TORCH_CHECK(n > 0, "batch without images");
+        torch::Tensor images = torch::empty({n}, textures.options());
+        Check(gy_batch_images_device(b, images.data_ptr<float>()), "gy_batch_images_device");
+        ctx->saved_data["batch"] = batch;
+        ctx->saved_data["n"] = textures.numel();
+        return images;
+    }
+
+    static torch::autograd::tensor_list backward(torch::autograd::AutogradContext* ctx, torch::autograd::tensor_list grads) {
+        gy_batch* b = reinterpret_cast<gy_batch*>(ctx->saved_data["batch"].toInt());
+        torch::Tensor dimages = grads[0].to(torch::kFloat32).contiguous();
+        torch::Tensor dtex = torch::zeros({ctx->saved_data["n"].toInt()}, dimages.options());
+        torch::cuda::synchronize();
+        // the batch films hold this forward's render unless the batch or the
+        // textures changed in between; gy_batch_adjoint_device refuses that
+        Check(gy_batch_adjoint_device(b, dimages.data_ptr<float>(), dtex.data_ptr<float>(), nullptr), "gy_batch_adjoint_device");
+        return {dtex, torch::Tensor(), torch::Tensor()};
+    }
+};
+
 torch::Tensor render_textures(torch::Tensor textures, int64_t scene, std::vector<double> camera, std::vector<int64_t> params) {
     return RenderTextures::apply(textures, scene, camera, params);
+}
+
+torch::Tensor render_views(torch::Tensor textures, int64_t batch, int64_t scene) {
+    return RenderViews::apply(textures, batch, scene);
 }
 
 }  // namespace
@@ -101,4 +141,6 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
     m.doc() = "Differentiable path-traced render of texture texels (path-replay adjoint)";
     m.def("render_textures", &render_textures, "image = render(textures); backward by path replay",
           py::arg("textures"), py::arg("scene"), py::arg("camera"), py::arg("params"));
+    m.def("render_views", &render_views, "images = render every view of the batch(textures); backward by path replay "
+          "of every view in one launch", py::arg("textures"), py::arg("batch"), py::arg("scene"));
 }

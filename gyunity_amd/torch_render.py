@@ -35,3 +35,10 @@ def render_textures(scene, cam, params, textures):
     prm = [params.spp, params.max_depth, params.rr_depth, params.mode, params.samples_per_pass, params.block_size,
            params.refill_threshold, params.sampler]
     return _load().render_textures(textures, scene.h.value, [float(v) for v in camera], [int(v) for v in prm])
+
+
+def render_views(scene, batch, textures):
+    """Every entry of `batch` (views of `scene`) rendered with `textures` in
+    one lane space; returns the entries' images (H, W, 3 each) concatenated
+    into one flat tensor.  Backward replays every view in one launch."""
+    return _load().render_views(textures, batch.h.value, scene.h.value)

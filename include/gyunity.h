@@ -231,6 +231,17 @@ int gy_batch_add(gy_batch* b, gy_scene* scene, const gy_camera_desc* cam, const 
 int gy_batch_render(gy_batch* b);
 int gy_batch_film_download(gy_batch* b, int32_t entry, float* color, float* weight);
 int gy_batch_destroy(gy_batch* b);
+/* Differentiable batch: every entry of the last gy_batch_render renders the
+ * same textured scene (views of one scene).  Images are concatenated in entry
+ * order, width*height*3 floats each (gy_batch_image_floats in total; -1 before
+ * the entries are built).  gy_batch_adjoint replays every sample of every entry
+ * in one launch (see gy_adjoint): dimages holds dl/dI of all images; grad_tex
+ * one float per texture float, grad_le 3 per area light (NULL: none);
+ * gy_batch_adjoint writes them, the _device form adds to them. */
+int64_t gy_batch_image_floats(gy_batch* b);
+int gy_batch_images_device(gy_batch* b, float* images_device);
+int gy_batch_adjoint(gy_batch* b, const float* dimages, float* grad_tex, float* grad_le);
+int gy_batch_adjoint_device(gy_batch* b, const float* dimages_device, float* grad_tex_device, float* grad_le_device);
 
 /* ---- samplers (SobolSampler::Sample, sampler/Sobol.h:66-77;
  *      SobolEnum::GetIndex, sampler/SobolEnum.h:24-28; HaltonSampler,
@@ -325,6 +336,15 @@ int gy_scene_set_textures_device(gy_scene* scene, const float* tex_data_device, 
 int gy_adjoint(gy_scene* scene, const gy_camera_desc* cam, const gy_render_params* p, const float* dimage, float* grad);
 int gy_adjoint_device(gy_scene* scene, const gy_camera_desc* cam, const gy_render_params* p, const float* dimage_device,
                       float* grad_device);
+/* The same, plus the gradient with respect to each area light's emitted
+ * radiance Le (AreaLight::Emission, light/AreaLight.h:16, 33-43): grad_le holds
+ * 3 floats per area light in light order (NULL: none).  The light-selection
+ * probabilities (Scene::ComputeLightPowerDistribution) are sampling
+ * probabilities and held fixed, as every pdf of the replay. */
+int gy_adjoint_emission(gy_scene* scene, const gy_camera_desc* cam, const gy_render_params* p, const float* dimage,
+                        float* grad, float* grad_le);
+int gy_adjoint_emission_device(gy_scene* scene, const gy_camera_desc* cam, const gy_render_params* p,
+                               const float* dimage_device, float* grad_device, float* grad_le_device);
 /* The roughness terms of the adjoint, evaluated on the host (tests): d log f / d
  * alpha of the isotropic GGX reflection lobe for local directions wo, wi, and
  * d alpha / d roughness of RoughnessToAlpha (visual/Microfacet.h:49-54). */

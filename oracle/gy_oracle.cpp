@@ -583,6 +583,14 @@ template <class R> struct Scene {
     std::vector<BVHNode<R>> nodes;
     std::vector<uint32_t> ordered;  // BVH leaf order -> primitive id
     int max_depth = 0;
+    // Test-only detached evaluation (or_set_detached): paths are sampled with
+    // tex and light_le (directions, pdfs, MIS weights, light choice), while
+    // BSDF values and emitted radiance come from tex_eval and light_le_eval.
+    // Differences of renders in this mode are the derivative a path-replay
+    // adjoint estimates, which holds every sampling decision fixed.
+    bool detached = false;
+    std::vector<Texture<R>> tex_eval;
+    std::vector<V3<R>> light_le_eval;
 };
 
 // Triangle(..., normal) / Triangle(..., n0, n1, n2) + Initialize
@@ -1298,7 +1306,14 @@ template <class R> struct BSDF {
 };
 // Material::ComputeScatteringFunction (material/Diffuse.h:27-31, Roughness.h:36-46,
 // Glass.h:30-34, RoughDielectric.h:21-41)
-template <class R> BSDF<R> make_bsdf(const Scene<R>& s, const Isect<R>& it) {
+template <class R> BSDF<R> make_bsdf_tex(const Scene<R>& s, const Isect<R>& it, const std::vector<Texture<R>>& texs);
+template <class R> BSDF<R> make_bsdf(const Scene<R>& s, const Isect<R>& it) { return make_bsdf_tex(s, it, s.tex); }
+// The BSDF whose values the estimator takes: the sampled one, or in detached
+// mode the same material with the evaluation textures.
+template <class R> BSDF<R> eval_bsdf(const Scene<R>& s, const Isect<R>& it, const BSDF<R>& sampled) {
+    return s.detached ? make_bsdf_tex(s, it, s.tex_eval) : sampled;
+}
+template <class R> BSDF<R> make_bsdf_tex(const Scene<R>& s, const Isect<R>& it, const std::vector<Texture<R>>& texs) {
     const Material& m = s.mats[s.tris[it.prim].mat];
     BSDF<R> b;
     b.type = (int)m.type;
@@ -1309,14 +1324,14 @@ template <class R> BSDF<R> make_bsdf(const Scene<R>& s, const Isect<R>& it) {
     b.Rc = V3<R>(m.R[0], m.R[1], m.R[2]);
     b.Tc = V3<R>(m.T[0], m.T[1], m.T[2]);
     if (m.tex_albedo >= 0) {
-        const Texture<R>& t = s.tex[m.tex_albedo];
+        const Texture<R>& t = texs[m.tex_albedo];
         // channels past the texture's last one read the last one
         b.Rc = V3<R>(tex_bilinear(t, it.u, it.v, 0), tex_bilinear(t, it.u, it.v, std::min(1, t.c - 1)),
                      tex_bilinear(t, it.u, it.v, std::min(2, t.c - 1)));
     }
     R ax = m.ax, ay = m.ay;
     if (m.tex_rough >= 0) {
-        ax = ay = tex_bilinear(s.tex[m.tex_rough], it.u, it.v, 0);
+        ax = ay = tex_bilinear(texs[m.tex_rough], it.u, it.v, 0);
     }
     if (!(m.flags & 1u)) {
         ax = roughness_to_alpha(ax);
@@ -1523,14 +1538,22 @@ template <class R> inline R power_heuristic(R f, R g) {  // integrator/Integrato
 }
 
 // Integrator::DirectIllumination (integrator/Integrator.cpp:9-107), surfaces only.
+// The value a BSDF sample takes in detached mode: the evaluation BSDF's lobe
+// at the sampled pair of directions, zero where the sample itself failed.
+template <class R> V3<R> detached_value(const BSDF<R>& be, const V3<R>& wo, const V3<R>& wi, const V3<R>& f_sampled) {
+    if (f_sampled == V3<R>()) return f_sampled;
+    return be.lobe_eval(be.to_local(wo), be.to_local(wi));
+}
+
 template <class R> V3<R> direct_illumination(const Scene<R>& s, const Isect<R>& it, const BSDF<R>& bsdf, R uLight,
                                              R u0, R u1, const V3<R>& v, bool use_bvh) {
     R lightSamplingPdf;
     R weight1 = 1, weight2 = 1;
     V3<R> L1, L2;
+    const BSDF<R> be = eval_bsdf(s, it, bsdf);
     int light = sample_one_light(s, uLight, &lightSamplingPdf);
     int ltri = s.light_tri[light];  // < 0: the environment light
-    V3<R> Le = s.light_le[light];
+    V3<R> Le = s.detached ? s.light_le_eval[light] : s.light_le[light];
     {
         R pdf;
         LightPoint<R> lp;
@@ -1551,7 +1574,7 @@ template <class R> V3<R> direct_illumination(const Scene<R>& s, const Isect<R>& 
         }
         if (pdf > 0 && Li != V3<R>(0)) {
             R scatteringPdf = bsdf.Pdf(it.outDir, wi);
-            V3<R> f = bsdf.Evaluate(it.outDir, wi);
+            V3<R> f = be.Evaluate(it.outDir, wi);
             R cosTheta = std::abs(dot(it.n, wi));
             if (f != V3<R>()) {
                 V3<R> Tr(1.f);
@@ -1566,6 +1589,7 @@ template <class R> V3<R> direct_illumination(const Scene<R>& s, const Isect<R>& 
         V3<R> wi;
         R pdf;
         V3<R> f = bsdf.Sample(it.outDir, &wi, &pdf, v);
+        if (s.detached) f = detached_value(be, it.outDir, wi, f);
         R cosTheta = std::abs(dot(it.n, wi));
         if (pdf != 0 && f != V3<R>()) {
             R lpdf = ltri < 0 ? env_pdf_dir(s.env, wi) : light_pdf(s, ltri, it, wi);
@@ -1630,7 +1654,7 @@ template <class R> V3<R> Li(const Scene<R>& s, Ray<R> ray, Seq& rnd, int maxDept
         BSDF<R> bsdf = make_bsdf(s, it);
         int light = s.tris[it.prim].light;
         if ((i == 0 || delta) && light >= 0) {
-            V3<R> Le = s.light_le[light];
+            V3<R> Le = s.detached ? s.light_le_eval[light] : s.light_le[light];
             V3<R> e = dot(it.n, it.outDir) > 0 ? Le : V3<R>(0, 0, 0);
             L = L + T * e;
         } else {
@@ -1644,6 +1668,7 @@ template <class R> V3<R> Li(const Scene<R>& s, Ray<R> ray, Seq& rnd, int maxDept
         R pdf;
         V3<R> f = bsdf.Sample(-ray.d, &wi, &pdf, V3<R>(r0, r1, r2));
         if (f == V3<R>() || pdf == 0) break;
+        if (s.detached) f = detached_value(eval_bsdf(s, it, bsdf), -ray.d, wi, f);
         V3<R> est = f * std::abs(dot(it.n, wi)) / pdf;
         delta = bsdf.delta();
         R p = rmin((R)1.0, lumY(est * T) / lumY(T));
@@ -2035,6 +2060,30 @@ template <class R> int64_t render_t(SceneHandle* h, const or_camera_desc* cd, co
     return paths.load();
 }
 
+// Test-only: switch a scene to detached evaluation (Scene::detached) with
+// texels tex_eval (the layout of the scene's tex_data; null: the scene's own)
+// and per-light emitted radiance le_eval (3 per light; null: the scene's
+// own); both null switches it off.
+template <class R> void set_detached(Scene<R>& s, const or_scene_desc& d, const float* tex, const float* le) {
+    if (!tex && !le) {
+        s.detached = false;
+        s.tex_eval.clear();
+        s.light_le_eval.clear();
+        return;
+    }
+    s.detached = true;
+    s.tex_eval = s.tex;
+    if (tex)
+        for (size_t i = 0; i < s.tex_eval.size(); ++i) {
+            Texture<R>& t = s.tex_eval[i];
+            const float* src = tex + d.tex_info[4 * i + 3];
+            for (size_t k = 0; k < t.data.size(); ++k) t.data[k] = (R)src[k];
+        }
+    s.light_le_eval = s.light_le;
+    if (le)
+        for (size_t i = 0; i < s.light_le_eval.size(); ++i) s.light_le_eval[i] = V3<R>(le[3 * i], le[3 * i + 1], le[3 * i + 2]);
+}
+
 }  // namespace orc
 
 using namespace orc;
@@ -2257,6 +2306,13 @@ int or_faure_perm(uint32_t base, uint16_t* out) {
     for (uint32_t i = 0; i < base; ++i) out[i] = T.perm[base][i];
     return 0;
 }
+int or_set_detached(void* hv, const float* tex_eval, const float* le_eval) {
+    SceneHandle* h = (SceneHandle*)hv;
+    set_detached(h->sf, h->desc, tex_eval, le_eval);
+    set_detached(scene_of<double>(h), h->desc, tex_eval, le_eval);
+    return 0;
+}
+
 int or_set_shadow_short(int on) {
     g_shadow_short = on;
     return 0;
@@ -2399,6 +2455,7 @@ int or_fr_conductor(double cosI, const double* eta, const double* k, double* out
     out[0] = r.x; out[1] = r.y; out[2] = r.z;
     return 0;
 }
+double or_roughness_to_alpha(double r) { return roughness_to_alpha(r); }
 double or_ggx_D(double ax, double ay, const double* m) {
     GGX<double> g{ax, ay};
     return g.D(V3<double>(m[0], m[1], m[2]));

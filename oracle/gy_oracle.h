@@ -165,11 +165,19 @@ int or_bsdf_sample_mode(const uint32_t* mat_words, int32_t n, const double* wo, 
                         double* pdf, int importance);
 double or_fr_dielectric(double cosI, double etaI, double etaT);
 int or_fr_conductor(double cosI, const double* eta, const double* k, double* out);
+/* GGXDistribution::RoughnessToAlpha (visual/Microfacet.h:49-54) in double. */
+double or_roughness_to_alpha(double r);
 double or_ggx_D(double ax, double ay, const double* m);
 int or_tex_lookup(int w, int h, int c, const float* data, int n, const float* u, const float* v, int ch, float* out);
 /* Test-only switch: end NEE shadow segments ShadowRayEps before the light point
  * (the reference's segment reaches it, see spawn_to in gy_oracle.cpp). */
 int or_set_shadow_short(int on);
+/* Test-only detached evaluation: paths keep being sampled with the scene's own
+ * texels and emitted radiance, while BSDF values and emission are taken from
+ * tex_eval (the layout of tex_data) and le_eval (3 floats per light); NULL
+ * keeps the scene's own; both NULL switches it off.  Differences of renders
+ * in this mode are what a path-replay adjoint estimates. */
+int or_set_detached(void* scene, const float* tex_eval, const float* le_eval);
 /* HaltonSampler::sample (Faure) and HaltonEnum::get_index; Faure permutation of a base. */
 float or_halton_sample(uint64_t index, uint32_t dim);
 uint64_t or_halton_enum_index(uint32_t w, uint32_t h, uint64_t i, uint32_t x, uint32_t y);

@@ -95,12 +95,14 @@ def lib():
         "or_fr_dielectric": (C.c_double, [C.c_double, C.c_double, C.c_double]),
         "or_fr_conductor": (C.c_int, [C.c_double, vp, vp, vp]),
         "or_ggx_D": (C.c_double, [C.c_double, C.c_double, vp]),
+        "or_roughness_to_alpha": (C.c_double, [C.c_double]),
         "or_li_path": (C.c_int, [vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp]),
         "or_halton_sample": (C.c_float, [C.c_uint64, C.c_uint32]),
         "or_halton_enum_index": (C.c_uint64, [C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32]),
         "or_faure_perm": (C.c_int, [C.c_uint32, vp]),
         "or_tex_lookup": (C.c_int, [C.c_int, C.c_int, C.c_int, vp, C.c_int, vp, vp, C.c_int, vp]),
         "or_set_shadow_short": (C.c_int, [C.c_int]),
+        "or_set_detached": (C.c_int, [vp, vp, vp]),
         "or_scene_set_environment": (C.c_int, [vp, vp, i32, i32, f32]),
         "or_env_query": (C.c_int, [vp, i32, vp, vp, vp, vp, vp, vp, C.c_int]),
         "or_env_info": (C.c_int, [vp, vp, vp, i32]),
@@ -274,6 +276,14 @@ class OracleScene:
         n = lib().or_env_info(self.h, C.byref(r), _p(f), 64)
         return r.value, f[:n].copy()
 
+    def set_detached(self, tex_eval=None, le_eval=None):
+        """Test-only detached evaluation: paths sampled with the scene's own
+        texels and Le, BSDF values and emission from tex_eval / le_eval
+        (None: the scene's own; both None: off)."""
+        self._tex_eval = None if tex_eval is None else np.ascontiguousarray(tex_eval, np.float32)
+        self._le_eval = None if le_eval is None else np.ascontiguousarray(le_eval, np.float32)
+        lib().or_set_detached(self.h, _p(self._tex_eval), _p(self._le_eval))
+
     def render_lt(self, cam, prm):
         """LightTracing (integrator/LT.h) in float mode: the (H, W, 3) splat sums."""
         c = camera_from(cam)
@@ -353,6 +363,10 @@ def fr_conductor(c, eta, k):
     out = np.zeros(3)
     lib().or_fr_conductor(c, eta.ctypes.data, k.ctypes.data, out.ctypes.data)
     return out
+
+
+def roughness_to_alpha(r):
+    return lib().or_roughness_to_alpha(float(r))
 
 
 def ggx_D(ax, ay, m):

@@ -544,6 +544,40 @@ def test_batch_films_equal_single_renders(gpu_ctx):
             sc.close()
 
 
+def test_batch64_members_equal_single_renders_and_oracle(gpu_ctx):
+    # The 64-scene batch workload as configured (GY_SCENE_BATCH_MEMBER seeds
+    # 0..63: 1.2e3..1.9e6 triangles, 128^2..1024^2, 4..64 spp, depth 4..12,
+    # conductors and rough dielectrics), in one lane space through the
+    # recorded graph, replayed twice: every member's film equals its own
+    # wavefront render bit for bit.  Three small members with rough
+    # dielectrics pass level (b) against the oracle.
+    members = [g.generate_scene(g.GY_SCENE_BATCH_MEMBER, s, 0) for s in range(64)]
+    scenes = [g.Scene(gpu_ctx, sa) for sa, _, _ in members]
+    try:
+        with g.Batch(gpu_ctx, use_graph=True) as b:
+            ids = [b.add(sc, cam, prm) for sc, (_, cam, prm) in zip(scenes, members)]
+            for rep in range(2):
+                b.render()
+                st = gpu_ctx.stats()
+                assert st.paths == sum(c.width * c.height * p.spp for _, c, p in members)
+                if rep == 0:
+                    films = [b.film(e) for e in ids]
+            for i, (sc, (_, cam, prm)) in enumerate(zip(scenes, members)):
+                prm.mode = g.GY_RENDER_WAVEFRONT
+                color, weight = sc.render(cam, prm)
+                bc, bw = b.film(ids[i])
+                assert np.array_equal(bw, weight) and np.array_equal(bc, color), i
+                assert np.array_equal(films[i][1], weight) and np.array_equal(films[i][0], color), i
+                assert np.isfinite(color).all(), i
+    finally:
+        for sc in scenes:
+            sc.close()
+    for seed in (1, 13, 54):
+        sa, cam, prm = members[seed]
+        assert (sa.mats[:, 0] == 3).any()  # a rough dielectric
+        _li_parity(gpu_ctx, sa, cam, prm, 5000, 70 + seed)
+
+
 def test_batch_errors(gpu_ctx):
     sa, cam, prm = g.generate_scene(g.GY_SCENE_CORNELL)
     with g.Scene(gpu_ctx, sa) as sc, g.Batch(gpu_ctx) as b:
@@ -659,6 +693,160 @@ def test_adjoint_roughness_converges_to_finite_differences(gpu_ctx):
     assert np.median(gaps[4096]) < 0.1
 
 
+def test_adjoint_emission_matches_finite_differences(gpu_ctx):
+    # d loss / d Le: the area lights' emitted radiance (AreaLight::Emission,
+    # light/AreaLight.h:33-43).  Scaling channel c of every light together
+    # keeps their power ratios, so the light-selection CDF stays and common
+    # random numbers give the same paths: the central difference must equal
+    # the sum of the lights' channel-c gradients.
+    import copy
+
+    sa, cam, prm, gt, theta, n_alb = _adjoint_setup(spp=64)
+    with g.Scene(gpu_ctx, sa) as sc:
+        target = _image(*sc.render(cam, prm))
+        sc.set_textures(theta)
+        img = _image(*sc.render(cam, prm))
+        dI = (2.0 * (img - target) / img.size).astype(np.float32)
+        grad, gle = sc.adjoint(cam, prm, dI, emission=True)
+        grad_plain = sc.adjoint(cam, prm, dI)
+    # the same scatter (float atomics: the summation order may differ)
+    assert np.abs(grad - grad_plain).max() <= 1e-5 * np.abs(grad_plain).max()
+    assert gle.shape == (sa.light_tri.shape[0], 3) and np.isfinite(gle).all() and np.abs(gle).max() > 0
+    le0 = sa.light_le.copy()
+    for c in range(3):
+        h = 1e-3 * float(le0[:, c].max())
+        fd = []
+        for sgn in (1, -1):
+            s2 = copy.copy(sa)
+            s2.light_le = le0.copy()
+            s2.light_le[:, c] += sgn * h
+            with g.Scene(gpu_ctx, s2) as sc2:
+                sc2.set_textures(theta)
+                fd.append(_loss(_image(*sc2.render(cam, prm)), target))
+        fdv = (fd[0] - fd[1]) / (2 * h)
+        rel = abs(gle[:, c].sum() - fdv) / abs(fdv)
+        print("emission channel %d: replay %+.6e  FD %+.6e  rel %.2e" % (c, gle[:, c].sum(), fdv, rel))
+        assert rel < 1e-2, (c, rel)
+
+
+def test_adjoint_small_configuration(gpu_ctx):
+    # The small adjoint as configured: the textured Cornell room with 64x64
+    # textures, a 256^2 target at 64 spp, L2 loss; 64 random albedo texels
+    # against central differences (h = 1e-3, common random numbers: the same
+    # sample indices).  Albedo is not sampled, so replay and the differences
+    # see the same paths and agree to float rounding.
+    sa, cam, prm, gt, theta, n_alb = _adjoint_setup(tex_res=64, res=256, spp=64, seed=11)
+    rng = np.random.default_rng(5)
+    with g.Scene(gpu_ctx, sa) as sc:
+        sc.set_textures(gt)
+        target = _image(*sc.render(cam, prm))
+        sc.set_textures(theta)
+        img = _image(*sc.render(cam, prm))
+        dI = (2.0 * (img - target) / img.size).astype(np.float32)
+        grad = sc.adjoint(cam, prm, dI)
+        ks = rng.choice(n_alb, 64, replace=False)
+        fd = np.zeros(64)
+        for j, k in enumerate(ks):
+            lv = []
+            for sgn in (1, -1):
+                th = theta.copy()
+                th[k] += sgn * 1e-3
+                sc.set_textures(th)
+                lv.append(_loss(_image(*sc.render(cam, prm)), target))
+            fd[j] = (lv[0] - lv[1]) / 2e-3
+        sc.set_textures(theta)
+    # Within 1e-2 relative over the 64 texels (L2), and texel by texel where
+    # the difference quotient stands clear of the FP32 image rounding it is
+    # taken from (|FD| at least a tenth of the largest)
+    g64 = grad[ks].astype(np.float64)
+    rel_l2 = np.linalg.norm(g64 - fd) / np.linalg.norm(fd)
+    big = np.abs(fd) > 0.1 * np.abs(fd).max()
+    rel = np.abs(g64[big] - fd[big]) / np.abs(fd[big])
+    print("albedo, 64 random texels: relative L2 %.2e, max relative %.2e over %d texels" % (rel_l2, rel.max(), big.sum()))
+    assert rel_l2 < 1e-2 and rel.max() < 1e-2
+
+
+def test_batch_adjoint_equals_per_view_adjoints(gpu_ctx):
+    # Views of one textured scene in one batch: the batch adjoint (one launch
+    # over every sample of every view) equals the sum of the views' own
+    # adjoints, textures and emission, up to the float atomics' order.
+    sa, cam, prm, gt, theta, n_alb = _adjoint_setup(spp=16)
+    views = g.orbit_views(4, 48, seed=9, centre=(0.5, 0.4, 0.5), radius=0.3)
+    with g.Scene(gpu_ctx, sa) as sc:
+        sc.set_textures(gt)
+        targets = []
+        for v in views:
+            targets.append(_image(*sc.render(v, prm)))
+        sc.set_textures(theta)
+        with g.Batch(gpu_ctx) as b:
+            for v in views:
+                b.add(sc, v, prm)
+            b.render()
+            imgs = [_image(*b.film(e)) for e in range(len(views))]
+            dIs = [(2.0 * (im - t) / im.size).astype(np.float32) for im, t in zip(imgs, targets)]
+            gb, lb = b.adjoint(dIs, sc.texture_floats(), sc.n_light)
+            sc.set_textures(gt)
+            with pytest.raises(g.GyError):  # textures changed since the batch render
+                b.adjoint(dIs, sc.texture_floats(), sc.n_light)
+            sc.set_textures(theta)
+        gs = np.zeros_like(gb)
+        ls = np.zeros_like(lb)
+        for v, dI, im in zip(views, dIs, imgs):
+            color, weight = sc.render(v, prm)
+            assert np.allclose(_image(color, weight), im, rtol=0, atol=0)
+            gv, lv = sc.adjoint(v, prm, dI, emission=True)
+            gs += gv
+            ls += lv
+    assert np.abs(gb - gs).max() <= 1e-5 * np.abs(gs).max()
+    assert np.abs(lb - ls).max() <= 1e-5 * np.abs(ls).max()
+    assert np.abs(gs).max() > 0
+
+
+def test_adjoint_at_scale_adam_steps(gpu_ctx):
+    # The adjoint at scale as configured: the 1.31M-triangle GGX scene with
+    # 1024^2 RGB albedo and roughness textures (4.2M parameters), 32 views of
+    # 512^2 at 32 spp, depth 6, rendered in one batch and replayed in one
+    # launch; three Adam steps (lr 1e-2) from a seeded uniform start: the
+    # gradients are finite and the loss falls.
+    sa, _, prm = g.generate_scene(g.GY_SCENE_ICOSPHERE_TEXTURED, 0, 1024)
+    prm.mode = g.GY_RENDER_WAVEFRONT
+    assert (prm.spp, prm.max_depth) == (32, 6)
+    gt = sa.tex_data.copy()
+    n_alb = int(sa.tex_info[1, 3])
+    assert gt.size == 4 * 1024 * 1024
+    views = g.orbit_views(32, 512)
+    rng = np.random.default_rng(3)
+    x = np.concatenate([rng.uniform(0.2, 0.8, n_alb), rng.uniform(0.05, 0.8, gt.size - n_alb)]).astype(np.float32)
+    lo = np.where(np.arange(gt.size) < n_alb, 0.01, 0.05).astype(np.float32)
+    hi = np.where(np.arange(gt.size) < n_alb, 0.99, 0.8).astype(np.float32)
+    m = np.zeros_like(x)
+    v = np.zeros_like(x)
+    losses = []
+    with g.Scene(gpu_ctx, sa) as sc, g.Batch(gpu_ctx) as b:
+        for c in views:
+            b.add(sc, c, prm)
+        sc.set_textures(gt)
+        b.render()
+        targets = [_image(*b.film(e)) for e in range(len(views))]
+        for step in range(1, 4):
+            sc.set_textures(x)
+            b.render()
+            imgs = [_image(*b.film(e)) for e in range(len(views))]
+            n = sum(im.size for im in imgs)
+            losses.append(sum(float(((im - t) ** 2).sum()) for im, t in zip(imgs, targets)) / n)
+            dIs = [(2.0 * (im - t) / n).astype(np.float32) for im, t in zip(imgs, targets)]
+            grad = b.adjoint(dIs, sc.texture_floats())
+            assert np.isfinite(grad).all() and np.abs(grad).max() > 0
+            # Adam (Kingma and Ba 2015), lr 1e-2, then the texel ranges
+            m = 0.9 * m + 0.1 * grad
+            v = 0.999 * v + 0.001 * grad * grad
+            mh = m / (1 - 0.9 ** step)
+            vh = v / (1 - 0.999 ** step)
+            x = np.clip(x - 1e-2 * mh / (np.sqrt(vh) + 1e-8), lo, hi).astype(np.float32)
+    print("adjoint at scale, loss per Adam step:", ["%.5e" % l for l in losses])
+    assert losses[2] < losses[1] < losses[0]
+
+
 def test_torch_autograd_function_matches_adjoint(gpu_ctx):
     # gy_autograd (LibTorch torch::autograd::Function) in a fresh process that
     # imports torch first, so libgyunity.so binds to the HIP runtime torch
@@ -672,8 +860,9 @@ def test_torch_autograd_function_matches_adjoint(gpu_ctx):
     print(r.stdout[-2000:], r.stderr[-2000:])
     assert r.returncode == 0
     res = json.loads(r.stdout.strip().splitlines()[-1])
-    assert res["image_equal"]
+    assert res["image_equal"] and res["views_equal"]
     assert res["grad_max_abs_diff"] <= 1e-4 * res["grad_max_abs"], res
+    assert res["views_grad_max_abs_diff"] <= 1e-4 * res["views_grad_max_abs"] and res["views_grad_max_abs"] > 0, res
 
 
 def test_autotune_returns_a_faster_equivalent_launch(gpu_ctx):
