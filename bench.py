@@ -458,6 +458,96 @@ def adjoint_scale_main(args):
     print(json.dumps(out), flush=True)
 
 
+def batch_leg(g, ctx, steps=2):
+    """The 64-scene batch (seeds 0..63), compact: one graph-replayed batch
+    render warmed up and `steps` timed, against one wavefront and one
+    megakernel call per member (each warmed up once)."""
+    t0 = time.perf_counter()
+    members = [g.generate_scene(g.GY_SCENE_BATCH_MEMBER, s, 0) for s in range(64)]
+    scenes = [g.Scene(ctx, sa) for sa, _, _ in members]
+    paths = sum(cam.width * cam.height * prm.spp for _, cam, prm in members)
+    log("batch64 leg: 64 members resident in %.1f s" % (time.perf_counter() - t0))
+    batch = g.Batch(ctx)
+    for sc, (_, cam, prm) in zip(scenes, members):
+        batch.add(sc, cam, prm)
+    batch.render()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.render()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    per = {}
+    for mode, name in ((g.GY_RENDER_WAVEFRONT, "wavefront"), (g.GY_RENDER_MEGAKERNEL, "megakernel")):
+        for sc, (_, cam, prm) in zip(scenes, members):
+            prm.mode = mode
+            sc.render(cam, prm, download=False)
+        t0 = time.perf_counter()
+        for sc, (_, cam, prm) in zip(scenes, members):
+            sc.render(cam, prm, download=False)
+        per[name] = (time.perf_counter() - t0) * 1e3
+    log("batch64 leg: batch %.1f ms, per member wavefront %.1f ms, megakernel %.1f ms" % (ms, per["wavefront"], per["megakernel"]))
+    batch.close()
+    for sc in scenes:
+        sc.close()
+    return {"workload": "64-scene batch, seeds 0..63 (1.2e3..1.9e6 triangles, 128^2..1024^2, 4..64 spp, depth 4..12)",
+            "paths_per_render": paths, "batch_mpaths_per_s": round(paths / ms / 1e3, 3), "batch_ms": round(ms, 2),
+            "per_member_wavefront_mpaths_per_s": round(paths / per["wavefront"] / 1e3, 3),
+            "per_member_megakernel_mpaths_per_s": round(paths / per["megakernel"] / 1e3, 3),
+            "form": "one lane space, recorded hipGraph replayed", "steps": steps}
+
+
+def adjoint_leg(g, ctx, steps_small=5, steps_scale=2):
+    """Seconds per gradient: the small adjoint (textured Cornell room, 64^2
+    textures, 256^2 at 64 spp: set texels, render, replay) and the adjoint at
+    scale (1.31M triangles, 1024^2 albedo + roughness textures, 32 views of
+    512^2 at 32 spp, depth 6: one batch render and one replay launch per
+    gradient, images and dl/dI on the host)."""
+    out = {}
+    sa, cam, prm = g.generate_scene(g.GY_SCENE_ADJOINT, 0, 64)
+    cam.width = cam.height = 256
+    prm.spp, prm.rr_depth, prm.mode = 64, 100, g.GY_RENDER_WAVEFRONT
+    gt = sa.tex_data.copy()
+    theta = np.clip(gt + np.random.default_rng(7).uniform(-0.2, 0.2, gt.size), 0.05, 0.8).astype(np.float32)
+    with g.Scene(ctx, sa) as sc:
+        target = _image(*sc.render(cam, prm))
+        times = []
+        for i in range(steps_small + 1):
+            t0 = time.perf_counter()
+            sc.set_textures(theta)
+            img = _image(*sc.render(cam, prm))
+            sc.adjoint(cam, prm, (2.0 * (img - target) / img.size).astype(np.float32))
+            times.append(time.perf_counter() - t0)
+    out["small"] = {"s_per_grad": round(float(np.mean(times[1:])), 4), "paths_per_grad": 256 * 256 * 64,
+                    "workload": "textured Cornell room, 64^2 textures, 256^2, 64 spp"}
+    log("adjoint leg small: %.4f s/grad" % out["small"]["s_per_grad"])
+    sa, _, prm = g.generate_scene(g.GY_SCENE_ICOSPHERE_TEXTURED, 0, 1024)
+    prm.mode = g.GY_RENDER_WAVEFRONT
+    gt = sa.tex_data.copy()
+    n_alb = int(sa.tex_info[1, 3])
+    rng = np.random.default_rng(3)
+    x = np.concatenate([rng.uniform(0.2, 0.8, n_alb), rng.uniform(0.05, 0.8, gt.size - n_alb)]).astype(np.float32)
+    views = g.orbit_views(32, 512)
+    with g.Scene(ctx, sa) as sc, g.Batch(ctx) as b:
+        for c in views:
+            b.add(sc, c, prm)
+        b.render()
+        targets = [_image(*b.film(e)) for e in range(len(views))]
+        times = []
+        for i in range(steps_scale + 1):
+            t0 = time.perf_counter()
+            sc.set_textures(x)
+            b.render()
+            imgs = [_image(*b.film(e)) for e in range(len(views))]
+            n = sum(im.size for im in imgs)
+            grad = b.adjoint([(2.0 * (im - t) / n).astype(np.float32) for im, t in zip(imgs, targets)], sc.texture_floats())
+            times.append(time.perf_counter() - t0)
+        assert np.isfinite(grad).all()
+    out["scale"] = {"s_per_grad": round(float(np.mean(times[1:])), 4), "paths_per_grad": 32 * 512 * 512 * 32,
+                    "workload": "1.31M-triangle GGX icosphere, 4.2M texture floats, 32 views 512^2, 32 spp, depth 6, "
+                                "one batch render + one replay launch"}
+    log("adjoint leg scale: %.3f s/grad" % out["scale"]["s_per_grad"])
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -467,6 +557,7 @@ def main():
     ap.add_argument("--level", type=int, default=8)
     ap.add_argument("--cpu-rows", type=int, default=128)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-legs", action="store_true", help="default workload: skip the batch64 and adjoint legs")
     ap.add_argument("--single", action="store_true", help="time only --mode, not the other integrator")
     ap.add_argument("--refill", type=int, default=0, help="wavefront refill threshold (0 = library default)")
     ap.add_argument("--spp-pass", type=int, default=0, help="samples per pass (0 = library default)")
@@ -523,6 +614,16 @@ def main():
     sst = sc.stats()
     log("scene: %d triangles, BVH %d nodes depth %d, build %.1f ms (scene create %.1f ms)"
         % (sa.n_tri, sst.n_nodes, sst.max_depth, sst.build_ms, (time.perf_counter() - t0) * 1e3))
+    # FP64 16384-spp reference of five rows of this image (tests/golden): the
+    # "at fixed RMSE" of the metric, measured on each integrator's last render
+    ref_rows, ref_img = None, None
+    gpath = os.path.join(ROOT, "tests", "golden", "level_c.json")
+    if args.level == 8 and env is None and os.path.exists(gpath):
+        with open(gpath) as f:
+            ref_rows = json.load(f)["ico_rows"]
+        ref_img = np.load(os.path.join(ROOT, "tests", "golden", "icosphere_rows_fp64_16384spp.npy"))
+    rmse = {}
+
     def run(mode, steps, warmup):
         prm.mode = g.GY_RENDER_WAVEFRONT if mode == "wavefront" else g.GY_RENDER_MEGAKERNEL
         for i in range(warmup):
@@ -537,6 +638,9 @@ def main():
             kn += np.array(st.kernel_launches[:])
             log("%s step %d: %.1f ms, %.1f Mrays" % (mode, i, st.total_ms, st.rays / 1e6))
         ms = (time.perf_counter() - t0) * 1e3 / steps
+        if ref_img is not None:
+            rmse[mode] = sc.film_rmse(ref_img, ref_rows)
+            log("%s rmse against the FP64 16384-spp reference rows: %.6f" % (mode, rmse[mode]))
         log("%s per-kernel ms/step: %s" % (mode, ", ".join(
             "%s %.1f" % (g.lib().gy_kernel_name(k).decode(), kms[k] / steps) for k in range(8) if kn[k] > 0)))
         return ms, kms, kn, st
@@ -579,6 +683,12 @@ def main():
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": traffic,
+        # HBM bytes the counters measured per launch over the live launch time
+        "measured_hbm_gbs": round(traffic / (avg_launch_ms / 1e3) / 1e9, 2) if traffic else None,
+        "measured_hbm_frac": round(traffic / (avg_launch_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
+        "limiter": "VALU with divergent lanes (profiles/counters_icosphere-L8-1024-64spp.txt): the upper BVH levels "
+                   "are re-read from L2 and the Infinity Cache, so counter-measured HBM bytes are about half the "
+                   "algorithmic bytes",
         "algorithmic_bytes_per_launch": bytes_per_launch,
         "avg_launch_ms": round(avg_launch_ms, 4),
         "launches_per_step": launches_per_step,
@@ -608,6 +718,9 @@ def main():
             "wavefront_mpaths_per_s": round(paths / ((ms_per_step if args.mode == "wavefront" else other_ms) / 1e3) / 1e6, 3)
             if (args.mode == "wavefront" or other_ms) else None,
             "same_estimator": "wavefront and megakernel films are bit-identical (tests/test_gpu_parity.py), so RMSE is equal at equal spp",
+            "rmse_vs_fp64_reference": ({k: round(v, 6) for k, v in rmse.items()} if rmse else None),
+            "rmse_reference": ("FP64 16384-spp oracle render of rows %s (tests/golden), all channels" % ref_rows)
+            if rmse else None,
             "paths_per_step": paths,
             "mrays_per_s": round(rays / (ms_per_step / 1e3) / 1e6, 2),
             "rays_per_step": int(rays),
@@ -618,6 +731,8 @@ def main():
         "roofline": roof,
     }
     sc.close()
+    if not args.no_legs and env is None and args.level == 8:
+        out["legs"] = {"batch64": batch_leg(g, ctx), "adjoint": adjoint_leg(g, ctx)}
     if args.calibrate_copy > 0:
         nbytes = args.calibrate_copy << 20
         cms = ctx.copy_probe(nbytes)

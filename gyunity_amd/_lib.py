@@ -153,6 +153,7 @@ def lib():
         "gy_count_visits": (C.c_int, [vp, vp, i64, P(i64), P(i64)]),
         "gy_render": (C.c_int, [vp, P(CameraDesc), P(RenderParams), P(f32), P(f32)]),
         "gy_render_noalloc": (C.c_int, [vp, P(CameraDesc), P(RenderParams), vp]),
+        "gy_film_rmse": (C.c_int, [vp, P(f32), P(i32), i32, P(C.c_double)]),
         "gy_film_download": (C.c_int, [vp, P(f32), P(f32)]),
         "gy_scene_texture_floats": (i64, [vp]),
         "gy_scene_context": (C.c_int, [vp, P(vp)]),
@@ -571,6 +572,18 @@ class Scene:
             return color, weight
         check(lib().gy_render(self.h, C.byref(cam), C.byref(params), C.POINTER(C.c_float)(), C.POINTER(C.c_float)()))
         return None
+
+    def film_rmse(self, reference, rows=None):
+        """RMSE of the last render's pixels against `reference` (the given
+        image rows, or the whole image) over every channel (gy_film_rmse)."""
+        ref = np.ascontiguousarray(reference, dtype=np.float32)
+        out = C.c_double()
+        if rows is None:
+            check(lib().gy_film_rmse(self.h, _fp(ref), C.POINTER(C.c_int32)(), 0, C.byref(out)))
+        else:
+            r = np.ascontiguousarray(rows, dtype=np.int32)
+            check(lib().gy_film_rmse(self.h, _fp(ref), _ip(r), r.shape[0], C.byref(out)))
+        return out.value
 
     def autotune(self, cam, params, pilot_spp=0):
         """Fastest launch choice for this view (gy_autotune): (params, ms per Mpath)."""

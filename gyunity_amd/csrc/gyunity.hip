@@ -1388,6 +1388,35 @@ extern "C" int gy_film_save(gy_scene* s, const char* path) {
     return WriteImage(path, rgb.data(), sc->film.width, sc->film.height);
 }
 
+extern "C" int gy_film_rmse(gy_scene* s, const float* reference, const int32_t* rows, int32_t n_rows, double* rmse) {
+    if (!s || !reference || !rmse) return Fail(GY_ERR_INVALID, "gy_film_rmse: null argument");
+    Scene* sc = reinterpret_cast<Scene*>(s);
+    if (!sc->film.film) return Fail(GY_ERR_INVALID, "gy_film_rmse: nothing rendered");
+    const int W = sc->film.width, H = sc->film.height;
+    if (!rows) n_rows = H;
+    if (n_rows <= 0) return Fail(GY_ERR_INVALID, "gy_film_rmse: no rows");
+    for (int32_t i = 0; rows && i < n_rows; ++i)
+        if (rows[i] < 0 || rows[i] >= H) return Fail(GY_ERR_INVALID, "gy_film_rmse: row outside the image");
+    std::vector<float4> line(W);
+    double sum = 0.0;
+    for (int32_t i = 0; i < n_rows; ++i) {
+        const int y = rows ? rows[i] : i;
+        hipError_t e = hipMemcpy(line.data(), sc->film.film + (int64_t)y * W, W * sizeof(float4), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return HipFail(e, "film D2H");
+        const float* ref = reference + (int64_t)i * W * 3;
+        for (int x = 0; x < W; ++x) {
+            const float w = line[x].w;
+            const float c[3] = {line[x].x, line[x].y, line[x].z};
+            for (int k = 0; k < 3; ++k) {
+                const double d = (double)(w == 0.f ? 0.f : c[k] / w) - (double)ref[3 * x + k];
+                sum += d * d;
+            }
+        }
+    }
+    *rmse = std::sqrt(sum / ((double)n_rows * W * 3));
+    return GY_OK;
+}
+
 extern "C" int gy_render_stats_get(gy_context* ctx, gy_render_stats* out) {
     if (!ctx || !out) return Fail(GY_ERR_INVALID, "gy_render_stats_get: null argument");
     *out = reinterpret_cast<Ctx*>(ctx)->stats;

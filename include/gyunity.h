@@ -265,6 +265,12 @@ uint64_t gy_halton_enum_index(uint32_t width, uint32_t height, uint64_t i, uint3
 int gy_image_write(const char* path, const float* rgb, int32_t width, int32_t height);
 int gy_image_to_bytes(const float* rgb, int64_t n, uint8_t* out);
 int gy_film_save(gy_scene* scene, const char* path);
+/* RMSE of the last render's pixels (colour / weight, the pixel values
+ * Film::SaveImage writes, visual/Film.cpp:6-9) against a reference image, over
+ * every channel of the given image rows: reference holds n_rows * width * 3
+ * floats, row rows[i] at i; rows NULL: all rows, reference the whole image.
+ * The harness of "Mpaths/s at fixed RMSE". */
+int gy_film_rmse(gy_scene* scene, const float* reference, const int32_t* rows, int32_t n_rows, double* rmse);
 /* LightTracing (integrator/LT.h:11-241): LightTracing(sampler, maxDepth, spp)
  * ::Render with p->spp, p->max_depth and p->sampler.  Light paths from the
  * area lights, every vertex connected to the pinhole camera and splatted

@@ -411,6 +411,8 @@ def test_level_c_icosphere_rows_against_fp64_reference(gpu_ctx):
             prm.mode = mode
             img = _image(*sc.render(cam, prm))
             r[mode] = _rmse(img[rows], ref)
+            # the library's RMSE harness (gy_film_rmse) on the same rows
+            assert abs(sc.film_rmse(ref, rows) - r[mode]) <= 1e-6 * r[mode]
     print("icosphere rows %s: rmse %.6f (stated %.6f)" % (rows, r[g.GY_RENDER_WAVEFRONT], meta["ico_fp32_rmse"]))
     assert r[g.GY_RENDER_MEGAKERNEL] == r[g.GY_RENDER_WAVEFRONT]
     assert abs(r[g.GY_RENDER_WAVEFRONT] - meta["ico_fp32_rmse"]) <= 0.02 * meta["ico_fp32_rmse"]
