@@ -12,6 +12,7 @@ from ._lib import (  # noqa: F401
     GyError,
     RAY_DTYPE,
     HIT_DTYPE,
+    SPHERE_DTYPE,
     camera,
     camera_matrix,
     generate_scene,

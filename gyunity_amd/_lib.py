@@ -114,6 +114,9 @@ class RenderStats(C.Structure):
 
 RAY_DTYPE = np.dtype([("org", "<f4", 3), ("tmin", "<f4"), ("dir", "<f4", 3), ("tmax", "<f4")])
 HIT_DTYPE = np.dtype([("t", "<f4"), ("prim", "<i4"), ("b1", "<f4"), ("b2", "<f4")])
+# gy_sphere: centre, radius, inside flag, material, emissive flag, emitted radiance
+SPHERE_DTYPE = np.dtype([("center", "<f4", 3), ("radius", "<f4"), ("inside", "<i4"), ("material", "<u4"),
+                         ("emissive", "<i4"), ("le", "<f4", 3)])
 
 _lib = None
 _torch_first = False
@@ -194,6 +197,7 @@ def lib():
         "gy_image_read": (C.c_int, [C.c_char_p, P(i32), P(i32), vp, i64]),
         "gy_film_save": (C.c_int, [vp, C.c_char_p]),
         "gy_scene_set_environment": (C.c_int, [vp, vp, C.c_int32, C.c_int32, C.c_float]),
+        "gy_scene_set_spheres": (C.c_int, [vp, C.c_int32, vp]),
         "gy_render_light_tracing": (C.c_int, [vp, P(CameraDesc), P(RenderParams), vp]),
         "gy_mesh_load_obj": (C.c_int, [C.c_char_p, i32, P(f32), P(f32), P(vp)]),
         "gy_mesh_triangles": (i64, [vp]),
@@ -552,6 +556,14 @@ class Scene:
         if rgb.ndim != 3 or rgb.shape[2] != 3:
             raise GyError(1, "set_environment: rgb must be (height, width, 3)")
         check(lib().gy_scene_set_environment(self.h, rgb.ctypes.data, rgb.shape[1], rgb.shape[0], float(scale)))
+
+    def set_spheres(self, spheres):
+        """Analytic spheres after the triangles (gy_scene_set_spheres); spheres
+        is an array of SPHERE_DTYPE.  Emissive ones become area lights after the
+        scene's own."""
+        sp = np.ascontiguousarray(spheres, dtype=SPHERE_DTYPE)
+        check(lib().gy_scene_set_spheres(self.h, sp.shape[0], sp.ctypes.data))
+        self.n_light += int(np.count_nonzero(sp["emissive"]))
 
     def render_light_tracing(self, cam, params):
         """LightTracing::Render (gy_render_light_tracing): the (H, W, 3) splat sums."""

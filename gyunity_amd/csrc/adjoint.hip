@@ -278,26 +278,26 @@ __global__ void __launch_bounds__(BLOCK) k_pt_adjoint(SceneDev S, PathParams P, 
     adjoint_sample<STACK, BLOCK, STORED>(S, P, F, dI, G, s0 + s_local, px, py, stack + threadIdx.x);
 }
 
-// Every sample of every entry of a batch render in one launch, in the batch's
-// lane order (entry, then the entry's sample-major 8x8-tile order, as
-// k_wf_raygen_batch deals them).  A batch entry's pass buffer holds all its
-// samples, so the radiance is always stored.  Entry e's dl/dI starts at
-// dI + E[e].image_off.
+// Every sample of every entry of a batch render in one launch.  Each entry owns
+// whole workgroups (from E[e].adj_block0), so the entry, its camera and film
+// are the same for the whole workgroup and their fields load as scalars; the
+// scene, which every entry shares, is an argument.  Within an entry the lanes
+// follow its sample-major 8x8-tile order, as k_wf_raygen_batch deals them.  A
+// batch entry's pass buffer holds all its samples, so the radiance is always
+// stored.  Entry e's dl/dI starts at dI + E[e].image_off.
 template <int STACK, int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_batch_adjoint(const BatchEntryDev* __restrict__ E, int32_t n, const float* __restrict__ dI,
-                                                         Grad G, int64_t base, int64_t count) {
+__global__ void __launch_bounds__(BLOCK) k_batch_adjoint(SceneDev S, const BatchEntryDev* __restrict__ E, int32_t n,
+                                                         const float* __restrict__ dI, Grad G, int64_t block_base) {
     __shared__ uint32_t stack[STACK * BLOCK];
-    const int64_t t = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (t >= count) return;
-    const int64_t g = base + t;
-    int lo = 0, hi = n - 1;  // last entry with first <= g
+    const int64_t blk = block_base + blockIdx.x;
+    int lo = 0, hi = n - 1;  // last entry with adj_block0 <= blk
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (E[mid].first <= g) lo = mid;
+        if (E[mid].adj_block0 <= blk) lo = mid;
         else hi = mid - 1;
     }
     const BatchEntryDev& B = E[lo];
-    const int64_t r = g - B.first;
+    const int64_t r = (blk - B.adj_block0) * BLOCK + threadIdx.x;
     if (r >= B.paths) return;
     const int64_t per_s = (int64_t)B.tilesX * B.tilesY * 64;
     const int s = (int)(r / per_s);
@@ -307,7 +307,7 @@ __global__ void __launch_bounds__(BLOCK) k_batch_adjoint(const BatchEntryDev* __
     const int px = (int)(tile % B.tilesX) * 8 + (local & 7);
     const int py = (int)(tile / B.tilesX) * 8 + (local >> 3);
     if (px >= B.P.cam.width || py >= B.P.cam.height) return;
-    adjoint_sample<STACK, BLOCK, true>(B.S, B.P, B.F, dI + B.image_off, G, s, px, py, stack + threadIdx.x);
+    adjoint_sample<STACK, BLOCK, true>(S, B.P, B.F, dI + B.image_off, G, s, px, py, stack + threadIdx.x);
 }
 
 // Pixel values colour / weight of the film (Film::WriteToPixelBuffer without
@@ -356,6 +356,7 @@ static int AdjointDevice(Scene* sc, const gy_camera_desc* cam, const gy_render_p
         return Fail(GY_ERR_INVALID, "gy_adjoint: render this camera and these parameters with gy_render first");
     if (sc->tex_floats == 0) return Fail(GY_ERR_INVALID, "gy_adjoint: the scene has no textures");
     if (sc->dev.env.w > 0) return Fail(GY_ERR_UNSUPPORTED, "gy_adjoint: scenes with an environment light");
+    if (sc->dev.n_sph > 0) return Fail(GY_ERR_UNSUPPORTED, "gy_adjoint: scenes with spheres");
     if (sc->stats.max_depth + 1 > 64) return Fail(GY_ERR_UNSUPPORTED, "gy_adjoint: BVH deeper than the traversal stack");
     PathParams P;
     rc = MakePathParams(c, sc, *cam, *p, &P);
@@ -397,21 +398,20 @@ int BatchAdjoint(Ctx* c, Batch* b, const float* dI, float* grad, float* gle) {
     if (sc->tex_floats == 0) return Fail(GY_ERR_INVALID, "gy_batch_adjoint: the scene has no textures");
     if (sc->dev.env.w > 0) return Fail(GY_ERR_UNSUPPORTED, "gy_batch_adjoint: scenes with an environment light");
     if (sc->stats.max_depth + 1 > 64) return Fail(GY_ERR_UNSUPPORTED, "gy_batch_adjoint: BVH deeper than the traversal stack");
-    constexpr int kBlock = 128;
+    constexpr int kBlock = kBatchAdjointBlock;
     const Grad G{grad, gle};
-    // launches of at most 2^30 samples
-    constexpr int64_t kChunk = (int64_t)1 << 30;
+    // launches of at most 2^23 workgroups (2^30 samples)
+    constexpr int64_t kChunk = (int64_t)1 << 23;
     int rc = GY_OK;
     TimerBegin(c, kKernOther);
-    for (int64_t base = 0; base < b->total_paths && rc == GY_OK; base += kChunk) {
-        const int64_t count = std::min(kChunk, b->total_paths - base);
-        const unsigned blocks = (unsigned)((count + kBlock - 1) / kBlock);
+    for (int64_t base = 0; base < b->adj_blocks && rc == GY_OK; base += kChunk) {
+        const unsigned blocks = (unsigned)std::min(kChunk, b->adj_blocks - base);
         if (sc->stack <= 32)
-            hipLaunchKernelGGL((k_batch_adjoint<32, kBlock>), dim3(blocks), dim3(kBlock), 0, c->stream, (const BatchEntryDev*)b->table,
-                               (int32_t)b->entries.size(), dI, G, base, count);
+            hipLaunchKernelGGL((k_batch_adjoint<32, kBlock>), dim3(blocks), dim3(kBlock), 0, c->stream, sc->dev,
+                               (const BatchEntryDev*)b->table, (int32_t)b->entries.size(), dI, G, base);
         else
-            hipLaunchKernelGGL((k_batch_adjoint<64, kBlock>), dim3(blocks), dim3(kBlock), 0, c->stream, (const BatchEntryDev*)b->table,
-                               (int32_t)b->entries.size(), dI, G, base, count);
+            hipLaunchKernelGGL((k_batch_adjoint<64, kBlock>), dim3(blocks), dim3(kBlock), 0, c->stream, sc->dev,
+                               (const BatchEntryDev*)b->table, (int32_t)b->entries.size(), dI, G, base);
         rc = CheckLaunch(c, "k_batch_adjoint");
     }
     TimerEnd(c);

@@ -30,7 +30,7 @@ int BatchBuild(Ctx* c, Batch* b, int64_t lanes_req) {
     if (b->entries.empty()) return Fail(GY_ERR_INVALID, "gy_batch_render: empty batch");
     BatchDropGraph(b);
     std::vector<BatchEntryDev> tab(b->entries.size());
-    int64_t first = 0, image_off = 0;
+    int64_t first = 0, image_off = 0, adj_blocks = 0;
     int depth_max = 0;
     b->rendered = false;
     for (size_t i = 0; i < b->entries.size(); ++i) {
@@ -51,11 +51,13 @@ int BatchBuild(Ctx* c, Batch* b, int64_t lanes_req) {
         d.paths = (int64_t)d.tilesX * d.tilesY * 64 * en.params.spp;
         d.first = first;
         d.image_off = image_off;
+        d.adj_block0 = adj_blocks;
         en.first = first;
         en.paths = d.paths;
         en.image_off = image_off;
         first += d.paths;
         image_off += (int64_t)W * H * 3;
+        adj_blocks += (d.paths + kBatchAdjointBlock - 1) / kBatchAdjointBlock;
         depth_max = std::max(depth_max, en.params.max_depth);
     }
     DevFree(b->table);
@@ -68,6 +70,7 @@ int BatchBuild(Ctx* c, Batch* b, int64_t lanes_req) {
     }
     b->total_paths = first;
     b->image_floats = image_off;
+    b->adj_blocks = adj_blocks;
     b->depth_max = depth_max;
     const int64_t want = lanes_req > 0 ? lanes_req : (int64_t)32 << 20;
     b->lanes = std::min(want, (first + 255) / 256 * 256);
@@ -111,6 +114,7 @@ extern "C" int gy_batch_add(gy_batch* bh, gy_scene* scene, const gy_camera_desc*
     Scene* sc = reinterpret_cast<Scene*>(scene);
     if (sc->ctx != b.ctx) return Fail(GY_ERR_INVALID, "gy_batch_add: scene belongs to another context");
     if (sc->stats.max_depth + 1 > 64) return Fail(GY_ERR_UNSUPPORTED, "gy_batch_add: BVH deeper than the traversal stack");
+    if (sc->dev.n_sph > 0) return Fail(GY_ERR_UNSUPPORTED, "gy_batch_add: scenes with spheres (the batch runs the wavefront kernels)");
     BatchEntry en;
     en.scene = sc;
     en.cam = *cam;

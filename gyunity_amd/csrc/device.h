@@ -243,6 +243,11 @@ struct SceneDev {
     int32_t n_light;
     int32_t n_tri;
     int32_t n_nodes;
+    // analytic spheres (primitive ids n_tri + i): two float4 each, centre and
+    // radius; inside flag, material, light id (-1: none) as integer bits and
+    // the area 4 pi r^2
+    const float4* __restrict__ sph;
+    int32_t n_sph;
 };
 
 struct Hit {
@@ -330,13 +335,106 @@ GY_D bool ref_triangle(V3 p0, V3 p1, V3 p2, V3 o, V3 d, V3 inv, float tmin, floa
     return true;
 }
 
+// CoordinateSystem (math/GeometryUtils.h:11-17)
+GY_D void coordinate_system(V3 v1, V3* v2, V3* v3o) {
+    if (fabsf(v1.x) > fabsf(v1.y)) *v2 = v3(-v1.z, 0.f, v1.x) / sqrtf(v1.x * v1.x + v1.z * v1.z);
+    else *v2 = v3(0.f, v1.z, -v1.y) / sqrtf(v1.y * v1.y + v1.z * v1.z);
+    *v3o = cross(v1, *v2);
+}
+
+// ------------------------------------------------------------------ analytic spheres
+// Sphere (shape/Sphere.h:10-351) with the identity object transform, which is
+// what every reference scene gives its spheres.  The identity applied to a
+// point or vector is exact except that -0 becomes +0.
+GY_D V3 xf_identity(V3 p) { return v3(0.f + p.x, 0.f + p.y, 0.f + p.z); }
+// WorldToObject(ray) (math/Transform.h:226-239: origin error bound, move dt
+// along d, tMax - dt, tMin + dt, the Ray constructor's RayEps shift), then
+// Sphere::Intersect (shape/Sphere.h:29-128): the hit distance and (u, v).  The
+// reference builds the hit record and drops it; only uv survives.  Out of
+// line: the per-path kernels keep their registers.
+__device__ __noinline__ bool sph_hit(float4 s0, V3 o, V3 d, float tmin, float tmax, float* t, float* u, float* v) {
+    const V3 c = v3(s0.x, s0.y, s0.z);
+    const float rad = s0.w;
+    const V3 oErr = gamma_n(3) * v3(fabsf(o.x), fabsf(o.y), fabsf(o.z));
+    V3 ro = xf_identity(o);
+    const V3 rd = xf_identity(d);
+    const float length2 = dot(rd, rd);
+    float tMax = tmax, tMin = tmin;
+    if (length2 > 0.f) {
+        const float dt = dot(vabs(rd), oErr) / length2;
+        ro = ro + rd * dt;
+        tMax = tMax - dt;
+        tMin = tMin + dt;
+    }
+    ro = ro + rd * kRayEps;
+    const V3 po = ro - c;
+    const float vd = dot(po, rd), d2 = dot(rd, rd), po2 = dot(po, po);
+    float det = vd * vd - d2 * po2 + d2 * rad * rad;
+    if (det < 0.f) return false;
+    det = sqrtf(det);
+    float tHit = (-vd - det) / d2;
+    if (tHit < tMin || tHit > tMax) {
+        tHit = (-vd + det) / d2;
+        if (tHit < tMin || tHit > tMax) return false;
+    }
+    V3 hit = ro + rd * tHit;
+    const V3 scaledDir = (hit - c) * rad / length(hit - c);
+    hit = c + scaledDir;
+    const V3 pHit = hit - c;
+    float phi = datan2(pHit.z, pHit.x);
+    const float PI = 3.14159265358979f;
+    if (phi < 0.f) phi += 2 * PI;
+    *u = phi / (2 * PI);
+    const float theta = dacos(clampf(pHit.y / rad, -1.f, 1.f));
+    *v = 1.f - (theta - 0.f) / (PI - 0.f);
+    *t = tHit;
+    return true;
+}
+// The sphere's leaf box in the reference's tree: ObjectToWorld(AABB(c - r, c + r)).
+GY_D void sph_box(float4 s0, V3* mn, V3* mx) {
+    const V3 c = v3(s0.x, s0.y, s0.z), r = v3(s0.w, s0.w, s0.w);
+    *mn = xf_identity(c - r);
+    *mx = xf_identity(c + r);
+}
+// Spheres after the triangles of a closest-hit query: gate at the ray's own
+// tMax, test at the closest distance so far, ties to the larger primitive id.
+__device__ __noinline__ void sph_closest(const SceneDev& S, V3 o, V3 d, V3 inv, float tmin, float tmax0, float* best,
+                                         int32_t* best_prim, float* bb1, float* bb2) {
+    for (int i = 0; i < S.n_sph; ++i) {
+        const float4 s0 = S.sph[2 * i];
+        V3 mn, mx;
+        sph_box(s0, &mn, &mx);
+        if (!ref_box(mn, mx, o, inv, tmin, tmax0)) continue;
+        float th, u, v;
+        const int32_t prim = S.n_tri + i;
+        if (sph_hit(s0, o, d, tmin, *best, &th, &u, &v) && (th < *best || prim > *best_prim)) {
+            *best = th;
+            *best_prim = prim;
+            *bb1 = u;
+            *bb2 = v;
+        }
+    }
+}
+// Any hit against the spheres: Sphere::Intersect(const Ray&) (shape/Sphere.h:
+// 131-161) drops WorldToObject(ray) and tests a default Ray, whose tHit =
+// 0 / 0 passes both range tests, so a sphere occludes every ray that reaches
+// its leaf, i.e. whose segment passes its box.
+__device__ __noinline__ bool sph_any(const SceneDev& S, V3 o, V3 inv, float tmin, float tmax) {
+    for (int i = 0; i < S.n_sph; ++i) {
+        V3 mn, mx;
+        sph_box(S.sph[2 * i], &mn, &mx);
+        if (ref_box(mn, mx, o, inv, tmin, tmax)) return true;
+    }
+    return false;
+}
+
 // Closest-hit / any-hit BVH traversal with a per-thread stack column in LDS
 // (stack[i * stride]).  Node boxes are padded and tested conservatively, so
 // every triangle the reference's traversal would test is tested here; each is
 // gated by ref_triangle.  Closest hit: minimum t, ties to the larger primitive
 // id, independent of visit order.  The stack never exceeds the BVH depth, which
 // the host checks against STACK before launch.
-template <bool ANY, int STACK>
+template <bool ANY, int STACK, bool SPH = false>
 GY_D bool traverse(const SceneDev& S, V3 o, V3 d, float tmin, float tmax, Hit* hit, uint32_t* stack, int stride,
                    uint32_t* n_node = nullptr, uint32_t* n_tri = nullptr) {
     const V3 inv = v3(1.f / d.x, 1.f / d.y, 1.f / d.z);
@@ -412,6 +510,10 @@ GY_D bool traverse(const SceneDev& S, V3 o, V3 d, float tmin, float tmax, Hit* h
         }
         if (sp == 0) break;
         node = stack[--sp * stride];
+    }
+    if (SPH && S.n_sph > 0) {
+        if (ANY) return sph_any(S, o, inv, tmin, tmax);
+        sph_closest(S, o, d, inv, tmin, tmax, &best, &best_prim, &bb1, &bb2);
     }
     if (ANY) return false;
     if (best_prim < 0) return false;
@@ -567,13 +669,17 @@ GY_D int trav_run(const SceneDev& Sc, Trav& t, uint32_t* lds, int stride, uint32
 // STACK, so the ring never spills and no spill buffer is passed.  COUNT: the
 // counting pass keeps the depth-first traverse(), whose visits are the
 // algorithmic bytes' lower bound.
-template <bool ANY, int STACK, bool COUNT = false>
+template <bool ANY, int STACK, bool COUNT = false, bool SPH = false>
 GY_D bool find_hit(const SceneDev& S, V3 o, V3 d, float tmin, float tmax, Hit* hit, uint32_t* stack, int stride,
                    uint32_t* n_node = nullptr, uint32_t* n_tri = nullptr) {
     if (COUNT) return traverse<ANY, STACK>(S, o, d, tmin, tmax, hit, stack, stride, n_node, n_tri);
     Trav t;
     trav_init(t, o, d, tmin, tmax);
     const int r = trav_run<ANY, STACK>(S, t, stack, stride, nullptr, 0, 0);
+    if (SPH && S.n_sph > 0) {
+        if (ANY) return r == 2 || sph_any(S, t.o, t.inv, tmin, tmax);
+        sph_closest(S, t.o, t.d, t.inv, tmin, tmax, &t.best, &t.best_prim, &t.bb1, &t.bb2);
+    }
     if (ANY) return r == 2;
     if (t.best_prim < 0) return false;
     hit->t = t.best;
@@ -593,7 +699,23 @@ struct Isect {
 };
 // Triangle::QueryIntersectionInfo (shape/Triangle.cpp:6-69) + mPos
 // (Triangle.cpp:97).  dpdu/dpdv are not used by the integrator and not formed.
+template <bool SPH = false>
 GY_D Isect query(const SceneDev& S, V3 ro, V3 rd, const Hit& h) {
+    if (SPH && h.prim >= S.n_tri) {
+        // a sphere hit: the default Intersection (zero position, normals,
+        // frame, outgoing direction and error bound) with the hit's uv; the
+        // reference drops the record Sphere::Intersect builds and
+        // Scene::QueryIntersectionInfo adds nothing for spheres
+        const uint4 w = __builtin_bit_cast(uint4, S.sph[2 * (h.prim - S.n_tri) + 1]);
+        Isect it;
+        it.pos = it.n = it.absN = it.ng = it.outDir = it.sdpdu = it.sdpdv = it.pErr = v3(0.f, 0.f, 0.f);
+        it.u = h.b1;
+        it.v = h.b2;
+        it.prim = h.prim;
+        it.mat = w.y;
+        it.light = (int32_t)w.z;
+        return it;
+    }
     const float4* q = S.shade + (size_t)h.prim * 8;
     const float4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3], q4 = q[4], q5 = q[5], q6 = q[6], q7 = q[7];
     const V3 p0 = ld3(q0), p1 = ld3(q1), p2 = ld3(q2);
@@ -1035,8 +1157,90 @@ GY_D int sample_one_light(const SceneDev& S, float u, float* pdf) {
 struct LightPoint {
     V3 pos, n, fn, pErr;
 };
+// The area light of primitive `prim` (-1: none).
+template <bool SPH = false>
+GY_D int32_t prim_light(const SceneDev& S, int32_t prim) {
+    if (SPH && prim >= S.n_tri) return __float_as_int(S.sph[2 * (prim - S.n_tri) + 1].z);
+    return __float_as_int(S.shade[(size_t)prim * 8 + 1].w);
+}
+// Sphere::Sample(real*, const Vec2&) (shape/Sphere.h:163-193): a uniform point
+// with its ObjectToWorld error bound; the absolute normal is the new record's
+// own zero normal.
+__device__ __noinline__ LightPoint sph_sample_area(float4 s0, uint32_t inside, float u0, float u1, float* pdf) {
+    const V3 c = v3(s0.x, s0.y, s0.z);
+    const float rad = s0.w, PI = 3.14159265358979f;
+    *pdf = 1.f / (4.f * PI * rad * rad);
+    const float z = 1 - 2 * u0;  // UniformSampleSphere (visual/Sampling.cpp:25-30)
+    const float rr = sqrtf(rmax(0.f, 1.f - z * z));
+    const float phi = 2 * PI * u1;
+    const V3 pHit = v3(rr * dcos(phi), rr * dsin(phi), z) * rad + c;
+    const V3 scaledDir = (pHit - c) * rad / length(pHit - c);
+    const V3 hit = c + scaledDir;
+    const V3 n = inside ? norm(c - hit) : norm(hit - c);
+    const V3 pError = vabs(c) * gamma_n(1) + vabs(scaledDir) * gamma_n(6);
+    const float g3 = gamma_n(3);
+    LightPoint lp;
+    lp.pErr = v3((g3 + 1.f) * ((pError.x + 0.f * pError.y) + 0.f * pError.z) + g3 * (((fabsf(hit.x) + 0.f) + 0.f) + 0.f),
+                 (g3 + 1.f) * ((0.f * pError.x + pError.y) + 0.f * pError.z) + g3 * (((0.f + fabsf(hit.y)) + 0.f) + 0.f),
+                 (g3 + 1.f) * ((0.f * pError.x + 0.f * pError.y) + pError.z) + g3 * (((0.f + 0.f) + fabsf(hit.z)) + 0.f));
+    lp.pos = xf_identity(hit);
+    lp.n = norm(xf_identity(n));
+    lp.fn = v3(0.f, 0.f, 0.f);
+    return lp;
+}
+// Sphere::Sample(const Intersection&, real*, const Vec2&) (shape/Sphere.h:243-291):
+// inside the sphere an area sample whose pdf is multiplied by the distance
+// (not its square) over the cosine; outside, a direction in the cone the
+// sphere subtends, the point 1.5e-3 r beyond the surface (r * 1.5e-3 formed in
+// double, a double literal), pdf 1 / cone solid angle.
+__device__ __noinline__ LightPoint sph_sample_li(const SceneDev& S, int i, V3 ref, float u0, float u1, float* pdf) {
+    const float4 s0 = S.sph[2 * i];
+    const uint4 w = __builtin_bit_cast(uint4, S.sph[2 * i + 1]);
+    const V3 pCenter = xf_identity(v3(s0.x, s0.y, s0.z));
+    const float rad = s0.w, PI = 3.14159265358979f;
+    if (dot(pCenter - ref, pCenter - ref) <= rad * rad) {
+        LightPoint lp = sph_sample_area(s0, w.x, u0, u1, pdf);
+        V3 wi = lp.pos - ref;
+        if (dot(wi, wi) == 0.f) {
+            *pdf = 0.f;
+        } else {
+            const float sd = length(wi);
+            wi = norm(wi);
+            *pdf = *pdf * (sd / fabsf(dot(norm(lp.pos - pCenter), wi * -1.f)));
+        }
+        if (isinf(*pdf)) *pdf = 0.f;
+        return lp;
+    }
+    V3 localZ = pCenter - ref;
+    const float dis = length(localZ);
+    localZ = norm(localZ);
+    V3 localX, localY;
+    coordinate_system(localZ, &localX, &localY);
+    const float sin2ThetaMax = rad * rad / (dis * dis);
+    const float cosThetaMax = sqrtf(rmax(1.f - sin2ThetaMax, 0.f));
+    const float cosTheta = (1.f - u0) + u0 * cosThetaMax;
+    const float sinTheta = sqrtf(rmax(1.f - cosTheta * cosTheta, 0.f));
+    const float phi = 2 * PI * u1;
+    const float sdist = dis * cosTheta - sqrtf(rmax(rad * rad - dis * dis * sinTheta * sinTheta, 0.f));
+    const float cosAlpha = (dis * dis + rad * rad - sdist * sdist) / (2 * dis * rad);
+    const float sinAlpha = sqrtf(rmax(1.f - cosAlpha * cosAlpha, 0.f));
+    const float cphi = dcos(phi), sphi = dsin(phi);
+    V3 nWorld = ((-1.f * sinAlpha * cphi) * localX - (sinAlpha * sphi) * localY) - cosAlpha * localZ;
+    nWorld = norm(nWorld);
+    const float off = (float)((double)rad * 1.5e-3);
+    const V3 pWorld = (pCenter + (rad * nWorld) / length(nWorld)) + off * norm(nWorld);
+    *pdf = 1 / (2 * PI * (1 - cosThetaMax));
+    LightPoint lp;
+    lp.pos = pWorld;
+    lp.n = nWorld;
+    lp.fn = nWorld;
+    lp.pErr = gamma_n(5) * vabs(pWorld);
+    return lp;
+}
 // Triangle::Sample (shape/Triangle.cpp:141-158) + Shape::Sample (shape/Shape.cpp:6-19)
+template <bool SPH = false>
 GY_D LightPoint sample_light_point(const SceneDev& S, int tri, V3 ref, float u0, float u1, float* pdf) {
+    if (SPH && tri >= S.n_tri) return sph_sample_li(S, tri - S.n_tri, ref, u0, u1, pdf);
     const float4* q = S.shade + (size_t)tri * 8;
     const V3 p0 = ld3(q[0]), p1 = ld3(q[1]), p2 = ld3(q[2]);
     const float area = q[2].w;
@@ -1062,7 +1266,23 @@ GY_D LightPoint sample_light_point(const SceneDev& S, int tri, V3 ref, float u0,
     return lp;
 }
 // Shape::Pdf (shape/Shape.h:34-47): intersect only the light's own triangle.
+// Shape::Pdf (shape/Shape.h:34-47) for a sphere light: the sphere's own hit
+// record is the default one (zero position and normal), so the pdf is
+// |d|^2 / 0: infinite, which returns 0, or NaN when the point is the origin.
+__device__ __noinline__ float sph_light_pdf(const SceneDev& S, int i, const Isect& it, V3 wi) {
+    V3 o;
+    spawn_ray(it, wi, &o);
+    const float4 s0 = S.sph[2 * i];
+    float th, u, v;
+    if (!sph_hit(s0, o, wi, 0.f, kInf, &th, &u, &v)) return 0.f;
+    const V3 d = v3(0.f, 0.f, 0.f) - it.pos;
+    float pdf = dot(d, d) / (fabsf(dot(v3(0.f, 0.f, 0.f), wi * -1.f)) * S.sph[2 * i + 1].w);
+    if (isinf(pdf)) pdf = 0.f;
+    return pdf;
+}
+template <bool SPH = false>
 GY_D float light_pdf(const SceneDev& S, int tri, const Isect& it, V3 wi) {
+    if (SPH && tri >= S.n_tri) return sph_light_pdf(S, tri - S.n_tri, it, wi);
     V3 o;
     spawn_ray(it, wi, &o);
     const float4* q = S.shade + (size_t)tri * 8;

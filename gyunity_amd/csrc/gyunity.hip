@@ -135,7 +135,9 @@ void TimersCollect(Ctx* c) {
 }
 
 // ------------------------------------------------------------------ ray query kernels
-template <int STACK>
+// SPH: the scene has analytic spheres.  Each kernel is also built without the
+// sphere code, which keeps the triangle-only instantiations' registers.
+template <int STACK, bool SPH>
 __global__ void __launch_bounds__(256) k_trace_closest(SceneDev S, const gy_ray* __restrict__ rays, int64_t n,
                                                        gy_hit* __restrict__ hits) {
     __shared__ uint32_t stack[STACK * 256];
@@ -145,7 +147,7 @@ __global__ void __launch_bounds__(256) k_trace_closest(SceneDev S, const gy_ray*
     const float4 a = rp[0], b = rp[1];
     Hit h;
     gy_hit out;
-    if (traverse<false, STACK>(S, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, b.w, &h, stack + threadIdx.x, blockDim.x)) {
+    if (traverse<false, STACK, SPH>(S, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, b.w, &h, stack + threadIdx.x, blockDim.x)) {
         out.t = h.t;
         out.prim = h.prim;
         out.b1 = h.b1;
@@ -159,7 +161,7 @@ __global__ void __launch_bounds__(256) k_trace_closest(SceneDev S, const gy_ray*
     *reinterpret_cast<float4*>(hits + i) = make_float4(out.t, __int_as_float(out.prim), out.b1, out.b2);
 }
 
-template <int STACK>
+template <int STACK, bool SPH>
 __global__ void __launch_bounds__(256) k_trace_any(SceneDev S, const gy_ray* __restrict__ rays, int64_t n,
                                                    int32_t* __restrict__ occ) {
     __shared__ uint32_t stack[STACK * 256];
@@ -168,7 +170,7 @@ __global__ void __launch_bounds__(256) k_trace_any(SceneDev S, const gy_ray* __r
     const float4* rp = reinterpret_cast<const float4*>(rays + i);
     const float4 a = rp[0], b = rp[1];
     Hit h;
-    occ[i] = traverse<true, STACK>(S, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, b.w, &h, stack + threadIdx.x, blockDim.x) ? 1 : 0;
+    occ[i] = traverse<true, STACK, SPH>(S, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, b.w, &h, stack + threadIdx.x, blockDim.x) ? 1 : 0;
 }
 
 __device__ inline void wave_add_u64(unsigned long long* dst, uint64_t v) {
@@ -177,7 +179,7 @@ __device__ inline void wave_add_u64(unsigned long long* dst, uint64_t v) {
     if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst, (unsigned long long)v);
 }
 
-template <int STACK>
+template <int STACK, bool SPH>
 __global__ void __launch_bounds__(256) k_count_visits(SceneDev S, const gy_ray* __restrict__ rays, int64_t n,
                                                       unsigned long long* counters) {
     __shared__ uint32_t stack[STACK * 256];
@@ -187,13 +189,13 @@ __global__ void __launch_bounds__(256) k_count_visits(SceneDev S, const gy_ray* 
         const float4* rp = reinterpret_cast<const float4*>(rays + i);
         const float4 a = rp[0], b = rp[1];
         Hit h;
-        traverse<false, STACK>(S, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, b.w, &h, stack + threadIdx.x, blockDim.x, &nn, &nt);
+        traverse<false, STACK, SPH>(S, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), a.w, b.w, &h, stack + threadIdx.x, blockDim.x, &nn, &nt);
     }
     wave_add_u64(&counters[0], nn);
     wave_add_u64(&counters[1], nt);
 }
 
-template <int STACK, int BLOCK, bool ENV>
+template <int STACK, int BLOCK, bool ENV, bool SPH>
 __global__ void __launch_bounds__(BLOCK) k_pt_megakernel(SceneDev S, PathParams P, FilmDev F, int s0, int n_s,
                                                          unsigned long long* counters) {
     __shared__ uint32_t stack[STACK * BLOCK];
@@ -203,7 +205,7 @@ __global__ void __launch_bounds__(BLOCK) k_pt_megakernel(SceneDev S, PathParams 
     PathCounters pc = {};
     if (s_local < n_s && px < F.width && py < F.height) {
         float fx, fy;
-        const V3 L = trace_path<STACK, false, ENV>(S, P, px, py, s0 + s_local, stack + threadIdx.x, BLOCK, &fx, &fy, nullptr, pc);
+        const V3 L = trace_path<STACK, false, ENV, SPH>(S, P, px, py, s0 + s_local, stack + threadIdx.x, BLOCK, &fx, &fy, nullptr, pc);
         record_sample(F, s_local, px, py, fx, fy, L);
     }
     wave_add_u64(&counters[0], pc.rays);
@@ -212,7 +214,7 @@ __global__ void __launch_bounds__(BLOCK) k_pt_megakernel(SceneDev S, PathParams 
 }
 
 // Instrumented copy: counts node and triangle visits and shading points.
-template <int STACK, int BLOCK, bool ENV>
+template <int STACK, int BLOCK, bool ENV, bool SPH>
 __global__ void __launch_bounds__(BLOCK) k_pt_count(SceneDev S, PathParams P, FilmDev F, int s0, int n_s,
                                                     unsigned long long* counters) {
     __shared__ uint32_t stack[STACK * BLOCK];
@@ -222,7 +224,7 @@ __global__ void __launch_bounds__(BLOCK) k_pt_count(SceneDev S, PathParams P, Fi
     PathCounters pc = {};
     if (s_local < n_s && px < F.width && py < F.height) {
         float fx, fy;
-        (void)trace_path<STACK, true, ENV>(S, P, px, py, s0 + s_local, stack + threadIdx.x, BLOCK, &fx, &fy, nullptr, pc);
+        (void)trace_path<STACK, true, ENV, SPH>(S, P, px, py, s0 + s_local, stack + threadIdx.x, BLOCK, &fx, &fy, nullptr, pc);
     }
     wave_add_u64(&counters[0], pc.rays);
     wave_add_u64(&counters[1], pc.shadow);
@@ -234,7 +236,7 @@ __global__ void __launch_bounds__(BLOCK) k_pt_count(SceneDev S, PathParams P, Fi
     }
 }
 
-template <int STACK, bool ENV>
+template <int STACK, bool ENV, bool SPH>
 __global__ void __launch_bounds__(128) k_li_samples(SceneDev S, PathParams P, int n, const int32_t* __restrict__ xs,
                                                     const int32_t* __restrict__ ys, const int32_t* __restrict__ ss,
                                                     float* __restrict__ L, float* __restrict__ raster, int32_t* __restrict__ dims) {
@@ -244,7 +246,7 @@ __global__ void __launch_bounds__(128) k_li_samples(SceneDev S, PathParams P, in
     PathCounters pc = {};
     float fx, fy;
     uint32_t nd;
-    const V3 v = trace_path<STACK, false, ENV>(S, P, xs[i], ys[i], ss[i], stack + threadIdx.x, 128, &fx, &fy, &nd, pc);
+    const V3 v = trace_path<STACK, false, ENV, SPH>(S, P, xs[i], ys[i], ss[i], stack + threadIdx.x, 128, &fx, &fy, &nd, pc);
     L[3 * i] = v.x;
     L[3 * i + 1] = v.y;
     L[3 * i + 2] = v.z;
@@ -315,24 +317,30 @@ __global__ void k_film_accumulate(FilmDev F, int n_s) {
 }
 
 // ------------------------------------------------------------------ launch helpers
-template <int STACK>
+template <int STACK, bool SPH>
 int LaunchTraceClosestT(Ctx* c, const SceneDev& S, const gy_ray* rays, int64_t n, gy_hit* hits) {
-    hipLaunchKernelGGL(k_trace_closest<STACK>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, S, rays, n, hits);
+    hipLaunchKernelGGL((k_trace_closest<STACK, SPH>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, S, rays, n, hits);
     return CheckLaunch(c, "k_trace_closest");
 }
-template <int STACK>
+template <int STACK, bool SPH>
 int LaunchTraceAnyT(Ctx* c, const SceneDev& S, const gy_ray* rays, int64_t n, int32_t* occ) {
-    hipLaunchKernelGGL(k_trace_any<STACK>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, S, rays, n, occ);
+    hipLaunchKernelGGL((k_trace_any<STACK, SPH>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, S, rays, n, occ);
     return CheckLaunch(c, "k_trace_any");
 }
 
 int LaunchTraceClosest(Ctx* c, Scene* sc, const gy_ray* rays, int64_t n, gy_hit* hits) {
     if (n <= 0) return GY_OK;
-    return sc->stack <= 32 ? LaunchTraceClosestT<32>(c, sc->dev, rays, n, hits) : LaunchTraceClosestT<64>(c, sc->dev, rays, n, hits);
+    if (sc->dev.n_sph > 0)
+        return sc->stack <= 32 ? LaunchTraceClosestT<32, true>(c, sc->dev, rays, n, hits)
+                               : LaunchTraceClosestT<64, true>(c, sc->dev, rays, n, hits);
+    return sc->stack <= 32 ? LaunchTraceClosestT<32, false>(c, sc->dev, rays, n, hits)
+                           : LaunchTraceClosestT<64, false>(c, sc->dev, rays, n, hits);
 }
 int LaunchTraceAny(Ctx* c, Scene* sc, const gy_ray* rays, int64_t n, int32_t* occ) {
     if (n <= 0) return GY_OK;
-    return sc->stack <= 32 ? LaunchTraceAnyT<32>(c, sc->dev, rays, n, occ) : LaunchTraceAnyT<64>(c, sc->dev, rays, n, occ);
+    if (sc->dev.n_sph > 0)
+        return sc->stack <= 32 ? LaunchTraceAnyT<32, true>(c, sc->dev, rays, n, occ) : LaunchTraceAnyT<64, true>(c, sc->dev, rays, n, occ);
+    return sc->stack <= 32 ? LaunchTraceAnyT<32, false>(c, sc->dev, rays, n, occ) : LaunchTraceAnyT<64, false>(c, sc->dev, rays, n, occ);
 }
 
 // ------------------------------------------------------------------ scene upload
@@ -590,7 +598,7 @@ fail:
 void SceneFree(Scene* sc) {
     if (!sc) return;
     void* bufs[] = {sc->d_tri, sc->d_shade, sc->d_nodes, sc->d_mats, sc->d_tex, sc->d_texinfo, sc->d_ltri, sc->d_lle, sc->d_lcdf, sc->d_lfunc,
-                    sc->d_env};
+                    sc->d_env, sc->d_sph};
     for (void* b : bufs) DevFree(b);
     FilmFree(&sc->film);
     WavefrontFree(sc);
@@ -734,7 +742,7 @@ int ValidateRender(const gy_camera_desc* cam, const gy_render_params* p) {
     return GY_OK;
 }
 
-template <int STACK, int BLOCK>
+template <int STACK, int BLOCK, bool SPH>
 int LaunchMegaT(Ctx* c, Scene* sc, const FilmState& film, const PathParams& P, int s0, int n_s, bool count, bool timed) {
     FilmDev F = FilmViewState(film);
     const int tilesX = (F.width + 7) >> 3, tilesY = (F.height + 7) >> 3;
@@ -743,31 +751,35 @@ int LaunchMegaT(Ctx* c, Scene* sc, const FilmState& film, const PathParams& P, i
     unsigned long long* ctr = film.counters;
     if (count) {
         if (sc->dev.env.w > 0)
-            hipLaunchKernelGGL((k_pt_count<STACK, BLOCK, true>), dim3(blocks), dim3(BLOCK), 0, c->stream, sc->dev, P, F, s0, n_s, ctr);
+            hipLaunchKernelGGL((k_pt_count<STACK, BLOCK, true, SPH>), dim3(blocks), dim3(BLOCK), 0, c->stream, sc->dev, P, F, s0, n_s, ctr);
         else
-            hipLaunchKernelGGL((k_pt_count<STACK, BLOCK, false>), dim3(blocks), dim3(BLOCK), 0, c->stream, sc->dev, P, F, s0, n_s, ctr);
+            hipLaunchKernelGGL((k_pt_count<STACK, BLOCK, false, SPH>), dim3(blocks), dim3(BLOCK), 0, c->stream, sc->dev, P, F, s0, n_s, ctr);
         return CheckLaunch(c, "k_pt_count");
     }
     if (timed) TimerBegin(c, kKernMega);
     if (sc->dev.env.w > 0)
-        hipLaunchKernelGGL((k_pt_megakernel<STACK, BLOCK, true>), dim3(blocks), dim3(BLOCK), 0, c->stream, sc->dev, P, F, s0, n_s, ctr);
+        hipLaunchKernelGGL((k_pt_megakernel<STACK, BLOCK, true, SPH>), dim3(blocks), dim3(BLOCK), 0, c->stream, sc->dev, P, F, s0, n_s, ctr);
     else
-        hipLaunchKernelGGL((k_pt_megakernel<STACK, BLOCK, false>), dim3(blocks), dim3(BLOCK), 0, c->stream, sc->dev, P, F, s0, n_s, ctr);
+        hipLaunchKernelGGL((k_pt_megakernel<STACK, BLOCK, false, SPH>), dim3(blocks), dim3(BLOCK), 0, c->stream, sc->dev, P, F, s0, n_s, ctr);
     if (timed) TimerEnd(c);
     return CheckLaunch(c, "k_pt_megakernel");
 }
 
 // The megakernel over one pass into `film` (the scene's own film, or the
-// no-alloc entry point's).
+// no-alloc entry point's).  Scenes with spheres run the sphere instantiation
+// at 128 threads a block.
 int LaunchMegaFilm(Ctx* c, Scene* sc, const FilmState& film, const PathParams& P, int s0, int n_s, int block, bool count,
                    bool timed) {
+    if (sc->dev.n_sph > 0)
+        return sc->stack <= 32 ? LaunchMegaT<32, 128, true>(c, sc, film, P, s0, n_s, count, timed)
+                               : LaunchMegaT<64, 128, true>(c, sc, film, P, s0, n_s, count, timed);
     if (sc->stack <= 32) {
-        if (block == 64) return LaunchMegaT<32, 64>(c, sc, film, P, s0, n_s, count, timed);
-        if (block == 256) return LaunchMegaT<32, 256>(c, sc, film, P, s0, n_s, count, timed);
-        return LaunchMegaT<32, 128>(c, sc, film, P, s0, n_s, count, timed);
+        if (block == 64) return LaunchMegaT<32, 64, false>(c, sc, film, P, s0, n_s, count, timed);
+        if (block == 256) return LaunchMegaT<32, 256, false>(c, sc, film, P, s0, n_s, count, timed);
+        return LaunchMegaT<32, 128, false>(c, sc, film, P, s0, n_s, count, timed);
     }
-    if (block == 64) return LaunchMegaT<64, 64>(c, sc, film, P, s0, n_s, count, timed);
-    return LaunchMegaT<64, 128>(c, sc, film, P, s0, n_s, count, timed);
+    if (block == 64) return LaunchMegaT<64, 64, false>(c, sc, film, P, s0, n_s, count, timed);
+    return LaunchMegaT<64, 128, false>(c, sc, film, P, s0, n_s, count, timed);
 }
 
 int LaunchMega(Ctx* c, Scene* sc, const PathParams& P, int s0, int n_s, int block, bool count) {
@@ -938,6 +950,65 @@ extern "C" int gy_scene_create(gy_context* ctx, const gy_scene_desc* desc, const
 // ConvertImageToPdf + Distribution2D (light/EnvMap.h:82-94, visual/Sampling.h:102-119),
 // the bounding-sphere radius (visual/Scene.cpp:9-11) and the light power
 // distribution with EnvironmentLight::Power (light/EnvLight.cpp:60-63), in FP32.
+// Sphere(ObjectToWorld = WorldToObject = identity, radius, position, inside)
+// primitives (shape/Sphere.h:21-27), after the triangles in primitive order;
+// an emissive one also gets an AreaLight (light/AreaLight.h) after the
+// existing area lights, with power Le * 4 pi r^2 * pi (light/AreaLight.h:45-48).
+extern "C" int gy_scene_set_spheres(gy_scene* s, int32_t n, const gy_sphere* spheres) {
+    if (!s || n < 0 || (n > 0 && !spheres)) return Fail(GY_ERR_INVALID, "gy_scene_set_spheres: bad argument");
+    Scene* sc = reinterpret_cast<Scene*>(s);
+    if (sc->dev.n_sph > 0) return Fail(GY_ERR_INVALID, "gy_scene_set_spheres: the scene already has spheres");
+    if (sc->dev.env.w > 0) return Fail(GY_ERR_UNSUPPORTED, "gy_scene_set_spheres: add spheres before the environment light");
+    if (n > 65536) return Fail(GY_ERR_UNSUPPORTED, "gy_scene_set_spheres: at most 65536 spheres (each ray tests every sphere)");
+    if (n == 0) return GY_OK;
+    if ((int64_t)sc->n_tri + n > gyd::kTravMaxTriangles) return Fail(GY_ERR_UNSUPPORTED, "gy_scene_set_spheres: too many primitives");
+    Ctx* c = sc->ctx;
+    (void)hipSetDevice(c->device);
+    const float PI = 3.14159265358979f;
+    std::vector<float4> rec(2 * (size_t)n);
+    std::vector<int32_t> ltri = sc->area_tri;
+    std::vector<float> lle = sc->area_le, func = sc->area_func, cdf;
+    for (int32_t i = 0; i < n; ++i) {
+        const gy_sphere& q = spheres[i];
+        if (!(q.radius > 0.f)) return Fail(GY_ERR_INVALID, "gy_scene_set_spheres: radius must be positive");
+        const float area = ((4.f * PI) * q.radius) * q.radius;
+        int32_t light = -1;
+        if (q.emissive) {
+            light = (int32_t)ltri.size();
+            ltri.push_back(sc->n_tri + i);
+            lle.insert(lle.end(), {q.le[0], q.le[1], q.le[2]});
+            const float P[3] = {(q.le[0] * area) * PI, (q.le[1] * area) * PI, (q.le[2] * area) * PI};
+            func.push_back(0.212671f * P[0] + 0.715160f * P[1] + 0.072169f * P[2]);
+        }
+        rec[2 * i] = make_float4(q.center[0], q.center[1], q.center[2], q.radius);
+        rec[2 * i + 1] = make_float4(__builtin_bit_cast(float, (uint32_t)(q.inside != 0)), __builtin_bit_cast(float, q.material),
+                                     __builtin_bit_cast(float, light), area);
+        for (int k = 0; k < 3; ++k) {
+            sc->world_lo[k] = std::min(sc->world_lo[k], q.center[k] - q.radius);
+            sc->world_hi[k] = std::max(sc->world_hi[k], q.center[k] + q.radius);
+        }
+    }
+    void* d = nullptr;
+    int rc = Upload(c, &d, rec.data(), rec.size() * sizeof(float4));
+    if (rc != GY_OK) return rc;
+    const float func_int = LightCdf(func, &cdf);
+    rc = UploadLights(c, sc, ltri, lle, func, cdf);
+    if (rc != GY_OK) {
+        DevFree(d);
+        return rc;
+    }
+    sc->area_tri = ltri;
+    sc->area_le = lle;
+    sc->area_func = func;
+    sc->dev.light_func_int = func_int;
+    sc->d_sph = d;
+    sc->dev.sph = (const float4*)d;
+    sc->dev.n_sph = n;
+    ++sc->revision;
+    sc->film_valid = false;
+    return GY_OK;
+}
+
 extern "C" int gy_scene_set_environment(gy_scene* s, const float* rgb, int32_t width, int32_t height, float scale) {
     if (!s || !rgb || width <= 0 || height <= 0) return Fail(GY_ERR_INVALID, "gy_scene_set_environment: need an image and a positive size");
     if ((int64_t)width * height > (1ll << 26)) return Fail(GY_ERR_UNSUPPORTED, "gy_scene_set_environment: image larger than 2^26 texels");
@@ -1043,6 +1114,7 @@ extern "C" int gy_render_light_tracing(gy_scene* s, const gy_camera_desc* cam, c
     Ctx* c = sc->ctx;
     (void)hipSetDevice(c->device);
     if (sc->area_tri.empty() && sc->dev.env.w <= 0) return Fail(GY_ERR_INVALID, "gy_render_light_tracing: the scene has no light");
+    if (sc->dev.n_sph > 0) return Fail(GY_ERR_UNSUPPORTED, "gy_render_light_tracing: scenes with spheres");
     const int64_t n = (int64_t)cam->width * cam->height * p->spp;
     if (n >= (1ll << 36)) return Fail(GY_ERR_UNSUPPORTED, "gy_render_light_tracing: more than 2^36 light paths");
     if (sc->stats.max_depth + 1 > 64) return Fail(GY_ERR_UNSUPPORTED, "gy_render_light_tracing: BVH deeper than the traversal stack");
@@ -1207,12 +1279,11 @@ extern "C" int gy_count_visits(gy_scene* s, const gy_ray* rays, int64_t n, int64
     rc = Fill32(c, c->ctr.ptr, 0u, 4);
     if (rc != GY_OK) return rc;
     if (n > 0) {
-        if (sc->stack <= 32)
-            hipLaunchKernelGGL(k_count_visits<32>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, sc->dev,
-                               (const gy_ray*)c->rays.ptr, n, (unsigned long long*)c->ctr.ptr);
-        else
-            hipLaunchKernelGGL(k_count_visits<64>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, sc->dev,
-                               (const gy_ray*)c->rays.ptr, n, (unsigned long long*)c->ctr.ptr);
+        const bool sph = sc->dev.n_sph > 0;
+        auto kern = sc->stack <= 32 ? (sph ? k_count_visits<32, true> : k_count_visits<32, false>)
+                                    : (sph ? k_count_visits<64, true> : k_count_visits<64, false>);
+        hipLaunchKernelGGL(kern, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, sc->dev,
+                           (const gy_ray*)c->rays.ptr, n, (unsigned long long*)c->ctr.ptr);
         rc = CheckLaunch(c, "k_count_visits");
         if (rc != GY_OK) return rc;
     }
@@ -1245,9 +1316,11 @@ extern "C" int gy_render(gy_scene* s, const gy_camera_desc* cam, const gy_render
     (void)hipEventCreate(&t1);
     (void)hipEventRecord(t0, c->stream);
     int block = p->block_size > 0 ? p->block_size : 128;
+    // scenes with spheres render with the per-path kernels (same film)
+    const bool wavefront = p->mode == GY_RENDER_WAVEFRONT && sc->dev.n_sph == 0;
     for (int s0 = 0; s0 < p->spp && rc == GY_OK; s0 += n_pass) {
         const int n_s = std::min(n_pass, p->spp - s0);
-        if (p->mode == GY_RENDER_WAVEFRONT) rc = WavefrontPass(c, sc, P, *p, s0, n_s);
+        if (wavefront) rc = WavefrontPass(c, sc, P, *p, s0, n_s);
         else rc = LaunchMega(c, sc, P, s0, n_s, block, false);
         if (rc == GY_OK) rc = FilmResolvePass(c, sc, n_s);
     }
@@ -1454,8 +1527,11 @@ extern "C" int gy_li_samples(gy_scene* s, const gy_camera_desc* cam, const gy_re
     if (rc == GY_OK) {
         const unsigned blocks = (unsigned)((n + 127) / 128);
         const bool env = sc->dev.env.w > 0;
-        auto kern = sc->stack <= 32 ? (env ? k_li_samples<32, true> : k_li_samples<32, false>)
-                                    : (env ? k_li_samples<64, true> : k_li_samples<64, false>);
+        auto kern = sc->stack <= 32 ? (env ? k_li_samples<32, true, false> : k_li_samples<32, false, false>)
+                                    : (env ? k_li_samples<64, true, false> : k_li_samples<64, false, false>);
+        if (sc->dev.n_sph > 0)
+            kern = sc->stack <= 32 ? (env ? k_li_samples<32, true, true> : k_li_samples<32, false, true>)
+                                   : (env ? k_li_samples<64, true, true> : k_li_samples<64, false, true>);
         hipLaunchKernelGGL(kern, dim3(blocks), dim3(128), 0, c->stream, sc->dev, P, n, (const int32_t*)dx, (const int32_t*)dy,
                            (const int32_t*)ds, (float*)dL, (float*)dr, (int32_t*)dd);
         rc = CheckLaunch(c, "k_li_samples");

@@ -27,7 +27,7 @@ struct DIParts {
     int light;   // the light chosen (SampleOneLight)
     V3 F1, F2;   // L1 and L2 per unit emitted radiance (the adjoint's emission terms)
 };
-template <int STACK, bool COUNT, bool ENV = false>
+template <int STACK, bool COUNT, bool ENV = false, bool SPH = false>
 __device__ DIParts di_parts(const SceneDev& S, const Isect& it, const BSDF& b, float uL, float u0, float u1, float v0,
                             float v1, float v2, uint32_t* stack, int stride, PathCounters& pc) {
     DIParts r;
@@ -48,7 +48,7 @@ __device__ DIParts di_parts(const SceneDev& S, const Isect& it, const BSDF& b, f
         if (ENV && ltri < 0) {
             lp = env_light_point_call(S.env, it.pos, u0, u1, &wi, &Li, &pdf);
         } else {
-            lp = sample_light_point(S, ltri, it.pos, u0, u1, &pdf);
+            lp = sample_light_point<SPH>(S, ltri, it.pos, u0, u1, &pdf);
             wi = norm(lp.pos - it.pos);
             Li = dot(lp.n, neg(wi)) > 0.f ? Le : v3(0.f, 0.f, 0.f);
         }
@@ -65,7 +65,7 @@ __device__ DIParts di_parts(const SceneDev& S, const Isect& it, const BSDF& b, f
                 uint32_t* nn = COUNT ? &pc.nodes[1] : nullptr;
                 uint32_t* nt = COUNT ? &pc.tris[1] : nullptr;
                 pc.shadow++;
-                const bool occluded = find_hit<true, STACK, COUNT>(S, so, sd, 0.f, stmax, &hs, stack, stride, nn, nt);
+                const bool occluded = find_hit<true, STACK, COUNT, SPH>(S, so, sd, 0.f, stmax, &hs, stack, stride, nn, nt);
                 if (occluded) Li = v3s(0.f);
                 r.w1 = power_heuristic(pdf, sp);
                 r.L1 = Li * v3s(1.f) * f * c / pdf;
@@ -80,7 +80,7 @@ __device__ DIParts di_parts(const SceneDev& S, const Isect& it, const BSDF& b, f
         r.wb = wi;
         const float c = fabsf(dot(it.n, wi));
         if (pdf != 0.f && !is_zero(f)) {
-            const float lpdf = (ENV && ltri < 0) ? env_pdf_dir_call(S.env, wi) : light_pdf(S, ltri, it, wi);
+            const float lpdf = (ENV && ltri < 0) ? env_pdf_dir_call(S.env, wi) : light_pdf<SPH>(S, ltri, it, wi);
             if (lpdf == 0.f) {
                 r.w2 = 0.f;
             } else {
@@ -91,10 +91,10 @@ __device__ DIParts di_parts(const SceneDev& S, const Isect& it, const BSDF& b, f
                 uint32_t* nn = COUNT ? &pc.nodes[2] : nullptr;
                 uint32_t* nt = COUNT ? &pc.tris[2] : nullptr;
                 pc.mis++;
-                if (find_hit<false, STACK, COUNT>(S, ro, wi, 0.f, kInf, &h2, stack, stride, nn, nt)) {
-                    const int hl = __float_as_int(S.shade[(size_t)h2.prim * 8 + 1].w);
+                if (find_hit<false, STACK, COUNT, SPH>(S, ro, wi, 0.f, kInf, &h2, stack, stride, nn, nt)) {
+                    const int hl = prim_light<SPH>(S, h2.prim);
                     if (hl == light) {
-                        const Isect li = query(S, ro, wi, h2);
+                        const Isect li = query<SPH>(S, ro, wi, h2);
                         const bool facing = dot(li.n, neg(wi)) > 0.f;
                         const V3 Lem = facing ? Le : v3(0.f, 0.f, 0.f);
                         r.L2 = Lem * f * v3s(1.f) * c / pdf;
@@ -109,14 +109,14 @@ __device__ DIParts di_parts(const SceneDev& S, const Isect& it, const BSDF& b, f
     return r;
 }
 
-template <int STACK, bool COUNT, bool ENV = false>
+template <int STACK, bool COUNT, bool ENV = false, bool SPH = false>
 __device__ V3 direct_illumination(const SceneDev& S, const Isect& it, const BSDF& b, float uL, float u0, float u1,
                                   float v0, float v1, float v2, uint32_t* stack, int stride, PathCounters& pc) {
-    const DIParts d = di_parts<STACK, COUNT, ENV>(S, it, b, uL, u0, u1, v0, v1, v2, stack, stride, pc);
+    const DIParts d = di_parts<STACK, COUNT, ENV, SPH>(S, it, b, uL, u0, u1, v0, v1, v2, stack, stride, pc);
     return (d.L1 * d.w1 + d.L2 * d.w2) / d.lsp;
 }
 
-template <int STACK, bool COUNT, bool ENV = false>
+template <int STACK, bool COUNT, bool ENV = false, bool SPH = false>
 __device__ V3 trace_path(const SceneDev& S, const PathParams& P, int px, int py, int s, uint32_t* stack, int stride,
                          float* fx, float* fy, uint32_t* dims, PathCounters& pc) {
     const SampleSrc& M = P.smp;
@@ -136,12 +136,12 @@ __device__ V3 trace_path(const SceneDev& S, const PathParams& P, int px, int py,
         pc.rays++;
         uint32_t* nn = COUNT ? &pc.nodes[0] : nullptr;
         uint32_t* nt = COUNT ? &pc.tris[0] : nullptr;
-        if (!find_hit<false, STACK, COUNT>(S, o, d, 0.f, kInf, &h, stack, stride, nn, nt)) {
+        if (!find_hit<false, STACK, COUNT, SPH>(S, o, d, 0.f, kInf, &h, stack, stride, nn, nt)) {
             // integrator/PT.cpp:18-21: T * Le(ray) in place of L
             if (ENV && (i == 0 || delta) && S.env.w > 0) L = T * env_emission_call(S.env, d);
             break;
         }
-        const Isect it = query(S, o, d, h);
+        const Isect it = query<SPH>(S, o, d, h);
         const BSDF b = make_bsdf(S, it);
         pc.shade++;
         if ((i == 0 || delta) && it.light >= 0) {
@@ -152,7 +152,7 @@ __device__ V3 trace_path(const SceneDev& S, const PathParams& P, int px, int py,
             const float uL = seq_next(M, q);
             const float u0 = seq_next(M, q), u1 = seq_next(M, q);
             const float v0 = seq_next(M, q), v1 = seq_next(M, q), v2 = seq_next(M, q);
-            L = L + T * direct_illumination<STACK, COUNT, ENV>(S, it, b, uL, u0, u1, v0, v1, v2, stack, stride, pc);
+            L = L + T * direct_illumination<STACK, COUNT, ENV, SPH>(S, it, b, uL, u0, u1, v0, v1, v2, stack, stride, pc);
         }
         const float r0 = seq_next(M, q), r1 = seq_next(M, q), r2 = seq_next(M, q);
         V3 wi;
@@ -207,12 +207,6 @@ __device__ inline V3 xf_point(const float* m, V3 p) {
     V3 q = v3(r[0], r[1], r[2]);
     if (!(r[3] == 1.f || r[3] == 0.f)) q = q / r[3];
     return q;
-}
-// CoordinateSystem (math/GeometryUtils.h:11-17)
-__device__ inline void coordinate_system(V3 v1, V3* v2, V3* v3o) {
-    if (fabsf(v1.x) > fabsf(v1.y)) *v2 = v3(-v1.z, 0.f, v1.x) / sqrtf(v1.x * v1.x + v1.z * v1.z);
-    else *v2 = v3(0.f, v1.z, -v1.y) / sqrtf(v1.y * v1.y + v1.z * v1.z);
-    *v3o = cross(v1, *v2);
 }
 // ConnectToCamera (integrator/LT.h:135-183) + WorldToScreen (:116-129) +
 // Film::AddSplat (visual/Film.h:89-103) of one vertex.  s0: the light vertex,

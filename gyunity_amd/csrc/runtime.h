@@ -77,6 +77,7 @@ struct Scene {
     void *d_tri = nullptr, *d_shade = nullptr, *d_nodes = nullptr, *d_mats = nullptr, *d_tex = nullptr, *d_texinfo = nullptr;
     void *d_ltri = nullptr, *d_lle = nullptr, *d_lcdf = nullptr, *d_lfunc = nullptr;
     void* d_env = nullptr;  // environment light: texels and sampling tables in one block
+    void* d_sph = nullptr;  // analytic spheres (gy_scene_set_spheres)
     uint64_t revision = 0;  // bumped when device buffers the kernels point at are replaced
     float world_lo[3] = {0, 0, 0}, world_hi[3] = {0, 0, 0};
     std::vector<int32_t> area_tri;  // area lights as created (the environment light follows them)
@@ -135,8 +136,10 @@ struct BatchEntryDev {
     int64_t first;  // first global path index of the entry
     int64_t paths;  // tile-padded paths: tilesX * tilesY * 64 * spp
     int64_t image_off;  // the entry's first float in the batch's concatenated images (W * H * 3 per entry)
+    int64_t adj_block0; // the entry's first workgroup in the batch adjoint launch (kBatchAdjointBlock lanes each)
     int32_t tilesX, tilesY;
 };
+constexpr int kBatchAdjointBlock = 128;
 struct BatchEntry {
     Scene* scene = nullptr;
     gy_camera_desc cam = {};
@@ -162,6 +165,7 @@ struct Batch {
     bool dirty = true;        // entries changed since the table / graph were built
     bool rendered = false;    // the films hold a render of the current table
     int64_t image_floats = 0; // floats of all entries' images, concatenated
+    int64_t adj_blocks = 0;   // workgroups of the batch adjoint launch
     hipGraphExec_t graph_exec = nullptr;
 };
 int BatchRender(Ctx* c, Batch* b);

@@ -153,6 +153,25 @@ int gy_scene_destroy(gy_scene* scene);
  * created with no area light and take its light from here.  Calling it again
  * replaces the map.  Not supported by gy_adjoint. */
 int gy_scene_set_environment(gy_scene* scene, const float* rgb, int32_t width, int32_t height, float scale);
+/* Analytic spheres: Sphere(Transform(), Transform(), radius, position, inside)
+ * (shape/Sphere.h:21-27) with a material, and for emissive ones an AreaLight
+ * (light/AreaLight.h:12-85) appended after the scene's area lights.  Primitive
+ * ids follow the triangles' (n_tri + i).  Once per scene, before any
+ * environment light.  The reference's sphere is restated with its defects:
+ * a closest hit leaves the default (zero) intersection record but its uv, so
+ * a path ends there; any-hit queries are occluded by every sphere whose box
+ * the segment passes; a sphere light's Pdf_Li is 0.  Scenes with spheres
+ * render with the per-path kernels (gy_render's wavefront form runs the
+ * megakernel; the batch, light tracing and the adjoint refuse them). */
+typedef struct {
+    float center[3];
+    float radius;
+    int32_t inside;     /* normals point inwards (GetNorm, shape/Sphere.h:338-342) */
+    uint32_t material;
+    int32_t emissive;   /* 1: an AreaLight with emitted radiance le */
+    float le[3];
+} gy_sphere;
+int gy_scene_set_spheres(gy_scene* scene, int32_t n, const gy_sphere* spheres);
 int gy_scene_stats_get(gy_scene* scene, gy_scene_stats* out);
 /* The context a scene was created on. */
 int gy_scene_context(gy_scene* scene, gy_context** ctx);

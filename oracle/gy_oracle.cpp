@@ -591,7 +591,24 @@ template <class R> struct Scene {
     bool detached = false;
     std::vector<Texture<R>> tex_eval;
     std::vector<V3<R>> light_le_eval;
+    // Analytic spheres (or_scene_set_spheres): primitive ids after the
+    // triangles, n_tri + i.
+    struct Sph {
+        V3<R> c;
+        R r = 0, area = 0;
+        bool inside = false;
+        uint32_t mat = 0;
+        int32_t light = -1;
+        AABB<R> box;  // ObjectToWorld(AABB(c - r, c + r)) with the identity transform
+    };
+    std::vector<Sph> sph;
 };
+template <class R> inline uint32_t prim_mat(const Scene<R>& s, int32_t prim) {
+    return prim < (int32_t)s.tris.size() ? s.tris[prim].mat : s.sph[prim - s.tris.size()].mat;
+}
+template <class R> inline int32_t prim_light(const Scene<R>& s, int32_t prim) {
+    return prim < (int32_t)s.tris.size() ? s.tris[prim].light : s.sph[prim - s.tris.size()].light;
+}
 
 // Triangle(..., normal) / Triangle(..., n0, n1, n2) + Initialize
 // (shape/Triangle.h:13-55, 107-118); Area (Triangle.h:67-69).
@@ -794,6 +811,58 @@ template <class R> struct Hit {
 // BVHAccel::Intersect closest (BVHAccel.cpp:401-445) with Primitive::Intersect
 // shrinking tMax (visual/Primitive.cpp:5-18); brute force in scene order
 // (accelerator/BruteForceAccel.h:20-29).
+// ---------------------------------------------------------------- analytic spheres
+// Sphere (shape/Sphere.h:10-351) with the identity object transform, which is
+// what every reference scene gives its spheres (Transform()).  The identity
+// applied to a point or vector (math/Transform.h:106-115, 167-172; SSE column
+// order ((c3 w + c2 z) + c1 y) + c0 x) is exact except that -0 becomes +0.
+template <class R> inline V3<R> xf_identity(const V3<R>& p) { return V3<R>(R(0) + p.x, R(0) + p.y, R(0) + p.z); }
+// WorldToObject(ray) (math/Transform.h:226-239): the origin with its gamma(3)
+// error bound, moved dt along d, tMax - dt, tMin + dt, through the Ray
+// constructor, which moves the origin RayEps along d again.
+template <class R> inline Ray<R> sphere_object_ray(const Ray<R>& ray) {
+    V3<R> oErr = gamma_n<R>(3) * V3<R>(std::abs(ray.o.x), std::abs(ray.o.y), std::abs(ray.o.z));
+    V3<R> o = xf_identity(ray.o), d = xf_identity(ray.d);
+    R length2 = dot(d, d), tMax = ray.tmax, tMin = ray.tmin;
+    if (length2 > 0) {
+        R dt = dot(vabs(d), oErr) / length2;
+        o = o + d * dt;
+        tMax = tMax - dt;
+        tMin = tMin + dt;
+    }
+    return make_ray(o, d, tMax, tMin);
+}
+// Sphere::Intersect(const Ray&, Intersection*, real*) (shape/Sphere.h:29-128):
+// the hit distance and (u, v).  Its intersection record is built and then
+// dropped (`ObjectToWorld(it)` is evaluated and discarded, and *isect never
+// receives it), so a hit leaves only uv, the shape and the primitive behind;
+// sphere_isect restates that.
+template <class R> bool sphere_hit(const typename Scene<R>::Sph& sp, const Ray<R>& ray, R* t, R* u, R* v) {
+    const Ray<R> r = sphere_object_ray(ray);
+    V3<R> po = r.o - sp.c;
+    R vd = dot(po, r.d), d2 = dot(r.d, r.d), po2 = dot(po, po);
+    R det = vd * vd - d2 * po2 + d2 * sp.r * sp.r;
+    if (det < 0) return false;
+    det = std::sqrt(det);
+    R tHit = (-vd - det) / d2;
+    if (tHit < r.tmin || tHit > r.tmax) {
+        tHit = (-vd + det) / d2;
+        if (tHit < r.tmin || tHit > r.tmax) return false;
+    }
+    V3<R> hit = r.o + r.d * tHit;
+    V3<R> scaledDir = (hit - sp.c) * sp.r / length(hit - sp.c);
+    hit = sp.c + scaledDir;
+    V3<R> pHit = hit - sp.c;
+    R phi = ratan2(pHit.z, pHit.x);
+    if (phi < 0) phi += 2 * K<R>::PI;
+    const R phiMax = 2 * K<R>::PI;
+    *u = phi / phiMax;
+    R theta = racos(clampr(pHit.y / sp.r, (R)-1, (R)1));
+    R vv = (theta - 0) / (K<R>::PI - 0);
+    *v = (R)1.f - vv;
+    *t = tHit;
+    return true;
+}
 template <class R> bool intersect(const Scene<R>& s, Ray<R>& ray, Hit<R>* h, bool use_bvh, int64_t* cnt_node = nullptr,
                                   int64_t* cnt_tri = nullptr) {
     bool hit = false;
@@ -806,9 +875,31 @@ template <class R> bool intersect(const Scene<R>& s, Ray<R>& ray, Hit<R>* h, boo
                 hit = true;
             }
         }
+        for (size_t i = 0; i < s.sph.size(); ++i) {
+            R t, u, v;
+            if (sphere_hit<R>(s.sph[i], ray, &t, &u, &v)) {
+                ray.tmax = t;
+                h->t = t; h->b1 = u; h->b2 = v; h->prim = (int32_t)(s.tris.size() + i);
+                hit = true;
+            }
+        }
         return hit;
     }
-    if (s.nodes.empty()) return false;
+    // spheres (few; one leaf each in the reference's tree): the leaf box
+    // gate (BVHAccel::IntersectP) at the current tMax, then the sphere
+    if (!s.sph.empty()) {
+        V3<R> inv(1.f / ray.d.x, 1.f / ray.d.y, 1.f / ray.d.z);
+        uint32_t ng[3] = {inv.x < 0, inv.y < 0, inv.z < 0};
+        for (size_t i = 0; i < s.sph.size(); ++i) {
+            R t, u, v;
+            if (box_hit(s.sph[i].box, ray, inv, ng) && sphere_hit<R>(s.sph[i], ray, &t, &u, &v)) {
+                ray.tmax = t;
+                h->t = t; h->b1 = u; h->b2 = v; h->prim = (int32_t)(s.tris.size() + i);
+                hit = true;
+            }
+        }
+    }
+    if (s.nodes.empty()) return hit;
     V3<R> invDir(1.f / ray.d.x, 1.f / ray.d.y, 1.f / ray.d.z);
     uint32_t neg[3] = {invDir.x < 0, invDir.y < 0, invDir.z < 0};
     uint32_t todoOffset = 0, nodeNum = 0;
@@ -850,6 +941,19 @@ template <class R> bool intersect(const Scene<R>& s, Ray<R>& ray, Hit<R>* h, boo
 // BVHAccel::Intersect any-hit (BVHAccel.cpp:448-487)
 template <class R> bool intersect_any(const Scene<R>& s, const Ray<R>& ray, bool use_bvh) {
     R t, b1, b2;
+    // Sphere::Intersect(const Ray&) (shape/Sphere.h:131-161) drops
+    // WorldToObject(ray) and tests a default Ray (zero origin and direction):
+    // tHit = 0 / 0 passes both range tests, so any sphere the traversal
+    // reaches occludes the ray.  The reference's tree reaches a sphere's leaf
+    // when the ray passes its box (BVHAccel::IntersectP); brute force reaches
+    // every sphere.
+    if (!s.sph.empty()) {
+        if (!use_bvh) return true;
+        V3<R> inv(1.f / ray.d.x, 1.f / ray.d.y, 1.f / ray.d.z);
+        uint32_t ng[3] = {inv.x < 0, inv.y < 0, inv.z < 0};
+        for (const auto& sp : s.sph)
+            if (box_hit(sp.box, ray, inv, ng)) return true;
+    }
     if (!use_bvh) {
         for (size_t i = 0; i < s.tris.size(); ++i)
             if (tri_hit(s.tris[i], ray, &t, &b1, &b2)) return true;
@@ -895,7 +999,21 @@ template <class R> struct Isect {
 // Triangle::QueryIntersectionInfo (shape/Triangle.cpp:6-69); mPos from
 // Triangle::Intersect (Triangle.cpp:97).  dpdu/dpdv do not reach the
 // integrator and are not formed.
+// The intersection PathTracing::Li sees after a sphere hit: the default
+// Intersection (zero position, normals, frame, outgoing direction and error
+// bound) with the hit's uv; Scene::QueryIntersectionInfo adds nothing
+// (Shape::QueryIntersectionInfo is empty for spheres, shape/Shape.h:54).
+template <class R> Isect<R> sphere_isect(const Hit<R>& h) {
+    Isect<R> it;
+    it.pos = it.n = it.absN = it.ng = it.outDir = it.sdpdu = it.sdpdv = it.pErr = V3<R>(0, 0, 0);
+    it.u = h.b1;
+    it.v = h.b2;
+    it.prim = h.prim;
+    return it;
+}
+
 template <class R> Isect<R> query(const Scene<R>& s, const Ray<R>& ray, const Hit<R>& h) {
+    if (h.prim >= (int32_t)s.tris.size()) return sphere_isect(h);
     const Tri<R>& t = s.tris[h.prim];
     Isect<R> it;
     R b1 = h.b1, b2 = h.b2;
@@ -1314,7 +1432,7 @@ template <class R> BSDF<R> eval_bsdf(const Scene<R>& s, const Isect<R>& it, cons
     return s.detached ? make_bsdf_tex(s, it, s.tex_eval) : sampled;
 }
 template <class R> BSDF<R> make_bsdf_tex(const Scene<R>& s, const Isect<R>& it, const std::vector<Texture<R>>& texs) {
-    const Material& m = s.mats[s.tris[it.prim].mat];
+    const Material& m = s.mats[prim_mat(s, it.prim)];
     BSDF<R> b;
     b.type = (int)m.type;
     b.ng = it.ng;
@@ -1360,7 +1478,8 @@ template <class R> void build_light_distribution(Scene<R>& s) {
         if (s.light_tri[i] < 0)  // EnvironmentLight::Power (light/EnvLight.cpp:60-63)
             P = env_radiance(s.env, (R)0.5f, (R)0.5f) * K<R>::PI * s.env.radius * s.env.radius;
         else
-            P = s.light_le[i] * s.tris[s.light_tri[i]].area * K<R>::PI;
+            P = s.light_le[i] * (s.light_tri[i] < (int32_t)s.tris.size() ? s.tris[s.light_tri[i]].area
+                                                                           : s.sph[s.light_tri[i] - s.tris.size()].area) * K<R>::PI;
         s.light_func[i] = lumY(P);
     }
     s.light_cdf.assign(n + 1, 0);
@@ -1404,7 +1523,94 @@ template <class R> LightPoint<R> sample_light_point(const Tri<R>& t, const Isect
     return lp;
 }
 // Shape::Pdf (shape/Shape.h:34-47): intersect only the light's triangle.
+// Sphere::Sample(real*, const Vec2&) (shape/Sphere.h:163-193): a uniform
+// point with its ObjectToWorld error bound; the absolute normal is copied
+// from the new record's own zero normal.
+template <class R> LightPoint<R> sphere_sample_area(const typename Scene<R>::Sph& sp, R u0, R u1, R* pdf) {
+    *pdf = 1.f / (4.f * K<R>::PI * sp.r * sp.r);
+    // UniformSampleSphere (visual/Sampling.cpp:25-30)
+    R z = 1 - 2 * u0;
+    R rr = std::sqrt(rmax((R)0, (R)1 - z * z));
+    R phi = 2 * K<R>::PI * u1;
+    V3<R> pHit = V3<R>(rr * rcos(phi), rr * rsin(phi), z) * sp.r + sp.c;
+    V3<R> scaledDir = (pHit - sp.c) * sp.r / length(pHit - sp.c);
+    V3<R> hit = sp.c + scaledDir;
+    V3<R> n = sp.inside ? norm(sp.c - hit) : norm(hit - sp.c);  // GetNorm (Sphere.h:338-342)
+    V3<R> pError = vabs(sp.c) * gamma_n<R>(1) + vabs(scaledDir) * gamma_n<R>(6);
+    LightPoint<R> lp;
+    // ObjectToWorld(hit, pError, &err) with the identity (math/Transform.h:137-164)
+    const R g3 = gamma_n<R>(3);
+    const V3<R> h = hit;
+    lp.pErr = V3<R>((g3 + (R)1) * ((pError.x + 0 * pError.y) + 0 * pError.z) + g3 * (((std::abs(h.x) + 0) + 0) + 0),
+                    (g3 + (R)1) * ((0 * pError.x + pError.y) + 0 * pError.z) + g3 * (((0 + std::abs(h.y)) + 0) + 0),
+                    (g3 + (R)1) * ((0 * pError.x + 0 * pError.y) + pError.z) + g3 * (((0 + 0) + std::abs(h.z)) + 0));
+    lp.pos = xf_identity(hit);
+    lp.n = norm(xf_identity(n));  // TransformNormal(n).Norm()
+    lp.fn = V3<R>(0, 0, 0);       // mAbsNormal = the new record's zero normal
+    return lp;
+}
+// Sphere::Sample(const Intersection&, real*, const Vec2&) (shape/Sphere.h:243-291):
+// inside the sphere, an area sample whose pdf is multiplied by the distance
+// (not its square) over the cosine; outside, a direction in the cone the
+// sphere subtends, the point 1.5e-3 r beyond the surface, pdf 1 / cone solid angle.
+template <class R> LightPoint<R> sphere_sample_li(const typename Scene<R>::Sph& sp, const Isect<R>& it, R u0, R u1, R* pdf) {
+    V3<R> pCenter = xf_identity(sp.c);
+    if (dot(pCenter - it.pos, pCenter - it.pos) <= sp.r * sp.r) {
+        LightPoint<R> lp = sphere_sample_area<R>(sp, u0, u1, pdf);
+        V3<R> wi = lp.pos - it.pos;
+        if (dot(wi, wi) == 0) {
+            *pdf = 0;
+        } else {
+            R sd = length(wi);
+            wi = norm(wi);
+            *pdf = *pdf * (sd / std::abs(dot(norm(lp.pos - pCenter), wi * (R)-1)));
+        }
+        if (std::isinf(*pdf)) *pdf = 0.f;
+        return lp;
+    }
+    V3<R> localZ = pCenter - it.pos;
+    R dis = length(localZ);
+    localZ = norm(localZ);
+    V3<R> localX, localY;
+    coordinate_system(localZ, &localX, &localY);
+    R sin2ThetaMax = sp.r * sp.r / (dis * dis);
+    R cosThetaMax = std::sqrt(rmax((R)1 - sin2ThetaMax, (R)0));
+    R cosTheta = ((R)1 - u0) + u0 * cosThetaMax;
+    R sinTheta = std::sqrt(rmax((R)1 - cosTheta * cosTheta, (R)0));
+    R phi = 2 * K<R>::PI * u1;
+    R sdist = dis * cosTheta - std::sqrt(rmax(sp.r * sp.r - dis * dis * sinTheta * sinTheta, (R)0));
+    R cosAlpha = (dis * dis + sp.r * sp.r - sdist * sdist) / (2 * dis * sp.r);
+    R sinAlpha = std::sqrt(rmax((R)1 - cosAlpha * cosAlpha, (R)0));
+    R cphi = rcos(phi), sphi = rsin(phi);
+    V3<R> nWorld = ((R)-1 * sinAlpha * cphi) * localX - (sinAlpha * sphi) * localY - cosAlpha * localZ;
+    nWorld = norm(nWorld);
+    // 1.5e-3 is a double literal: r * 1.5e-3 is formed in double and rounded
+    const R off = (R)((double)sp.r * 1.5e-3);
+    V3<R> pWorld = pCenter + (sp.r * nWorld) / length(nWorld) + off * norm(nWorld);
+    *pdf = 1 / (2 * K<R>::PI * (1 - cosThetaMax));
+    LightPoint<R> lp;
+    lp.pos = pWorld;
+    lp.n = nWorld;
+    lp.fn = nWorld;
+    lp.pErr = gamma_n<R>(5) * vabs(pWorld);
+    return lp;
+}
+// Shape::Pdf (shape/Shape.h:34-47) for a sphere light: the sphere's own hit
+// record is the default one (zero position and normal), so the pdf is
+// |d|^2 / 0: infinite, which returns 0; NaN when the shading point is at the
+// origin (d = 0).
+template <class R> R sphere_light_pdf(const typename Scene<R>::Sph& sp, const Isect<R>& it, const V3<R>& wi) {
+    Ray<R> ray = spawn_ray(it, wi);
+    R t, u, v;
+    if (!sphere_hit<R>(sp, ray, &t, &u, &v)) return 0;
+    V3<R> d = V3<R>(0, 0, 0) - it.pos;
+    R pdf = dot(d, d) / (std::abs(dot(V3<R>(0, 0, 0), wi * (R)-1)) * sp.area);
+    if (std::isinf(pdf)) return 0;
+    return pdf;
+}
+
 template <class R> R light_pdf(const Scene<R>& s, int tri, const Isect<R>& it, const V3<R>& wi) {
+    if (tri >= (int)s.tris.size()) return sphere_light_pdf<R>(s.sph[tri - s.tris.size()], it, wi);
     Ray<R> ray = spawn_ray(it, wi);
     const Tri<R>& t = s.tris[tri];
     Hit<R> h;
@@ -1568,7 +1774,8 @@ template <class R> V3<R> direct_illumination(const Scene<R>& s, const Isect<R>& 
             lp.n = lp.fn = wi;
             lp.pErr = V3<R>(0, 0, 0);
         } else {
-            lp = sample_light_point(s.tris[ltri], it, u0, u1, &pdf);
+            lp = ltri < (int)s.tris.size() ? sample_light_point(s.tris[ltri], it, u0, u1, &pdf)
+                                           : sphere_sample_li<R>(s.sph[ltri - s.tris.size()], it, u0, u1, &pdf);
             wi = norm(lp.pos - it.pos);
             Li = dot(lp.n, -wi) > 0 ? Le : V3<R>(0, 0, 0);
         }
@@ -1600,7 +1807,7 @@ template <class R> V3<R> direct_illumination(const Scene<R>& s, const Isect<R>& 
                 Ray<R> ray = spawn_ray(it, wi);
                 Hit<R> h;
                 if (intersect(s, ray, &h, use_bvh)) {
-                    if (s.tris[h.prim].light == light) {
+                    if (prim_light(s, h.prim) == light) {
                         Isect<R> li = query(s, ray, h);
                         V3<R> Lemit = dot(li.n, -wi) > 0 ? Le : V3<R>(0, 0, 0);
                         V3<R> Tr(1.f);
@@ -1641,7 +1848,7 @@ template <class R> V3<R> Li(const Scene<R>& s, Ray<R> ray, Seq& rnd, int maxDept
             r[8] = hit ? h.prim : -1; r[9] = hit ? h.t : 0;
             r[10] = rnd.cursor;
             r[11] = T.x; r[12] = T.y; r[13] = T.z;
-            r[14] = hit ? s.tris[h.prim].mat : -1;
+            r[14] = hit ? prim_mat(s, h.prim) : -1;
             r[15] = delta ? 1 : 0;
         }
         if (!hit) {
@@ -1652,7 +1859,7 @@ template <class R> V3<R> Li(const Scene<R>& s, Ray<R> ray, Seq& rnd, int maxDept
         }
         Isect<R> it = query(s, ray, h);
         BSDF<R> bsdf = make_bsdf(s, it);
-        int light = s.tris[it.prim].light;
+        int light = prim_light(s, it.prim);
         if ((i == 0 || delta) && light >= 0) {
             V3<R> Le = s.detached ? s.light_le_eval[light] : s.light_le[light];
             V3<R> e = dot(it.n, it.outDir) > 0 ? Le : V3<R>(0, 0, 0);
@@ -2084,9 +2291,53 @@ template <class R> void set_detached(Scene<R>& s, const or_scene_desc& d, const 
         for (size_t i = 0; i < s.light_le_eval.size(); ++i) s.light_le_eval[i] = V3<R>(le[3 * i], le[3 * i + 1], le[3 * i + 2]);
 }
 
+// Spheres appended to a scene: 9 floats each (center xyz, radius, inside,
+// material, emissive, Le rgb, as gy_sphere); an emissive sphere's AreaLight
+// joins the light list after the existing area lights.
+template <class R> void add_spheres(Scene<R>& s, int32_t n, const or_sphere* sp) {
+    for (int32_t i = 0; i < n; ++i) {
+        typename Scene<R>::Sph q;
+        q.c = V3<R>(sp[i].center[0], sp[i].center[1], sp[i].center[2]);
+        q.r = sp[i].radius;
+        q.inside = sp[i].inside != 0;
+        q.mat = sp[i].material;
+        const V3<R> rv(q.r, q.r, q.r);
+        // AABB(c - r, c + r), ObjectToWorld of its eight corners
+        q.box.mn = xf_identity(q.c - rv);
+        q.box.mx = xf_identity(q.c + rv);
+        q.area = 4.f * K<R>::PI * q.r * q.r;
+        if (sp[i].emissive) {
+            q.light = (int32_t)s.light_tri.size();
+            s.light_tri.push_back((int32_t)(s.tris.size() + s.sph.size()));
+            s.light_le.push_back(V3<R>(sp[i].le[0], sp[i].le[1], sp[i].le[2]));
+        }
+        s.sph.push_back(q);
+    }
+    build_light_distribution(s);
+}
+
 }  // namespace orc
 
 using namespace orc;
+
+// Test hook: sphere i's Sample(Intersection, pdf, u) and Pdf_Li from a
+// shading point at ref (zero error bound, normal along wi for the spawned ray).
+template <class R> static int sphere_light_probe(Scene<R>& s, int32_t i, const double* ref, const double* u,
+                                                 const double* wi, double* out) {
+    if (i < 0 || i >= (int32_t)s.sph.size()) return -1;
+    Isect<R> it{};
+    it.pos = V3<R>((R)ref[0], (R)ref[1], (R)ref[2]);
+    it.pErr = V3<R>(0, 0, 0);
+    R pdf;
+    const LightPoint<R> lp = sphere_sample_li<R>(s.sph[i], it, (R)u[0], (R)u[1], &pdf);
+    out[0] = lp.pos.x; out[1] = lp.pos.y; out[2] = lp.pos.z;
+    out[3] = lp.n.x; out[4] = lp.n.y; out[5] = lp.n.z;
+    out[6] = pdf;
+    const V3<R> w((R)wi[0], (R)wi[1], (R)wi[2]);
+    it.absN = w;
+    out[7] = sphere_light_pdf<R>(s.sph[i], it, w);
+    return 0;
+}
 
 extern "C" {
 
@@ -2306,6 +2557,21 @@ int or_faure_perm(uint32_t base, uint16_t* out) {
     for (uint32_t i = 0; i < base; ++i) out[i] = T.perm[base][i];
     return 0;
 }
+int or_scene_set_spheres(void* hv, int32_t n, const or_sphere* spheres) {
+    SceneHandle* h = (SceneHandle*)hv;
+    if (n < 0 || (n > 0 && !spheres) || !h->env.empty()) return -1;
+    add_spheres(h->sf, n, spheres);
+    add_spheres(scene_of<double>(h), n, spheres);
+    return 0;
+}
+
+int or_sphere_light_probe(void* hv, int32_t i, int precision, const double* ref, const double* u, const double* wi,
+                          double* out) {
+    SceneHandle* h = (SceneHandle*)hv;
+    return precision ? sphere_light_probe(scene_of<double>(h), i, ref, u, wi, out)
+                     : sphere_light_probe(h->sf, i, ref, u, wi, out);
+}
+
 int or_set_detached(void* hv, const float* tex_eval, const float* le_eval) {
     SceneHandle* h = (SceneHandle*)hv;
     set_detached(h->sf, h->desc, tex_eval, le_eval);
@@ -2324,6 +2590,7 @@ int or_render_lt(void* hv, const or_camera_desc* cd, const or_render_params* p, 
     SceneHandle* h = (SceneHandle*)hv;
     const Scene<float>& sc = h->sf;
     if (sc.light_tri.empty()) return -1;
+    if (!sc.sph.empty()) return -2;  // light tracing of sphere scenes is not restated
     LTCamera<float> cam = make_lt_camera<float>(*cd);
     const int W = cd->width, H = cd->height;
     const int64_t total = (int64_t)p->spp * W * H;

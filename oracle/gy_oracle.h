@@ -111,7 +111,24 @@ int or_camera_matrix(const or_camera_desc* cam, float* raster_to_world16);
 int or_camera_rays(const or_camera_desc* cam, int32_t n, const int32_t* px, const int32_t* py,
                    const float* u, const float* v, float* org, float* dir);
 
+/* An analytic sphere (shape/Sphere.h:10-351, identity transform), as gy_sphere. */
+typedef struct {
+    float center[3];
+    float radius;
+    int32_t inside;    /* normals point inwards */
+    uint32_t material;
+    int32_t emissive;  /* an AreaLight on the sphere, after the existing lights */
+    float le[3];
+} or_sphere;
+
 void* or_scene_create(const or_scene_desc* desc);
+/* Appends spheres (primitive ids after the triangles); before any environment light. */
+int or_scene_set_spheres(void* scene, int32_t n, const or_sphere* spheres);
+/* Test hook: from a shading point at ref (zero error bound), sphere i's light
+ * sample at u = (u0, u1) and Pdf_Li towards wi; out = pos[3], n[3], pdf, pdf_li.
+ * precision 0: FP32, 1: FP64. */
+int or_sphere_light_probe(void* scene, int32_t i, int precision, const double* ref, const double* u, const double* wi,
+                          double* out);
 void or_scene_destroy(void* scene);
 /* EnvironmentLight appended after the area lights: rgb is the lat-long image,
  * top row first, 3 floats a texel (as stbi_loadf returns it). */

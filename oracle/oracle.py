@@ -104,6 +104,8 @@ def lib():
         "or_set_shadow_short": (C.c_int, [C.c_int]),
         "or_set_detached": (C.c_int, [vp, vp, vp]),
         "or_scene_set_environment": (C.c_int, [vp, vp, i32, i32, f32]),
+        "or_scene_set_spheres": (C.c_int, [vp, i32, vp]),
+        "or_sphere_light_probe": (C.c_int, [vp, i32, C.c_int, vp, vp, vp, vp]),
         "or_env_query": (C.c_int, [vp, i32, vp, vp, vp, vp, vp, vp, C.c_int]),
         "or_env_info": (C.c_int, [vp, vp, vp, i32]),
         "or_render_lt": (C.c_int, [vp, vp, vp, vp]),
@@ -246,6 +248,26 @@ class OracleScene:
         rows = np.zeros((max_rows, 16), np.float64)
         n = lib().or_li_path(self.h, C.byref(c), C.byref(prm), int(x), int(y), int(s), max_rows, _p(rows))
         return rows[:n]
+
+    def set_spheres(self, spheres):
+        """Sphere primitives after the triangles (shape/Sphere.h:21-27), emissive
+        ones with an AreaLight after the existing lights; spheres is a
+        structured array with gy_sphere's fields (or_sphere has the same layout)."""
+        self._sph = np.ascontiguousarray(spheres)
+        assert self._sph.dtype.itemsize == 40
+        rc = lib().or_scene_set_spheres(self.h, self._sph.shape[0], _p(self._sph))
+        assert rc == 0
+
+    def sphere_light_probe(self, i, ref, u, wi, precision=1):
+        """Sphere i's Sample(Intersection, pdf, u) from a shading point at ref
+        and its Pdf_Li towards wi: (pos, n, pdf, pdf_li)."""
+        ref = np.ascontiguousarray(ref, np.float64)
+        u = np.ascontiguousarray(u, np.float64)
+        wi = np.ascontiguousarray(wi, np.float64)
+        out = np.zeros(8, np.float64)
+        rc = lib().or_sphere_light_probe(self.h, int(i), int(precision), _p(ref), _p(u), _p(wi), _p(out))
+        assert rc == 0
+        return out[0:3], out[3:6], out[6], out[7]
 
     def set_environment(self, rgb, scale=1.0):
         """EnvironmentLight(image, rotate, scale) added after the area lights;

@@ -1,4 +1,4 @@
-"""Per-sample parity against the oracle (level b), shared by the GPU tests."""
+"""Parity against the oracle (levels a and b), shared by the GPU tests."""
 import numpy as np
 
 import gyunity_amd as g
@@ -7,6 +7,32 @@ from oracle import oracle as O
 
 T_REL = 1e-6      # hit distance, relative
 L_REL = 1e-4      # per-sample radiance, relative to max(|L|, 1e-3)
+
+
+def check_closest(sc, os_, rays):
+    """Level (a) closest hits against the oracle's BVH traversal; returns the
+    fraction of hits on the same primitive."""
+    hits = sc.trace_closest(rays)
+    t, prim, b1, b2 = os_.trace_closest(rays, True)
+    miss_g = hits["prim"] < 0
+    miss_o = prim < 0
+    assert np.array_equal(miss_g, miss_o), "hit/miss disagree on %d rays" % (miss_g != miss_o).sum()
+    both = ~miss_g
+    same = hits["prim"][both] == prim[both]
+    # A tie: two triangles hit at t within 1e-6 relative (exactly equal, or an
+    # ulp apart on coplanar triangles such as a box face lying on the floor);
+    # which one wins then depends on traversal order in the reference too.
+    tie = np.abs(hits["t"][both] - t[both]) <= T_REL * np.abs(t[both])
+    diff = ~same & ~tie
+    assert diff.sum() == 0, "closest primitive differs on %d of %d rays" % (diff.sum(), both.sum())
+    assert (~same).mean() < 1e-2, "too many ties: %d of %d" % ((~same).sum(), both.sum())
+    rel = np.abs(hits["t"][both] - t[both]) / np.maximum(np.abs(t[both]), 1e-30)
+    assert rel.max() <= T_REL
+    # barycentrics of the same triangle (a sphere's uv) are computed identically
+    eq = both.copy()
+    eq[both] = same
+    assert np.array_equal(hits["b1"][eq], b1[eq]) and np.array_equal(hits["b2"][eq], b2[eq])
+    return same.mean()
 
 
 def assert_tie_explains(sc, os_, cam, prm_o, x, y, smp):
@@ -42,7 +68,7 @@ def assert_tie_explains(sc, os_, cam, prm_o, x, y, smp):
     return "bvh"
 
 
-def li_parity(gpu_ctx, sa, cam, prm, n, seed, xys=None, env=None):
+def li_parity(gpu_ctx, sa, cam, prm, n, seed, xys=None, env=None, spheres=None):
     """Per-sample radiance on identical sample indices (level b).  Every sample
     agrees within L_REL (maximum over samples and channels), or its path
     differs from the oracle's and the difference starts at an exact-t tie."""
@@ -54,11 +80,15 @@ def li_parity(gpu_ctx, sa, cam, prm, n, seed, xys=None, env=None):
     else:
         x, y, s = (np.ascontiguousarray(a, np.int32) for a in xys)
     os_ = O.OracleScene(sa)
+    if spheres is not None:
+        os_.set_spheres(spheres)
     if env is not None:
         os_.set_environment(env)
     prm_o = O.params(prm.spp, prm.max_depth, prm.rr_depth, 0, 8, 1, sampler=prm.sampler)
     Lo, ro, do = os_.li_samples(cam, prm_o, x, y, s)
     with g.Scene(gpu_ctx, sa) as sc:
+        if spheres is not None:
+            sc.set_spheres(spheres)
         if env is not None:
             sc.set_environment(env)
         Lg, rg, dg = sc.li_samples(cam, prm, x, y, s)

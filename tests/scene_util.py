@@ -348,3 +348,21 @@ def film_edge_samples(W, H, spp):
         out.append(np.stack([px[e], py[e], frame[e].astype(np.int64)], 1))
     xys = np.concatenate(out).astype(np.int32)
     return xys[:, 0], xys[:, 1], xys[:, 2]
+
+
+def sphere_scene(res=64, spp=16):
+    """The Cornell room (36 triangles, two area-light triangles) with three
+    analytic spheres after the triangles: white, coloured and a small emissive
+    one whose AreaLight follows the room's two."""
+    import gyunity_amd as g
+
+    sa, cam, prm = g.generate_scene(g.GY_SCENE_CORNELL)
+    cam.width = cam.height = res
+    prm.spp = spp
+    sp = np.zeros(3, g.SPHERE_DTYPE)
+    sp["center"] = [(0.3, 0.18, 0.55), (0.68, 0.25, 0.4), (0.5, 0.72, 0.45)]
+    sp["radius"] = [0.15, 0.2, 0.07]
+    sp["material"] = [0, 3, 0]
+    sp["emissive"] = [0, 0, 1]
+    sp["le"][2] = (6.0, 6.0, 6.0)
+    return sa, cam, prm, sp

@@ -17,7 +17,7 @@ import pytest
 
 import gyunity_amd as g
 from oracle import oracle as O
-from tests.parity_util import L_REL, T_REL, li_parity as _li_parity
+from tests.parity_util import L_REL, T_REL, check_closest as _check_closest, li_parity as _li_parity
 from tests.scene_util import camera_rays, film_edge_samples, film_weight_scan, furnace_scene, random_rays
 
 pytestmark = pytest.mark.gpu
@@ -39,30 +39,6 @@ def scenes():
     if SCENES is None:
         SCENES = _scenes()
     return SCENES
-
-
-def _check_closest(sc, os_, rays):
-    hits = sc.trace_closest(rays)
-    t, prim, b1, b2 = os_.trace_closest(rays, True)
-    miss_g = hits["prim"] < 0
-    miss_o = prim < 0
-    assert np.array_equal(miss_g, miss_o), "hit/miss disagree on %d rays" % (miss_g != miss_o).sum()
-    both = ~miss_g
-    same = hits["prim"][both] == prim[both]
-    # A tie: two triangles hit at t within 1e-6 relative (exactly equal, or an
-    # ulp apart on coplanar triangles such as a box face lying on the floor);
-    # which one wins then depends on traversal order in the reference too.
-    tie = np.abs(hits["t"][both] - t[both]) <= T_REL * np.abs(t[both])
-    diff = ~same & ~tie
-    assert diff.sum() == 0, "closest primitive differs on %d of %d rays" % (diff.sum(), both.sum())
-    assert (~same).mean() < 1e-2, "too many ties: %d of %d" % ((~same).sum(), both.sum())
-    rel = np.abs(hits["t"][both] - t[both]) / np.maximum(np.abs(t[both]), 1e-30)
-    assert rel.max() <= T_REL
-    # barycentrics of the same triangle are computed identically
-    eq = both.copy()
-    eq[both] = same
-    assert np.array_equal(hits["b1"][eq], b1[eq]) and np.array_equal(hits["b2"][eq], b2[eq])
-    return same.mean()
 
 
 @pytest.mark.parametrize("name", ["cornell", "ico5"])
