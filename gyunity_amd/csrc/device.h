@@ -672,7 +672,7 @@ GY_D int trav_run(const SceneDev& Sc, Trav& t, uint32_t* lds, int stride, uint32
 template <bool ANY, int STACK, bool COUNT = false, bool SPH = false>
 GY_D bool find_hit(const SceneDev& S, V3 o, V3 d, float tmin, float tmax, Hit* hit, uint32_t* stack, int stride,
                    uint32_t* n_node = nullptr, uint32_t* n_tri = nullptr) {
-    if (COUNT) return traverse<ANY, STACK>(S, o, d, tmin, tmax, hit, stack, stride, n_node, n_tri);
+    if (COUNT) return traverse<ANY, STACK, SPH>(S, o, d, tmin, tmax, hit, stack, stride, n_node, n_tri);
     Trav t;
     trav_init(t, o, d, tmin, tmax);
     const int r = trav_run<ANY, STACK>(S, t, stack, stride, nullptr, 0, 0);
