@@ -509,8 +509,9 @@ class Scene:
         check(lib().gy_scene_create(ctx.h, C.byref(self._desc), C.byref(bp), C.byref(self.h)))
 
     def close(self):
+        """gy_scene_destroy; raises (and keeps the scene) while a batch holds it."""
         if self.h:
-            lib().gy_scene_destroy(self.h)
+            check(lib().gy_scene_destroy(self.h))
             self.h = C.c_void_p()
 
     def __enter__(self):

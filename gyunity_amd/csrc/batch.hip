@@ -121,6 +121,7 @@ extern "C" int gy_batch_add(gy_batch* bh, gy_scene* scene, const gy_camera_desc*
     en.params = *p;
     b.entries.push_back(en);
     b.dirty = true;
+    ++sc->batch_refs;
     if (entry) *entry = (int32_t)b.entries.size() - 1;
     return GY_OK;
 }
@@ -261,7 +262,10 @@ extern "C" int gy_batch_destroy(gy_batch* bh) {
     (void)hipSetDevice(b.ctx->device);
     (void)hipStreamSynchronize(b.ctx->stream);
     BatchDropGraph(&b);
-    for (BatchEntry& en : b.entries) FilmFree(&en.film);
+    for (BatchEntry& en : b.entries) {
+        FilmFree(&en.film);
+        --en.scene->batch_refs;
+    }
     DevFree(b.table);
     DevFree(b.counters);
     WavefrontStateFree(&b.wf);

@@ -1169,6 +1169,8 @@ extern "C" int gy_render_light_tracing(gy_scene* s, const gy_camera_desc* cam, c
 extern "C" int gy_scene_destroy(gy_scene* s) {
     if (!s) return GY_OK;
     Scene* sc = reinterpret_cast<Scene*>(s);
+    // a batch's entry table and recorded graph hold this scene's device pointers
+    if (sc->batch_refs > 0) return Fail(GY_ERR_INVALID, "gy_scene_destroy: a batch still renders this scene (destroy the batch first)");
     (void)hipStreamSynchronize(sc->ctx->stream);
     SceneFree(sc);
     return GY_OK;

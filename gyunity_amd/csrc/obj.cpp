@@ -1,3 +1,29 @@
+// The number parser (ParseNumber) and the ear clipping (Triangulate) follow
+// tinyobjloader's steps closely, so its notice is kept here:
+//
+//   The MIT License (MIT)
+//
+//   Copyright (c) 2012-2018 Syoyo Fujita and many contributors.
+//
+//   Permission is hereby granted, free of charge, to any person obtaining a
+//   copy of this software and associated documentation files (the
+//   "Software"), to deal in the Software without restriction, including
+//   without limitation the rights to use, copy, modify, merge, publish,
+//   distribute, sublicense, and/or sell copies of the Software, and to permit
+//   persons to whom the Software is furnished to do so, subject to the
+//   following conditions:
+//
+//   The above copyright notice and this permission notice shall be included
+//   in all copies or substantial portions of the Software.
+//
+//   THE SOFTWARE IS PROVIDED "AS IS", WITHOUT WARRANTY OF ANY KIND, EXPRESS
+//   OR IMPLIED, INCLUDING BUT NOT LIMITED TO THE WARRANTIES OF
+//   MERCHANTABILITY, FITNESS FOR A PARTICULAR PURPOSE AND NONINFRINGEMENT. IN
+//   NO EVENT SHALL THE AUTHORS OR COPYRIGHT HOLDERS BE LIABLE FOR ANY CLAIM,
+//   DAMAGES OR OTHER LIABILITY, WHETHER IN AN ACTION OF CONTRACT, TORT OR
+//   OTHERWISE, ARISING FROM, OUT OF OR IN CONNECTION WITH THE SOFTWARE OR THE
+//   USE OR OTHER DEALINGS IN THE SOFTWARE.
+//
 // OBJ mesh ingest: the triangles Gyunity's scene holds after
 // Mesh::LoadFromFile (mesh/Mesh.cpp:7-84) and Scene::AddMesh
 // (visual/Scene.h:68-77), as flat per-triangle arrays for gy_scene_desc.

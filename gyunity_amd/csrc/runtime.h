@@ -73,6 +73,7 @@ struct WavefrontState {
 struct Scene {
     Ctx* ctx = nullptr;
     int32_t n_tri = 0;
+    int32_t batch_refs = 0;  // batch entries that render this scene
     gyd::SceneDev dev = {};
     void *d_tri = nullptr, *d_shade = nullptr, *d_nodes = nullptr, *d_mats = nullptr, *d_tex = nullptr, *d_texinfo = nullptr;
     void *d_ltri = nullptr, *d_lle = nullptr, *d_lcdf = nullptr, *d_lfunc = nullptr;

@@ -138,6 +138,27 @@ int main(int argc, char** argv) {
         std::printf("light tracing: splat sum %.4f, relative difference %.2e, %s\n", lsum, ldiff / (lsum > 0 ? lsum : 1),
                     lt_ok ? "splats agree" : "SPLAT MISMATCH");
         if (!lt_ok) ++bad;
+
+        // A copy of the scene with an analytic sphere is another scene (new
+        // stamp): the adapter uploads it, its film equals the C-ABI render of
+        // it, and rendering the first scene again gives the first film back.
+        {
+            const std::vector<float> first_color = film->color, first_weight = film->weight;
+            gy::Scene with_sphere = scene;
+            with_sphere.AddSphere({0.3f, 0.18f, 0.55f}, 0.15f, 0);
+            pt.Render(with_sphere, camera);
+            gy::DeviceScene dss(*dev, with_sphere);
+            p2.mode = GY_RENDER_MEGAKERNEL;
+            gy::Check(gy_render(dss.Handle(), &c2, &p2, col.data(), wgt.data()), "gy_render");
+            const bool sph_same = std::memcmp(col.data(), film->color.data(), col.size() * 4) == 0 &&
+                                  std::memcmp(wgt.data(), film->weight.data(), wgt.size() * 4) == 0;
+            const bool sph_differs = film->color != first_color;
+            pt.Render(scene, camera);
+            const bool back = film->color == first_color && film->weight == first_weight;
+            std::printf("scene with a sphere: %s, %s; first scene again: %s\n", sph_same ? "same film as the C ABI" : "FILM DIFFERS",
+                        sph_differs ? "differs from the first" : "SAME AS THE FIRST", back ? "first film" : "FILM DIFFERS");
+            if (!sph_same || !sph_differs || !back) ++bad;
+        }
         return bad ? 1 : 0;
     } catch (const gy::Error& e) {
         std::printf("error: %s\n", e.what());

@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -218,12 +219,30 @@ struct Material {
 };
 
 // ---------------------------------------------------------------- scene
+// A process-unique stamp: every Scene object, copy and change gets a new one,
+// so a device copy cached under it is never taken for another scene, even one
+// built at the same address.
+struct SceneStamp {
+    uint64_t v = Next();
+    SceneStamp() = default;
+    SceneStamp(const SceneStamp&) : v(Next()) {}
+    SceneStamp& operator=(const SceneStamp&) {
+        v = Next();
+        return *this;
+    }
+    static uint64_t Next() {
+        static std::atomic<uint64_t> n{0};
+        return ++n;
+    }
+};
+
 class Scene {
 public:
     // Scene::AddPrimitive for one triangle (primitive id = insertion order,
     // visual/Scene.h:43).  Returns the primitive id.
     int AddTriangle(const std::array<Vec3, 3>& p, const std::array<Vec3, 3>& n, const Vec3& faceNormal, int material,
                     const std::array<float, 6>& uv = {0, 0, 0, 0, 0, 0}) {
+        Touch();
         for (int v = 0; v < 3; ++v)
             for (int k = 0; k < 3; ++k) {
                 pos.push_back(p[v][k]);
@@ -236,11 +255,13 @@ public:
         return (int)mat.size() - 1;
     }
     int AddMaterial(const Material& m) {
+        Touch();
         mats.insert(mats.end(), m.words.begin(), m.words.end());
         return (int)(mats.size() / GY_MAT_WORDS) - 1;
     }
     // Scene::AddLight(AreaLight(shape = triangle, Le)) (light/AreaLight.h:12-85)
     int AddAreaLight(int triangle, const Vec3& Le) {
+        Touch();
         light[(size_t)triangle] = (int32_t)light_tri.size();
         light_tri.push_back(triangle);
         for (float v : Le) light_le.push_back(v);
@@ -250,6 +271,7 @@ public:
     // lights (light/EnvLight.h:14-17): the latitude-longitude image, top row
     // first, w x h x 3 floats.
     void SetEnvironmentLight(int w, int h, const std::vector<float>& rgb, float scale = 1.f) {
+        Touch();
         env_w = w;
         env_h = h;
         env_rgb = rgb;
@@ -257,10 +279,33 @@ public:
     }
     // ImageTexture (texture/ImageTexture.h:50-69): width x height x channels floats.
     int AddTexture(int w, int h, int channels, const std::vector<float>& data) {
+        Touch();
         tex_info.insert(tex_info.end(), {w, h, channels, (int32_t)tex_data.size()});
         tex_data.insert(tex_data.end(), data.begin(), data.end());
         return (int)(tex_info.size() / 4) - 1;
     }
+    // Sphere(Transform(), Transform(), radius, position, inside) with a
+    // material (shape/Sphere.h:21-27), after every triangle; emissive: an
+    // AreaLight on it with emitted radiance Le, after the triangle lights.
+    int AddSphere(const Vec3& center, float radius, int material, bool inside = false, bool emissive = false,
+                  const Vec3& Le = Vec3{0, 0, 0}) {
+        Touch();
+        gy_sphere q = {};
+        for (int k = 0; k < 3; ++k) {
+            q.center[k] = center[k];
+            q.le[k] = Le[k];
+        }
+        q.radius = radius;
+        q.inside = inside ? 1 : 0;
+        q.material = (uint32_t)material;
+        q.emissive = emissive ? 1 : 0;
+        spheres.push_back(q);
+        return (int)(mat.size() + spheres.size()) - 1;
+    }
+    // Call after changing the arrays below directly: the next Render uploads again.
+    void Touch() { stamp = SceneStamp(); }
+    uint64_t Stamp() const { return stamp.v; }
+
     gy_scene_desc Desc() const {
         gy_scene_desc d = {};
         d.n_tri = (int32_t)mat.size();
@@ -319,6 +364,10 @@ public:
     int env_w = 0, env_h = 0;  // env_w == 0: no environment light
     std::vector<float> env_rgb;
     float env_scale = 1.f;
+    std::vector<gy_sphere> spheres;
+
+private:
+    SceneStamp stamp;
 };
 
 // Scene resident on a device: triangles, BVH (built on the GPU), shading data.
@@ -327,6 +376,13 @@ public:
     DeviceScene(const Device& dev, const Scene& s, const gy_bvh_params* bvh = nullptr) {
         const gy_scene_desc d = s.Desc();
         Check(gy_scene_create(dev.Handle(), &d, bvh, &h_), "gy_scene_create");
+        if (!s.spheres.empty()) {
+            const int rc = gy_scene_set_spheres(h_, (int32_t)s.spheres.size(), s.spheres.data());
+            if (rc != GY_OK) {
+                gy_scene_destroy(h_);
+                Check(rc, "gy_scene_set_spheres");
+            }
+        }
         if (s.env_w > 0) {
             const int rc = gy_scene_set_environment(h_, s.env_rgb.data(), s.env_w, s.env_h, s.env_scale);
             if (rc != GY_OK) {
@@ -379,10 +435,10 @@ public:
     }
 
     void Render(const Scene& scene, const Camera& camera) override {
-        if (mpSceneSrc != &scene || !mpScene) {
+        if (mSceneStamp != scene.Stamp() || !mpScene) {
             mpScene.reset();
             mpScene.reset(new DeviceScene(*mpDevice, scene));
-            mpSceneSrc = &scene;
+            mSceneStamp = scene.Stamp();
         }
         const gy_camera_desc cam = camera.Desc();
         gy_render_params p = Params();
@@ -413,7 +469,7 @@ private:
     RenderParams mParams;
     std::shared_ptr<Device> mpDevice;
     std::unique_ptr<DeviceScene> mpScene;
-    const Scene* mpSceneSrc = nullptr;
+    uint64_t mSceneStamp = 0;  // the Scene::Stamp() of mpScene
 };
 
 // LightTracing on the GPU: LightTracing(pSampler, maxDepth, spp)
@@ -432,10 +488,10 @@ public:
     }
 
     void Render(const Scene& scene, const Camera& camera) override {
-        if (mpSceneSrc != &scene || !mpScene) {
+        if (mSceneStamp != scene.Stamp() || !mpScene) {
             mpScene.reset();
             mpScene.reset(new DeviceScene(*mpDevice, scene));
-            mpSceneSrc = &scene;
+            mSceneStamp = scene.Stamp();
         }
         const gy_camera_desc cam = camera.Desc();
         const gy_render_params p = Params();
@@ -463,7 +519,7 @@ private:
     RenderParams mParams;
     std::shared_ptr<Device> mpDevice;
     std::unique_ptr<DeviceScene> mpScene;
-    const Scene* mpSceneSrc = nullptr;
+    uint64_t mSceneStamp = 0;  // the Scene::Stamp() of mpScene
 };
 
 }  // namespace gy

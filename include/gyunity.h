@@ -142,6 +142,8 @@ int gy_device_name(gy_context* ctx, char* buf, int len);
 /* ---- scene (Scene::Initialize + BVHAccel::BVHAccel, visual/Scene.h:26-33,
  *      accelerator/BVHAccel.cpp:154-200) ------------------------------------ */
 int gy_scene_create(gy_context* ctx, const gy_scene_desc* desc, const gy_bvh_params* bvh, gy_scene** out);
+/* Fails (GY_ERR_INVALID, the scene kept) while a batch holds the scene:
+ * destroy the batch first. */
 int gy_scene_destroy(gy_scene* scene);
 /* Environment light: EnvironmentLight(file, rotate, scale) + Scene::AddLight
  * (light/EnvLight.h:14-17, visual/Scene.h:59-64) after the area lights, with the
@@ -246,6 +248,9 @@ typedef struct {
     int32_t use_graph;        /* 1: record once into a hipGraph and replay */
 } gy_batch_params;
 int gy_batch_create(gy_context* ctx, const gy_batch_params* p, gy_batch** out);
+/* The batch holds the scene until gy_batch_destroy (gy_scene_destroy refuses
+ * it meanwhile); changes to the scene's textures after a render are seen by the
+ * next gy_batch_render. */
 int gy_batch_add(gy_batch* b, gy_scene* scene, const gy_camera_desc* cam, const gy_render_params* p, int32_t* entry);
 int gy_batch_render(gy_batch* b);
 int gy_batch_film_download(gy_batch* b, int32_t entry, float* color, float* weight);

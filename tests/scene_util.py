@@ -366,3 +366,45 @@ def sphere_scene(res=64, spp=16):
     sp["emissive"] = [0, 0, 1]
     sp["le"][2] = (6.0, 6.0, 6.0)
     return sa, cam, prm, sp
+
+
+def reference_sphere_cornell(width=64, height=48, spp=16):
+    """The reference's sphere Cornell box (Gyunity_exe/CornellBox.h:22-100,
+    camera of Gyunity_exe/main.cpp:236-243): five inside-out walls of radius
+    1e5, a diffuse ball, three glass balls and a light ball (Le = 30), all
+    analytic spheres.  gy_scene_create needs a triangle, so the scene holds one
+    degenerate triangle (three equal vertices, which no ray hits) far below."""
+    import gyunity_amd as g
+
+    tri = np.tile(np.array([50.0, -1000.0, 50.0], np.float32), 3)[None, :]
+    n = np.tile(np.array([0.0, 1.0, 0.0], np.float32), 3)[None, :]
+    red, blue, white, full = (0.75, 0.25, 0.25), (0.25, 0.25, 0.75), (0.75, 0.75, 0.75), (1.0, 1.0, 1.0)
+    mats = np.stack([material(0, R=red), material(0, R=blue), material(0, R=white), material(0, R=full),
+                     material(2, R=full, T=full), material(0, R=(0.0, 0.0, 0.0))])
+    sa = SceneArrays(tri, n, np.array([[0.0, 1.0, 0.0]], np.float32), np.array([2], np.uint32),
+                     np.array([-1], np.int32), mats, np.zeros(0, np.int32), np.zeros((0, 3), np.float32))
+    rows = [  # centre, radius, material, inside, Le
+        ((1e5 + 1, 40.8, 81.6), 1e5, 0, 1, None),
+        ((-1e5 + 99, 40.8, 81.6), 1e5, 1, 1, None),
+        ((50, 40.8, 1e5), 1e5, 2, 1, None),
+        ((50, 1e5, 81.6), 1e5, 2, 1, None),
+        ((50, -1e5 + 81.6, 81.6), 1e5, 2, 1, None),
+        ((23, 0.0, 98), 9.5, 3, 0, None),
+        ((73, 26.5, 78), 16.5, 4, 0, None),
+        ((27, 16.5, 47), 16.5, 4, 0, None),
+        ((53, 9.5, 88), 9.5, 4, 0, None),
+        ((50, 81.6 - 16.5, 81.6), 8.0, 5, 0, (30.0, 30.0, 30.0)),
+    ]
+    sp = np.zeros(len(rows), g.SPHERE_DTYPE)
+    for i, (c, r, m, inside, le) in enumerate(rows):
+        sp[i]["center"], sp[i]["radius"], sp[i]["material"], sp[i]["inside"] = c, r, m, inside
+        if le is not None:
+            sp[i]["emissive"], sp[i]["le"] = 1, le
+    cz = np.array([0.0, -0.042612, -1.0])
+    pos = np.array([50.0, 52.0, 295.6])
+    cx = np.array([1.0, 0.0, 0.0])
+    cy = np.cross(cx, cz)
+    cy /= np.linalg.norm(cy)
+    cam = g.camera(tuple(pos), tuple(pos + cz), tuple(cy), 28.7993, width, height, float(np.linalg.norm(cz)))
+    prm = g.render_params(spp=spp, max_depth=5, rr_depth=3)
+    return sa, cam, prm, sp

@@ -568,6 +568,14 @@ def test_batch_errors(gpu_ctx):
         e = b.add(sc, cam, prm)
         with pytest.raises(g.GyError):
             b.film(e)  # not rendered yet
+        # the batch holds the scene: destroying it first is refused and keeps it
+        with pytest.raises(g.GyError) as err:
+            sc.close()
+        assert "batch" in str(err.value)
+        b.render()
+        assert b.film(e)[1].sum() >= cam.width * cam.height * prm.spp
+        b.close()
+        sc.close()
 
 
 # ---------------------------------------------------------------- path-replay adjoint

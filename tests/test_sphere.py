@@ -22,7 +22,7 @@ import pytest
 import gyunity_amd as g
 from gyunity_amd import RAY_DTYPE
 from oracle import oracle as O
-from tests.scene_util import camera_rays, random_rays, sphere_scene
+from tests.scene_util import camera_rays, random_rays, reference_sphere_cornell, sphere_scene
 
 RAY_EPS = 1e-4
 
@@ -246,6 +246,19 @@ def test_sphere_hits_carry_no_radiance():
     assert img[~covered].max() > 0.0
 
 
+def test_reference_sphere_cornell_renders_black():
+    # The reference's all-sphere Cornell box: every camera ray hits a sphere
+    # (inside-out walls enclose the camera), so with the default intersection
+    # record every pixel is exactly 0 while every sample still adds its weight.
+    sa, cam, prm, sp = reference_sphere_cornell(32, 24, 4)
+    os_ = _oracle(sa, sp)
+    _, prim, _, _ = os_.trace_closest(camera_rays(cam, 4000, 1), True)
+    assert np.all(prim >= sa.n_tri)
+    color, weight, _, _ = os_.render(cam, O.params(4, prm.max_depth, prm.rr_depth, precision=1, nthreads=8))
+    assert np.all(color == 0.0)
+    assert weight.sum() >= 32 * 24 * 4
+
+
 # ----------------------------------------------------------------------------- GPU
 
 
@@ -279,6 +292,23 @@ def test_sphere_level_b_per_sample(gpu_ctx, seed):
     sa, cam, prm, sp = sphere_scene(res=64, spp=16)
     err = li_parity(gpu_ctx, sa, cam, prm, 20000, seed, spheres=sp)
     assert err.size > 19000 and err.max() <= 1e-4
+
+
+@pytest.mark.gpu
+def test_reference_sphere_cornell_level_b_and_film(gpu_ctx):
+    # the all-sphere scene through the library: per-sample radiance (all 0)
+    # and dimensions consumed equal the oracle's; the film is black with the
+    # oracle's weights
+    from tests.parity_util import li_parity
+
+    sa, cam, prm, sp = reference_sphere_cornell(64, 48, 8)
+    err = li_parity(gpu_ctx, sa, cam, prm, 10000, 3, spheres=sp)
+    assert err.size >= 9980
+    with g.Scene(gpu_ctx, sa) as sc:
+        sc.set_spheres(sp)
+        color, weight = sc.render(cam, prm)
+    _, w, _, _ = _oracle(sa, sp).render(cam, O.params(prm.spp, prm.max_depth, prm.rr_depth, 0, 8))
+    assert np.all(color == 0.0) and np.array_equal(weight, w.astype(np.float32))
 
 
 @pytest.mark.gpu
