@@ -13,6 +13,8 @@ from ._lib import (  # noqa: F401
     RAY_DTYPE,
     HIT_DTYPE,
     SPHERE_DTYPE,
+    MEDIUM_DTYPE,
+    TRI_MEDIUM_DTYPE,
     camera,
     camera_matrix,
     generate_scene,

@@ -114,6 +114,10 @@ class RenderStats(C.Structure):
 
 RAY_DTYPE = np.dtype([("org", "<f4", 3), ("tmin", "<f4"), ("dir", "<f4", 3), ("tmax", "<f4")])
 HIT_DTYPE = np.dtype([("t", "<f4"), ("prim", "<i4"), ("b1", "<f4"), ("b2", "<f4")])
+# gy_medium (HomogeneousMedium) and gy_tri_medium (per-triangle MediumInterface,
+# -1: none, and the no-material flag of a medium boundary)
+MEDIUM_DTYPE = np.dtype([("sigma_a", "<f4", 3), ("sigma_s", "<f4", 3), ("g", "<f4")])
+TRI_MEDIUM_DTYPE = np.dtype([("inside", "<i4"), ("outside", "<i4"), ("no_material", "<i4")])
 # gy_sphere: centre, radius, inside flag, material, emissive flag, emitted radiance
 SPHERE_DTYPE = np.dtype([("center", "<f4", 3), ("radius", "<f4"), ("inside", "<i4"), ("material", "<u4"),
                          ("emissive", "<i4"), ("le", "<f4", 3)])
@@ -181,6 +185,10 @@ def lib():
         "gy_render_stats_get": (C.c_int, [vp, P(RenderStats)]),
         "gy_kernel_name": (C.c_char_p, [C.c_int]),
         "gy_li_samples": (C.c_int, [vp, P(CameraDesc), P(RenderParams), i32, P(i32), P(i32), P(i32), P(f32), P(f32), P(i32)]),
+        "gy_scene_set_media": (C.c_int, [vp, i32, vp, vp]),
+        "gy_render_vpt": (C.c_int, [vp, P(CameraDesc), P(RenderParams), i32, vp, vp]),
+        "gy_vpt_li_samples": (C.c_int, [vp, P(CameraDesc), P(RenderParams), i32, i32, P(i32), P(i32), P(i32), P(f32), P(f32),
+                                        P(i32)]),
         "gy_count_traffic": (C.c_int, [vp, P(CameraDesc), P(RenderParams), P(RenderStats)]),
         "gy_copy_probe": (C.c_int, [vp, i64, P(C.c_double)]),
         "gy_sobol_sample": (f32, [u64, u32]),
@@ -641,6 +649,36 @@ class Scene:
         raster = np.zeros((n, 2), dtype=np.float32)
         dims = np.zeros(n, dtype=np.int32)
         check(lib().gy_li_samples(self.h, C.byref(cam), C.byref(params), n, _ip(x), _ip(y), _ip(s), _fp(L), _fp(raster), _ip(dims)))
+        return L, raster, dims
+
+    def set_media(self, media, tri):
+        """gy_scene_set_media: media (MEDIUM_DTYPE) and one TRI_MEDIUM_DTYPE
+        record per triangle."""
+        self._media = np.ascontiguousarray(media, dtype=MEDIUM_DTYPE)
+        self._tri_media = np.ascontiguousarray(tri, dtype=TRI_MEDIUM_DTYPE)
+        if self._tri_media.shape[0] != self.arrays.n_tri:
+            raise GyError(1, "set_media: one record per triangle")
+        check(lib().gy_scene_set_media(self.h, self._media.shape[0], self._media.ctypes.data, self._tri_media.ctypes.data))
+
+    def render_vpt(self, cam, params, equiangular=True):
+        """VolPathTracing (gy_render_vpt): (color (H, W, 3), weight (H, W))."""
+        W, H = cam.width, cam.height
+        color = np.zeros((H, W, 3), dtype=np.float32)
+        weight = np.zeros((H, W), dtype=np.float32)
+        check(lib().gy_render_vpt(self.h, C.byref(cam), C.byref(params), 1 if equiangular else 0, color.ctypes.data,
+                                  weight.ctypes.data))
+        return color, weight
+
+    def vpt_li_samples(self, cam, params, x, y, s, equiangular=True):
+        x = np.ascontiguousarray(x, dtype=np.int32)
+        y = np.ascontiguousarray(y, dtype=np.int32)
+        s = np.ascontiguousarray(s, dtype=np.int32)
+        n = x.shape[0]
+        L = np.zeros((n, 3), dtype=np.float32)
+        raster = np.zeros((n, 2), dtype=np.float32)
+        dims = np.zeros(n, dtype=np.int32)
+        check(lib().gy_vpt_li_samples(self.h, C.byref(cam), C.byref(params), 1 if equiangular else 0, n, _ip(x), _ip(y), _ip(s),
+                                      _fp(L), _fp(raster), _ip(dims)))
         return L, raster, dims
 
     def count_traffic(self, cam, params):

@@ -115,6 +115,7 @@ extern "C" int gy_batch_add(gy_batch* bh, gy_scene* scene, const gy_camera_desc*
     if (sc->ctx != b.ctx) return Fail(GY_ERR_INVALID, "gy_batch_add: scene belongs to another context");
     if (sc->stats.max_depth + 1 > 64) return Fail(GY_ERR_UNSUPPORTED, "gy_batch_add: BVH deeper than the traversal stack");
     if (sc->dev.n_sph > 0) return Fail(GY_ERR_UNSUPPORTED, "gy_batch_add: scenes with spheres (the batch runs the wavefront kernels)");
+    if (sc->has_no_mat) return Fail(GY_ERR_UNSUPPORTED, "gy_batch_add: scenes with material-less triangles (PathTracing only)");
     BatchEntry en;
     en.scene = sc;
     en.cam = *cam;

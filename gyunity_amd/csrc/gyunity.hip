@@ -598,7 +598,7 @@ fail:
 void SceneFree(Scene* sc) {
     if (!sc) return;
     void* bufs[] = {sc->d_tri, sc->d_shade, sc->d_nodes, sc->d_mats, sc->d_tex, sc->d_texinfo, sc->d_ltri, sc->d_lle, sc->d_lcdf, sc->d_lfunc,
-                    sc->d_env, sc->d_sph};
+                    sc->d_env, sc->d_sph, sc->d_media, sc->d_tri_media};
     for (void* b : bufs) DevFree(b);
     FilmFree(&sc->film);
     WavefrontFree(sc);
@@ -684,7 +684,8 @@ int FilmResolveState(Ctx* c, const FilmState& f, int n_s, bool timed) {
 }
 int FilmResolvePass(Ctx* c, Scene* sc, int n_s) { return FilmResolveState(c, sc->film, n_s); }
 
-int MakePathParams(Ctx* c, Scene* sc, const gy_camera_desc& cam, const gy_render_params& p, PathParams* P) {
+// vpt: for gy_render_vpt, which also renders material-less medium boundaries.
+int MakePathParamsFor(Ctx* c, Scene* sc, const gy_camera_desc& cam, const gy_render_params& p, PathParams* P, bool vpt) {
     CameraRT rt;
     int rc = BuildCamera(cam, &rt);
     if (rc != GY_OK) return rc;
@@ -731,7 +732,14 @@ int MakePathParams(Ctx* c, Scene* sc, const gy_camera_desc& cam, const gy_render
     }
     // Halton: dimensions past 255 read 0 as HaltonSampler::sample returns
     if (sc->dev.n_light == 0) return Fail(GY_ERR_INVALID, "render: the scene has no light (area or environment)");
+    // PathTracing dereferences every hit's BSDF; material-less medium
+    // boundaries are for VolPathTracing only
+    if (sc->has_no_mat && !vpt)
+        return Fail(GY_ERR_UNSUPPORTED, "render: the scene has material-less triangles (medium boundaries): use gy_render_vpt");
     return GY_OK;
+}
+int MakePathParams(Ctx* c, Scene* sc, const gy_camera_desc& cam, const gy_render_params& p, PathParams* P) {
+    return MakePathParamsFor(c, sc, cam, p, P, false);
 }
 
 int ValidateRender(const gy_camera_desc* cam, const gy_render_params* p) {
@@ -1114,6 +1122,7 @@ extern "C" int gy_render_light_tracing(gy_scene* s, const gy_camera_desc* cam, c
     Ctx* c = sc->ctx;
     (void)hipSetDevice(c->device);
     if (sc->area_tri.empty() && sc->dev.env.w <= 0) return Fail(GY_ERR_INVALID, "gy_render_light_tracing: the scene has no light");
+    if (sc->has_no_mat) return Fail(GY_ERR_UNSUPPORTED, "gy_render_light_tracing: scenes with material-less triangles");
     if (sc->dev.n_sph > 0) return Fail(GY_ERR_UNSUPPORTED, "gy_render_light_tracing: scenes with spheres");
     const int64_t n = (int64_t)cam->width * cam->height * p->spp;
     if (n >= (1ll << 36)) return Fail(GY_ERR_UNSUPPORTED, "gy_render_light_tracing: more than 2^36 light paths");

@@ -79,6 +79,11 @@ struct Scene {
     void *d_ltri = nullptr, *d_lle = nullptr, *d_lcdf = nullptr, *d_lfunc = nullptr;
     void* d_env = nullptr;  // environment light: texels and sampling tables in one block
     void* d_sph = nullptr;  // analytic spheres (gy_scene_set_spheres)
+    // participating media (gy_scene_set_media): 2 float4 per medium, one word
+    // per triangle (inside + 1, outside + 1 << 16, no material << 31)
+    void *d_media = nullptr, *d_tri_media = nullptr;
+    int32_t n_media = 0;
+    bool has_no_mat = false;  // some triangle has no material (a medium boundary)
     uint64_t revision = 0;  // bumped when device buffers the kernels point at are replaced
     float world_lo[3] = {0, 0, 0}, world_hi[3] = {0, 0, 0};
     std::vector<int32_t> area_tri;  // area lights as created (the environment light follows them)

@@ -174,6 +174,39 @@ typedef struct {
     float le[3];
 } gy_sphere;
 int gy_scene_set_spheres(gy_scene* scene, int32_t n, const gy_sphere* spheres);
+/* Participating media for VolPathTracing: HomogeneousMedium(sigma_a, sigma_s, g)
+ * (volume/Homogeneous.h:11-12, Henyey-Greenstein phase, visual/Medium.h:39-48)
+ * and, per triangle (scene order), Scene::AddPrimitive's MediumInterface(inside,
+ * outside) (visual/Medium.h:51-59, -1: none) and whether the primitive has no
+ * material (a medium boundary, crossed by rays).  Calling it again replaces
+ * the media.  Scenes with material-less triangles render only through
+ * gy_render_vpt / gy_vpt_li_samples. */
+typedef struct {
+    float sigma_a[3];
+    float sigma_s[3];
+    float g;              /* (-1, 1) */
+} gy_medium;
+typedef struct {
+    int32_t inside, outside;
+    int32_t no_material;
+} gy_tri_medium;
+int gy_scene_set_media(gy_scene* scene, int32_t n_media, const gy_medium* media, const gy_tri_medium* tri);
+/* VolPathTracing(spp, maxDepth, sampler, samplerEnum, equiAngular)
+ * (integrator/VPT.h:12-19, VPT.cpp:7-171) through TiledIntegrator::Render:
+ * params as gy_render's (spp, max_depth, sampler, samples_per_pass; rr_depth,
+ * mode, block_size and refill_threshold are not used: the reference's VPT has
+ * no working roulette and runs one path per thread here).  equiangular 1:
+ * equi-angular distance sampling about one light point per bounce
+ * (HomogeneousMedium::EquiAngularSampling, the reference's default); 0:
+ * free-flight sampling with DirectIllumination at medium points.  Not
+ * supported: spheres; equi-angular sampling with an environment light. */
+int gy_render_vpt(gy_scene* scene, const gy_camera_desc* cam, const gy_render_params* params, int32_t equiangular,
+                  float* color, float* weight);
+/* Per-sample VolPathTracing::Li on n (x, y, sample) indices: radiance (3n),
+ * film position (2n) and dimensions drawn (n). */
+int gy_vpt_li_samples(gy_scene* scene, const gy_camera_desc* cam, const gy_render_params* params, int32_t equiangular,
+                      int32_t n, const int32_t* x, const int32_t* y, const int32_t* sample, float* L, float* raster,
+                      int32_t* dims);
 int gy_scene_stats_get(gy_scene* scene, gy_scene_stats* out);
 /* The context a scene was created on. */
 int gy_scene_context(gy_scene* scene, gy_context** ctx);

@@ -571,6 +571,11 @@ template <class R> struct Env {
     R radius = 0;
 };
 
+// HomogeneousMedium (volume/Homogeneous.h:9-24): sigma_t = sigma_s + sigma_a.
+template <class R> struct Medium {
+    V3<R> sa, ss, st;
+    R g = 0;
+};
 template <class R> struct Scene {
     Env<R> env;
     std::vector<Tri<R>> tris;
@@ -602,6 +607,12 @@ template <class R> struct Scene {
         AABB<R> box;  // ObjectToWorld(AABB(c - r, c + r)) with the identity transform
     };
     std::vector<Sph> sph;
+    // Participating media (or_scene_set_media): per triangle the medium
+    // interface's inside and outside media (-1: none) and whether the
+    // primitive has no material (a medium boundary).  Empty: no media.
+    std::vector<Medium<R>> media;
+    std::vector<int32_t> mi_in, mi_out;
+    std::vector<uint8_t> no_mat;
 };
 template <class R> inline uint32_t prim_mat(const Scene<R>& s, int32_t prim) {
     return prim < (int32_t)s.tris.size() ? s.tris[prim].mat : s.sph[prim - s.tris.size()].mat;
@@ -1890,9 +1901,12 @@ template <class R> V3<R> Li(const Scene<R>& s, Ray<R> ray, Seq& rnd, int maxDept
 }
 
 // One camera sample of TiledIntegrator::Render (integrator/TiledIntegrator.cpp:40-47).
+template <class R> V3<R> Li_vpt(const Scene<R>& s, Ray<R> ray, Seq& rnd, int maxDepth, bool equiAngular, bool use_bvh);
+// integrator 0: PathTracing; 1: VolPathTracing with free-flight sampling; 2:
+// VolPathTracing with equi-angular sampling.
 template <class R> V3<R> camera_sample(const Scene<R>& s, const Camera<R>& cam, int res, int m, int x, int y, int smp,
                                        int maxDepth, int rrDepth, bool use_bvh, R* fx, R* fy, int* dims,
-                                       const hal::Enum* henum = nullptr) {
+                                       const hal::Enum* henum = nullptr, int integrator = 0) {
     Seq rnd;
     R u, v;
     if (henum) {
@@ -1907,7 +1921,7 @@ template <class R> V3<R> camera_sample(const Scene<R>& s, const Camera<R>& cam, 
         v = res * (R)rnd.next() - R(y);
     }
     Ray<R> ray = generate_ray(cam, x, y, u, v);
-    V3<R> L = Li(s, ray, rnd, maxDepth, rrDepth, use_bvh);
+    V3<R> L = integrator > 0 ? Li_vpt(s, ray, rnd, maxDepth, integrator == 2, use_bvh) : Li(s, ray, rnd, maxDepth, rrDepth, use_bvh);
     *fx = x + u;
     *fy = y + v;
     if (dims) *dims = rnd.cursor;
@@ -2080,6 +2094,404 @@ template <class R, class F> void lt_path(const Scene<R>& s, const LTCamera<R>& c
 }
 
 // Film::AddSample with the radius-0.5 box filter (visual/Film.h:61-87, math/Filter.h:26-38)
+// ---------------------------------------------------------------- volumetric path tracing
+// VolPathTracing (integrator/VPT.cpp:7-171) over HomogeneousMedium
+// (volume/Homogeneous.cpp:6-64), HenyeyGreenstein (visual/Medium.cpp:8-29),
+// MediumInterface (visual/Medium.h:51-59) and primitives without a material
+// (medium boundaries).  Media are numbered; -1 is no medium (nullptr).
+// Camera rays carry no medium (PinHoleCamera::GenerateRay sets none).
+// Loops over medium boundaries are bounded (kVptMaxCrossings per loop), which
+// no scene with closed boundaries reaches.  The reference reads past its 1024
+// Sobol dimensions when a path draws more (an assert compiled out): here a
+// path ends at the top of an iteration once it has drawn more than
+// kVptDimBudget dimensions (an iteration draws at most 12).
+constexpr int kVptMaxCrossings = 256;
+constexpr int kVptDimBudget = 1024 - 16;
+inline float rexp(float x) { return (float)std::exp((double)x); }
+inline double rexp(double x) { return std::exp(x); }
+inline float rtan(float x) { return (float)std::tan((double)x); }
+inline double rtan(double x) { return std::tan(x); }
+template <class R> inline V3<R> vexp(const V3<R>& a) { return V3<R>(rexp(a.x), rexp(a.y), rexp(a.z)); }
+template <class R> constexpr R kInv4Pi = (R)0.07957747154594766788;  // common/Core.h:188
+template <class R> constexpr R kMaxReal = (R)std::numeric_limits<float>::max();  // common/Core.h:198 (real = float)
+
+// PhaseHG (visual/Medium.h:21-24)
+template <class R> inline R phase_hg(R cosTheta, R g) {
+    R denom = 1 + g * g + 2 * g * cosTheta;
+    return kInv4Pi<R> * (1 - g * g) / (denom * std::sqrt(denom));
+}
+// HenyeyGreenstein::Sample_p: cos theta about -wo by inversion, the direction
+// built on CoordinateSystem(wo) (SphericalDirection, math/GeometryUtils.h:29-33)
+template <class R> R hg_sample(R g, const V3<R>& wo, V3<R>* wi, R u0, R u1) {
+    R cosTheta;
+    if ((double)std::abs(g) < 1e-3) {
+        cosTheta = 1 - 2 * u0;
+    } else {
+        R sqrTerm = (1 - g * g) / (1 - g + 2 * g * u0);
+        cosTheta = (1 + g * g - sqrTerm * sqrTerm) / (2 * g);
+    }
+    R sinTheta = std::sqrt(rmax((R)0, 1 - cosTheta * cosTheta));
+    R phi = 2 * K<R>::PI * u1;
+    V3<R> v1, v2;
+    coordinate_system(wo, &v1, &v2);
+    *wi = ((sinTheta * rcos(phi)) * v1 + (sinTheta * rsin(phi)) * v2) + cosTheta * (-wo);
+    return phase_hg(-cosTheta, g);
+}
+template <class R> inline R hg_p(R g, const V3<R>& wo, const V3<R>& wi) { return phase_hg(dot(wo, wi), g); }
+
+// HomogeneousMedium::Tr: exp(-sigma_t min(tMax |d|, MaxReal))
+template <class R> inline V3<R> medium_tr(const Medium<R>& m, const Ray<R>& ray) {
+    return vexp((-m.st) * rmin(ray.tmax * length(ray.d), kMaxReal<R>));
+}
+// A scattering point in a medium (MediumIntersection): position, outgoing
+// direction, medium.  Its normals and error bound are zero.
+template <class R> struct MediumRec {
+    bool valid = false;
+    V3<R> pos, wo;
+    int32_t medium = -1;
+};
+template <class R> inline Isect<R> medium_isect(const MediumRec<R>& mi) {
+    Isect<R> it;
+    it.pos = mi.pos;
+    it.n = it.absN = it.ng = it.sdpdu = it.sdpdv = it.pErr = V3<R>(0, 0, 0);
+    it.outDir = mi.wo;
+    return it;
+}
+// HomogeneousMedium::Sample: a channel, then a free-flight distance in it;
+// the weight over the channel-averaged density.
+template <class R> V3<R> medium_sample(const Medium<R>& m, int32_t med, const Ray<R>& ray, R u_ch, R u_d, MediumRec<R>* mi,
+                                       R* t_out = nullptr) {
+    int channel = std::min((int)(u_ch * 3), 2);
+    R dist = -rlog(1 - u_d) / m.st[channel];
+    R t = rmin(dist / length(ray.d), ray.tmax);
+    bool sampled = t < ray.tmax;
+    if (t_out) *t_out = t;
+    if (sampled) {
+        mi->valid = true;
+        mi->pos = ray.o + ray.d * t;
+        mi->wo = -ray.d;
+        mi->medium = med;
+    }
+    V3<R> Tr = vexp(((-m.st) * rmin(t, kMaxReal<R>)) * length(ray.d));
+    V3<R> density = sampled ? (m.st * Tr) : Tr;
+    R pdf = 0;
+    for (int i = 0; i < 3; ++i) pdf += density[i];
+    pdf *= 1 / (R)(3);
+    if (pdf == 0) pdf = 1;
+    return sampled ? (Tr * m.ss / pdf) : (Tr / pdf);
+}
+// HomogeneousMedium::EquiAngularSampling: t about the light point's foot on
+// the ray over [0, 3] (tMax = 3 is fixed), pdf D / ((theta_b - theta_a)(D^2 + tau^2));
+// past the surface, the probability of t beyond min(3, tMax).
+template <class R> V3<R> medium_equiangular(const Medium<R>& m, int32_t med, const Ray<R>& ray, R u, const V3<R>& lightPos,
+                                            MediumRec<R>* mi, R* t_out = nullptr, R* pdf_out = nullptr) {
+    const R tMax = 3;
+    R delta = dot(lightPos - ray.o, ray.d);
+    R D = length((ray.o + delta * ray.d) - lightPos);
+    R theta_a = ratan2((R)0.f - delta, D);
+    R theta_b = ratan2(tMax - delta, D);
+    R tau = D * rtan((1 - u) * theta_a + u * theta_b);
+    R t = delta + tau;
+    bool sampleMedia = t < ray.tmax;
+    V3<R> Tr = sampleMedia ? vexp((-m.st) * t) : medium_tr(m, ray);
+    R pdf = sampleMedia ? D / ((theta_b - theta_a) * (D * D + tau * tau))
+                        : 1 - (ratan2(rmin(tMax, ray.tmax) - delta, D) - theta_a) / (theta_b - theta_a);
+    if (t_out) *t_out = t;
+    if (pdf_out) *pdf_out = pdf;
+    if (sampleMedia) {
+        mi->valid = true;
+        mi->pos = ray.o + ray.d * t;
+        mi->wo = -ray.d;
+        mi->medium = med;
+    }
+    return sampleMedia ? Tr * m.ss / pdf : medium_tr(m, ray) / pdf;
+}
+
+// The medium interface a closest hit leaves in its Intersection
+// (Primitive::Intersect, visual/Primitive.cpp:11-14): the primitive's own when
+// it is a transition (inside != outside), else the ray's medium on both sides.
+template <class R> inline void hit_media(const Scene<R>& s, int32_t prim, int32_t ray_medium, int32_t* in, int32_t* out) {
+    const int32_t a = s.mi_in.empty() ? -1 : s.mi_in[prim], b = s.mi_out.empty() ? -1 : s.mi_out[prim];
+    if (a != b) {
+        *in = a;
+        *out = b;
+    } else {
+        *in = *out = ray_medium;
+    }
+}
+// Intersection::GetMedium(w) (visual/Intersection.h:80-82): outside when w
+// leaves along the shading normal, else inside.
+template <class R> inline int32_t isect_medium(const Isect<R>& it, int32_t in, int32_t out, const V3<R>& w) {
+    return dot(w, it.n) > 0 ? out : in;
+}
+template <class R> inline bool has_material(const Scene<R>& s, int32_t prim) { return s.no_mat.empty() || !s.no_mat[prim]; }
+
+// VisibilityTester::Tr (visual/Visibility.cpp:13-35): closest hits from p0
+// to the light point; a surface with a material occludes; transmittance of
+// each segment's medium; a material-less surface is crossed (queried, then
+// SpawnTo the light point again).
+template <class R> V3<R> vis_tr(const Scene<R>& s, const Isect<R>& p0, int32_t p0_in, int32_t p0_out, const LightPoint<R>& p1,
+                                bool use_bvh) {
+    Ray<R> ray = spawn_to(p0, p1);
+    int32_t med = isect_medium(p0, p0_in, p0_out, ray.d);
+    V3<R> Tr(1.f);
+    for (int k = 0; k < kVptMaxCrossings; ++k) {
+        Hit<R> h;
+        const bool hit = intersect(s, ray, &h, use_bvh);
+        if (hit && has_material(s, h.prim)) return V3<R>();
+        if (med >= 0) Tr = Tr * medium_tr(s.media[med], ray);
+        if (!hit) return Tr;
+        Isect<R> it = query(s, ray, h);
+        int32_t in, out;
+        hit_media(s, h.prim, med, &in, &out);
+        ray = spawn_to(it, p1);
+        med = isect_medium(it, in, out, ray.d);
+    }
+    return V3<R>();
+}
+// Scene::IntersectTr (visual/Scene.cpp:14-26): closest hits with the
+// transmittance of each segment, crossing material-less surfaces.  The crossed
+// record is never queried: it holds Triangle::Intersect's position and no
+// normal or error bound, so the next segment starts at the hit point itself
+// and takes the interface's inside medium.
+// g_vpt_query_crossings is a test-only switch that queries the crossed
+// surface and takes the medium on the far side of its normal instead.
+static int g_vpt_query_crossings = 0;
+template <class R> bool intersect_tr(const Scene<R>& s, Ray<R>& ray, int32_t* med, Hit<R>* h, V3<R>* Tr, bool use_bvh) {
+    *Tr = V3<R>(1.f);
+    for (int k = 0; k < kVptMaxCrossings; ++k) {
+        const bool hit = intersect(s, ray, h, use_bvh);
+        if (*med >= 0) *Tr = *Tr * medium_tr(s.media[*med], ray);
+        if (!hit) return false;
+        if (has_material(s, h->prim)) return true;
+        const Tri<R>& t = s.tris[h->prim];
+        const R b1 = h->b1, b2 = h->b2;
+        const V3<R> pos = ((1 - b1 - b2) * t.p0 + b1 * t.p1) + b2 * t.p2;
+        int32_t in, out;
+        hit_media(s, h->prim, *med, &in, &out);
+        if (g_vpt_query_crossings) {
+            const Isect<R> it = query(s, ray, *h);
+            *med = isect_medium(it, in, out, ray.d);
+            ray = spawn_ray(it, ray.d);
+            continue;
+        }
+        ray = make_ray(offset_ray_origin(pos, V3<R>(0, 0, 0), V3<R>(0, 0, 0), ray.d), ray.d, K<R>::Inf, (R)0);
+        *med = in;
+    }
+    return false;
+}
+
+// Integrator::DirectIllumination with handleMedia (integrator/Integrator.cpp:9-107):
+// at a surface (bsdf given) or a medium point (phase g), visibility by
+// VisibilityTester::Tr and the BSDF/phase sample's light hit by IntersectTr.
+template <class R> V3<R> di_vpt(const Scene<R>& s, const Isect<R>& it, int32_t in, int32_t out, const BSDF<R>* bsdf, R g,
+                                R uLight, R u0, R u1, const V3<R>& v, bool use_bvh) {
+    R lightSamplingPdf;
+    R weight1 = 1, weight2 = 1;
+    V3<R> L1, L2;
+    int light = sample_one_light(s, uLight, &lightSamplingPdf);
+    int ltri = s.light_tri[light];  // < 0: the environment light
+    V3<R> Le = s.light_le[light];
+    {
+        R pdf;
+        LightPoint<R> lp;
+        V3<R> wi, Li;
+        if (ltri < 0) {
+            V3<R> sd;
+            Li = env_sample(s.env, u0, u1, &sd, &pdf);
+            wi = norm(sd);
+            lp.pos = it.pos + wi * (2 * s.env.radius);
+            lp.n = lp.fn = wi;
+            lp.pErr = V3<R>(0, 0, 0);
+        } else {
+            lp = sample_light_point(s.tris[ltri], it, u0, u1, &pdf);
+            wi = norm(lp.pos - it.pos);
+            Li = dot(lp.n, -wi) > 0 ? Le : V3<R>(0, 0, 0);
+        }
+        if (pdf > 0 && Li != V3<R>(0)) {
+            R scatteringPdf, cosTheta = 1;
+            V3<R> f;
+            if (bsdf) {
+                scatteringPdf = bsdf->Pdf(it.outDir, wi);
+                f = bsdf->Evaluate(it.outDir, wi);
+                cosTheta = std::abs(dot(it.n, wi));
+            } else {
+                R p = hg_p(g, it.outDir, wi);
+                f = V3<R>(p);
+                scatteringPdf = p;
+            }
+            if (f != V3<R>()) {
+                V3<R> Tr = vis_tr(s, it, in, out, lp, use_bvh);
+                weight1 = power_heuristic(pdf, scatteringPdf);
+                L1 = Li * Tr * f * cosTheta / pdf;
+            }
+        }
+    }
+    {
+        V3<R> wi;
+        R pdf, cosTheta = 1;
+        V3<R> f;
+        if (bsdf) {
+            f = bsdf->Sample(it.outDir, &wi, &pdf, v);
+            cosTheta = std::abs(dot(it.n, wi));
+        } else {
+            R p = hg_sample(g, it.outDir, &wi, v.x, v.y);
+            f = V3<R>(p);
+            pdf = p;
+        }
+        if (pdf != 0 && f != V3<R>()) {
+            R lpdf = ltri < 0 ? env_pdf_dir(s.env, wi) : light_pdf(s, ltri, it, wi);
+            if (lpdf == 0) {
+                weight2 = 0;
+            } else {
+                weight2 = power_heuristic(pdf, lpdf);
+                Ray<R> ray = spawn_ray(it, wi);
+                int32_t med = isect_medium(it, in, out, wi);
+                Hit<R> h;
+                V3<R> Tr;
+                if (intersect_tr(s, ray, &med, &h, &Tr, use_bvh)) {
+                    if (prim_light(s, h.prim) == light) {
+                        Isect<R> li = query(s, ray, h);
+                        V3<R> Lemit = dot(li.n, -wi) > 0 ? Le : V3<R>(0, 0, 0);
+                        L2 = Lemit * f * Tr * cosTheta / pdf;
+                    }
+                } else if (ltri < 0) {
+                    L2 = env_emission(s.env, wi) * f * Tr * cosTheta / pdf;
+                }
+            }
+        }
+    }
+    return (L1 * weight1 + L2 * weight2) / lightSamplingPdf;
+}
+
+// VolPathTracing::ConnectToLight (integrator/VPT.cpp:108-171): the light
+// point alone, L1 without its MIS weight.
+template <class R> V3<R> connect_to_light(const Scene<R>& s, const MediumRec<R>& mi, const LightPoint<R>& lp, const V3<R>& Le,
+                                          R lightPdf, bool use_bvh) {
+    R cosTheta = 1;
+    V3<R> dir = mi.pos - lp.pos;
+    V3<R> wi = -norm(dir);
+    if (lp.n != V3<R>()) {
+        cosTheta = dot(norm(dir), lp.n);
+        if (cosTheta < 0) return V3<R>();
+    }
+    const Isect<R> it = medium_isect(mi);
+    V3<R> Tr = vis_tr(s, it, mi.medium, mi.medium, lp, use_bvh);
+    R p = hg_p(s.media[mi.medium].g, mi.wo, wi);
+    V3<R> f = V3<R>(p);
+    return Le * f * Tr * cosTheta / dot(dir, dir) / lightPdf;
+}
+// Triangle::Sample(pdf, u) (shape/Triangle.cpp:141-158): the area sample
+// AreaLight::SampleOnePoint returns, pdf 1 / area.
+template <class R> LightPoint<R> tri_sample_area(const Tri<R>& t, R u0, R u1, R* pdf) {
+    R su0 = std::sqrt(u0);
+    R b0 = 1 - su0, b1 = u1 * su0;
+    R b2 = 1 - b0 - b1;
+    LightPoint<R> lp;
+    lp.pos = (b0 * t.p0 + b1 * t.p1) + b2 * t.p2;
+    lp.fn = t.fn;
+    lp.n = (b0 * t.n0 + b1 * t.n1) + b2 * t.n2;
+    lp.pErr = gamma_n<R>(6) * ((vabs(b0 * t.p0) + vabs(b1 * t.p1)) + vabs(b2 * t.p2));
+    *pdf = 1 / t.area;
+    return lp;
+}
+
+// VolPathTracing::Li.  The dimensions of Vec2(rand(), rand()) /
+// Vec3(rand(), rand(), rand()) arguments are drawn left to right, as for
+// PathTracing.  The roulette test min(1, Y(T) / Y(T)) < 1 never passes.
+template <class R> V3<R> Li_vpt(const Scene<R>& s, Ray<R> ray, Seq& rnd, int maxDepth, bool equiAngular, bool use_bvh) {
+    int32_t medium = -1;
+    bool delta = false;
+    V3<R> T(1, 1, 1);
+    V3<R> L;
+    int crossings = 0;
+    for (int i = 0; i < maxDepth; ++i) {
+        if (rnd.cursor > kVptDimBudget) break;
+        Hit<R> h;
+        const bool hit = intersect(s, ray, &h, use_bvh);
+        if (t_rec && t_rec_n < t_rec_max) {  // the path record of or_li_path, as Li's
+            double* r = t_rec + (size_t)kRecWords * t_rec_n++;
+            r[0] = ray.o.x; r[1] = ray.o.y; r[2] = ray.o.z;
+            r[3] = ray.d.x; r[4] = ray.d.y; r[5] = ray.d.z;
+            r[6] = ray.tmin; r[7] = ray.tmax;
+            r[8] = hit ? h.prim : -1; r[9] = hit ? h.t : 0;
+            r[10] = rnd.cursor;
+            r[11] = T.x; r[12] = T.y; r[13] = T.z;
+            r[14] = medium;
+            r[15] = delta ? 1 : 0;
+        }
+        MediumRec<R> mi;
+        R eqProb = 0, eqPdf = 0;
+        V3<R> eqLe;
+        LightPoint<R> eqLp;
+        if (equiAngular) {
+            const int light = sample_one_light(s, (R)rnd.next(), &eqProb);
+            R a = rnd.next(), b = rnd.next();
+            eqLp = tri_sample_area(s.tris[s.light_tri[light]], a, b, &eqPdf);
+            eqLe = s.light_le[light];
+            if (medium >= 0) T = T * medium_equiangular(s.media[medium], medium, ray, (R)rnd.next(), eqLp.pos, &mi);
+        } else if (medium >= 0) {
+            R a = rnd.next(), b = rnd.next();
+            T = T * medium_sample(s.media[medium], medium, ray, a, b, &mi);
+        }
+        if (T == V3<R>()) break;
+        if (mi.valid) {
+            if (equiAngular) {
+                L = L + T * connect_to_light(s, mi, eqLp, eqLe, eqPdf, use_bvh) / eqProb;
+            } else {
+                R uL = rnd.next();
+                R u0 = rnd.next(), u1 = rnd.next();
+                R v0 = rnd.next(), v1 = rnd.next(), v2 = rnd.next();
+                L = L + T * di_vpt(s, medium_isect(mi), mi.medium, mi.medium, (const BSDF<R>*)nullptr, s.media[mi.medium].g, uL,
+                                   u0, u1, V3<R>(v0, v1, v2), use_bvh);
+            }
+            R a = rnd.next(), b = rnd.next();
+            V3<R> wi;
+            hg_sample(s.media[mi.medium].g, mi.wo, &wi, a, b);
+            ray = spawn_ray(medium_isect(mi), wi);
+            medium = mi.medium;
+            delta = false;
+        } else {
+            if (!hit) {
+                if ((i == 0 || delta) && s.env.w > 0) return T * env_emission(s.env, ray.d);
+                break;
+            }
+            Isect<R> it = query(s, ray, h);
+            int32_t in, out;
+            hit_media(s, h.prim, medium, &in, &out);
+            if (!has_material(s, h.prim)) {
+                if (++crossings > kVptMaxCrossings) break;
+                ray = spawn_ray(it, ray.d);
+                medium = isect_medium(it, in, out, ray.d);
+                --i;
+                continue;
+            }
+            BSDF<R> bsdf = make_bsdf(s, it);
+            int light = prim_light(s, it.prim);
+            if ((i == 0 || delta) && light >= 0) {
+                V3<R> e = dot(it.n, it.outDir) > 0 ? s.light_le[light] : V3<R>(0, 0, 0);
+                L = L + T * e;
+            } else {
+                R uL = rnd.next();
+                R u0 = rnd.next(), u1 = rnd.next();
+                R v0 = rnd.next(), v1 = rnd.next(), v2 = rnd.next();
+                L = L + T * di_vpt(s, it, in, out, &bsdf, (R)0, uL, u0, u1, V3<R>(v0, v1, v2), use_bvh);
+            }
+            R r0 = rnd.next(), r1 = rnd.next(), r2 = rnd.next();
+            V3<R> wi;
+            R pdf;
+            V3<R> f = bsdf.Sample(-ray.d, &wi, &pdf, V3<R>(r0, r1, r2));
+            if (f == V3<R>() || pdf == 0) break;
+            V3<R> est = f * std::abs(dot(it.n, wi)) / pdf;
+            delta = bsdf.delta();
+            T = T * est;
+            ray = spawn_ray(it, wi);
+            medium = isect_medium(it, in, out, wi);
+        }
+    }
+    return L;
+}
+
 template <class R> struct Film {
     int W, H;
     std::vector<R> color, weight;
@@ -2123,7 +2535,31 @@ struct SceneHandle {
     std::vector<float> env;  // environment image, top row first (empty: none)
     int env_w = 0, env_h = 0;
     float env_scale = 1.f;
+    std::vector<or_medium> media;  // or_scene_set_media (empty: none)
+    std::vector<or_tri_medium> tri_media;
 };
+
+template <class R> void attach_media(Scene<R>& s, const SceneHandle* h) {
+    if (h->tri_media.empty()) return;
+    s.media.resize(h->media.size());
+    for (size_t i = 0; i < h->media.size(); ++i) {
+        const or_medium& m = h->media[i];
+        Medium<R>& d = s.media[i];
+        d.sa = V3<R>(m.sigma_a[0], m.sigma_a[1], m.sigma_a[2]);
+        d.ss = V3<R>(m.sigma_s[0], m.sigma_s[1], m.sigma_s[2]);
+        d.st = d.ss + d.sa;
+        d.g = m.g;
+    }
+    const size_t n = h->tri_media.size();
+    s.mi_in.resize(n);
+    s.mi_out.resize(n);
+    s.no_mat.resize(n);
+    for (size_t i = 0; i < n; ++i) {
+        s.mi_in[i] = h->tri_media[i].inside;
+        s.mi_out[i] = h->tri_media[i].outside;
+        s.no_mat[i] = h->tri_media[i].no_material ? 1 : 0;
+    }
+}
 
 // Scene::AddLight of an EnvironmentLight after the area lights, then
 // Scene::Initialize: light init (bounding sphere), power distribution.
@@ -2174,6 +2610,7 @@ template <> Scene<double>& scene_of<double>(SceneHandle* h) {
     if (!h->sd) {
         h->sd.reset(new Scene<double>());
         fill_scene(*h->sd, h->desc);
+        attach_media(*h->sd, h);
         attach_env(*h->sd, h);
     }
     return *h->sd;
@@ -2197,7 +2634,7 @@ template <class R> int li_samples_t(SceneHandle* h, const or_camera_desc* cd, co
                 R fx, fy;
                 int dims;
                 V3<R> L = camera_sample(sc, cam, res, m, x[k], y[k], s[k], p->max_depth, p->rr_depth, p->use_bvh != 0, &fx, &fy,
-                                        &dims, p->sampler == 1 ? &henum : nullptr);
+                                        &dims, p->sampler == 1 ? &henum : nullptr, p->integrator);
                 out_L[3 * k] = L.x; out_L[3 * k + 1] = L.y; out_L[3 * k + 2] = L.z;
                 if (out_raster) { out_raster[2 * k] = (float)fx; out_raster[2 * k + 1] = (float)fy; }
                 if (out_dims) out_dims[k] = dims;
@@ -2248,7 +2685,7 @@ template <class R> int64_t render_t(SceneHandle* h, const or_camera_desc* cd, co
                     for (int smp = 0; smp < p->spp; ++smp) {
                         R fx, fy;
                         V3<R> L = camera_sample(sc, cam, res, m, x, y, smp, p->max_depth, p->rr_depth, p->use_bvh != 0, &fx, &fy,
-                                                (int*)nullptr, p->sampler == 1 ? &henum : nullptr);
+                                                (int*)nullptr, p->sampler == 1 ? &henum : nullptr, p->integrator);
                         film.add(fx, fy, L);
                         ++local;
                     }
@@ -2339,6 +2776,45 @@ template <class R> static int sphere_light_probe(Scene<R>& s, int32_t i, const d
     return 0;
 }
 
+template <class R> static int hg_probe_t(double g, const double* wo, const double* u, double* wi, double* p) {
+    V3<R> w((R)wo[0], (R)wo[1], (R)wo[2]), o;
+    R v = hg_sample<R>((R)g, w, &o, (R)u[0], (R)u[1]);
+    wi[0] = o.x; wi[1] = o.y; wi[2] = o.z;
+    p[0] = v;
+    p[1] = hg_p<R>((R)g, w, o);
+    return 0;
+}
+template <class R> static int medium_probe_t(int kind, const double* sa, const double* ss, const double* o, const double* d,
+                                             double tmax, const double* u, const double* lp, double* out) {
+    Medium<R> m;
+    m.sa = V3<R>((R)sa[0], (R)sa[1], (R)sa[2]);
+    m.ss = V3<R>((R)ss[0], (R)ss[1], (R)ss[2]);
+    m.st = m.ss + m.sa;
+    Ray<R> ray{V3<R>((R)o[0], (R)o[1], (R)o[2]), V3<R>((R)d[0], (R)d[1], (R)d[2]), (R)0, (R)tmax};
+    MediumRec<R> mi;
+    R t = 0, pdf = 0;
+    V3<R> w;
+    if (kind == 0) w = medium_sample<R>(m, 0, ray, (R)u[0], (R)u[1], &mi, &t);
+    else if (kind == 1) w = medium_equiangular<R>(m, 0, ray, (R)u[0], V3<R>((R)lp[0], (R)lp[1], (R)lp[2]), &mi, &t, &pdf);
+    else w = medium_tr<R>(m, ray);
+    out[0] = t;
+    out[1] = mi.valid ? 1 : 0;
+    out[2] = w.x; out[3] = w.y; out[4] = w.z;
+    out[5] = pdf;
+    return 0;
+}
+// Test hook: Scene::IntersectTr from o along d (through the Ray constructor)
+// in the given medium: out = hit primitive (-1: none), Tr[3].
+template <class R> static int vpt_tr_probe(Scene<R>& s, const double* o, const double* d, int32_t medium, double* out) {
+    Ray<R> ray = make_ray(V3<R>((R)o[0], (R)o[1], (R)o[2]), V3<R>((R)d[0], (R)d[1], (R)d[2]), K<R>::Inf, (R)0);
+    int32_t med = medium;
+    Hit<R> h;
+    V3<R> Tr;
+    const bool hit = intersect_tr(s, ray, &med, &h, &Tr, true);
+    out[0] = hit ? h.prim : -1;
+    out[1] = Tr.x; out[2] = Tr.y; out[3] = Tr.z;
+    return 0;
+}
 extern "C" {
 
 int or_load_sobol(const char* dir) {
@@ -2572,10 +3048,41 @@ int or_sphere_light_probe(void* hv, int32_t i, int precision, const double* ref,
                      : sphere_light_probe(h->sf, i, ref, u, wi, out);
 }
 
+int or_scene_set_media(void* hv, int32_t n_media, const or_medium* media, int32_t n_tri, const or_tri_medium* tri) {
+    SceneHandle* h = (SceneHandle*)hv;
+    if (n_media < 0 || (n_media > 0 && !media) || n_tri != (int32_t)h->sf.tris.size() || !tri) return -1;
+    for (int32_t i = 0; i < n_tri; ++i)
+        if (tri[i].inside < -1 || tri[i].inside >= n_media || tri[i].outside < -1 || tri[i].outside >= n_media) return -1;
+    h->media.assign(media, media + n_media);
+    h->tri_media.assign(tri, tri + n_tri);
+    attach_media(h->sf, h);
+    if (h->sd) attach_media(*h->sd, h);
+    return 0;
+}
+
+int or_hg_probe(int precision, double g, const double* wo, const double* u, double* wi, double* p) {
+    return precision ? hg_probe_t<double>(g, wo, u, wi, p) : hg_probe_t<float>(g, wo, u, wi, p);
+}
+int or_medium_probe(int precision, int kind, const double* sigma_a, const double* sigma_s, const double* o, const double* d,
+                    double tmax, const double* u, const double* light_pos, double* out) {
+    return precision ? medium_probe_t<double>(kind, sigma_a, sigma_s, o, d, tmax, u, light_pos, out)
+                     : medium_probe_t<float>(kind, sigma_a, sigma_s, o, d, tmax, u, light_pos, out);
+}
+
 int or_set_detached(void* hv, const float* tex_eval, const float* le_eval) {
     SceneHandle* h = (SceneHandle*)hv;
     set_detached(h->sf, h->desc, tex_eval, le_eval);
     set_detached(scene_of<double>(h), h->desc, tex_eval, le_eval);
+    return 0;
+}
+
+int or_vpt_intersect_tr(void* hv, int precision, const double* o, const double* d, int32_t medium, double* out) {
+    SceneHandle* h = (SceneHandle*)hv;
+    return precision ? vpt_tr_probe(scene_of<double>(h), o, d, medium, out) : vpt_tr_probe(h->sf, o, d, medium, out);
+}
+
+int or_set_vpt_query_crossings(int on) {
+    g_vpt_query_crossings = on;
     return 0;
 }
 

@@ -98,6 +98,7 @@ typedef struct {
     int32_t nthreads;    /* tile worker threads (0 = hardware concurrency) */
     int32_t use_bvh;     /* 1 = reference BVH traversal, 0 = brute force */
     int32_t sampler;     /* 0 = SobolSampler + SobolEnum, 1 = HaltonSampler (Faure) + HaltonEnum */
+    int32_t integrator;  /* 0 = PathTracing, 1 = VolPathTracing (free-flight sampling), 2 = VolPathTracing (equi-angular) */
 } or_render_params;
 
 /* Sobol tables (fixtures extracted from the reference's data tables). */
@@ -124,6 +125,32 @@ typedef struct {
 void* or_scene_create(const or_scene_desc* desc);
 /* Appends spheres (primitive ids after the triangles); before any environment light. */
 int or_scene_set_spheres(void* scene, int32_t n, const or_sphere* spheres);
+/* Participating media for VolPathTracing: HomogeneousMedium(sigma_a, sigma_s, g)
+ * (volume/Homogeneous.h:9-24) and, per triangle, its MediumInterface(inside,
+ * outside) (-1: none) and whether it has no material (a medium boundary). */
+typedef struct {
+    float sigma_a[3];
+    float sigma_s[3];
+    float g;
+} or_medium;
+typedef struct {
+    int32_t inside, outside, no_material;
+} or_tri_medium;
+int or_scene_set_media(void* scene, int32_t n_media, const or_medium* media, int32_t n_tri, const or_tri_medium* tri);
+/* Test hooks (precision 0: FP32, 1: FP64).  HenyeyGreenstein::Sample_p(wo, u):
+ * wi[3], p[0] = the returned value, p[1] = p(wo, wi).  Medium probe for one ray
+ * (origin as given, no RayEps shift), kind 0: Sample (u = channel, distance),
+ * 1: EquiAngularSampling towards light_pos (u[0]), 2: Tr; out = t, sampled,
+ * weight[3], pdf (kind 1). */
+/* Test-only: 1 = Scene::IntersectTr queries each crossed material-less surface
+ * and continues in the medium beyond its normal (0 = the reference's rule). */
+int or_set_vpt_query_crossings(int on);
+/* Test hook: Scene::IntersectTr from o along d (origin moved RayEps by the Ray
+ * constructor) starting in medium (-1: none); out = hit primitive (-1), Tr[3]. */
+int or_vpt_intersect_tr(void* scene, int precision, const double* o, const double* d, int32_t medium, double* out);
+int or_hg_probe(int precision, double g, const double* wo, const double* u, double* wi, double* p);
+int or_medium_probe(int precision, int kind, const double* sigma_a, const double* sigma_s, const double* o, const double* d,
+                    double tmax, const double* u, const double* light_pos, double* out);
 /* Test hook: from a shading point at ref (zero error bound), sphere i's light
  * sample at u = (u0, u1) and Pdf_Li towards wi; out = pos[3], n[3], pdf, pdf_li.
  * precision 0: FP32, 1: FP64. */

@@ -56,7 +56,13 @@ class ORParams(C.Structure):
         ("nthreads", C.c_int32),
         ("use_bvh", C.c_int32),
         ("sampler", C.c_int32),
+        ("integrator", C.c_int32),
     ]
+
+
+# or_medium / or_tri_medium
+MEDIUM_DTYPE = np.dtype([("sigma_a", "<f4", 3), ("sigma_s", "<f4", 3), ("g", "<f4")])
+TRI_MEDIUM_DTYPE = np.dtype([("inside", "<i4"), ("outside", "<i4"), ("no_material", "<i4")])
 
 
 _lib = None
@@ -106,6 +112,11 @@ def lib():
         "or_scene_set_environment": (C.c_int, [vp, vp, i32, i32, f32]),
         "or_scene_set_spheres": (C.c_int, [vp, i32, vp]),
         "or_sphere_light_probe": (C.c_int, [vp, i32, C.c_int, vp, vp, vp, vp]),
+        "or_scene_set_media": (C.c_int, [vp, i32, vp, i32, vp]),
+        "or_set_vpt_query_crossings": (C.c_int, [C.c_int]),
+        "or_vpt_intersect_tr": (C.c_int, [vp, C.c_int, vp, vp, i32, vp]),
+        "or_hg_probe": (C.c_int, [C.c_int, C.c_double, vp, vp, vp, vp]),
+        "or_medium_probe": (C.c_int, [C.c_int, C.c_int, vp, vp, vp, vp, C.c_double, vp, vp, vp]),
         "or_env_query": (C.c_int, [vp, i32, vp, vp, vp, vp, vp, vp, C.c_int]),
         "or_env_info": (C.c_int, [vp, vp, vp, i32]),
         "or_render_lt": (C.c_int, [vp, vp, vp, vp]),
@@ -134,11 +145,34 @@ def camera_from(cam):
     return o
 
 
-def params(spp, max_depth, rr_depth, precision=0, nthreads=0, use_bvh=1, sampler=0):
+def params(spp, max_depth, rr_depth, precision=0, nthreads=0, use_bvh=1, sampler=0, integrator=0):
+    """integrator 0: PathTracing; 1: VolPathTracing, free-flight sampling;
+    2: VolPathTracing, equi-angular sampling."""
     p = ORParams()
     p.spp, p.max_depth, p.rr_depth, p.precision, p.nthreads, p.use_bvh = spp, max_depth, rr_depth, precision, nthreads, use_bvh
-    p.sampler = sampler
+    p.sampler, p.integrator = sampler, integrator
     return p
+
+
+def hg_probe(g, wo, u, precision=1):
+    """HenyeyGreenstein::Sample_p(wo, u): (wi, returned value, p(wo, wi))."""
+    wo = np.ascontiguousarray(wo, np.float64)
+    u = np.ascontiguousarray(u, np.float64)
+    wi = np.zeros(3, np.float64)
+    pv = np.zeros(2, np.float64)
+    lib().or_hg_probe(int(precision), float(g), _p(wo), _p(u), _p(wi), _p(pv))
+    return wi, pv[0], pv[1]
+
+
+def medium_probe(kind, sigma_a, sigma_s, o, d, tmax, u=(0.0, 0.0), light_pos=(0.0, 0.0, 0.0), precision=1):
+    """kind 0: HomogeneousMedium::Sample (u = channel, distance); 1:
+    EquiAngularSampling towards light_pos (u[0]); 2: Tr.  Returns (t, sampled,
+    weight[3], pdf)."""
+    a = [np.ascontiguousarray(x, np.float64) for x in (sigma_a, sigma_s, o, d, u, light_pos)]
+    out = np.zeros(6, np.float64)
+    lib().or_medium_probe(int(precision), int(kind), _p(a[0]), _p(a[1]), _p(a[2]), _p(a[3]), float(tmax), _p(a[4]),
+                          _p(a[5]), _p(out))
+    return out[0], bool(out[1]), out[2:5], out[5]
 
 
 def sobol_sample(index, dim):
@@ -257,6 +291,23 @@ class OracleScene:
         assert self._sph.dtype.itemsize == 40
         rc = lib().or_scene_set_spheres(self.h, self._sph.shape[0], _p(self._sph))
         assert rc == 0
+
+    def set_media(self, media, tri):
+        """HomogeneousMedium list (MEDIUM_DTYPE) and per-triangle interface
+        (TRI_MEDIUM_DTYPE: inside, outside, -1 none; no_material)."""
+        self._media = np.ascontiguousarray(media, MEDIUM_DTYPE)
+        self._tri_media = np.ascontiguousarray(tri, TRI_MEDIUM_DTYPE)
+        rc = lib().or_scene_set_media(self.h, self._media.shape[0], _p(self._media), self._tri_media.shape[0],
+                                      _p(self._tri_media))
+        assert rc == 0, rc
+
+    def intersect_tr(self, o, d, medium, precision=1):
+        """Scene::IntersectTr from o along d in medium: (hit primitive or -1, Tr)."""
+        o = np.ascontiguousarray(o, np.float64)
+        d = np.ascontiguousarray(d, np.float64)
+        out = np.zeros(4, np.float64)
+        lib().or_vpt_intersect_tr(self.h, int(precision), _p(o), _p(d), int(medium), _p(out))
+        return int(out[0]), out[1:4]
 
     def sphere_light_probe(self, i, ref, u, wi, precision=1):
         """Sphere i's Sample(Intersection, pdf, u) from a shading point at ref

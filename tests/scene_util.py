@@ -408,3 +408,64 @@ def reference_sphere_cornell(width=64, height=48, spp=16):
     cam = g.camera(tuple(pos), tuple(pos + cz), tuple(cy), 28.7993, width, height, float(np.linalg.norm(cz)))
     prm = g.render_params(spp=spp, max_depth=5, rr_depth=3)
     return sa, cam, prm, sp
+
+
+def _medium_arrays(n_tri_room, n_cube, medium_id=0):
+    from oracle.oracle import TRI_MEDIUM_DTYPE
+
+    tri = np.zeros(n_tri_room + n_cube, TRI_MEDIUM_DTYPE)
+    tri["inside"], tri["outside"] = -1, -1
+    tri["inside"][n_tri_room:] = medium_id
+    tri["no_material"][n_tri_room:] = 1
+    return tri
+
+
+def furnace_medium_scene(sigma_s=(0.5, 0.5, 0.5), sigma_a=(0.0, 0.0, 0.0), g=0.0, res=16, spp=64, depth=12):
+    """The furnace box (12 inward-facing black emitters, Le = 1) holding a
+    material-less cube [0.25, 0.75]^3 (outward normals) filled with a
+    homogeneous medium; the pinhole camera sits inside the box, outside the
+    cube, looking through it.  Returns (arrays, camera, params, media, tri)."""
+    import gyunity_amd as g_
+    from oracle.oracle import MEDIUM_DTYPE
+
+    room, room_n = box_triangles((0, 0, 0), (1, 1, 1), inward=True)
+    cube, cube_n = box_triangles((0.25, 0.25, 0.25), (0.75, 0.75, 0.75), inward=False)
+    pos = np.concatenate([room, cube])
+    fn = np.concatenate([room_n, cube_n])
+    nrm = np.repeat(fn, 3, axis=0).reshape(-1, 9)
+    n_room, n = room.shape[0], pos.shape[0]
+    mats = material(0, R=(0.0, 0.0, 0.0))[None, :]
+    light = np.full(n, -1, np.int32)
+    light[:n_room] = np.arange(n_room)
+    sa = SceneArrays(pos, nrm, fn, np.zeros(n, np.uint32), light, mats, np.arange(n_room, dtype=np.int32),
+                     np.ones((n_room, 3), np.float32))
+    media = np.zeros(1, MEDIUM_DTYPE)
+    media["sigma_a"], media["sigma_s"], media["g"] = sigma_a, sigma_s, g
+    tri = _medium_arrays(n_room, n - n_room)
+    cam = g_.camera((0.5, 0.5, 0.97), (0.5, 0.5, 0.0), (0.0, 1.0, 0.0), 70.0, res, res)
+    prm = g_.render_params(spp=spp, max_depth=depth, rr_depth=depth)
+    return sa, cam, prm, media, tri
+
+
+def medium_cornell(res=64, spp=16, depth=8):
+    """The Cornell room (36 triangles) with a material-less slab
+    [0.15, 0.85] x [0.62, 0.92] x [0.1, 0.9] (outward normals) of coloured,
+    forward-scattering medium between the boxes and the light."""
+    import gyunity_amd as g_
+    from oracle.oracle import MEDIUM_DTYPE
+
+    sa0, cam, prm = g_.generate_scene(g_.GY_SCENE_CORNELL)
+    cube, cube_n = box_triangles((0.15, 0.62, 0.1), (0.85, 0.92, 0.9), inward=False)
+    n0, nc = sa0.pos.shape[0], cube.shape[0]
+    pos = np.concatenate([sa0.pos, cube])
+    nrm = np.concatenate([sa0.nrm, np.repeat(cube_n, 3, axis=0).reshape(-1, 9)])
+    fn = np.concatenate([sa0.face_n, cube_n])
+    mat = np.concatenate([sa0.mat, np.zeros(nc, np.uint32)])
+    light = np.concatenate([sa0.light, np.full(nc, -1, np.int32)])
+    sa = SceneArrays(pos, nrm, fn, mat, light, sa0.mats, sa0.light_tri, sa0.light_le)
+    media = np.zeros(1, MEDIUM_DTYPE)
+    media["sigma_a"], media["sigma_s"], media["g"] = (0.3, 0.6, 0.9), (2.0, 1.5, 1.0), 0.5
+    tri = _medium_arrays(n0, nc)
+    cam.width = cam.height = res
+    prm.spp, prm.max_depth, prm.rr_depth = spp, depth, depth
+    return sa, cam, prm, media, tri
