@@ -191,6 +191,112 @@ def lt_main(args):
     print(json.dumps(out), flush=True)
 
 
+def fog_box(g, sa, lo=(0.06, 0.01, 0.06), hi=(0.94, 0.9, 0.94), sigma_a=(0.05, 0.05, 0.08), sigma_s=(0.8, 0.7, 0.6),
+            gg=0.3):
+    """The scene with a material-less box of homogeneous fog (outward normals,
+    inside = medium 0, outside = none) around the icosphere, below the light."""
+    x0, y0, z0 = lo
+    x1, y1, z1 = hi
+    quads = [((x0, y0, z0), (x1, y0, z0), (x1, y0, z1), (x0, y0, z1), (0, -1, 0)),
+             ((x0, y1, z0), (x0, y1, z1), (x1, y1, z1), (x1, y1, z0), (0, 1, 0)),
+             ((x0, y0, z0), (x0, y0, z1), (x0, y1, z1), (x0, y1, z0), (-1, 0, 0)),
+             ((x1, y0, z0), (x1, y1, z0), (x1, y1, z1), (x1, y0, z1), (1, 0, 0)),
+             ((x0, y0, z0), (x0, y1, z0), (x1, y1, z0), (x1, y0, z0), (0, 0, -1)),
+             ((x0, y0, z1), (x1, y0, z1), (x1, y1, z1), (x0, y1, z1), (0, 0, 1))]
+    tris, normals = [], []
+    for a, b, c, d, n in quads:
+        tris += [(a, b, c), (a, c, d)]
+        normals += [n, n]
+    box = np.array(tris, np.float32).reshape(-1, 9)
+    bn = np.array(normals, np.float32)
+    nb = box.shape[0]
+    out = g.SceneArrays(np.concatenate([sa.pos, box]), np.concatenate([sa.nrm, np.repeat(bn, 3, axis=0).reshape(-1, 9)]),
+                        np.concatenate([sa.face_n, bn]), np.concatenate([sa.mat, np.zeros(nb, np.uint32)]),
+                        np.concatenate([sa.light, np.full(nb, -1, np.int32)]), sa.mats, sa.light_tri, sa.light_le,
+                        uv=np.concatenate([sa.uv, np.zeros((nb, 6), np.float32)]) if sa.uv is not None else None)
+    media = np.zeros(1, g.MEDIUM_DTYPE)
+    media["sigma_a"], media["sigma_s"], media["g"] = sigma_a, sigma_s, gg
+    tri = np.zeros(out.n_tri, g.TRI_MEDIUM_DTYPE)
+    tri["inside"], tri["outside"] = -1, -1
+    tri["inside"][sa.n_tri:] = 0
+    tri["no_material"][sa.n_tri:] = 1
+    return out, media, tri
+
+
+def vpt_main(args):
+    """VolPathTracing (gy_render_vpt) on the icosphere workload inside a box of
+    homogeneous fog: the same camera, spp and depth; equi-angular sampling (the
+    reference's default) and free-flight sampling."""
+    import gyunity_amd as g
+
+    sa0, cam, prm = g.generate_scene(g.GY_SCENE_ICOSPHERE_GGX, 0, args.level)
+    sa, media, tri = fog_box(g, sa0)
+    paths = cam.width * cam.height * prm.spp
+    ctx = g.Context(0)
+    log("device:", ctx.device_name())
+    sc = g.Scene(ctx, sa)
+    sc.set_media(media, tri)
+    res = {}
+    for eqa in (True, False):
+        name = "equiangular" if eqa else "free_flight"
+        for i in range(args.warmup):
+            sc.render_vpt(cam, prm, equiangular=eqa)
+            log("vpt %s warmup %d done" % (name, i))
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            c, w = sc.render_vpt(cam, prm, equiangular=eqa)
+            log("vpt %s step %d done" % (name, i))
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        res[name] = {"ms_per_step": round(ms, 3), "mpaths_per_s": round(paths / (ms / 1e3) / 1e6, 3),
+                     "finite_fraction": float(np.isfinite(c).mean()), "mean": float((c / np.maximum(w, 1)[..., None]).mean())}
+    value = res["equiangular"]["mpaths_per_s"]
+    out = {
+        "metric": "Mpaths/s at fixed RMSE: batched wavefront vs plain megakernel; adjoint s/grad",
+        "value": value,
+        "unit": "Mpaths/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": res["equiangular"]["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: procedural displaced icosphere (seed 0) in the Cornell room, in a box of homogeneous fog",
+        "config": {
+            "workload": "VolPathTracing, icosphere-L%d GGX conductor, %d triangles (12 material-less fog-box walls), %dx%d, %d spp, "
+                        "depth %d, fog sigma_a %s sigma_s %s g %.2f"
+                        % (args.level, sa.n_tri, cam.width, cam.height, prm.spp, prm.max_depth,
+                           [round(float(v), 3) for v in media["sigma_a"][0]], [round(float(v), 3) for v in media["sigma_s"][0]],
+                           float(media["g"][0])),
+            "integrator": "VolPathTracing per path (one path per thread)",
+            "forms": res,
+            "paths_per_step": paths,
+            "parallelism": "single GPU",
+        },
+    }
+    sc.close()
+    ctx.close()
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+
+        log("cpu baseline ...")
+        model, logical, allowed = host_cpu()
+        cores = max(1, min(CPU_SHARE, allowed))
+        rows = 32
+        r0 = cam.height // 2 - rows // 2
+        os_ = O.OracleScene(sa)
+        os_.set_media(media, tri)
+        t0 = time.perf_counter()
+        _, _, npaths, _ = os_.render(cam, O.params(prm.spp, prm.max_depth, prm.rr_depth, 0, cores, 1, integrator=2), r0, r0 + rows)
+        secs = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(npaths / secs / 1e6, 4), "unit": "Mpaths/s", "cores": cores, "kind": "port",
+                               "sample": "rows %d-%d, equi-angular (%d paths, %.1f s), same scene, spp and depth"
+                               % (r0, r0 + rows - 1, npaths, secs),
+                               "cpu_model": model, "host_logical_cpus": logical, "cpus_allowed": allowed}
+    print(json.dumps(out), flush=True)
+
+
 def batch_main(args):
     """The 64-scene batch workload: every member traced in one lane space
     (gy_batch_render, recorded into a hipGraph once and replayed), against the
@@ -562,7 +668,7 @@ def main():
     ap.add_argument("--refill", type=int, default=0, help="wavefront refill threshold (0 = library default)")
     ap.add_argument("--spp-pass", type=int, default=0, help="samples per pass (0 = library default)")
     ap.add_argument("--leaf", type=int, default=0, help="BVH leaf size (0 = library default, 4)")
-    ap.add_argument("--workload", default="icosphere", choices=["icosphere", "env", "lt", "batch64", "adjoint", "adjoint-scale"],
+    ap.add_argument("--workload", default="icosphere", choices=["icosphere", "env", "lt", "vpt", "batch64", "adjoint", "adjoint-scale"],
                     help="icosphere: the 1.31M-triangle GGX render; batch64: the 64-scene batch; adjoint: "
                          "gradient of the textured Cornell room; adjoint-scale: Adam on 4.2M texels over 32 views")
     ap.add_argument("--fd-texels", type=int, default=0, help="adjoint: texels checked against finite differences")
@@ -593,6 +699,9 @@ def main():
         return
     if args.workload == "lt":
         lt_main(args)
+        return
+    if args.workload == "vpt":
+        vpt_main(args)
         return
 
     import gyunity_amd as g
