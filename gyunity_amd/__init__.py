@@ -24,6 +24,7 @@ from ._lib import (  # noqa: F401
     image_write,
     load_obj,
     orbit_views,
+    sppm_params,
     render_params,
     sobol_enum_index,
     sobol_sample,

@@ -95,6 +95,26 @@ class SceneStats(C.Structure):
     ]
 
 
+class SppmParams(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int32),
+        ("max_depth", C.c_int32),
+        ("photons_per_iteration", C.c_int64),
+        ("initial_radius", C.c_float),
+        ("alpha", C.c_float),
+        ("batch_shrink", C.c_int32),
+        ("sampler", C.c_int32),
+        ("trace_glossy_ray", C.c_int32),
+    ]
+
+
+def sppm_params(iterations, photons_per_iteration, max_depth, initial_radius, alpha, batch_shrink=False, sampler=0):
+    p = SppmParams()
+    p.iterations, p.photons_per_iteration, p.max_depth = iterations, photons_per_iteration, max_depth
+    p.initial_radius, p.alpha, p.batch_shrink, p.sampler = initial_radius, alpha, 1 if batch_shrink else 0, sampler
+    return p
+
+
 class RenderStats(C.Structure):
     _fields_ = [
         ("paths", C.c_int64),
@@ -186,6 +206,7 @@ def lib():
         "gy_kernel_name": (C.c_char_p, [C.c_int]),
         "gy_li_samples": (C.c_int, [vp, P(CameraDesc), P(RenderParams), i32, P(i32), P(i32), P(i32), P(f32), P(f32), P(i32)]),
         "gy_scene_set_media": (C.c_int, [vp, i32, vp, vp]),
+        "gy_render_sppm": (C.c_int, [vp, P(CameraDesc), P(SppmParams), vp, vp, vp]),
         "gy_render_vpt": (C.c_int, [vp, P(CameraDesc), P(RenderParams), i32, vp, vp]),
         "gy_vpt_li_samples": (C.c_int, [vp, P(CameraDesc), P(RenderParams), i32, i32, P(i32), P(i32), P(i32), P(f32), P(f32),
                                         P(i32)]),
@@ -668,6 +689,15 @@ class Scene:
         check(lib().gy_render_vpt(self.h, C.byref(cam), C.byref(params), 1 if equiangular else 0, color.ctypes.data,
                                   weight.ctypes.data))
         return color, weight
+
+    def render_sppm(self, cam, sp):
+        """SPPM (gy_render_sppm): (image (H, W, 3), radius^2 (H, W), photon count N (H, W))."""
+        W, H = cam.width, cam.height
+        img = np.zeros((H, W, 3), dtype=np.float32)
+        r2 = np.zeros((H, W), dtype=np.float32)
+        nn = np.zeros((H, W), dtype=np.float32)
+        check(lib().gy_render_sppm(self.h, C.byref(cam), C.byref(sp), img.ctypes.data, r2.ctypes.data, nn.ctypes.data))
+        return img, r2, nn
 
     def vpt_li_samples(self, cam, params, x, y, s, equiangular=True):
         x = np.ascontiguousarray(x, dtype=np.int32)

@@ -202,6 +202,25 @@ int gy_scene_set_media(gy_scene* scene, int32_t n_media, const gy_medium* media,
  * supported: spheres; equi-angular sampling with an environment light. */
 int gy_render_vpt(gy_scene* scene, const gy_camera_desc* cam, const gy_render_params* params, int32_t equiangular,
                   float* color, float* weight);
+/* SPPM(iterations, nPhotonsPerStage, maxDepth, initialRadius, alpha, batchShrink,
+ * sampler, samplerEnum, traceGlossyRay) (integrator/SPPM.h:48-49, SPPM.cpp:198-277):
+ * rgb = width x height x 3, the image SPPM::Render gives Film::SetImage;
+ * radius2 and photon_count (NULL: not wanted) = each pixel's final radius^2
+ * and photon count N.  Triangle scenes with area lights; traceGlossyRay is not
+ * supported.  The reference updates hit points from concurrent photons without
+ * locks; here each hit point applies its photons in photon-index order. */
+typedef struct {
+    int32_t iterations;
+    int32_t max_depth;
+    int64_t photons_per_iteration;
+    float initial_radius;
+    float alpha;
+    int32_t batch_shrink;
+    int32_t sampler;          /* GY_SAMPLER_SOBOL or GY_SAMPLER_HALTON (sampler and enumeration) */
+    int32_t trace_glossy_ray; /* must be 0 */
+} gy_sppm_params;
+int gy_render_sppm(gy_scene* scene, const gy_camera_desc* cam, const gy_sppm_params* params, float* rgb, float* radius2,
+                   float* photon_count);
 /* Per-sample VolPathTracing::Li on n (x, y, sample) indices: radiance (3n),
  * film position (2n) and dimensions drawn (n). */
 int gy_vpt_li_samples(gy_scene* scene, const gy_camera_desc* cam, const gy_render_params* params, int32_t equiangular,

@@ -2616,6 +2616,260 @@ template <> Scene<double>& scene_of<double>(SceneHandle* h) {
     return *h->sd;
 }
 
+// ---------------------------------------------------------------- SPPM
+// Stochastic progressive photon mapping (integrator/SPPM.cpp:11-277) with its
+// spatial hash (integrator/HashGrid.h:11-125), for triangle scenes with area
+// lights.  The reference runs eye paths and photons in parallel and updates
+// hit points from concurrent photons without locks; here photons update the
+// hit points one at a time in photon-index order (and, within a photon, in
+// vertex order), the canonical order of the same arithmetic.
+template <class R> struct SppmState {
+    int W = 0, H = 0;
+    std::vector<R> r2, N;
+    std::vector<V3<R>> flux, direct;
+    std::vector<V3<R>> imp, pos, nrm, out;  // HPoint: importance, pos, nrm (mAbsNormal), outDir
+    std::vector<uint8_t> used;
+    std::vector<int64_t> m;
+};
+// SPPM::TraceEyePath (SPPM.cpp:45-108) for pixel (x, y) of iteration iter; the
+// sample index is SamplerEnum::GetIndex(iter, x, y) (SPPM.cpp:212).
+template <class R> void sppm_eye(const Scene<R>& s, const Camera<R>& cam, int res, int m, const hal::Enum* henum, int x, int y,
+                                 int iter, int maxDepth, SppmState<R>& st) {
+    const int64_t px = x + (int64_t)y * st.W;
+    st.used[px] = 0;
+    Seq rnd;
+    R u, v;
+    if (henum) {
+        rnd.kind = 1;
+        rnd.instance = henum->index((uint64_t)iter, (uint32_t)x, (uint32_t)y);
+        u = (R)rnd.next() * (R)henum->w - R(x);
+        v = (R)rnd.next() * (R)henum->h - R(y);
+    } else {
+        rnd.instance = enum_look_up((uint32_t)m, (uint32_t)iter, (uint32_t)x, (uint32_t)y);
+        u = res * (R)rnd.next() - R(x);
+        v = res * (R)rnd.next() - R(y);
+    }
+    Ray<R> r = generate_ray(cam, x, y, u, v);
+    bool delta = false;
+    V3<R> importance(1.0, 1.0, 1.0);
+    for (int i = 0; i < maxDepth; ++i) {
+        Hit<R> h;
+        if (!intersect(s, r, &h, true)) {
+            if ((i == 0 || delta) && s.env.w > 0) st.direct[px] = st.direct[px] + importance * env_emission(s.env, r.d);
+            return;
+        }
+        Isect<R> it = query(s, r, h);
+        BSDF<R> bsdf = make_bsdf(s, it);
+        if (bsdf.delta()) {
+            R a = rnd.next(), b = rnd.next(), c = rnd.next();
+            V3<R> wi;
+            R pdf;
+            V3<R> f = bsdf.Sample((R)-1 * r.d, &wi, &pdf, V3<R>(a, b, c));
+            if (f == V3<R>() || pdf == 0) return;
+            importance = f * std::abs(dot(wi, it.n)) * importance / pdf;
+            r = spawn_ray(it, wi);
+            delta = true;
+        } else {
+            st.used[px] = 1;
+            st.imp[px] = importance;
+            st.pos[px] = it.pos;
+            st.nrm[px] = it.absN;
+            st.out[px] = (R)-1 * r.d;
+            const int light = prim_light(s, it.prim);
+            if ((i == 0 || delta) && light >= 0) {
+                V3<R> e = dot(it.n, it.outDir) > 0 ? s.light_le[light] : V3<R>(0, 0, 0);
+                st.direct[px] = st.direct[px] + importance * e;
+            } else {
+                R uL = rnd.next();
+                R u0 = rnd.next(), u1 = rnd.next();
+                R v0 = rnd.next(), v1 = rnd.next(), v2 = rnd.next();
+                V3<R> Ld = importance * direct_illumination(s, it, bsdf, uL, u0, u1, V3<R>(v0, v1, v2), true);
+                st.direct[px] = st.direct[px] + Ld;
+            }
+            return;
+        }
+    }
+}
+// HashGrid: buckets = used hit points, cells of twice the search radius over
+// the hit points' box inflated by it; a hit point is listed in the bucket of
+// every cell its own radius box touches (a bucket may list it more than once).
+template <class R> struct SppmGrid {
+    std::vector<std::vector<int64_t>> buckets;
+    int64_t num = 0;
+    R inv = 0;
+    V3<R> bmin;
+};
+inline uint32_t sppm_hash(int ix, int iy, int iz, int64_t num) {  // HashGrid::hash: int products, wrapping
+    const uint32_t h = ((uint32_t)ix * 73856093u) ^ ((uint32_t)iy * 19349663u) ^ ((uint32_t)iz * 83492791u);
+    return (uint32_t)((int64_t)h % num);
+}
+template <class R> void sppm_grid_build(const SppmState<R>& st, SppmGrid<R>* g) {
+    R maxR2 = 0;
+    int64_t n = 0;
+    for (size_t i = 0; i < st.used.size(); ++i)
+        if (st.used[i]) {
+            maxR2 = rmax(maxR2, st.r2[i]);
+            ++n;
+        }
+    const R search = std::sqrt(maxR2) + (R)1e-6f;  // Eps (common/Core.h:191)
+    V3<R> mn(K<R>::Inf, K<R>::Inf, K<R>::Inf);
+    for (size_t i = 0; i < st.used.size(); ++i)
+        if (st.used[i]) {
+            const V3<R> a = st.pos[i] - V3<R>(search), b = st.pos[i] + V3<R>(search);
+            for (int k = 0; k < 3; ++k) {
+                if (a[k] < mn[k]) mn[k] = a[k];
+                if (b[k] < mn[k]) mn[k] = b[k];
+            }
+        }
+    g->bmin = mn;
+    g->inv = 1.f / (search * 2.f);
+    g->num = n;
+    g->buckets.assign((size_t)n, {});
+    if (n == 0) return;
+    for (size_t i = 0; i < st.used.size(); ++i) {
+        if (!st.used[i]) continue;
+        const R rad = std::sqrt(st.r2[i]);
+        const V3<R> BMin = ((st.pos[i] - V3<R>(rad)) - mn) * g->inv;
+        const V3<R> BMax = ((st.pos[i] + V3<R>(rad)) - mn) * g->inv;
+        for (int iz = std::abs(int(BMin.z)); iz <= std::abs(int(BMax.z)); iz++)
+            for (int iy = std::abs(int(BMin.y)); iy <= std::abs(int(BMax.y)); iy++)
+                for (int ix = std::abs(int(BMin.x)); ix <= std::abs(int(BMax.x)); ix++)
+                    g->buckets[sppm_hash(ix, iy, iz, n)].push_back((int64_t)i);
+    }
+}
+template <class R> const std::vector<int64_t>& sppm_grid_bucket(const SppmGrid<R>& g, const V3<R>& p) {
+    const V3<R> hh = (p - g.bmin) * g.inv;
+    return g.buckets[sppm_hash(std::abs(int(hh.x)), std::abs(int(hh.y)), std::abs(int(hh.z)), g.num)];
+}
+// SPPM::GeneratePhoton + TracePhoton (SPPM.cpp:28-43, 110-167) for photon
+// `index` (the sampler instance iter * photons + j, SPPM.cpp:238).
+template <class R> void sppm_photon(const Scene<R>& s, uint64_t index, int sampler, int maxDepth, R alpha, bool batch,
+                                    SppmState<R>& st, const SppmGrid<R>& g) {
+    Seq rnd;
+    rnd.kind = sampler;
+    rnd.instance = index;
+    R lightPdf;
+    const int light = sample_one_light(s, (R)rnd.next(), &lightPdf);
+    R u0 = rnd.next(), u1 = rnd.next();
+    R w0 = rnd.next(), w1 = rnd.next();
+    // AreaLight::SampleLight = SampleOnLight (light/AreaLight.h:60-71) + Emission()
+    const Tri<R>& t = s.tris[s.light_tri[light]];
+    R su0 = std::sqrt(u0);
+    R b0 = 1 - su0, b1 = u1 * su0;
+    const V3<R> lpos = (b0 * t.p0 + b1 * t.p1) + (1 - b0 - b1) * t.p2;
+    const V3<R> ln = (b0 * t.n0 + b1 * t.n1) + (1 - b0 - b1) * t.n2;
+    const V3<R> lerr = gamma_n<R>(6) * ((vabs(b0 * t.p0) + vabs(b1 * t.p1)) + vabs((1 - b0 - b1) * t.p2));
+    const R pdfPos = 1 / t.area;
+    V3<R> ss, ts;
+    coordinate_system(ln, &ss, &ts);
+    const V3<R> dl = cosine_hemisphere(w0, w1);
+    const V3<R> dir = norm((ss * dl.x + ts * dl.y) + ln * dl.z);
+    const R pdfDir = dl.z * K<R>::INV_PI;
+    Ray<R> r = make_ray(offset_ray_origin(lpos, lerr, t.fn, dir), dir, K<R>::Inf, (R)0);  // lightPoint.SpawnRay
+    const R cosTheta = std::abs(dot(ln, dir));
+    V3<R> flux = (pdfPos == 0 || pdfDir == 0) ? V3<R>(0, 0, 0) : s.light_le[light] * cosTheta / (pdfPos * pdfDir * lightPdf);
+    for (int i = 0; i < maxDepth; ++i) {
+        Hit<R> h;
+        if (!intersect(s, r, &h, true)) return;
+        Isect<R> it = query(s, r, h);
+        BSDF<R> bsdf = make_bsdf(s, it);
+        bsdf.importance = true;  // ComputeScatteringFunction(arena, TransportMode::Importance)
+        R a = rnd.next(), b = rnd.next(), c = rnd.next();
+        V3<R> wi;
+        R pdf;
+        V3<R> f = bsdf.Sample((R)-1 * r.d, &wi, &pdf, V3<R>(a, b, c));
+        if (f == V3<R>() || pdf == 0) break;
+        V3<R> est = f * std::abs(dot(wi, it.n)) / pdf;
+        if (!bsdf.delta() && i > 0 && g.num > 0) {
+            for (int64_t hp : sppm_grid_bucket(g, it.pos)) {
+                const V3<R> v = st.pos[hp] - it.pos;
+                if ((double)dot(st.nrm[hp], it.absN) > 0.0015 && dot(v, v) <= st.r2[hp]) {
+                    const V3<R> contribution = st.imp[hp] * bsdf.Evaluate(st.out[hp], (R)-1 * r.d) * flux;
+                    if (!batch) {
+                        R gg = (st.N[hp] * alpha + alpha) / (st.N[hp] * alpha + 1.f);
+                        st.r2[hp] = st.r2[hp] * gg;
+                        st.N[hp] += 1;
+                        st.flux[hp] = (st.flux[hp] + contribution) * gg;
+                    } else {
+                        st.m[hp]++;
+                        st.flux[hp] = st.flux[hp] + contribution;
+                    }
+                }
+            }
+        }
+        R p = rmin((R)1.0, lumY(est * flux) / lumY(flux));
+        if (p < 1) {
+            if ((R)rnd.next() < p) flux = flux / p;
+            else break;
+        }
+        flux = flux * est;
+        r = spawn_ray(it, wi);
+    }
+}
+// SPPM::Render (SPPM.cpp:198-277): iterations of eye pass, grid, photons and
+// (batchShrink) the per-hit-point update; then the density estimate.
+template <class R> void sppm_render(SceneHandle* h, const or_camera_desc* cd, const or_sppm_params* p, double* color, double* r2_out,
+                                    double* n_out) {
+    Scene<R>& s = scene_of<R>(h);
+    Camera<R> cam = make_camera<R>(*cd);
+    const int W = cd->width, H = cd->height;
+    const int res = round_up_pow2(std::max(W, H));
+    const int m = log2_int(res);
+    const hal::Enum henum((uint32_t)W, (uint32_t)H);
+    SppmState<R> st;
+    st.W = W;
+    st.H = H;
+    const size_t n = (size_t)W * H;
+    const R r0 = (R)p->initial_radius, alpha = (R)p->alpha;
+    st.r2.assign(n, r0 * r0);
+    st.N.assign(n, 0);
+    st.flux.assign(n, V3<R>());
+    st.direct.assign(n, V3<R>());
+    st.imp.assign(n, V3<R>());
+    st.pos.assign(n, V3<R>());
+    st.nrm.assign(n, V3<R>());
+    st.out.assign(n, V3<R>());
+    st.used.assign(n, 0);
+    st.m.assign(n, 0);
+    const int nth = p->nthreads > 0 ? p->nthreads : (int)std::thread::hardware_concurrency();
+    for (int iter = 0; iter < p->iterations; ++iter) {
+        std::atomic<int> next(0);
+        auto work = [&]() {
+            for (int y; (y = next.fetch_add(1)) < H;)
+                for (int x = 0; x < W; ++x)
+                    sppm_eye(s, cam, res, m, p->sampler == 1 ? &henum : nullptr, x, y, iter, p->max_depth, st);
+        };
+        std::vector<std::thread> th;
+        for (int t = 0; t < nth; ++t) th.emplace_back(work);
+        for (auto& t : th) t.join();
+        SppmGrid<R> grid;
+        sppm_grid_build(st, &grid);
+        for (int64_t j = 0; j < p->photons; ++j)
+            sppm_photon(s, (uint64_t)iter * (uint64_t)p->photons + (uint64_t)j, p->sampler, p->max_depth, alpha, p->batch_shrink != 0,
+                        st, grid);
+        if (p->batch_shrink) {
+            for (size_t i = 0; i < n; ++i) {
+                if (st.m[i] > 0) {
+                    R gg = (st.N[i] + alpha * (R)st.m[i]) / (st.N[i] + (R)st.m[i]);
+                    st.r2[i] = st.r2[i] * gg;
+                    st.flux[i] = st.flux[i] * gg;
+                    st.N[i] = st.N[i] + alpha * (R)st.m[i];
+                    st.m[i] = 0;
+                }
+            }
+        }
+    }
+    for (size_t i = 0; i < n; ++i) {
+        const R k = 1.f / (K<R>::PI * st.r2[i] * (R)p->iterations * (R)p->photons);
+        const V3<R> c = (V3<R>() + st.flux[i] * k) + st.direct[i] / (R)p->iterations;
+        color[3 * i] = c.x;
+        color[3 * i + 1] = c.y;
+        color[3 * i + 2] = c.z;
+        if (r2_out) r2_out[i] = st.r2[i];
+        if (n_out) n_out[i] = st.N[i];
+    }
+}
+
 template <class R> int li_samples_t(SceneHandle* h, const or_camera_desc* cd, const or_render_params* p, int32_t n,
                                     const int32_t* x, const int32_t* y, const int32_t* s, double* out_L, float* out_raster,
                                     int32_t* out_dims) {
@@ -2813,6 +3067,31 @@ template <class R> static int vpt_tr_probe(Scene<R>& s, const double* o, const d
     const bool hit = intersect_tr(s, ray, &med, &h, &Tr, true);
     out[0] = hit ? h.prim : -1;
     out[1] = Tr.x; out[2] = Tr.y; out[3] = Tr.z;
+    return 0;
+}
+// Test hook: the hash grid of n points (all in use) with radii^2 r2; for each
+// query point the bucket it looks up (point indices, duplicates kept).
+static int sppm_grid_query_t(int32_t n, const float* pos, const float* r2, int32_t nq, const float* q, int32_t* counts,
+                             int64_t* idx, int64_t cap) {
+    SppmState<float> st;
+    st.W = n;
+    st.H = 1;
+    st.used.assign(n, 1);
+    st.r2.assign(r2, r2 + n);
+    st.pos.resize(n);
+    for (int32_t i = 0; i < n; ++i) st.pos[i] = V3<float>(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+    SppmGrid<float> g;
+    sppm_grid_build(st, &g);
+    int64_t w = 0;
+    for (int32_t k = 0; k < nq; ++k) {
+        const auto& b = sppm_grid_bucket(g, V3<float>(q[3 * k], q[3 * k + 1], q[3 * k + 2]));
+        counts[k] = (int32_t)b.size();
+        if (!idx) continue;
+        for (int64_t e : b) {
+            if (w >= cap) return -1;
+            idx[w++] = e;
+        }
+    }
     return 0;
 }
 extern "C" {
@@ -3079,6 +3358,19 @@ int or_set_detached(void* hv, const float* tex_eval, const float* le_eval) {
 int or_vpt_intersect_tr(void* hv, int precision, const double* o, const double* d, int32_t medium, double* out) {
     SceneHandle* h = (SceneHandle*)hv;
     return precision ? vpt_tr_probe(scene_of<double>(h), o, d, medium, out) : vpt_tr_probe(h->sf, o, d, medium, out);
+}
+
+int or_render_sppm(void* hv, const or_camera_desc* cd, const or_sppm_params* p, double* color, double* radius2, double* photons) {
+    SceneHandle* h = (SceneHandle*)hv;
+    if (!p || p->iterations <= 0 || p->photons <= 0 || p->max_depth <= 0) return -1;
+    if (!h->env.empty() || !h->sf.sph.empty() || h->sf.light_tri.empty()) return -2;  // area lights, triangles
+    if (p->precision) sppm_render<double>(h, cd, p, color, radius2, photons);
+    else sppm_render<float>(h, cd, p, color, radius2, photons);
+    return 0;
+}
+int or_sppm_grid_query(int32_t n, const float* pos, const float* r2, int32_t nq, const float* q, int32_t* counts, int64_t* idx,
+                       int64_t cap) {
+    return sppm_grid_query_t(n, pos, r2, nq, q, counts, idx, cap);
 }
 
 int or_set_vpt_query_crossings(int on) {

@@ -145,6 +145,27 @@ int or_scene_set_media(void* scene, int32_t n_media, const or_medium* media, int
 /* Test-only: 1 = Scene::IntersectTr queries each crossed material-less surface
  * and continues in the medium beyond its normal (0 = the reference's rule). */
 int or_set_vpt_query_crossings(int on);
+/* SPPM(iterations, photons, maxDepth, initialRadius, alpha, batchShrink, sampler)
+ * (integrator/SPPM.h:48-49), traceGlossyRay off; triangle scenes with area
+ * lights.  color = W x H x 3 (Film::SetImage's image); radius2 and photons
+ * (optional) = each pixel's final radius^2 and photon count N. */
+typedef struct {
+    int32_t iterations;
+    int32_t max_depth;
+    int64_t photons;        /* per iteration */
+    float initial_radius;
+    float alpha;
+    int32_t batch_shrink;
+    int32_t sampler;        /* 0 Sobol, 1 Halton */
+    int32_t precision;      /* 0 FP32, 1 FP64 */
+    int32_t nthreads;       /* eye pass threads (0: all) */
+} or_sppm_params;
+int or_render_sppm(void* scene, const or_camera_desc* cam, const or_sppm_params* p, double* color, double* radius2, double* photons);
+/* Test hook: HashGrid over n points (all used, radii^2 r2); per query point the
+ * bucket it looks up: counts[nq], then the point indices in idx (duplicates
+ * kept; idx NULL: counts only). */
+int or_sppm_grid_query(int32_t n, const float* pos, const float* r2, int32_t nq, const float* q, int32_t* counts, int64_t* idx,
+                       int64_t cap);
 /* Test hook: Scene::IntersectTr from o along d (origin moved RayEps by the Ray
  * constructor) starting in medium (-1: none); out = hit primitive (-1), Tr[3]. */
 int or_vpt_intersect_tr(void* scene, int precision, const double* o, const double* d, int32_t medium, double* out);

@@ -60,6 +60,42 @@ class ORParams(C.Structure):
     ]
 
 
+class ORSppmParams(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int32),
+        ("max_depth", C.c_int32),
+        ("photons", C.c_int64),
+        ("initial_radius", C.c_float),
+        ("alpha", C.c_float),
+        ("batch_shrink", C.c_int32),
+        ("sampler", C.c_int32),
+        ("precision", C.c_int32),
+        ("nthreads", C.c_int32),
+    ]
+
+
+def sppm_params(iterations, photons, max_depth, initial_radius, alpha, batch_shrink=0, sampler=0, precision=0, nthreads=0):
+    p = ORSppmParams()
+    p.iterations, p.photons, p.max_depth = iterations, photons, max_depth
+    p.initial_radius, p.alpha, p.batch_shrink = initial_radius, alpha, batch_shrink
+    p.sampler, p.precision, p.nthreads = sampler, precision, nthreads
+    return p
+
+
+def sppm_grid_query(pos, r2, q):
+    """The HashGrid bucket each query point looks up (point indices, duplicates kept)."""
+    pos = np.ascontiguousarray(pos, np.float32)
+    r2 = np.ascontiguousarray(r2, np.float32)
+    q = np.ascontiguousarray(q, np.float32)
+    counts = np.zeros(q.shape[0], np.int32)
+    lib().or_sppm_grid_query(pos.shape[0], _p(pos), _p(r2), q.shape[0], _p(q), _p(counts), None, 0)
+    cap = int(counts.sum())
+    idx = np.zeros(max(cap, 1), np.int64)
+    rc = lib().or_sppm_grid_query(pos.shape[0], _p(pos), _p(r2), q.shape[0], _p(q), _p(counts), _p(idx), cap)
+    assert rc == 0
+    return np.split(idx[:counts.sum()], np.cumsum(counts)[:-1])
+
+
 # or_medium / or_tri_medium
 MEDIUM_DTYPE = np.dtype([("sigma_a", "<f4", 3), ("sigma_s", "<f4", 3), ("g", "<f4")])
 TRI_MEDIUM_DTYPE = np.dtype([("inside", "<i4"), ("outside", "<i4"), ("no_material", "<i4")])
@@ -114,6 +150,8 @@ def lib():
         "or_sphere_light_probe": (C.c_int, [vp, i32, C.c_int, vp, vp, vp, vp]),
         "or_scene_set_media": (C.c_int, [vp, i32, vp, i32, vp]),
         "or_set_vpt_query_crossings": (C.c_int, [C.c_int]),
+        "or_render_sppm": (C.c_int, [vp, vp, vp, vp, vp, vp]),
+        "or_sppm_grid_query": (C.c_int, [i32, vp, vp, i32, vp, vp, vp, C.c_int64]),
         "or_vpt_intersect_tr": (C.c_int, [vp, C.c_int, vp, vp, i32, vp]),
         "or_hg_probe": (C.c_int, [C.c_int, C.c_double, vp, vp, vp, vp]),
         "or_medium_probe": (C.c_int, [C.c_int, C.c_int, vp, vp, vp, vp, C.c_double, vp, vp, vp]),
@@ -300,6 +338,16 @@ class OracleScene:
         rc = lib().or_scene_set_media(self.h, self._media.shape[0], _p(self._media), self._tri_media.shape[0],
                                       _p(self._tri_media))
         assert rc == 0, rc
+
+    def render_sppm(self, cam, sp):
+        """SPPM (or_render_sppm): (image (H, W, 3), radius^2 (H, W), photon count N (H, W))."""
+        c = camera_from(cam)
+        img = np.zeros((cam.height, cam.width, 3), np.float64)
+        r2 = np.zeros((cam.height, cam.width), np.float64)
+        nn = np.zeros((cam.height, cam.width), np.float64)
+        rc = lib().or_render_sppm(self.h, C.byref(c), C.byref(sp), _p(img), _p(r2), _p(nn))
+        assert rc == 0, rc
+        return img, r2, nn
 
     def intersect_tr(self, o, d, medium, precision=1):
         """Scene::IntersectTr from o along d in medium: (hit primitive or -1, Tr)."""
