@@ -297,6 +297,72 @@ def vpt_main(args):
     print(json.dumps(out), flush=True)
 
 
+def sppm_main(args):
+    """SPPM (gy_render_sppm) on the icosphere workload with the reference's own
+    SPPM settings (Gyunity_exe/main.cpp:54-56, 247-249): 300,000 photons per
+    iteration, alpha 0.75, depth 20, the Halton sampler; initial radius 0.01
+    (1.0 in the reference's 100-unit Cornell box)."""
+    import gyunity_amd as g
+
+    sa, cam, prm = g.generate_scene(g.GY_SCENE_ICOSPHERE_GGX, 0, args.level)
+    iters, photons = args.sppm_iterations, 300000
+    sp = g.sppm_params(iters, photons, 20, 0.01, 0.75, batch_shrink=False, sampler=g.GY_SAMPLER_HALTON)
+    ctx = g.Context(0)
+    log("device:", ctx.device_name())
+    sc = g.Scene(ctx, sa)
+    for i in range(args.warmup):
+        sc.render_sppm(cam, g.sppm_params(2, photons, 20, 0.01, 0.75, sampler=g.GY_SAMPLER_HALTON))
+        log("sppm warmup %d done" % i)
+    t0 = time.perf_counter()
+    img, r2, N = sc.render_sppm(cam, sp)
+    secs = time.perf_counter() - t0
+    log("sppm: %d iterations in %.2f s" % (iters, secs))
+    out = {
+        "metric": "Mpaths/s at fixed RMSE: batched wavefront vs plain megakernel; adjoint s/grad",
+        "value": round(iters * photons / secs / 1e6, 3),
+        "unit": "Mphotons/s",
+        "n_gpus": 1,
+        "steps": iters,
+        "warmup": args.warmup,
+        "ms_per_step": round(secs * 1e3 / iters, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: procedural displaced icosphere (seed 0) in the Cornell room",
+        "config": {
+            "workload": "SPPM, icosphere-L%d GGX conductor, %d triangles, %dx%d eye paths and %d photons per iteration, depth 20, "
+                        "alpha 0.75, initial radius 0.01, Halton" % (args.level, sa.n_tri, cam.width, cam.height, photons),
+            "step": "one SPPM iteration: eye pass, hash grid, photon pass, per-hit-point updates in photon order",
+            "iterations": iters,
+            "eye_paths_per_s_millions": round(iters * cam.width * cam.height / secs / 1e6, 3),
+            "mean_photons_per_pixel": float(N.mean()),
+            "mean_radius": float(np.sqrt(r2).mean()),
+            "finite_fraction": float(np.isfinite(img).mean()),
+            "parallelism": "single GPU",
+        },
+    }
+    sc.close()
+    ctx.close()
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+
+        log("cpu baseline ...")
+        model, logical, allowed = host_cpu()
+        cores = max(1, min(CPU_SHARE, allowed))
+        res, it_c, ph_c = 256, 2, 300000
+        small = g.camera(tuple(cam.pos), tuple(cam.look_at), tuple(cam.up), cam.fovy, res, res, cam.film_dist)
+        os_ = O.OracleScene(sa)
+        t0 = time.perf_counter()
+        os_.render_sppm(small, O.sppm_params(it_c, ph_c, 20, 0.01, 0.75, 0, 1, 0, cores))
+        secs_c = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(it_c * ph_c / secs_c / 1e6, 4), "unit": "Mphotons/s", "cores": cores, "kind": "port",
+                               "sample": "%dx%d, %d iterations of %d photons (%.1f s): eye pass on %d threads, photons one at a "
+                                         "time in index order" % (res, res, it_c, ph_c, secs_c, cores),
+                               "cpu_model": model, "host_logical_cpus": logical, "cpus_allowed": allowed}
+    print(json.dumps(out), flush=True)
+
+
 def batch_main(args):
     """The 64-scene batch workload: every member traced in one lane space
     (gy_batch_render, recorded into a hipGraph once and replayed), against the
@@ -668,9 +734,11 @@ def main():
     ap.add_argument("--refill", type=int, default=0, help="wavefront refill threshold (0 = library default)")
     ap.add_argument("--spp-pass", type=int, default=0, help="samples per pass (0 = library default)")
     ap.add_argument("--leaf", type=int, default=0, help="BVH leaf size (0 = library default, 4)")
-    ap.add_argument("--workload", default="icosphere", choices=["icosphere", "env", "lt", "vpt", "batch64", "adjoint", "adjoint-scale"],
+    ap.add_argument("--workload", default="icosphere",
+                    choices=["icosphere", "env", "lt", "vpt", "sppm", "batch64", "adjoint", "adjoint-scale"],
                     help="icosphere: the 1.31M-triangle GGX render; batch64: the 64-scene batch; adjoint: "
                          "gradient of the textured Cornell room; adjoint-scale: Adam on 4.2M texels over 32 views")
+    ap.add_argument("--sppm-iterations", type=int, default=16)
     ap.add_argument("--fd-texels", type=int, default=0, help="adjoint: texels checked against finite differences")
     ap.add_argument("--fd-spp", type=int, default=4096)
     ap.add_argument("--fd-h", type=float, default=1e-3)
@@ -702,6 +770,9 @@ def main():
         return
     if args.workload == "vpt":
         vpt_main(args)
+        return
+    if args.workload == "sppm":
+        sppm_main(args)
         return
 
     import gyunity_amd as g
