@@ -378,7 +378,8 @@ def batch_main(args):
     paths = sum(cam.width * cam.height * prm.spp for _, cam, prm in members)
     tris = sum(sa.n_tri for sa, _, _ in members)
     log("scenes resident: %d triangles, %d paths per batch render" % (tris, paths))
-    batch = g.Batch(ctx, lanes_per_pass=args.batch_lanes, refill_threshold=args.refill, use_graph=not args.batch_no_graph)
+    batch = g.Batch(ctx, lanes_per_pass=args.batch_lanes, refill_threshold=args.refill, use_graph=not args.batch_no_graph,
+                    ray_sort=RAY_SORT[args.ray_sort])
     for sc, (_, cam, prm) in zip(scenes, members):
         batch.add(sc, cam, prm)
     for i in range(args.warmup):
@@ -720,6 +721,9 @@ def adjoint_leg(g, ctx, steps_small=5, steps_scale=2):
     return out
 
 
+RAY_SORT = {"none": 0, "octant": 1, "octant-local": 2}  # GY_RAY_SORT_*
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -732,6 +736,8 @@ def main():
     ap.add_argument("--no-legs", action="store_true", help="default workload: skip the batch64 and adjoint legs")
     ap.add_argument("--single", action="store_true", help="time only --mode, not the other integrator")
     ap.add_argument("--refill", type=int, default=0, help="wavefront refill threshold (0 = library default)")
+    ap.add_argument("--ray-sort", default="none", choices=["none", "octant", "octant-local"],
+                    help="wavefront closest-hit queue order from the second bounce on")
     ap.add_argument("--spp-pass", type=int, default=0, help="samples per pass (0 = library default)")
     ap.add_argument("--leaf", type=int, default=0, help="BVH leaf size (0 = library default, 4)")
     ap.add_argument("--workload", default="icosphere",
@@ -783,6 +789,7 @@ def main():
         sa, env = open_room_sky(g, sa)
     prm.mode = g.GY_RENDER_WAVEFRONT if args.mode == "wavefront" else g.GY_RENDER_MEGAKERNEL
     prm.refill_threshold = args.refill
+    prm.ray_sort = RAY_SORT[args.ray_sort]
     prm.samples_per_pass = args.spp_pass
     paths = cam.width * cam.height * prm.spp
     ctx = g.Context(0)

@@ -28,6 +28,9 @@ GY_SCENE_BATCH_MEMBER = 2
 GY_SCENE_ADJOINT = 3
 GY_SCENE_ICOSPHERE_TEXTURED = 4
 GY_SAMPLER_HALTON = 1
+GY_RAY_SORT_NONE = 0
+GY_RAY_SORT_OCTANT = 1
+GY_RAY_SORT_OCTANT_LOCAL = 2
 FAULT_NONE, FAULT_ALLOC, FAULT_LAUNCH, FAULT_SCENE = 0, 1, 2, 3
 
 
@@ -73,11 +76,13 @@ class RenderParams(C.Structure):
         ("block_size", C.c_int32),
         ("refill_threshold", C.c_int32),
         ("sampler", C.c_int32),
+        ("ray_sort", C.c_int32),
     ]
 
 
 class BatchParams(C.Structure):
-    _fields_ = [("lanes_per_pass", C.c_int64), ("refill_threshold", C.c_int32), ("use_graph", C.c_int32)]
+    _fields_ = [("lanes_per_pass", C.c_int64), ("refill_threshold", C.c_int32), ("use_graph", C.c_int32),
+                ("ray_sort", C.c_int32), ("reserved", C.c_int32)]
 
 
 class BVHParams(C.Structure):
@@ -185,6 +190,7 @@ def lib():
         "gy_scene_texture_floats": (i64, [vp]),
         "gy_scene_context": (C.c_int, [vp, P(vp)]),
         "gy_autotune": (C.c_int, [vp, P(CameraDesc), P(RenderParams), i32, P(RenderParams), P(C.c_double)]),
+        "gy_autotune_clear_cache": (C.c_int, []),
         "gy_ggx_dlogf_dalpha": (f32, [f32, P(f32), P(f32)]),
         "gy_dalpha_droughness": (f32, [f32]),
         "gy_scene_set_textures": (C.c_int, [vp, P(f32), i64]),
@@ -356,11 +362,17 @@ def orbit_views(n, res, seed=3, centre=(0.5, 0.4, 0.5), radius=0.45, fovy=60.0):
 
 
 def render_params(spp, max_depth, rr_depth=5, mode=GY_RENDER_MEGAKERNEL, samples_per_pass=0, block_size=0,
-                  sampler=GY_SAMPLER_SOBOL):
+                  sampler=GY_SAMPLER_SOBOL, ray_sort=GY_RAY_SORT_NONE):
     p = RenderParams()
     p.spp, p.max_depth, p.rr_depth, p.mode = spp, max_depth, rr_depth, mode
     p.samples_per_pass, p.block_size, p.sampler = samples_per_pass, block_size, sampler
+    p.ray_sort = ray_sort
     return p
+
+
+def autotune_clear_cache():
+    """Forget the autotuner's kept choices (gy_autotune_clear_cache)."""
+    check(lib().gy_autotune_clear_cache())
 
 
 def halton_sample(index, dim):
@@ -476,10 +488,11 @@ class Batch:
     """Many renders traced together (gy_batch_*): add (scene, camera, params)
     entries, render them all in one lane space, download each entry's film."""
 
-    def __init__(self, ctx, lanes_per_pass=0, refill_threshold=0, use_graph=True):
+    def __init__(self, ctx, lanes_per_pass=0, refill_threshold=0, use_graph=True, ray_sort=GY_RAY_SORT_NONE):
         self.ctx = ctx
         bp = BatchParams()
         bp.lanes_per_pass, bp.refill_threshold, bp.use_graph = lanes_per_pass, refill_threshold, 1 if use_graph else 0
+        bp.ray_sort = ray_sort
         self.h = C.c_void_p()
         self.cams = []
         check(lib().gy_batch_create(ctx.h, C.byref(bp), C.byref(self.h)))

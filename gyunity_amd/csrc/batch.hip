@@ -93,12 +93,15 @@ extern "C" int gy_batch_create(gy_context* ctx, const gy_batch_params* p, gy_bat
     if (!ctx || !out) return Fail(GY_ERR_INVALID, "gy_batch_create: null argument");
     if (p && (p->lanes_per_pass < 0 || p->refill_threshold < 0 || p->refill_threshold > 64))
         return Fail(GY_ERR_INVALID, "gy_batch_create: lanes_per_pass >= 0 and 0 <= refill_threshold <= 64");
+    if (p && (p->ray_sort < GY_RAY_SORT_NONE || p->ray_sort > GY_RAY_SORT_OCTANT_LOCAL))
+        return Fail(GY_ERR_INVALID, "gy_batch_create: unknown ray_sort");
     BatchHandle* h = new BatchHandle;
     h->b.ctx = reinterpret_cast<Ctx*>(ctx);
     if (p) {
         h->lanes_req = p->lanes_per_pass;
         h->b.refill = p->refill_threshold > 0 ? p->refill_threshold : 0;  // 0: per-kind automatic
         h->b.use_graph = p->use_graph != 0;
+        h->b.ray_sort = p->ray_sort;
     }
     *out = reinterpret_cast<gy_batch*>(h);
     return GY_OK;

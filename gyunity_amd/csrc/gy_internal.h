@@ -13,6 +13,10 @@ namespace gy {
 void SetError(const std::string& msg);
 int Fail(int code, const std::string& msg);
 
+// A scene's process-unique id and the revision of its device buffers
+// (environment light, spheres, media); texel updates keep the revision.
+int SceneSignature(gy_scene* s, uint64_t* uid, uint64_t* revision);
+
 namespace sobol {
 constexpr int kDims = 1024;       // dimensions of the Joe-Kuo table
 constexpr int kCols = 52;         // index bits per dimension

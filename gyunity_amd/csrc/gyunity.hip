@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -747,6 +748,7 @@ int ValidateRender(const gy_camera_desc* cam, const gy_render_params* p) {
     if (p->spp <= 0 || p->max_depth < 0 || p->max_depth > 100) return Fail(GY_ERR_INVALID, "render: spp > 0 and 0 <= max_depth <= 100");
     if ((int64_t)cam->width * cam->height > (1ll << 26)) return Fail(GY_ERR_UNSUPPORTED, "render: image larger than 2^26 pixels");
     if (p->sampler != GY_SAMPLER_SOBOL && p->sampler != GY_SAMPLER_HALTON) return Fail(GY_ERR_INVALID, "render: unknown sampler");
+    if (p->ray_sort < GY_RAY_SORT_NONE || p->ray_sort > GY_RAY_SORT_OCTANT_LOCAL) return Fail(GY_ERR_INVALID, "render: unknown ray_sort");
     return GY_OK;
 }
 
@@ -949,7 +951,17 @@ extern "C" int gy_scene_create(gy_context* ctx, const gy_scene_desc* desc, const
     Scene* sc = nullptr;
     int rc = SceneUpload(c, *desc, bvh, &sc);
     if (rc != GY_OK) return rc;
+    static std::atomic<uint64_t> next_uid{1};
+    sc->uid = next_uid++;
     *out = reinterpret_cast<gy_scene*>(sc);
+    return GY_OK;
+}
+
+int gy::SceneSignature(gy_scene* s, uint64_t* uid, uint64_t* revision) {
+    if (!s) return Fail(GY_ERR_INVALID, "null scene");
+    const Scene* sc = reinterpret_cast<const Scene*>(s);
+    *uid = sc->uid;
+    *revision = sc->revision;
     return GY_OK;
 }
 

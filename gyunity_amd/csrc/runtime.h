@@ -85,6 +85,7 @@ struct Scene {
     int32_t n_media = 0;
     bool has_no_mat = false;  // some triangle has no material (a medium boundary)
     uint64_t revision = 0;  // bumped when device buffers the kernels point at are replaced
+    uint64_t uid = 0;       // unique over the process (never reused, unlike the address)
     float world_lo[3] = {0, 0, 0}, world_hi[3] = {0, 0, 0};
     std::vector<int32_t> area_tri;  // area lights as created (the environment light follows them)
     std::vector<float> area_le, area_func;
@@ -167,6 +168,7 @@ struct Batch {
     int64_t total_paths = 0;
     int depth_max = 0;
     int refill = 0;  // 0: per-kind automatic threshold
+    int ray_sort = GY_RAY_SORT_NONE;
     bool use_graph = true;
     bool dirty = true;        // entries changed since the table / graph were built
     bool rendered = false;    // the films hold a render of the current table

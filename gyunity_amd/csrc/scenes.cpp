@@ -46,7 +46,7 @@ struct Generated {
     std::vector<int32_t> light, ltri;
     std::vector<float> le;
     gy_camera_desc cam;
-    gy_render_params params;
+    gy_render_params params = {};
 };
 
 namespace {

@@ -66,7 +66,7 @@ struct WfDev {
     float4* L;      // radiance (xyz), film y in w
     uint4* seq;     // sample index lo/hi, cursor, flags
     float4* hit;    // t, prim, b1, b2
-    uint8_t* mtype;  // per active-queue position: material type of the hit (0..3), 255 for a miss
+    uint8_t* mtype;  // per slot: material type of the closest hit (0..3), 255 for a miss
     float4* pend0;  // L1 w1 before the shadow test (xyz), P(light)
     float4* pend1;  // MIS ray only: BSDF value of the MIS sample (xyz) / L2 w2 after trace_mis, cos
     float4* pend2;  // MIS ray only: pdf of the MIS sample, w2, -, light id
@@ -78,6 +78,10 @@ struct WfDev {
     uint32_t* ent;    // batch: entry of the lane
     uint32_t* pslot;  // batch: the path's slot in its entry's pass buffer
     uint32_t* q[Q_COUNT];  // Q_ACTIVE, Q_NEXT, 4 shade queues, shadow, mis
+    uint32_t* sort_q;      // ray sort: the partitioned active queue
+    const uint32_t* trace_q;  // queue of the closest-hit trace when it differs from the active queue (else null)
+    uint32_t* sort_hist;   // ray sort: 8 x sort_blocks octant counts, then their offsets; 8 octant totals
+    uint32_t sort_blocks;  // ray sort: workgroups of kSortPer queue entries over the capacity
     uint32_t* counts;          // Q_COUNT counters
     int64_t capacity;
     uint32_t* spill;       // traversal stack spill, 64 entries per lane
@@ -337,7 +341,7 @@ __global__ void __launch_bounds__(256) k_wf_trace(R Rd, WfDev W, int refill, int
         if (my >= 0) {
             if (fresh) {
                 fresh = false;
-                slot = W.q[qid][my];
+                slot = (MODE == 0 && W.trace_q) ? W.trace_q[my] : W.q[qid][my];
                 e = Rd.entry(W, slot);
                 if (R::kBatch) {
                     Sc.nodes = Rd.scene(e).nodes;
@@ -360,9 +364,9 @@ __global__ void __launch_bounds__(256) k_wf_trace(R Rd, WfDev W, int refill, int
                             const SceneDev& Sf = Rd.scene(e);
                             W.hit[slot] = make_float4(t.best, __int_as_float(t.best_prim), t.bb1, t.bb2);
                             const uint32_t mat = __float_as_uint(Sf.shade[(size_t)t.best_prim * 8].w);
-                            W.mtype[my] = (uint8_t)Sf.mats[(size_t)mat * 32];
+                            W.mtype[slot] = (uint8_t)Sf.mats[(size_t)mat * 32];
                         } else {
-                            W.mtype[my] = 255;
+                            W.mtype[slot] = 255;
                             W.pend3[slot].w = __uint_as_float(P_DONE);  // the path ends (k_wf_env: environment)
                         }
                     } else if (MODE == 1) {
@@ -417,11 +421,150 @@ __global__ void __launch_bounds__(256) k_wf_trace(R Rd, WfDev W, int refill, int
     }
 }
 
+// ---------------------------------------------------------------- ray sort
+// Octant partition of the closest-hit queue: a stable counting sort of the
+// active queue by the sign bits of each ray's direction.  Rays of one octant
+// visit the BVH's children in the same order, so the lanes of a wave diverge
+// less; within an octant the queue keeps its order (neighbouring paths stay
+// together).  No result depends on queue order: each lane's path is its own,
+// and resolve walks the slots.  Only the closest-hit trace takes its rays from
+// the partitioned queue: binning, shading and the other traces keep the active
+// queue's order, in which neighbouring lanes read neighbouring slots.  Three
+// launches: per-workgroup octant counts (the octant of queue entry i is kept in
+// mtype[i]; the closest-hit trace then writes mtype of every slot in the
+// queue, the only entries read after it), an exclusive scan of each octant's
+// counts over the workgroups, and the scatter (octant offset + workgroup
+// offset + rank in the workgroup).
+constexpr int kSortPer = 1024;  // queue entries per workgroup, 4 per thread
+
+__device__ inline uint32_t ray_octant(const float4 d) {
+    return (__float_as_uint(d.x) >> 31) | ((__float_as_uint(d.y) >> 31) << 1) | ((__float_as_uint(d.z) >> 31) << 2);
+}
+
+// Lanes of the wave with the same octant as this lane, among `valid` lanes.
+__device__ inline uint64_t same_octant(uint32_t oc, bool valid) {
+    const uint64_t v = __ballot(valid);
+    const uint64_t b0 = __ballot(valid && (oc & 1u)), b1 = __ballot(valid && (oc & 2u)), b2 = __ballot(valid && (oc & 4u));
+    return v & ((oc & 1u) ? b0 : ~b0) & ((oc & 2u) ? b1 : ~b1) & ((oc & 4u) ? b2 : ~b2);
+}
+
+__global__ void __launch_bounds__(256) k_wf_sort_count(WfDev W) {
+    __shared__ uint32_t h[8];
+    const uint32_t n = W.counts[Q_ACTIVE * kCtrStride];
+    const uint32_t base = blockIdx.x * (uint32_t)kSortPer;
+    if (base >= n) return;  // workgroup-uniform; the scan reads only the workgroups below n
+    if (threadIdx.x < 8) h[threadIdx.x] = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t i = base + k * 256 + threadIdx.x;
+        const bool valid = i < n;
+        uint32_t oc = 0;
+        if (valid) {
+            oc = ray_octant(W.d[W.q[Q_ACTIVE][i]]);
+            W.mtype[i] = (uint8_t)oc;
+        }
+        const uint64_t same = same_octant(oc, valid);
+        if (valid && __popcll(same & ((1ull << lane) - 1ull)) == 0) atomicAdd(&h[oc], (uint32_t)__popcll(same));
+    }
+    __syncthreads();
+    if (threadIdx.x < 8) W.sort_hist[threadIdx.x * W.sort_blocks + blockIdx.x] = h[threadIdx.x];
+}
+
+// Exclusive scan of one octant's counts over the workgroups below n, in place
+// (workgroup o scans octant o); the octant's total goes to sort_hist[8 *
+// sort_blocks + o].  Each thread scans a contiguous run of the counts.
+__global__ void __launch_bounds__(1024) k_wf_sort_scan(WfDev W) {
+    __shared__ uint32_t part[1024];
+    const uint32_t n = W.counts[Q_ACTIVE * kCtrStride];
+    const uint32_t nb = (n + kSortPer - 1) / kSortPer;
+    const uint32_t per = (nb + 1023) / 1024;
+    const uint32_t lo = min(nb, threadIdx.x * per), hi = min(nb, lo + per);
+    uint32_t* h = W.sort_hist + (size_t)blockIdx.x * W.sort_blocks;
+    uint32_t sum = 0;
+    for (uint32_t j = lo; j < hi; ++j) sum += h[j];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {  // inclusive Hillis-Steele scan
+        const uint32_t v = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[threadIdx.x] - sum;
+    for (uint32_t j = lo; j < hi; ++j) {
+        const uint32_t v = h[j];
+        h[j] = run;
+        run += v;
+    }
+    if (threadIdx.x == 1023) W.sort_hist[(size_t)8 * W.sort_blocks + blockIdx.x] = part[1023];
+}
+
+// Entry p = k * 256 + t of a workgroup goes to its octant's offset plus the
+// entries of that octant before it in the workgroup: the order of (k, wave, lane).
+// LOCAL: the partition stays within each workgroup's kSortPer entries (octant
+// offsets are the workgroup's own), so the queue keeps its coarse order and
+// only the waves' contents change; the octants come from the directions.
+template <bool LOCAL>
+__global__ void __launch_bounds__(256) k_wf_sort_scatter(WfDev W) {
+    __shared__ uint32_t cnt[8][4][4];  // octant, k, wave
+    __shared__ uint32_t tot[8];
+    const uint32_t n = W.counts[Q_ACTIVE * kCtrStride];
+    const uint32_t base = blockIdx.x * (uint32_t)kSortPer;
+    if (base >= n) return;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (threadIdx.x < 128) (&cnt[0][0][0])[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t oc[4], rank[4], slot[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t i = base + k * 256 + threadIdx.x;
+        const bool valid = i < n;
+        slot[k] = valid ? W.q[Q_ACTIVE][i] : 0u;
+        if (LOCAL) oc[k] = valid ? ray_octant(W.d[slot[k]]) : 0u;
+        else oc[k] = valid ? W.mtype[i] : 0u;
+        const uint64_t same = same_octant(oc[k], valid);
+        rank[k] = valid ? (uint32_t)__popcll(same & ((1ull << lane) - 1ull)) : ~0u;
+        if (rank[k] == 0) cnt[oc[k]][k][wave] = (uint32_t)__popcll(same);
+    }
+    __syncthreads();
+    if (LOCAL) {
+        if (threadIdx.x < 8) {
+            uint32_t t = 0;
+            for (int j = 0; j < 16; ++j) t += (&cnt[threadIdx.x][0][0])[j];
+            tot[threadIdx.x] = t;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 8) {  // exclusive scan of each octant's counts in (k, wave) order, from its offset
+        uint32_t run;
+        if (LOCAL) {
+            run = base;
+            for (uint32_t o = 0; o < threadIdx.x; ++o) run += tot[o];
+        } else {
+            const uint32_t* gt = W.sort_hist + (size_t)8 * W.sort_blocks;
+            run = W.sort_hist[threadIdx.x * W.sort_blocks + blockIdx.x];
+            for (uint32_t o = 0; o < threadIdx.x; ++o) run += gt[o];
+        }
+        uint32_t* c = &cnt[threadIdx.x][0][0];
+        for (int j = 0; j < 16; ++j) {
+            const uint32_t v = c[j];
+            c[j] = run;
+            run += v;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (rank[k] != ~0u) W.sort_q[cnt[oc[k]][k][wave] + rank[k]] = slot[k];
+}
+
 // ---------------------------------------------------------------- binning
-// The closest-hit trace writes each hit's material type at the ray's position
-// in the active queue; this pass bins the active queue into the four shade
-// queues in queue order.  The order of the shade queues (neighbouring paths
-// together) then does not depend on when each ray finished, which lets the
+// The closest-hit trace writes each hit's material type at the ray's slot;
+// this pass bins the active queue into the four shade queues in queue order.
+// The order of the shade queues (neighbouring paths together) then does not
+// depend on when each ray finished, which lets the
 // closest-hit waves refill early.  Each thread takes four consecutive entries
 // and the workgroup claims its block of each queue with one atomic.
 constexpr int kBinPerThread = 4;
@@ -437,7 +580,7 @@ __global__ void __launch_bounds__(256) k_wf_bin(WfDev W) {
     for (int k = 0; k < kBinPerThread; ++k) {
         const uint32_t i = i0 + k;
         slot[k] = i < n ? W.q[Q_ACTIVE][i] : 0u;
-        m[k] = i < n ? W.mtype[i] : 255u;
+        m[k] = i < n ? W.mtype[slot[k]] : 255u;
         if (m[k] < 4u) cnt += 1ull << (16 * m[k]);
     }
     // inclusive scan of the packed counts over the wave, then over the waves
@@ -487,7 +630,7 @@ __global__ void __launch_bounds__(256) k_wf_env(R Rd, WfDev W) {
     const SceneDev& S = Rd.scene(Rd.entry(W, slot));
     if (S.env.w == 0) return;
     if (MODE == 0) {
-        if (W.mtype[i] != 255) return;
+        if (W.mtype[slot] != 255) return;
         const uint32_t sw = W.seq[slot].w;
         if ((sw & F_DEPTH_MASK) != 0u && !(sw & F_DELTA)) return;
         const float4 Lv = W.L[slot];
@@ -763,6 +906,10 @@ static WfDev g_view(const WavefrontState& wf, bool batch, size_t* used) {
     W.ent = batch ? (uint32_t*)take(n * 4) : nullptr;
     W.pslot = batch ? (uint32_t*)take(n * 4) : nullptr;
     for (int q = 0; q < Q_COUNT; ++q) W.q[q] = (uint32_t*)take(n * 4);
+    W.sort_blocks = (uint32_t)((n + kSortPer - 1) / kSortPer);
+    W.sort_q = (uint32_t*)take(n * 4);
+    W.trace_q = nullptr;
+    W.sort_hist = (uint32_t*)take(((size_t)W.sort_blocks * 8 + 8) * 4);
     W.counts = (uint32_t*)take(C_WORDS * kCtrStride * 4);
     W.capacity = n;
     W.spill = (uint32_t*)wf.spill;
@@ -772,8 +919,9 @@ static WfDev g_view(const WavefrontState& wf, bool batch, size_t* used) {
 }
 
 static size_t WfBytes(int64_t n, bool batch) {
-    const size_t per = 14 * 16 + 1 + Q_COUNT * 4 + (batch ? 8 : 0);
-    return (size_t)n * per + 256 * (15 + Q_COUNT + 4) + C_WORDS * kCtrStride * 4 + 4096;
+    const size_t per = 14 * 16 + 1 + (Q_COUNT + 1) * 4 + (batch ? 8 : 0);
+    const size_t hist = ((size_t)((n + kSortPer - 1) / kSortPer) * 8 + 8) * 4;
+    return (size_t)n * per + hist + 256 * (15 + Q_COUNT + 6) + C_WORDS * kCtrStride * 4 + 4096;
 }
 
 void WavefrontStateFree(WavefrontState* wf) {
@@ -831,14 +979,28 @@ static unsigned ResolveBlocks(uint32_t n) { return (unsigned)(((uint64_t)n + 256
 constexpr int kTraceChunks[3] = {8, 4, 1};
 constexpr int kAutoRefill = 48;
 template <int STACK, class R>
-static int WfBounceLoop(Ctx* c, const R& Rd, const WfDev& W0, int depth_max, int refill, unsigned blocks, uint32_t n_slots,
-                        bool env, unsigned long long* counters) {
+static int WfBounceLoop(Ctx* c, const R& Rd, const WfDev& W0, int depth_max, int refill, int ray_sort, unsigned blocks,
+                        uint32_t n_slots, bool env, unsigned long long* counters) {
     WfDev W = W0;
     const unsigned pblocks = PersistBlocks(c);
     for (int depth = 0; depth < depth_max; ++depth) {
+        if (ray_sort != GY_RAY_SORT_NONE && depth > 0) {
+            // camera rays keep their pixel-tile order; continuation rays are partitioned
+            TimerBegin(c, kKernOther);
+            if (ray_sort == GY_RAY_SORT_OCTANT) {
+                hipLaunchKernelGGL(k_wf_sort_count, dim3(W.sort_blocks), dim3(256), 0, c->stream, W);
+                hipLaunchKernelGGL(k_wf_sort_scan, dim3(8), dim3(1024), 0, c->stream, W);
+                hipLaunchKernelGGL(k_wf_sort_scatter<false>, dim3(W.sort_blocks), dim3(256), 0, c->stream, W);
+            } else {
+                hipLaunchKernelGGL(k_wf_sort_scatter<true>, dim3(W.sort_blocks), dim3(256), 0, c->stream, W);
+            }
+            TimerEnd(c);
+            W.trace_q = W.sort_q;
+        }
         TimerBegin(c, kKernTraceClosest);
         hipLaunchKernelGGL((k_wf_trace<STACK, 0, R>), dim3(pblocks), dim3(256), 0, c->stream, Rd, W, refill > 0 ? refill : kAutoRefill, kTraceChunks[0]);
         TimerEnd(c);
+        W.trace_q = nullptr;
         TimerBegin(c, kKernOther);  // queue bookkeeping
         hipLaunchKernelGGL(k_wf_bin, dim3((blocks + kBinPerThread - 1) / kBinPerThread), dim3(256), 0, c->stream, W);
         if (env) hipLaunchKernelGGL((k_wf_env<0, R>), dim3(blocks), dim3(256), 0, c->stream, Rd, W);
@@ -909,7 +1071,8 @@ int WavefrontPass(Ctx* c, Scene* sc, const PathParams& P, const gy_render_params
         return CheckLaunch(c, "k_wf_resolve");
     }
     if (sc->stats.max_depth + 1 > 64) return Fail(GY_ERR_UNSUPPORTED, "BVH deeper than the traversal stack");
-    return WfBounceLoop<kStackLds>(c, Rd, W, p.max_depth, RefillOf(p.refill_threshold), blocks, (uint32_t)cap, sc->dev.env.w > 0,
+    return WfBounceLoop<kStackLds>(c, Rd, W, p.max_depth, RefillOf(p.refill_threshold), p.ray_sort, blocks, (uint32_t)cap,
+                                   sc->dev.env.w > 0,
                                    sc->film.counters);
 }
 
@@ -1014,7 +1177,7 @@ __global__ void __launch_bounds__(256) k_lt_shade(SceneDev S, PathParams P, WfDe
     if (blockIdx.x * 256 >= n) return;
     uint32_t slot = 0;
     bool conn = false, next = false;
-    if (i < n && W.mtype[i] != 255) {
+    if (i < n && W.mtype[W.q[Q_ACTIVE][i]] != 255) {
         slot = W.q[Q_ACTIVE][i];
         const V3 o = f3(W.o[slot]), d = f3(W.d[slot]);
         const float4 hv = W.hit[slot];
@@ -1142,7 +1305,7 @@ static int BatchSequence(Ctx* c, Batch* b) {
         hipLaunchKernelGGL(k_wf_raygen_batch, dim3(blocks), dim3(256), 0, c->stream, W, Rd, base, count, lanes);
         TimerEnd(c);
         rc = CheckLaunch(c, "k_wf_raygen_batch");
-        if (rc == GY_OK) rc = WfBounceLoop<kStackLds>(c, Rd, W, b->depth_max, b->refill, blocks, (uint32_t)lanes, env, b->counters);
+        if (rc == GY_OK) rc = WfBounceLoop<kStackLds>(c, Rd, W, b->depth_max, b->refill, b->ray_sort, blocks, (uint32_t)lanes, env, b->counters);
     }
     for (auto& en : b->entries)
         if (rc == GY_OK) rc = FilmResolveState(c, en.film, en.params.spp);

@@ -65,7 +65,7 @@ int main(int argc, char** argv) {
         gy::Check(gy_generate_scene(GY_SCENE_CORNELL, 0, 0, &g), "gy_generate_scene");
         gy_scene_desc desc;
         gy_camera_desc cd;
-        gy_render_params rp;
+        gy_render_params rp = {};
         gy::Check(gy_generated_desc(g, &desc, &cd, &rp), "gy_generated_desc");
         const gy::Scene scene = gy::Scene::FromDesc(desc);
         const gy::Scene flat = FlattenPrimitives(desc);

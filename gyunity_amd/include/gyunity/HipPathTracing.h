@@ -420,6 +420,7 @@ struct RenderParams {
     int samplesPerPass = 0;          // 0 = automatic
     int blockSize = 0;               // megakernel block size, 0 = automatic
     int refillThreshold = 0;         // wavefront traversal refill, 0 = automatic
+    int raySort = GY_RAY_SORT_NONE;  // wavefront closest-hit queue order (same film either way)
 };
 
 // PathTracing on the GPU.  Render() uploads the scene on first use (or when a
@@ -456,6 +457,7 @@ public:
         p.block_size = mParams.blockSize;
         p.refill_threshold = mParams.refillThreshold;
         p.sampler = mpSampler->Kind();
+        p.ray_sort = mParams.raySort;
         return p;
     }
     const Device& GetDevice() const { return *mpDevice; }

@@ -43,9 +43,9 @@ gy_camera_desc CameraOf(const std::vector<double>& c) {
 }
 
 gy_render_params ParamsOf(const std::vector<int64_t>& p) {
-    // spp, max_depth, rr_depth, mode, samples_per_pass, block_size, refill_threshold, sampler
-    if (p.size() != 8) throw std::invalid_argument("params: 8 values expected");
-    gy_render_params r;
+    // spp, max_depth, rr_depth, mode, samples_per_pass, block_size, refill_threshold, sampler, ray_sort
+    if (p.size() != 9) throw std::invalid_argument("params: 9 values expected");
+    gy_render_params r = {};
     r.spp = (int32_t)p[0];
     r.max_depth = (int32_t)p[1];
     r.rr_depth = (int32_t)p[2];
@@ -54,6 +54,7 @@ gy_render_params ParamsOf(const std::vector<int64_t>& p) {
     r.block_size = (int32_t)p[5];
     r.refill_threshold = (int32_t)p[6];
     r.sampler = (int32_t)p[7];
+    r.ray_sort = (int32_t)p[8];
     return r;
 }
 

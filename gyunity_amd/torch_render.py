@@ -33,7 +33,7 @@ def _load():
 def render_textures(scene, cam, params, textures):
     camera = [*cam.pos, *cam.look_at, *cam.up, cam.fovy, cam.film_dist, cam.width, cam.height]
     prm = [params.spp, params.max_depth, params.rr_depth, params.mode, params.samples_per_pass, params.block_size,
-           params.refill_threshold, params.sampler]
+           params.refill_threshold, params.sampler, params.ray_sort]
     return _load().render_textures(textures, scene.h.value, [float(v) for v in camera], [int(v) for v in prm])
 
 

@@ -91,7 +91,14 @@ typedef struct {
                                 wave than this are busy (0 = automatic, 48; 1..64) */
     int32_t sampler;         /* GY_SAMPLER_SOBOL: SobolSampler + SobolEnum;
                                 GY_SAMPLER_HALTON: HaltonSampler (Faure) + HaltonEnum */
+    int32_t ray_sort;        /* wavefront: order of the closest-hit queue from the
+                                second bounce on: GY_RAY_SORT_NONE (as generated),
+                                GY_RAY_SORT_OCTANT (stable partition of the whole queue
+                                by direction octant), GY_RAY_SORT_OCTANT_LOCAL (the same
+                                within each run of 1024 entries).  The image does not
+                                depend on it. */
 } gy_render_params;
+enum { GY_RAY_SORT_NONE = 0, GY_RAY_SORT_OCTANT = 1, GY_RAY_SORT_OCTANT_LOCAL = 2 };
 enum { GY_SAMPLER_SOBOL = 0, GY_SAMPLER_HALTON = 1 };
 
 /* BVH build knobs (BVHAccel(prims, maxPrims, "sah"), accelerator/BVHAccel.h:50-51). */
@@ -278,11 +285,16 @@ int gy_count_traffic(gy_scene* scene, const gy_camera_desc* cam, const gy_render
  * integrator/TiledIntegrator.cpp:13).  Times pilot renders of this scene and
  * view (pilot_spp samples per pixel; 0 = the render itself up to 2^23 paths,
  * else the fewest samples that reach 2^23 paths): the integrator form first,
- * then the megakernel workgroup size or the wavefront refill threshold of the
- * faster form; writes base with the fastest choice into best.  The image does
- * not depend on the choice. */
+ * then the megakernel workgroup size, or the wavefront refill threshold and
+ * closest-hit queue order (ray_sort), of the faster form; writes base with the
+ * fastest choice into best.  The image does not depend on the choice.  Choices
+ * are kept per scene signature (the scene, the revision of its environment
+ * light / spheres / media, camera, spp, depth, roulette, sampler, pilot_spp):
+ * asking again returns the kept choice without timing. */
 int gy_autotune(gy_scene* scene, const gy_camera_desc* cam, const gy_render_params* base, int32_t pilot_spp,
                 gy_render_params* best, double* best_ms_per_mpath);
+/* Forget every kept choice: the next gy_autotune times again. */
+int gy_autotune_clear_cache(void);
 
 /* ---- batched rendering ------------------------------------------------------
  * No counterpart in the reference (TiledIntegrator::Render draws one image per
@@ -298,6 +310,8 @@ typedef struct {
     int64_t lanes_per_pass;   /* paths in flight per pass: 0 = automatic (32M) */
     int32_t refill_threshold; /* persistent trace refill (0 = automatic, 1..64) */
     int32_t use_graph;        /* 1: record once into a hipGraph and replay */
+    int32_t ray_sort;         /* GY_RAY_SORT_*, as in gy_render_params */
+    int32_t reserved;
 } gy_batch_params;
 int gy_batch_create(gy_context* ctx, const gy_batch_params* p, gy_batch** out);
 /* The batch holds the scene until gy_batch_destroy (gy_scene_destroy refuses

@@ -3,6 +3,7 @@ include/gyunity.h declares, and fails cleanly where a device is required."""
 import ctypes
 import os
 import re
+import subprocess
 
 import numpy as np
 import pytest
@@ -27,10 +28,29 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
-def test_struct_sizes_match_header_layout():
+def test_struct_sizes_match_header_layout(tmp_path):
     assert ctypes.sizeof(_lib.CameraDesc) == 52
-    assert ctypes.sizeof(_lib.RenderParams) == 32
-    assert ctypes.sizeof(_lib.BatchParams) == 16
+    assert ctypes.sizeof(_lib.RenderParams) == 36
+    assert ctypes.sizeof(_lib.BatchParams) == 24
+    # the C compiler's layout of the parameter structs equals the binding's
+    structs = {"gy_render_params": _lib.RenderParams, "gy_batch_params": _lib.BatchParams,
+               "gy_camera_desc": _lib.CameraDesc, "gy_sppm_params": _lib.SppmParams}
+    src = ["#include <stdio.h>", "#include <stddef.h>", '#include "gyunity.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        src.append('printf("%%s %%zu\\n", "%s", sizeof(%s));' % (cname, cname))
+        for f, _ in py._fields_:
+            src.append('printf("%%s.%%s %%zu\\n", "%s", "%s", offsetof(%s, %s));' % (cname, f, cname, f))
+    src.append("return 0; }")
+    (tmp_path / "layout.c").write_text("\n".join(src))
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    r = subprocess.run(["gcc", "-I", inc, str(tmp_path / "layout.c"), "-o", str(tmp_path / "layout")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    out = dict(line.rsplit(" ", 1) for line in subprocess.run([str(tmp_path / "layout")], capture_output=True, text=True,
+                                                               check=True).stdout.split("\n") if line)
+    for cname, py in structs.items():
+        assert int(out[cname]) == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert int(out["%s.%s" % (cname, f)]) == getattr(py, f).offset, (cname, f)
     assert g.RAY_DTYPE.itemsize == 32 and g.HIT_DTYPE.itemsize == 16
 
 

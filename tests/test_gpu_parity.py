@@ -11,6 +11,7 @@ tests/golden/level_c.json); megakernel and wavefront identical.
 """
 import json
 import os
+import time
 
 import numpy as np
 import pytest
@@ -246,6 +247,14 @@ def test_wavefront_refill_does_not_change_the_film(gpu_ctx):
             prm.refill_threshold = r
             c, w = sc.render(cam, prm)
             assert np.array_equal(c, ref[0]) and np.array_equal(w, ref[1]), r
+        # The octant partition of the closest-hit queue changes only which
+        # lane traces which path.
+        for srt in (g.GY_RAY_SORT_OCTANT, g.GY_RAY_SORT_OCTANT_LOCAL):
+            prm.ray_sort = srt
+            for r in (0, 16):
+                prm.refill_threshold = r
+                c, w = sc.render(cam, prm)
+                assert np.array_equal(c, ref[0]) and np.array_equal(w, ref[1]), (srt, r)
 
 
 def test_film_save_matches_downloaded_film(gpu_ctx, tmp_path):
@@ -503,8 +512,9 @@ def test_batch_films_equal_single_renders(gpu_ctx):
         prm.samples_per_pass = prm.spp
         refs.append(scenes[key].render(cam, prm))
     try:
-        for use_graph in (True, False):
-            with g.Batch(gpu_ctx, lanes_per_pass=40000, use_graph=use_graph) as b:
+        for use_graph, ray_sort in ((True, g.GY_RAY_SORT_NONE), (False, g.GY_RAY_SORT_NONE), (True, g.GY_RAY_SORT_OCTANT),
+                                    (False, g.GY_RAY_SORT_OCTANT_LOCAL)):
+            with g.Batch(gpu_ctx, lanes_per_pass=40000, use_graph=use_graph, ray_sort=ray_sort) as b:
                 ids = [b.add(scenes[id(sa)], cam, prm) for sa, cam, prm in entries]
                 for rep in range(3 if use_graph else 1):
                     b.render()
@@ -512,8 +522,8 @@ def test_batch_films_equal_single_renders(gpu_ctx):
                     assert st.paths == sum(c.width * c.height * p.spp for _, c, p in entries)
                     for i, e in enumerate(ids):
                         color, weight = b.film(e)
-                        assert np.array_equal(weight, refs[i][1]), (use_graph, rep, i)
-                        assert np.array_equal(color, refs[i][0]), (use_graph, rep, i)
+                        assert np.array_equal(weight, refs[i][1]), (use_graph, ray_sort, rep, i)
+                        assert np.array_equal(color, refs[i][0]), (use_graph, ray_sort, rep, i)
                     # eager work on the same stream between replays
                     sa, cam, prm = entries[0]
                     scenes[id(sa)].render(cam, prm)
@@ -857,12 +867,27 @@ def test_autotune_returns_a_faster_equivalent_launch(gpu_ctx):
     sa, cam, prm = scenes()["cornell"]
     cam.width = cam.height = 96
     prm.spp = 8
+    g.autotune_clear_cache()
     with g.Scene(gpu_ctx, sa) as sc:
         best, ms_per_mpath = sc.autotune(cam, prm, 4)
         assert best.mode in (g.GY_RENDER_MEGAKERNEL, g.GY_RENDER_WAVEFRONT) and ms_per_mpath > 0
+        assert best.ray_sort in (g.GY_RAY_SORT_NONE, g.GY_RAY_SORT_OCTANT, g.GY_RAY_SORT_OCTANT_LOCAL)
         assert best.spp == prm.spp and best.max_depth == prm.max_depth
+        # the same scene signature is answered from the kept choice: no pilot render
+        before = gpu_ctx.stats().total_ms
+        t0 = time.perf_counter()
+        again, ms2 = sc.autotune(cam, prm, 4)
+        assert time.perf_counter() - t0 < 0.05
+        assert gpu_ctx.stats().total_ms == before and ms2 == ms_per_mpath
+        assert (again.mode, again.block_size, again.refill_threshold, again.ray_sort) == (
+            best.mode, best.block_size, best.refill_threshold, best.ray_sort)
         best.samples_per_pass = prm.samples_per_pass = prm.spp
         a = sc.render(cam, best)
         prm.mode = g.GY_RENDER_WAVEFRONT
         b = sc.render(cam, prm)
+        # another depth is another signature: timed again
+        prm.max_depth += 1
+        before = gpu_ctx.stats().total_ms
+        sc.autotune(cam, prm, 4)
+        assert gpu_ctx.stats().total_ms != before
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
