@@ -38,10 +38,9 @@ def kernel_bytes(name, st, paths):
     if name == "wf_trace_closest":
         # + ray (32 B) read, hit (16 B) written, queue entry read and pushed
         return 64.0 * nv[0] + 48.0 * tv[0] + 56.0 * primary
-    if name == "wf_trace_shadow":
-        return 64.0 * nv[1] + 48.0 * tv[1] + 36.0 * st.shadow_rays
-    if name == "wf_trace_mis":
-        return 64.0 * nv[2] + 48.0 * tv[2] + 36.0 * st.mis_rays
+    if name == "wf_trace_shadow_mis":
+        # shadow rays, then MIS rays, in one launch
+        return 64.0 * (nv[1] + nv[2]) + 48.0 * (tv[1] + tv[2]) + 36.0 * (st.shadow_rays + st.mis_rays)
     if name == "wf_shade":
         # shading record 128 B, material 128 B, path state ~200 B per shading point
         return 456.0 * st.shade_points

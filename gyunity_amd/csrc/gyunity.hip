@@ -1521,7 +1521,7 @@ extern "C" int gy_render_stats_get(gy_context* ctx, gy_render_stats* out) {
 
 extern "C" const char* gy_kernel_name(int i) {
     static const char* names[8] = {"pt_megakernel", "film_resolve", "wf_raygen", "wf_trace_closest", "wf_shade",
-                                   "wf_trace_shadow", "wf_trace_mis", "wf_other"};
+                                   "wf_trace_shadow_mis", "wf_trace_mis", "wf_other"};  // [6]: no longer timed apart
     return (i >= 0 && i < 8) ? names[i] : "";
 }
 

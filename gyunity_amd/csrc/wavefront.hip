@@ -5,7 +5,8 @@
 //
 //   raygen -> [ trace_closest (bins hits by material)
 //               -> shade<material> (emission / NEE set-up / BSDF continuation)
-//               -> trace_shadow (any hit) -> trace_mis (closest hit + emission)
+//               -> trace_shadow_mis (shadow rays: any hit; then MIS rays:
+//                  closest hit + emission; one persistent launch)
 //               -> resolve (adds T_i (L1 w1 + L2 w2) / P(light), finishes paths) ] x depth
 //
 // Every FP32 operation is the megakernel's, in the same order, so a render is
@@ -279,8 +280,7 @@ __global__ void __launch_bounds__(256) k_wf_raygen_batch(WfDev W, ManyRenders R,
 constexpr uint32_t kBatch = 64;
 
 template <int S, int MODE, class R>
-__global__ void __launch_bounds__(256) k_wf_trace(R Rd, WfDev W, int refill, int chunks) {
-    __shared__ uint32_t stack[S * 256];
+__device__ inline void wf_trace_loop(const R& Rd, const WfDev& W, int refill, int chunks, uint32_t* stack) {
     constexpr bool kAny = MODE == 1 || MODE == 3;
     constexpr int qid = MODE == 0 ? Q_ACTIVE : (kAny ? Q_SHADOW : Q_MIS);
     const uint32_t n = W.counts[(qid) * kCtrStride];
@@ -370,10 +370,8 @@ __global__ void __launch_bounds__(256) k_wf_trace(R Rd, WfDev W, int refill, int
                             W.pend3[slot].w = __uint_as_float(P_DONE);  // the path ends (k_wf_env: environment)
                         }
                     } else if (MODE == 1) {
-                        if (r == 2) {
-                            float& fw = W.pend3[slot].w;
-                            fw = __uint_as_float(__float_as_uint(fw) | P_OCCLUDED);
-                        }
+                        // atomic: the slot's MIS ray may be in flight in the same launch
+                        if (r == 2) atomicOr(reinterpret_cast<uint32_t*>(&W.pend3[slot].w), P_OCCLUDED);
                     } else if (MODE == 3) {
                         if constexpr (MODE == 3) {
                             // Film::AddSplat (visual/Film.h:89-103) of an unoccluded connection
@@ -387,10 +385,7 @@ __global__ void __launch_bounds__(256) k_wf_trace(R Rd, WfDev W, int refill, int
                         }
                     } else if (t.best_prim < 0) {
                         // a miss counts for an environment light (k_wf_env)
-                        if (Rd.scene(e).env.w > 0) {
-                            float& fw = W.pend3[slot].w;
-                            fw = __uint_as_float(__float_as_uint(fw) | P_ESC);
-                        }
+                        if (Rd.scene(e).env.w > 0) atomicOr(reinterpret_cast<uint32_t*>(&W.pend3[slot].w), P_ESC);
                     } else {
                         // MIS branch of DirectIllumination (integrator/Integrator.cpp:88-98)
                         const SceneDev& Sf = Rd.scene(e);
@@ -410,8 +405,7 @@ __global__ void __launch_bounds__(256) k_wf_trace(R Rd, WfDev W, int refill, int
                             const V3 L2 = Lem * f3(p1) * v3s(1.f) * p1.w / p2.x;
                             const V3 L2w2 = L2 * p2.y;
                             W.pend1[slot] = make_float4(L2w2.x, L2w2.y, L2w2.z, p1.w);
-                            float& fw = W.pend3[slot].w;
-                            fw = __uint_as_float(__float_as_uint(fw) | P_L2);
+                            atomicOr(reinterpret_cast<uint32_t*>(&W.pend3[slot].w), P_L2);
                         }
                     }
                     my = -1;
@@ -419,6 +413,24 @@ __global__ void __launch_bounds__(256) k_wf_trace(R Rd, WfDev W, int refill, int
             }
         }
     }
+}
+
+template <int S, int MODE, class R>
+__global__ void __launch_bounds__(256) k_wf_trace(R Rd, WfDev W, int refill, int chunks) {
+    __shared__ uint32_t stack[S * 256];
+    wf_trace_loop<S, MODE, R>(Rd, W, refill, chunks, stack);
+}
+
+// Shadow rays, then MIS rays, in one persistent launch: a wave that finds the
+// shadow queue drained and its lanes idle goes on to the MIS queue while other
+// waves finish their last shadow rays, so the small MIS trace fills the shadow
+// trace's tail instead of running as a launch of its own.  The two kinds set
+// different flag bits of a slot with atomic ORs.
+template <int S, class R>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) k_wf_trace_shadow_mis(R Rd, WfDev W, int refill, int chunks_shadow, int chunks_mis) {
+    __shared__ uint32_t stack[S * 256];
+    wf_trace_loop<S, 1, R>(Rd, W, refill, chunks_shadow, stack);
+    wf_trace_loop<S, 2, R>(Rd, W, refill, chunks_mis, stack);
 }
 
 // ---------------------------------------------------------------- ray sort
@@ -940,11 +952,11 @@ static unsigned PersistBlocks(Ctx* c) {
     if (per_cu == 0) {
         int k[6] = {0, 0, 0, 0, 0, 0};
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&k[0], k_wf_trace<kStackLds, 0, OneRender>, 256, 0);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&k[1], k_wf_trace<kStackLds, 1, OneRender>, 256, 0);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&k[2], k_wf_trace<kStackLds, 2, OneRender>, 256, 0);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&k[3], k_wf_trace<kStackLds, 0, ManyRenders>, 256, 0);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&k[4], k_wf_trace<kStackLds, 1, ManyRenders>, 256, 0);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&k[5], k_wf_trace<kStackLds, 2, ManyRenders>, 256, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&k[1], k_wf_trace_shadow_mis<kStackLds, OneRender>, 256, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&k[2], k_wf_trace<kStackLds, 0, ManyRenders>, 256, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&k[3], k_wf_trace_shadow_mis<kStackLds, ManyRenders>, 256, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&k[4], k_wf_trace<kStackLds, 0, LightRender>, 256, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&k[5], k_wf_trace<kStackLds, 3, LightRender>, 256, 0);
         per_cu = std::max(1, *std::max_element(k, k + 6));
     }
     return (unsigned)(per_cu * c->num_cu);
@@ -1018,11 +1030,9 @@ static int WfBounceLoop(Ctx* c, const R& Rd, const WfDev& W0, int depth_max, int
             hipLaunchKernelGGL((k_wf_shade<3, R, false>), dim3(blocks), dim3(256), 0, c->stream, Rd, W);
         }
         TimerEnd(c);
-        TimerBegin(c, kKernTraceShadow);
-        hipLaunchKernelGGL((k_wf_trace<STACK, 1, R>), dim3(pblocks), dim3(256), 0, c->stream, Rd, W, refill > 0 ? refill : kAutoRefill, kTraceChunks[1]);
-        TimerEnd(c);
-        TimerBegin(c, kKernTraceMis);
-        hipLaunchKernelGGL((k_wf_trace<STACK, 2, R>), dim3(pblocks), dim3(256), 0, c->stream, Rd, W, refill > 0 ? refill : kAutoRefill, kTraceChunks[2]);
+        TimerBegin(c, kKernTraceShadow);  // shadow and MIS rays
+        hipLaunchKernelGGL((k_wf_trace_shadow_mis<STACK, R>), dim3(pblocks), dim3(256), 0, c->stream, Rd, W,
+                           refill > 0 ? refill : kAutoRefill, kTraceChunks[1], kTraceChunks[2]);
         TimerEnd(c);
         TimerBegin(c, kKernOther);
         if (env) hipLaunchKernelGGL((k_wf_env<1, R>), dim3(blocks), dim3(256), 0, c->stream, Rd, W);
