@@ -8,10 +8,15 @@
 //      hit point in use, cells of twice the search radius over the hit points'
 //      box inflated by it, each hit point listed in the bucket of every cell
 //      its radius box touches; built with per-bucket counts, a scan and a fill;
-//   3. photon pass, one thread per photon (GeneratePhoton + TracePhoton): at
-//      every non-delta vertex after the first, each listed hit point within
-//      the iteration's starting radius and facing the same way becomes a
-//      record (hit point, photon * depth + vertex, contribution, distance^2);
+//   3. photons (GeneratePhoton + TracePhoton): at every non-delta vertex
+//      after the first, each listed hit point within the iteration's starting
+//      radius and facing the same way becomes a record (hit point, photon *
+//      depth + vertex, contribution, distance^2).  A photon's path does not
+//      depend on the hit points, so the paths of many iterations are traced
+//      in one launch (one thread per photon; 300,000 photons alone are too few
+//      threads to hide the traversal's latency), each depositing vertex kept
+//      in its iteration's vertex list; the iteration's deposit pass then runs
+//      one thread per vertex against that iteration's hash grid;
 //   4. the records are sorted by (hit point, photon, vertex) and each hit
 //      point applies its own in that order: the progressive radius/flux update
 //      with the radius test at the current radius, or (batch shrink) the count
@@ -47,7 +52,7 @@ struct SppmDev {
 struct GridDev {
     const uint32_t* start;  // num + 1 bucket offsets
     uint32_t* cursor;
-    uint32_t* entries;      // hit point (pixel) per entry
+    float4* entries;        // 2 per entry: hit point position + radius^2, |normal| + hit point (pixel)
     float3 bmin;
     float inv;
     int64_t num;
@@ -56,6 +61,15 @@ struct RecDev {
     unsigned long long* keys;  // (hit point << 32) | (photon * max_depth + vertex)
     float4* val;               // contribution, distance^2
     unsigned int* count;
+    unsigned int cap;
+};
+// Depositing photon vertices, one list of `cap` per iteration of a chunk:
+// 4 float4 each: ray origin + hit t, ray direction + primitive, barycentrics
+// + photon + vertex, photon flux.  The deposit pass rebuilds the surface
+// interaction and BSDF from the ray and hit, as the tracing thread had them.
+struct VtxDev {
+    float4* v;
+    unsigned int* count;  // per iteration of the chunk; counts past cap too
     unsigned int cap;
 };
 
@@ -68,6 +82,34 @@ __device__ inline V3 mul_m1(V3 d) { return d * -1.f; }  // -1 * v (a product, no
 __device__ inline uint32_t grid_hash(int ix, int iy, int iz, int64_t num) {
     const uint32_t h = ((uint32_t)ix * 73856093u) ^ ((uint32_t)iy * 19349663u) ^ ((uint32_t)iz * 83492791u);
     return (uint32_t)((int64_t)h % num);
+}
+// Appends for the lanes with pred set, one atomic per (wave, list): lane l of
+// list r gets counters[r] + its rank among that list's lanes.  Every lane of
+// the wave that reaches the call must make it.
+__device__ inline uint32_t wave_append(unsigned int* counters, uint32_t list, uint32_t n, bool pred) {
+    const int lane = threadIdx.x & 63;
+    uint64_t pend = __ballot(pred && n > 0);
+    uint32_t mine = 0;
+    while (pend) {
+        const int leader = __ffsll((unsigned long long)pend) - 1;
+        const uint32_t lr = __shfl(list, leader, 64);
+        const bool in = pred && n > 0 && list == lr;
+        const uint64_t grp = __ballot(in);
+        // exclusive prefix of n over the group's lanes
+        uint32_t incl = in ? n : 0u;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t v = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += v;
+        }
+        const uint32_t total = __shfl(incl, 63, 64);
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&counters[lr], total);
+        base = __shfl(base, leader, 64);
+        if (in) mine = base + incl - n;
+        pend &= ~grp;
+    }
+    return mine;
 }
 __device__ inline uint32_t ordered(float f) {  // float -> uint32 with the same order
     const uint32_t u = __float_as_uint(f);
@@ -135,22 +177,54 @@ __global__ void __launch_bounds__(128) k_sppm_eye(gyd::SceneDev S, PathParams P,
     if (has_add) D.direct[p] = f4(f3(D.direct[p]) + add, 0.f);
 }
 
+__device__ inline uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v = max(v, (uint32_t)__shfl_xor(v, m, 64));
+    return v;
+}
+__device__ inline uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v = min(v, (uint32_t)__shfl_xor(v, m, 64));
+    return v;
+}
 // Hit points in use: count and the largest radius^2 (non-negative floats
-// order as their bits).
-__global__ void k_sppm_stats(SppmDev D, unsigned int* out) {
-    const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (p >= (int64_t)D.W * D.H || D.pos[p].w == 0.f) return;
-    atomicAdd(&out[0], 1u);
-    atomicMax(&out[1], __float_as_uint(D.r2[p]));
+// order as their bits).  A grid-stride loop over few workgroups, reduced over
+// each wave: one atomic per wave and quantity.
+constexpr unsigned kReduceBlocks = 256;
+__global__ void __launch_bounds__(256) k_sppm_stats(SppmDev D, unsigned int* out) {
+    uint32_t n = 0, mx = 0;
+    for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < (int64_t)D.W * D.H; p += (int64_t)gridDim.x * 256)
+        if (D.pos[p].w != 0.f) {
+            ++n;
+            mx = max(mx, __float_as_uint(D.r2[p]));
+        }
+    mx = wave_max_u32(mx);
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) n += (uint32_t)__shfl_xor(n, m, 64);
+    if ((threadIdx.x & 63) == 0 && n > 0) {
+        atomicAdd(&out[0], n);
+        atomicMax(&out[1], mx);
+    }
 }
 // The lower corner of the hit points' box inflated by the search radius.
-__global__ void k_sppm_bbox(SppmDev D, float search, unsigned int* out) {
-    const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (p >= (int64_t)D.W * D.H || D.pos[p].w == 0.f) return;
-    const float4 a = D.pos[p];
-    atomicMin(&out[0], ordered(a.x - search));
-    atomicMin(&out[1], ordered(a.y - search));
-    atomicMin(&out[2], ordered(a.z - search));
+__global__ void __launch_bounds__(256) k_sppm_bbox(SppmDev D, float search, unsigned int* out) {
+    uint32_t x = 0xFFFFFFFFu, y = 0xFFFFFFFFu, z = 0xFFFFFFFFu;
+    for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p < (int64_t)D.W * D.H; p += (int64_t)gridDim.x * 256) {
+        const float4 a = D.pos[p];
+        if (a.w != 0.f) {
+            x = min(x, ordered(a.x - search));
+            y = min(y, ordered(a.y - search));
+            z = min(z, ordered(a.z - search));
+        }
+    }
+    x = wave_min_u32(x);
+    y = wave_min_u32(y);
+    z = wave_min_u32(z);
+    if ((threadIdx.x & 63) == 0 && x != 0xFFFFFFFFu) {
+        atomicMin(&out[0], x);
+        atomicMin(&out[1], y);
+        atomicMin(&out[2], z);
+    }
 }
 template <bool FILL>
 __global__ void k_sppm_grid(SppmDev D, GridDev G, uint32_t* counts) {
@@ -165,86 +239,162 @@ __global__ void k_sppm_grid(SppmDev D, GridDev G, uint32_t* counts) {
         for (int iy = abs(int(BMin.y)); iy <= abs(int(BMax.y)); iy++)
             for (int ix = abs(int(BMin.x)); ix <= abs(int(BMax.x)); ix++) {
                 const uint32_t h = grid_hash(ix, iy, iz, G.num);
-                if (FILL) G.entries[atomicAdd(&G.cursor[h], 1u)] = (uint32_t)p;
+                if (FILL) {
+                    const size_t k = atomicAdd(&G.cursor[h], 1u);
+                    G.entries[2 * k] = make_float4(pos.x, pos.y, pos.z, D.r2[p]);
+                    const float4 nv = D.nrm[p];
+                    G.entries[2 * k + 1] = make_float4(nv.x, nv.y, nv.z, __uint_as_float((uint32_t)p));
+                }
                 else atomicAdd(&counts[h], 1u);
             }
 }
 
-// SPPM::GeneratePhoton + TracePhoton for photon j of iteration iter (sampler
-// instance iter * photons + j).
+// SPPM::GeneratePhoton + TracePhoton for photon j of iteration iter0 + r
+// (thread r * photons + j; sampler instance iter * photons + j).  Each
+// depositing vertex (non-delta, after the first) goes to list r.
 template <int STACK>
-__global__ void __launch_bounds__(128) k_sppm_photon(gyd::SceneDev S, PathParams P, SppmDev D, GridDev G, RecDev Rec, int iter,
-                                                     int64_t photons) {
+__global__ void __launch_bounds__(128) k_sppm_trace(gyd::SceneDev S, PathParams P, VtxDev V, int iter0, int n_iter,
+                                                    int64_t photons) {
     __shared__ uint32_t stack[STACK * 128];
-    const int64_t j = (int64_t)blockIdx.x * 128 + threadIdx.x;
-    if (j >= photons) return;
+    const int64_t t = (int64_t)blockIdx.x * 128 + threadIdx.x;
+    if ((int64_t)blockIdx.x * 128 >= photons * n_iter) return;  // workgroup-uniform
+    const bool valid = t < photons * n_iter;
+    const uint32_t r = valid ? (uint32_t)(t / photons) : 0u;
+    const int64_t j = valid ? t - (int64_t)r * photons : 0;
     uint32_t* st = stack + threadIdx.x;
     const gyd::SampleSrc& M = P.smp;
     gyd::Seq q;
-    q.index = (uint64_t)iter * (uint64_t)photons + (uint64_t)j;
+    q.index = (uint64_t)(iter0 + (int64_t)r) * (uint64_t)photons + (uint64_t)j;
     q.cursor = 0;
-    float lightPdf;
-    const int light = gyd::sample_one_light(S, gyd::seq_next(M, q), &lightPdf);
-    const float u0 = gyd::seq_next(M, q), u1 = gyd::seq_next(M, q);
-    const float w0 = gyd::seq_next(M, q), w1 = gyd::seq_next(M, q);
-    // AreaLight::SampleLight = SampleOnLight (light/AreaLight.h:60-71): Triangle::Sample, cosine direction
-    const int ltri = S.light_tri[light];
-    const float4* tq = S.shade + (size_t)ltri * 8;
-    const V3 p0 = gyd::ld3(tq[0]), p1 = gyd::ld3(tq[1]), p2 = gyd::ld3(tq[2]);
-    const V3 n0 = gyd::ld3(tq[3]), n1 = gyd::ld3(tq[4]), n2 = gyd::ld3(tq[5]), fn = gyd::ld3(tq[6]);
-    const float su0 = sqrtf(u0);
-    const float b0 = 1 - su0, b1 = u1 * su0;
-    const V3 lpos = (b0 * p0 + b1 * p1) + (1 - b0 - b1) * p2;
-    const V3 ln = (b0 * n0 + b1 * n1) + (1 - b0 - b1) * n2;
-    const V3 lerr = gyd::gamma_n(6) * ((gyd::vabs(b0 * p0) + gyd::vabs(b1 * p1)) + gyd::vabs((1 - b0 - b1) * p2));
-    const float pdfPos = 1 / tq[2].w;
-    V3 ss, ts;
-    gyd::coordinate_system(ln, &ss, &ts);
-    const V3 dl = gyd::cosine_hemisphere(w0, w1);
-    V3 d = gyd::norm((ss * dl.x + ts * dl.y) + ln * dl.z);
-    const float pdfDir = dl.z * gyd::kInvPI;
-    V3 o = gyd::offset_origin(lpos, lerr, fn, d) + d * gyd::kRayEps;  // lightPoint.SpawnRay
-    const float cosTheta = fabsf(gyd::dot(ln, d));
-    const V3 Le = v3(S.light_le[3 * light], S.light_le[3 * light + 1], S.light_le[3 * light + 2]);
-    V3 flux = (pdfPos == 0.f || pdfDir == 0.f) ? v3s(0.f) : Le * cosTheta / (pdfPos * pdfDir * lightPdf);
+    bool alive = valid;
+    V3 o = v3s(0.f), d = v3s(0.f), flux = v3s(0.f);
+    if (alive) {
+        float lightPdf;
+        const int light = gyd::sample_one_light(S, gyd::seq_next(M, q), &lightPdf);
+        const float u0 = gyd::seq_next(M, q), u1 = gyd::seq_next(M, q);
+        const float w0 = gyd::seq_next(M, q), w1 = gyd::seq_next(M, q);
+        // AreaLight::SampleLight = SampleOnLight (light/AreaLight.h:60-71): Triangle::Sample, cosine direction
+        const int ltri = S.light_tri[light];
+        const float4* tq = S.shade + (size_t)ltri * 8;
+        const V3 p0 = gyd::ld3(tq[0]), p1 = gyd::ld3(tq[1]), p2 = gyd::ld3(tq[2]);
+        const V3 n0 = gyd::ld3(tq[3]), n1 = gyd::ld3(tq[4]), n2 = gyd::ld3(tq[5]), fn = gyd::ld3(tq[6]);
+        const float su0 = sqrtf(u0);
+        const float b0 = 1 - su0, b1 = u1 * su0;
+        const V3 lpos = (b0 * p0 + b1 * p1) + (1 - b0 - b1) * p2;
+        const V3 ln = (b0 * n0 + b1 * n1) + (1 - b0 - b1) * n2;
+        const V3 lerr = gyd::gamma_n(6) * ((gyd::vabs(b0 * p0) + gyd::vabs(b1 * p1)) + gyd::vabs((1 - b0 - b1) * p2));
+        const float pdfPos = 1 / tq[2].w;
+        V3 ss, ts;
+        gyd::coordinate_system(ln, &ss, &ts);
+        const V3 dl = gyd::cosine_hemisphere(w0, w1);
+        d = gyd::norm((ss * dl.x + ts * dl.y) + ln * dl.z);
+        const float pdfDir = dl.z * gyd::kInvPI;
+        o = gyd::offset_origin(lpos, lerr, fn, d) + d * gyd::kRayEps;  // lightPoint.SpawnRay
+        const float cosTheta = fabsf(gyd::dot(ln, d));
+        const V3 Le = v3(S.light_le[3 * light], S.light_le[3 * light + 1], S.light_le[3 * light + 2]);
+        flux = (pdfPos == 0.f || pdfDir == 0.f) ? v3s(0.f) : Le * cosTheta / (pdfPos * pdfDir * lightPdf);
+    }
     for (int i = 0; i < P.max_depth; ++i) {
+        if (__ballot(alive) == 0) break;
         gyd::Hit h;
-        if (!gyd::find_hit<false, STACK>(S, o, d, 0.f, gyd::kInf, &h, st, 128)) return;
-        const gyd::Isect it = gyd::query(S, o, d, h);
-        gyd::BSDF b = gyd::make_bsdf(S, it);
-        b.importance = true;  // ComputeScatteringFunction(arena, TransportMode::Importance)
-        const float r0 = gyd::seq_next(M, q), r1 = gyd::seq_next(M, q), r2 = gyd::seq_next(M, q);
-        V3 wi;
-        float pdf;
-        const V3 f = gyd::bsdf_sample(b, mul_m1(d), &wi, &pdf, r0, r1, r2);
-        if (gyd::is_zero(f) || pdf == 0.f) break;
-        const V3 est = f * fabsf(gyd::dot(wi, it.n)) / pdf;
-        if (!gyd::bsdf_delta(b) && i > 0 && G.num > 0) {
-            const V3 hh = (it.pos - v3(G.bmin.x, G.bmin.y, G.bmin.z)) * G.inv;
-            const uint32_t bk = grid_hash(abs(int(hh.x)), abs(int(hh.y)), abs(int(hh.z)), G.num);
-            const uint32_t e1 = G.start[bk + 1];
-            for (uint32_t e = G.start[bk]; e < e1; ++e) {
-                const uint32_t hp = G.entries[e];
-                const V3 vv = f3(D.pos[hp]) - it.pos;
-                const float dd = gyd::dot(vv, vv);
-                if ((double)gyd::dot(f3(D.nrm[hp]), it.absN) > 0.0015 && dd <= D.r2[hp]) {
-                    const V3 c = f3(D.imp[hp]) * gyd::bsdf_eval(b, f3(D.out[hp]), mul_m1(d)) * flux;
-                    const unsigned int slot = atomicAdd(Rec.count, 1u);
-                    if (slot < Rec.cap) {
-                        Rec.keys[slot] = ((unsigned long long)hp << 32) | (unsigned long long)((uint64_t)j * P.max_depth + i);
-                        Rec.val[slot] = f4(c, dd);
+        bool deposit = false;
+        V3 o_next = o, wi = d, flux_next = flux;
+        if (alive) {
+            if (!gyd::find_hit<false, STACK>(S, o, d, 0.f, gyd::kInf, &h, st, 128)) {
+                alive = false;
+            } else {
+                const gyd::Isect it = gyd::query(S, o, d, h);
+                gyd::BSDF b = gyd::make_bsdf(S, it);
+                b.importance = true;  // ComputeScatteringFunction(arena, TransportMode::Importance)
+                const float r0 = gyd::seq_next(M, q), r1 = gyd::seq_next(M, q), r2 = gyd::seq_next(M, q);
+                float pdf;
+                const V3 f = gyd::bsdf_sample(b, mul_m1(d), &wi, &pdf, r0, r1, r2);
+                if (gyd::is_zero(f) || pdf == 0.f) {
+                    alive = false;
+                } else {
+                    const V3 est = f * fabsf(gyd::dot(wi, it.n)) / pdf;
+                    deposit = !gyd::bsdf_delta(b) && i > 0;
+                    const float pr = gyd::rmin(1.0f, gyd::lumY(est * flux) / gyd::lumY(flux));
+                    if (pr < 1.f) {
+                        if (gyd::seq_next(M, q) < pr) flux_next = flux / pr;
+                        else alive = false;
+                    }
+                    if (alive) {
+                        flux_next = flux_next * est;
+                        gyd::spawn_ray(it, wi, &o_next);
                     }
                 }
             }
         }
-        const float pr = gyd::rmin(1.0f, gyd::lumY(est * flux) / gyd::lumY(flux));
-        if (pr < 1.f) {
-            if (gyd::seq_next(M, q) < pr) flux = flux / pr;
-            else break;
+        // the vertex deposits with the flux that arrived at it
+        const uint32_t slot = wave_append(V.count, r, 1u, deposit);
+        if (deposit && slot < V.cap) {
+            float4* w = V.v + ((size_t)r * V.cap + slot) * 4;
+            w[0] = f4(o, h.t);
+            w[1] = make_float4(d.x, d.y, d.z, __int_as_float(h.prim));
+            w[2] = make_float4(h.b1, h.b2, __uint_as_float((uint32_t)j), __uint_as_float((uint32_t)i));
+            w[3] = f4(flux, 0.f);
         }
-        flux = flux * est;
-        gyd::spawn_ray(it, wi, &o);
+        o = o_next;
         d = wi;
+        flux = flux_next;
+    }
+}
+
+// The deposits of one iteration, one thread per photon vertex: each hit point
+// listed in the vertex's bucket, within its radius and facing the same way,
+// gets a record (SPPM.cpp:127-150 before the update).
+__global__ void __launch_bounds__(256) k_sppm_deposit(gyd::SceneDev S, int max_depth, SppmDev D, GridDev G, const float4* vtx,
+                                                      uint32_t n, RecDev Rec) {
+    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    const bool valid = t < n;
+    gyd::Isect it = {};
+    gyd::BSDF b = {};
+    V3 d = v3s(0.f), flux = v3s(0.f);
+    uint32_t e0 = 0, e1 = 0, key_lo = 0;
+    if (valid) {
+        const float4* w = vtx + (size_t)t * 4;
+        const float4 a = w[0], c = w[1], bb = w[2], fl = w[3];
+        gyd::Hit h;
+        h.t = a.w;
+        h.prim = __float_as_int(c.w);
+        h.b1 = bb.x;
+        h.b2 = bb.y;
+        d = v3(c.x, c.y, c.z);
+        it = gyd::query(S, f3(a), d, h);
+        b = gyd::make_bsdf(S, it);
+        b.importance = true;
+        flux = f3(fl);
+        key_lo = __float_as_uint(bb.z) * (uint32_t)max_depth + __float_as_uint(bb.w);
+        const V3 hh = (it.pos - v3(G.bmin.x, G.bmin.y, G.bmin.z)) * G.inv;
+        const uint32_t bk = grid_hash(abs(int(hh.x)), abs(int(hh.y)), abs(int(hh.z)), G.num);
+        e0 = G.start[bk];
+        e1 = G.start[bk + 1];
+    }
+    // Each entry carries its hit point's position, radius^2 and |normal|, so
+    // a candidate costs one contiguous 32-byte read and only accepted ones
+    // gather more.  Count, one atomic per wave, then write in bucket order.
+    uint32_t cnt = 0;
+    for (uint32_t e = e0; e < e1; ++e) {
+        const float4 g0 = G.entries[2 * (size_t)e], g1 = G.entries[2 * (size_t)e + 1];
+        const V3 vv = v3(g0.x, g0.y, g0.z) - it.pos;
+        const float dd = gyd::dot(vv, vv);
+        if ((double)gyd::dot(v3(g1.x, g1.y, g1.z), it.absN) > 0.0015 && dd <= g0.w) ++cnt;
+    }
+    uint32_t slot = wave_append(Rec.count, 0u, cnt, cnt > 0);
+    for (uint32_t e = e0; e < e1 && cnt > 0; ++e) {
+        const float4 g0 = G.entries[2 * (size_t)e], g1 = G.entries[2 * (size_t)e + 1];
+        const V3 vv = v3(g0.x, g0.y, g0.z) - it.pos;
+        const float dd = gyd::dot(vv, vv);
+        if ((double)gyd::dot(v3(g1.x, g1.y, g1.z), it.absN) > 0.0015 && dd <= g0.w) {
+            if (slot < Rec.cap) {
+                const uint32_t hp = __float_as_uint(g1.w);
+                const V3 c = f3(D.imp[hp]) * gyd::bsdf_eval(b, f3(D.out[hp]), mul_m1(d)) * flux;
+                Rec.keys[slot] = ((unsigned long long)hp << 32) | (unsigned long long)key_lo;
+                Rec.val[slot] = f4(c, dd);
+            }
+            ++slot;
+        }
     }
 }
 
@@ -405,7 +555,8 @@ extern "C" int gy_render_sppm(gy_scene* s, const gy_camera_desc* cam, const gy_s
     hipLaunchKernelGGL(k_sppm_init, dim3(G1(WH, 256)), dim3(256), 0, c->stream, D, r0 * r0);
     // grid, record and scratch buffers, grown as needed
     Bufs GB, EB, RB, TB;
-    uint32_t *bcount = nullptr, *bstart = nullptr, *bcursor = nullptr, *entries = nullptr;
+    uint32_t *bcount = nullptr, *bstart = nullptr, *bcursor = nullptr;
+    float4* entries = nullptr;
     size_t bucket_cap = 0, entry_cap = 0;
     unsigned long long *keys = nullptr, *keys2 = nullptr;
     float4* val = nullptr;
@@ -436,7 +587,65 @@ extern "C" int gy_render_sppm(gy_scene* s, const gy_camera_desc* cam, const gy_s
     int end_bit = 32;
     while ((1ll << (end_bit - 32)) < WH) ++end_bit;
     const bool stack64 = sc->stack > 32;
+    // Photon vertices: one list per iteration of a chunk of iterations traced
+    // together (about 4M photon paths per launch), grown and traced again
+    // when a list overflows; the chunk shrinks to keep the lists within 16 GiB.
+    const int64_t photons = sp->photons_per_iteration;
+    constexpr uint64_t kVtxBudget = 16ull << 30;
+    int chunk = (int)std::min<int64_t>(std::max<int64_t>(((1ll << 22) + photons - 1) / photons, 1), sp->iterations);
+    {
+        const int n_chunks = (sp->iterations + chunk - 1) / chunk;  // chunks of even size
+        chunk = (sp->iterations + n_chunks - 1) / n_chunks;
+    }
+    Bufs VB;
+    float4* vtx = nullptr;
+    unsigned int* vcount = nullptr;
+    uint32_t vcap = 0;
+    std::vector<unsigned int> vn;
+    auto grow_vtx = [&](uint64_t per_iter) -> int {
+        VB.reset();
+        vcap = 0;
+        if (per_iter >= 0xFFFFFFFFull) return Fail(GY_ERR_UNSUPPORTED, "gy_render_sppm: 2^32 depositing photon vertices in one iteration");
+        chunk = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)chunk, kVtxBudget / (per_iter * 64)));
+        int r = VB.get(c, &vtx, (size_t)chunk * per_iter * 4);
+        if (r == GY_OK) r = VB.get(c, &vcount, (size_t)chunk);
+        if (r == GY_OK) vcap = (uint32_t)per_iter;
+        return r;
+    };
+    int iter0 = 0, n_iter = 0;
     for (int iter = 0; iter < sp->iterations && rc == GY_OK; ++iter) {
+        if (iter == iter0 + n_iter) {
+            // 0. the photon paths of the next chunk of iterations
+            iter0 = iter;
+            for (;;) {
+                if (vcap == 0) {
+                    rc = grow_vtx((uint64_t)std::min<int64_t>(photons * (sp->max_depth - 1), std::max<int64_t>(photons * 4, 1 << 16)) + 1);
+                    if (rc != GY_OK) break;
+                }
+                n_iter = std::min(chunk, sp->iterations - iter0);
+                hipLaunchKernelGGL(k_u32_fill, dim3(1), dim3(256), 0, c->stream, vcount, 0u, (int64_t)n_iter);
+                if (stack64)
+                    hipLaunchKernelGGL(k_sppm_trace<64>, dim3(G1(photons * n_iter, 128)), dim3(128), 0, c->stream, sc->dev, P,
+                                       VtxDev{vtx, vcount, vcap}, iter0, n_iter, photons);
+                else
+                    hipLaunchKernelGGL(k_sppm_trace<32>, dim3(G1(photons * n_iter, 128)), dim3(128), 0, c->stream, sc->dev, P,
+                                       VtxDev{vtx, vcount, vcap}, iter0, n_iter, photons);
+                rc = CheckLaunch(c, "k_sppm_trace");
+                if (rc != GY_OK) break;
+                vn.assign((size_t)n_iter, 0u);
+                hipError_t e = hipMemcpyAsync(vn.data(), vcount, (size_t)n_iter * 4, hipMemcpyDeviceToHost, c->stream);
+                if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+                if (e != hipSuccess) {
+                    rc = HipFail(e, "sppm vertices");
+                    break;
+                }
+                const uint32_t most = *std::max_element(vn.begin(), vn.end());
+                if (most <= vcap) break;
+                rc = grow_vtx((uint64_t)most + most / 4);
+                if (rc != GY_OK) break;
+            }
+            if (rc != GY_OK) break;
+        }
         // 1. eye pass
         if (stack64) hipLaunchKernelGGL(k_sppm_eye<64>, dim3(G1(WH, 128)), dim3(128), 0, c->stream, sc->dev, P, D, iter);
         else hipLaunchKernelGGL(k_sppm_eye<32>, dim3(G1(WH, 128)), dim3(128), 0, c->stream, sc->dev, P, D, iter);
@@ -444,7 +653,7 @@ extern "C" int gy_render_sppm(gy_scene* s, const gy_camera_desc* cam, const gy_s
         if (rc != GY_OK) break;
         // 2. hash grid
         hipLaunchKernelGGL(k_u32_fill, dim3(1), dim3(256), 0, c->stream, stats, 0u, (int64_t)8);
-        hipLaunchKernelGGL(k_sppm_stats, dim3(G1(WH, 256)), dim3(256), 0, c->stream, D, stats);
+        hipLaunchKernelGGL(k_sppm_stats, dim3(kReduceBlocks), dim3(256), 0, c->stream, D, stats);
         uint32_t hs[8];
         hipError_t e = hipMemcpyAsync(hs, stats, 8, hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -460,7 +669,7 @@ extern "C" int gy_render_sppm(gy_scene* s, const gy_camera_desc* cam, const gy_s
             std::memcpy(&maxr2, &hs[1], 4);
             const float search = std::sqrt(maxr2) + 1e-6f;  // Eps (common/Core.h:191)
             hipLaunchKernelGGL(k_u32_fill, dim3(1), dim3(256), 0, c->stream, stats, 0xFFFFFFFFu, (int64_t)3);
-            hipLaunchKernelGGL(k_sppm_bbox, dim3(G1(WH, 256)), dim3(256), 0, c->stream, D, search, stats);
+            hipLaunchKernelGGL(k_sppm_bbox, dim3(kReduceBlocks), dim3(256), 0, c->stream, D, search, stats);
             e = hipMemcpyAsync(hs, stats, 12, hipMemcpyDeviceToHost, c->stream);
             if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
             if (e != hipSuccess) {
@@ -500,7 +709,7 @@ extern "C" int gy_render_sppm(gy_scene* s, const gy_camera_desc* cam, const gy_s
             if (total > entry_cap) {
                 EB.reset();
                 entry_cap = (size_t)total + total / 4 + 16;
-                rc = EB.get(c, &entries, entry_cap);
+                rc = EB.get(c, &entries, entry_cap * 2);
                 if (rc != GY_OK) break;
             }
             e = hipMemcpyAsync(bcursor, bstart, (size_t)(num + 1) * 4, hipMemcpyDeviceToDevice, c->stream);
@@ -515,23 +724,22 @@ extern "C" int gy_render_sppm(gy_scene* s, const gy_camera_desc* cam, const gy_s
             rc = CheckLaunch(c, "k_sppm_grid");
             if (rc != GY_OK) break;
         }
-        // 3. photons: records, grown and re-traced when they overflow
+        // 3. deposits of the iteration's photon vertices: records, grown and
+        // deposited again when they overflow
+        const uint32_t nv = std::min<uint32_t>(vn[iter - iter0], vcap);
+        const float4* vlist = vtx + (size_t)(iter - iter0) * vcap * 4;
         unsigned int nrec = 0;
         for (;;) {
             if (rec_cap == 0) {
-                rc = grow_records((size_t)std::min<int64_t>(std::max<int64_t>(sp->photons_per_iteration * 8, 1 << 20), 0x7FFFFFFF));
+                // a first guess of 16 records per photon (40 bytes each); grown on overflow
+                rc = grow_records((size_t)std::min<int64_t>(std::max<int64_t>(sp->photons_per_iteration * 16, 1 << 20), 0x7FFFFFFF));
                 if (rc != GY_OK) break;
             }
-            if (num == 0) break;
+            if (num == 0 || nv == 0) break;
             hipLaunchKernelGGL(k_u32_fill, dim3(1), dim3(256), 0, c->stream, rcount, 0u, (int64_t)1);
             RecDev R = {keys, val, rcount, rec_cap};
-            if (stack64)
-                hipLaunchKernelGGL(k_sppm_photon<64>, dim3(G1(sp->photons_per_iteration, 128)), dim3(128), 0, c->stream, sc->dev, P, D, Gd,
-                                   R, iter, (int64_t)sp->photons_per_iteration);
-            else
-                hipLaunchKernelGGL(k_sppm_photon<32>, dim3(G1(sp->photons_per_iteration, 128)), dim3(128), 0, c->stream, sc->dev, P, D, Gd,
-                                   R, iter, (int64_t)sp->photons_per_iteration);
-            rc = CheckLaunch(c, "k_sppm_photon");
+            hipLaunchKernelGGL(k_sppm_deposit, dim3(G1(nv, 256)), dim3(256), 0, c->stream, sc->dev, sp->max_depth, D, Gd, vlist, nv, R);
+            rc = CheckLaunch(c, "k_sppm_deposit");
             if (rc != GY_OK) break;
             e = hipMemcpyAsync(&nrec, rcount, 4, hipMemcpyDeviceToHost, c->stream);
             if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -544,7 +752,7 @@ extern "C" int gy_render_sppm(gy_scene* s, const gy_camera_desc* cam, const gy_s
                 rc = Fail(GY_ERR_UNSUPPORTED, "gy_render_sppm: more than 2^31 photon records in one iteration");
                 break;
             }
-            // too many records: room for all of them, and the iteration's photons traced again
+            // too many records: room for all of them, and the iteration's vertices deposited again
             rc = grow_records((size_t)nrec + nrec / 4);
             if (rc != GY_OK) break;
         }
