@@ -442,8 +442,11 @@ __device__ V3 vpt_path(const SceneDev& S, const MediaDev& M, const PathParams& P
     return L;
 }
 
+// Three waves per SIMD: the equi-angular kernel otherwise holds 256 VGPRs (one
+// wave) and runs 2.3x slower; with the hint it keeps 168 and spills some.
 template <int STACK, bool ENV, bool EQA>
-__global__ void __launch_bounds__(128) k_vpt(SceneDev S, MediaDev M, PathParams P, FilmDev F, int s0, int n_s) {
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3)))
+k_vpt(SceneDev S, MediaDev M, PathParams P, FilmDev F, int s0, int n_s) {
     __shared__ uint32_t stack[STACK * 128];
     const int64_t t = (int64_t)blockIdx.x * 128 + threadIdx.x;
     int s_local, px, py;
