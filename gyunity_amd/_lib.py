@@ -208,6 +208,7 @@ def lib():
         "gy_batch_render": (C.c_int, [vp]),
         "gy_batch_film_download": (C.c_int, [vp, i32, P(f32), P(f32)]),
         "gy_batch_destroy": (C.c_int, [vp]),
+        "gy_batch_autotune": (C.c_int, [vp, P(BatchParams), P(C.c_double)]),
         "gy_render_stats_get": (C.c_int, [vp, P(RenderStats)]),
         "gy_kernel_name": (C.c_char_p, [C.c_int]),
         "gy_li_samples": (C.c_int, [vp, P(CameraDesc), P(RenderParams), i32, P(i32), P(i32), P(i32), P(f32), P(f32), P(i32)]),
@@ -505,6 +506,14 @@ class Batch:
 
     def render(self):
         check(lib().gy_batch_render(self.h))
+
+    def autotune(self):
+        """Fastest refill threshold and ray-sort key for the whole batch
+        (gy_batch_autotune), kept in the batch: (BatchParams, ms per render)."""
+        best = BatchParams()
+        ms = C.c_double()
+        check(lib().gy_batch_autotune(self.h, C.byref(best), C.byref(ms)))
+        return best, ms.value
 
     def film(self, entry):
         W, H = self.cams[entry]

@@ -259,6 +259,57 @@ extern "C" int gy_batch_film_download(gy_batch* bh, int32_t entry, float* color,
     return GY_OK;
 }
 
+// Launch choices of the batch as a whole: the batch's entries share one lane
+// space and every launch, so a choice holds for all of them.  Times the
+// batch's own renders (the second of two per choice: the first re-records the
+// graph) over the refill threshold (automatic, 16, 32) and then, at the best
+// threshold, the closest-hit ray-sort keys; keeps the fastest.
+extern "C" int gy_batch_autotune(gy_batch* bh, gy_batch_params* best, double* best_ms) {
+    if (!bh || !best) return Fail(GY_ERR_INVALID, "gy_batch_autotune: null argument");
+    BatchHandle* h = reinterpret_cast<BatchHandle*>(bh);
+    Batch& b = h->b;
+    if (b.entries.empty()) return Fail(GY_ERR_INVALID, "gy_batch_autotune: empty batch");
+    struct Cand {
+        int refill, sort;
+    };
+    Cand won = {b.refill, b.ray_sort};
+    double best_t = 1e300;
+    auto trial = [&](const Cand& k) -> int {
+        b.refill = k.refill;
+        b.ray_sort = k.sort;
+        b.dirty = true;
+        double t = 1e300;
+        for (int rep = 0; rep < 2; ++rep) {
+            const int r = gy_batch_render(bh);
+            if (r != GY_OK) return r;
+            t = b.ctx->stats.total_ms;
+        }
+        if (t < best_t) {
+            best_t = t;
+            won = k;
+        }
+        return GY_OK;
+    };
+    int rc = GY_OK;
+    const Cand refills[] = {{0, GY_RAY_SORT_NONE}, {16, GY_RAY_SORT_NONE}, {32, GY_RAY_SORT_NONE}};
+    for (const Cand& k : refills)
+        if (rc == GY_OK) rc = trial(k);
+    const int refill = won.refill;
+    if (rc == GY_OK) rc = trial({refill, GY_RAY_SORT_OCTANT});
+    if (rc == GY_OK) rc = trial({refill, GY_RAY_SORT_OCTANT_LOCAL});
+    b.refill = won.refill;
+    b.ray_sort = won.sort;
+    b.dirty = true;
+    if (rc != GY_OK) return rc;
+    std::memset(best, 0, sizeof(*best));
+    best->lanes_per_pass = h->lanes_req;
+    best->refill_threshold = won.refill;
+    best->use_graph = b.use_graph ? 1 : 0;
+    best->ray_sort = won.sort;
+    if (best_ms) *best_ms = best_t;
+    return GY_OK;
+}
+
 extern "C" int gy_batch_destroy(gy_batch* bh) {
     if (!bh) return GY_OK;
     BatchHandle* h = reinterpret_cast<BatchHandle*>(bh);

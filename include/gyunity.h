@@ -321,6 +321,14 @@ int gy_batch_add(gy_batch* b, gy_scene* scene, const gy_camera_desc* cam, const 
 int gy_batch_render(gy_batch* b);
 int gy_batch_film_download(gy_batch* b, int32_t entry, float* color, float* weight);
 int gy_batch_destroy(gy_batch* b);
+/* Launch choices for the batch as a whole (its entries share one lane space and
+ * every launch): times the batch's own renders over the refill threshold
+ * (automatic, 16, 32), then the ray-sort keys at the best threshold; keeps the
+ * fastest in the batch (the graph is re-recorded at the next render) and writes
+ * it into best, with best_ms the time of one render.  The films then hold a
+ * render of the batch; they do not depend on the choice.  No counterpart in the
+ * reference. */
+int gy_batch_autotune(gy_batch* b, gy_batch_params* best, double* best_ms);
 /* Differentiable batch: every entry of the last gy_batch_render renders the
  * same textured scene (views of one scene).  Images are concatenated in entry
  * order, width*height*3 floats each (gy_batch_image_floats in total; -1 before

@@ -566,6 +566,33 @@ def test_batch64_members_equal_single_renders_and_oracle(gpu_ctx):
         _li_parity(gpu_ctx, sa, cam, prm, 5000, 70 + seed)
 
 
+def test_batch_autotune_keeps_the_films(gpu_ctx):
+    # The batch-wide launch choice changes no film: every entry's film after
+    # the tuned render equals its film before tuning, bit for bit.
+    entries = _batch_entries(gpu_ctx)
+    scenes = {}
+    try:
+        with g.Batch(gpu_ctx, lanes_per_pass=40000) as b:
+            ids = []
+            for sa, cam, prm in entries:
+                if id(sa) not in scenes:
+                    scenes[id(sa)] = g.Scene(gpu_ctx, sa)
+                ids.append(b.add(scenes[id(sa)], cam, prm))
+            b.render()
+            before = [b.film(e) for e in ids]
+            best, ms = b.autotune()
+            assert ms > 0 and best.lanes_per_pass == 40000
+            assert best.refill_threshold in (0, 16, 32)
+            assert best.ray_sort in (g.GY_RAY_SORT_NONE, g.GY_RAY_SORT_OCTANT, g.GY_RAY_SORT_OCTANT_LOCAL)
+            b.render()
+            for e, (c0, w0) in zip(ids, before):
+                c1, w1 = b.film(e)
+                assert np.array_equal(c0, c1) and np.array_equal(w0, w1), e
+    finally:
+        for sc in scenes.values():
+            sc.close()
+
+
 def test_batch_errors(gpu_ctx):
     sa, cam, prm = g.generate_scene(g.GY_SCENE_CORNELL)
     with g.Scene(gpu_ctx, sa) as sc, g.Batch(gpu_ctx) as b:
