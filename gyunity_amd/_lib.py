@@ -110,13 +110,16 @@ class SppmParams(C.Structure):
         ("batch_shrink", C.c_int32),
         ("sampler", C.c_int32),
         ("trace_glossy_ray", C.c_int32),
+        ("photons_per_launch", C.c_int64),
     ]
 
 
-def sppm_params(iterations, photons_per_iteration, max_depth, initial_radius, alpha, batch_shrink=False, sampler=0):
+def sppm_params(iterations, photons_per_iteration, max_depth, initial_radius, alpha, batch_shrink=False, sampler=0,
+                photons_per_launch=0):
     p = SppmParams()
     p.iterations, p.photons_per_iteration, p.max_depth = iterations, photons_per_iteration, max_depth
     p.initial_radius, p.alpha, p.batch_shrink, p.sampler = initial_radius, alpha, 1 if batch_shrink else 0, sampler
+    p.photons_per_launch = photons_per_launch
     return p
 
 

@@ -510,6 +510,7 @@ extern "C" int gy_render_sppm(gy_scene* s, const gy_camera_desc* cam, const gy_s
         !(sp->initial_radius > 0.f) || !(sp->alpha > 0.f && sp->alpha <= 1.f))
         return Fail(GY_ERR_INVALID, "gy_render_sppm: iterations, photons, 0 < max_depth <= 100, radius > 0, 0 < alpha <= 1");
     if (sp->trace_glossy_ray) return Fail(GY_ERR_UNSUPPORTED, "gy_render_sppm: traceGlossyRay");
+    if (sp->photons_per_launch < 0) return Fail(GY_ERR_INVALID, "gy_render_sppm: photons_per_launch >= 0");
     if ((uint64_t)sp->photons_per_iteration * (uint64_t)sp->max_depth >= (1ull << 32))
         return Fail(GY_ERR_UNSUPPORTED, "gy_render_sppm: photons_per_iteration * max_depth must stay below 2^32");
     Scene* sc = reinterpret_cast<Scene*>(s);
@@ -591,11 +592,13 @@ extern "C" int gy_render_sppm(gy_scene* s, const gy_camera_desc* cam, const gy_s
     while ((1ll << (end_bit - 32)) < WH) ++end_bit;
     const bool stack64 = sc->stack > 32;
     // Photon vertices: one list per iteration of a chunk of iterations traced
-    // together (about 4M photon paths per launch), grown and traced again
-    // when a list overflows; the chunk shrinks to keep the lists within 16 GiB.
+    // together (photons_per_launch photon paths per launch, about 4M by
+    // default), grown and traced again when a list overflows; the chunk
+    // shrinks to keep the lists within 16 GiB.
     const int64_t photons = sp->photons_per_iteration;
     constexpr uint64_t kVtxBudget = 16ull << 30;
-    int chunk = (int)std::min<int64_t>(std::max<int64_t>(((1ll << 22) + photons - 1) / photons, 1), sp->iterations);
+    const int64_t per_launch = sp->photons_per_launch > 0 ? sp->photons_per_launch : (1ll << 22);
+    int chunk = (int)std::min<int64_t>(std::max<int64_t>((per_launch + photons - 1) / photons, 1), sp->iterations);
     {
         const int n_chunks = (sp->iterations + chunk - 1) / chunk;  // chunks of even size
         chunk = (sp->iterations + n_chunks - 1) / n_chunks;

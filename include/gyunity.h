@@ -225,6 +225,8 @@ typedef struct {
     int32_t batch_shrink;
     int32_t sampler;          /* GY_SAMPLER_SOBOL or GY_SAMPLER_HALTON (sampler and enumeration) */
     int32_t trace_glossy_ray; /* must be 0 */
+    int64_t photons_per_launch; /* photon paths traced per launch, over whole iterations
+                                   (0 = automatic, about 4M); the image does not depend on it */
 } gy_sppm_params;
 int gy_render_sppm(gy_scene* scene, const gy_camera_desc* cam, const gy_sppm_params* params, float* rgb, float* radius2,
                    float* photon_count);

@@ -133,6 +133,36 @@ def test_sppm_matches_oracle(gpu_ctx, batch):
 
 
 @pytest.mark.gpu
+def test_sppm_chunks_and_regrowth_match_oracle(gpu_ctx):
+    # Three iterations' photon paths in two launches (100,000 photon paths per
+    # launch, 50,000 photons per iteration), vertex lists that outgrow their
+    # first size of 4 per photon (albedo-0.9 photons bounce up to the depth
+    # limit) and record buffers that outgrow their first 2^20 (a radius that
+    # holds most hit points): the same image as one launch, and the oracle's
+    # counts, radii and colours.
+    sa = furnace_scene(albedo=0.9, le=1.0)
+    cam = _furnace_cam(32)
+    with g.Scene(gpu_ctx, sa) as sc:
+        img, r2, N = sc.render_sppm(cam, g.sppm_params(3, 50000, 12, 0.3, 0.7, photons_per_launch=100000))
+        img1, r21, N1 = sc.render_sppm(cam, g.sppm_params(3, 50000, 12, 0.3, 0.7))
+        _, _, first = sc.render_sppm(cam, g.sppm_params(1, 50000, 12, 0.3, 0.7, batch_shrink=True))
+    # with the batch update a hit point accepts every record of the iteration
+    # (its radius holds until the iteration ends), and N = alpha * records
+    assert first.sum() / 0.7 > 2 ** 20
+    assert np.array_equal(img, img1) and np.array_equal(r2, r21) and np.array_equal(N, N1)
+    os_ = O.OracleScene(sa)
+    oimg, or2, oN = os_.render_sppm(cam, O.sppm_params(3, 50000, 12, 0.3, 0.7, 0, 0, 0, 8))
+    same_n = np.mean(N == oN.astype(np.float32))
+    same_r = np.mean(r2 == or2.astype(np.float32))
+    err = (np.abs(img - oimg) / np.maximum(np.abs(oimg), 1e-2)).max(axis=2)
+    print("photon counts equal %.4f, radii equal %.4f, pixels within 1e-4 %.4f, mean count %.1f"
+          % (same_n, same_r, (err < 1e-4).mean(), N.mean()))
+    assert N.mean() > 100  # records from most photons: the record buffers grew
+    assert same_n > 0.995 and same_r > 0.995
+    assert (err < 1e-4).mean() > 0.995
+
+
+@pytest.mark.gpu
 def test_sppm_refusals(gpu_ctx):
     sa, cam = _sppm_scene()
     with g.Scene(gpu_ctx, sa) as sc:
@@ -141,7 +171,8 @@ def test_sppm_refusals(gpu_ctx):
         with pytest.raises(g.GyError) as e:
             sc.render_sppm(cam, sp)
         assert e.value.code == GY_ERR_UNSUPPORTED
-        for bad in (g.sppm_params(0, 1000, 4, 0.05, 0.7), g.sppm_params(1, 1000, 4, 0.0, 0.7), g.sppm_params(1, 1000, 4, 0.05, 1.5)):
+        for bad in (g.sppm_params(0, 1000, 4, 0.05, 0.7), g.sppm_params(1, 1000, 4, 0.0, 0.7), g.sppm_params(1, 1000, 4, 0.05, 1.5),
+                    g.sppm_params(1, 1000, 4, 0.05, 0.7, photons_per_launch=-1)):
             with pytest.raises(g.GyError):
                 sc.render_sppm(cam, bad)
         sc.set_environment(env_image())
