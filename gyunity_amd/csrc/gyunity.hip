@@ -261,8 +261,9 @@ __global__ void __launch_bounds__(128) k_li_samples(SceneDev S, PathParams P, in
 }
 
 // LightTracing: one light path per thread, sample index = thread index.
+// Five waves per SIMD (136 VGPRs and three without the hint): 9% faster.
 template <int STACK, int BLOCK, bool ENV>
-__global__ void __launch_bounds__(BLOCK) k_lt(SceneDev S, PathParams P, LightCamDev C, int64_t n, float total,
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) k_lt(SceneDev S, PathParams P, LightCamDev C, int64_t n, float total,
                                               float* __restrict__ splat) {
     __shared__ uint32_t stack[STACK * BLOCK];
     const int64_t t = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
