@@ -133,6 +133,21 @@ def test_sppm_matches_oracle(gpu_ctx, batch):
 
 
 @pytest.mark.gpu
+def test_sppm_few_photons_match_oracle(gpu_ctx):
+    # 37 photons per iteration: partial waves in the photon launch and in the
+    # deposit pass, five iterations in one launch, an image whose hit points
+    # mostly see no photon.
+    sa, cam = _sppm_scene()
+    with g.Scene(gpu_ctx, sa) as sc:
+        img, r2, N = sc.render_sppm(cam, g.sppm_params(5, 37, 8, 0.2, 0.7))
+    oimg, or2, oN = O.OracleScene(sa).render_sppm(cam, O.sppm_params(5, 37, 8, 0.2, 0.7, 0, 0, 0, 8))
+    assert np.array_equal(N, oN.astype(np.float32)) and np.array_equal(r2, or2.astype(np.float32))
+    assert N.max() > 0
+    err = (np.abs(img - oimg) / np.maximum(np.abs(oimg), 1e-2)).max(axis=2)
+    assert (err < 1e-4).mean() > 0.995
+
+
+@pytest.mark.gpu
 def test_sppm_chunks_and_regrowth_match_oracle(gpu_ctx):
     # Three iterations' photon paths in two launches (100,000 photon paths per
     # launch, 50,000 photons per iteration), vertex lists that outgrow their
