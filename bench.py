@@ -236,18 +236,24 @@ def vpt_main(args):
     sc = g.Scene(ctx, sa)
     sc.set_media(media, tri)
     res = {}
-    for eqa in (True, False):
-        name = "equiangular" if eqa else "free_flight"
-        for i in range(args.warmup):
-            sc.render_vpt(cam, prm, equiangular=eqa)
-            log("vpt %s warmup %d done" % (name, i))
-        t0 = time.perf_counter()
-        for i in range(args.steps):
-            c, w = sc.render_vpt(cam, prm, equiangular=eqa)
-            log("vpt %s step %d done" % (name, i))
-        ms = (time.perf_counter() - t0) * 1e3 / args.steps
-        res[name] = {"ms_per_step": round(ms, 3), "mpaths_per_s": round(paths / (ms / 1e3) / 1e6, 3),
-                     "finite_fraction": float(np.isfinite(c).mean()), "mean": float((c / np.maximum(w, 1)[..., None]).mean())}
+    forms = [("wavefront", g.GY_RENDER_WAVEFRONT)] + ([] if args.single else [("per_path", g.GY_RENDER_MEGAKERNEL)])
+    for form, mode in forms:
+        prm.mode = mode
+        for eqa in (True, False):
+            name = ("equiangular" if eqa else "free_flight") + ("" if form == "wavefront" else "_per_path")
+            for i in range(args.warmup):
+                sc.render_vpt(cam, prm, equiangular=eqa)
+                log("vpt %s warmup %d done" % (name, i))
+            t0 = time.perf_counter()
+            for i in range(args.steps):
+                c, w = sc.render_vpt(cam, prm, equiangular=eqa)
+                log("vpt %s step %d done" % (name, i))
+            ms = (time.perf_counter() - t0) * 1e3 / args.steps
+            st = ctx.stats()
+            log("vpt %s per-kernel ms/step: %s" % (name, ", ".join(
+                "%s %.1f" % (g.lib().gy_kernel_name(k).decode(), st.kernel_ms[k]) for k in range(8) if st.kernel_ms[k] > 0)))
+            res[name] = {"ms_per_step": round(ms, 3), "mpaths_per_s": round(paths / (ms / 1e3) / 1e6, 3),
+                         "finite_fraction": float(np.isfinite(c).mean()), "mean": float((c / np.maximum(w, 1)[..., None]).mean())}
     value = res["equiangular"]["mpaths_per_s"]
     out = {
         "metric": "Mpaths/s at fixed RMSE: batched wavefront vs plain megakernel; adjoint s/grad",
@@ -268,7 +274,7 @@ def vpt_main(args):
                         % (args.level, sa.n_tri, cam.width, cam.height, prm.spp, prm.max_depth,
                            [round(float(v), 3) for v in media["sigma_a"][0]], [round(float(v), 3) for v in media["sigma_s"][0]],
                            float(media["g"][0])),
-            "integrator": "VolPathTracing per path (one path per thread)",
+            "integrator": "VolPathTracing, wavefront form (per-path form alongside unless --single; equal films)",
             "forms": res,
             "paths_per_step": paths,
             "parallelism": "single GPU",

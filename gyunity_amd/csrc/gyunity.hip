@@ -119,8 +119,10 @@ void TimerBegin(Ctx* c, int cat) {
     c->ev_cat[c->ev_used / 2] = cat;
     (void)hipEventRecord(c->events[c->ev_used], c->stream);
 }
+// Timers do not nest: each TimerBegin is closed by one TimerEnd before the next.
 void TimerEnd(Ctx* c) {
     if (c->capturing) return;
+    if (c->ev_used + 1 >= (int)c->events.size()) return;  // an unmatched end records nothing
     (void)hipEventRecord(c->events[c->ev_used + 1], c->stream);
     c->ev_used += 2;
 }
@@ -608,6 +610,7 @@ void SceneFree(Scene* sc) {
     for (void* b : bufs) DevFree(b);
     FilmFree(&sc->film);
     WavefrontFree(sc);
+    WavefrontStateFree(&sc->vwf);
     delete sc;
 }
 

@@ -96,6 +96,7 @@ struct Scene {
     std::vector<uint32_t> tri_order;
     FilmState film;
     WavefrontState wf;
+    WavefrontState vwf;  // VolPathTracing's wavefront state
     int64_t tex_floats = 0;     // floats of texture data on the device
     uint64_t tex_revision = 0;  // bumped whenever the texels are replaced
     bool film_valid = false;    // the film holds the last gy_render of film_cam / film_params

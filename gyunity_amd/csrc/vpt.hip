@@ -476,6 +476,636 @@ __global__ void __launch_bounds__(128) k_vpt_li_samples(SceneDev S, MediaDev M, 
     dims[i] = (int32_t)nd;
 }
 
+// ---------------------------------------------------------------- wavefront form
+// VolPathTracing::Li split into per-vertex stages over queues of path slots,
+// with the same FP32 operations in the same order as vpt_path, so that its
+// film equals the per-path kernel's bit for bit:
+//
+//   start -> [ trace (closest hit of the path ray)
+//              -> vertex (medium sampling, the vertex's light set-up,
+//                 the next ray: phase or BSDF sample, or a boundary crossing)
+//              -> trace_shadow (VisibilityTester::Tr: closest hits through
+//                 material-less surfaces, each segment's transmittance)
+//              -> trace_mis (Scene::IntersectTr towards the chosen light)
+//              -> resolve (adds the vertex's light in vpt_path's order,
+//                 finishes paths, builds the next active queue in slot order) ]
+//
+// until no path is left: a boundary crossing takes a round without adding a
+// bounce, so the number of rounds is found on the device.  The persistent
+// trace waves run each shadow and MIS segment loop in the lane that holds it,
+// refilling idle lanes from the queue.
+enum VwfQueue { VQ_ACTIVE = 0, VQ_NEXT = 1, VQ_SHADOW = 2, VQ_MIS = 3, VQ_COUNT = 4 };
+constexpr int kVwfCtr = 32;                  // counter stride in words (one 128-byte line each)
+constexpr int kVwfChunks = 8;                // queue chunks a trace launch claims from, one per XCD
+constexpr int kVwfWords = VQ_COUNT + 3 * kVwfChunks;  // queue lengths, then the fetch counters of each trace mode
+// Path flags (pk): the slot holds a path; its vertex set up direct light (DI)
+// or an equi-angular connection (CONN, CONN0 when the light faces away); a
+// shadow or MIS segment was traced for it; the path ends after this round.
+constexpr uint32_t VK_LIVE = 1u, VK_DI = 2u, VK_CONN = 4u, VK_CONN0 = 8u, VK_SH = 16u, VK_MIS = 32u, VK_DONE = 64u;
+
+struct VwfDev {
+    float4 *o, *d;        // path ray (xyz)
+    float4 *T, *L;        // throughput (xyz) + film x, radiance (xyz) + film y
+    uint4* st;            // sample index lo / hi, cursor, depth | delta << 8 | crossings << 9
+    int32_t* med;         // the path ray's medium (-1: none)
+    float4* hit;          // t, prim (-1: miss), b1, b2
+    uint32_t* pk;         // VK_* flags
+    float4 *pT;           // throughput at the vertex (xyz); P(light) (DI) or the light pick's probability (CONN)
+    float4 *pA;           // DI: Li (xyz), |cos| of the light sample | CONN: Le * phase (xyz), cos at the light
+    float4 *pB;           // DI: f of the light sample (xyz), its pdf | CONN: squared distance, the area pdf
+    float4 *pC;           // DI: w1, w2, |cos| and pdf of the BSDF / phase sample
+    float4 *pD;           // DI: f of the BSDF / phase sample (xyz), light (int)
+    float4 *tr1;          // shadow transmittance
+    float4 *e2;           // MIS: radiance reached (xyz), 1 when it counts
+    float4 *tr2;          // MIS transmittance
+    float4 *sh_o, *sh_d;  // shadow segment: origin + tmax, direction + medium (int)
+    float4 *lp_pos, *lp_fn, *lp_err;  // the light point the shadow segments aim at
+    float4 *mis_o, *mis_d;            // MIS segment: origin, direction + medium (int)
+    uint32_t* q[VQ_COUNT];
+    uint32_t* counts;
+    int64_t capacity;
+    uint32_t* spill;
+    int64_t spill_stride;
+};
+
+__device__ inline float4 f4v(V3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
+__device__ inline V3 v3f(float4 a) { return v3(a.x, a.y, a.z); }
+
+// Appends for a 256-lane workgroup, K items per lane, NQ queues, one atomic
+// per queue per workgroup (a queue counter shared by every workgroup is the
+// contended word).  Every lane of the workgroup must call it.
+template <int NQ, int K>
+__device__ inline void vwf_block_push(uint32_t* const (&q)[NQ], uint32_t* const (&cnt)[NQ], const uint32_t (&item)[K],
+                                      const bool (&pred)[NQ][K]) {
+    __shared__ uint32_t nw[NQ][K][4];
+    __shared__ uint32_t base[NQ];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint64_t mask[NQ][K];
+#pragma unroll
+    for (int j = 0; j < NQ; ++j)
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            mask[j][k] = __ballot(pred[j][k]);
+            if (lane == 0) nw[j][k][wave] = (uint32_t)__popcll(mask[j][k]);
+        }
+    __syncthreads();
+    if (threadIdx.x < NQ) {
+        uint32_t total = 0;
+        for (int k = 0; k < K; ++k) total += nw[threadIdx.x][k][0] + nw[threadIdx.x][k][1] + nw[threadIdx.x][k][2] + nw[threadIdx.x][k][3];
+        base[threadIdx.x] = total ? atomicAdd(cnt[threadIdx.x], total) : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < NQ; ++j) {
+        uint32_t off = base[j];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            uint32_t o = off;
+            for (int w = 0; w < wave; ++w) o += nw[j][k][w];
+            if (pred[j][k]) q[j][o + __popcll(mask[j][k] & ((1ull << lane) - 1ull))] = item[k];
+            off += nw[j][k][0] + nw[j][k][1] + nw[j][k][2] + nw[j][k][3];
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_vwf_start(VwfDev W, PathParams P, FilmDev F, int s0, int n_s, int64_t n_slots) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    int s_local = 0, px = 0, py = 0;
+    if (t < n_slots) pass_pixel(F, t, &s_local, &px, &py);
+    const bool valid = t < n_slots && s_local < n_s && px < F.width && py < F.height;
+    const uint32_t slot = (uint32_t)t;
+    if (valid) {
+        gyd::Seq q;
+        q.index = gyd::enum_index(P.en, (uint32_t)(s0 + s_local), (uint32_t)px, (uint32_t)py);
+        q.cursor = 0;
+        const float u = P.en.sx * gyd::seq_next(P.smp, q) - (float)px;
+        const float v = P.en.sy * gyd::seq_next(P.smp, q) - (float)py;
+        V3 o, d;
+        gyd::generate_ray(P.cam, px, py, u, v, &o, &d);
+        W.o[slot] = f4v(o, 0.f);
+        W.d[slot] = f4v(d, 0.f);
+        W.T[slot] = make_float4(1.f, 1.f, 1.f, px + u);
+        W.L[slot] = make_float4(0.f, 0.f, 0.f, py + v);
+        W.st[slot] = make_uint4((uint32_t)q.index, (uint32_t)(q.index >> 32), q.cursor, 0u);
+        W.med[slot] = -1;  // camera rays carry no medium
+        W.pk[slot] = P.max_depth > 0 ? VK_LIVE : (VK_LIVE | VK_DONE);
+    } else if (t < n_slots) {
+        W.pk[slot] = 0u;
+    }
+    uint32_t* const qs[1] = {W.q[VQ_ACTIVE]};
+    uint32_t* const cs[1] = {&W.counts[VQ_ACTIVE * kVwfCtr]};
+    const uint32_t it[1] = {slot};
+    const bool pr[1][1] = {{valid && P.max_depth > 0}};
+    vwf_block_push<1, 1>(qs, cs, it, pr);
+}
+
+// Integrator::DirectIllumination with handleMedia set up at a surface (b) or
+// a medium point (phase g): the light sample's segment and the BSDF / phase
+// sample's segment go to their queues; the factors di_vpt multiplies are kept.
+template <bool ENV>
+__device__ inline void vwf_di_setup(const SceneDev& S, const VwfDev& W, uint32_t slot, const Isect& it, int32_t in, int32_t out,
+                                    const gyd::BSDF* b, float g, V3 T, const float (&u)[6], uint32_t* pk, bool* push_sh,
+                                    bool* push_mis) {
+    float lsp;
+    const int light = gyd::sample_one_light(S, u[0], &lsp);
+    const int ltri = S.light_tri[light];  // < 0: the environment light
+    const V3 Le = v3(S.light_le[3 * light], S.light_le[3 * light + 1], S.light_le[3 * light + 2]);
+    float w1 = 1.f, w2 = 1.f;
+    {
+        float pdf;
+        LightPoint lp;
+        V3 wi, Li;
+        if (ENV && ltri < 0) {
+            lp = gyd::env_light_point_call(S.env, it.pos, u[1], u[2], &wi, &Li, &pdf);
+        } else {
+            lp = gyd::sample_light_point(S, ltri, it.pos, u[1], u[2], &pdf);
+            wi = gyd::norm(lp.pos - it.pos);
+            Li = gyd::dot(lp.n, gyd::neg(wi)) > 0.f ? Le : v3s(0.f);
+        }
+        if (pdf > 0.f && !gyd::is_zero(Li)) {
+            float sp, c = 1.f;
+            V3 f;
+            if (b) {
+                sp = gyd::bsdf_pdf(*b, it.outDir, wi);
+                f = gyd::bsdf_eval(*b, it.outDir, wi);
+                c = fabsf(gyd::dot(it.n, wi));
+            } else {
+                const float p = hg_p(g, it.outDir, wi);
+                f = v3s(p);
+                sp = p;
+            }
+            if (!gyd::is_zero(f)) {
+                V3 so, sd;
+                float stmax;
+                gyd::spawn_to(it, lp, &so, &sd, &stmax);
+                const int32_t smed = isect_medium(it, in, out, sd);
+                W.sh_o[slot] = f4v(so, stmax);
+                W.sh_d[slot] = f4v(sd, __int_as_float(smed));
+                W.lp_pos[slot] = f4v(lp.pos, 0.f);
+                W.lp_fn[slot] = f4v(lp.fn, 0.f);
+                W.lp_err[slot] = f4v(lp.pErr, 0.f);
+                w1 = gyd::power_heuristic(pdf, sp);
+                W.pA[slot] = f4v(Li, c);
+                W.pB[slot] = f4v(f, pdf);
+                *pk |= VK_SH;
+                *push_sh = true;
+            }
+        }
+    }
+    V3 wi;
+    float pdf, c = 1.f;
+    V3 f;
+    if (b) {
+        f = gyd::bsdf_sample(*b, it.outDir, &wi, &pdf, u[3], u[4], u[5]);
+        c = fabsf(gyd::dot(it.n, wi));
+    } else {
+        const float p = hg_sample(g, it.outDir, &wi, u[3], u[4]);
+        f = v3s(p);
+        pdf = p;
+    }
+    if (pdf != 0.f && !gyd::is_zero(f)) {
+        const float lpdf = (ENV && ltri < 0) ? gyd::env_pdf_dir_call(S.env, wi) : gyd::light_pdf(S, ltri, it, wi);
+        if (lpdf == 0.f) {
+            w2 = 0.f;
+        } else {
+            w2 = gyd::power_heuristic(pdf, lpdf);
+            V3 ro;
+            gyd::spawn_ray(it, wi, &ro);
+            W.mis_o[slot] = f4v(ro, 0.f);
+            W.mis_d[slot] = f4v(wi, __int_as_float(isect_medium(it, in, out, wi)));
+            W.pD[slot] = f4v(f, __int_as_float(light));
+            *pk |= VK_MIS;
+            *push_mis = true;
+        }
+    }
+    W.pC[slot] = make_float4(w1, w2, c, pdf);
+    W.pT[slot] = f4v(T, lsp);
+    *pk |= VK_DI;
+}
+
+// One vertex of vpt_path for each path of the active queue.
+template <bool ENV, bool EQA>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_vwf_vertex(SceneDev S, MediaDev M, PathParams P, VwfDev W) {
+    const uint32_t n = W.counts[VQ_ACTIVE * kVwfCtr];
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (blockIdx.x * 256 >= n) return;  // workgroup-uniform
+    bool push_sh = false, push_mis = false;
+    uint32_t slot = 0;
+    if (i < n) {
+        slot = W.q[VQ_ACTIVE][i];
+        const gyd::SampleSrc& Q = P.smp;
+        const uint4 sv = W.st[slot];
+        gyd::Seq q;
+        q.index = (uint64_t)sv.x | ((uint64_t)sv.y << 32);
+        q.cursor = sv.z;
+        int depth = (int)(sv.w & 0xFFu);
+        bool delta = (sv.w & 0x100u) != 0u;
+        int crossings = (int)(sv.w >> 9);
+        uint32_t pk = VK_LIVE;
+        bool ends = false, moved = false;  // moved: a new path ray (o, d, medium) is written
+        const float4 Tv = W.T[slot];
+        V3 T = v3f(Tv);
+        V3 o = v3f(W.o[slot]), d = v3f(W.d[slot]);
+        int32_t med = W.med[slot];
+        if (q.cursor > (uint32_t)kVptDimBudget) {
+            ends = true;
+        } else {
+            const float4 hv = W.hit[slot];
+            Hit h;
+            h.t = hv.x;
+            h.prim = __float_as_int(hv.y);
+            h.b1 = hv.z;
+            h.b2 = hv.w;
+            const bool hit = h.prim >= 0;
+            const float tmax = hit ? h.t : gyd::kInf;  // Primitive::Intersect shrinks the ray's tMax
+            MedRec mi;
+            mi.valid = false;
+            float eqProb = 0.f, eqPdf = 0.f;
+            V3 eqLe = v3s(0.f);
+            LightPoint eqLp;
+            if (EQA) {
+                const int light = gyd::sample_one_light(S, gyd::seq_next(Q, q), &eqProb);
+                const float a = gyd::seq_next(Q, q), b = gyd::seq_next(Q, q);
+                eqLp = tri_sample_area(S, S.light_tri[light], a, b, &eqPdf);
+                eqLe = v3(S.light_le[3 * light], S.light_le[3 * light + 1], S.light_le[3 * light + 2]);
+                if (med >= 0) T = T * medium_equiangular(load_medium(M, med), med, o, d, tmax, gyd::seq_next(Q, q), eqLp.pos, &mi);
+            } else if (med >= 0) {
+                const float a = gyd::seq_next(Q, q), b = gyd::seq_next(Q, q);
+                T = T * medium_sample(load_medium(M, med), med, o, d, tmax, a, b, &mi);
+            }
+            if (gyd::is_zero(T)) {
+                ends = true;
+            } else if (mi.valid) {
+                const float g = M.med[2 * mi.medium].w;
+                const Isect mit = medium_isect(mi);
+                if (EQA) {
+                    // VolPathTracing::ConnectToLight, up to the transmittance
+                    float cosTheta = 1.f;
+                    const V3 dir = mi.pos - eqLp.pos;
+                    const V3 wi = gyd::neg(gyd::norm(dir));
+                    bool faces = true;
+                    if (!gyd::is_zero(eqLp.n)) {
+                        cosTheta = gyd::dot(gyd::norm(dir), eqLp.n);
+                        faces = !(cosTheta < 0.f);
+                    }
+                    if (faces) {
+                        V3 so, sd;
+                        float stmax;
+                        gyd::spawn_to(mit, eqLp, &so, &sd, &stmax);
+                        const int32_t smed = isect_medium(mit, mi.medium, mi.medium, sd);
+                        W.sh_o[slot] = f4v(so, stmax);
+                        W.sh_d[slot] = f4v(sd, __int_as_float(smed));
+                        W.lp_pos[slot] = f4v(eqLp.pos, 0.f);
+                        W.lp_fn[slot] = f4v(eqLp.fn, 0.f);
+                        W.lp_err[slot] = f4v(eqLp.pErr, 0.f);
+                        const float p = hg_p(g, mi.wo, wi);
+                        W.pA[slot] = f4v(eqLe * v3s(p), cosTheta);
+                        W.pB[slot] = make_float4(gyd::dot(dir, dir), eqPdf, 0.f, 0.f);
+                        pk |= VK_CONN | VK_SH;
+                        push_sh = true;
+                    } else {
+                        pk |= VK_CONN0;
+                    }
+                    W.pT[slot] = f4v(T, eqProb);
+                } else {
+                    float u[6];
+                    for (int k = 0; k < 6; ++k) u[k] = gyd::seq_next(Q, q);
+                    vwf_di_setup<ENV>(S, W, slot, mit, mi.medium, mi.medium, nullptr, g, T, u, &pk, &push_sh, &push_mis);
+                }
+                const float a = gyd::seq_next(Q, q), b = gyd::seq_next(Q, q);
+                V3 wi;
+                hg_sample(g, mi.wo, &wi, a, b);
+                gyd::spawn_ray(mit, wi, &o);
+                d = wi;
+                med = mi.medium;
+                delta = false;
+                ++depth;
+                moved = true;
+            } else if (!hit) {
+                // VPT.cpp:60-64: a camera ray, or one after a delta bounce,
+                // that leaves the scene returns T * Le(ray) in place of L
+                if (ENV && (depth == 0 || delta) && S.env.w > 0) {
+                    const V3 Le = T * gyd::env_emission_call(S.env, d);
+                    W.L[slot] = f4v(Le, W.L[slot].w);
+                }
+                ends = true;
+            } else {
+                const Isect it = gyd::query(S, o, d, h);
+                int32_t in, out;
+                hit_media(M, h.prim, med, &in, &out);
+                if (no_material(M, h.prim)) {
+                    if (++crossings > kVptMaxCrossings) {
+                        ends = true;
+                    } else {
+                        gyd::spawn_ray(it, d, &o);
+                        med = isect_medium(it, in, out, d);
+                        moved = true;  // the same vertex number again
+                    }
+                } else {
+                    const gyd::BSDF b = gyd::make_bsdf(S, it);
+                    if ((depth == 0 || delta) && it.light >= 0) {
+                        const V3 Le = v3(S.light_le[3 * it.light], S.light_le[3 * it.light + 1], S.light_le[3 * it.light + 2]);
+                        const V3 e = gyd::dot(it.n, it.outDir) > 0.f ? Le : v3s(0.f);
+                        const float4 Lv = W.L[slot];
+                        const V3 L = v3f(Lv) + T * e;
+                        W.L[slot] = f4v(L, Lv.w);
+                    } else {
+                        float u[6];
+                        for (int k = 0; k < 6; ++k) u[k] = gyd::seq_next(Q, q);
+                        vwf_di_setup<ENV>(S, W, slot, it, in, out, &b, 0.f, T, u, &pk, &push_sh, &push_mis);
+                    }
+                    const float r0 = gyd::seq_next(Q, q), r1 = gyd::seq_next(Q, q), r2 = gyd::seq_next(Q, q);
+                    V3 wi;
+                    float pdf;
+                    const V3 f = gyd::bsdf_sample(b, gyd::neg(d), &wi, &pdf, r0, r1, r2);
+                    if (gyd::is_zero(f) || pdf == 0.f) {
+                        ends = true;
+                    } else {
+                        const V3 est = f * fabsf(gyd::dot(it.n, wi)) / pdf;
+                        delta = gyd::bsdf_delta(b);
+                        T = T * est;
+                        gyd::spawn_ray(it, wi, &o);
+                        d = wi;
+                        med = isect_medium(it, in, out, wi);
+                        ++depth;
+                        moved = true;
+                    }
+                }
+            }
+        }
+        if (moved && depth >= P.max_depth) ends = true;
+        if (!ends) {
+            W.o[slot] = f4v(o, 0.f);
+            W.d[slot] = f4v(d, 0.f);
+            W.med[slot] = med;
+            W.T[slot] = f4v(T, Tv.w);
+            W.st[slot] = make_uint4(sv.x, sv.y, q.cursor, (uint32_t)depth | (delta ? 0x100u : 0u) | ((uint32_t)crossings << 9));
+        }
+        W.pk[slot] = pk | (ends ? VK_DONE : 0u);
+    }
+    uint32_t* const qs[2] = {W.q[VQ_SHADOW], W.q[VQ_MIS]};
+    uint32_t* const cs[2] = {&W.counts[VQ_SHADOW * kVwfCtr], &W.counts[VQ_MIS * kVwfCtr]};
+    const uint32_t it[1] = {slot};
+    const bool pr[2][1] = {{push_sh}, {push_mis}};
+    vwf_block_push<2, 1>(qs, cs, it, pr);
+}
+
+// Persistent trace waves.  MODE 0: closest hit of the path ray (active
+// queue).  MODE 1: VisibilityTester::Tr of a shadow segment: a surface with
+// a material occludes; each segment's medium attenuates; a material-less
+// surface is queried and the segment restarts from it towards the light
+// point (vis_tr).  MODE 2: Scene::IntersectTr of the MIS segment: closest
+// hits that cross material-less surfaces unqueried; then the chosen light's
+// emission if it is what the segment reaches (intersect_tr and di_vpt's tail).
+// A lane keeps its segment loop until the loop ends; idle lanes take new
+// entries, 64 per atomic.
+template <int S, int MODE, bool ENV>
+__device__ inline void vwf_trace_loop(const SceneDev& Sc, const MediaDev& M, const VwfDev& W, int refill, uint32_t* stack) {
+    constexpr int qid = MODE == 0 ? VQ_ACTIVE : (MODE == 1 ? VQ_SHADOW : VQ_MIS);
+    const uint32_t n = W.counts[qid * kVwfCtr];
+    // The queue is cut into kVwfChunks regions with a fetch counter each; a
+    // wave starts on the region of its XCD (workgroups are dealt to the XCDs
+    // round-robin) and moves on when it runs dry, so no one counter takes
+    // every claim.
+    uint32_t* fetch = &W.counts[(VQ_COUNT + MODE * kVwfChunks) * kVwfCtr];
+    int ck = (int)(blockIdx.x % kVwfChunks), tries = 0;  // wave-uniform
+    const int lane = threadIdx.x & 63;
+    const int64_t gl = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    uint32_t* lds = stack + threadIdx.x;
+    uint32_t* spill = W.spill + gl;
+    int64_t my = -1;                          // -1 wants work, >= 0 queue index
+    uint32_t batch_base = 0, batch_left = 0;  // wave-uniform
+    bool fresh = false;
+    uint32_t slot = 0;
+    gyd::Trav t;
+    V3 Tr = v3s(1.f);
+    int32_t med = -1;
+    int k = 0;
+    float tmax = gyd::kInf;
+    while (true) {
+        const bool need = my == -1;
+        uint64_t nm = __ballot(need);
+        while (nm) {
+            if (batch_left == 0) {
+                while (tries < kVwfChunks) {
+                    const uint32_t lo = (uint32_t)(((uint64_t)n * ck) / kVwfChunks);
+                    const uint32_t hi = (uint32_t)(((uint64_t)n * (ck + 1)) / kVwfChunks);
+                    uint32_t base = 0;
+                    if (lane == 0) base = atomicAdd(&fetch[ck * kVwfCtr], 64u);
+                    base = __shfl(base, 0, 64);
+                    if (base < hi - lo) {
+                        batch_base = lo + base;
+                        batch_left = min(64u, hi - batch_base);
+                        break;
+                    }
+                    ck = ck + 1 == kVwfChunks ? 0 : ck + 1;
+                    ++tries;
+                }
+                if (batch_left == 0) break;  // every region is drained
+            }
+            const uint32_t rank = (uint32_t)__popcll(nm & ((1ull << lane) - 1ull));
+            const uint32_t take = min((uint32_t)__popcll(nm), batch_left);
+            if (need && my == -1 && rank < take) {
+                my = batch_base + rank;
+                fresh = true;
+            }
+            batch_base += take;
+            batch_left -= take;
+            nm = __ballot(need && my == -1);
+        }
+        if (__ballot(my >= 0) == 0) break;
+        if (my < 0) continue;
+        if (fresh) {
+            fresh = false;
+            slot = W.q[qid][my];
+            Tr = v3s(1.f);
+            k = 0;
+            if (MODE == 0) {
+                gyd::trav_init(t, v3f(W.o[slot]), v3f(W.d[slot]), 0.f, gyd::kInf);
+            } else if (MODE == 1) {
+                const float4 so = W.sh_o[slot], sd = W.sh_d[slot];
+                tmax = so.w;
+                med = __float_as_int(sd.w);
+                gyd::trav_init(t, v3f(so), v3f(sd), 0.f, tmax);
+            } else {
+                const float4 sd = W.mis_d[slot];
+                med = __float_as_int(sd.w);
+                gyd::trav_init(t, v3f(W.mis_o[slot]), v3f(sd), 0.f, gyd::kInf);
+            }
+        }
+        if (gyd::trav_run<false, S>(Sc, t, lds, 256, spill, W.spill_stride, refill) == 0) continue;
+        const bool hit = t.best_prim >= 0;
+        bool more = false;  // the lane's segment loop goes on
+        if (MODE == 0) {
+            W.hit[slot] = hit ? make_float4(t.best, __int_as_float(t.best_prim), t.bb1, t.bb2)
+                              : make_float4(0.f, __int_as_float(-1), 0.f, 0.f);
+        } else if (MODE == 1) {
+            if (hit && !no_material(M, t.best_prim)) {
+                Tr = v3s(0.f);
+            } else {
+                if (med >= 0) Tr = Tr * medium_tr(load_medium(M, med), t.d, hit ? t.best : tmax);
+                if (hit) {
+                    if (++k >= kVptMaxCrossings) {
+                        Tr = v3s(0.f);
+                    } else {
+                        Hit h;
+                        h.t = t.best;
+                        h.prim = t.best_prim;
+                        h.b1 = t.bb1;
+                        h.b2 = t.bb2;
+                        const Isect it = gyd::query(Sc, t.o, t.d, h);
+                        int32_t in, out;
+                        hit_media(M, h.prim, med, &in, &out);
+                        LightPoint lp;
+                        lp.pos = v3f(W.lp_pos[slot]);
+                        lp.fn = v3f(W.lp_fn[slot]);
+                        lp.pErr = v3f(W.lp_err[slot]);
+                        lp.n = v3s(0.f);
+                        V3 o2, d2;
+                        gyd::spawn_to(it, lp, &o2, &d2, &tmax);
+                        med = isect_medium(it, in, out, d2);
+                        gyd::trav_init(t, o2, d2, 0.f, tmax);
+                        more = true;
+                    }
+                }
+            }
+            if (!more) W.tr1[slot] = f4v(Tr, 0.f);
+        } else {
+            if (med >= 0) Tr = Tr * medium_tr(load_medium(M, med), t.d, hit ? t.best : gyd::kInf);
+            bool reached = false;  // IntersectTr found a surface with a material
+            if (hit) {
+                if (!no_material(M, t.best_prim)) {
+                    reached = true;
+                } else if (++k < kVptMaxCrossings) {
+                    const float4* qv = Sc.shade + (size_t)t.best_prim * 8;
+                    const V3 p0 = gyd::ld3(qv[0]), p1 = gyd::ld3(qv[1]), p2 = gyd::ld3(qv[2]);
+                    const float b1 = t.bb1, b2 = t.bb2;
+                    const V3 pos = ((1 - b1 - b2) * p0 + b1 * p1) + b2 * p2;
+                    int32_t in, out;
+                    hit_media(M, t.best_prim, med, &in, &out);
+                    const V3 o2 = gyd::offset_origin(pos, v3s(0.f), v3s(0.f), t.d) + t.d * gyd::kRayEps;
+                    med = in;
+                    gyd::trav_init(t, o2, t.d, 0.f, gyd::kInf);
+                    more = true;
+                }
+            }
+            if (!more) {
+                const int light = __float_as_int(W.pD[slot].w);
+                const int ltri = Sc.light_tri[light];
+                float4 e = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (reached) {
+                    if (gyd::prim_light(Sc, t.best_prim) == light) {
+                        Hit h;
+                        h.t = t.best;
+                        h.prim = t.best_prim;
+                        h.b1 = t.bb1;
+                        h.b2 = t.bb2;
+                        const Isect li = gyd::query(Sc, t.o, t.d, h);
+                        const V3 Le = v3(Sc.light_le[3 * light], Sc.light_le[3 * light + 1], Sc.light_le[3 * light + 2]);
+                        e = f4v(gyd::dot(li.n, gyd::neg(t.d)) > 0.f ? Le : v3s(0.f), 1.f);
+                    }
+                } else if (ENV && ltri < 0) {
+                    e = f4v(gyd::env_emission_call(Sc.env, t.d), 1.f);
+                }
+                W.e2[slot] = e;
+                W.tr2[slot] = f4v(Tr, 0.f);
+            }
+        }
+        if (!more) my = -1;
+    }
+}
+template <int S, int MODE, bool ENV>
+__global__ void __launch_bounds__(256) k_vwf_trace(SceneDev Sc, MediaDev M, VwfDev W, int refill) {
+    __shared__ uint32_t stack[S * 256];
+    vwf_trace_loop<S, MODE, ENV>(Sc, M, W, refill, stack);
+}
+// Shadow segments, then MIS segments, in one persistent launch: a wave that
+// finds the shadow queue drained and its lanes idle goes on to the MIS queue
+// while other waves finish their last shadow loops.  The two write different
+// records of a slot.
+template <int S, bool ENV>
+__global__ void __launch_bounds__(256) k_vwf_trace_shadow_mis(SceneDev Sc, MediaDev M, VwfDev W, int refill) {
+    __shared__ uint32_t stack[S * 256];
+    vwf_trace_loop<S, 1, ENV>(Sc, M, W, refill, stack);
+    vwf_trace_loop<S, 2, ENV>(Sc, M, W, refill, stack);
+}
+
+// Adds each vertex's light in vpt_path's operation order and sends ended
+// paths to the film; returns whether the path goes on.
+__device__ inline bool vwf_resolve_slot(const VwfDev& W, const FilmDev& F, uint32_t slot) {
+    const uint32_t pk = W.pk[slot];
+    if (!(pk & VK_LIVE)) return false;
+    if (pk & (VK_DI | VK_CONN | VK_CONN0)) {
+        const float4 pt = W.pT[slot];
+        const V3 Tv = v3f(pt);
+        const float4 Lv = W.L[slot];
+        V3 L = v3f(Lv);
+        if (pk & VK_DI) {
+            const float4 c = W.pC[slot];
+            V3 L1 = v3s(0.f), L2 = v3s(0.f);
+            if (pk & VK_SH) {
+                const float4 a = W.pA[slot], b = W.pB[slot];
+                L1 = v3f(a) * v3f(W.tr1[slot]) * v3f(b) * a.w / b.w;
+            }
+            if (pk & VK_MIS) {
+                const float4 e = W.e2[slot];
+                if (e.w != 0.f) L2 = v3f(e) * v3f(W.pD[slot]) * v3f(W.tr2[slot]) * c.z / c.w;
+            }
+            L = L + Tv * ((L1 * c.x + L2 * c.y) / pt.w);
+        } else {
+            V3 conn = v3s(0.f);
+            if (pk & VK_CONN) {
+                const float4 a = W.pA[slot], b = W.pB[slot];
+                conn = v3f(a) * v3f(W.tr1[slot]) * a.w / b.x / b.y;
+            }
+            L = L + Tv * conn / pt.w;
+        }
+        W.L[slot] = f4v(L, Lv.w);
+    }
+    if (pk & VK_DONE) {
+        const float4 Lv = W.L[slot];
+        int s_local, px, py;
+        pass_pixel(F, slot, &s_local, &px, &py);
+        record_sample(F, s_local, px, py, W.T[slot].w, Lv.w, v3f(Lv));
+        W.pk[slot] = 0u;
+        return false;
+    }
+    W.pk[slot] = VK_LIVE;
+    return true;
+}
+// Walks the slots in order, four per lane (slot = (block * 4 + k) * 256 +
+// lane), and forms the next active queue with one atomic per workgroup.
+constexpr int kVwfResolvePer = 4;
+__global__ void __launch_bounds__(256) k_vwf_resolve(VwfDev W, FilmDev F, int64_t n_slots) {
+    uint32_t slot[kVwfResolvePer];
+    bool next[1][kVwfResolvePer];
+#pragma unroll
+    for (int k = 0; k < kVwfResolvePer; ++k) {
+        const int64_t t = ((int64_t)blockIdx.x * kVwfResolvePer + k) * 256 + threadIdx.x;
+        slot[k] = (uint32_t)t;
+        next[0][k] = t < n_slots && vwf_resolve_slot(W, F, slot[k]);
+    }
+    uint32_t* const qs[1] = {W.q[VQ_NEXT]};
+    uint32_t* const cs[1] = {&W.counts[VQ_NEXT * kVwfCtr]};
+    vwf_block_push<1, kVwfResolvePer>(qs, cs, slot, next);
+}
+
+// Next round: the next queue becomes the active one; returns its length.
+__global__ void k_vwf_advance(VwfDev W, uint32_t* out, unsigned long long* rays) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        rays[0] += W.counts[VQ_ACTIVE * kVwfCtr];
+        rays[1] += W.counts[VQ_SHADOW * kVwfCtr];
+        rays[2] += W.counts[VQ_MIS * kVwfCtr];
+        const uint32_t nx = W.counts[VQ_NEXT * kVwfCtr];
+        for (int w = 0; w < kVwfWords; ++w) W.counts[w * kVwfCtr] = 0u;
+        W.counts[VQ_ACTIVE * kVwfCtr] = nx;
+        *out = nx;
+    }
+}
+__global__ void k_vwf_reset(VwfDev W) {
+    if (threadIdx.x < kVwfWords && blockIdx.x == 0) W.counts[threadIdx.x * kVwfCtr] = 0u;
+}
+
 MediaDev MediaView(const Scene* sc) {
     MediaDev M;
     M.med = (const float4*)sc->d_media;
@@ -509,6 +1139,128 @@ int LaunchVpt(Ctx* c, Scene* sc, const PathParams& P, int s0, int n_s, bool eqa)
     return CheckLaunch(c, "k_vpt");
 }
 
+// ---------------------------------------------------------------- wavefront host side
+constexpr int kVwfStack = 16;  // LDS ring entries per lane; deeper stacks spill
+constexpr int kVwfRefill = 48;
+
+static size_t VwfBytes(int64_t n) {
+    const size_t per = 24 * 16 + 4 + 4 + VQ_COUNT * 4;  // 24 float4 / uint4 arrays, med, pk, queues
+    return (size_t)n * per + 256 * (24 + 2 + VQ_COUNT + 2) + kVwfWords * kVwfCtr * 4 + 4096;
+}
+static VwfDev VwfView(const WavefrontState& wf, size_t* used) {
+    VwfDev W;
+    char* base = (char*)wf.mem;
+    const int64_t n = wf.capacity;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        void* p = base + off;
+        off += (bytes + 255) & ~(size_t)255;
+        return p;
+    };
+    float4** f4s[] = {&W.o, &W.d, &W.T, &W.L, &W.hit, &W.pT, &W.pA, &W.pB, &W.pC, &W.pD, &W.tr1, &W.e2, &W.tr2,
+                      &W.sh_o, &W.sh_d, &W.lp_pos, &W.lp_fn, &W.lp_err, &W.mis_o, &W.mis_d};
+    for (float4** p : f4s) *p = (float4*)take(n * 16);
+    W.st = (uint4*)take(n * 16);
+    W.med = (int32_t*)take(n * 4);
+    W.pk = (uint32_t*)take(n * 4);
+    for (int q = 0; q < VQ_COUNT; ++q) W.q[q] = (uint32_t*)take(n * 4);
+    W.counts = (uint32_t*)take((size_t)(kVwfWords + 1) * kVwfCtr * 4);  // + the round's active count for the host
+    W.capacity = n;
+    W.spill = (uint32_t*)wf.spill;
+    W.spill_stride = 0;
+    *used = off;
+    return W;
+}
+
+template <bool ENV>
+static unsigned VwfPersistBlocks(Ctx* c) {
+    int k[2] = {0, 0};
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&k[0], k_vwf_trace<kVwfStack, 0, ENV>, 256, 0);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&k[1], k_vwf_trace_shadow_mis<kVwfStack, ENV>, 256, 0);
+    return (unsigned)(std::max(1, std::max(k[0], k[1])) * c->num_cu);
+}
+
+// One pass (samples s0 .. s0 + n_s - 1 of every pixel) of the wavefront form.
+template <bool ENV, bool EQA>
+static int VwfPass(Ctx* c, Scene* sc, const PathParams& P, const FilmDev& F, int s0, int n_s) {
+    const int tilesX = (F.width + 7) >> 3, tilesY = (F.height + 7) >> 3;
+    const int64_t n_slots = (int64_t)tilesX * tilesY * 64 * n_s;
+    if (n_slots >= (1ll << 31)) return Fail(GY_ERR_UNSUPPORTED, "VolPathTracing wavefront pass larger than 2^31 paths");
+    const unsigned pblocks = VwfPersistBlocks<ENV>(c);
+    WavefrontState& wf = sc->vwf;
+    if (wf.capacity < n_slots || !wf.mem) {
+        WavefrontStateFree(&wf);
+        int rc = DevAlloc(c, &wf.mem, VwfBytes(n_slots));
+        if (rc != GY_OK) return rc;
+        wf.bytes = VwfBytes(n_slots);
+        wf.capacity = n_slots;
+    }
+    const size_t sbytes = (size_t)pblocks * 256 * 64 * 4;
+    if (wf.spill_bytes < sbytes) {
+        DevFree(wf.spill);
+        wf.spill = nullptr;
+        wf.spill_bytes = 0;
+        int rc = DevAlloc(c, &wf.spill, sbytes);
+        if (rc != GY_OK) return rc;
+        wf.spill_bytes = sbytes;
+    }
+    size_t used = 0;
+    VwfDev W = VwfView(wf, &used);
+    if (used > wf.bytes) return Fail(GY_ERR_ALLOC, "VolPathTracing wavefront layout exceeds its allocation");
+    W.spill_stride = (int64_t)pblocks * 256;
+    uint32_t* d_n = W.counts + kVwfWords * kVwfCtr;
+    const MediaDev M = MediaView(sc);
+    const unsigned blocks = (unsigned)((n_slots + 255) / 256);
+    const unsigned rblocks = (unsigned)((n_slots + 256 * kVwfResolvePer - 1) / (256 * kVwfResolvePer));
+    hipLaunchKernelGGL(k_vwf_reset, dim3(1), dim3(64), 0, c->stream, W);
+    TimerBegin(c, kKernRaygen);
+    hipLaunchKernelGGL(k_vwf_start, dim3(blocks), dim3(256), 0, c->stream, W, P, F, s0, n_s, n_slots);
+    TimerEnd(c);
+    int rc = CheckLaunch(c, "k_vwf_start");
+    if (rc != GY_OK) return rc;
+    if (P.max_depth == 0) {
+        hipLaunchKernelGGL(k_vwf_resolve, dim3(rblocks), dim3(256), 0, c->stream, W, F, n_slots);
+        return CheckLaunch(c, "k_vwf_resolve");
+    }
+    // rounds until no path is left (a boundary crossing adds a round but no
+    // bounce; a path makes at most max_depth bounces and, per loop, 256
+    // crossings, and draws a bounded number of dimensions)
+    const int max_rounds = P.max_depth + kVptMaxCrossings + kVptDimBudget + 2;
+    for (int round = 0;; ++round) {
+        if (round > max_rounds) return Fail(GY_ERR_DEVICE, "VolPathTracing wavefront: paths left after the last possible round");
+        TimerBegin(c, kKernTraceClosest);
+        hipLaunchKernelGGL((k_vwf_trace<kVwfStack, 0, ENV>), dim3(pblocks), dim3(256), 0, c->stream, sc->dev, M, W, kVwfRefill);
+        TimerEnd(c);
+        TimerBegin(c, kKernShade);
+        hipLaunchKernelGGL((k_vwf_vertex<ENV, EQA>), dim3(blocks), dim3(256), 0, c->stream, sc->dev, M, P, W);
+        TimerEnd(c);
+        TimerBegin(c, kKernTraceShadow);
+        hipLaunchKernelGGL((k_vwf_trace_shadow_mis<kVwfStack, ENV>), dim3(pblocks), dim3(256), 0, c->stream, sc->dev, M, W, kVwfRefill);
+        TimerEnd(c);
+        TimerBegin(c, kKernOther);
+        hipLaunchKernelGGL(k_vwf_resolve, dim3(rblocks), dim3(256), 0, c->stream, W, F, n_slots);
+        hipLaunchKernelGGL(k_vwf_advance, dim3(1), dim3(64), 0, c->stream, W, d_n, sc->film.counters);
+        TimerEnd(c);
+        rc = CheckLaunch(c, "VolPathTracing wavefront round");
+        if (rc != GY_OK) return rc;
+        uint32_t left = 0;
+        hipError_t e = hipMemcpyAsync(&left, d_n, 4, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) return HipFail(e, "VolPathTracing wavefront round");
+        if (left == 0) return GY_OK;
+        uint32_t* tq = W.q[VQ_ACTIVE];
+        W.q[VQ_ACTIVE] = W.q[VQ_NEXT];
+        W.q[VQ_NEXT] = tq;
+    }
+}
+
+int LaunchVwf(Ctx* c, Scene* sc, const PathParams& P, int s0, int n_s, bool eqa) {
+    const FilmDev F = FilmViewState(sc->film);
+    const bool env = sc->dev.env.w > 0;
+    if (env) return eqa ? VwfPass<true, true>(c, sc, P, F, s0, n_s) : VwfPass<true, false>(c, sc, P, F, s0, n_s);
+    return eqa ? VwfPass<false, true>(c, sc, P, F, s0, n_s) : VwfPass<false, false>(c, sc, P, F, s0, n_s);
+}
+
 int ValidateVpt(Scene* sc, int32_t equiangular) {
     if (sc->dev.n_sph > 0) return Fail(GY_ERR_UNSUPPORTED, "VolPathTracing: scenes with spheres");
     // AreaLight::SampleOnePoint is the only SampleOnePoint the reference
@@ -524,6 +1276,7 @@ int ValidateRender(const gy_camera_desc* cam, const gy_render_params* p);
 int FilmPrepare(Ctx* c, Scene* sc, int W, int H, int n_pass);
 int FilmResolvePass(Ctx* c, Scene* sc, int n_s);
 int SamplesPerPass(const gy_camera_desc& cam, const gy_render_params& p);
+void TimersCollect(Ctx* c);
 
 }  // namespace gy
 
@@ -601,10 +1354,11 @@ extern "C" int gy_render_vpt(gy_scene* s, const gy_camera_desc* cam, const gy_re
     (void)hipEventCreate(&t0);
     (void)hipEventCreate(&t1);
     (void)hipEventRecord(t0, c->stream);
+    const bool wavefront = p->mode == GY_RENDER_WAVEFRONT;
     for (int s0 = 0; s0 < p->spp && rc == GY_OK; s0 += n_pass) {
         const int n_s = std::min(n_pass, p->spp - s0);
-        rc = LaunchVpt(c, sc, P, s0, n_s, equiangular != 0);
-        if (rc == GY_OK) rc = FilmResolvePass(c, sc, n_s);
+        rc = wavefront ? LaunchVwf(c, sc, P, s0, n_s, equiangular != 0) : LaunchVpt(c, sc, P, s0, n_s, equiangular != 0);
+        if (rc == GY_OK) rc = FilmResolvePass(c, sc, n_s);  // timed as kKernFilm inside
     }
     (void)hipEventRecord(t1, c->stream);
     hipError_t e = hipStreamSynchronize(c->stream);
@@ -613,8 +1367,17 @@ extern "C" int gy_render_vpt(gy_scene* s, const gy_camera_desc* cam, const gy_re
     (void)hipEventElapsedTime(&ms, t0, t1);
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
+    TimersCollect(c);
     c->stats.total_ms = ms;
     c->stats.paths = (int64_t)cam->width * cam->height * p->spp;
+    if (rc == GY_OK && wavefront) {  // path, shadow and MIS segments started (crossings not counted)
+        unsigned long long h[3] = {0, 0, 0};
+        e = hipMemcpy(h, sc->film.counters, sizeof(h), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = HipFail(e, "render_vpt counters");
+        c->stats.rays = (int64_t)(h[0] + h[1] + h[2]);
+        c->stats.shadow_rays = (int64_t)h[1];
+        c->stats.mis_rays = (int64_t)h[2];
+    }
     sc->film_valid = false;  // the adjoint replays PathTracing renders only
     if (rc == GY_OK) rc = FilmCheck(sc->film);
     if (rc == GY_OK && (color || weight)) rc = gy_film_download(s, color, weight);

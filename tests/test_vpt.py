@@ -31,7 +31,7 @@ import pytest
 import gyunity_amd as g
 from gyunity_amd._lib import GY_ERR_UNSUPPORTED
 from oracle import oracle as O
-from tests.scene_util import furnace_medium_scene, medium_cornell
+from tests.scene_util import env_image, furnace_medium_scene, medium_cornell
 
 RAY_EPS = 1e-4
 
@@ -288,6 +288,34 @@ def test_vpt_film_matches_oracle(gpu_ctx, equiangular):
     err = (np.abs(color - oc) / np.maximum(np.abs(oc), 1e-2)).max(axis=2)
     assert (err < 1e-4).mean() > 0.99, (err >= 1e-4).sum()
     assert np.abs(color - oc).sum() / np.abs(oc).sum() < 2e-3
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["cornell-free-flight", "cornell-equiangular", "furnace-chromatic", "cornell-env"])
+def test_vpt_wavefront_film_equals_per_path(gpu_ctx, case):
+    # The wavefront form (trace, vertex, shadow and MIS segment loops, resolve)
+    # performs vpt_path's FP32 operations in its order: the films are equal bit
+    # for bit, in one pass and in ragged passes, at every depth limit.
+    if case == "furnace-chromatic":
+        sa, cam, prm, media, tri = furnace_medium_scene(sigma_s=(2.0, 1.0, 0.5), sigma_a=(0.1, 0.3, 0.0), g=0.6, res=32, spp=8,
+                                                        depth=10)
+    else:
+        sa, cam, prm, media, tri = medium_cornell(res=48, spp=8, depth=6)
+    eqa = case == "cornell-equiangular"
+    with g.Scene(gpu_ctx, sa) as sc:
+        sc.set_media(media, tri)
+        if case == "cornell-env":
+            sc.set_environment(env_image())
+        for spp_pass, depth in ((0, prm.max_depth), (3, prm.max_depth), (0, 1), (0, 0)):
+            prm.samples_per_pass, prm.max_depth = spp_pass, depth
+            prm.mode = g.GY_RENDER_MEGAKERNEL
+            c0, w0 = sc.render_vpt(cam, prm, equiangular=eqa)
+            prm.mode = g.GY_RENDER_WAVEFRONT
+            c1, w1 = sc.render_vpt(cam, prm, equiangular=eqa)
+            assert np.array_equal(w0, w1), (spp_pass, depth)
+            assert np.array_equal(c0, c1, equal_nan=True), (spp_pass, depth, np.abs(c0 - c1).max())
+            if depth > 0:
+                assert c0.sum() > 0
 
 
 @pytest.mark.gpu
