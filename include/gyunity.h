@@ -200,9 +200,10 @@ typedef struct {
 int gy_scene_set_media(gy_scene* scene, int32_t n_media, const gy_medium* media, const gy_tri_medium* tri);
 /* VolPathTracing(spp, maxDepth, sampler, samplerEnum, equiAngular)
  * (integrator/VPT.h:12-19, VPT.cpp:7-171) through TiledIntegrator::Render:
- * params as gy_render's (spp, max_depth, sampler, samples_per_pass; rr_depth,
- * mode, block_size and refill_threshold are not used: the reference's VPT has
- * no working roulette and runs one path per thread here).  equiangular 1:
+ * params as gy_render's (spp, max_depth, sampler, samples_per_pass, mode:
+ * GY_RENDER_WAVEFRONT runs the wavefront form, GY_RENDER_MEGAKERNEL one path
+ * per thread, with equal films; rr_depth, block_size, refill_threshold and
+ * ray_sort are not used: the reference's VPT has no working roulette).  equiangular 1:
  * equi-angular distance sampling about one light point per bounce
  * (HomogeneousMedium::EquiAngularSampling, the reference's default); 0:
  * free-flight sampling with DirectIllumination at medium points.  Not
