@@ -684,6 +684,8 @@ __device__ inline void vwf_di_setup(const SceneDev& S, const VwfDev& W, uint32_t
 }
 
 // One vertex of vpt_path for each path of the active queue.
+// Four waves per SIMD (128 VGPRs, some spilled; 132-145 and three waves
+// without the hint): vertices 161 -> 144 ms a render; five waves took 213.
 template <bool ENV, bool EQA>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_vwf_vertex(SceneDev S, MediaDev M, PathParams P, VwfDev W) {
     const uint32_t n = W.counts[VQ_ACTIVE * kVwfCtr];
