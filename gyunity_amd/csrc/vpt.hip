@@ -1,5 +1,6 @@
-// VolPathTracing (integrator/VPT.cpp:7-171) on the GPU, one path per thread:
-// homogeneous media (volume/Homogeneous.cpp:6-64), the Henyey-Greenstein
+// VolPathTracing (integrator/VPT.cpp:7-171) on the GPU, in two forms with
+// equal films: one path per thread (k_vpt), and a wavefront form (k_vwf_*,
+// below); homogeneous media (volume/Homogeneous.cpp:6-64), the Henyey-Greenstein
 // phase function (visual/Medium.cpp:8-29), medium interfaces on triangles
 // (visual/Medium.h:51-59, visual/Primitive.cpp:11-14) and material-less
 // triangles that bound a medium.  Both of the reference's distance samplers:
