@@ -72,7 +72,7 @@ int BatchBuild(Ctx* c, Batch* b, int64_t lanes_req) {
     b->image_floats = image_off;
     b->adj_blocks = adj_blocks;
     b->depth_max = depth_max;
-    const int64_t want = lanes_req > 0 ? lanes_req : (int64_t)32 << 20;
+    const int64_t want = lanes_req > 0 ? lanes_req : (int64_t)64 << 20;
     b->lanes = std::min(want, (first + 255) / 256 * 256);
     if (b->lanes >= (1ll << 31)) return Fail(GY_ERR_UNSUPPORTED, "batch pass larger than 2^31 paths");
     b->dirty = false;

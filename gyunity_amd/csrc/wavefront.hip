@@ -1228,7 +1228,7 @@ __global__ void __launch_bounds__(256) k_lt_shade(SceneDev S, PathParams P, WfDe
 }
 
 int LightTracingWavefront(Ctx* c, Scene* sc, const PathParams& P, const LightCamDev& C, int64_t n, float total, float* splat) {
-    const int64_t cap = std::min<int64_t>(n, (int64_t)1 << 25);
+    const int64_t cap = std::min<int64_t>(n, (int64_t)1 << 26);
     int rc = WfPrepareState(c, sc->wf, std::max<int64_t>(cap, 1), false);
     if (rc != GY_OK) return rc;
     size_t used = 0;

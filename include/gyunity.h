@@ -310,7 +310,7 @@ int gy_autotune_clear_cache(void);
  * batch; adding an entry re-records the graph. */
 typedef struct gy_batch gy_batch;
 typedef struct {
-    int64_t lanes_per_pass;   /* paths in flight per pass: 0 = automatic (32M) */
+    int64_t lanes_per_pass;   /* paths in flight per pass: 0 = automatic (64M) */
     int32_t refill_threshold; /* persistent trace refill (0 = automatic, 1..64) */
     int32_t use_graph;        /* 1: record once into a hipGraph and replay */
     int32_t ray_sort;         /* GY_RAY_SORT_*, as in gy_render_params */
