@@ -214,9 +214,17 @@ def test_env_wavefront_bit_identical_to_megakernel(gpu_ctx):
         for mode in (g.GY_RENDER_MEGAKERNEL, g.GY_RENDER_WAVEFRONT):
             prm.mode = mode
             films[mode] = sc.render(cam, prm)
+        # the environment lookups of escaped rays read each slot's hit record
+        # whatever order the closest-hit queue was traced in
+        for srt in (g.GY_RAY_SORT_OCTANT, g.GY_RAY_SORT_OCTANT_LOCAL):
+            prm.ray_sort = srt
+            films[srt + 10] = sc.render(cam, prm)
     a, b = films[g.GY_RENDER_MEGAKERNEL], films[g.GY_RENDER_WAVEFRONT]
     assert np.array_equal(a[1], b[1]) and np.array_equal(a[0], b[0])
     assert np.isfinite(a[0]).all() and a[0].sum() > 0
+    for srt in (g.GY_RAY_SORT_OCTANT, g.GY_RAY_SORT_OCTANT_LOCAL):
+        c = films[srt + 10]
+        assert np.array_equal(a[1], c[1]) and np.array_equal(a[0], c[0]), srt
 
 
 @pytest.mark.gpu
