@@ -1,7 +1,8 @@
 // Renders the Cornell box through gy::HipPathTracing (the Integrator
 // interface) and checks the film against direct C-ABI renders: the adapter
 // must add nothing, and the megakernel and wavefront films must be equal bit
-// for bit.  Exit status 0 on success.
+// for bit.  Then the same for gy::HipLightTracing, gy::HipVolPathTracing (the
+// room with a slab of medium) and gy::HipSPPM.  Exit status 0 on success.
 //
 //   render_cornell [--expect-no-device]
 // With --expect-no-device the program checks that opening a device fails with
@@ -158,6 +159,58 @@ int main(int argc, char** argv) {
             std::printf("scene with a sphere: %s, %s; first scene again: %s\n", sph_same ? "same film as the C ABI" : "FILM DIFFERS",
                         sph_differs ? "differs from the first" : "SAME AS THE FIRST", back ? "first film" : "FILM DIFFERS");
             if (!sph_same || !sph_differs || !back) ++bad;
+        }
+
+        // VolPathTracing through gy::HipVolPathTracing on the room with a
+        // material-less slab of medium (12 outward-facing triangles, medium
+        // inside, none outside): the film equals direct C-ABI renders in both
+        // forms.  SPPM through gy::HipSPPM: the image equals a direct call.
+        {
+            gy::Scene fog = scene;
+            const int medium = fog.AddMedium({0.3f, 0.6f, 0.9f}, {2.0f, 1.5f, 1.0f}, 0.5f);
+            const float lo[3] = {0.15f, 0.62f, 0.1f}, hi[3] = {0.85f, 0.92f, 0.9f};
+            for (int axis = 0; axis < 3; ++axis)
+                for (int side = 0; side < 2; ++side) {
+                    const int u = (axis + 1) % 3, v = (axis + 2) % 3;
+                    gy::Vec3 n = {0, 0, 0};
+                    n[axis] = side ? 1.f : -1.f;
+                    std::array<gy::Vec3, 4> q;
+                    for (int c = 0; c < 4; ++c) {
+                        q[c][axis] = side ? hi[axis] : lo[axis];
+                        q[c][u] = (c == 1 || c == 2) ? hi[u] : lo[u];
+                        q[c][v] = (c >= 2) ? hi[v] : lo[v];
+                    }
+                    for (const auto& t : {std::array<int, 3>{0, 1, 2}, std::array<int, 3>{0, 2, 3}}) {
+                        const int id = fog.AddTriangle({q[t[0]], q[t[1]], q[t[2]]}, {n, n, n}, n, 0);
+                        fog.SetTriangleMedium(id, medium, -1, true);
+                    }
+                }
+            gy::RenderParams vparams;
+            vparams.mode = GY_RENDER_WAVEFRONT;
+            gy::HipVolPathTracing vpt(4, 6, sampler, senum, true, vparams, dev);
+            vpt.Render(fog, camera);
+            gy::DeviceScene dsf(*dev, fog);
+            gy_render_params pv = vpt.Params();
+            bool vpt_same = true;
+            for (int mode : {GY_RENDER_WAVEFRONT, GY_RENDER_MEGAKERNEL}) {
+                pv.mode = mode;
+                gy::Check(gy_render_vpt(dsf.Handle(), &c2, &pv, 1, col.data(), wgt.data()), "gy_render_vpt");
+                vpt_same = vpt_same && std::memcmp(col.data(), film->color.data(), col.size() * 4) == 0 &&
+                           std::memcmp(wgt.data(), film->weight.data(), wgt.size() * 4) == 0;
+            }
+            double vsum = 0;
+            for (float x : film->color) vsum += x;
+            std::printf("volumetric path tracing: colour sum %.2f, %s\n", vsum, vpt_same ? "same film as the C ABI" : "FILM DIFFERS");
+            if (!vpt_same || !(vsum > 0)) ++bad;
+
+            gy::HipSPPM sppm(2, 20000, 5, 0.05f, 0.7f, false, sampler, senum, dev);
+            sppm.Render(scene, camera);
+            gy_sppm_params ps = sppm.Params();
+            gy::Check(gy_render_sppm(ds.Handle(), &c2, &ps, col.data(), nullptr, nullptr), "gy_render_sppm");
+            const bool sppm_same = std::memcmp(col.data(), film->color.data(), col.size() * 4) == 0;
+            const gy::Vec3 px = film->Pixel(W / 2, H / 2);
+            std::printf("sppm: centre (%.4f %.4f %.4f), %s\n", px[0], px[1], px[2], sppm_same ? "same image as the C ABI" : "IMAGE DIFFERS");
+            if (!sppm_same) ++bad;
         }
         return bad ? 1 : 0;
     } catch (const gy::Error& e) {

@@ -302,6 +302,27 @@ public:
         spheres.push_back(q);
         return (int)(mat.size() + spheres.size()) - 1;
     }
+    // HomogeneousMedium(sigma_a, sigma_s, scale 1, g) (volume/Homogeneous.h:11-12).
+    // Returns the medium id.
+    int AddMedium(const Vec3& sigma_a, const Vec3& sigma_s, float g) {
+        Touch();
+        gy_medium m = {};
+        for (int k = 0; k < 3; ++k) {
+            m.sigma_a[k] = sigma_a[k];
+            m.sigma_s[k] = sigma_s[k];
+        }
+        m.g = g;
+        media.push_back(m);
+        return (int)media.size() - 1;
+    }
+    // The primitive's MediumInterface(inside, outside) (visual/Medium.h:51-59;
+    // -1: no medium) and, with noMaterial, a null material: the triangle only
+    // bounds a medium (visual/Primitive.cpp:11-14).
+    void SetTriangleMedium(int triangle, int inside, int outside, bool noMaterial) {
+        Touch();
+        if (tri_media.size() < mat.size()) tri_media.resize(mat.size(), gy_tri_medium{-1, -1, 0});
+        tri_media[(size_t)triangle] = gy_tri_medium{inside, outside, noMaterial ? 1 : 0};
+    }
     // Call after changing the arrays below directly: the next Render uploads again.
     void Touch() { stamp = SceneStamp(); }
     uint64_t Stamp() const { return stamp.v; }
@@ -365,6 +386,8 @@ public:
     std::vector<float> env_rgb;
     float env_scale = 1.f;
     std::vector<gy_sphere> spheres;
+    std::vector<gy_medium> media;
+    std::vector<gy_tri_medium> tri_media;  // per triangle, or empty: no media
 
 private:
     SceneStamp stamp;
@@ -388,6 +411,15 @@ public:
             if (rc != GY_OK) {
                 gy_scene_destroy(h_);
                 Check(rc, "gy_scene_set_environment");
+            }
+        }
+        if (!s.tri_media.empty()) {
+            std::vector<gy_tri_medium> tri(s.tri_media);
+            tri.resize(s.TriangleCount(), gy_tri_medium{-1, -1, 0});
+            const int rc = gy_scene_set_media(h_, (int32_t)s.media.size(), s.media.data(), tri.data());
+            if (rc != GY_OK) {
+                gy_scene_destroy(h_);
+                Check(rc, "gy_scene_set_media");
             }
         }
     }
@@ -522,6 +554,100 @@ private:
     std::shared_ptr<Device> mpDevice;
     std::unique_ptr<DeviceScene> mpScene;
     uint64_t mSceneStamp = 0;  // the Scene::Stamp() of mpScene
+};
+
+// VolPathTracing(spp, maxDepth, sampler, samplerEnum, equiAngular)
+// (integrator/VPT.h:12-19) on the GPU: fills camera.GetFilm() as
+// TiledIntegrator::Render does.  params.mode picks the wavefront form or one
+// path per thread (equal films).
+class HipVolPathTracing : public Integrator {
+public:
+    HipVolPathTracing(int samplePerPixel, int maxDepth, const std::shared_ptr<Sampler>& pSampler,
+                      const std::shared_ptr<SamplerEnum>& pSamplerEnum, bool equiAngular = true,
+                      RenderParams params = RenderParams(), std::shared_ptr<Device> device = nullptr)
+        : spp(samplePerPixel), mMaxDepth(maxDepth), mpSampler(pSampler), mpSamplerEnum(pSamplerEnum), mEquiAngular(equiAngular),
+          mParams(params), mpDevice(device ? std::move(device) : std::make_shared<Device>(0)) {
+        if (!mpSampler) throw Error(GY_ERR_INVALID, "HipVolPathTracing: no sampler");
+    }
+
+    void Render(const Scene& scene, const Camera& camera) override {
+        if (mSceneStamp != scene.Stamp() || !mpScene) {
+            mpScene.reset();
+            mpScene.reset(new DeviceScene(*mpDevice, scene));
+            mSceneStamp = scene.Stamp();
+        }
+        const gy_camera_desc cam = camera.Desc();
+        const gy_render_params p = Params();
+        auto film = camera.GetFilm();
+        Check(gy_render_vpt(mpScene->Handle(), &cam, &p, mEquiAngular ? 1 : 0, film->color.data(), film->weight.data()),
+              "gy_render_vpt");
+    }
+
+    gy_render_params Params() const {
+        gy_render_params p = {};
+        p.spp = spp;
+        p.max_depth = mMaxDepth;
+        p.rr_depth = mParams.rrStartDepth;
+        p.mode = mParams.mode;
+        p.samples_per_pass = mParams.samplesPerPass;
+        p.sampler = mpSampler->Kind();
+        return p;
+    }
+
+private:
+    int spp, mMaxDepth;
+    std::shared_ptr<Sampler> mpSampler;
+    std::shared_ptr<SamplerEnum> mpSamplerEnum;
+    bool mEquiAngular;
+    RenderParams mParams;
+    std::shared_ptr<Device> mpDevice;
+    std::unique_ptr<DeviceScene> mpScene;
+    uint64_t mSceneStamp = 0;
+};
+
+// SPPM(iterations, nPhotonsPerStage, maxDepth, initialRadius, alpha,
+// batchShrink, sampler, samplerEnum) (integrator/SPPM.h:48-49) on the GPU.
+// Render sets the film's image as Film::SetImage does: color = the image,
+// weight = 1, no splats, so Film::Pixel returns it.
+class HipSPPM : public Integrator {
+public:
+    HipSPPM(int iterations, int64_t photonsPerStage, int maxDepth, float initialRadius, float alpha, bool batchShrink,
+            const std::shared_ptr<Sampler>& pSampler, const std::shared_ptr<SamplerEnum>& pSamplerEnum,
+            std::shared_ptr<Device> device = nullptr)
+        : mpSampler(pSampler), mpSamplerEnum(pSamplerEnum), mpDevice(device ? std::move(device) : std::make_shared<Device>(0)) {
+        if (!mpSampler) throw Error(GY_ERR_INVALID, "HipSPPM: no sampler");
+        mParams = gy_sppm_params{};
+        mParams.iterations = iterations;
+        mParams.max_depth = maxDepth;
+        mParams.photons_per_iteration = photonsPerStage;
+        mParams.initial_radius = initialRadius;
+        mParams.alpha = alpha;
+        mParams.batch_shrink = batchShrink ? 1 : 0;
+        mParams.sampler = mpSampler->Kind();
+    }
+
+    void Render(const Scene& scene, const Camera& camera) override {
+        if (mSceneStamp != scene.Stamp() || !mpScene) {
+            mpScene.reset();
+            mpScene.reset(new DeviceScene(*mpDevice, scene));
+            mSceneStamp = scene.Stamp();
+        }
+        const gy_camera_desc cam = camera.Desc();
+        auto film = camera.GetFilm();
+        Check(gy_render_sppm(mpScene->Handle(), &cam, &mParams, film->color.data(), nullptr, nullptr), "gy_render_sppm");
+        std::fill(film->weight.begin(), film->weight.end(), 1.f);
+        std::fill(film->splat.begin(), film->splat.end(), 0.f);
+    }
+
+    const gy_sppm_params& Params() const { return mParams; }
+
+private:
+    std::shared_ptr<Sampler> mpSampler;
+    std::shared_ptr<SamplerEnum> mpSamplerEnum;
+    gy_sppm_params mParams;
+    std::shared_ptr<Device> mpDevice;
+    std::unique_ptr<DeviceScene> mpScene;
+    uint64_t mSceneStamp = 0;
 };
 
 }  // namespace gy
