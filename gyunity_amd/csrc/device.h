@@ -287,6 +287,22 @@ GY_D float box_enter(const float* b, V3 o, V3 inv, float tmin, float tmax) {
     return (tn <= tf * 1.0000004f || tn <= tf) ? tn : kInf;
 }
 
+// Entry distances of a node's two children (+inf on a miss or for an absent
+// child, index 0xffffffff).  Both slab tests run whatever the child indices
+// hold, and the indices only mask the results: a test guarded by its index
+// let the compiler sink the first box's load behind the wait for that index,
+// two memory round trips per node instead of one.
+GY_D void node_enter(uint4 q0, uint4 q1, uint4 q2, uint4 q3, V3 o, V3 inv, float tmin, float tmax, float* t0, float* t1) {
+    const float b0[6] = {__uint_as_float(q0.x), __uint_as_float(q0.y), __uint_as_float(q0.z),
+                         __uint_as_float(q0.w), __uint_as_float(q1.x), __uint_as_float(q1.y)};
+    const float b1[6] = {__uint_as_float(q1.z), __uint_as_float(q1.w), __uint_as_float(q2.x),
+                         __uint_as_float(q2.y), __uint_as_float(q2.z), __uint_as_float(q2.w)};
+    const float e0 = box_enter(b0, o, inv, tmin, tmax);
+    const float e1 = box_enter(b1, o, inv, tmin, tmax);
+    *t0 = q3.x != 0xffffffffu ? e0 : kInf;
+    *t1 = q3.y != 0xffffffffu ? e1 : kInf;
+}
+
 // IntersectP (accelerator/BVHAccel.cpp:127-149) on a triangle's own bounds.
 // The reference BVH keeps one primitive per leaf (maxPrimsInNode = 1), so this
 // slab test, with its strict comparisons and NaN fall-through, gates every
@@ -447,12 +463,8 @@ GY_D bool traverse(const SceneDev& S, V3 o, V3 d, float tmin, float tmax, Hit* h
         const uint4* np = S.nodes + (size_t)node * 4;
         const uint4 q0 = np[0], q1 = np[1], q2 = np[2], q3 = np[3];
         if (n_node) ++*n_node;
-        const float b0[6] = {__uint_as_float(q0.x), __uint_as_float(q0.y), __uint_as_float(q0.z),
-                             __uint_as_float(q0.w), __uint_as_float(q1.x), __uint_as_float(q1.y)};
-        const float b1[6] = {__uint_as_float(q1.z), __uint_as_float(q1.w), __uint_as_float(q2.x),
-                             __uint_as_float(q2.y), __uint_as_float(q2.z), __uint_as_float(q2.w)};
-        const float t0 = (q3.x != 0xffffffffu) ? box_enter(b0, o, inv, tmin, best) : kInf;
-        const float t1 = (q3.y != 0xffffffffu) ? box_enter(b1, o, inv, tmin, best) : kInf;
+        float t0, t1;
+        node_enter(q0, q1, q2, q3, o, inv, tmin, best, &t0, &t1);
         uint32_t next[2];
         float tn[2];
         int nn = 0;
@@ -603,12 +615,8 @@ GY_D int trav_run(const SceneDev& Sc, Trav& t, uint32_t* lds, int stride, uint32
         while (trav_inner(t.node)) {
             const uint4* np = Sc.nodes + (size_t)t.node * 4;
             const uint4 q0 = np[0], q1 = np[1], q2 = np[2], q3 = np[3];
-            const float b0[6] = {__uint_as_float(q0.x), __uint_as_float(q0.y), __uint_as_float(q0.z),
-                                 __uint_as_float(q0.w), __uint_as_float(q1.x), __uint_as_float(q1.y)};
-            const float b1[6] = {__uint_as_float(q1.z), __uint_as_float(q1.w), __uint_as_float(q2.x),
-                                 __uint_as_float(q2.y), __uint_as_float(q2.z), __uint_as_float(q2.w)};
-            const float t0 = (q3.x != 0xffffffffu) ? box_enter(b0, t.o, t.inv, t.tmin, t.best) : kInf;
-            const float t1 = (q3.y != 0xffffffffu) ? box_enter(b1, t.o, t.inv, t.tmin, t.best) : kInf;
+            float t0, t1;
+            node_enter(q0, q1, q2, q3, t.o, t.inv, t.tmin, t.best, &t0, &t1);
             const bool keep0 = t0 != kInf && (q3.z != 0u || t0 <= t.best);
             const bool keep1 = t1 != kInf && (q3.w != 0u || t1 <= t.best);
             const uint32_t e0 = q3.z != 0u ? (kTravLeaf | (q3.z << 27) | q3.x) : q3.x;
