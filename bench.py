@@ -878,7 +878,7 @@ def main():
         # HBM bytes the counters measured per launch over the live launch time
         "measured_hbm_gbs": round(traffic / (avg_launch_ms / 1e3) / 1e9, 2) if traffic else None,
         "measured_hbm_frac": round(traffic / (avg_launch_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
-        "limiter": "VALU with divergent lanes and node-load latency (profiles/counters_icosphere-L8-1024-64spp.txt): "
+        "limiter": "VALU with divergent lanes, 0.41 busy per instruction (profiles/counters_icosphere-L8-1024-64spp.txt): "
                    "the upper BVH levels are re-read from L2 and the Infinity Cache, so counter-measured HBM bytes "
                    "are about half the algorithmic bytes, and the algorithmic rate (frac) can pass HBM's peak",
         "algorithmic_bytes_per_launch": bytes_per_launch,

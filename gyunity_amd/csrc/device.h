@@ -599,19 +599,24 @@ GY_D uint32_t trav_pop(Trav& t, uint32_t* lds, int stride, uint32_t* __restrict_
 
 // Traversal in two phases per round (Aila and Laine, "Understanding the
 // efficiency of ray traversal on GPUs", 2009: while-while with postponed
-// leaves).  Interior nodes are visited until every lane of the wave still
-// traversing holds one leaf (a lane's first leaf is postponed and it keeps
-// walking); then the lanes test their postponed leaves together, and a second
-// leaf met on the way.  The node tests, culling and triangle gate are
+// leaves).  Interior nodes are visited until at most 1/kSpecWait of the
+// round's lanes still search for a leaf (a lane's first leaf is postponed and
+// it keeps walking); then the lanes test their postponed leaves together, and
+// a second leaf met on the way, while the few still searching wait and resume
+// in the next round.  Waiting for every lane idles the lanes that hold a leaf
+// behind the last few deep searches; the closest hit does not depend on the
+// visit order, so the ratio moves time only.  The node tests, culling and triangle gate are
 // traverse()'s (a leaf child is kept when its padded box is entered, an
 // interior child when its entry distance is also <= best), and triangles are
 // tested in the order their leaves are reached.  Returns 1 when the ray is
 // finished (t.best_prim >= 0 on a hit), 2 when an any-hit query found a hit,
 // and 0 when fewer than `refill` lanes remain, so that the caller can refill.
+constexpr int kSpecWait = 8;
 template <bool ANY, int S>
 GY_D int trav_run(const SceneDev& Sc, Trav& t, uint32_t* lds, int stride, uint32_t* __restrict__ spill, int64_t spill_stride,
                   int refill) {
     while (true) {
+        const int nact = __popcll(__ballot(true));
         while (trav_inner(t.node)) {
             const uint4* np = Sc.nodes + (size_t)t.node * 4;
             const uint4 q0 = np[0], q1 = np[1], q2 = np[2], q3 = np[3];
@@ -636,7 +641,7 @@ GY_D int trav_run(const SceneDev& Sc, Trav& t, uint32_t* lds, int stride, uint32
                 t.leaf = t.node;
                 t.node = trav_pop<S>(t, lds, stride, spill, spill_stride);
             }
-            if (__ballot(t.leaf == kTravNone) == 0) break;
+            if (__popcll(__ballot(t.leaf == kTravNone)) * kSpecWait <= nact) break;
         }
         while (t.leaf != kTravNone) {
             const uint32_t ref = t.leaf & 0x07ffffffu, cnt = (t.leaf >> 27) & 15u;

@@ -821,7 +821,11 @@ def test_batch_adjoint_equals_per_view_adjoints(gpu_ctx):
             gs += gv
             ls += lv
     assert np.abs(gb - gs).max() <= 1e-5 * np.abs(gs).max()
-    assert np.abs(lb - ls).max() <= 1e-5 * np.abs(ls).max()
+    # The emission gradient is 2 lights x 3 channels, each one float-atomic
+    # sum over about 10^5 path terms: its rounding moves with the atomics'
+    # order by about sqrt(10^5) * 2^-24 = 2e-5 of its size, so the bound is
+    # 1e-4 (the texels' sums hold far fewer terms each).
+    assert np.abs(lb - ls).max() <= 1e-4 * np.abs(ls).max()
     assert np.abs(gs).max() > 0
 
 
